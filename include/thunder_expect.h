@@ -1,0 +1,203 @@
+/*
+ * thunder_expect.h -- C ABI of the MI355X (gfx950) expectation step.
+ *
+ * This header is the drop-in boundary for THUNDER's GPU expectation
+ * (gpu/interface/Interface.h:16-237, called only from
+ * Optimiser::expectationG, src/Optimiser.cpp:1684-3403).  Every entry point
+ * takes plain pointers and sizes, returns an int status (THX_OK = 0) and
+ * never exits the process; the reference instead prints and calls exit(1)
+ * on any CUDA error (gpu/config/Device.cuh.in:28-40).  thx_last_error()
+ * returns the message of the calling thread's last failure.
+ *
+ * Two layers:
+ *   1. Reference-shaped entry points (thx_Expect*): same argument meaning,
+ *      order and array layouts as the Interface.h functions they replace,
+ *      host buffers in and out, synchronous.  Complex is {float re, im}
+ *      interleaved, exactly as cuthunder::Complex (reinterpret-cast from
+ *      THUNDER's Complex at gpu/interface/Interface.cpp:47-50).
+ *   2. A context API (thx_ctx_*) that keeps the volume, pixel set, particle
+ *      arrays and pose grids resident in HBM between calls, which is how the
+ *      hot loop is driven when the caller owns no host copies.
+ *
+ * Types: RFLOAT is float (SINGLE_PRECISION=ON, include/Precision.h:65);
+ * rotations, translations and prior weights are double
+ * (include/Typedef.h:59-64).  Marginals produced by the context API are FP64;
+ * the reference-shaped layer converts them to the reference's RFLOAT arrays.
+ */
+#ifndef THUNDER_EXPECT_H
+#define THUNDER_EXPECT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum thx_status {
+    THX_OK = 0,
+    THX_ERR_ARG = 1,     /* null pointer, bad size or inconsistent shapes */
+    THX_ERR_HIP = 2,     /* a HIP runtime call or kernel launch failed   */
+    THX_ERR_OOM = 3,     /* device or host allocation failed             */
+    THX_ERR_STATE = 4,   /* call order violated (e.g. no volume set)     */
+    THX_ERR_RANGE = 5    /* a size exceeds what the kernels support      */
+};
+
+/* Arithmetic of the cross-term contraction.  The norm terms, the per-particle
+ * maximum and the marginal sums are FP32 inside a tile and FP64 across tiles
+ * in every mode. */
+enum thx_precision {
+    THX_FP32 = 0,   /* plain path: FP32 operands, FP32 MFMA accumulation   */
+    THX_BF16 = 1,   /* BF16 operands, FP32 accumulation                    */
+    THX_FP16 = 2,   /* FP16 operands with per-shell scales, FP32 accum.    */
+    THX_FP8 = 3     /* OCP e4m3 operands with per-shell scales, FP32 accum.*/
+};
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char* thx_last_error(void);
+
+/* Replaces getAviDevice (gpu/interface/Interface.h:16): number of usable
+ * gfx950 devices; the list is 0..n-1. */
+int thx_device_count(int* n);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Reference-shaped entry points (host buffers, synchronous, device 0).   */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces ExpectRotran (gpu/interface/Interface.h:213-222, host side
+ * cuthunder::expectRotran gpu/src/cuthunder.cu:1410-1596).
+ * traP[nT*npxl] complex out: exp(-2 pi i (iCol tx + iRow ty) / idim);
+ * trans[nT*2] in; rot[nR*4] quaternions (w,x,y,z) in;
+ * rotMat[nR*9] out, column-major 3x3 per rotation as kernel_getRotMat stores it. */
+int thx_ExpectRotran(float* traP, const double* trans, const double* rot,
+                     double* rotMat, const int* iCol, const int* iRow,
+                     int nR, int nT, int idim, int npxl);
+
+/* Replaces ExpectProject (gpu/interface/Interface.h:224-233, host side
+ * gpu/src/cuthunder.cu:1604-1818).  volume: (vdim/2+1)*vdim*vdim complex,
+ * x fastest; rotP[nR*npxl] complex out; rotMat[nR*9] column-major;
+ * interp must be 1 (LINEAR_INTERP, the only mode the 3-D search uses,
+ * include/Model.h:90-93). */
+int thx_ExpectProject(const float* volume, float* rotP, const double* rotMat,
+                      const int* iCol, const int* iRow, int nR, int pf,
+                      int interp, int vdim, int npxl);
+
+/* Replaces ExpectGlobal3D (gpu/interface/Interface.h:235-252, host side
+ * gpu/src/cuthunder.cu:1826-2379).  Scores every image against every
+ * (rotation, translation) of class kIdx and updates the marginals:
+ *   wC[img*nK + kIdx]          = sum_{r,t} exp(dvp - baseL) pR[r] pT[t]
+ *   wR[(img*nK + kIdx)*nR + r] += sum_t  exp(dvp - baseL) pT[t]
+ *   wT[(img*nK + kIdx)*nT + t] += sum_r  exp(dvp - baseL) pR[r]
+ * with dvp = sum_k sigRcp |dat - ctf (traP (.) rotP)|^2 and baseL the running
+ * maximum over classes 0..kIdx; when class kIdx raises baseL the entries
+ * already stored for earlier classes are rescaled, as kernel_setBaseLine
+ * (gpu/src/Kernel.cu:1096-1128) does.  datP/ctfP/sigRcpP are image-major
+ * [imgNum*npxl]. */
+int thx_ExpectGlobal3D(const float* rotP, const float* traP, const float* datP,
+                       const float* ctfP, const float* sigRcpP,
+                       float* wC, float* wR, float* wT,
+                       const double* pR, const double* pT, float* baseL,
+                       int kIdx, int nK, int nR, int nT, int npxl, int imgNum);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Context API: data resident in HBM.                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct thx_ctx thx_ctx;
+
+/* One context per device; owns a stream and all device buffers.  Calls on
+ * one context must not overlap; distinct contexts are independent.
+ * Replaces the per-device setup of Optimiser::expectationG
+ * (src/Optimiser.cpp:1696-1711, 2197-2342). */
+int thx_ctx_create(int device, thx_ctx** out);
+int thx_ctx_destroy(thx_ctx* ctx);
+
+/* Pixel set of the E-step (Optimiser::allocPreCalIdx, src/Optimiser.cpp:
+ * 7942-7992): iCol, iRow, iSig[K]; idim = image size N; pf = padding factor.
+ * Replaces ExpectPreidx (gpu/interface/Interface.h:18-23). */
+int thx_ctx_set_pixels(thx_ctx* ctx, const int* iCol, const int* iRow,
+                       const int* iSig, int K, int idim, int pf);
+
+/* Half-complex padded Fourier volume of class cls, (vdim/2+1)*vdim*vdim
+ * complex, x fastest (Projector::projectee3D, include/Projector.h).
+ * Replaces ExpectLocalV3D (gpu/interface/Interface.h:45-48) and the volume
+ * upload inside ExpectProject. */
+int thx_ctx_set_volume(thx_ctx* ctx, int cls, const float* vol, int vdim);
+int thx_ctx_get_volume(thx_ctx* ctx, int cls, float* vol);
+
+/* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)
+ * packs them (src/Optimiser.cpp:7994-8121).  Replaces ExpectLocalIn +
+ * ExpectLocalP (gpu/interface/Interface.h:30-62) for all particles at once. */
+int thx_ctx_set_particles(thx_ctx* ctx, const float* datP, const float* ctfP,
+                          const float* sigRcpP, int P);
+int thx_ctx_get_particles(thx_ctx* ctx, float* datP, float* ctfP,
+                          float* sigRcpP);
+
+/* Global pose grid: quaternions (w,x,y,z) [R*4], translations in pixels
+ * [T*2] and their prior weights pR[R], pT[T] (Particle::reset,
+ * src/Particle.cpp:87-169, and the par.t / par.rot calls of
+ * src/Optimiser.cpp:711-725, 768-782). */
+int thx_ctx_set_grid(thx_ctx* ctx, const double* quat, int R,
+                     const double* trans, int T, const double* pR,
+                     const double* pT);
+
+/* Global scan of class cls at the given precision (the doGlobalSearch hot
+ * loop, src/Optimiser.cpp:757-915, and ExpectRotran + ExpectProject +
+ * ExpectGlobal3D on the GPU path).  Results stay on the device until
+ * thx_ctx_get_global is called.  Enqueued on the context stream; returns
+ * once the work is queued. */
+int thx_ctx_score_global(thx_ctx* ctx, int cls, int precision);
+
+/* Copies the marginals of the last scan of every class merged over classes
+ * with one baseline: wC[P*nK], wR[(P*nK)*R], wT[(P*nK)*T], baseL[P] (FP64),
+ * the same layouts as thx_ExpectGlobal3D.  Synchronises the stream. */
+int thx_ctx_get_global(thx_ctx* ctx, int nK, double* wC, double* wR,
+                       double* wT, double* baseL);
+
+/* Per-particle best pose of the last scan of class cls (argmax over r,t):
+ * bestR[P], bestT[P] (indices) and bestScore[P] = max dvp. */
+int thx_ctx_get_best(thx_ctx* ctx, int cls, int* bestR, int* bestT,
+                     double* bestScore);
+
+/* Scores of particle p against every (r, t) of the grid, FP32, for
+ * checking: out[R*T] (r slowest).  Uses the plain FP32 path. */
+int thx_ctx_score_one(thx_ctx* ctx, int cls, int p, float* out);
+
+/* Waits for all work queued on the context stream. */
+int thx_ctx_sync(thx_ctx* ctx);
+
+/* Timing of the last thx_ctx_score_global: per-kernel average duration in
+ * milliseconds measured with HIP events on the context stream. names is a
+ * ';'-separated list written into buf (cap bytes); ms[i] in the same order. */
+int thx_ctx_last_timing(thx_ctx* ctx, char* names, int cap, double* ms,
+                        int maxn, int* n);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic workloads (seeded; Philox4x32-10 counter streams).              */
+/* ------------------------------------------------------------------------ */
+
+/* Volume of nBlob isotropic Gaussian blobs (width sigma voxels, amplitude 1,
+ * centres uniform in a ball of radius rBall about the box centre) of an
+ * idim^3 box, generated directly as its Fourier transform on the pf-padded
+ * half-complex grid (vdim = pf*idim).  centresOut[nBlob*3] (optional). */
+int thx_ctx_synth_volume(thx_ctx* ctx, int cls, int idim, int pf, int nBlob,
+                         double sigma, double rBall, uint64_t seed,
+                         double* centresOut);
+
+/* Particle images X = CTF (.) S_t (.) P_r + complex white noise at the
+ * given real-space SNR, at the context's pixel set, from class cls.
+ * True quaternion uniform on S^3, shifts uniform in [-maxShift, maxShift]^2,
+ * defocus U, V uniform in [dfMin, dfMax] Angstrom, astigmatism angle
+ * uniform in [0, pi).  CTF: voltage V, Cs Angstrom, amplitude contrast,
+ * pixelSize Angstrom/px.  sigRcp = -1 / (N^2 sigma_n^2) on every pixel.
+ * trueQuat[P*4], trueShift[P*2], ctfParams[P*3] (dU, dV, theta) optional. */
+int thx_ctx_synth_particles(thx_ctx* ctx, int cls, int P, double maxShift,
+                            double dfMin, double dfMax, double voltage,
+                            double Cs, double ampContrast, double pixelSize,
+                            double snr, uint64_t seed, double* trueQuat,
+                            double* trueShift, double* ctfParams);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

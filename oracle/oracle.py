@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (oracle/thx_oracle.c) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and the CPU-baseline leg of bench.py may
+import this module, and only as the checker.  Each wrapper names the
+reference routine its C function restates.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_thx.so")
+
+_lib = None
+_i, _d, _f, _p = C.c_int, C.c_double, C.c_float, C.c_void_p
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    sigs = {
+        "thxo_pixel_set": ([_i, _d, _d, _i, _p, _p, _p, _p, _i], _i),
+        "thxo_ctf": ([_p, _f, _f, _f, _f, _f, _f, _f, _f, _i, _i, _p, _p, _i], None),
+        "thxo_translate": ([_p, _f, _f, _i, _i, _p, _p, _i], None),
+        "thxo_rotmat": ([_p, _p], None),
+        "thxo_project": ([_p, _p, _i, _p, _p, _p, _i, _i], None),
+        "thxo_project_f64": ([_p, _p, _i, _p, _p, _p, _i, _i], None),
+        "thxo_logdatavs_pm": ([_p, _p, _p, _p, _i, _i, _p], None),
+        "thxo_logdatavs_f64": ([_p, _p, _p, _p, _i], _d),
+        "thxo_global_scan_f32": ([_p, _i, _i, _p, _p, _i, _i, _p, _i, _p, _i, _p, _p, _p, _i,
+                                  _p, _p, _p, _p, _p, _p, _i], _i),
+        "thxo_global_scan_f64": ([_p, _i, _i, _p, _p, _i, _i, _p, _i, _p, _i, _p, _p, _p, _i,
+                                  _p, _p, _p, _p, _p, _p, _p, _i], _i),
+        "thxo_resample": ([_p, _p, _i, _i, _d, _p, _p], _i),
+        "thxo_blob_ft": ([_p, _i, _i, _p, _p, _i, _d, _p, _p, _p, _i], None),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def pixel_set(N, rU, rL=0.0, pf=2):
+    """Optimiser::allocPreCalIdx, src/Optimiser.cpp:7942-7992."""
+    cap = (N // 2 + 1) * N
+    iCol = np.zeros(cap, np.int32)
+    iRow = np.zeros(cap, np.int32)
+    iSig = np.zeros(cap, np.int32)
+    iPxl = np.zeros(cap, np.int32)
+    K = load().thxo_pixel_set(N, rU, rL, pf, _ptr(iCol), _ptr(iRow), _ptr(iSig), _ptr(iPxl), cap)
+    assert K >= 0
+    return iCol[:K].copy(), iRow[:K].copy(), iSig[:K].copy(), iPxl[:K].copy()
+
+
+def ctf(iCol, iRow, N, pixelSize, voltage, dU, dV, theta, Cs, ampC, phaseShift=0.0):
+    """CTF(RFLOAT* dst, ...), src/CTF.cpp:109-145."""
+    iCol, iRow = _c(iCol, np.int32), _c(iRow, np.int32)
+    out = np.zeros(iCol.size, np.float32)
+    load().thxo_ctf(_ptr(out), pixelSize, voltage, dU, dV, theta, Cs, ampC, phaseShift, N, N,
+                    _ptr(iCol), _ptr(iRow), iCol.size)
+    return out
+
+
+def translate(tx, ty, N, iCol, iRow):
+    """translate(Complex*, ...), src/Image/ImageFunctions.cpp:233-251."""
+    iCol, iRow = _c(iCol, np.int32), _c(iRow, np.int32)
+    out = np.zeros((iCol.size, 2), np.float32)
+    load().thxo_translate(_ptr(out), tx, ty, N, N, _ptr(iCol), _ptr(iRow), iCol.size)
+    return out
+
+
+def rotmat(q):
+    """rotate3D(dmat33&, const dvec4&), src/Geometry/Euler.cpp:181-189 (row-major)."""
+    q = _c(q, np.float64)
+    M = np.zeros(9)
+    load().thxo_rotmat(_ptr(M), _ptr(q))
+    return M.reshape(3, 3)
+
+
+def project(vol, V, M, iCol, iRow, pf, f64=False):
+    """Projector::project, src/Projector.cpp:356-374 (+ Volume interpolation)."""
+    vol = _c(vol, np.float32)
+    M = _c(M, np.float64).reshape(9)
+    iCol, iRow = _c(iCol, np.int32), _c(iRow, np.int32)
+    if f64:
+        out = np.zeros((iCol.size, 2), np.float64)
+        load().thxo_project_f64(_ptr(out), _ptr(vol), V, _ptr(M), _ptr(iCol), _ptr(iRow),
+                                iCol.size, pf)
+    else:
+        out = np.zeros((iCol.size, 2), np.float32)
+        load().thxo_project(_ptr(out), _ptr(vol), V, _ptr(M), _ptr(iCol), _ptr(iRow),
+                            iCol.size, pf)
+    return out
+
+
+def logdatavs_pm(dat, pri, ctf_, sig):
+    """logDataVSPrior_m_n_huabin, src/Optimiser.cpp:9666-9708; arrays pixel-major [K][n]."""
+    dat, pri = _c(dat, np.float32), _c(pri, np.float32)
+    ctf_, sig = _c(ctf_, np.float32), _c(sig, np.float32)
+    m, n = ctf_.shape
+    res = np.zeros(n, np.float32)
+    load().thxo_logdatavs_pm(_ptr(dat), _ptr(pri), _ptr(ctf_), _ptr(sig), n, m, _ptr(res))
+    return res
+
+
+def logdatavs_f64(dat, pri, ctf_, sig):
+    dat, pri = _c(dat, np.float32), _c(pri, np.float32)
+    ctf_, sig = _c(ctf_, np.float32), _c(sig, np.float32)
+    return load().thxo_logdatavs_f64(_ptr(dat), _ptr(pri), _ptr(ctf_), _ptr(sig), ctf_.size)
+
+
+def global_scan(vol, V, pf, iCol, iRow, N, quat, trans, dat, ctf_, sig, pR, pT,
+                f64=True, threads=0, want_dvp=False):
+    """doGlobalSearch hot loop, src/Optimiser.cpp:711-915 (one class)."""
+    vol = _c(vol, np.float32)
+    iCol, iRow = _c(iCol, np.int32), _c(iRow, np.int32)
+    quat = _c(quat, np.float64).reshape(-1, 4)
+    trans = _c(trans, np.float64).reshape(-1, 2)
+    dat, ctf_, sig = _c(dat, np.float32), _c(ctf_, np.float32), _c(sig, np.float32)
+    pR, pT = _c(pR, np.float64), _c(pT, np.float64)
+    K, R, T, P = iCol.size, quat.shape[0], trans.shape[0], ctf_.shape[0]
+    if f64:
+        wC, wR, wT, base = np.zeros(P), np.zeros((P, R)), np.zeros((P, T)), np.zeros(P)
+        dvp = np.zeros((P, R, T)) if want_dvp else None
+        st = load().thxo_global_scan_f64(_ptr(vol), V, pf, _ptr(iCol), _ptr(iRow), K, N,
+                                         _ptr(quat), R, _ptr(trans), T, _ptr(dat), _ptr(ctf_),
+                                         _ptr(sig), P, _ptr(pR), _ptr(pT), _ptr(wC), _ptr(wR),
+                                         _ptr(wT), _ptr(base), _ptr(dvp), threads)
+        assert st == 0
+        return (wC, wR, wT, base, dvp) if want_dvp else (wC, wR, wT, base)
+    wC = np.zeros(P, np.float32)
+    wR = np.zeros((P, R), np.float32)
+    wT = np.zeros((P, T), np.float32)
+    base = np.zeros(P, np.float32)
+    st = load().thxo_global_scan_f32(_ptr(vol), V, pf, _ptr(iCol), _ptr(iRow), K, N, _ptr(quat),
+                                     R, _ptr(trans), T, _ptr(dat), _ptr(ctf_), _ptr(sig), P,
+                                     _ptr(pR), _ptr(pT), _ptr(wC), _ptr(wR), _ptr(wT),
+                                     _ptr(base), threads)
+    assert st == 0
+    return wC, wR, wT, base
+
+
+def resample(w, u, n_out, u0):
+    """Particle::resample(n, pt), src/Particle.cpp:1275-1462, u0 injected, no shuffle."""
+    w, u = _c(w, np.float64), _c(u, np.float64)
+    idx = np.zeros(n_out, np.int32)
+    wn = np.zeros(n_out)
+    st = load().thxo_resample(_ptr(w), _ptr(u), w.size, n_out, u0, _ptr(idx), _ptr(wn))
+    assert st == 0
+    return idx, wn
+
+
+def blob_ft(V, N, centres, amps, sigma, ix, iy, iz):
+    centres = _c(centres, np.float64)
+    amps = _c(amps, np.float64)
+    ix, iy, iz = _c(ix, np.int32), _c(iy, np.int32), _c(iz, np.int32)
+    out = np.zeros((ix.size, 2), np.float32)
+    load().thxo_blob_ft(_ptr(out), V, N, _ptr(centres), _ptr(amps), amps.size, sigma,
+                        _ptr(ix), _ptr(iy), _ptr(iz), ix.size)
+    return out

@@ -1,0 +1,479 @@
+/*
+ * CPU oracle for the THUNDER expectation step -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, the smoke check and the CPU-baseline leg of the benchmark may
+ * load this library, and only as the checker.  It restates the reference
+ * algorithms (THUNDER 1.4.11) in plain C; each function cites the reference
+ * file:line it follows.  Arithmetic types follow the default reference build
+ * (SINGLE_PRECISION=ON, so RFLOAT = float; rotations and particle weights are
+ * double), except the *_f64 variants, which are the FP64 judges.
+ *
+ * Parity pinning: the reference ships no golden vectors or tests for this
+ * path, so each function is pinned by closed forms, invariants and brute
+ * force in tests/test_oracle.py (named there per function).
+ */
+#include "thx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* AROUND(a) = (int)rint(a), NORM = gsl_hypot (include/Functions/Functions.h:30-70). */
+static int around(double a) { return (int)rint(a); }
+
+/*
+ * Optimiser::allocPreCalIdx, src/Optimiser.cpp:7942-7992.
+ * Loop IMAGE_FOR_PIXEL_R_FT(rU + 1) (include/Image/Image.h:68-70): j outer in
+ * [-r, r), i inner in [0, r]; skip (i == 0, j < 0); keep rL^2 <= i^2+j^2 < rU^2
+ * and rL <= rint(hypot) < rU.  iPxl = iFTHalf(i, j) of an N x N image
+ * (include/Image/Image.h:426-432).  Returns K, or -1 if cap is exceeded.
+ */
+int thxo_pixel_set(int N, double rU, double rL, int pf,
+                   int* iCol, int* iRow, int* iSig, int* iPxl, int cap)
+{
+    (void)pf;
+    float rUf = (float)rU, rLf = (float)rL;       /* RFLOAT arguments */
+    float r = rUf + 1;
+    float rU2 = rUf * rUf, rL2 = rLf * rLf;
+    int nColFT = N / 2 + 1;
+    int K = 0;
+    for (int j = (int)(-r); j < r; j++)
+        for (int i = 0; i <= r; i++) {
+            if (i == 0 && j < 0) continue;
+            float u = (float)((double)i * i + (double)j * j);
+            if (u < rU2 && u >= rL2) {
+                int v = around(hypot((double)i, (double)j));
+                if (v < rUf && v >= rLf) {
+                    if (K >= cap) return -1;
+                    if (iPxl) iPxl[K] = (j >= 0 ? j : j + N) * nColFT + i;
+                    iCol[K] = i;
+                    iRow[K] = j;
+                    if (iSig) iSig[K] = v;
+                    K++;
+                }
+            }
+        }
+    return K;
+}
+
+/*
+ * CTF(RFLOAT* dst, ...), src/CTF.cpp:109-145.  Units: V, Angstrom, rad.
+ * lambda = 12.2643247 / sqrt(V (1 + 0.978466e-6 V)); K1 = pi lambda;
+ * K2 = pi/2 Cs lambda^3; chi = K1 df u^2 + K2 u^4 - phaseShift;
+ * ctf = -sqrt(1 - A^2) sin(chi) + A cos(chi).
+ */
+void thxo_ctf(float* dst, float pixelSize, float voltage, float defocusU,
+              float defocusV, float theta, float Cs, float amplitudeContrast,
+              float phaseShift, int nCol, int nRow, const int* iCol,
+              const int* iRow, int K)
+{
+    float lambda = (float)(12.2643247 / sqrt(voltage * (1 + voltage * 0.978466e-6)));
+    float w1 = sqrtf(1 - amplitudeContrast * amplitudeContrast);
+    float w2 = amplitudeContrast;
+    float K1 = (float)(M_PI * lambda);
+    float K2 = (float)(M_PI / 2 * Cs * ((double)lambda * lambda * lambda));
+    for (int i = 0; i < K; i++) {
+        float u = (float)hypot(iCol[i] / (pixelSize * nCol), iRow[i] / (pixelSize * nRow));
+        float angle = (float)(atan2((double)iRow[i], (double)iCol[i]) - theta);
+        float defocus = -(defocusU + defocusV + (defocusU - defocusV) * cosf(2 * angle)) / 2;
+        float u2 = u * u;
+        float ki = K1 * defocus * u2 + K2 * (u2 * u2) - phaseShift;
+        dst[i] = -w1 * sinf(ki) + w2 * cosf(ki);
+    }
+}
+
+/*
+ * translate(Complex* dst, tx, ty, nCol, nRow, iCol, iRow, nPxl),
+ * src/Image/ImageFunctions.cpp:233-251 with COMPLEX_POLAR (include/Complex.h:33-38):
+ * dst = exp(-i 2 pi (iCol tx / nCol + iRow ty / nRow)).
+ */
+void thxo_translate(float* dst, float tx, float ty, int nCol, int nRow,
+                    const int* iCol, const int* iRow, int K)
+{
+    float rCol = tx / nCol, rRow = ty / nRow;
+    for (int i = 0; i < K; i++) {
+        float phase = (float)(2 * M_PI) * (iCol[i] * rCol + iRow[i] * rRow);
+        dst[2 * i] = cosf(-phase);
+        dst[2 * i + 1] = sinf(-phase);
+    }
+}
+
+/*
+ * rotate3D(dmat33&, const dvec4&), src/Geometry/Euler.cpp:181-189:
+ * R = I + 2 q0 A + 2 A^2 with A = [q]x (cross-product matrix of (q1, q2, q3)).
+ */
+void thxo_rotmat(double* M, const double* q)
+{
+    double A[9] = {0, -q[3], q[2], q[3], 0, -q[1], -q[2], q[1], 0};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            double aa = 0;
+            for (int c = 0; c < 3; c++) aa += A[a * 3 + c] * A[c * 3 + b];
+            M[a * 3 + b] = (a == b ? 1.0 : 0.0) + 2 * q[0] * A[a * 3 + b] + 2 * aa;
+        }
+}
+
+/* Volume::iFTHalf (include/Image/Volume.h:566-574): wrap negative y, z. */
+static size_t ifthalf(int V, int i, int j, int k)
+{
+    int nColFT = V / 2 + 1;
+    return (size_t)(k >= 0 ? k : k + V) * nColFT * V + (size_t)(j >= 0 ? j : j + V) * nColFT + i;
+}
+
+/*
+ * Projector::project(Complex*, const dmat33&, iCol, iRow, nPxl),
+ * src/Projector.cpp:356-374 -> Volume::getByInterpolationFT,
+ * src/Image/Volume.cpp:308-332 -> conjHalf (include/Image/Volume.h:139-151)
+ * -> WG_TRI_INTERP_LINEAR (include/Functions/Interpolation.h:195-208)
+ * -> Volume::getFTHalf(w, x0), src/Image/Volume.cpp:481-553.
+ * Coordinates are formed in FP64 and passed as RFLOAT (float); x < 0 is
+ * mirrored and conjugated; corners are fetched with y/z wrap; the 8 products
+ * are summed in the order w[k][j][i] flattened (k slowest).
+ */
+void thxo_project(float* dst, const float* vol, int V, const double* M,
+                  const int* iCol, const int* iRow, int K, int pf)
+{
+    for (int n = 0; n < K; n++) {
+        double c0 = (double)(iCol[n] * pf), c1 = (double)(iRow[n] * pf);
+        float x[3];
+        for (int a = 0; a < 3; a++) x[a] = (float)(M[a * 3 + 0] * c0 + M[a * 3 + 1] * c1);
+        int conj = 0;
+        if (x[0] < 0) { x[0] = -x[0]; x[1] = -x[1]; x[2] = -x[2]; conj = 1; }
+        int x0[3];
+        float v[3][2];
+        for (int a = 0; a < 3; a++) {
+            x0[a] = (int)floorf(x[a]);
+            float xd = x[a] - x0[a];
+            v[a][0] = 1 - xd;
+            v[a][1] = xd;
+        }
+        float re = 0, im = 0;
+        for (int k = 0; k < 2; k++)
+            for (int j = 0; j < 2; j++)
+                for (int i = 0; i < 2; i++) {
+                    float w = v[0][i] * v[1][j] * v[2][k];
+                    size_t idx = ifthalf(V, x0[0] + i, x0[1] + j, x0[2] + k);
+                    re += vol[2 * idx] * w;
+                    im += vol[2 * idx + 1] * w;
+                }
+        dst[2 * n] = re;
+        dst[2 * n + 1] = conj ? -im : im;
+    }
+}
+
+void thxo_project_f64(double* dst, const float* vol, int V, const double* M,
+                      const int* iCol, const int* iRow, int K, int pf)
+{
+    for (int n = 0; n < K; n++) {
+        double c0 = (double)(iCol[n] * pf), c1 = (double)(iRow[n] * pf);
+        double x[3];
+        for (int a = 0; a < 3; a++) x[a] = M[a * 3 + 0] * c0 + M[a * 3 + 1] * c1;
+        int conj = 0;
+        if (x[0] < 0) { x[0] = -x[0]; x[1] = -x[1]; x[2] = -x[2]; conj = 1; }
+        int x0[3];
+        double v[3][2];
+        for (int a = 0; a < 3; a++) {
+            x0[a] = (int)floor(x[a]);
+            double xd = x[a] - x0[a];
+            v[a][0] = 1 - xd;
+            v[a][1] = xd;
+        }
+        double re = 0, im = 0;
+        for (int k = 0; k < 2; k++)
+            for (int j = 0; j < 2; j++)
+                for (int i = 0; i < 2; i++) {
+                    double w = v[0][i] * v[1][j] * v[2][k];
+                    size_t idx = ifthalf(V, x0[0] + i, x0[1] + j, x0[2] + k);
+                    re += vol[2 * idx] * w;
+                    im += vol[2 * idx + 1] * w;
+                }
+        dst[2 * n] = re;
+        dst[2 * n + 1] = conj ? -im : im;
+    }
+}
+
+/*
+ * logDataVSPrior_m_n_huabin, src/Optimiser.cpp:9666-9708 (scalar form of
+ * SIMD256Float :8957-9041): pixel-major arrays idx = i * n + j;
+ * result[j] += sigRcp * |dat - ctf * pri[i]|^2.
+ */
+void thxo_logdatavs_pm(const float* dat, const float* pri, const float* ctf,
+                       const float* sigRcp, int n, int m, float* result)
+{
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < n; j++) {
+            size_t idx = (size_t)i * n + j;
+            float cr = ctf[idx] * pri[2 * i], ci = ctf[idx] * pri[2 * i + 1];
+            float dr = dat[2 * idx] - cr, di = dat[2 * idx + 1] - ci;
+            float t = dr * dr + di * di;
+            result[j] += t * sigRcp[idx];
+        }
+}
+
+/* FP64 judge of the same sum for one image-major particle. */
+double thxo_logdatavs_f64(const float* dat, const float* pri, const float* ctf,
+                          const float* sigRcp, int K)
+{
+    double s = 0;
+    for (int i = 0; i < K; i++) {
+        double cr = (double)ctf[i] * pri[2 * i], ci = (double)ctf[i] * pri[2 * i + 1];
+        double dr = dat[2 * i] - cr, di = dat[2 * i + 1] - ci;
+        s += (dr * dr + di * di) * sigRcp[i];
+    }
+    return s;
+}
+
+/*
+ * The hot loop of Optimiser::doGlobalSearch, src/Optimiser.cpp:711-915, for
+ * one class (t = 0, k = 1): traP per translation (:711-725); OpenMP over
+ * rotations (:759); per rotation project (:782), per translation
+ * pri = tra (.) rot (:793-794), dvp over all particles pixel-major (:801-827),
+ * then per particle under its lock the streaming baseline and
+ * wC += w pR pT, wR += w pT, wT += w pR (:835-895).  Inputs are image-major
+ * (l*K + i) as the GPU interface takes them; they are transposed to the
+ * pixel-major order the CPU path uses (allocPreCal, :7994-8121).
+ * wC[P], wR[P*R], wT[P*T], baseL[P].
+ */
+int thxo_global_scan_f32(const float* vol, int V, int pf,
+                         const int* iCol, const int* iRow, int K, int N,
+                         const double* quat, int R, const double* trans, int T,
+                         const float* datP, const float* ctfP,
+                         const float* sigRcpP, int P,
+                         const double* pR, const double* pT,
+                         float* wC, float* wR, float* wT, float* baseL,
+                         int nThreads)
+{
+#ifdef _OPENMP
+    if (nThreads > 0) omp_set_num_threads(nThreads);
+#else
+    (void)nThreads;
+#endif
+    float* datPM = (float*)malloc(sizeof(float) * 2 * (size_t)P * K);
+    float* ctfPM = (float*)malloc(sizeof(float) * (size_t)P * K);
+    float* sigPM = (float*)malloc(sizeof(float) * (size_t)P * K);
+    float* traP = (float*)malloc(sizeof(float) * 2 * (size_t)T * K);
+    if (!datPM || !ctfPM || !sigPM || !traP) {
+        free(datPM); free(ctfPM); free(sigPM); free(traP);
+        return -1;
+    }
+    for (int l = 0; l < P; l++)
+        for (int i = 0; i < K; i++) {
+            size_t s = (size_t)l * K + i, d = (size_t)i * P + l;
+            datPM[2 * d] = datP[2 * s];
+            datPM[2 * d + 1] = datP[2 * s + 1];
+            ctfPM[d] = ctfP[s];
+            sigPM[d] = sigRcpP[s];
+        }
+    for (int m = 0; m < T; m++)
+        thxo_translate(traP + (size_t)2 * m * K, (float)trans[2 * m], (float)trans[2 * m + 1],
+                       N, N, iCol, iRow, K);
+
+    memset(wC, 0, sizeof(float) * P);
+    memset(wR, 0, sizeof(float) * (size_t)P * R);
+    memset(wT, 0, sizeof(float) * (size_t)P * T);
+#ifdef _OPENMP
+    omp_lock_t* mtx = (omp_lock_t*)malloc(sizeof(omp_lock_t) * P);
+    for (int l = 0; l < P; l++) omp_init_lock(&mtx[l]);
+#endif
+    for (int l = 0; l < P; l++) baseL[l] = NAN;
+
+    #pragma omp parallel
+    {
+        float* priRotP = (float*)malloc(sizeof(float) * 2 * K);
+        float* priAllP = (float*)malloc(sizeof(float) * 2 * K);
+        float* res = (float*)malloc(sizeof(float) * P);
+        #pragma omp for schedule(dynamic)
+        for (int m = 0; m < R; m++) {
+            double M[9];
+            thxo_rotmat(M, quat + 4 * m);
+            thxo_project(priRotP, vol, V, M, iCol, iRow, K, pf);
+            for (int n = 0; n < T; n++) {
+                const float* tra = traP + (size_t)2 * n * K;
+                for (int i = 0; i < K; i++) {
+                    float ar = tra[2 * i], ai = tra[2 * i + 1];
+                    float br = priRotP[2 * i], bi = priRotP[2 * i + 1];
+                    priAllP[2 * i] = ar * br - ai * bi;
+                    priAllP[2 * i + 1] = ar * bi + ai * br;
+                }
+                memset(res, 0, sizeof(float) * P);
+                thxo_logdatavs_pm(datPM, priAllP, ctfPM, sigPM, P, K, res);
+                for (int l = 0; l < P; l++) {
+#ifdef _OPENMP
+                    omp_set_lock(&mtx[l]);
+#endif
+                    if (isnan(baseL[l]))
+                        baseL[l] = res[l];
+                    else if (res[l] > baseL[l]) {
+                        float offset = res[l] - baseL[l];
+                        float nf = expf(-offset);
+                        wC[l] *= nf;
+                        for (int r = 0; r < R; r++) wR[(size_t)l * R + r] *= nf;
+                        for (int t = 0; t < T; t++) wT[(size_t)l * T + t] *= nf;
+                        baseL[l] += offset;
+                    }
+                    float w = expf(res[l] - baseL[l]);
+                    wC[l] += (float)(w * (pR[m] * pT[n]));
+                    wR[(size_t)l * R + m] += (float)(w * pT[n]);
+                    wT[(size_t)l * T + n] += (float)(w * pR[m]);
+#ifdef _OPENMP
+                    omp_unset_lock(&mtx[l]);
+#endif
+                }
+            }
+        }
+        free(priRotP); free(priAllP); free(res);
+    }
+#ifdef _OPENMP
+    for (int l = 0; l < P; l++) omp_destroy_lock(&mtx[l]);
+    free(mtx);
+#endif
+    free(datPM); free(ctfPM); free(sigPM); free(traP);
+    return 0;
+}
+
+/*
+ * FP64 judge of the same marginals: dvp[l][m][n] in FP64 from the FP64
+ * projection and phases, baseline = max over (m, n) (two passes, so the
+ * result does not depend on the streaming order of :835-895), then
+ * wC = sum w pR pT, wR[m] = sum_n w pT, wT[n] = sum_m w pR.
+ * dvp (optional) is P x R x T.
+ */
+int thxo_global_scan_f64(const float* vol, int V, int pf,
+                         const int* iCol, const int* iRow, int K, int N,
+                         const double* quat, int R, const double* trans, int T,
+                         const float* datP, const float* ctfP,
+                         const float* sigRcpP, int P,
+                         const double* pR, const double* pT,
+                         double* wC, double* wR, double* wT, double* baseL,
+                         double* dvp, int nThreads)
+{
+#ifdef _OPENMP
+    if (nThreads > 0) omp_set_num_threads(nThreads);
+#else
+    (void)nThreads;
+#endif
+    double* rot = (double*)malloc(sizeof(double) * 2 * (size_t)R * K);
+    double* tra = (double*)malloc(sizeof(double) * 2 * (size_t)T * K);
+    double* sc = (double*)malloc(sizeof(double) * (size_t)P * R * T);
+    if (!rot || !tra || !sc) { free(rot); free(tra); free(sc); return -1; }
+    #pragma omp parallel for schedule(dynamic)
+    for (int m = 0; m < R; m++) {
+        double M[9];
+        thxo_rotmat(M, quat + 4 * m);
+        thxo_project_f64(rot + (size_t)2 * m * K, vol, V, M, iCol, iRow, K, pf);
+    }
+    for (int n = 0; n < T; n++)
+        for (int i = 0; i < K; i++) {
+            double ph = 2 * M_PI * (iCol[i] * trans[2 * n] / N + iRow[i] * trans[2 * n + 1] / N);
+            tra[(size_t)2 * (n * K + i)] = cos(-ph);
+            tra[(size_t)2 * (n * K + i) + 1] = sin(-ph);
+        }
+    #pragma omp parallel for schedule(dynamic)
+    for (int l = 0; l < P; l++) {
+        const float* d = datP + (size_t)2 * l * K;
+        const float* c = ctfP + (size_t)l * K;
+        const float* s = sigRcpP + (size_t)l * K;
+        for (int m = 0; m < R; m++)
+            for (int n = 0; n < T; n++) {
+                const double* pr = rot + (size_t)2 * m * K;
+                const double* tr = tra + (size_t)2 * n * K;
+                double acc = 0;
+                for (int i = 0; i < K; i++) {
+                    double qr = tr[2 * i] * pr[2 * i] - tr[2 * i + 1] * pr[2 * i + 1];
+                    double qi = tr[2 * i] * pr[2 * i + 1] + tr[2 * i + 1] * pr[2 * i];
+                    double er = d[2 * i] - c[i] * qr, ei = d[2 * i + 1] - c[i] * qi;
+                    acc += (er * er + ei * ei) * s[i];
+                }
+                sc[((size_t)l * R + m) * T + n] = acc;
+            }
+        double mx = -INFINITY;
+        for (size_t q = 0; q < (size_t)R * T; q++)
+            if (sc[(size_t)l * R * T + q] > mx) mx = sc[(size_t)l * R * T + q];
+        baseL[l] = mx;
+        double c0 = 0;
+        for (int m = 0; m < R; m++) wR[(size_t)l * R + m] = 0;
+        for (int n = 0; n < T; n++) wT[(size_t)l * T + n] = 0;
+        for (int m = 0; m < R; m++)
+            for (int n = 0; n < T; n++) {
+                double w = exp(sc[((size_t)l * R + m) * T + n] - mx);
+                c0 += w * pR[m] * pT[n];
+                wR[(size_t)l * R + m] += w * pT[n];
+                wT[(size_t)l * T + n] += w * pR[m];
+            }
+        wC[l] = c0;
+    }
+    if (dvp) memcpy(dvp, sc, sizeof(double) * (size_t)P * R * T);
+    free(rot); free(tra); free(sc);
+    return 0;
+}
+
+/*
+ * Particle::resample(n, pt), src/Particle.cpp:1275-1462, one sample type,
+ * without the random shuffle (:2173-2271) and with u0 injected:
+ * w <- w (.) u, normalise, cdf (d_cumsum) / cdf[last], u_j = u0 + j / nOut,
+ * advance i while u_j > cdf[i]; new weight 1 / u_i (PARTICLE_PRIOR_ONE,
+ * include/Config.h:63), then normW (src/Particle.cpp:813-819) normalises.
+ * Returns 0, or -1 on bad input.
+ */
+int thxo_resample(const double* w, const double* u, int nIn, int nOut,
+                  double u0, int* idx, double* wNew)
+{
+    if (nIn <= 0 || nOut <= 0) return -1;
+    double* cdf = (double*)malloc(sizeof(double) * nIn);
+    if (!cdf) return -1;
+    double sum = 0;
+    for (int i = 0; i < nIn; i++) sum += w[i] * u[i];
+    double acc = 0;
+    for (int i = 0; i < nIn; i++) {
+        acc += (w[i] * u[i]) / sum;
+        cdf[i] = acc;
+    }
+    double last = cdf[nIn - 1];
+    for (int i = 0; i < nIn; i++) cdf[i] /= last;
+    int i = 0;
+    double wsum = 0;
+    for (int j = 0; j < nOut; j++) {
+        double uj = u0 + j * 1.0 / nOut;
+        while (uj > cdf[i] && i < nIn - 1) i++;
+        idx[j] = i;
+        wNew[j] = 1.0 / u[i];
+        wsum += wNew[j];
+    }
+    for (int j = 0; j < nOut; j++) wNew[j] /= wsum;
+    free(cdf);
+    return 0;
+}
+
+/*
+ * Closed-form FT of f(x) = sum_b a_b exp(-|x - c_b|^2 / (2 s^2)) (x in voxels
+ * of the N^3 box, origin at the box centre) sampled at padded-grid index m of
+ * a V = pf N transform:
+ * F(m) = sum_b a_b (2 pi)^{3/2} s^3 exp(-2 pi^2 s^2 |m|^2 / V^2) exp(-2 pi i m.c_b / V).
+ */
+void thxo_blob_ft(float* dst, int V, int N, const double* centres,
+                  const double* amps, int nBlob, double sigma,
+                  const int* ix, const int* iy, const int* iz, int n)
+{
+    (void)N;
+    double g0 = pow(2 * M_PI, 1.5) * sigma * sigma * sigma;
+    for (int q = 0; q < n; q++) {
+        double m2 = (double)ix[q] * ix[q] + (double)iy[q] * iy[q] + (double)iz[q] * iz[q];
+        double g = g0 * exp(-2 * M_PI * M_PI * sigma * sigma * m2 / ((double)V * V));
+        double re = 0, im = 0;
+        for (int b = 0; b < nBlob; b++) {
+            double ph = -2 * M_PI * (ix[q] * centres[3 * b] + iy[q] * centres[3 * b + 1] +
+                                     iz[q] * centres[3 * b + 2]) / V;
+            re += amps[b] * cos(ph);
+            im += amps[b] * sin(ph);
+        }
+        dst[2 * q] = (float)(g * re);
+        dst[2 * q + 1] = (float)(g * im);
+    }
+}

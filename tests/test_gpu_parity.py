@@ -1,0 +1,180 @@
+"""GPU parity of the expectation step against the CPU oracle (needs gfx950).
+
+Tolerances (stated here, per kernel):
+  * projection, phases, rotation matrices: max |gpu - oracle| <= 1e-5 * max|oracle|
+    (same FP32 arithmetic, only FMA contraction and libm ulps differ);
+  * plain FP32 scores: relative log-likelihood error <= 1e-4 against the FP64
+    oracle and identical top-ranked pose;
+  * marginals wC / wR / wT (FP64 merge of FP32 tile sums): relative error
+    <= 1e-3 of each particle's largest entry.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import thx
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+BOX = 64
+PF = 2
+V = BOX * PF
+
+
+@pytest.fixture(scope="module")
+def small(gpu_ctx):
+    """box-64 pixel set, synthetic volume and 40 particles, 192 x 49 grid."""
+    print("\n[setup] box64 P=40 R=192 T=49", flush=True)
+    info = wl.setup_context(gpu_ctx, "box64", n_particles=40, n_rot=192)
+    vol = gpu_ctx.get_volume(0, V)
+    dat, ctf, sig = gpu_ctx.get_particles()
+    info.update(vol=vol, dat=dat, ctf=ctf, sig=sig)
+    return info
+
+
+def test_pixel_set_matches_oracle():
+    a = wl.pixel_set(BOX, 24)
+    b = oracle.pixel_set(BOX, 24)
+    for x, y in zip(a, b[:3]):
+        np.testing.assert_array_equal(x, y)
+    assert len(a[0]) == 871
+
+
+def test_rotran_matches_oracle(gpu_ctx, small):
+    quat, trans = small["quat"][:37], small["trans"]
+    traP, rotMat = thx.expect_rotran(trans, quat, small["iCol"], small["iRow"], BOX)
+    for r in range(len(quat)):
+        M = oracle.rotmat(quat[r])
+        np.testing.assert_allclose(rotMat[r].reshape(3, 3).T, M, atol=1e-14)
+    for t in range(len(trans)):
+        ref = oracle.translate(trans[t, 0], trans[t, 1], BOX, small["iCol"], small["iRow"])
+        np.testing.assert_allclose(traP[t], ref, atol=2e-6)
+
+
+def test_project_matches_oracle(gpu_ctx, small):
+    quat = small["quat"][:64]
+    _, rotMat = thx.expect_rotran(small["trans"][:1], quat, small["iCol"], small["iRow"], BOX)
+    rotP = thx.expect_project(small["vol"], rotMat, small["iCol"], small["iRow"], PF, V)
+    scale = 0.0
+    err = 0.0
+    for r in range(len(quat)):
+        ref = oracle.project(small["vol"], V, oracle.rotmat(quat[r]), small["iCol"],
+                             small["iRow"], PF)
+        scale = max(scale, np.abs(ref).max())
+        err = max(err, np.abs(rotP[r] - ref).max())
+    print(f"project max abs err {err:.3e} of scale {scale:.3e}")
+    assert err <= 1e-5 * scale
+
+
+def test_synth_volume_matches_closed_form(gpu_ctx, small):
+    rng = np.random.default_rng(1)
+    n = 300
+    ix = rng.integers(0, V // 2 + 1, n)
+    iy = rng.integers(-V // 2, V // 2, n)
+    iz = rng.integers(-V // 2, V // 2, n)
+    w = wl.WORKLOADS["box64"]
+    ref = oracle.blob_ft(V, BOX, small["centres"], np.ones(len(small["centres"])),
+                         w["blob_sigma"], ix, iy, iz)
+    got = small["vol"][iz % V, iy % V, ix]
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 2e-5 * scale
+
+
+def test_synth_ctf_matches_oracle(gpu_ctx, small):
+    cp = small["ctf_params"]
+    c = wl.CTF_DEFAULTS
+    for p in range(0, 40, 7):
+        ref = oracle.ctf(small["iCol"], small["iRow"], BOX, c["pixelSize"], c["voltage"],
+                         cp[p, 0], cp[p, 1], cp[p, 2], c["Cs"], c["ampContrast"])
+        np.testing.assert_allclose(small["ctf"][p], ref, atol=2e-5)
+
+
+@pytest.fixture(scope="module")
+def f64_ref(small):
+    P, R, T = 40, len(small["quat"]), len(small["trans"])
+    pR, pT = np.full(R, 1.0 / R), np.full(T, 1.0 / T)
+    out = oracle.global_scan(small["vol"], V, PF, small["iCol"], small["iRow"], BOX,
+                             small["quat"], small["trans"], small["dat"], small["ctf"],
+                             small["sig"], pR, pT, f64=True, want_dvp=True)
+    return out
+
+
+def test_fp32_scores_match_oracle(gpu_ctx, small, f64_ref):
+    dvp = f64_ref[4]
+    for p in (0, 13, 39):
+        got = gpu_ctx.score_one(0, p)
+        rel = np.abs(got - dvp[p]).max() / np.abs(dvp[p]).max()
+        print(f"particle {p}: max rel dvp err {rel:.2e}")
+        assert rel <= 1e-4
+
+
+def test_fp32_global_scan_matches_oracle(gpu_ctx, small, f64_ref):
+    wC0, wR0, wT0, base0, dvp = f64_ref
+    gpu_ctx.score_global(0, thx.FP32)
+    wC, wR, wT, base = gpu_ctx.get_global(1)
+    bR, bT, bS = gpu_ctx.get_best(0)
+    rel_base = np.abs(base - base0) / np.abs(base0)
+    print(f"FP32 baseline max rel err {rel_base.max():.2e}")
+    assert rel_base.max() <= 1e-4
+    flat = dvp.reshape(len(dvp), -1).argmax(1)
+    R, T = dvp.shape[1], dvp.shape[2]
+    np.testing.assert_array_equal(bR, flat // T)
+    np.testing.assert_array_equal(bT, flat % T)
+    np.testing.assert_allclose(wC[:, 0], wC0, rtol=1e-3)
+    for p in range(len(wR)):
+        assert np.abs(wR[p, 0] - wR0[p]).max() <= 1e-3 * wR0[p].max()
+        assert np.abs(wT[p, 0] - wT0[p]).max() <= 1e-3 * wT0[p].max()
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_low_precision_top_pose(gpu_ctx, small, f64_ref, prec):
+    dvp = f64_ref[4]
+    gpu_ctx.score_global(0, thx.PRECISIONS[prec])
+    bR, bT, bS = gpu_ctx.get_best(0)
+    flat = dvp.reshape(len(dvp), -1)
+    T = dvp.shape[2]
+    best = flat.max(1)
+    rel = np.abs(bS - best) / np.abs(best)
+    agree = np.mean((bR == flat.argmax(1) // T) & (bT == flat.argmax(1) % T))
+    # score of the chosen pose under FP64 versus the FP64 best
+    chosen = flat[np.arange(len(flat)), bR * T + bT]
+    gap = (best - chosen)
+    print(f"{prec}: max rel best-score err {rel.max():.2e}, top-pose agreement {agree:.3f}, "
+          f"max FP64 gap of chosen pose {gap.max():.3e}")
+    assert rel.max() <= 2e-3
+    assert agree >= 0.9
+
+
+def test_expect_global3d_two_classes(gpu_ctx, small):
+    """Reference-shaped ExpectGlobal3D over two classes with the running baseline."""
+    P, R, T = 12, 64, len(small["trans"])
+    quat = small["quat"][:R]
+    iCol, iRow = small["iCol"], small["iRow"]
+    K = len(iCol)
+    traP, rotMat = thx.expect_rotran(small["trans"], quat, iCol, iRow, BOX)
+    vol0 = small["vol"]
+    vol1 = vol0.copy()
+    vol1[..., 1] *= -1.0      # a second, different class (mirrored map)
+    rot0 = thx.expect_project(vol0, rotMat, iCol, iRow, PF, V)
+    rot1 = thx.expect_project(vol1, rotMat, iCol, iRow, PF, V)
+    dat, ctf, sig = small["dat"][:P], small["ctf"][:P], small["sig"][:P]
+    pR = np.full(R, 1.0 / R)
+    pT = np.full(T, 1.0 / T)
+    nK = 2
+    wC = np.zeros((P, nK), np.float32)
+    wR = np.zeros((P, nK, R), np.float32)
+    wT = np.zeros((P, nK, T), np.float32)
+    base = np.zeros(P, np.float32)
+    thx.expect_global3d(rot0, traP, dat, ctf, sig, pR, pT, 0, nK, wC, wR, wT, base)
+    thx.expect_global3d(rot1, traP, dat, ctf, sig, pR, pT, 1, nK, wC, wR, wT, base)
+    refs = [oracle.global_scan(v, V, PF, iCol, iRow, BOX, quat, small["trans"], dat, ctf, sig,
+                               pR, pT, f64=True) for v in (vol0, vol1)]
+    b = np.maximum(refs[0][3], refs[1][3])
+    np.testing.assert_allclose(base, b, rtol=1e-4)
+    for k in range(2):
+        g = np.exp(refs[k][3] - b)
+        np.testing.assert_allclose(wC[:, k], g * refs[k][0], rtol=2e-3, atol=1e-30)
+        for p in range(P):
+            sc = max(wR[p].max(), 1e-300)
+            assert np.abs(wR[p, k] - g[p] * refs[k][1][p]).max() <= 2e-3 * sc

@@ -1,0 +1,56 @@
+"""Build recipe of the expectation-step library and of the CPU oracle.
+
+`build_library()` compiles the HIP sources for gfx950 with hipcc into
+`libthunder_expect.so` next to this file; `build_oracle()` compiles the C
+oracle (test infrastructure) with gcc into `oracle/liboracle_thx.so`.
+Both always recompile.  Run as `python build.py` from anywhere.
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB = os.path.join(PKG, "libthunder_expect.so")
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_thx.so")
+
+HIP_SOURCES = ["thx_kernels.hip", "thx_api.hip"]
+
+HIPCC_FLAGS = [
+    "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",
+    # packed FP32 VALU ops are disabled for every HIP translation unit
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_library():
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    srcs = [os.path.join(PKG, "csrc", s) for s in HIP_SOURCES]
+    tmp = LIB + ".tmp"
+    _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"),
+                                  "-I", os.path.join(PKG, "csrc"), "-o", tmp] + srcs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    src = os.path.join(ROOT, "oracle", "thx_oracle.c")
+    tmp = ORACLE_LIB + ".tmp"
+    _run(["gcc", "-O2", "-fopenmp", "-mavx2", "-mfma", "-fPIC", "-shared",
+          "-std=gnu11", "-o", tmp, src, "-lm"])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["lib", "oracle"]
+    if "oracle" in what:
+        build_oracle()
+    if "lib" in what:
+        build_library()

@@ -1,0 +1,822 @@
+// C ABI of the expectation step: context management, the reference-shaped
+// entry points and the synthetic workload generator (host side).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "thx_internal.hpp"
+
+namespace thx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+void clear_error() { g_err.clear(); }
+
+int DevBuf::ensure(size_t need) {
+    if (need <= bytes && p) return THX_OK;
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (e != hipSuccess) {
+            set_error(std::string("hipFree failed: ") + hipGetErrorString(e));
+            return THX_ERR_HIP;
+        }
+    }
+    if (need == 0) return THX_OK;
+    hipError_t e = hipMalloc(&p, need);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        set_error("hipMalloc of " + std::to_string(need) + " bytes failed: " + hipGetErrorString(e));
+        return THX_ERR_OOM;
+    }
+    bytes = need;
+    return THX_OK;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// Translation chunking: rows per particle TC in {32, 64, 128} minimising the
+// padded count ceil(T / TC) * TC, ties to the larger TC.
+static void choose_tc(int T, int& TC, int& nTc) {
+    int best = 128, bestPad = ((T + 127) / 128) * 128;
+    for (int tc : {64, 32}) {
+        const int pad = ((T + tc - 1) / tc) * tc;
+        if (pad < bestPad) { best = tc; bestPad = pad; }
+    }
+    TC = best;
+    nTc = (T + best - 1) / best;
+}
+
+static int set_device(thx_ctx* c) {
+    THX_HIP(hipSetDevice(c->device));
+    return THX_OK;
+}
+
+static int ensure_grid_buffers(thx_ctx* c) {
+    THX_TRY(c->rotP.ensure(sizeof(float2) * (size_t)c->Rpad * c->Kp));
+    THX_TRY(c->normP.ensure(sizeof(float) * (size_t)c->Rpad * c->Kp));
+    THX_TRY(c->S.ensure(sizeof(float2) * (size_t)c->Tpad * c->Kp));
+    return THX_OK;
+}
+
+static int ensure_scan_buffers(thx_ctx* c, int cls) {
+    const size_t P = (size_t)c->P;
+    const size_t nt = (size_t)c->nTc * c->nRb;
+    THX_TRY(c->Cterm.ensure(sizeof(float) * (size_t)c->Ppad * c->Rpad));
+    THX_TRY(c->partMax.ensure(sizeof(float) * P * nt));
+    THX_TRY(c->partWC.ensure(sizeof(float) * P * nt));
+    THX_TRY(c->partArg.ensure(sizeof(int) * P * nt));
+    THX_TRY(c->partWR.ensure(sizeof(float) * P * c->nTc * c->Rpad));
+    THX_TRY(c->partWT.ensure(sizeof(float) * P * c->nRb * c->Tpad));
+    if ((int)c->res.size() <= cls) c->res.resize(cls + 1);
+    ClassResult& r = c->res[cls];
+    THX_TRY(r.wC.ensure(sizeof(double) * P));
+    THX_TRY(r.wR.ensure(sizeof(double) * P * c->R));
+    THX_TRY(r.wT.ensure(sizeof(double) * P * c->T));
+    THX_TRY(r.base.ensure(sizeof(double) * P));
+    THX_TRY(r.bestR.ensure(sizeof(int) * P));
+    THX_TRY(r.bestT.ensure(sizeof(int) * P));
+    THX_TRY(r.bestScore.ensure(sizeof(double) * P));
+    return THX_OK;
+}
+
+static int record(thx_ctx* c, const char* name) {
+    if (c->nev >= 16) return THX_OK;
+    THX_HIP(hipEventRecord(c->ev[c->nev], c->stream));
+    c->evNames.push_back(name);
+    c->nev++;
+    return THX_OK;
+}
+
+// Scoring of the particles against the current rotP / normP / S; the
+// projection (or the caller's upload) has already filled rotP and normP.
+static int run_scan(thx_ctx* c, int cls, int precision) {
+    THX_TRY(ensure_scan_buffers(c, cls));
+    if (precision == THX_FP16 || precision == THX_FP8) {
+        THX_TRY(c->shellScale.ensure(sizeof(float) * (c->maxShell + 1)));
+        THX_TRY(c->shellMax.ensure(sizeof(float) * 2 * (c->maxShell + 1)));
+        THX_TRY(launch_shell_scale(c, precision));
+        THX_TRY(record(c, "shell_scale"));
+    }
+    THX_TRY(launch_prep_particles(c, precision));
+    THX_TRY(record(c, "prep"));
+    if (precision != THX_FP32) {
+        THX_TRY(c->rotLo.ensure(sizeof(uint32_t) * (size_t)c->Rpad * c->Kp));
+        THX_TRY(launch_convert_lo(c, precision));
+        THX_TRY(record(c, "convert"));
+    }
+    THX_TRY(launch_norm_gemm(c));
+    THX_TRY(record(c, "norm_gemm"));
+    THX_TRY(launch_score(c, precision));
+    THX_TRY(record(c, "score"));
+    THX_TRY(launch_merge(c, cls));
+    THX_TRY(record(c, "merge"));
+    c->res[cls].valid = true;
+    return THX_OK;
+}
+
+static std::mutex g_default_mu;
+static thx_ctx* g_default = nullptr;
+
+static int default_ctx(thx_ctx** out) {
+    std::lock_guard<std::mutex> lk(g_default_mu);
+    if (!g_default) THX_TRY(thx_ctx_create(0, &g_default));
+    *out = g_default;
+    return THX_OK;
+}
+
+}  // namespace thx
+
+using namespace thx;
+
+extern "C" {
+
+const char* thx_last_error(void) { return g_err.c_str(); }
+
+int thx_device_count(int* n) {
+    THX_CHECK(n, THX_ERR_ARG, "thx_device_count: null output");
+    int cnt = 0;
+    hipError_t e = hipGetDeviceCount(&cnt);
+    if (e != hipSuccess) {
+        *n = 0;
+        set_error(std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+        return THX_ERR_HIP;
+    }
+    *n = cnt;
+    return THX_OK;
+}
+
+int thx_ctx_create(int device, thx_ctx** out) {
+    clear_error();
+    THX_CHECK(out, THX_ERR_ARG, "thx_ctx_create: null output");
+    int n = 0;
+    THX_TRY(thx_device_count(&n));
+    THX_CHECK(device >= 0 && device < n, THX_ERR_ARG, "thx_ctx_create: no such device");
+    thx_ctx* c = new thx_ctx();
+    c->device = device;
+    if (set_device(c) != THX_OK) { delete c; return THX_ERR_HIP; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        delete c;
+        return THX_ERR_HIP;
+    }
+    for (int i = 0; i < 16; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            set_error("hipEventCreate failed");
+            return THX_ERR_HIP;
+        }
+    *out = c;
+    return THX_OK;
+}
+
+int thx_ctx_destroy(thx_ctx* c) {
+    if (!c) return THX_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->iCol, &c->iRow, &c->iSig, &c->dat, &c->ctf, &c->sig, &c->Aterm,
+                      &c->Y, &c->Wt, &c->quat, &c->trans, &c->pR, &c->pT, &c->pRf, &c->pTf,
+                      &c->S, &c->rotP, &c->rotLo, &c->normP, &c->Cterm, &c->shellScale,
+                      &c->shellMax, &c->partMax, &c->partWC, &c->partArg, &c->partWR,
+                      &c->partWT};
+    for (DevBuf* b : bufs) b->release();
+    for (auto& v : c->vol) v.release();
+    for (auto& r : c->res) {
+        r.wC.release(); r.wR.release(); r.wT.release(); r.base.release();
+        r.bestR.release(); r.bestT.release(); r.bestScore.release();
+    }
+    for (int i = 0; i < 16; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return THX_OK;
+}
+
+int thx_ctx_set_pixels(thx_ctx* c, const int* iCol, const int* iRow, const int* iSig,
+                       int K, int idim, int pf) {
+    clear_error();
+    THX_CHECK(c && iCol && iRow, THX_ERR_ARG, "set_pixels: null argument");
+    THX_CHECK(K > 0 && idim > 0 && pf > 0, THX_ERR_ARG, "set_pixels: bad size");
+    THX_TRY(set_device(c));
+    const int Kp = round_up(K, kChunk);
+    std::vector<int> col(Kp, 0), row(Kp, 0), sg(Kp, 0);
+    int maxShell = 0;
+    double maxR = 0;
+    for (int k = 0; k < K; k++) {
+        col[k] = iCol[k];
+        row[k] = iRow[k];
+        THX_CHECK(iCol[k] >= 0, THX_ERR_ARG, "set_pixels: iCol must be >= 0 (half plane)");
+        const double rr = std::hypot((double)iCol[k], (double)iRow[k]);
+        maxR = std::max(maxR, rr);
+        sg[k] = iSig ? iSig[k] : (int)std::rint(rr);
+        THX_CHECK(sg[k] >= 0, THX_ERR_ARG, "set_pixels: negative shell index");
+        maxShell = std::max(maxShell, sg[k]);
+    }
+    THX_CHECK(maxR * pf + 2 <= (double)idim * pf / 2, THX_ERR_RANGE,
+              "set_pixels: pixel radius too close to Nyquist of the padded volume");
+    THX_TRY(c->iCol.ensure(sizeof(int) * Kp));
+    THX_TRY(c->iRow.ensure(sizeof(int) * Kp));
+    THX_TRY(c->iSig.ensure(sizeof(int) * Kp));
+    THX_HIP(hipMemcpy(c->iCol.p, col.data(), sizeof(int) * Kp, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->iRow.p, row.data(), sizeof(int) * Kp, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->iSig.p, sg.data(), sizeof(int) * Kp, hipMemcpyHostToDevice));
+    c->K = K;
+    c->Kp = Kp;
+    c->idim = idim;
+    c->pf = pf;
+    c->maxShell = maxShell;
+    c->P = 0;   // particle arrays depend on K
+    c->R = c->T = 0;
+    return THX_OK;
+}
+
+int thx_ctx_set_volume(thx_ctx* c, int cls, const float* vol, int vdim) {
+    clear_error();
+    THX_CHECK(c && vol && cls >= 0 && vdim > 0 && vdim % 2 == 0, THX_ERR_ARG,
+              "set_volume: bad argument");
+    THX_TRY(set_device(c));
+    if ((int)c->vol.size() <= cls) {
+        c->vol.resize(cls + 1);
+        c->vdim.resize(cls + 1, 0);
+    }
+    const size_t n = (size_t)(vdim / 2 + 1) * vdim * vdim;
+    THX_TRY(c->vol[cls].ensure(sizeof(float2) * n));
+    THX_HIP(hipMemcpy(c->vol[cls].p, vol, sizeof(float2) * n, hipMemcpyHostToDevice));
+    c->vdim[cls] = vdim;
+    return THX_OK;
+}
+
+int thx_ctx_get_volume(thx_ctx* c, int cls, float* vol) {
+    clear_error();
+    THX_CHECK(c && vol && cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] > 0,
+              THX_ERR_STATE, "get_volume: no such class");
+    THX_TRY(set_device(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const int V = c->vdim[cls];
+    THX_HIP(hipMemcpy(vol, c->vol[cls].p, sizeof(float2) * (size_t)(V / 2 + 1) * V * V,
+                      hipMemcpyDeviceToHost));
+    return THX_OK;
+}
+
+static int alloc_particles(thx_ctx* c, int P) {
+    c->P = P;
+    c->Ppad = round_up(P, 128);
+    THX_TRY(c->dat.ensure(sizeof(float2) * (size_t)P * c->K));
+    THX_TRY(c->ctf.ensure(sizeof(float) * (size_t)P * c->K));
+    THX_TRY(c->sig.ensure(sizeof(float) * (size_t)P * c->K));
+    THX_TRY(c->Aterm.ensure(sizeof(double) * c->Ppad));
+    THX_TRY(c->Y.ensure(sizeof(float2) * (size_t)c->Ppad * c->Kp));
+    THX_TRY(c->Wt.ensure(sizeof(float) * (size_t)c->Ppad * c->Kp));
+    return THX_OK;
+}
+
+int thx_ctx_set_particles(thx_ctx* c, const float* datP, const float* ctfP,
+                          const float* sigRcpP, int P) {
+    clear_error();
+    THX_CHECK(c && datP && ctfP && sigRcpP && P > 0, THX_ERR_ARG, "set_particles: bad argument");
+    THX_CHECK(c->K > 0, THX_ERR_STATE, "set_particles: pixel set not defined");
+    THX_TRY(set_device(c));
+    THX_TRY(alloc_particles(c, P));
+    const size_t n = (size_t)P * c->K;
+    THX_HIP(hipMemcpy(c->dat.p, datP, sizeof(float2) * n, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->ctf.p, ctfP, sizeof(float) * n, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->sig.p, sigRcpP, sizeof(float) * n, hipMemcpyHostToDevice));
+    return THX_OK;
+}
+
+int thx_ctx_get_particles(thx_ctx* c, float* datP, float* ctfP, float* sigRcpP) {
+    clear_error();
+    THX_CHECK(c && c->P > 0, THX_ERR_STATE, "get_particles: no particles");
+    THX_TRY(set_device(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->P * c->K;
+    if (datP) THX_HIP(hipMemcpy(datP, c->dat.p, sizeof(float2) * n, hipMemcpyDeviceToHost));
+    if (ctfP) THX_HIP(hipMemcpy(ctfP, c->ctf.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (sigRcpP) THX_HIP(hipMemcpy(sigRcpP, c->sig.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return THX_OK;
+}
+
+static int set_grid_shape(thx_ctx* c, int R, int T) {
+    c->R = R;
+    c->T = T;
+    c->Rpad = round_up(R, 128);
+    c->nRb = c->Rpad / 128;
+    choose_tc(T, c->TC, c->nTc);
+    c->Tpad = c->TC * c->nTc;
+    THX_CHECK((long long)c->nTc * c->nRb <= kMaxTiles, THX_ERR_RANGE,
+              "set_grid: too many rotation/translation tiles per particle");
+    return ensure_grid_buffers(c);
+}
+
+static int upload_priors(thx_ctx* c, const double* pR, const double* pT) {
+    std::vector<float> fr(c->Rpad, 0.f), ft(c->Tpad, 0.f);
+    for (int r = 0; r < c->R; r++) fr[r] = (float)pR[r];
+    for (int t = 0; t < c->T; t++) ft[t] = (float)pT[t];
+    THX_TRY(c->pRf.ensure(sizeof(float) * c->Rpad));
+    THX_TRY(c->pTf.ensure(sizeof(float) * c->Tpad));
+    THX_HIP(hipMemcpy(c->pRf.p, fr.data(), sizeof(float) * c->Rpad, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pTf.p, ft.data(), sizeof(float) * c->Tpad, hipMemcpyHostToDevice));
+    return THX_OK;
+}
+
+int thx_ctx_set_grid(thx_ctx* c, const double* quat, int R, const double* trans, int T,
+                     const double* pR, const double* pT) {
+    clear_error();
+    THX_CHECK(c && quat && trans && pR && pT && R > 0 && T > 0, THX_ERR_ARG,
+              "set_grid: bad argument");
+    THX_CHECK(c->K > 0, THX_ERR_STATE, "set_grid: pixel set not defined");
+    THX_TRY(set_device(c));
+    THX_TRY(set_grid_shape(c, R, T));
+    THX_TRY(c->quat.ensure(sizeof(double) * 4 * R));
+    THX_TRY(c->trans.ensure(sizeof(double) * 2 * T));
+    THX_HIP(hipMemcpy(c->quat.p, quat, sizeof(double) * 4 * R, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->trans.p, trans, sizeof(double) * 2 * T, hipMemcpyHostToDevice));
+    THX_TRY(upload_priors(c, pR, pT));
+    THX_TRY(launch_translate_table(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    return THX_OK;
+}
+
+int thx_ctx_score_global(thx_ctx* c, int cls, int precision) {
+    clear_error();
+    THX_CHECK(c, THX_ERR_ARG, "score_global: null context");
+    THX_CHECK(precision >= THX_FP32 && precision <= THX_FP16, THX_ERR_ARG,
+              "score_global: precision must be FP32, BF16 or FP16");
+    THX_CHECK(c->K > 0 && c->P > 0 && c->R > 0 && c->T > 0, THX_ERR_STATE,
+              "score_global: pixels, particles and grid must be set");
+    THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] > 0, THX_ERR_STATE,
+              "score_global: volume of this class not set");
+    THX_CHECK(c->vdim[cls] == c->idim * c->pf, THX_ERR_ARG,
+              "score_global: volume size differs from idim * pf");
+    THX_TRY(set_device(c));
+    c->nev = 0;
+    c->evNames.clear();
+    THX_TRY(record(c, "start"));
+    THX_TRY(launch_project(c, cls, c->quat.as<const double>(), c->R, c->Rpad,
+                           c->rotP.as<float2>(), c->normP.as<float>()));
+    THX_TRY(record(c, "project"));
+    return run_scan(c, cls, precision);
+}
+
+int thx_ctx_sync(thx_ctx* c) {
+    THX_CHECK(c, THX_ERR_ARG, "sync: null context");
+    THX_TRY(set_device(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    return THX_OK;
+}
+
+int thx_ctx_last_timing(thx_ctx* c, char* names, int cap, double* ms, int maxn, int* n) {
+    clear_error();
+    THX_CHECK(c && n, THX_ERR_ARG, "last_timing: null argument");
+    THX_TRY(set_device(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    std::string all;
+    int cnt = 0;
+    for (int i = 1; i < c->nev && cnt < maxn; i++) {
+        float t = 0.f;
+        THX_HIP(hipEventElapsedTime(&t, c->ev[i - 1], c->ev[i]));
+        if (ms) ms[cnt] = t;
+        if (!all.empty()) all += ";";
+        all += c->evNames[i];
+        cnt++;
+    }
+    if (names && cap > 0) {
+        std::strncpy(names, all.c_str(), cap - 1);
+        names[cap - 1] = 0;
+    }
+    *n = cnt;
+    return THX_OK;
+}
+
+int thx_ctx_get_global(thx_ctx* c, int nK, double* wC, double* wR, double* wT, double* baseL) {
+    clear_error();
+    THX_CHECK(c && nK > 0 && wC && wR && wT && baseL, THX_ERR_ARG, "get_global: bad argument");
+    THX_CHECK((int)c->res.size() >= nK, THX_ERR_STATE, "get_global: classes not scored");
+    for (int k = 0; k < nK; k++)
+        THX_CHECK(c->res[k].valid, THX_ERR_STATE, "get_global: a class was not scored");
+    THX_TRY(set_device(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const int P = c->P, R = c->R, T = c->T;
+    std::vector<double> cw(P), cb(P), cr((size_t)P * R), ct((size_t)P * T);
+    for (int l = 0; l < P; l++) baseL[l] = -INFINITY;
+    std::vector<std::vector<double>> bases(nK);
+    for (int k = 0; k < nK; k++) {
+        THX_HIP(hipMemcpy(cb.data(), c->res[k].base.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+        bases[k] = cb;
+        for (int l = 0; l < P; l++) baseL[l] = std::max(baseL[l], cb[l]);
+    }
+    for (int k = 0; k < nK; k++) {
+        const ClassResult& r = c->res[k];
+        THX_HIP(hipMemcpy(cw.data(), r.wC.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(cr.data(), r.wR.p, sizeof(double) * P * R, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(ct.data(), r.wT.p, sizeof(double) * P * T, hipMemcpyDeviceToHost));
+        for (int l = 0; l < P; l++) {
+            const double g = std::exp(bases[k][l] - baseL[l]);
+            wC[(size_t)l * nK + k] = g * cw[l];
+            for (int i = 0; i < R; i++) wR[((size_t)l * nK + k) * R + i] = g * cr[(size_t)l * R + i];
+            for (int i = 0; i < T; i++) wT[((size_t)l * nK + k) * T + i] = g * ct[(size_t)l * T + i];
+        }
+    }
+    return THX_OK;
+}
+
+int thx_ctx_get_best(thx_ctx* c, int cls, int* bestR, int* bestT, double* bestScore) {
+    clear_error();
+    THX_CHECK(c && cls >= 0 && cls < (int)c->res.size() && c->res[cls].valid, THX_ERR_STATE,
+              "get_best: class not scored");
+    THX_TRY(set_device(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const ClassResult& r = c->res[cls];
+    if (bestR) THX_HIP(hipMemcpy(bestR, r.bestR.p, sizeof(int) * c->P, hipMemcpyDeviceToHost));
+    if (bestT) THX_HIP(hipMemcpy(bestT, r.bestT.p, sizeof(int) * c->P, hipMemcpyDeviceToHost));
+    if (bestScore)
+        THX_HIP(hipMemcpy(bestScore, r.bestScore.p, sizeof(double) * c->P, hipMemcpyDeviceToHost));
+    return THX_OK;
+}
+
+int thx_ctx_score_one(thx_ctx* c, int cls, int p, float* out) {
+    clear_error();
+    THX_CHECK(c && out, THX_ERR_ARG, "score_one: null argument");
+    THX_CHECK(c->K > 0 && c->P > 0 && c->R > 0 && c->T > 0, THX_ERR_STATE,
+              "score_one: pixels, particles and grid must be set");
+    THX_CHECK(p >= 0 && p < c->P, THX_ERR_ARG, "score_one: particle out of range");
+    THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] > 0, THX_ERR_STATE,
+              "score_one: volume not set");
+    THX_TRY(set_device(c));
+    THX_TRY(launch_project(c, cls, c->quat.as<const double>(), c->R, c->Rpad,
+                           c->rotP.as<float2>(), c->normP.as<float>()));
+    DevBuf tmp;
+    THX_TRY(tmp.ensure(sizeof(float) * (size_t)c->R * c->T));
+    int s = launch_score_one(c, cls, p, tmp.as<float>());
+    if (s == THX_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpy(out, tmp.p, sizeof(float) * (size_t)c->R * c->T, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            set_error(std::string("score_one: ") + hipGetErrorString(e));
+            s = THX_ERR_HIP;
+        }
+    }
+    tmp.release();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic workloads
+// ---------------------------------------------------------------------------
+
+static void philox_normals(uint64_t seed, uint32_t a, uint32_t stream, double out[4]) {
+    uint32_t w[4], v[4];
+    Philox::draw(seed, a, 0, stream, w);
+    Philox::draw(seed, a, 1, stream, v);
+    const double u1 = Philox::u01(w[0], w[1]), u2 = Philox::u01(w[2], w[3]);
+    const double u3 = Philox::u01(v[0], v[1]), u4 = Philox::u01(v[2], v[3]);
+    const double r1 = std::sqrt(-2 * std::log(u1)), r2 = std::sqrt(-2 * std::log(u3));
+    out[0] = r1 * std::cos(2 * M_PI * u2);
+    out[1] = r1 * std::sin(2 * M_PI * u2);
+    out[2] = r2 * std::cos(2 * M_PI * u4);
+    out[3] = r2 * std::sin(2 * M_PI * u4);
+}
+
+static void philox_uniforms(uint64_t seed, uint32_t a, uint32_t stream, double out[2]) {
+    uint32_t w[4];
+    Philox::draw(seed, a, 2, stream, w);
+    out[0] = Philox::u01(w[0], w[1]);
+    out[1] = Philox::u01(w[2], w[3]);
+}
+
+int thx_ctx_synth_volume(thx_ctx* c, int cls, int idim, int pf, int nBlob, double sigma,
+                         double rBall, uint64_t seed, double* centresOut) {
+    clear_error();
+    THX_CHECK(c && cls >= 0 && idim > 0 && idim % 2 == 0 && pf > 0 && nBlob > 0 && sigma > 0,
+              THX_ERR_ARG, "synth_volume: bad argument");
+    THX_TRY(set_device(c));
+    std::vector<double> cen(3 * (size_t)nBlob);
+    for (int b = 0; b < nBlob; b++) {
+        double g[4], u[2];
+        philox_normals(seed, (uint32_t)b, kStreamBlob, g);
+        philox_uniforms(seed, (uint32_t)b, kStreamBlob, u);
+        const double nrm = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        const double rad = rBall * std::cbrt(u[0]);
+        for (int a = 0; a < 3; a++) cen[3 * b + a] = g[a] / nrm * rad;
+    }
+    if (centresOut) std::memcpy(centresOut, cen.data(), sizeof(double) * cen.size());
+    const int V = idim * pf;
+    if ((int)c->vol.size() <= cls) {
+        c->vol.resize(cls + 1);
+        c->vdim.resize(cls + 1, 0);
+    }
+    THX_TRY(c->vol[cls].ensure(sizeof(float2) * (size_t)(V / 2 + 1) * V * V));
+    c->vdim[cls] = V;
+    DevBuf dc;
+    THX_TRY(dc.ensure(sizeof(double) * cen.size()));
+    THX_HIP(hipMemcpy(dc.p, cen.data(), sizeof(double) * cen.size(), hipMemcpyHostToDevice));
+    int s = launch_synth_volume(c, cls, idim, pf, dc.as<const double>(), nBlob, sigma);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    dc.release();
+    if (s != THX_OK) return s;
+    THX_CHECK(e == hipSuccess, THX_ERR_HIP, "synth_volume: kernel failed");
+    return THX_OK;
+}
+
+int thx_ctx_synth_particles(thx_ctx* c, int cls, int P, double maxShift, double dfMin,
+                            double dfMax, double voltage, double Cs, double ampContrast,
+                            double pixelSize, double snr, uint64_t seed, double* trueQuat,
+                            double* trueShift, double* ctfParams) {
+    clear_error();
+    THX_CHECK(c && P > 0 && snr > 0 && pixelSize > 0, THX_ERR_ARG, "synth_particles: bad argument");
+    THX_CHECK(c->K > 0, THX_ERR_STATE, "synth_particles: pixel set not defined");
+    THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] == c->idim * c->pf,
+              THX_ERR_STATE, "synth_particles: volume of this class not set");
+    THX_TRY(set_device(c));
+    std::vector<double> prm(9 * (size_t)P);
+    for (int p = 0; p < P; p++) {
+        double g[4], u[2], v[2];
+        philox_normals(seed, (uint32_t)p, kStreamPose, g);
+        const double n = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + g[3] * g[3]);
+        for (int a = 0; a < 4; a++) prm[9 * (size_t)p + a] = g[a] / n;
+        philox_uniforms(seed, (uint32_t)p, kStreamShift, u);
+        prm[9 * (size_t)p + 4] = (2 * u[0] - 1) * maxShift;
+        prm[9 * (size_t)p + 5] = (2 * u[1] - 1) * maxShift;
+        philox_uniforms(seed, (uint32_t)p, kStreamCtf, v);
+        double w[2];
+        philox_uniforms(seed ^ 0x9e3779b97f4a7c15ull, (uint32_t)p, kStreamCtf, w);
+        prm[9 * (size_t)p + 6] = dfMin + (dfMax - dfMin) * v[0];
+        prm[9 * (size_t)p + 7] = dfMin + (dfMax - dfMin) * v[1];
+        prm[9 * (size_t)p + 8] = M_PI * w[0];
+        if (trueQuat)
+            for (int a = 0; a < 4; a++) trueQuat[4 * (size_t)p + a] = prm[9 * (size_t)p + a];
+        if (trueShift) {
+            trueShift[2 * (size_t)p] = prm[9 * (size_t)p + 4];
+            trueShift[2 * (size_t)p + 1] = prm[9 * (size_t)p + 5];
+        }
+        if (ctfParams)
+            for (int a = 0; a < 3; a++) ctfParams[3 * (size_t)p + a] = prm[9 * (size_t)p + 6 + a];
+    }
+    THX_TRY(alloc_particles(c, P));
+    DevBuf dp, de;
+    THX_TRY(dp.ensure(sizeof(double) * prm.size()));
+    THX_TRY(de.ensure(sizeof(double) * P));
+    THX_HIP(hipMemcpy(dp.p, prm.data(), sizeof(double) * prm.size(), hipMemcpyHostToDevice));
+    THX_HIP(hipMemsetAsync(de.p, 0, sizeof(double) * P, c->stream));
+    THX_TRY(launch_synth_signal(c, cls, P, dp.as<const double>(), de.as<double>(), voltage, Cs,
+                                ampContrast, pixelSize));
+    std::vector<double> en(P);
+    THX_HIP(hipStreamSynchronize(c->stream));
+    THX_HIP(hipMemcpy(en.data(), de.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    dp.release();
+    de.release();
+    // real-space variance of the band-limited signal (Parseval over the
+    // half plane, doubled): var = 2 sum |S_k|^2 / N^4; sigma_n^2 = var / SNR.
+    const double N = c->idim;
+    double mean = 0;
+    for (int p = 0; p < P; p++) mean += 2 * en[p] / (N * N * N * N);
+    mean /= P;
+    const double sn2 = mean / snr;
+    // per-component sd of the unnormalised DFT of real white noise: N sigma_n / sqrt(2)
+    const double compSigma = N * std::sqrt(sn2 / 2);
+    THX_TRY(launch_synth_noise(c, P, compSigma, seed ^ 0x6e6f697365ull));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-shaped entry points
+// ---------------------------------------------------------------------------
+
+__global__ void k_rotmat_colmajor(const double* __restrict__ q, int nR, double* __restrict__ m) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nR) return;
+    const double w = q[4 * r], x = q[4 * r + 1], y = q[4 * r + 2], z = q[4 * r + 3];
+    const double A[3][3] = {{0, -z, y}, {z, 0, -x}, {-y, x, 0}};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            const double aa = A[a][0] * A[0][b] + A[a][1] * A[1][b] + A[a][2] * A[2][b];
+            m[9 * (size_t)r + a + 3 * b] = (a == b ? 1.0 : 0.0) + 2 * w * A[a][b] + 2 * aa;
+        }
+}
+
+__global__ __launch_bounds__(256) void k_project_mat(
+    const float2* __restrict__ vol, int V, int pf, const double* __restrict__ mat, int R,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, float2* __restrict__ out) {
+    const int r = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K || r >= R) return;
+    const double* m = mat + 9 * (size_t)r;   // column-major
+    const double c0 = (double)(iCol[k] * pf), c1 = (double)(iRow[k] * pf);
+    const float x = (float)(m[0] * c0 + m[3] * c1);
+    const float y = (float)(m[1] * c0 + m[4] * c1);
+    const float z = (float)(m[2] * c0 + m[5] * c1);
+    bool conj = false;
+    float xx = x, yy = y, zz = z;
+    if (xx < 0) { xx = -xx; yy = -yy; zz = -zz; conj = true; }
+    const int x0 = (int)floorf(xx), y0 = (int)floorf(yy), z0 = (int)floorf(zz);
+    const float xd = xx - x0, yd = yy - y0, zd = zz - z0;
+    const float v0[2] = {1 - xd, xd}, v1[2] = {1 - yd, yd}, v2[2] = {1 - zd, zd};
+    const int nColFT = V / 2 + 1;
+    float re = 0.f, im = 0.f;
+    for (int kk = 0; kk < 2; kk++) {
+        int zi = z0 + kk;
+        zi = zi < 0 ? zi + V : zi;
+        for (int j = 0; j < 2; j++) {
+            int yi = y0 + j;
+            yi = yi < 0 ? yi + V : yi;
+            for (int i = 0; i < 2; i++) {
+                const float w = v0[i] * v1[j] * v2[kk];
+                const float2 d = vol[((size_t)zi * V + yi) * nColFT + x0 + i];
+                re += d.x * w;
+                im += d.y * w;
+            }
+        }
+    }
+    out[(size_t)r * K + k] = conj ? make_float2(re, -im) : make_float2(re, im);
+}
+
+__global__ void k_norm_rows(const float2* __restrict__ rotP, int n, float* __restrict__ normP) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n) return;
+    const float2 v = rotP[i];
+    normP[i] = v.x * v.x + v.y * v.y;
+}
+
+int thx_ExpectRotran(float* traP, const double* trans, const double* rot, double* rotMat,
+                     const int* iCol, const int* iRow, int nR, int nT, int idim, int npxl) {
+    clear_error();
+    THX_CHECK(traP && trans && rot && rotMat && iCol && iRow, THX_ERR_ARG,
+              "ExpectRotran: null argument");
+    THX_CHECK(nR > 0 && nT > 0 && idim > 0 && npxl > 0, THX_ERR_ARG, "ExpectRotran: bad size");
+    thx_ctx* c = nullptr;
+    THX_TRY(default_ctx(&c));
+    THX_TRY(set_device(c));
+    DevBuf dCol, dRow, dTr, dS, dQ, dM;
+    int s = THX_OK;
+    do {
+        if ((s = dCol.ensure(sizeof(int) * npxl)) || (s = dRow.ensure(sizeof(int) * npxl)) ||
+            (s = dTr.ensure(sizeof(double) * 2 * nT)) ||
+            (s = dS.ensure(sizeof(float2) * (size_t)nT * npxl)) ||
+            (s = dQ.ensure(sizeof(double) * 4 * nR)) || (s = dM.ensure(sizeof(double) * 9 * nR)))
+            break;
+        if (hipMemcpy(dCol.p, iCol, sizeof(int) * npxl, hipMemcpyHostToDevice) ||
+            hipMemcpy(dRow.p, iRow, sizeof(int) * npxl, hipMemcpyHostToDevice) ||
+            hipMemcpy(dTr.p, trans, sizeof(double) * 2 * nT, hipMemcpyHostToDevice) ||
+            hipMemcpy(dQ.p, rot, sizeof(double) * 4 * nR, hipMemcpyHostToDevice)) {
+            set_error("ExpectRotran: upload failed");
+            s = THX_ERR_HIP;
+            break;
+        }
+        if ((s = launch_translate_raw(c->stream, dTr.as<const double>(), nT, nT,
+                                      dCol.as<const int>(), dRow.as<const int>(), npxl, npxl,
+                                      idim, dS.as<float2>())))
+            break;
+        hipLaunchKernelGGL(k_rotmat_colmajor, dim3((nR + 127) / 128), dim3(128), 0, c->stream,
+                           dQ.as<const double>(), nR, dM.as<double>());
+        if (hipStreamSynchronize(c->stream) != hipSuccess ||
+            hipMemcpy(traP, dS.p, sizeof(float2) * (size_t)nT * npxl, hipMemcpyDeviceToHost) ||
+            hipMemcpy(rotMat, dM.p, sizeof(double) * 9 * nR, hipMemcpyDeviceToHost)) {
+            set_error("ExpectRotran: kernel or download failed");
+            s = THX_ERR_HIP;
+        }
+    } while (0);
+    dCol.release(); dRow.release(); dTr.release(); dS.release(); dQ.release(); dM.release();
+    return s;
+}
+
+int thx_ExpectProject(const float* volume, float* rotP, const double* rotMat, const int* iCol,
+                      const int* iRow, int nR, int pf, int interp, int vdim, int npxl) {
+    clear_error();
+    THX_CHECK(volume && rotP && rotMat && iCol && iRow, THX_ERR_ARG,
+              "ExpectProject: null argument");
+    THX_CHECK(nR > 0 && pf > 0 && vdim > 0 && vdim % 2 == 0 && npxl > 0, THX_ERR_ARG,
+              "ExpectProject: bad size");
+    THX_CHECK(interp == 1, THX_ERR_ARG, "ExpectProject: only LINEAR_INTERP (1) is supported");
+    double maxR = 0;
+    for (int k = 0; k < npxl; k++) {
+        THX_CHECK(iCol[k] >= 0, THX_ERR_ARG, "ExpectProject: iCol must be >= 0");
+        maxR = std::max(maxR, std::hypot((double)iCol[k], (double)iRow[k]));
+    }
+    THX_CHECK(maxR * pf + 2 <= vdim / 2, THX_ERR_RANGE,
+              "ExpectProject: pixel radius too close to Nyquist of the padded volume");
+    thx_ctx* c = nullptr;
+    THX_TRY(default_ctx(&c));
+    THX_TRY(set_device(c));
+    const size_t nv = (size_t)(vdim / 2 + 1) * vdim * vdim;
+    DevBuf dV, dCol, dRow, dM, dOut;
+    int s = THX_OK;
+    do {
+        if ((s = dV.ensure(sizeof(float2) * nv)) || (s = dCol.ensure(sizeof(int) * npxl)) ||
+            (s = dRow.ensure(sizeof(int) * npxl)) || (s = dM.ensure(sizeof(double) * 9 * nR)) ||
+            (s = dOut.ensure(sizeof(float2) * (size_t)nR * npxl)))
+            break;
+        if (hipMemcpy(dV.p, volume, sizeof(float2) * nv, hipMemcpyHostToDevice) ||
+            hipMemcpy(dCol.p, iCol, sizeof(int) * npxl, hipMemcpyHostToDevice) ||
+            hipMemcpy(dRow.p, iRow, sizeof(int) * npxl, hipMemcpyHostToDevice) ||
+            hipMemcpy(dM.p, rotMat, sizeof(double) * 9 * nR, hipMemcpyHostToDevice)) {
+            set_error("ExpectProject: upload failed");
+            s = THX_ERR_HIP;
+            break;
+        }
+        hipLaunchKernelGGL(k_project_mat, dim3((npxl + 255) / 256, nR), dim3(256), 0, c->stream,
+                           dV.as<const float2>(), vdim, pf, dM.as<const double>(), nR,
+                           dCol.as<const int>(), dRow.as<const int>(), npxl, dOut.as<float2>());
+        if (hipStreamSynchronize(c->stream) != hipSuccess ||
+            hipMemcpy(rotP, dOut.p, sizeof(float2) * (size_t)nR * npxl, hipMemcpyDeviceToHost)) {
+            set_error("ExpectProject: kernel or download failed");
+            s = THX_ERR_HIP;
+        }
+    } while (0);
+    dV.release(); dCol.release(); dRow.release(); dM.release(); dOut.release();
+    return s;
+}
+
+int thx_ExpectGlobal3D(const float* rotP, const float* traP, const float* datP, const float* ctfP,
+                       const float* sigRcpP, float* wC, float* wR, float* wT, const double* pR,
+                       const double* pT, float* baseL, int kIdx, int nK, int nR, int nT, int npxl,
+                       int imgNum) {
+    clear_error();
+    THX_CHECK(rotP && traP && datP && ctfP && sigRcpP && wC && wR && wT && pR && pT && baseL,
+              THX_ERR_ARG, "ExpectGlobal3D: null argument");
+    THX_CHECK(nK > 0 && kIdx >= 0 && kIdx < nK && nR > 0 && nT > 0 && npxl > 0 && imgNum > 0,
+              THX_ERR_ARG, "ExpectGlobal3D: bad size");
+    thx_ctx* c = nullptr;
+    THX_TRY(default_ctx(&c));
+    THX_TRY(set_device(c));
+    // pixel set of npxl entries (shell indices are not needed at FP32)
+    c->K = npxl;
+    c->Kp = round_up(npxl, kChunk);
+    c->maxShell = 0;
+    THX_TRY(c->iSig.ensure(sizeof(int) * c->Kp));
+    THX_HIP(hipMemset(c->iSig.p, 0, sizeof(int) * c->Kp));
+    THX_TRY(alloc_particles(c, imgNum));
+    const size_t n = (size_t)imgNum * npxl;
+    THX_HIP(hipMemcpy(c->dat.p, datP, sizeof(float2) * n, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->ctf.p, ctfP, sizeof(float) * n, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->sig.p, sigRcpP, sizeof(float) * n, hipMemcpyHostToDevice));
+    THX_TRY(set_grid_shape(c, nR, nT));
+    THX_TRY(upload_priors(c, pR, pT));
+    // projections and phases into the padded layouts (zero padding)
+    THX_HIP(hipMemset(c->rotP.p, 0, sizeof(float2) * (size_t)c->Rpad * c->Kp));
+    THX_HIP(hipMemset(c->S.p, 0, sizeof(float2) * (size_t)c->Tpad * c->Kp));
+    THX_HIP(hipMemcpy2D(c->rotP.p, sizeof(float2) * c->Kp, rotP, sizeof(float2) * npxl,
+                        sizeof(float2) * npxl, nR, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy2D(c->S.p, sizeof(float2) * c->Kp, traP, sizeof(float2) * npxl,
+                        sizeof(float2) * npxl, nT, hipMemcpyHostToDevice));
+    const size_t nrp = (size_t)c->Rpad * c->Kp;
+    hipLaunchKernelGGL(k_norm_rows, dim3((unsigned)((nrp + 255) / 256)), dim3(256), 0, c->stream,
+                       c->rotP.as<const float2>(), (int)nrp, c->normP.as<float>());
+    THX_HIP(hipGetLastError());
+    c->nev = 0;
+    c->evNames.clear();
+    THX_TRY(record(c, "start"));
+    THX_TRY(run_scan(c, 0, THX_FP32));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const int P = imgNum;
+    std::vector<double> dwC(P), dwR((size_t)P * nR), dwT((size_t)P * nT), dB(P);
+    const ClassResult& r = c->res[0];
+    THX_HIP(hipMemcpy(dwC.data(), r.wC.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(dwR.data(), r.wR.p, sizeof(double) * P * nR, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(dwT.data(), r.wT.p, sizeof(double) * P * nT, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(dB.data(), r.base.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    // class merge with the running baseline (kernel_getMaxBase /
+    // kernel_setBaseLine / kernel_UpdateW semantics)
+    for (int l = 0; l < P; l++) {
+        double base = dB[l];
+        if (kIdx > 0) {
+            const double old = baseL[l];
+            if (dB[l] > old) {
+                const double nf = std::exp(old - dB[l]);
+                for (int k = 0; k < nK; k++) {
+                    wC[(size_t)l * nK + k] = (float)(wC[(size_t)l * nK + k] * nf);
+                    for (int i = 0; i < nR; i++)
+                        wR[((size_t)l * nK + k) * nR + i] = (float)(wR[((size_t)l * nK + k) * nR + i] * nf);
+                    for (int i = 0; i < nT; i++)
+                        wT[((size_t)l * nK + k) * nT + i] = (float)(wT[((size_t)l * nK + k) * nT + i] * nf);
+                }
+                base = dB[l];
+            } else {
+                base = old;
+            }
+        }
+        const double g = std::exp(dB[l] - base);
+        wC[(size_t)l * nK + kIdx] = (float)(g * dwC[l]);
+        for (int i = 0; i < nR; i++)
+            wR[((size_t)l * nK + kIdx) * nR + i] += (float)(g * dwR[(size_t)l * nR + i]);
+        for (int i = 0; i < nT; i++)
+            wT[((size_t)l * nK + kIdx) * nT + i] += (float)(g * dwT[(size_t)l * nT + i]);
+        baseL[l] = (float)base;
+    }
+    return THX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// Internal declarations shared by the HIP translation units of the
+// expectation-step library.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "thunder_expect.h"
+
+namespace thx {
+
+void set_error(const std::string& msg);
+void clear_error();
+
+#define THX_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            thx::set_error(std::string(#call) + " failed: " + hipGetErrorString(e_)); \
+            return THX_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+#define THX_CHECK(cond, code, msg)      \
+    do {                                \
+        if (!(cond)) {                  \
+            thx::set_error(msg);        \
+            return (code);              \
+        }                               \
+    } while (0)
+
+#define THX_TRY(expr)               \
+    do {                            \
+        int s_ = (expr);            \
+        if (s_ != THX_OK) return s_; \
+    } while (0)
+
+// Device buffer that grows on demand and is freed with the context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Tile geometry of the scoring kernel: a block covers 128 (particle,
+// translation) rows x 128 rotations.  TC rows belong to one particle.
+constexpr int kTileM = 128;
+constexpr int kTileN = 128;
+constexpr int kChunk = 32;     // complex pixels per K step
+constexpr int kMaxTiles = 8192; // nTc * nRb per particle handled by the merge
+
+struct ClassResult {
+    DevBuf wC, wR, wT, base;   // double: P, P*R, P*T, P
+    DevBuf bestR, bestT;       // int: P
+    DevBuf bestScore;          // double: P
+    bool valid = false;
+};
+
+struct Timing {
+    std::vector<std::string> names;
+    std::vector<double> ms;
+};
+
+}  // namespace thx
+
+struct thx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // pixel set
+    int K = 0, Kp = 0, idim = 0, pf = 2, maxShell = 0;
+    thx::DevBuf iCol, iRow, iSig;
+    // volumes per class
+    std::vector<thx::DevBuf> vol;
+    std::vector<int> vdim;
+    // particles (image-major, as uploaded) and derived operands
+    int P = 0, Ppad = 0;
+    thx::DevBuf dat, ctf, sig;
+    thx::DevBuf Aterm;      // double [Ppad]
+    thx::DevBuf Y;          // float2 [Ppad][Kp]  (sig ctf conj(X)) * shell scale
+    thx::DevBuf Wt;         // float  [Ppad][Kp]  sig ctf^2
+    int prepPrecision = -1;
+    // pose grid
+    int R = 0, T = 0, Rpad = 0, TC = 128, nTc = 1, Tpad = 0, nRb = 0;
+    thx::DevBuf quat, trans, pR, pT, pRf, pTf;
+    thx::DevBuf S;          // float2 [Tpad][Kp]
+    // scan workspaces
+    thx::DevBuf rotP;       // float2 [Rpad][Kp]
+    thx::DevBuf rotLo;      // 2 x 16-bit (or 2 x 8-bit) [Rpad][Kp]
+    thx::DevBuf normP;      // float [Rpad][Kp]  |P|^2
+    thx::DevBuf Cterm;      // float [Ppad][Rpad]
+    thx::DevBuf shellScale; // float [maxShell+1]
+    thx::DevBuf shellMax;   // float [2][maxShell+1] (scratch of the scale pass)
+    thx::DevBuf partMax, partWC, partArg, partWR, partWT;
+    std::vector<thx::ClassResult> res;
+    // timing
+    hipEvent_t ev[16] = {};
+    int nev = 0;
+    thx::Timing timing;
+    std::vector<std::string> evNames;
+};
+
+namespace thx {
+// Kernel launchers (thx_kernels.hip).  All enqueue on ctx->stream.
+int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
+                   float2* out, float* outNorm);
+int launch_translate_table(thx_ctx* c);
+int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, const int* iCol,
+                         const int* iRow, int K, int Kp, int N, float2* S);
+int launch_prep_particles(thx_ctx* c, int precision);
+int launch_shell_scale(thx_ctx* c, int precision);
+int launch_convert_lo(thx_ctx* c, int precision);
+int launch_norm_gemm(thx_ctx* c);
+int launch_score(thx_ctx* c, int precision);
+int launch_merge(thx_ctx* c, int cls);
+int launch_synth_volume(thx_ctx* c, int cls, int idim, int pf,
+                        const double* dCentres, int nBlob, double sigma);
+int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams,
+                        double* dEnergy, double voltage, double Cs,
+                        double ampContrast, double pixelSize);
+int launch_synth_noise(thx_ctx* c, int P, double noiseSigma, uint64_t seed);
+int launch_score_one(thx_ctx* c, int cls, int p, float* dOut);
+
+// Philox4x32-10 (Salmon et al., SC'11), shared by host and device code.
+struct Philox {
+    __host__ __device__ static inline void mulhilo(uint32_t a, uint32_t b,
+                                                   uint32_t& hi, uint32_t& lo) {
+        uint64_t p = (uint64_t)a * (uint64_t)b;
+        hi = (uint32_t)(p >> 32);
+        lo = (uint32_t)p;
+    }
+    __host__ __device__ static inline void round10(uint32_t c[4], uint32_t k0,
+                                                   uint32_t k1) {
+        for (int r = 0; r < 10; r++) {
+            uint32_t hi0, lo0, hi1, lo1;
+            mulhilo(0xD2511F53u, c[0], hi0, lo0);
+            mulhilo(0xCD9E8D57u, c[2], hi1, lo1);
+            uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+            c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+    }
+    // Four 32-bit words for (seed, a, b, stream).
+    __host__ __device__ static inline void draw(uint64_t seed, uint32_t a,
+                                                uint32_t b, uint32_t stream,
+                                                uint32_t out[4]) {
+        out[0] = a; out[1] = b; out[2] = stream; out[3] = 0x7468756eu;
+        round10(out, (uint32_t)seed, (uint32_t)(seed >> 32));
+    }
+    // Uniform double in (0, 1) from two words.
+    __host__ __device__ static inline double u01(uint32_t hi, uint32_t lo) {
+        uint64_t x = ((uint64_t)hi << 21) ^ (uint64_t)(lo >> 11);
+        x &= ((1ull << 53) - 1);
+        return ((double)x + 0.5) * (1.0 / 9007199254740992.0);
+    }
+};
+
+enum Stream : uint32_t {
+    kStreamBlob = 1,
+    kStreamPose = 2,
+    kStreamShift = 3,
+    kStreamCtf = 4,
+    kStreamNoise = 5
+};
+
+}  // namespace thx

@@ -1,0 +1,986 @@
+// HIP kernels of the expectation step for gfx950 (CDNA4).
+//
+// Data layout in HBM (all row-major, pixel index fastest):
+//   vol    float2 [(V/2+1) * V * V]   half-complex padded volume, x fastest
+//   rotP   float2 [Rpad][Kp]          projected slices (plain path B operand)
+//   rotLo  2 x 16-bit [Rpad][Kp]      low-precision B operand, shell-scaled
+//   normP  float  [Rpad][Kp]          |P_rk|^2
+//   S      float2 [Tpad][Kp]          translation phase factors
+//   Y      float2 [Ppad][Kp]          sigRcp * ctf * conj(X) * shell scale
+//   Wt     float  [Ppad][Kp]          sigRcp * ctf^2
+//   Aterm  double [Ppad]              sum_k sigRcp |X|^2
+//   Cterm  float  [Ppad][Rpad]        sum_k Wt |P|^2
+// The log-likelihood of the reference,
+//   dvp(p,r,t) = sum_k sigRcp_pk |X_pk - ctf_pk S_tk P_rk|^2
+// (kernel_logDataVS, gpu/src/Kernel.cu:947-1004; logDataVSPrior_m_n_huabin,
+// src/Optimiser.cpp:9666-9708) is evaluated as
+//   dvp = Aterm_p + Cterm_pr - 2 Re sum_k (Y_pk S_tk) P_rk
+// where the last sum is a dense (particle x translation) x rotation
+// contraction over 2K reals that runs on MFMA.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <type_traits>
+
+#include "thx_internal.hpp"
+
+namespace thx {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// Geometry helpers (device)
+// ---------------------------------------------------------------------------
+
+// rotate3D (src/Geometry/Euler.cpp:181-189): R = I + 2 q0 [q]x + 2 [q]x^2.
+// Only the first two columns are needed because slices lie in z = 0.
+__device__ inline void quat_cols01(const double* q, double m[3][2]) {
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    // A = [[0,-z,y],[z,0,-x],[-y,x,0]]; (A^2)(a,b) = sum_c A(a,c) A(c,b)
+    const double A[3][3] = {{0, -z, y}, {z, 0, -x}, {-y, x, 0}};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 2; b++) {
+            double aa = A[a][0] * A[0][b] + A[a][1] * A[1][b] + A[a][2] * A[2][b];
+            m[a][b] = (a == b ? 1.0 : 0.0) + 2 * w * A[a][b] + 2 * aa;
+        }
+}
+
+// Trilinear Fourier-space sample of the half-complex volume at (x, y, z) in
+// padded-grid units (Volume::getByInterpolationFT, src/Image/Volume.cpp:
+// 308-332; getFTHalf(w, x0), :481-553).  Mirror and conjugate when x < 0;
+// corners wrap in y and z.  The caller guarantees |(x, y, z)| + 1 < V / 2.
+__device__ inline float2 sample_trilinear(const float2* __restrict__ vol, int V,
+                                          float x, float y, float z) {
+    bool conj = false;
+    if (x < 0) { x = -x; y = -y; z = -z; conj = true; }
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y), z0 = (int)floorf(z);
+    const float xd = x - x0, yd = y - y0, zd = z - z0;
+    const float v0[2] = {1 - xd, xd}, v1[2] = {1 - yd, yd}, v2[2] = {1 - zd, zd};
+    const int nColFT = V / 2 + 1;
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        int zz = z0 + k;
+        zz = zz < 0 ? zz + V : zz;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int yy = y0 + j;
+            yy = yy < 0 ? yy + V : yy;
+            const size_t row = ((size_t)zz * V + yy) * nColFT + x0;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const float w = v0[i] * v1[j] * v2[k];
+                const float2 d = vol[row + i];
+                re += d.x * w;
+                im += d.y * w;
+            }
+        }
+    }
+    return conj ? make_float2(re, -im) : make_float2(re, im);
+}
+
+// CTF(RFLOAT* dst, ...) closed form, src/CTF.cpp:109-145, in float.
+__device__ inline float ctf_value(int ic, int ir, float pixelSize, float lambda,
+                                  float dU, float dV, float theta, float Cs,
+                                  float ampC, int N) {
+    const float w1 = sqrtf(1 - ampC * ampC), w2 = ampC;
+    const float K1 = (float)M_PI * lambda;
+    const float K2 = (float)(M_PI / 2 * (double)Cs * ((double)lambda * lambda * lambda));
+    const float u = (float)hypot((double)(ic / (pixelSize * N)), (double)(ir / (pixelSize * N)));
+    const float angle = (float)(atan2((double)ir, (double)ic) - theta);
+    const float defocus = -(dU + dV + (dU - dV) * cosf(2 * angle)) / 2;
+    const float u2 = u * u;
+    const float ki = K1 * defocus * u2 + K2 * (u2 * u2);
+    return -w1 * sinf(ki) + w2 * cosf(ki);
+}
+
+// ---------------------------------------------------------------------------
+// Projection: one thread per (rotation, pixel).  The rotation matrix is
+// formed in registers from the quaternion (FP64), the coordinates are cast to
+// float as the reference does, and the 8 corner gathers are served from L2 /
+// MALL because consecutive pixels of the set are spatial neighbours.
+// Replaces kernel_getRotMat + kernel_Project3D (gpu/src/Kernel.cu:572-697).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_project(
+    const float2* __restrict__ vol, int V, int pf, const double* __restrict__ quat,
+    int R, const int* __restrict__ iCol, const int* __restrict__ iRow, int K,
+    int Kp, float2* __restrict__ out, float* __restrict__ outNorm) {
+    const int r = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Kp) return;
+    const size_t o = (size_t)r * Kp + k;
+    if (r >= R || k >= K) {
+        out[o] = make_float2(0.f, 0.f);
+        if (outNorm) outNorm[o] = 0.f;
+        return;
+    }
+    double m[3][2];
+    quat_cols01(quat + 4 * (size_t)r, m);
+    const double c0 = (double)(iCol[k] * pf), c1 = (double)(iRow[k] * pf);
+    const float x = (float)(m[0][0] * c0 + m[0][1] * c1);
+    const float y = (float)(m[1][0] * c0 + m[1][1] * c1);
+    const float z = (float)(m[2][0] * c0 + m[2][1] * c1);
+    const float2 v = sample_trilinear(vol, V, x, y, z);
+    out[o] = v;
+    if (outNorm) outNorm[o] = v.x * v.x + v.y * v.y;
+}
+
+int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
+                   float2* out, float* outNorm) {
+    dim3 grid((c->Kp + 255) / 256, Rpad);
+    hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c->stream,
+                       c->vol[cls].as<const float2>(), c->vdim[cls], c->pf, dQuat, R,
+                       c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->Kp,
+                       out, outNorm);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Translation phase table (translate, src/Image/ImageFunctions.cpp:233-251;
+// kernel_Translate, gpu/src/Kernel.cu:443-473): S[t][k] = exp(-i phase).
+// Rows t >= T and pixels k >= K are zero so they contribute nothing.
+// ---------------------------------------------------------------------------
+__global__ void k_translate(const double* __restrict__ trans, int T, int Tpad,
+                            const int* __restrict__ iCol, const int* __restrict__ iRow,
+                            int K, int Kp, int N, float2* __restrict__ S) {
+    const int t = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Kp || t >= Tpad) return;
+    float2 v = make_float2(0.f, 0.f);
+    if (t < T && k < K) {
+        const float rCol = (float)trans[2 * t] / N, rRow = (float)trans[2 * t + 1] / N;
+        const float phase = (float)(2 * M_PI) * (iCol[k] * rCol + iRow[k] * rRow);
+        v = make_float2(cosf(-phase), sinf(-phase));
+    }
+    S[(size_t)t * Kp + k] = v;
+}
+
+int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, const int* iCol,
+                         const int* iRow, int K, int Kp, int N, float2* S) {
+    dim3 grid((Kp + 255) / 256, Tpad);
+    hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, st, trans, T, Tpad, iCol, iRow, K, Kp, N,
+                       S);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+int launch_translate_table(thx_ctx* c) {
+    dim3 grid((c->Kp + 255) / 256, c->Tpad);
+    hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, c->stream, c->trans.as<const double>(),
+                       c->T, c->Tpad, c->iCol.as<const int>(), c->iRow.as<const int>(), c->K,
+                       c->Kp, c->idim, c->S.as<float2>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Particle operands: one block per particle.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prep(
+    const float2* __restrict__ dat, const float* __restrict__ ctf,
+    const float* __restrict__ sig, int P, int K, int Kp, const int* __restrict__ iSig,
+    const float* __restrict__ shellScale, double* __restrict__ Aterm,
+    float2* __restrict__ Y, float* __restrict__ Wt) {
+    const int p = blockIdx.x;
+    double a = 0.0;
+    for (int k = threadIdx.x; k < Kp; k += blockDim.x) {
+        float2 y = make_float2(0.f, 0.f);
+        float w = 0.f;
+        if (p < P && k < K) {
+            const size_t i = (size_t)p * K + k;
+            const float2 x = dat[i];
+            const float cc = ctf[i], s = sig[i];
+            a += (double)s * ((double)x.x * x.x + (double)x.y * x.y);
+            const float sc = s * cc * (shellScale ? shellScale[iSig[k]] : 1.f);
+            y = make_float2(sc * x.x, -sc * x.y);
+            w = s * cc * cc;
+        }
+        Y[(size_t)p * Kp + k] = y;
+        Wt[(size_t)p * Kp + k] = w;
+    }
+    // block reduction of a
+    __shared__ double red[256];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) Aterm[p] = red[0];
+}
+
+int launch_prep_particles(thx_ctx* c, int precision) {
+    const float* scale = (precision == THX_FP16 || precision == THX_FP8)
+                             ? c->shellScale.as<const float>() : nullptr;
+    hipLaunchKernelGGL(k_prep, dim3(c->Ppad), dim3(256), 0, c->stream,
+                       c->dat.as<const float2>(), c->ctf.as<const float>(),
+                       c->sig.as<const float>(), c->P, c->K, c->Kp, c->iSig.as<const int>(),
+                       scale, c->Aterm.as<double>(), c->Y.as<float2>(), c->Wt.as<float>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-shell scales for FP16 / FP8 operands.  The cross term is invariant to
+// a per-pixel factor a_k applied as Y_k a_k and P_k / a_k; a_k is chosen per
+// resolution shell so that both operands sit in the middle of the format's
+// range: a = sqrt(maxP / maxY) * 2^e (e balances the two ranges).
+// ---------------------------------------------------------------------------
+__device__ inline void atomic_max_pos(float* addr, float v) {
+    // v >= 0: the int ordering of non-negative floats matches their order
+    atomicMax(reinterpret_cast<int*>(addr), __float_as_int(v));
+}
+
+__global__ void k_shell_max(const float2* __restrict__ src, int rows, int Kp, int K,
+                            const int* __restrict__ iSig, float* __restrict__ shellMax) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    float m = 0.f;
+    for (int r = blockIdx.y; r < rows; r += gridDim.y) {
+        const float2 v = src[(size_t)r * Kp + k];
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    atomic_max_pos(&shellMax[iSig[k]], m);
+}
+
+__global__ void k_shell_finish(const float* __restrict__ maxP, const float* __restrict__ maxY,
+                               int nShell, float target, float* __restrict__ scale) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nShell) return;
+    const float mp = maxP[s], my = maxY[s];
+    float a = 1.f;
+    if (mp > 0.f && my > 0.f) a = sqrtf(mp / my);
+    // round to a power of two so scaling is exact
+    a = exp2f(rintf(log2f(a)));
+    (void)target;
+    scale[s] = a;
+}
+
+// ---------------------------------------------------------------------------
+// Low-precision B operand: rotLo[r][k] = P_rk / a_shell(k) in bf16 / fp16.
+// ---------------------------------------------------------------------------
+template <int PREC>
+__global__ void k_convert_lo(const float2* __restrict__ src, int rows, int Kp,
+                             const int* __restrict__ iSig, int K,
+                             const float* __restrict__ shellScale,
+                             uint32_t* __restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * Kp) return;
+    const int k = (int)(i % Kp);
+    float2 v = src[i];
+    if (shellScale && k < K) {
+        const float inv = 1.f / shellScale[iSig[k]];
+        v.x *= inv;
+        v.y *= inv;
+    }
+    uint32_t lo, hi;
+    if (PREC == THX_BF16) {
+        __bf16 a = (__bf16)v.x, b = (__bf16)v.y;
+        lo = __builtin_bit_cast(unsigned short, a);
+        hi = __builtin_bit_cast(unsigned short, b);
+    } else {
+        _Float16 a = (_Float16)v.x, b = (_Float16)v.y;
+        lo = __builtin_bit_cast(unsigned short, a);
+        hi = __builtin_bit_cast(unsigned short, b);
+    }
+    dst[i] = lo | (hi << 16);
+}
+
+int launch_shell_scale(thx_ctx* c, int precision) {
+    const int nShell = c->maxShell + 1;
+    if (precision != THX_FP16 && precision != THX_FP8) return THX_OK;
+    THX_HIP(hipMemsetAsync(c->shellMax.p, 0, sizeof(float) * 2 * nShell, c->stream));
+    float* maxP = c->shellMax.as<float>();
+    float* maxY = maxP + nShell;
+    dim3 gridP((c->K + 255) / 256, 64);
+    hipLaunchKernelGGL(k_shell_max, gridP, dim3(256), 0, c->stream, c->rotP.as<const float2>(),
+                       c->Rpad, c->Kp, c->K, c->iSig.as<const int>(), maxP);
+    THX_HIP(hipGetLastError());
+    // Y with unit scale: run the prep once without scales, take its max.
+    hipLaunchKernelGGL(k_prep, dim3(c->Ppad), dim3(256), 0, c->stream,
+                       c->dat.as<const float2>(), c->ctf.as<const float>(),
+                       c->sig.as<const float>(), c->P, c->K, c->Kp, c->iSig.as<const int>(),
+                       (const float*)nullptr, c->Aterm.as<double>(), c->Y.as<float2>(),
+                       c->Wt.as<float>());
+    THX_HIP(hipGetLastError());
+    dim3 gridY((c->K + 255) / 256, 64);
+    hipLaunchKernelGGL(k_shell_max, gridY, dim3(256), 0, c->stream, c->Y.as<const float2>(),
+                       c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), maxY);
+    THX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_shell_finish, dim3((nShell + 63) / 64), dim3(64), 0, c->stream, maxP,
+                       maxY, nShell, 1.f, c->shellScale.as<float>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+int launch_convert_lo(thx_ctx* c, int precision) {
+    const size_t n = (size_t)c->Rpad * c->Kp;
+    const float* scale = (precision == THX_FP16 || precision == THX_FP8)
+                             ? c->shellScale.as<const float>() : nullptr;
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (precision == THX_BF16)
+        hipLaunchKernelGGL(k_convert_lo<THX_BF16>, grid, dim3(256), 0, c->stream,
+                           c->rotP.as<const float2>(), c->Rpad, c->Kp, c->iSig.as<const int>(),
+                           c->K, scale, c->rotLo.as<uint32_t>());
+    else
+        hipLaunchKernelGGL(k_convert_lo<THX_FP16>, grid, dim3(256), 0, c->stream,
+                           c->rotP.as<const float2>(), c->Rpad, c->Kp, c->iSig.as<const int>(),
+                           c->K, scale, c->rotLo.as<uint32_t>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs;
+// this remaps the hardware order so each XCD walks a contiguous range of
+// logical tiles, which keeps one rotation block hot in that XCD's L2 while
+// all particle groups stream past it.
+// ---------------------------------------------------------------------------
+__device__ inline int xcd_logical(int b, int nb) {
+    const int q = nb / 8, rmd = nb % 8;
+    const int xcd = b % 8, local = b / 8;
+    return (xcd < rmd) ? xcd * (q + 1) + local : rmd * (q + 1) + (xcd - rmd) * q + local;
+}
+
+// ---------------------------------------------------------------------------
+// Norm term: Cterm[p][r] = sum_k Wt[p][k] normP[r][k] (FP32 MFMA 32x32x2).
+// Block tile 128 x 128, 4 waves of 64 x 64, K step 64.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 2) void k_norm_gemm(
+    const float* __restrict__ Wt, const float* __restrict__ normP, int Kp, int Rpad,
+    int nPb, int nRb, float* __restrict__ Cterm) {
+    constexpr int LD = 65;
+    __shared__ float As[128 * LD];
+    __shared__ float Bs[128 * LD];
+    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    const int pb = L % nPb, rb = L / nPb;
+    const int p0 = pb * 128, r0 = rb * 128;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = floatx16{0};
+    const int lr = tid >> 1, lh = (tid & 1) * 32;
+    for (int k0 = 0; k0 < Kp; k0 += 64) {
+        // each thread loads 32 consecutive floats of one row of A and of B
+        const float4* ga = reinterpret_cast<const float4*>(Wt + (size_t)(p0 + lr) * Kp + k0 + lh);
+        const float4* gb = reinterpret_cast<const float4*>(normP + (size_t)(r0 + lr) * Kp + k0 + lh);
+        const bool okA = (k0 + lh) < Kp, okB = okA;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            float4 x = okA ? ga[v] : make_float4(0, 0, 0, 0);
+            float4 y = okB ? gb[v] : make_float4(0, 0, 0, 0);
+            float* da = As + lr * LD + lh + 4 * v;
+            float* db = Bs + lr * LD + lh + 4 * v;
+            da[0] = x.x; da[1] = x.y; da[2] = x.z; da[3] = x.w;
+            db[0] = y.x; db[1] = y.y; db[2] = y.z; db[3] = y.w;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 64; kk += 2) {
+            const int kc = kk + (lane >> 5);
+            const float a0 = As[(wm + (lane & 31)) * LD + kc];
+            const float a1 = As[(wm + 32 + (lane & 31)) * LD + kc];
+            const float b0 = Bs[(wn + (lane & 31)) * LD + kc];
+            const float b1 = Bs[(wn + 32 + (lane & 31)) * LD + kc];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = wm + 32 * mt + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int n = wn + 32 * nt + (lane & 31);
+                Cterm[(size_t)(p0 + m) * Rpad + r0 + n] = acc[mt][nt][i];
+            }
+}
+
+int launch_norm_gemm(thx_ctx* c) {
+    const int nPb = c->Ppad / 128, nRb = c->Rpad / 128;
+    hipLaunchKernelGGL(k_norm_gemm, dim3(nPb * nRb), dim3(256), 0, c->stream,
+                       c->Wt.as<const float>(), c->normP.as<const float>(), c->Kp, c->Rpad,
+                       nPb, nRb, c->Cterm.as<float>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Scoring kernel.  A block owns PB = 128 / TC particles x TC translations
+// (rows) and 128 rotations (columns).  Per K step of 32 complex pixels:
+//   A[m][2k] = Re Z, A[m][2k+1] = -Im Z with Z = Y[p(m)][k] S[t(m)][k]
+//   B[n][2k] = Re P, B[n][2k+1] = Im P
+// so that sum_j A[m][j] B[n][j] = Re sum_k Z P.  Z is formed in registers
+// from the particle row and the phase table, never stored in HBM.
+// Epilogue: dvp = Aterm + Cterm - 2 acc, then per particle the tile maximum,
+// exp(dvp - max) and the partial marginals (column sums weighted by pT,
+// row sums weighted by pR) go to HBM for the FP64 merge.
+// ---------------------------------------------------------------------------
+template <int PREC>
+struct ScoreLds {
+    static constexpr bool F32 = (PREC == THX_FP32);
+    static constexpr int LD = F32 ? 65 : 72;                // elements per row
+    static constexpr int ESZ = F32 ? 4 : 2;
+    static constexpr int AB_BYTES = 2 * 128 * LD * ESZ;
+    static constexpr int D_LD = 129;
+    static constexpr int D_BYTES = 128 * D_LD * 4;
+    static constexpr int MAIN = AB_BYTES > D_BYTES ? AB_BYTES : D_BYTES;
+};
+
+template <int PREC>
+__device__ inline unsigned short to16(float x) {
+    if (PREC == THX_BF16) return __builtin_bit_cast(unsigned short, (__bf16)x);
+    return __builtin_bit_cast(unsigned short, (_Float16)x);
+}
+
+template <int PREC, int TC>
+__global__ __launch_bounds__(256, 2) void k_score(
+    const float2* __restrict__ Y, const float2* __restrict__ S,
+    const void* __restrict__ Bop, const float* __restrict__ Cterm,
+    const double* __restrict__ Aterm, const float* __restrict__ pRf,
+    const float* __restrict__ pTf, int P, int R, int T, int Kp, int Rpad,
+    int Tpad, int nRb, int nTc, int nPg, float* __restrict__ partMax,
+    float* __restrict__ partWC, int* __restrict__ partArg,
+    float* __restrict__ partWR, float* __restrict__ partWT) {
+    using L = ScoreLds<PREC>;
+    constexpr int PB = 128 / TC;
+    constexpr bool F32 = L::F32;
+    constexpr int LD = L::LD;
+    __shared__ __attribute__((aligned(16))) char smem[L::MAIN];
+    __shared__ float redMax[PB][4];
+    __shared__ int redArg[PB][4];
+    __shared__ float tileMax[PB];
+    __shared__ int tileArg[PB];
+    __shared__ float colRed[PB][128];
+    __shared__ float rowW[128];
+
+    const int Lg = xcd_logical(blockIdx.x, gridDim.x);
+    const int pg = Lg % nPg;
+    const int rest = Lg / nPg;
+    const int tc = rest % nTc;
+    const int rb = rest / nTc;
+    const int p0 = pg * PB, r0 = rb * 128, t0 = tc * TC;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = floatx16{0};
+
+    // loader roles: row lr (0..127), half lh (16 complex pixels)
+    const int lr = tid >> 1, lh = (tid & 1) * 16;
+    const int lp = p0 + lr / TC, lt = t0 + lr % TC;
+    const float2* yrow = Y + (size_t)lp * Kp + lh;
+    const float2* srow = S + (size_t)lt * Kp + lh;
+    const int lrot = r0 + lr;
+
+    for (int k0 = 0; k0 < Kp; k0 += kChunk) {
+        // ---- stage A (Z = Y S) and B into LDS ----
+        if constexpr (F32) {
+            float* As = reinterpret_cast<float*>(smem);
+            float* Bs = As + 128 * LD;
+            const float4* yv = reinterpret_cast<const float4*>(yrow + k0);
+            const float4* sv = reinterpret_cast<const float4*>(srow + k0);
+            const float4* bv = reinterpret_cast<const float4*>(
+                static_cast<const float2*>(Bop) + (size_t)lrot * Kp + k0 + lh);
+            float* da = As + lr * LD + 2 * lh;
+            float* db = Bs + lr * LD + 2 * lh;
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const float4 y = yv[v], s = sv[v], b = bv[v];
+                // two complex values per float4
+                da[4 * v + 0] = y.x * s.x - y.y * s.y;
+                da[4 * v + 1] = -(y.x * s.y + y.y * s.x);
+                da[4 * v + 2] = y.z * s.z - y.w * s.w;
+                da[4 * v + 3] = -(y.z * s.w + y.w * s.z);
+                db[4 * v + 0] = b.x;
+                db[4 * v + 1] = b.y;
+                db[4 * v + 2] = b.z;
+                db[4 * v + 3] = b.w;
+            }
+        } else {
+            unsigned short* As = reinterpret_cast<unsigned short*>(smem);
+            unsigned short* Bs = As + 128 * LD;
+            const float4* yv = reinterpret_cast<const float4*>(yrow + k0);
+            const float4* sv = reinterpret_cast<const float4*>(srow + k0);
+            const uint4* bv = reinterpret_cast<const uint4*>(
+                static_cast<const uint32_t*>(Bop) + (size_t)lrot * Kp + k0 + lh);
+            uint32_t* da = reinterpret_cast<uint32_t*>(As + lr * LD + 2 * lh);
+            uint4* db = reinterpret_cast<uint4*>(Bs + lr * LD + 2 * lh);
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const float4 y = yv[v], s = sv[v];
+                const float z0r = y.x * s.x - y.y * s.y, z0i = -(y.x * s.y + y.y * s.x);
+                const float z1r = y.z * s.z - y.w * s.w, z1i = -(y.z * s.w + y.w * s.z);
+                da[2 * v + 0] = (uint32_t)to16<PREC>(z0r) | ((uint32_t)to16<PREC>(z0i) << 16);
+                da[2 * v + 1] = (uint32_t)to16<PREC>(z1r) | ((uint32_t)to16<PREC>(z1i) << 16);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) db[v] = bv[v];
+        }
+        __syncthreads();
+        // ---- MFMA ----
+        if constexpr (F32) {
+            const float* As = reinterpret_cast<const float*>(smem);
+            const float* Bs = As + 128 * LD;
+#pragma unroll 8
+            for (int kk = 0; kk < 64; kk += 2) {
+                const int kc = kk + (lane >> 5);
+                const float a0 = As[(wm + (lane & 31)) * LD + kc];
+                const float a1 = As[(wm + 32 + (lane & 31)) * LD + kc];
+                const float b0 = Bs[(wn + (lane & 31)) * LD + kc];
+                const float b1 = Bs[(wn + 32 + (lane & 31)) * LD + kc];
+                acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+            }
+        } else {
+            const unsigned short* As = reinterpret_cast<const unsigned short*>(smem);
+            const unsigned short* Bs = As + 128 * LD;
+#pragma unroll
+            for (int kk = 0; kk < 64; kk += 16) {
+                const int kc = kk + 8 * (lane >> 5);
+                const uint4 ra0 = *reinterpret_cast<const uint4*>(As + (wm + (lane & 31)) * LD + kc);
+                const uint4 ra1 = *reinterpret_cast<const uint4*>(As + (wm + 32 + (lane & 31)) * LD + kc);
+                const uint4 rb0 = *reinterpret_cast<const uint4*>(Bs + (wn + (lane & 31)) * LD + kc);
+                const uint4 rb1 = *reinterpret_cast<const uint4*>(Bs + (wn + 32 + (lane & 31)) * LD + kc);
+                if constexpr (PREC == THX_BF16) {
+                    const bf16x8 a0 = __builtin_bit_cast(bf16x8, ra0), a1 = __builtin_bit_cast(bf16x8, ra1);
+                    const bf16x8 b0 = __builtin_bit_cast(bf16x8, rb0), b1 = __builtin_bit_cast(bf16x8, rb1);
+                    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+                    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+                    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+                    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+                } else {
+                    const halfx8 a0 = __builtin_bit_cast(halfx8, ra0), a1 = __builtin_bit_cast(halfx8, ra1);
+                    const halfx8 b0 = __builtin_bit_cast(halfx8, rb0), b1 = __builtin_bit_cast(halfx8, rb1);
+                    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b0, acc[0][0], 0, 0, 0);
+                    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a0, b1, acc[0][1], 0, 0, 0);
+                    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b0, acc[1][0], 0, 0, 0);
+                    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a1, b1, acc[1][1], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- epilogue: dvp tile into LDS ----
+    float* Ds = reinterpret_cast<float*>(smem);
+    constexpr int DLD = L::D_LD;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = wm + 32 * mt + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int n = wn + 32 * nt + (lane & 31);
+                const int p = p0 + m / TC, t = t0 + m % TC, r = r0 + n;
+                float v = -INFINITY;
+                if (p < P && t < T && r < R)
+                    v = (float)Aterm[p] + Cterm[(size_t)p * Rpad + r] - 2.f * acc[mt][nt][i];
+                Ds[m * DLD + n] = v;
+            }
+    __syncthreads();
+
+    // ---- per-particle maximum and its position ----
+    const int cn = tid & 127, ch = tid >> 7;
+#pragma unroll
+    for (int pi = 0; pi < PB; pi++) {
+        float mx = -INFINITY;
+        int arg = -1;
+        const int mb = pi * TC + ch * (TC / 2);
+        for (int m = mb; m < mb + TC / 2; m++) {
+            const float v = Ds[m * DLD + cn];
+            if (v > mx) { mx = v; arg = m * 128 + cn; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float o = __shfl_xor(mx, off);
+            const int oa = __shfl_xor(arg, off);
+            if (o > mx || (o == mx && oa < arg && oa >= 0)) { mx = o; arg = oa; }
+        }
+        if (lane == 0) { redMax[pi][wave] = mx; redArg[pi][wave] = arg; }
+    }
+    __syncthreads();
+    if (tid < PB) {
+        float mx = redMax[tid][0];
+        int arg = redArg[tid][0];
+        for (int w = 1; w < 4; w++)
+            if (redMax[tid][w] > mx || (redMax[tid][w] == mx && redArg[tid][w] < arg)) {
+                mx = redMax[tid][w];
+                arg = redArg[tid][w];
+            }
+        tileMax[tid] = mx;
+        tileArg[tid] = arg;
+    }
+    __syncthreads();
+
+    // ---- weights and column sums (over translations, weighted by pT) ----
+#pragma unroll
+    for (int pi = 0; pi < PB; pi++) {
+        const float mx = tileMax[pi];
+        const bool live = mx > -INFINITY;
+        float cs = 0.f;
+        const int mb = pi * TC + ch * (TC / 2);
+        for (int m = mb; m < mb + TC / 2; m++) {
+            const float v = Ds[m * DLD + cn];
+            const float w = live ? __expf(v - mx) : 0.f;
+            Ds[m * DLD + cn] = w;
+            cs += w * pTf[t0 + m % TC];
+        }
+        if (ch == 1) colRed[pi][cn] = cs;
+        __syncthreads();
+        if (ch == 0) {
+            cs += colRed[pi][cn];
+            const int p = p0 + pi;
+            if (p < P) partWR[((size_t)p * nTc + tc) * Rpad + r0 + cn] = cs;
+        }
+        __syncthreads();
+    }
+
+    // ---- row sums (over rotations, weighted by pR) ----
+    {
+        const int m = tid >> 1, hc = (tid & 1) * 64;
+        float rs = 0.f;
+        for (int n = hc; n < hc + 64; n++) rs += Ds[m * DLD + n] * pRf[r0 + n];
+        rs += __shfl_xor(rs, 1);
+        if ((tid & 1) == 0) {
+            const int p = p0 + m / TC, t = t0 + m % TC;
+            if (p < P) partWT[((size_t)p * nRb + rb) * Tpad + t] = rs;
+            rowW[m] = rs * pTf[t];
+        }
+    }
+    __syncthreads();
+    if (tid < PB) {
+        const int p = p0 + tid;
+        if (p < P) {
+            float wc = 0.f;
+            for (int m = tid * TC; m < (tid + 1) * TC; m++) wc += rowW[m];
+            const size_t o = ((size_t)p * nTc + tc) * nRb + rb;
+            partMax[o] = tileMax[tid];
+            partWC[o] = wc;
+            // global (r, t) of the argmax
+            const int a = tileArg[tid];
+            partArg[o] = a < 0 ? -1 : ((r0 + (a & 127)) * Tpad + (t0 + (a >> 7) % TC));
+        }
+    }
+}
+
+template <int PREC>
+static void launch_score_prec(thx_ctx* c, const void* B, int nPg, int nblk) {
+#define THX_SCORE_ARGS                                                                         \
+    c->Y.as<const float2>(), c->S.as<const float2>(), B, c->Cterm.as<const float>(),          \
+        c->Aterm.as<const double>(), c->pRf.as<const float>(), c->pTf.as<const float>(), c->P, \
+        c->R, c->T, c->Kp, c->Rpad, c->Tpad, c->nRb, c->nTc, nPg, c->partMax.as<float>(),      \
+        c->partWC.as<float>(), c->partArg.as<int>(), c->partWR.as<float>(), c->partWT.as<float>()
+    if (c->TC == 128)
+        hipLaunchKernelGGL((k_score<PREC, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
+    else if (c->TC == 64)
+        hipLaunchKernelGGL((k_score<PREC, 64>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
+    else
+        hipLaunchKernelGGL((k_score<PREC, 32>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
+#undef THX_SCORE_ARGS
+}
+
+int launch_score(thx_ctx* c, int precision) {
+    const int PB = 128 / c->TC;
+    const int nPg = (c->P + PB - 1) / PB;
+    const long long nblk = (long long)nPg * c->nTc * c->nRb;
+    THX_CHECK(nblk < (1ll << 31), THX_ERR_RANGE, "score grid too large");
+    if (precision == THX_FP32)
+        launch_score_prec<THX_FP32>(c, c->rotP.p, nPg, (int)nblk);
+    else if (precision == THX_BF16)
+        launch_score_prec<THX_BF16>(c, c->rotLo.p, nPg, (int)nblk);
+    else if (precision == THX_FP16)
+        launch_score_prec<THX_FP16>(c, c->rotLo.p, nPg, (int)nblk);
+    else {
+        set_error("precision not supported by the global scorer");
+        return THX_ERR_ARG;
+    }
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 merge of the tile partials, one block per particle: the baseline is
+// the maximum of the tile maxima; every partial is rescaled by
+// exp(tileMax - baseline) (the log-sum-exp form of the streaming baseline of
+// src/Optimiser.cpp:835-895 and of kernel_getMaxBase + kernel_setBaseLine +
+// kernel_UpdateW, gpu/src/Kernel.cu:1052-1222).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_merge(
+    const float* __restrict__ partMax, const float* __restrict__ partWC,
+    const int* __restrict__ partArg, const float* __restrict__ partWR,
+    const float* __restrict__ partWT, int P, int R, int T, int Rpad, int Tpad,
+    int TC, int nTc, int nRb, double* __restrict__ wC, double* __restrict__ wR,
+    double* __restrict__ wT, double* __restrict__ base, int* __restrict__ bestR,
+    int* __restrict__ bestT, double* __restrict__ bestScore) {
+    const int p = blockIdx.x;
+    const int nt = nTc * nRb;
+    __shared__ double f[kMaxTiles];
+    __shared__ double red[256];
+    __shared__ int redi[256];
+    const float* pm = partMax + (size_t)p * nt;
+    double mx = -INFINITY;
+    int arg = -1;
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+        const double v = pm[i];
+        if (v > mx) { mx = v; arg = i; }
+    }
+    red[threadIdx.x] = mx;
+    redi[threadIdx.x] = arg;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            const int oi = redi[threadIdx.x + s];
+            if (o > red[threadIdx.x] || (o == red[threadIdx.x] && oi >= 0 && oi < redi[threadIdx.x])) {
+                red[threadIdx.x] = o;
+                redi[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const double M = red[0];
+    const int bestTile = redi[0];
+    __syncthreads();
+    double wc = 0.0;
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+        const double e = exp((double)pm[i] - M);
+        f[i] = e;
+        wc += e * partWC[(size_t)p * nt + i];
+    }
+    red[threadIdx.x] = wc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        wC[p] = red[0];
+        base[p] = M;
+        const int a = bestTile >= 0 ? partArg[(size_t)p * nt + bestTile] : -1;
+        bestR[p] = a < 0 ? -1 : a / Tpad;
+        bestT[p] = a < 0 ? -1 : a % Tpad;
+        bestScore[p] = M;
+    }
+    // f index = tc * nRb + rb
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        const int rb = r / 128;
+        double s = 0.0;
+        for (int tc = 0; tc < nTc; tc++)
+            s += f[tc * nRb + rb] * partWR[((size_t)p * nTc + tc) * Rpad + r];
+        wR[(size_t)p * R + r] = s;
+    }
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        const int tc = t / TC;
+        double s = 0.0;
+        for (int rb = 0; rb < nRb; rb++)
+            s += f[tc * nRb + rb] * partWT[((size_t)p * nRb + rb) * Tpad + t];
+        wT[(size_t)p * T + t] = s;
+    }
+}
+
+int launch_merge(thx_ctx* c, int cls) {
+    ClassResult& r = c->res[cls];
+    hipLaunchKernelGGL(k_merge, dim3(c->P), dim3(256), 0, c->stream, c->partMax.as<const float>(),
+                       c->partWC.as<const float>(), c->partArg.as<const int>(),
+                       c->partWR.as<const float>(), c->partWT.as<const float>(), c->P, c->R,
+                       c->T, c->Rpad, c->Tpad, c->TC, c->nTc, c->nRb, r.wC.as<double>(),
+                       r.wR.as<double>(), r.wT.as<double>(), r.base.as<double>(),
+                       r.bestR.as<int>(), r.bestT.as<int>(), r.bestScore.as<double>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Direct-form scores of one particle (checking aid): one block per (r, t),
+// dvp = sum_k sig |X - ctf S P|^2 in FP32 (the kernel_logDataVS form).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_score_one(
+    const float2* __restrict__ dat, const float* __restrict__ ctf,
+    const float* __restrict__ sig, const float2* __restrict__ rotP,
+    const float2* __restrict__ S, int p, int K, int Kp, int T, float* __restrict__ out) {
+    const int r = blockIdx.x / T, t = blockIdx.x % T;
+    float acc = 0.f;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const float2 pr = rotP[(size_t)r * Kp + k], s = S[(size_t)t * Kp + k];
+        const float qr = s.x * pr.x - s.y * pr.y, qi = s.x * pr.y + s.y * pr.x;
+        const size_t i = (size_t)p * K + k;
+        const float c = ctf[i];
+        const float er = dat[i].x - c * qr, ei = dat[i].y - c * qi;
+        acc += (er * er + ei * ei) * sig[i];
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+int launch_score_one(thx_ctx* c, int cls, int p, float* dOut) {
+    (void)cls;
+    hipLaunchKernelGGL(k_score_one, dim3(c->R * c->T), dim3(256), 0, c->stream,
+                       c->dat.as<const float2>(), c->ctf.as<const float>(),
+                       c->sig.as<const float>(), c->rotP.as<const float2>(),
+                       c->S.as<const float2>(), p, c->K, c->Kp, c->T, dOut);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic workloads.
+// ---------------------------------------------------------------------------
+
+// Analytic FT of the blob sum on the padded half-complex grid:
+// F(m) = (2 pi)^{3/2} s^3 exp(-2 pi^2 s^2 |m|^2 / V^2) sum_b exp(-2 pi i m.c_b / V).
+__global__ __launch_bounds__(256) void k_synth_volume(
+    const double* __restrict__ centres, int nBlob, double sigma, int V,
+    float2* __restrict__ vol) {
+    const int nColFT = V / 2 + 1;
+    const size_t n = (size_t)nColFT * V * V;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double cs[3 * 256];
+    double re = 0.0, im = 0.0;
+    int mx = 0, my = 0, mz = 0;
+    if (idx < n) {
+        mx = (int)(idx % nColFT);
+        my = (int)((idx / nColFT) % V);
+        mz = (int)(idx / ((size_t)nColFT * V));
+        if (my >= V / 2) my -= V;
+        if (mz >= V / 2) mz -= V;
+    }
+    for (int b0 = 0; b0 < nBlob; b0 += 256) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 3 * 256; i += blockDim.x)
+            cs[i] = (b0 * 3 + i < 3 * nBlob) ? centres[b0 * 3 + i] : 0.0;
+        __syncthreads();
+        const int nb = min(256, nBlob - b0);
+        for (int b = 0; b < nb; b++) {
+            const double ph = -2.0 * (mx * cs[3 * b] + my * cs[3 * b + 1] + mz * cs[3 * b + 2]) / V;
+            double sn, c;
+            sincospi(ph, &sn, &c);
+            re += c;
+            im += sn;
+        }
+    }
+    if (idx < n) {
+        const double m2 = (double)mx * mx + (double)my * my + (double)mz * mz;
+        const double g = pow(2 * M_PI, 1.5) * sigma * sigma * sigma *
+                         exp(-2 * M_PI * M_PI * sigma * sigma * m2 / ((double)V * V));
+        vol[idx] = make_float2((float)(g * re), (float)(g * im));
+    }
+}
+
+int launch_synth_volume(thx_ctx* c, int cls, int idim, int pf, const double* dCentres,
+                        int nBlob, double sigma) {
+    const int V = idim * pf;
+    const size_t n = (size_t)(V / 2 + 1) * V * V;
+    hipLaunchKernelGGL(k_synth_volume, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       dCentres, nBlob, sigma, V, c->vol[cls].as<float2>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// Particle signal CTF (.) S_t (.) P_r at the pixel set.  params per particle:
+// q(4), shift(2), dU, dV, theta (9 doubles).  energy[p] += |signal|^2.
+__global__ __launch_bounds__(256) void k_synth_signal(
+    const float2* __restrict__ vol, int V, int pf, const double* __restrict__ params,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int N,
+    float voltage, float Cs, float ampC, float pixelSize, float2* __restrict__ dat,
+    float* __restrict__ ctfOut, double* __restrict__ energy) {
+    const int p = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0;
+    if (k < K) {
+        const double* q = params + 9 * (size_t)p;
+        double m[3][2];
+        quat_cols01(q, m);
+        const double c0 = (double)(iCol[k] * pf), c1 = (double)(iRow[k] * pf);
+        const float x = (float)(m[0][0] * c0 + m[0][1] * c1);
+        const float y = (float)(m[1][0] * c0 + m[1][1] * c1);
+        const float z = (float)(m[2][0] * c0 + m[2][1] * c1);
+        const float2 v = sample_trilinear(vol, V, x, y, z);
+        const float rCol = (float)q[4] / N, rRow = (float)q[5] / N;
+        const float phase = (float)(2 * M_PI) * (iCol[k] * rCol + iRow[k] * rRow);
+        const float sr = cosf(-phase), si = sinf(-phase);
+        const float lambda = (float)(12.2643247 / sqrt((double)voltage * (1 + voltage * 0.978466e-6)));
+        const float cv = ctf_value(iCol[k], iRow[k], pixelSize, lambda, (float)q[6], (float)q[7],
+                                   (float)q[8], Cs, ampC, N);
+        const float qr = sr * v.x - si * v.y, qi = sr * v.y + si * v.x;
+        const float2 s = make_float2(cv * qr, cv * qi);
+        dat[(size_t)p * K + k] = s;
+        ctfOut[(size_t)p * K + k] = cv;
+        e = (double)s.x * s.x + (double)s.y * s.y;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&energy[p], red[0]);
+}
+
+int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams, double* dEnergy,
+                        double voltage, double Cs, double ampContrast, double pixelSize) {
+    dim3 grid((c->K + 255) / 256, P);
+    hipLaunchKernelGGL(k_synth_signal, grid, dim3(256), 0, c->stream, c->vol[cls].as<const float2>(),
+                       c->vdim[cls], c->pf, dParams, c->iCol.as<const int>(),
+                       c->iRow.as<const int>(), c->K, c->idim, (float)voltage, (float)Cs,
+                       (float)ampContrast, (float)pixelSize, c->dat.as<float2>(),
+                       c->ctf.as<float>(), dEnergy);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// Complex white noise of per-component standard deviation noiseSigma
+// (Box-Muller on Philox words), and sigRcp = -1 / (2 noiseSigma^2).
+__global__ __launch_bounds__(256) void k_synth_noise(float2* __restrict__ dat,
+                                                     float* __restrict__ sig, int P, int K,
+                                                     double noiseSigma, uint64_t seed) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)P * K) return;
+    uint32_t w[4];
+    Philox::draw(seed, (uint32_t)i, (uint32_t)(i >> 32), kStreamNoise, w);
+    const double u1 = Philox::u01(w[0], w[1]), u2 = Philox::u01(w[2], w[3]);
+    const double rad = sqrt(-2.0 * log(u1));
+    double s, c;
+    sincospi(2.0 * u2, &s, &c);
+    float2 d = dat[i];
+    d.x += (float)(noiseSigma * rad * c);
+    d.y += (float)(noiseSigma * rad * s);
+    dat[i] = d;
+    sig[i] = (float)(-1.0 / (2.0 * noiseSigma * noiseSigma));
+}
+
+int launch_synth_noise(thx_ctx* c, int P, double noiseSigma, uint64_t seed) {
+    const size_t n = (size_t)P * c->K;
+    hipLaunchKernelGGL(k_synth_noise, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->dat.as<float2>(), c->sig.as<float>(), P, c->K, noiseSigma, seed);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+}  // namespace thx

@@ -1,0 +1,265 @@
+"""ctypes binding of libthunder_expect.so (the C ABI in include/thunder_expect.h).
+
+Python is only the test and benchmark driver: every computation goes through
+the HIP library.  Loading fails loudly when the library is missing; there is
+no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libthunder_expect.so")
+
+FP32, BF16, FP16, FP8 = 0, 1, 2, 3
+PRECISIONS = {"fp32": FP32, "bf16": BF16, "fp16": FP16, "fp8": FP8}
+
+_lib = None
+
+_i = C.c_int
+_d = C.c_double
+_p = C.c_void_p
+_u64 = C.c_uint64
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGS = {
+    "thx_last_error": [],
+    "thx_device_count": [_p],
+    "thx_ExpectRotran": [_p, _p, _p, _p, _p, _p, _i, _i, _i, _i],
+    "thx_ExpectProject": [_p, _p, _p, _p, _p, _i, _i, _i, _i, _i],
+    "thx_ExpectGlobal3D": [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                           _i, _i, _i, _i, _i, _i],
+    "thx_ctx_create": [_i, _p],
+    "thx_ctx_destroy": [_p],
+    "thx_ctx_set_pixels": [_p, _p, _p, _p, _i, _i, _i],
+    "thx_ctx_set_volume": [_p, _i, _p, _i],
+    "thx_ctx_get_volume": [_p, _i, _p],
+    "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
+    "thx_ctx_get_particles": [_p, _p, _p, _p],
+    "thx_ctx_set_grid": [_p, _p, _i, _p, _i, _p, _p],
+    "thx_ctx_score_global": [_p, _i, _i],
+    "thx_ctx_get_global": [_p, _i, _p, _p, _p, _p],
+    "thx_ctx_get_best": [_p, _i, _p, _p, _p],
+    "thx_ctx_score_one": [_p, _i, _i, _p],
+    "thx_ctx_sync": [_p],
+    "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
+    "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
+    "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
+                                _u64, _p, _p, _p],
+}
+_RESTYPES = {"thx_last_error": C.c_char_p}
+
+
+class THXError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise THXError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, C.c_int)
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = load().thx_last_error().decode(errors="replace")
+        raise THXError(f"{what} failed with status {status}: {msg}")
+
+
+def c32(a, dtype=np.float32):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def device_count():
+    n = C.c_int(0)
+    st = load().thx_device_count(C.byref(n))
+    return n.value if st == 0 else 0
+
+
+# ----------------------------------------------------------------------------
+# Reference-shaped entry points
+# ----------------------------------------------------------------------------
+
+def expect_rotran(trans, rot, iCol, iRow, idim):
+    trans = c32(trans, np.float64).reshape(-1, 2)
+    rot = c32(rot, np.float64).reshape(-1, 4)
+    iCol, iRow = c32(iCol, np.int32), c32(iRow, np.int32)
+    nR, nT, K = rot.shape[0], trans.shape[0], iCol.size
+    traP = np.zeros((nT, K, 2), np.float32)
+    rotMat = np.zeros((nR, 9), np.float64)
+    check(load().thx_ExpectRotran(ptr(traP), ptr(trans), ptr(rot), ptr(rotMat), ptr(iCol),
+                                  ptr(iRow), nR, nT, idim, K), "ExpectRotran")
+    return traP, rotMat
+
+
+def expect_project(volume, rotMat, iCol, iRow, pf, vdim, interp=1):
+    volume = c32(volume)
+    rotMat = c32(rotMat, np.float64).reshape(-1, 9)
+    iCol, iRow = c32(iCol, np.int32), c32(iRow, np.int32)
+    nR, K = rotMat.shape[0], iCol.size
+    rotP = np.zeros((nR, K, 2), np.float32)
+    check(load().thx_ExpectProject(ptr(volume), ptr(rotP), ptr(rotMat), ptr(iCol), ptr(iRow),
+                                   nR, pf, interp, vdim, K), "ExpectProject")
+    return rotP
+
+
+def expect_global3d(rotP, traP, datP, ctfP, sigRcpP, pR, pT, kIdx, nK, wC, wR, wT, baseL):
+    """In-place update of wC/wR/wT/baseL (float32 arrays), as ExpectGlobal3D."""
+    rotP, traP, datP = c32(rotP), c32(traP), c32(datP)
+    ctfP, sigRcpP = c32(ctfP), c32(sigRcpP)
+    pR, pT = c32(pR, np.float64), c32(pT, np.float64)
+    nR, nT = pR.size, pT.size
+    P = ctfP.shape[0]
+    K = ctfP.shape[1]
+    for a in (wC, wR, wT, baseL):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    check(load().thx_ExpectGlobal3D(ptr(rotP), ptr(traP), ptr(datP), ptr(ctfP), ptr(sigRcpP),
+                                    ptr(wC), ptr(wR), ptr(wT), ptr(pR), ptr(pT), ptr(baseL),
+                                    kIdx, nK, nR, nT, K, P), "ExpectGlobal3D")
+
+
+# ----------------------------------------------------------------------------
+# Context API
+# ----------------------------------------------------------------------------
+
+class Context:
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        check(load().thx_ctx_create(device, C.byref(self._h)), "ctx_create")
+        self.K = 0
+        self.P = 0
+        self.R = 0
+        self.T = 0
+
+    def close(self):
+        if self._h:
+            load().thx_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_pixels(self, iCol, iRow, iSig, idim, pf):
+        iCol, iRow = c32(iCol, np.int32), c32(iRow, np.int32)
+        iSig = None if iSig is None else c32(iSig, np.int32)
+        check(load().thx_ctx_set_pixels(self._h, ptr(iCol), ptr(iRow), ptr(iSig), iCol.size,
+                                        idim, pf), "set_pixels")
+        self.K = iCol.size
+        self.idim, self.pf = idim, pf
+
+    def set_volume(self, cls, vol, vdim):
+        vol = c32(vol)
+        check(load().thx_ctx_set_volume(self._h, cls, ptr(vol), vdim), "set_volume")
+
+    def get_volume(self, cls, vdim):
+        out = np.zeros((vdim, vdim, vdim // 2 + 1, 2), np.float32)
+        check(load().thx_ctx_get_volume(self._h, cls, ptr(out)), "get_volume")
+        return out
+
+    def set_particles(self, dat, ctf, sig):
+        dat, ctf, sig = c32(dat), c32(ctf), c32(sig)
+        P = ctf.shape[0]
+        check(load().thx_ctx_set_particles(self._h, ptr(dat), ptr(ctf), ptr(sig), P),
+              "set_particles")
+        self.P = P
+
+    def get_particles(self):
+        dat = np.zeros((self.P, self.K, 2), np.float32)
+        ctf = np.zeros((self.P, self.K), np.float32)
+        sig = np.zeros((self.P, self.K), np.float32)
+        check(load().thx_ctx_get_particles(self._h, ptr(dat), ptr(ctf), ptr(sig)),
+              "get_particles")
+        return dat, ctf, sig
+
+    def set_grid(self, quat, trans, pR=None, pT=None):
+        quat = c32(quat, np.float64).reshape(-1, 4)
+        trans = c32(trans, np.float64).reshape(-1, 2)
+        R, T = quat.shape[0], trans.shape[0]
+        pR = np.full(R, 1.0 / R) if pR is None else c32(pR, np.float64)
+        pT = np.full(T, 1.0 / T) if pT is None else c32(pT, np.float64)
+        check(load().thx_ctx_set_grid(self._h, ptr(quat), R, ptr(trans), T, ptr(pR), ptr(pT)),
+              "set_grid")
+        self.R, self.T = R, T
+        self.pR, self.pT = pR, pT
+
+    def score_global(self, cls=0, precision=FP32):
+        check(load().thx_ctx_score_global(self._h, cls, precision), "score_global")
+
+    def sync(self):
+        check(load().thx_ctx_sync(self._h), "sync")
+
+    def get_global(self, nK=1):
+        P, R, T = self.P, self.R, self.T
+        wC = np.zeros((P, nK))
+        wR = np.zeros((P, nK, R))
+        wT = np.zeros((P, nK, T))
+        base = np.zeros(P)
+        check(load().thx_ctx_get_global(self._h, nK, ptr(wC), ptr(wR), ptr(wT), ptr(base)),
+              "get_global")
+        return wC, wR, wT, base
+
+    def get_best(self, cls=0):
+        bR = np.zeros(self.P, np.int32)
+        bT = np.zeros(self.P, np.int32)
+        sc = np.zeros(self.P)
+        check(load().thx_ctx_get_best(self._h, cls, ptr(bR), ptr(bT), ptr(sc)), "get_best")
+        return bR, bT, sc
+
+    def score_one(self, cls, p):
+        out = np.zeros((self.R, self.T), np.float32)
+        check(load().thx_ctx_score_one(self._h, cls, p, ptr(out)), "score_one")
+        return out
+
+    def last_timing(self):
+        buf = C.create_string_buffer(1024)
+        ms = np.zeros(16)
+        n = C.c_int(0)
+        check(load().thx_ctx_last_timing(self._h, buf, 1024, ptr(ms), 16, C.byref(n)),
+              "last_timing")
+        names = buf.value.decode().split(";") if n.value else []
+        return dict(zip(names, ms[: n.value].tolist()))
+
+    def synth_volume(self, cls, idim, pf, nBlob, sigma, rBall, seed):
+        cen = np.zeros((nBlob, 3))
+        check(load().thx_ctx_synth_volume(self._h, cls, idim, pf, nBlob, sigma, rBall, seed,
+                                          ptr(cen)), "synth_volume")
+        return cen
+
+    def synth_particles(self, cls, P, maxShift, dfMin, dfMax, snr, seed, voltage=300e3,
+                        Cs=2.7e7, ampContrast=0.07, pixelSize=1.5):
+        q = np.zeros((P, 4))
+        t = np.zeros((P, 2))
+        cp = np.zeros((P, 3))
+        check(load().thx_ctx_synth_particles(self._h, cls, P, maxShift, dfMin, dfMax, voltage,
+                                             Cs, ampContrast, pixelSize, snr, seed, ptr(q),
+                                             ptr(t), ptr(cp)), "synth_particles")
+        self.P = P
+        return q, t, cp

@@ -1,0 +1,130 @@
+"""Synthetic workloads of the expectation step and their pose grids.
+
+Each workload is one global E-step over synthetic particles.  Volumes and
+particles come from the library's seeded generator (thx_ctx_synth_*); the
+pose grids below are deterministic so every run scores the same poses.
+Parameters a workload description leaves open are fixed here and stated:
+blob radius and width scale with box / 64 from the box-64 workload
+(radius 24, width 2 voxels) unless given.
+"""
+import math
+
+import numpy as np
+
+# Super-Fibonacci constants (Alexa, "Super-Fibonacci Spirals: Fast,
+# Low-Discrepancy Sampling of SO(3)", CVPR 2022): phi = sqrt(2) and psi the
+# real root > 1 of psi^4 = psi + 4.
+_PHI = math.sqrt(2.0)
+_PSI = 1.533751168755204288118041
+
+
+def super_fibonacci(n):
+    """n unit quaternions (w, x, y, z) spread near-uniformly over S^3."""
+    s = np.arange(n, dtype=np.float64) + 0.5
+    t = s / n
+    r = np.sqrt(t)
+    R = np.sqrt(1.0 - t)
+    alpha = 2.0 * np.pi * s / _PHI
+    beta = 2.0 * np.pi * s / _PSI
+    q = np.stack([r * np.sin(alpha), r * np.cos(alpha), R * np.sin(beta), R * np.cos(beta)], 1)
+    return q / np.linalg.norm(q, axis=1, keepdims=True)
+
+
+def shift_grid(n_side, step):
+    """Regular n_side x n_side translation grid centred on 0, in pixels."""
+    h = (n_side - 1) / 2.0
+    a = (np.arange(n_side) - h) * step
+    tx, ty = np.meshgrid(a, a, indexing="xy")
+    return np.stack([tx.ravel(), ty.ravel()], 1)
+
+
+def geodesic_angle(q1, q2):
+    """Rotation angle (rad) between unit quaternions, q and -q identified."""
+    d = np.abs(np.sum(q1 * q2, axis=-1)).clip(0, 1)
+    return 2.0 * np.arccos(d)
+
+
+def sampling_step(quat, sample=2000, seed=0):
+    """Mean nearest-neighbour geodesic angle of a rotation set (rad)."""
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(len(quat), size=min(sample, len(quat)), replace=False)
+    best = []
+    for i in idx:
+        d = np.abs(quat @ quat[i]).clip(0, 1)
+        d[i] = 0
+        best.append(2 * np.arccos(d.max()))
+    return float(np.mean(best))
+
+
+WORKLOADS = {
+    # Plain 3D expectation, CPU-runnable: box 64.
+    "box64": dict(box=64, pf=2, n_blob=50, blob_sigma=2.0, blob_radius=24.0,
+                  n_particles=1000, n_rot=1000, shift_side=7, shift_step=1.0,
+                  max_shift=3.0, df_min=10000.0, df_max=30000.0, snr=0.05, cutoff=24,
+                  n_class=1),
+    # Plain FP32 vs mixed precision, same pose grid: box 128.
+    "box128": dict(box=128, pf=2, n_blob=200, blob_sigma=3.0, blob_radius=48.0,
+                   n_particles=10000, n_rot=5000, shift_side=11, shift_step=1.0,
+                   max_shift=5.0, df_min=8000.0, df_max=35000.0, snr=0.03, cutoff=50,
+                   n_class=1),
+    # FP8 scoring at large box: box 256.
+    "box256": dict(box=256, pf=2, n_blob=500, blob_sigma=6.0, blob_radius=96.0,
+                   n_particles=20000, n_rot=10000, shift_side=9, shift_step=2.0,
+                   max_shift=8.0, df_min=5000.0, df_max=40000.0, snr=0.02, cutoff=100,
+                   n_class=1),
+}
+
+CTF_DEFAULTS = dict(voltage=300e3, Cs=2.7e7, ampContrast=0.07, pixelSize=1.5)
+
+
+def seeds(name):
+    base = 0x7468756E + sum(ord(ch) for ch in name)
+    return dict(volume=base, particles=base + 1)
+
+
+def pixel_set(N, rU, rL=0.0):
+    """Pixel set of Optimiser::allocPreCalIdx (src/Optimiser.cpp:7942-7992):
+    loop j in [-(rU+1), rU+1), i in [0, rU+1]; skip (0, j<0); keep
+    rL^2 <= i^2 + j^2 < rU^2 and rL <= rint(|(i,j)|) < rU.  Same order."""
+    r = float(np.float32(rU) + 1)
+    iCol, iRow, iSig = [], [], []
+    rU2 = np.float32(rU) ** 2
+    rL2 = np.float32(rL) ** 2
+    for j in range(int(-r), int(math.ceil(r))):
+        if not j < r:
+            continue
+        for i in range(0, int(math.floor(r)) + 1):
+            if i == 0 and j < 0:
+                continue
+            u = np.float32(i * i + j * j)
+            if rL2 <= u < rU2:
+                v = int(np.rint(math.hypot(i, j)))
+                if rL <= v < rU:
+                    iCol.append(i)
+                    iRow.append(j)
+                    iSig.append(v)
+    return (np.array(iCol, np.int32), np.array(iRow, np.int32), np.array(iSig, np.int32))
+
+
+def grid(name, n_rot=None):
+    w = WORKLOADS[name]
+    quat = super_fibonacci(n_rot or w["n_rot"])
+    trans = shift_grid(w["shift_side"], w["shift_step"])
+    return quat, trans
+
+
+def setup_context(ctx, name, n_particles=None, n_rot=None, cls=0):
+    """Pixel set, volume, particles and grid of a workload on a thx.Context."""
+    w = WORKLOADS[name]
+    sd = seeds(name)
+    iCol, iRow, iSig = pixel_set(w["box"], w["cutoff"])
+    ctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    centres = ctx.synth_volume(cls, w["box"], w["pf"], w["n_blob"], w["blob_sigma"],
+                               w["blob_radius"], sd["volume"] + cls)
+    P = n_particles or w["n_particles"]
+    q, t, cp = ctx.synth_particles(cls, P, w["max_shift"], w["df_min"], w["df_max"], w["snr"],
+                                   sd["particles"], **CTF_DEFAULTS)
+    quat, trans = grid(name, n_rot)
+    ctx.set_grid(quat, trans)
+    return dict(iCol=iCol, iRow=iRow, iSig=iSig, centres=centres, true_quat=q, true_shift=t,
+                ctf_params=cp, quat=quat, trans=trans)
