@@ -1,0 +1,391 @@
+"""Pinning of the CPU oracle (no GPU).
+
+The reference ships no golden vectors, tests or fixtures for the expectation
+path, so each oracle function is pinned by checks that do not restate its own
+formula: closed forms, textbook constants, independent algorithms (numpy FFTs,
+a Hermitian-extended full-volume trilinear interpolator), invariants and
+hand-worked cases.  Each test names the plausible mistake it would catch.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import workloads as wl
+
+
+# ---------------------------------------------------------------------------
+# a1 pixel set (Optimiser::allocPreCalIdx)
+# ---------------------------------------------------------------------------
+
+def _brute_pixels(N, rU, rL=0.0):
+    i, j = np.meshgrid(np.arange(0, N // 2 + 1), np.arange(-N // 2, N // 2), indexing="xy")
+    i, j = i.ravel(), j.ravel()
+    r2 = i * i + j * j
+    v = np.rint(np.hypot(i, j))
+    keep = ~((i == 0) & (j < 0)) & (r2 < rU * rU) & (r2 >= rL * rL) & (v < rU) & (v >= rL)
+    return set(zip(i[keep].tolist(), j[keep].tolist()))
+
+
+@pytest.mark.parametrize("N,rU,K", [(64, 24, 871), (128, 50, 3853), (160, 64, 6323),
+                                    (256, 100, 15559)])
+def test_pixel_set_brute_force(N, rU, K):
+    iCol, iRow, iSig, iPxl = oracle.pixel_set(N, rU)
+    assert len(iCol) == K
+    assert set(zip(iCol.tolist(), iRow.tolist())) == _brute_pixels(N, rU)
+    # shells are the rounded radius; iPxl is the half-complex row-major index
+    np.testing.assert_array_equal(iSig, np.rint(np.hypot(iCol, iRow)).astype(int))
+    np.testing.assert_array_equal(iPxl, np.where(iRow >= 0, iRow, iRow + N) * (N // 2 + 1) + iCol)
+    # loop order: j outer ascending, i inner ascending
+    key = iRow * 10000 + iCol
+    assert np.all(np.diff(key) > 0)
+
+
+def test_pixel_set_lower_cutoff():
+    iCol, iRow, iSig, _ = oracle.pixel_set(64, 20, 5)
+    assert set(zip(iCol.tolist(), iRow.tolist())) == _brute_pixels(64, 20, 5)
+    assert iSig.min() >= 5 and iSig.max() < 20
+
+
+def test_workload_pixel_set_equals_oracle():
+    for N, rU in [(64, 24), (128, 50)]:
+        a = wl.pixel_set(N, rU)
+        b = oracle.pixel_set(N, rU)
+        for x, y in zip(a, b[:3]):
+            np.testing.assert_array_equal(x, y)
+
+
+# ---------------------------------------------------------------------------
+# a2 CTF
+# ---------------------------------------------------------------------------
+
+def _lambda_codata(volt):
+    h, m0, e, c = 6.62607015e-34, 9.1093837015e-31, 1.602176634e-19, 299792458.0
+    lam = h / math.sqrt(2 * m0 * e * volt * (1 + e * volt / (2 * m0 * c * c)))
+    return lam * 1e10    # Angstrom
+
+
+def test_ctf_at_origin_is_amplitude_contrast():
+    # u = 0: chi = -phaseShift -> ctf = -sqrt(1-A^2) sin(-ps) + A cos(ps)
+    for A, ps in [(0.07, 0.0), (0.1, 0.3)]:
+        v = oracle.ctf([0], [0], 128, 1.5, 300e3, 20000, 15000, 0.4, 2.7e7, A, ps)
+        assert v[0] == pytest.approx(-math.sqrt(1 - A * A) * math.sin(-ps) + A * math.cos(ps),
+                                     abs=1e-6)
+
+
+def test_ctf_wavelength_matches_codata():
+    # Cs = 0, A = 0, no astigmatism: ctf = sin(pi lambda dU u^2); first zero
+    # at u^2 = 1 / (lambda dU).  Evaluate on a fine pixel line and locate it.
+    N, px, dU = 4096, 0.25, 20000.0
+    lam = _lambda_codata(300e3)
+    ic = np.arange(1, 400)
+    v = oracle.ctf(ic, np.zeros_like(ic), N, px, 300e3, dU, dU, 0.0, 0.0, 0.0)
+    u = ic / (N * px)
+    chi = np.pi * lam * dU * u * u
+    # the reference constant 12.2643247 differs from CODATA by ~6e-6 relative
+    np.testing.assert_allclose(v, np.sin(chi), atol=2e-4 + 1e-5 * chi.max())
+    assert abs(lam - 0.019687) < 2e-6     # textbook 300 kV wavelength
+
+
+def test_ctf_astigmatism_symmetry():
+    iCol, iRow, _, _ = oracle.pixel_set(128, 50)
+    a = oracle.ctf(iCol, iRow, 128, 1.5, 300e3, 25000, 18000, 0.3, 2.7e7, 0.07)
+    b = oracle.ctf(iCol, iRow, 128, 1.5, 300e3, 25000, 18000, 0.3 + math.pi, 2.7e7, 0.07)
+    np.testing.assert_allclose(a, b, atol=2e-5)
+    # swapping dU and dV equals rotating the astigmatism axis by 90 degrees
+    c = oracle.ctf(iCol, iRow, 128, 1.5, 300e3, 18000, 25000, 0.3 + math.pi / 2, 2.7e7, 0.07)
+    np.testing.assert_allclose(a, c, atol=2e-5)
+    # no astigmatism -> depends on |(i, j)| only
+    d = oracle.ctf(iCol, iRow, 128, 1.5, 300e3, 20000, 20000, 0.0, 2.7e7, 0.07)
+    e = oracle.ctf(iRow * 0 + np.hypot(iCol, iRow).round().astype(int) * 0 + iRow, iCol, 128,
+                   1.5, 300e3, 20000, 20000, 0.0, 2.7e7, 0.07)
+    sw = np.abs(iCol) == np.abs(iRow)
+    np.testing.assert_allclose(d[sw], e[sw], atol=2e-5)
+
+
+def test_ctf_spherical_aberration_sign():
+    # with zero defocus chi = K2 u^4 > 0: ctf = -w1 sin(chi) + A cos(chi)
+    lam = _lambda_codata(300e3)
+    u = 0.3
+    chi = math.pi / 2 * 2.7e7 * lam ** 3 * u ** 4
+    N, px = 200, 1.0
+    v = oracle.ctf([int(u * N * px)], [0], N, px, 300e3, 0.0, 0.0, 0.0, 2.7e7, 0.07)
+    assert v[0] == pytest.approx(-math.sqrt(1 - 0.07 ** 2) * math.sin(chi) + 0.07 * math.cos(chi),
+                                 abs=2e-4)
+
+
+# ---------------------------------------------------------------------------
+# a5 translation phases
+# ---------------------------------------------------------------------------
+
+def test_translate_properties():
+    N = 128
+    iCol, iRow, _, _ = oracle.pixel_set(N, 50)
+    s1 = oracle.translate(1.5, -2.0, N, iCol, iRow)
+    s2 = oracle.translate(-0.5, 3.0, N, iCol, iRow)
+    s12 = oracle.translate(1.0, 1.0, N, iCol, iRow)
+    z1, z2, z12 = (s[:, 0] + 1j * s[:, 1] for s in (s1, s2, s12))
+    np.testing.assert_allclose(np.abs(z1), 1, atol=1e-6)
+    np.testing.assert_allclose(z1 * z2, z12, atol=2e-5)
+    # shift theorem against numpy's FFT: a delta at (tx, ty) has FT exp(-2 pi i k.t / N)
+    img = np.zeros((N, N))
+    img[3, 5] = 1.0              # row y = 3, column x = 5
+    F = np.fft.rfft2(img)        # F[j, i]
+    s = oracle.translate(5.0, 3.0, N, iCol, iRow)
+    np.testing.assert_allclose(s[:, 0] + 1j * s[:, 1], F[iRow % N, iCol], atol=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# a4 rotation matrix
+# ---------------------------------------------------------------------------
+
+def test_rotmat_axis_angle():
+    th = 0.7
+    q = [math.cos(th / 2), 0, 0, math.sin(th / 2)]
+    M = oracle.rotmat(q)
+    c, s = math.cos(th), math.sin(th)
+    np.testing.assert_allclose(M, [[c, -s, 0], [s, c, 0], [0, 0, 1]], atol=1e-15)
+    q = [math.cos(th / 2), math.sin(th / 2), 0, 0]
+    np.testing.assert_allclose(oracle.rotmat(q), [[1, 0, 0], [0, c, -s], [0, s, c]], atol=1e-15)
+
+
+def test_rotmat_orthonormal_and_composition():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        a, b = rng.normal(size=4), rng.normal(size=4)
+        a /= np.linalg.norm(a)
+        b /= np.linalg.norm(b)
+        A, B = oracle.rotmat(a), oracle.rotmat(b)
+        np.testing.assert_allclose(A @ A.T, np.eye(3), atol=1e-14)
+        assert np.linalg.det(A) == pytest.approx(1.0, abs=1e-14)
+        # Hamilton product (quaternion_mul, src/Geometry/Euler.cpp:13-26)
+        w = a[0] * b[0] - a[1:] @ b[1:]
+        v = a[0] * b[1:] + b[0] * a[1:] + np.cross(a[1:], b[1:])
+        np.testing.assert_allclose(oracle.rotmat(np.r_[w, v]), A @ B, atol=1e-14)
+        np.testing.assert_allclose(oracle.rotmat(-a), A, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# a6 projection
+# ---------------------------------------------------------------------------
+
+def _real_volume(V, seed=0):
+    rng = np.random.default_rng(seed)
+    vol = rng.normal(size=(V, V, V))
+    half = np.fft.rfftn(vol)                    # [z][y][x], x = V/2 + 1
+    full = np.fft.fftn(vol)
+    hv = np.stack([half.real, half.imag], -1).astype(np.float32)
+    return hv, full
+
+
+def _trilinear_full(full, p):
+    """Trilinear interpolation of the full complex spectrum with periodic wrap."""
+    V = full.shape[0]
+    x0 = np.floor(p).astype(int)
+    d = p - x0
+    acc = 0j
+    for k in range(2):
+        for j in range(2):
+            for i in range(2):
+                w = (d[0] if i else 1 - d[0]) * (d[1] if j else 1 - d[1]) * (d[2] if k else 1 - d[2])
+                acc += w * full[(x0[2] + k) % V, (x0[1] + j) % V, (x0[0] + i) % V]
+    return acc
+
+
+def test_project_identity_is_exact_voxels():
+    V, N, pf = 32, 16, 2
+    hv, full = _real_volume(V)
+    iCol, iRow, _, _ = oracle.pixel_set(N, 6)
+    out = oracle.project(hv, V, np.eye(3), iCol, iRow, pf)
+    ref = hv[0, (iRow * pf) % V, iCol * pf]
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_project_matches_full_spectrum_interpolation():
+    V, N, pf = 32, 16, 2
+    hv, full = _real_volume(V, 1)
+    iCol, iRow, _, _ = oracle.pixel_set(N, 6)
+    rng = np.random.default_rng(2)
+    for _ in range(6):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        M = oracle.rotmat(q)
+        got = oracle.project(hv, V, M, iCol, iRow, pf, f64=True)
+        got32 = oracle.project(hv, V, M, iCol, iRow, pf)
+        for n in range(len(iCol)):
+            p = M @ np.array([iCol[n] * pf, iRow[n] * pf, 0.0])
+            ref = _trilinear_full(full.astype(np.complex64).astype(np.complex128), p)
+            assert abs(got[n, 0] + 1j * got[n, 1] - ref) <= 1e-9 * (1 + abs(ref))
+            assert abs(got32[n, 0] + 1j * got32[n, 1] - ref) <= 1e-4 * (1 + abs(ref))
+
+
+def test_project_half_turn_conjugates():
+    # 180 degrees about y maps (x, y, 0) to (-x, y, 0): the mirrored half-space
+    # sample is conj(F(x, -y, 0)).
+    V, N, pf = 32, 16, 2
+    hv, full = _real_volume(V, 3)
+    iCol, iRow, _, _ = oracle.pixel_set(N, 6)
+    q = [0.0, 0.0, 1.0, 0.0]
+    out = oracle.project(hv, V, oracle.rotmat(q), iCol, iRow, pf)
+    ref = hv[0, (-iRow * pf) % V, iCol * pf] * np.array([1, -1], np.float32)
+    sel = iCol > 0
+    np.testing.assert_allclose(out[sel], ref[sel], atol=1e-5)
+
+
+# ---------------------------------------------------------------------------
+# a7 likelihood and a8 marginals
+# ---------------------------------------------------------------------------
+
+def test_logdatavs_zero_at_truth_and_expansion():
+    rng = np.random.default_rng(4)
+    K, n = 200, 7
+    pri = rng.normal(size=(K, 2)).astype(np.float32)
+    ctf = rng.uniform(-1, 1, size=(K, n)).astype(np.float32)
+    sig = -rng.uniform(0.5, 2, size=(K, n)).astype(np.float32)
+    dat = (ctf[..., None] * pri[:, None, :]).astype(np.float32)
+    res = oracle.logdatavs_pm(dat, pri, ctf, sig)
+    np.testing.assert_allclose(res, 0, atol=1e-6)
+    noise = rng.normal(size=(K, n, 2)).astype(np.float32)
+    dat2 = (dat + noise).astype(np.float32)
+    res2 = oracle.logdatavs_pm(dat2, pri, ctf, sig)
+    # expansion: sig |X|^2 + sig c^2 |P|^2 - 2 Re(sig c conj(X) P)
+    X = dat2[..., 0] + 1j * dat2[..., 1]
+    Pc = pri[:, 0] + 1j * pri[:, 1]
+    A = np.sum(sig * np.abs(X) ** 2, 0)
+    C = np.sum(sig * ctf ** 2 * np.abs(Pc[:, None]) ** 2, 0)
+    B = np.sum(np.real(sig * ctf * np.conj(X) * Pc[:, None]), 0)
+    np.testing.assert_allclose(res2, A + C - 2 * B, rtol=1e-5)
+    # image-major FP64 form agrees
+    for j in range(n):
+        v = oracle.logdatavs_f64(dat2[:, j], pri, ctf[:, j], sig[:, j])
+        assert v == pytest.approx(res2[j], rel=1e-5)
+
+
+@pytest.fixture(scope="module")
+def scan_case():
+    V, N, pf = 32, 16, 2
+    hv, _ = _real_volume(V, 5)
+    hv /= np.abs(hv).max()
+    iCol, iRow, _, _ = oracle.pixel_set(N, 6)
+    K = len(iCol)
+    rng = np.random.default_rng(6)
+    quat = wl.super_fibonacci(24)
+    trans = wl.shift_grid(3, 1.0)
+    P = 5
+    dat = rng.normal(size=(P, K, 2)).astype(np.float32) * 0.01
+    ctf = rng.uniform(-1, 1, size=(P, K)).astype(np.float32)
+    sig = np.full((P, K), -2.0, np.float32)
+    # plant the true pose (r=7, t=4) into particle 2
+    pr = oracle.project(hv, V, oracle.rotmat(quat[7]), iCol, iRow, pf)
+    s = oracle.translate(trans[4, 0], trans[4, 1], N, iCol, iRow)
+    q = (pr[:, 0] + 1j * pr[:, 1]) * (s[:, 0] + 1j * s[:, 1])
+    dat[2, :, 0] += (ctf[2] * q.real).astype(np.float32)
+    dat[2, :, 1] += (ctf[2] * q.imag).astype(np.float32)
+    pR = rng.uniform(0.5, 1.5, 24)
+    pR /= pR.sum()
+    pT = rng.uniform(0.5, 1.5, 9)
+    pT /= pT.sum()
+    return dict(V=V, N=N, pf=pf, hv=hv, iCol=iCol, iRow=iRow, quat=quat, trans=trans, dat=dat,
+                ctf=ctf, sig=sig, pR=pR, pT=pT)
+
+
+def test_global_scan_f64_against_brute_force(scan_case):
+    c = scan_case
+    wC, wR, wT, base, dvp = oracle.global_scan(c["hv"], c["V"], c["pf"], c["iCol"], c["iRow"],
+                                               c["N"], c["quat"], c["trans"], c["dat"],
+                                               c["ctf"], c["sig"], c["pR"], c["pT"],
+                                               want_dvp=True)
+    # brute-force dvp from the oracle's pieces with an independent reduction
+    for l in range(len(base)):
+        X = c["dat"][l, :, 0] + 1j * c["dat"][l, :, 1]
+        for r in (0, 7, 23):
+            pr = oracle.project(c["hv"], c["V"], oracle.rotmat(c["quat"][r]), c["iCol"],
+                                c["iRow"], c["pf"], f64=True)
+            for t in (0, 4, 8):
+                ph = np.exp(-2j * np.pi * (c["iCol"] * c["trans"][t, 0] +
+                                          c["iRow"] * c["trans"][t, 1]) / c["N"])
+                model = c["ctf"][l] * (pr[:, 0] + 1j * pr[:, 1]) * ph
+                ref = np.sum(c["sig"][l] * np.abs(X - model) ** 2)
+                assert dvp[l, r, t] == pytest.approx(ref, rel=1e-9)
+    assert np.unravel_index(dvp[2].argmax(), dvp[2].shape) == (7, 4)
+    # marginal identities: sum_r pR wR = wC = sum_t pT wT; max term is 1
+    np.testing.assert_allclose(wR @ c["pR"], wC, rtol=1e-12)
+    np.testing.assert_allclose(wT @ c["pT"], wC, rtol=1e-12)
+    np.testing.assert_allclose(base, dvp.reshape(len(base), -1).max(1))
+    w = np.exp(dvp - base[:, None, None])
+    np.testing.assert_allclose(wC, np.einsum("lrt,r,t->l", w, c["pR"], c["pT"]), rtol=1e-12)
+
+
+def test_global_scan_f32_streaming_matches_f64(scan_case):
+    c = scan_case
+    args = (c["hv"], c["V"], c["pf"], c["iCol"], c["iRow"], c["N"], c["quat"], c["trans"],
+            c["dat"], c["ctf"], c["sig"], c["pR"], c["pT"])
+    w64 = oracle.global_scan(*args, f64=True)
+    for threads in (1, 4):
+        w32 = oracle.global_scan(*args, f64=False, threads=threads)
+        np.testing.assert_allclose(w32[3], w64[3], rtol=1e-5)
+        np.testing.assert_allclose(w32[0], w64[0], rtol=1e-4)
+        np.testing.assert_allclose(w32[1], w64[1], rtol=1e-4, atol=1e-6 * w64[1].max())
+        np.testing.assert_allclose(w32[2], w64[2], rtol=1e-4, atol=1e-6 * w64[2].max())
+
+
+# ---------------------------------------------------------------------------
+# a10 systematic resampling
+# ---------------------------------------------------------------------------
+
+def test_resample_hand_worked():
+    w = np.array([0.1, 0.2, 0.3, 0.4])
+    u = np.ones(4)
+    idx, wn = oracle.resample(w, u, 4, 0.05)
+    np.testing.assert_array_equal(idx, [0, 1, 2, 3])
+    idx, wn = oracle.resample(w, u, 4, 0.2)
+    np.testing.assert_array_equal(idx, [1, 2, 3, 3])
+    np.testing.assert_allclose(wn, 0.25)
+    # prior weights multiply the likelihood; new weight is 1/u (normalised)
+    idx, wn = oracle.resample(np.ones(4), np.array([1.0, 1.0, 6.0, 2.0]), 5, 0.1)
+    np.testing.assert_array_equal(idx, [0, 2, 2, 2, 3])
+    np.testing.assert_allclose(wn, np.array([1, 1 / 6, 1 / 6, 1 / 6, 1 / 2]) /
+                               (1 + 3 / 6 + 1 / 2))
+
+
+def test_resample_counts_kitagawa():
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        n_in, n_out = rng.integers(2, 40), rng.integers(1, 60)
+        w = rng.uniform(0, 1, n_in)
+        u = rng.uniform(0.1, 1, n_in)
+        u0 = rng.uniform(0, 1.0 / n_out)
+        idx, _ = oracle.resample(w, u, n_out, u0)
+        assert np.all(np.diff(idx) >= 0)
+        W = w * u / np.sum(w * u)
+        cnt = np.bincount(idx, minlength=n_in)
+        assert np.all(cnt >= np.floor(n_out * W) - 1) and np.all(cnt <= np.ceil(n_out * W) + 1)
+        assert cnt.sum() == n_out
+
+
+# ---------------------------------------------------------------------------
+# synthetic volume closed form
+# ---------------------------------------------------------------------------
+
+def test_blob_ft_matches_numpy_fft():
+    N, pf = 32, 2
+    V = N * pf
+    sigma = 2.0
+    centres = np.array([[3.0, -2.0, 5.0], [-4.0, 1.0, 0.0]])
+    amps = np.array([1.0, 0.5])
+    # real-space blobs on the padded grid with the box centre at index 0
+    g = np.fft.fftfreq(V, 1.0 / V)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    vol = np.zeros((V, V, V))
+    for c, a in zip(centres, amps):
+        vol += a * np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / (2 * sigma ** 2))
+    F = np.fft.rfftn(vol)
+    rng = np.random.default_rng(3)
+    ix = rng.integers(0, V // 2 + 1, 200)
+    iy = rng.integers(-V // 2, V // 2, 200)
+    iz = rng.integers(-V // 2, V // 2, 200)
+    got = oracle.blob_ft(V, N, centres, amps, sigma, ix, iy, iz)
+    ref = F[iz % V, iy % V, ix]
+    scale = np.abs(F).max()
+    np.testing.assert_allclose(got[:, 0] + 1j * got[:, 1], ref, atol=1e-5 * scale)
