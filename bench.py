@@ -50,7 +50,7 @@ def run_mode(ctx, prec, steps, warmup, P, R, T, K):
                 score_flops=4.0 * K * evals)
 
 
-def cpu_baseline(ctx, info, name, P_s=48, R_s=192):
+def cpu_baseline(ctx, info, name, P_s=384, R_s=1000):
     """Reference CPU loop (oracle restatement, OpenMP) on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -69,6 +69,20 @@ def cpu_baseline(ctx, info, name, P_s=48, R_s=192):
     return dict(value=P_s * R_s * T / dt, unit="pose-evals/s", cores=cores, kind="port",
                 sample=f"{P_s} particles x {R_s} rotations x {T} translations of {name}, "
                        f"FP32 reference loop order, {dt:.1f} s")
+
+
+def traffic_bytes(name, prec):
+    """HBM bytes per k_score launch from the committed rocprofv3 FETCH_SIZE +
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    tag = {"fp32": 0, "bf16": 1, "fp16": 2}[prec]
+    for k, v in d["kernels"].items():
+        if k.startswith(f"thx::k_score<{tag},"):
+            return (v["read_GB"] + v["write_GB"]) * 1e9
+    return None
 
 
 def main():
@@ -121,8 +135,11 @@ def main():
                                f"{main_m} cross term, no learned proposal",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[main_m] / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[main_m], "traffic": None,
-                     "kernel": "k_score", "kernel_ms": r["kernels_ms"]["score"]},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[main_m],
+                     "traffic": traffic_bytes(name, main_m),
+                     "kernel": "k_score", "kernel_ms": r["kernels_ms"]["score"],
+                     "flops_per_launch": r["score_flops"],
+                     "flops_definition": "4*K per scored (particle, rotation, translation)"},
         "kernels_ms": r["kernels_ms"],
     }
     if "fp32" in res:
