@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -48,16 +49,42 @@ void DevBuf::release() {
     bytes = 0;
 }
 
-// Translation chunking: rows per particle TC in {32, 64, 128} minimising the
-// padded count ceil(T / TC) * TC, ties to the larger TC.
-static void choose_tc(int T, int& TC, int& nTc) {
-    int best = 128, bestPad = ((T + 127) / 128) * 128;
-    for (int tc : {64, 32}) {
+// Translation chunking of a scan: rows per particle TC in {32, 64, 128}.
+// The plain kernel minimises the padded count ceil(T / TC) * TC (ties to the
+// larger TC); the 16-bit kernel prefers TC = 32 at equal padding, which lets
+// four particles share each phase-table row the block loads.  THX_TC16 in
+// the environment overrides the 16-bit choice (32, 64 or 128).
+static void choose_tc(int T, bool preferSmall, int& TC, int& nTc) {
+    const int order[3] = {128, 64, 32};
+    int best = -1, bestPad = 0;
+    for (int i = 0; i < 3; i++) {
+        const int tc = preferSmall ? order[2 - i] : order[i];
         const int pad = ((T + tc - 1) / tc) * tc;
-        if (pad < bestPad) { best = tc; bestPad = pad; }
+        if (best < 0 || pad < bestPad) { best = tc; bestPad = pad; }
     }
     TC = best;
     nTc = (T + best - 1) / best;
+}
+
+static int choose_scan_shape(thx_ctx* c, int precision) {
+    const bool lo = precision != THX_FP32;
+    choose_tc(c->T, lo, c->TC, c->nTc);
+    if (lo) {
+        if (const char* e = std::getenv("THX_TC16")) {
+            const int tc = std::atoi(e);
+            if (tc == 32 || tc == 64 || tc == 128) {
+                c->TC = tc;
+                c->nTc = (c->T + tc - 1) / tc;
+            }
+        }
+    }
+    c->Tpad = c->TC * c->nTc;
+    c->NB = score_tile_cols(precision);
+    c->nRb = c->Rpad / c->NB;
+    THX_CHECK(c->Tpad <= c->TpadS, THX_ERR_STATE, "scan shape: translation padding too large");
+    THX_CHECK((long long)c->nTc * c->nRb <= kMaxTiles, THX_ERR_RANGE,
+              "scan shape: too many rotation/translation tiles per particle");
+    return THX_OK;
 }
 
 static int set_device(thx_ctx* c) {
@@ -68,7 +95,7 @@ static int set_device(thx_ctx* c) {
 static int ensure_grid_buffers(thx_ctx* c) {
     THX_TRY(c->rotP.ensure(sizeof(float2) * (size_t)c->Rpad * c->Kp));
     THX_TRY(c->normP.ensure(sizeof(float) * (size_t)c->Rpad * c->Kp));
-    THX_TRY(c->S.ensure(sizeof(float2) * (size_t)c->Tpad * c->Kp));
+    THX_TRY(c->S.ensure(sizeof(float2) * (size_t)c->TpadS * c->Kp));
     return THX_OK;
 }
 
@@ -104,6 +131,7 @@ static int record(thx_ctx* c, const char* name) {
 // Scoring of the particles against the current rotP / normP / S; the
 // projection (or the caller's upload) has already filled rotP and normP.
 static int run_scan(thx_ctx* c, int cls, int precision) {
+    THX_TRY(choose_scan_shape(c, precision));
     THX_TRY(ensure_scan_buffers(c, cls));
     if (precision == THX_FP16 || precision == THX_FP8) {
         THX_TRY(c->shellScale.ensure(sizeof(float) * (c->maxShell + 1)));
@@ -311,23 +339,19 @@ int thx_ctx_get_particles(thx_ctx* c, float* datP, float* ctfP, float* sigRcpP) 
 static int set_grid_shape(thx_ctx* c, int R, int T) {
     c->R = R;
     c->T = T;
-    c->Rpad = round_up(R, 128);
-    c->nRb = c->Rpad / 128;
-    choose_tc(T, c->TC, c->nTc);
-    c->Tpad = c->TC * c->nTc;
-    THX_CHECK((long long)c->nTc * c->nRb <= kMaxTiles, THX_ERR_RANGE,
-              "set_grid: too many rotation/translation tiles per particle");
+    c->Rpad = round_up(R, 256);
+    c->TpadS = round_up(T, 128);
     return ensure_grid_buffers(c);
 }
 
 static int upload_priors(thx_ctx* c, const double* pR, const double* pT) {
-    std::vector<float> fr(c->Rpad, 0.f), ft(c->Tpad, 0.f);
+    std::vector<float> fr(c->Rpad, 0.f), ft(c->TpadS, 0.f);
     for (int r = 0; r < c->R; r++) fr[r] = (float)pR[r];
     for (int t = 0; t < c->T; t++) ft[t] = (float)pT[t];
     THX_TRY(c->pRf.ensure(sizeof(float) * c->Rpad));
-    THX_TRY(c->pTf.ensure(sizeof(float) * c->Tpad));
+    THX_TRY(c->pTf.ensure(sizeof(float) * c->TpadS));
     THX_HIP(hipMemcpy(c->pRf.p, fr.data(), sizeof(float) * c->Rpad, hipMemcpyHostToDevice));
-    THX_HIP(hipMemcpy(c->pTf.p, ft.data(), sizeof(float) * c->Tpad, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pTf.p, ft.data(), sizeof(float) * c->TpadS, hipMemcpyHostToDevice));
     return THX_OK;
 }
 
@@ -767,7 +791,7 @@ int thx_ExpectGlobal3D(const float* rotP, const float* traP, const float* datP, 
     THX_TRY(upload_priors(c, pR, pT));
     // projections and phases into the padded layouts (zero padding)
     THX_HIP(hipMemset(c->rotP.p, 0, sizeof(float2) * (size_t)c->Rpad * c->Kp));
-    THX_HIP(hipMemset(c->S.p, 0, sizeof(float2) * (size_t)c->Tpad * c->Kp));
+    THX_HIP(hipMemset(c->S.p, 0, sizeof(float2) * (size_t)c->TpadS * c->Kp));
     THX_HIP(hipMemcpy2D(c->rotP.p, sizeof(float2) * c->Kp, rotP, sizeof(float2) * npxl,
                         sizeof(float2) * npxl, nR, hipMemcpyHostToDevice));
     THX_HIP(hipMemcpy2D(c->S.p, sizeof(float2) * c->Kp, traP, sizeof(float2) * npxl,

@@ -87,7 +87,9 @@ struct thx_ctx {
     thx::DevBuf Wt;         // float  [Ppad][Kp]  sig ctf^2
     int prepPrecision = -1;
     // pose grid
-    int R = 0, T = 0, Rpad = 0, TC = 128, nTc = 1, Tpad = 0, nRb = 0;
+    int R = 0, T = 0, Rpad = 0, TpadS = 0;  // Rpad: multiple of 256; TpadS: rows of S, pTf
+    // shape of the current scan (chosen per precision by choose_scan_shape)
+    int TC = 128, nTc = 1, Tpad = 0, NB = 128, nRb = 0;
     thx::DevBuf quat, trans, pR, pT, pRf, pTf;
     thx::DevBuf S;          // float2 [Tpad][Kp]
     // scan workspaces
@@ -118,6 +120,7 @@ int launch_shell_scale(thx_ctx* c, int precision);
 int launch_convert_lo(thx_ctx* c, int precision);
 int launch_norm_gemm(thx_ctx* c);
 int launch_score(thx_ctx* c, int precision);
+int score_tile_cols(int precision);
 int launch_merge(thx_ctx* c, int cls);
 int launch_synth_volume(thx_ctx* c, int cls, int idim, int pf,
                         const double* dCentres, int nBlob, double sigma);

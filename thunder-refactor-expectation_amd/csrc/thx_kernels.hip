@@ -31,6 +31,7 @@ namespace thx {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // Geometry helpers (device)
@@ -170,9 +171,9 @@ int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, c
 }
 
 int launch_translate_table(thx_ctx* c) {
-    dim3 grid((c->Kp + 255) / 256, c->Tpad);
+    dim3 grid((c->Kp + 255) / 256, c->TpadS);
     hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, c->stream, c->trans.as<const double>(),
-                       c->T, c->Tpad, c->iCol.as<const int>(), c->iRow.as<const int>(), c->K,
+                       c->T, c->TpadS, c->iCol.as<const int>(), c->iRow.as<const int>(), c->K,
                        c->Kp, c->idim, c->S.as<float2>());
     THX_HIP(hipGetLastError());
     return THX_OK;
@@ -683,6 +684,285 @@ __global__ __launch_bounds__(256, 2) void k_score(
     }
 }
 
+// ---------------------------------------------------------------------------
+// 16-bit scoring kernel (BF16 / FP16 operands, FP32 accumulation).
+// Block tile: 128 rows (PB = 128 / TC particles x TC translations) x 256
+// rotations; 4 waves of 64 x 128 (2 x 4 tiles of v_mfma_f32_32x32x16).  The
+// next K step's global loads are issued before the current step's MFMAs so
+// their latency hides behind matrix work; the epilogue runs on registers
+// (butterfly reductions) with only small LDS exchanges.
+// ---------------------------------------------------------------------------
+constexpr int kNB16 = 256;
+constexpr int kLD16 = 72;   // bf16 elements per LDS row (64 + 8 pad)
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// (lo, hi) -> one packed pair with a single v_cvt_pk_{bf16,f16}_f32 (RNE)
+template <int PREC>
+__device__ inline uint32_t pack16(float lo, float hi) {
+    const floatx2 f = {lo, hi};
+    if constexpr (PREC == THX_BF16)
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
+    else
+        return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, halfx2));
+}
+
+template <int PREC, int TC>
+__global__ __launch_bounds__(512, 1) void k_score16(
+    const float2* __restrict__ Y, const float2* __restrict__ S,
+    const uint32_t* __restrict__ Bop, const float* __restrict__ Cterm,
+    const double* __restrict__ Aterm, const float* __restrict__ pRf,
+    const float* __restrict__ pTf, int P, int R, int T, int Kp, int Rpad,
+    int Tpad, int nRb, int nTc, int nPg, float* __restrict__ partMax,
+    float* __restrict__ partWC, int* __restrict__ partArg,
+    float* __restrict__ partWR, float* __restrict__ partWT) {
+    constexpr int PB = 128 / TC;
+    constexpr int NB = kNB16;
+    constexpr int LD = kLD16;
+    constexpr int NTH = 512;
+    __shared__ __attribute__((aligned(16))) unsigned short As[128 * LD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[NB * LD];
+    __shared__ float redMax[4][4];        // [32-row tile][column quarter]
+    __shared__ int redArg[4][4];
+    __shared__ float tileMax[PB];
+    __shared__ int tileArg[PB];
+    __shared__ float colPart[4][NB];      // [32-row tile][column]
+    __shared__ float rowPart[4][128];     // [column quarter][row]
+    __shared__ float rowW[128];
+
+    const int Lg = xcd_logical(blockIdx.x, gridDim.x);
+    const int pg = Lg % nPg;
+    const int rest = Lg / nPg;
+    const int tc = rest % nTc;
+    const int rb = rest / nTc;
+    const int p0 = pg * PB, r0 = rb * NB, t0 = tc * TC;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wmi = wave >> 2, wni = wave & 3;       // 2 x 4 wave grid, 64 x 64 each
+    const int wm = wmi * 64, wn = wni * 64;
+    const int l32 = lane & 31, hl = lane >> 5;
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = floatx16{0};
+
+    // Loader roles (lanes of one instruction read whole contiguous rows):
+    //   A: lane part = tid % 16 owns complex pixels 2 part, 2 part + 1 of the
+    //      step; rows m = tid / 16 + 32 v (v = 0..3).  Row m is particle
+    //      m / TC, translation m % TC: a thread needs NT = TC / 32 phase rows
+    //      and 4 / NT particle rows, one float4 each.
+    //   B: lane part8 = tid % 8 owns 4 complex pixels of rows tid / 8 + 64 v.
+    constexpr int NT = TC / 32;
+    constexpr int NY = 4 / NT;
+    const int part = tid & 15, rbase = tid >> 4;      // rbase 0..31
+    const int part8 = tid & 7, rb8 = tid >> 3;        // rb8 0..63
+    const float4* yp = reinterpret_cast<const float4*>(Y + (size_t)(p0 + rbase / TC) * Kp) + part;
+    const float4* sp = reinterpret_cast<const float4*>(S + (size_t)(t0 + rbase % TC) * Kp) + part;
+    const u32x4* bp = reinterpret_cast<const u32x4*>(Bop + (size_t)(r0 + rb8) * Kp) + part8;
+    const size_t rowF4 = (size_t)Kp / 2;
+    const size_t rowU4 = (size_t)Kp / 4;
+    // row m = rbase + 32 v: particle (rbase + 32 v) / TC, translation (rbase + 32 v) % TC
+    //   TC = 32:  particle v,        translation rbase           (NY = 4, NT = 1)
+    //   TC = 64:  particle v / 2,    translation rbase + 32 (v%2) (NY = 2, NT = 2)
+    //   TC = 128: particle 0,        translation rbase + 32 v     (NY = 1, NT = 4)
+
+    // (plain macro, not a lambda: a lambda capturing the staging arrays by
+    // reference keeps them in scratch memory instead of registers)
+#define THX_GLOAD16(K0)                                                          \
+    do {                                                                         \
+        const int f4_ = (K0) / 2, u4_ = (K0) / 4;                                \
+        _Pragma("unroll") for (int q = 0; q < NY; q++) ry[q] = yp[(size_t)q * rowF4 + f4_]; \
+        _Pragma("unroll") for (int u = 0; u < NT; u++) rs[u] = sp[(size_t)32 * u * rowF4 + f4_]; \
+        _Pragma("unroll") for (int v = 0; v < 4; v++) rbv[v] = bp[(size_t)64 * v * rowU4 + u4_]; \
+    } while (0)
+    float4 ry[NY], rs[NT];
+    u32x4 rbv[4];
+    THX_GLOAD16(0);
+    for (int k0 = 0; k0 < Kp; k0 += kChunk) {
+        // ---- stage this step's A (Z = Y S) and B into LDS ----
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const float4 y = ry[v / NT], sv = rs[v % NT];
+            const float z0r = y.x * sv.x - y.y * sv.y, z0i = -(y.x * sv.y + y.y * sv.x);
+            const float z1r = y.z * sv.z - y.w * sv.w, z1i = -(y.z * sv.w + y.w * sv.z);
+            uint2 z;
+            z.x = pack16<PREC>(z0r, z0i);
+            z.y = pack16<PREC>(z1r, z1i);
+            *reinterpret_cast<uint2*>(As + (rbase + 32 * v) * LD + 4 * part) = z;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            *reinterpret_cast<u32x4*>(Bs + (rb8 + 64 * v) * LD + 8 * part8) = rbv[v];
+        __syncthreads();
+        if (k0 + kChunk < Kp) THX_GLOAD16(k0 + kChunk);
+        // ---- MFMA ----
+#pragma unroll
+        for (int kk = 0; kk < 64; kk += 16) {
+            const int kc = kk + 8 * hl;
+            uint4 ra[2], rb4[2];
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+                ra[mt] = *reinterpret_cast<const uint4*>(As + (wm + 32 * mt + l32) * LD + kc);
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++)
+                rb4[nt] = *reinterpret_cast<const uint4*>(Bs + (wn + 32 * nt + l32) * LD + kc);
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++) {
+                    if constexpr (PREC == THX_BF16)
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, ra[mt]), __builtin_bit_cast(bf16x8, rb4[nt]),
+                            acc[mt][nt], 0, 0, 0);
+                    else
+                        acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            __builtin_bit_cast(halfx8, ra[mt]), __builtin_bit_cast(halfx8, rb4[nt]),
+                            acc[mt][nt], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+
+#undef THX_GLOAD16
+    // ---- epilogue: dvp in registers ----
+    // lane owns rows m = wm + 32 mt + 8 (i / 4) + 4 hl + (i % 4), columns n = wn + 32 nt + l32.
+    // TC >= 32, so all rows of one 32-row tile belong to one particle.
+    float lmax[2];
+    int larg[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+        const int mrow0 = wm + 32 * mt;
+        const int p = p0 + mrow0 / TC;
+        const bool pok = p < P;
+        const float a = pok ? (float)Aterm[p] : 0.f;
+        lmax[mt] = -INFINITY;
+        larg[mt] = -1;
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            const int n = wn + 32 * nt + l32;
+            const int r = r0 + n;
+            const bool rok = pok && r < R;
+            const float cterm = rok ? Cterm[(size_t)p * Rpad + r] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = mrow0 + 8 * (i >> 2) + 4 * hl + (i & 3);
+                const int t = t0 + m % TC;
+                float v = -INFINITY;
+                if (rok && t < T) v = a + cterm - 2.f * acc[mt][nt][i];
+                acc[mt][nt][i] = v;
+                if (v > lmax[mt]) { lmax[mt] = v; larg[mt] = m * NB + n; }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const float o = __shfl_xor(lmax[mt], off);
+            const int oa = __shfl_xor(larg[mt], off);
+            if (o > lmax[mt] || (o == lmax[mt] && oa >= 0 && (larg[mt] < 0 || oa < larg[mt]))) {
+                lmax[mt] = o;
+                larg[mt] = oa;
+            }
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            redMax[2 * wmi + mt][wni] = lmax[mt];
+            redArg[2 * wmi + mt][wni] = larg[mt];
+        }
+    }
+    __syncthreads();
+    if (tid < PB) {
+        constexpr int TPP = TC / 32;   // 32-row tiles per particle
+        float mx = -INFINITY;
+        int arg = -1;
+        for (int q = tid * TPP; q < (tid + 1) * TPP; q++)
+            for (int h = 0; h < 4; h++) {
+                const float v = redMax[q][h];
+                const int a = redArg[q][h];
+                if (v > mx || (v == mx && a >= 0 && (arg < 0 || a < arg))) { mx = v; arg = a; }
+            }
+        tileMax[tid] = mx;
+        tileArg[tid] = arg;
+    }
+    __syncthreads();
+
+    // ---- weights, column sums (x pT) and row sums (x pR) ----
+    float rowv[32];   // per-lane partial row sums, index mt * 16 + i
+#pragma unroll
+    for (int j = 0; j < 32; j++) rowv[j] = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++) {
+        const int mrow0 = wm + 32 * mt;
+        const float mx = tileMax[mrow0 / TC];
+        const bool live = mx > -INFINITY;
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++) {
+            const int n = wn + 32 * nt + l32;
+            const float pr = pRf[r0 + n];
+            float cs = 0.f;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = mrow0 + 8 * (i >> 2) + 4 * hl + (i & 3);
+                const float w = live ? __expf(acc[mt][nt][i] - mx) : 0.f;
+                cs += w * pTf[t0 + m % TC];
+                rowv[mt * 16 + i] += w * pr;
+            }
+            cs += __shfl_xor(cs, 32);
+            if (hl == 0) colPart[2 * wmi + mt][n] = cs;
+        }
+    }
+    // row sums: butterfly over the 32 lanes of each half (lane l32 ends with row index l32)
+#pragma unroll
+    for (int s = 16; s >= 1; s >>= 1) {
+        const bool up = (l32 & s) != 0;
+#pragma unroll
+        for (int j = 0; j < s; j++) {
+            const float send = up ? rowv[j] : rowv[j + s];
+            const float keep = up ? rowv[j + s] : rowv[j];
+            rowv[j] = keep + __shfl_xor(send, s);
+        }
+    }
+    {
+        const int j = l32;
+        const int mt = j >> 4, i = j & 15;
+        const int m = wm + 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+        rowPart[wni][m] = rowv[0];
+    }
+    __syncthreads();
+    // column sums per particle: sum of its 32-row tiles
+    for (int q = tid; q < PB * NB; q += NTH) {
+        const int pi = q / NB, n = q % NB;
+        const int p = p0 + pi;
+        float cs = 0.f;
+#pragma unroll
+        for (int u = 0; u < TC / 32; u++) cs += colPart[pi * (TC / 32) + u][n];
+        if (p < P) partWR[((size_t)p * nTc + tc) * Rpad + r0 + n] = cs;
+    }
+    if (tid < 128) {
+        const int m = tid;
+        const float rsum = rowPart[0][m] + rowPart[1][m] + rowPart[2][m] + rowPart[3][m];
+        const int p = p0 + m / TC, t = t0 + m % TC;
+        if (p < P) partWT[((size_t)p * nRb + rb) * Tpad + t] = rsum;
+        rowW[m] = rsum * pTf[t];
+    }
+    __syncthreads();
+    if (tid < PB) {
+        const int p = p0 + tid;
+        if (p < P) {
+            float wc = 0.f;
+            for (int m = tid * TC; m < (tid + 1) * TC; m++) wc += rowW[m];
+            const size_t o = ((size_t)p * nTc + tc) * nRb + rb;
+            partMax[o] = tileMax[tid];
+            partWC[o] = wc;
+            const int a = tileArg[tid];
+            partArg[o] = a < 0 ? -1 : ((r0 + (a % NB)) * Tpad + (t0 + (a / NB) % TC));
+        }
+    }
+}
+
 template <int PREC>
 static void launch_score_prec(thx_ctx* c, const void* B, int nPg, int nblk) {
 #define THX_SCORE_ARGS                                                                         \
@@ -690,20 +970,40 @@ static void launch_score_prec(thx_ctx* c, const void* B, int nPg, int nblk) {
         c->Aterm.as<const double>(), c->pRf.as<const float>(), c->pTf.as<const float>(), c->P, \
         c->R, c->T, c->Kp, c->Rpad, c->Tpad, c->nRb, c->nTc, nPg, c->partMax.as<float>(),      \
         c->partWC.as<float>(), c->partArg.as<int>(), c->partWR.as<float>(), c->partWT.as<float>()
-    if (c->TC == 128)
-        hipLaunchKernelGGL((k_score<PREC, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
-    else if (c->TC == 64)
-        hipLaunchKernelGGL((k_score<PREC, 64>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
-    else
-        hipLaunchKernelGGL((k_score<PREC, 32>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
+    if constexpr (PREC == THX_FP32) {
+        if (c->TC == 128)
+            hipLaunchKernelGGL((k_score<PREC, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
+        else if (c->TC == 64)
+            hipLaunchKernelGGL((k_score<PREC, 64>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
+        else
+            hipLaunchKernelGGL((k_score<PREC, 32>), dim3(nblk), dim3(256), 0, c->stream, THX_SCORE_ARGS);
+    } else {
+        const uint32_t* B16 = static_cast<const uint32_t*>(B);
+#define THX_SCORE16_ARGS                                                                       \
+    c->Y.as<const float2>(), c->S.as<const float2>(), B16, c->Cterm.as<const float>(),        \
+        c->Aterm.as<const double>(), c->pRf.as<const float>(), c->pTf.as<const float>(), c->P, \
+        c->R, c->T, c->Kp, c->Rpad, c->Tpad, c->nRb, c->nTc, nPg, c->partMax.as<float>(),      \
+        c->partWC.as<float>(), c->partArg.as<int>(), c->partWR.as<float>(), c->partWT.as<float>()
+        if (c->TC == 128)
+            hipLaunchKernelGGL((k_score16<PREC, 128>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
+        else if (c->TC == 64)
+            hipLaunchKernelGGL((k_score16<PREC, 64>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
+        else
+            hipLaunchKernelGGL((k_score16<PREC, 32>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
+#undef THX_SCORE16_ARGS
+    }
 #undef THX_SCORE_ARGS
 }
+
+int score_tile_cols(int precision) { return precision == THX_FP32 ? 128 : kNB16; }
 
 int launch_score(thx_ctx* c, int precision) {
     const int PB = 128 / c->TC;
     const int nPg = (c->P + PB - 1) / PB;
     const long long nblk = (long long)nPg * c->nTc * c->nRb;
     THX_CHECK(nblk < (1ll << 31), THX_ERR_RANGE, "score grid too large");
+    THX_CHECK(c->NB == score_tile_cols(precision) && c->Rpad % c->NB == 0, THX_ERR_STATE,
+              "score: scan shape not prepared for this precision");
     if (precision == THX_FP32)
         launch_score_prec<THX_FP32>(c, c->rotP.p, nPg, (int)nblk);
     else if (precision == THX_BF16)
@@ -729,7 +1029,7 @@ __global__ __launch_bounds__(256) void k_merge(
     const float* __restrict__ partMax, const float* __restrict__ partWC,
     const int* __restrict__ partArg, const float* __restrict__ partWR,
     const float* __restrict__ partWT, int P, int R, int T, int Rpad, int Tpad,
-    int TC, int nTc, int nRb, double* __restrict__ wC, double* __restrict__ wR,
+    int TC, int nTc, int nRb, int NB, double* __restrict__ wC, double* __restrict__ wR,
     double* __restrict__ wT, double* __restrict__ base, int* __restrict__ bestR,
     int* __restrict__ bestT, double* __restrict__ bestScore) {
     const int p = blockIdx.x;
@@ -783,7 +1083,7 @@ __global__ __launch_bounds__(256) void k_merge(
     }
     // f index = tc * nRb + rb
     for (int r = threadIdx.x; r < R; r += blockDim.x) {
-        const int rb = r / 128;
+        const int rb = r / NB;
         double s = 0.0;
         for (int tc = 0; tc < nTc; tc++)
             s += f[tc * nRb + rb] * partWR[((size_t)p * nTc + tc) * Rpad + r];
@@ -803,7 +1103,7 @@ int launch_merge(thx_ctx* c, int cls) {
     hipLaunchKernelGGL(k_merge, dim3(c->P), dim3(256), 0, c->stream, c->partMax.as<const float>(),
                        c->partWC.as<const float>(), c->partArg.as<const int>(),
                        c->partWR.as<const float>(), c->partWT.as<const float>(), c->P, c->R,
-                       c->T, c->Rpad, c->Tpad, c->TC, c->nTc, c->nRb, r.wC.as<double>(),
+                       c->T, c->Rpad, c->Tpad, c->TC, c->nTc, c->nRb, c->NB, r.wC.as<double>(),
                        r.wR.as<double>(), r.wT.as<double>(), r.base.as<double>(),
                        r.bestR.as<int>(), r.bestT.as<int>(), r.bestScore.as<double>());
     THX_HIP(hipGetLastError());
