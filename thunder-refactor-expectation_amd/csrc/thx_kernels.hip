@@ -6,7 +6,7 @@
 //   rotLo  2 x 16-bit [Rpad][Kp]      low-precision B operand, shell-scaled
 //   normP  float  [Rpad][Kp]          |P_rk|^2
 //   S      float2 [Tpad][Kp]          translation phase factors
-//   Y      float2 [Ppad][Kp]          sigRcp * ctf * conj(X) * shell scale
+//   Y      float2 [Ppad][Kp]          sigRcp * ctf * X * shell scale
 //   Wt     float  [Ppad][Kp]          sigRcp * ctf^2
 //   Aterm  double [Ppad]              sum_k sigRcp |X|^2
 //   Cterm  float  [Ppad][Rpad]        sum_k Wt |P|^2
@@ -14,9 +14,11 @@
 //   dvp(p,r,t) = sum_k sigRcp_pk |X_pk - ctf_pk S_tk P_rk|^2
 // (kernel_logDataVS, gpu/src/Kernel.cu:947-1004; logDataVSPrior_m_n_huabin,
 // src/Optimiser.cpp:9666-9708) is evaluated as
-//   dvp = Aterm_p + Cterm_pr - 2 Re sum_k (Y_pk S_tk) P_rk
+//   dvp = Aterm_p + Cterm_pr - 2 Re sum_k conj(Y_pk conj(S_tk)) P_rk
 // where the last sum is a dense (particle x translation) x rotation
-// contraction over 2K reals that runs on MFMA.
+// contraction over 2K reals that runs on MFMA: with Z = Y conj(S), the A
+// operand row is (Re Z, Im Z) per pixel and the B operand (Re P, Im P), so
+// sum_j A_j B_j = Re sum_k conj(Z) P.
 
 #include <hip/hip_runtime.h>
 
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(256) void k_prep(
             const float cc = ctf[i], s = sig[i];
             a += (double)s * ((double)x.x * x.x + (double)x.y * x.y);
             const float sc = s * cc * (shellScale ? shellScale[iSig[k]] : 1.f);
-            y = make_float2(sc * x.x, -sc * x.y);
+            y = make_float2(sc * x.x, sc * x.y);
             w = s * cc * cc;
         }
         Y[(size_t)p * Kp + k] = y;
@@ -422,7 +424,7 @@ int launch_norm_gemm(thx_ctx* c) {
 // ---------------------------------------------------------------------------
 // Scoring kernel.  A block owns PB = 128 / TC particles x TC translations
 // (rows) and 128 rotations (columns).  Per K step of 32 complex pixels:
-//   A[m][2k] = Re Z, A[m][2k+1] = -Im Z with Z = Y[p(m)][k] S[t(m)][k]
+//   A[m][2k] = Re Z, A[m][2k+1] = Im Z with Z = Y[p(m)][k] conj(S[t(m)][k])
 //   B[n][2k] = Re P, B[n][2k+1] = Im P
 // so that sum_j A[m][j] B[n][j] = Re sum_k Z P.  Z is formed in registers
 // from the particle row and the phase table, never stored in HBM.
@@ -492,7 +494,7 @@ __global__ __launch_bounds__(256, 2) void k_score(
     const int lrot = r0 + lr;
 
     for (int k0 = 0; k0 < Kp; k0 += kChunk) {
-        // ---- stage A (Z = Y S) and B into LDS ----
+        // ---- stage A (Z = Y conj(S)) and B into LDS ----
         if constexpr (F32) {
             float* As = reinterpret_cast<float*>(smem);
             float* Bs = As + 128 * LD;
@@ -506,10 +508,11 @@ __global__ __launch_bounds__(256, 2) void k_score(
             for (int v = 0; v < 8; v++) {
                 const float4 y = yv[v], s = sv[v], b = bv[v];
                 // two complex values per float4
-                da[4 * v + 0] = y.x * s.x - y.y * s.y;
-                da[4 * v + 1] = -(y.x * s.y + y.y * s.x);
-                da[4 * v + 2] = y.z * s.z - y.w * s.w;
-                da[4 * v + 3] = -(y.z * s.w + y.w * s.z);
+                // Z = Y conj(S), two complex values per float4
+                da[4 * v + 0] = y.x * s.x + y.y * s.y;
+                da[4 * v + 1] = y.y * s.x - y.x * s.y;
+                da[4 * v + 2] = y.z * s.z + y.w * s.w;
+                da[4 * v + 3] = y.w * s.z - y.z * s.w;
                 db[4 * v + 0] = b.x;
                 db[4 * v + 1] = b.y;
                 db[4 * v + 2] = b.z;
@@ -527,8 +530,8 @@ __global__ __launch_bounds__(256, 2) void k_score(
 #pragma unroll
             for (int v = 0; v < 8; v++) {
                 const float4 y = yv[v], s = sv[v];
-                const float z0r = y.x * s.x - y.y * s.y, z0i = -(y.x * s.y + y.y * s.x);
-                const float z1r = y.z * s.z - y.w * s.w, z1i = -(y.z * s.w + y.w * s.z);
+                const float z0r = y.x * s.x + y.y * s.y, z0i = y.y * s.x - y.x * s.y;
+                const float z1r = y.z * s.z + y.w * s.w, z1i = y.w * s.z - y.z * s.w;
                 da[2 * v + 0] = (uint32_t)to16<PREC>(z0r) | ((uint32_t)to16<PREC>(z0i) << 16);
                 da[2 * v + 1] = (uint32_t)to16<PREC>(z1r) | ((uint32_t)to16<PREC>(z1i) << 16);
             }
@@ -722,8 +725,9 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     constexpr int NB = kNB16;
     constexpr int LD = kLD16;
     constexpr int NTH = 512;
-    __shared__ __attribute__((aligned(16))) unsigned short As[128 * LD];
-    __shared__ __attribute__((aligned(16))) unsigned short Bs[NB * LD];
+    // two LDS stages: chunk c + 1 is staged while chunk c feeds the MFMAs
+    __shared__ __attribute__((aligned(16))) unsigned short As[2][128 * LD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NB * LD];
     __shared__ float redMax[4][4];        // [32-row tile][column quarter]
     __shared__ int redArg[4][4];
     __shared__ float tileMax[PB];
@@ -781,35 +785,44 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     } while (0)
     float4 ry[NY], rs[NT];
     u32x4 rbv[4];
+    // stage registers of one chunk into LDS buffer BUF: A = Z = Y conj(S), B as loaded
+#define THX_STAGE16(BUF)                                                                       \
+    do {                                                                                       \
+        _Pragma("unroll") for (int v = 0; v < 4; v++) {                                        \
+            const float4 y = ry[v / NT], sv = rs[v % NT];                                      \
+            const float z0r = y.x * sv.x + y.y * sv.y, z0i = y.y * sv.x - y.x * sv.y;         \
+            const float z1r = y.z * sv.z + y.w * sv.w, z1i = y.w * sv.z - y.z * sv.w;         \
+            uint2 z;                                                                           \
+            z.x = pack16<PREC>(z0r, z0i);                                                      \
+            z.y = pack16<PREC>(z1r, z1i);                                                      \
+            *reinterpret_cast<uint2*>(As[BUF] + (rbase + 32 * v) * LD + 4 * part) = z;         \
+        }                                                                                      \
+        _Pragma("unroll") for (int v = 0; v < 4; v++)                                          \
+            *reinterpret_cast<u32x4*>(Bs[BUF] + (rb8 + 64 * v) * LD + 8 * part8) = rbv[v];     \
+    } while (0)
     THX_GLOAD16(0);
+    THX_STAGE16(0);
+    if (kChunk < Kp) THX_GLOAD16(kChunk);
+    __syncthreads();
+    int buf = 0;
     for (int k0 = 0; k0 < Kp; k0 += kChunk) {
-        // ---- stage this step's A (Z = Y S) and B into LDS ----
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const float4 y = ry[v / NT], sv = rs[v % NT];
-            const float z0r = y.x * sv.x - y.y * sv.y, z0i = -(y.x * sv.y + y.y * sv.x);
-            const float z1r = y.z * sv.z - y.w * sv.w, z1i = -(y.z * sv.w + y.w * sv.z);
-            uint2 z;
-            z.x = pack16<PREC>(z0r, z0i);
-            z.y = pack16<PREC>(z1r, z1i);
-            *reinterpret_cast<uint2*>(As + (rbase + 32 * v) * LD + 4 * part) = z;
+        if (k0 + kChunk < Kp) {
+            THX_STAGE16(buf ^ 1);
+            if (k0 + 2 * kChunk < Kp) THX_GLOAD16(k0 + 2 * kChunk);
         }
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-            *reinterpret_cast<u32x4*>(Bs + (rb8 + 64 * v) * LD + 8 * part8) = rbv[v];
-        __syncthreads();
-        if (k0 + kChunk < Kp) THX_GLOAD16(k0 + kChunk);
-        // ---- MFMA ----
+        // ---- MFMA on the current buffer ----
+        const unsigned short* Ab = As[buf];
+        const unsigned short* Bb = Bs[buf];
 #pragma unroll
         for (int kk = 0; kk < 64; kk += 16) {
             const int kc = kk + 8 * hl;
             uint4 ra[2], rb4[2];
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                ra[mt] = *reinterpret_cast<const uint4*>(As + (wm + 32 * mt + l32) * LD + kc);
+                ra[mt] = *reinterpret_cast<const uint4*>(Ab + (wm + 32 * mt + l32) * LD + kc);
 #pragma unroll
             for (int nt = 0; nt < 2; nt++)
-                rb4[nt] = *reinterpret_cast<const uint4*>(Bs + (wn + 32 * nt + l32) * LD + kc);
+                rb4[nt] = *reinterpret_cast<const uint4*>(Bb + (wn + 32 * nt + l32) * LD + kc);
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
 #pragma unroll
@@ -825,8 +838,10 @@ __global__ __launch_bounds__(512, 1) void k_score16(
                 }
         }
         __syncthreads();
+        buf ^= 1;
     }
 
+#undef THX_STAGE16
 #undef THX_GLOAD16
     // ---- epilogue: dvp in registers ----
     // lane owns rows m = wm + 32 mt + 8 (i / 4) + 4 hl + (i % 4), columns n = wn + 32 nt + l32.
