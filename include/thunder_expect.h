@@ -163,6 +163,27 @@ int thx_ctx_get_best(thx_ctx* ctx, int cls, int* bestR, int* bestT,
  * checking: out[R*T] (r slowest).  Uses the plain FP32 path. */
 int thx_ctx_score_one(thx_ctx* ctx, int cls, int p, float* out);
 
+/* One local-search phase for every particle of the context, batched and
+ * ragged (Optimiser::updateSP, src/Optimiser.cpp:1139-1437; on the GPU path
+ * ExpectLocalRTD + ExpectLocalPreI3D + ExpectLocalM, gpu/interface/
+ * Interface.h:73-147, which handle one particle per call).  Particle p scores
+ * its own nR[p] rotations (quaternions, concatenated over particles) and nT[p]
+ * <= 128 translations (pixels, concatenated) with prior weights pR, pT (same
+ * concatenation), against the volume of class cls[p] (cls may be NULL: class
+ * 0).  Slices are projected on the fly; no R x K intermediate is stored.
+ * precision: THX_FP32 or THX_BF16.  Marginals as in the global scan:
+ *   wC[p] = sum w pR pT, wR[r] = sum_t w pT, wT[t] = sum_r w pR,
+ *   w = exp(dvp - baseL[p]), baseL[p] = max dvp. */
+int thx_ctx_score_local(thx_ctx* ctx, const int* cls, const double* quat,
+                        const int* nR, const double* trans, const int* nT,
+                        const double* pR, const double* pT, int precision);
+
+/* Results of the last local phase: wC[P], wR[sum nR], wT[sum nT], baseL[P],
+ * bestR[P] / bestT[P] (indices within the particle's own lists).  Any
+ * output may be NULL.  Synchronises the stream. */
+int thx_ctx_get_local(thx_ctx* ctx, double* wC, double* wR, double* wT,
+                      double* baseL, int* bestR, int* bestT);
+
 /* Waits for all work queued on the context stream. */
 int thx_ctx_sync(thx_ctx* ctx);
 

@@ -37,6 +37,8 @@ def load():
         "thxo_global_scan_f64": ([_p, _i, _i, _p, _p, _i, _i, _p, _i, _p, _i, _p, _p, _p, _i,
                                   _p, _p, _p, _p, _p, _p, _p, _i], _i),
         "thxo_resample": ([_p, _p, _i, _i, _d, _p, _p], _i),
+        "thxo_local_scan_f64": ([_p, _i, _i, _p, _p, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
+                                 _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _i], _i),
         "thxo_blob_ft": ([_p, _i, _i, _p, _p, _i, _d, _p, _p, _p, _i], None),
     }
     for name, (args, res) in sigs.items():
@@ -153,6 +155,34 @@ def global_scan(vol, V, pf, iCol, iRow, N, quat, trans, dat, ctf_, sig, pR, pT,
                                      _ptr(base), threads)
     assert st == 0
     return wC, wR, wT, base
+
+
+def local_scan(vol, V, pf, iCol, iRow, N, quat, nR, trans, nT, pR, pT, dat, ctf_, sig,
+               threads=0):
+    """One local-search phase (Optimiser::updateSP, src/Optimiser.cpp:1139-1437), FP64.
+    quat / pR are the particles' rotations concatenated (counts nR); trans / pT
+    likewise (counts nT).  Returns wC[P], wR (concatenated), wT (concatenated),
+    base[P], bestR[P], bestT[P]."""
+    vol = _c(vol, np.float32)
+    iCol, iRow = _c(iCol, np.int32), _c(iRow, np.int32)
+    quat = _c(quat, np.float64).reshape(-1, 4)
+    trans = _c(trans, np.float64).reshape(-1, 2)
+    nR, nT = _c(nR, np.int32), _c(nT, np.int32)
+    rOff = np.r_[0, np.cumsum(nR)[:-1]].astype(np.int32)
+    tOff = np.r_[0, np.cumsum(nT)[:-1]].astype(np.int32)
+    pR, pT = _c(pR, np.float64), _c(pT, np.float64)
+    dat, ctf_, sig = _c(dat, np.float32), _c(ctf_, np.float32), _c(sig, np.float32)
+    P = len(nR)
+    wC, base = np.zeros(P), np.zeros(P)
+    wR, wT = np.zeros(int(nR.sum())), np.zeros(int(nT.sum()))
+    bR, bT = np.zeros(P, np.int32), np.zeros(P, np.int32)
+    st = load().thxo_local_scan_f64(_ptr(vol), V, pf, _ptr(iCol), _ptr(iRow), iCol.size, N,
+                                    _ptr(quat), _ptr(rOff), _ptr(nR), _ptr(trans), _ptr(tOff),
+                                    _ptr(nT), _ptr(pR), _ptr(pT), _ptr(dat), _ptr(ctf_),
+                                    _ptr(sig), P, _ptr(wC), _ptr(wR), _ptr(wT), _ptr(base),
+                                    _ptr(bR), _ptr(bT), threads)
+    assert st == 0
+    return wC, wR, wT, base, bR, bT
 
 
 def resample(w, u, n_out, u0):

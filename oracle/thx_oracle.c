@@ -415,6 +415,95 @@ int thxo_global_scan_f64(const float* vol, int V, int pf,
 }
 
 /*
+ * One local-search phase for a batch of particles, FP64 judge
+ * (Optimiser::updateSP, src/Optimiser.cpp:1139-1437, one class, no CTF
+ * search): particle l scores its own rotations quat[rOff[l] .. + nR[l]) and
+ * translations trans[tOff[l] .. + nT[l]) with priors pR, pT (same offsets):
+ * base = max dvp, wC = sum w pR pT, wR[r] = sum_t w pT, wT[t] = sum_r w pR
+ * (the reference's streaming baseline of :1330-1349 in two-pass form, with
+ * wC(iC) = 1).  wR and wT use the rOff / tOff offsets; wC, base per particle.
+ */
+int thxo_local_scan_f64(const float* vol, int V, int pf, const int* iCol, const int* iRow,
+                        int K, int N, const double* quat, const int* rOff, const int* nR,
+                        const double* trans, const int* tOff, const int* nT,
+                        const double* pR, const double* pT, const float* datP,
+                        const float* ctfP, const float* sigRcpP, int P, double* wC,
+                        double* wR, double* wT, double* baseL, int* bestR, int* bestT,
+                        int nThreads)
+{
+#ifdef _OPENMP
+    if (nThreads > 0) omp_set_num_threads(nThreads);
+#else
+    (void)nThreads;
+#endif
+    int fail = 0;
+    #pragma omp parallel for schedule(dynamic)
+    for (int l = 0; l < P; l++) {
+        const int R = nR[l], T = nT[l];
+        double* rot = (double*)malloc(sizeof(double) * 2 * (size_t)R * K);
+        double* tra = (double*)malloc(sizeof(double) * 2 * (size_t)T * K);
+        double* sc = (double*)malloc(sizeof(double) * (size_t)R * T);
+        if (!rot || !tra || !sc) {
+            free(rot); free(tra); free(sc);
+            #pragma omp atomic write
+            fail = 1;
+            continue;
+        }
+        for (int m = 0; m < R; m++) {
+            double M[9];
+            thxo_rotmat(M, quat + 4 * (size_t)(rOff[l] + m));
+            thxo_project_f64(rot + (size_t)2 * m * K, vol, V, M, iCol, iRow, K, pf);
+        }
+        for (int n = 0; n < T; n++) {
+            const double* t = trans + 2 * (size_t)(tOff[l] + n);
+            for (int i = 0; i < K; i++) {
+                double ph = 2 * M_PI * (iCol[i] * t[0] / N + iRow[i] * t[1] / N);
+                tra[(size_t)2 * (n * K + i)] = cos(-ph);
+                tra[(size_t)2 * (n * K + i) + 1] = sin(-ph);
+            }
+        }
+        const float* d = datP + (size_t)2 * l * K;
+        const float* c = ctfP + (size_t)l * K;
+        const float* s = sigRcpP + (size_t)l * K;
+        double mx = -INFINITY;
+        int br = -1, bt = -1;
+        for (int m = 0; m < R; m++)
+            for (int n = 0; n < T; n++) {
+                const double* pr = rot + (size_t)2 * m * K;
+                const double* tr = tra + (size_t)2 * n * K;
+                double acc = 0;
+                for (int i = 0; i < K; i++) {
+                    double qr = tr[2 * i] * pr[2 * i] - tr[2 * i + 1] * pr[2 * i + 1];
+                    double qi = tr[2 * i] * pr[2 * i + 1] + tr[2 * i + 1] * pr[2 * i];
+                    double er = d[2 * i] - c[i] * qr, ei = d[2 * i + 1] - c[i] * qi;
+                    acc += (er * er + ei * ei) * s[i];
+                }
+                sc[(size_t)m * T + n] = acc;
+                if (acc > mx) { mx = acc; br = m; bt = n; }
+            }
+        baseL[l] = mx;
+        if (bestR) bestR[l] = br;
+        if (bestT) bestT[l] = bt;
+        double c0 = 0;
+        double* wr = wR + rOff[l];
+        double* wt = wT + tOff[l];
+        for (int m = 0; m < R; m++) wr[m] = 0;
+        for (int n = 0; n < T; n++) wt[n] = 0;
+        for (int m = 0; m < R; m++)
+            for (int n = 0; n < T; n++) {
+                double w = exp(sc[(size_t)m * T + n] - mx);
+                double a = pR[rOff[l] + m], b = pT[tOff[l] + n];
+                c0 += w * a * b;
+                wr[m] += w * b;
+                wt[n] += w * a;
+            }
+        wC[l] = c0;
+        free(rot); free(tra); free(sc);
+    }
+    return fail ? -1 : 0;
+}
+
+/*
  * Particle::resample(n, pt), src/Particle.cpp:1275-1462, one sample type,
  * without the random shuffle (:2173-2271) and with u0 injected:
  * w <- w (.) u, normalise, cdf (d_cumsum) / cdf[last], u_j = u0 + j / nOut,

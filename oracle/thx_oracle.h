@@ -75,6 +75,15 @@ int thxo_global_scan_f64(const float* vol, int V, int pf,
                          double* wC, double* wR, double* wT, double* baseL,
                          double* dvp, int nThreads);
 
+/* a12: one local-search phase for a ragged batch, FP64 (see the .c file). */
+int thxo_local_scan_f64(const float* vol, int V, int pf, const int* iCol, const int* iRow,
+                        int K, int N, const double* quat, const int* rOff, const int* nR,
+                        const double* trans, const int* tOff, const int* nT,
+                        const double* pR, const double* pT, const float* datP,
+                        const float* ctfP, const float* sigRcpP, int P, double* wC,
+                        double* wR, double* wT, double* baseL, int* bestR, int* bestT,
+                        int nThreads);
+
 /* a10: systematic resampling with an injected offset u0 in [0, 1/nOut). */
 int thxo_resample(const double* w, const double* u, int nIn, int nOut,
                   double u0, int* idx, double* wNew);

@@ -1,0 +1,110 @@
+"""Ragged local-search scorer against the FP64 oracle (needs gfx950).
+
+Tolerances: FP32 baseline (best log-likelihood) relative error <= 1e-4 and the
+same best (rotation, translation) per particle; marginals within 3e-3 of each
+particle's largest entry (an FP32 log-likelihood of magnitude ~1e3 carries an
+absolute rounding error ~1e-3, which is the relative error of exp(dvp - max)).
+BF16: best-score relative error <= 2e-3 and >= 80% identical best poses on
+this small noisy case.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import thx
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+BOX, PF = 64, 2
+V = BOX * PF
+
+
+def _batch(info, P, rng, rmax, tmax):
+    nR = rng.integers(20, rmax + 1, P).astype(np.int32)
+    nR[0] = rmax                      # at least one particle spans several blocks
+    nT = rng.integers(1, tmax + 1, P).astype(np.int32)
+    nT[1] = tmax
+    quats, trans = [], []
+    for p in range(P):
+        q0 = info["true_quat"][p]
+        # half near the true pose, half anywhere
+        qn = q0 + 0.05 * rng.normal(size=(nR[p], 4))
+        qa = rng.normal(size=(nR[p], 4))
+        q = np.where((np.arange(nR[p]) % 2 == 0)[:, None], qn, qa)
+        quats.append(q / np.linalg.norm(q, axis=1, keepdims=True))
+        trans.append(info["true_shift"][p] + rng.uniform(-2, 2, size=(nT[p], 2)))
+    pR = rng.uniform(0.5, 1.5, int(nR.sum()))
+    pT = rng.uniform(0.5, 1.5, int(nT.sum()))
+    return np.concatenate(quats), nR, np.concatenate(trans), nT, pR, pT
+
+
+@pytest.fixture(scope="module")
+def local_case(gpu_ctx):
+    print("\n[setup] local box64 P=24", flush=True)
+    info = wl.setup_context(gpu_ctx, "box64", n_particles=24, n_rot=64)
+    vol = gpu_ctx.get_volume(0, V)
+    dat, ctf, sig = gpu_ctx.get_particles()
+    info.update(vol=vol, dat=dat, ctf=ctf, sig=sig)
+    return info
+
+
+@pytest.mark.parametrize("tmax", [25, 60, 100])
+def test_local_fp32_matches_oracle(gpu_ctx, local_case, tmax):
+    c = local_case
+    rng = np.random.default_rng(tmax)
+    P = 24
+    quat, nR, trans, nT, pR, pT = _batch(c, P, rng, 300, tmax)
+    print(f"[case] tmax={tmax} Rtot={nR.sum()} Ttot={nT.sum()}", flush=True)
+    gpu_ctx.score_local(quat, nR, trans, nT, pR, pT, precision=thx.FP32)
+    wC, wR, wT, base, bR, bT = gpu_ctx.get_local()
+    ref = oracle.local_scan(c["vol"], V, PF, c["iCol"], c["iRow"], BOX, quat, nR, trans, nT,
+                            pR, pT, c["dat"], c["ctf"], c["sig"])
+    rel = np.abs(base - ref[3]) / np.abs(ref[3])
+    print(f"FP32 local: max rel base err {rel.max():.2e}")
+    assert rel.max() <= 1e-4
+    np.testing.assert_array_equal(bR, ref[4])
+    np.testing.assert_array_equal(bT, ref[5])
+    np.testing.assert_allclose(wC, ref[0], rtol=3e-3)
+    ro = np.r_[0, np.cumsum(nR)]
+    to = np.r_[0, np.cumsum(nT)]
+    for p in range(P):
+        a, b = wR[ro[p]:ro[p + 1]], ref[1][ro[p]:ro[p + 1]]
+        assert np.abs(a - b).max() <= 3e-3 * b.max()
+        a, b = wT[to[p]:to[p + 1]], ref[2][to[p]:to[p + 1]]
+        assert np.abs(a - b).max() <= 3e-3 * b.max()
+
+
+def test_local_bf16_close_to_oracle(gpu_ctx, local_case):
+    c = local_case
+    rng = np.random.default_rng(5)
+    quat, nR, trans, nT, pR, pT = _batch(c, 24, rng, 260, 49)
+    gpu_ctx.score_local(quat, nR, trans, nT, pR, pT, precision=thx.BF16)
+    wC, wR, wT, base, bR, bT = gpu_ctx.get_local()
+    ref = oracle.local_scan(c["vol"], V, PF, c["iCol"], c["iRow"], BOX, quat, nR, trans, nT,
+                            pR, pT, c["dat"], c["ctf"], c["sig"])
+    rel = np.abs(base - ref[3]) / np.abs(ref[3])
+    agree = np.mean((bR == ref[4]) & (bT == ref[5]))
+    print(f"BF16 local: max rel base err {rel.max():.2e}, best-pose agreement {agree:.3f}")
+    assert rel.max() <= 2e-3
+    assert agree >= 0.8
+
+
+def test_local_matches_global_on_shared_grid(gpu_ctx, local_case):
+    """Local scan of every particle over the global grid equals the global scan."""
+    c = local_case
+    P = 24
+    R, T = len(c["quat"]), len(c["trans"])
+    quat = np.tile(c["quat"], (P, 1))
+    trans = np.tile(c["trans"], (P, 1))
+    nR = np.full(P, R, np.int32)
+    nT = np.full(P, T, np.int32)
+    gpu_ctx.score_local(quat, nR, trans, nT, precision=thx.FP32)
+    wC, wR, wT, base, bR, bT = gpu_ctx.get_local()
+    gpu_ctx.score_global(0, thx.FP32)
+    gC, gR, gT, gb = gpu_ctx.get_global(1)
+    gbR, gbT, _ = gpu_ctx.get_best(0)
+    np.testing.assert_allclose(base, gb, rtol=1e-5)
+    np.testing.assert_array_equal(bR, gbR)
+    np.testing.assert_array_equal(bT, gbT)
+    np.testing.assert_allclose(wC, gC[:, 0], rtol=1e-3)

@@ -156,6 +156,13 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
     return THX_OK;
 }
 
+template <class T>
+static int upload(DevBuf& d, const T* h, size_t n, hipStream_t st) {
+    THX_TRY(d.ensure(sizeof(T) * (n ? n : 1)));
+    if (n) THX_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
+    return THX_OK;
+}
+
 static std::mutex g_default_mu;
 static thx_ctx* g_default = nullptr;
 
@@ -220,6 +227,12 @@ int thx_ctx_destroy(thx_ctx* c) {
                       &c->shellMax, &c->partMax, &c->partWC, &c->partArg, &c->partWR,
                       &c->partWT};
     for (DevBuf* b : bufs) b->release();
+    DevBuf* lbufs[] = {&c->locQuat, &c->locTrans, &c->locPRf, &c->locPTf, &c->locROff,
+                       &c->locNR, &c->locTOff, &c->locNT, &c->locBlocks, &c->locBStart,
+                       &c->locPcls, &c->locVols, &c->locBMax, &c->locBWC, &c->locBArg,
+                       &c->locBWT, &c->locWRrel, &c->locWC, &c->locWR, &c->locWT, &c->locBase,
+                       &c->locBestR, &c->locBestT};
+    for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();
     for (auto& r : c->res) {
         r.wC.release(); r.wR.release(); r.wT.release(); r.base.release();
@@ -495,6 +508,127 @@ int thx_ctx_score_one(thx_ctx* c, int cls, int p, float* out) {
     }
     tmp.release();
     return s;
+}
+
+int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const int* nR,
+                        const double* trans, const int* nT, const double* pR, const double* pT,
+                        int precision) {
+    clear_error();
+    THX_CHECK(c && quat && nR && trans && nT && pR && pT, THX_ERR_ARG,
+              "score_local: null argument");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
+              "score_local: precision must be FP32 or BF16");
+    THX_CHECK(c->K > 0 && c->P > 0, THX_ERR_STATE, "score_local: pixels and particles must be set");
+    THX_CHECK(!c->vol.empty(), THX_ERR_STATE, "score_local: no volume set");
+    const int P = c->P;
+    const int nCls = (int)c->vol.size();
+    for (int k = 0; k < nCls; k++)
+        THX_CHECK(c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
+                  "score_local: every class volume must be set with vdim = idim * pf");
+    std::vector<int> rOff(P), tOff(P), bStart(P + 1);
+    std::vector<int2> blocks;
+    size_t Rtot = 0, Ttot = 0;
+    int maxT = 0;
+    for (int p = 0; p < P; p++) {
+        THX_CHECK(nR[p] >= 1 && nR[p] <= 64 * kLocalRotBlock, THX_ERR_RANGE,
+                  "score_local: rotations per particle must be in [1, 8192]");
+        THX_CHECK(nT[p] >= 1 && nT[p] <= 128, THX_ERR_RANGE,
+                  "score_local: translations per particle must be in [1, 128]");
+        if (cls) THX_CHECK(cls[p] >= 0 && cls[p] < nCls, THX_ERR_ARG, "score_local: bad class");
+        rOff[p] = (int)Rtot;
+        tOff[p] = (int)Ttot;
+        bStart[p] = (int)blocks.size();
+        for (int r0 = 0; r0 < nR[p]; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
+        Rtot += nR[p];
+        Ttot += nT[p];
+        maxT = std::max(maxT, nT[p]);
+    }
+    bStart[P] = (int)blocks.size();
+    THX_CHECK(Rtot < (1u << 31) && Ttot < (1u << 31), THX_ERR_RANGE, "score_local: batch too large");
+    const int TPAD = maxT <= 32 ? 32 : (maxT <= 64 ? 64 : 128);
+    const int nblk = (int)blocks.size();
+    std::vector<float> fR(Rtot), fT(Ttot);
+    for (size_t i = 0; i < Rtot; i++) fR[i] = (float)pR[i];
+    for (size_t i = 0; i < Ttot; i++) fT[i] = (float)pT[i];
+    std::vector<const float2*> vp(nCls);
+    for (int k = 0; k < nCls; k++) vp[k] = c->vol[k].as<const float2>();
+    THX_TRY(set_device(c));
+    hipStream_t st = c->stream;
+    THX_TRY(upload(c->locQuat, quat, 4 * Rtot, st));
+    THX_TRY(upload(c->locTrans, trans, 2 * Ttot, st));
+    THX_TRY(upload(c->locPRf, fR.data(), Rtot, st));
+    THX_TRY(upload(c->locPTf, fT.data(), Ttot, st));
+    THX_TRY(upload(c->locROff, rOff.data(), P, st));
+    THX_TRY(upload(c->locNR, nR, P, st));
+    THX_TRY(upload(c->locTOff, tOff.data(), P, st));
+    THX_TRY(upload(c->locNT, nT, P, st));
+    THX_TRY(upload(c->locBlocks, blocks.data(), blocks.size(), st));
+    THX_TRY(upload(c->locBStart, bStart.data(), P + 1, st));
+    THX_TRY(upload(c->locVols, vp.data(), vp.size(), st));
+    if (cls) THX_TRY(upload(c->locPcls, cls, P, st));
+    THX_TRY(c->locBMax.ensure(sizeof(float) * nblk));
+    THX_TRY(c->locBWC.ensure(sizeof(float) * nblk));
+    THX_TRY(c->locBArg.ensure(sizeof(int) * nblk));
+    THX_TRY(c->locBWT.ensure(sizeof(float) * (size_t)nblk * TPAD));
+    THX_TRY(c->locWRrel.ensure(sizeof(float) * Rtot));
+    THX_TRY(c->locWC.ensure(sizeof(double) * P));
+    THX_TRY(c->locWR.ensure(sizeof(double) * Rtot));
+    THX_TRY(c->locWT.ensure(sizeof(double) * Ttot));
+    THX_TRY(c->locBase.ensure(sizeof(double) * P));
+    THX_TRY(c->locBestR.ensure(sizeof(int) * P));
+    THX_TRY(c->locBestT.ensure(sizeof(int) * P));
+    c->locP = P;
+    c->locTPAD = TPAD;
+    c->locNblk = nblk;
+    c->locRtot = Rtot;
+    c->locTtot = Ttot;
+    LocalBatch b;
+    b.vols = c->locVols.as<const float2* const>();
+    b.pcls = cls ? c->locPcls.as<const int>() : nullptr;
+    b.quat = c->locQuat.as<const double>();
+    b.pRf = c->locPRf.as<const float>();
+    b.rOff = c->locROff.as<const int>();
+    b.nR = c->locNR.as<const int>();
+    b.trans = c->locTrans.as<const double>();
+    b.pTf = c->locPTf.as<const float>();
+    b.tOff = c->locTOff.as<const int>();
+    b.nT = c->locNT.as<const int>();
+    b.blocks = c->locBlocks.as<const int2>();
+    b.bStart = c->locBStart.as<const int>();
+    b.bMax = c->locBMax.as<float>();
+    b.bWC = c->locBWC.as<float>();
+    b.bArg = c->locBArg.as<int>();
+    b.bWT = c->locBWT.as<float>();
+    b.wRrel = c->locWRrel.as<float>();
+    c->nev = 0;
+    c->evNames.clear();
+    THX_TRY(record(c, "start"));
+    THX_TRY(launch_prep_particles(c, THX_FP32));
+    THX_TRY(record(c, "prep"));
+    THX_TRY(launch_score_local(c, precision, TPAD, nblk, b));
+    THX_TRY(record(c, "score_local"));
+    THX_TRY(launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
+                               c->locWT.as<double>(), c->locBase.as<double>(),
+                               c->locBestR.as<int>(), c->locBestT.as<int>()));
+    THX_TRY(record(c, "merge_local"));
+    c->locValid = true;
+    return THX_OK;
+}
+
+int thx_ctx_get_local(thx_ctx* c, double* wC, double* wR, double* wT, double* baseL,
+                      int* bestR, int* bestT) {
+    clear_error();
+    THX_CHECK(c && c->locValid, THX_ERR_STATE, "get_local: no local scan done");
+    THX_TRY(set_device(c));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const int P = c->locP;
+    if (wC) THX_HIP(hipMemcpy(wC, c->locWC.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    if (wR) THX_HIP(hipMemcpy(wR, c->locWR.p, sizeof(double) * c->locRtot, hipMemcpyDeviceToHost));
+    if (wT) THX_HIP(hipMemcpy(wT, c->locWT.p, sizeof(double) * c->locTtot, hipMemcpyDeviceToHost));
+    if (baseL) THX_HIP(hipMemcpy(baseL, c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    if (bestR) THX_HIP(hipMemcpy(bestR, c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    if (bestT) THX_HIP(hipMemcpy(bestT, c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    return THX_OK;
 }
 
 // ---------------------------------------------------------------------------

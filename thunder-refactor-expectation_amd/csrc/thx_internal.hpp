@@ -63,6 +63,27 @@ struct ClassResult {
     bool valid = false;
 };
 
+// Device pointers of one ragged local-search batch.
+struct LocalBatch {
+    const float2* const* vols = nullptr;   // per-class volume base pointers
+    const int* pcls = nullptr;             // class per particle (or null: class 0)
+    const double* quat = nullptr;          // [Rtot][4]
+    const float* pRf = nullptr;            // [Rtot]
+    const int* rOff = nullptr;             // [P]
+    const int* nR = nullptr;               // [P]
+    const double* trans = nullptr;         // [Ttot][2]
+    const float* pTf = nullptr;            // [Ttot]
+    const int* tOff = nullptr;             // [P]
+    const int* nT = nullptr;               // [P]
+    const int2* blocks = nullptr;          // [nblk] (particle, first rotation)
+    const int* bStart = nullptr;           // [P + 1]
+    float* bMax = nullptr;                 // [nblk]
+    float* bWC = nullptr;                  // [nblk]
+    int* bArg = nullptr;                   // [nblk]
+    float* bWT = nullptr;                  // [nblk][TPAD]
+    float* wRrel = nullptr;                // [Rtot]
+};
+
 struct Timing {
     std::vector<std::string> names;
     std::vector<double> ms;
@@ -101,6 +122,13 @@ struct thx_ctx {
     thx::DevBuf shellMax;   // float [2][maxShell+1] (scratch of the scale pass)
     thx::DevBuf partMax, partWC, partArg, partWR, partWT;
     std::vector<thx::ClassResult> res;
+    // ragged local-search batch (inputs, partials, results)
+    int locP = 0, locTPAD = 0, locNblk = 0;
+    size_t locRtot = 0, locTtot = 0;
+    thx::DevBuf locQuat, locTrans, locPRf, locPTf, locROff, locNR, locTOff, locNT, locBlocks,
+        locBStart, locPcls, locVols, locBMax, locBWC, locBArg, locBWT, locWRrel;
+    thx::DevBuf locWC, locWR, locWT, locBase, locBestR, locBestT;
+    bool locValid = false;
     // timing
     hipEvent_t ev[16] = {};
     int nev = 0;
@@ -129,6 +157,10 @@ int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams,
                         double ampContrast, double pixelSize);
 int launch_synth_noise(thx_ctx* c, int P, double noiseSigma, uint64_t seed);
 int launch_score_one(thx_ctx* c, int cls, int p, float* dOut);
+constexpr int kLocalRotBlock = 128;   // rotations per local-search block
+int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);
+int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
+                       double* wT, double* base, int* bestR, int* bestT);
 
 // Philox4x32-10 (Salmon et al., SC'11), shared by host and device code.
 struct Philox {

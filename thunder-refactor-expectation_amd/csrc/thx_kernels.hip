@@ -1126,6 +1126,380 @@ int launch_merge(thx_ctx* c, int cls) {
 }
 
 // ---------------------------------------------------------------------------
+// Ragged local-search scorer (Optimiser::updateSP, src/Optimiser.cpp:
+// 1139-1437; kernel_Project3DL + kernel_logDataVSL + kernel_UpdateWL,
+// gpu/src/Kernel.cu:625-653, 827-881, 1349-1451), batched over particles.
+// Each particle has its own rotations (R_p) and translations (T_p <= 128).
+// A block owns one particle x up to 128 of its rotations x all its
+// translations.  Per K step of 32 pixels:
+//   B tile: the block's slices projected on the fly from the volume (lanes =
+//           consecutive pixels of one rotation, so the 8-corner gathers of a
+//           wave stay within a few cache lines);
+//   A tile: Z[t][k] = Y_pk exp(+i phase_tk) = Y_pk conj(S_tk);
+// then the MFMA contraction TPAD x 128 over 64 reals.  |P|^2 weighted by
+// sigRcp ctf^2 accumulates in registers for the norm term.  The epilogue
+// writes per-block maximum, argmax, wC, wT partials and the per-rotation wR
+// (relative to the block maximum); k_merge_local combines them in FP64.
+// ---------------------------------------------------------------------------
+constexpr int kNBL = 128;
+
+template <int PREC, int TPAD>
+__global__ __launch_bounds__(256, 2) void k_score_local(
+    const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int Kp, int N,
+    const float2* __restrict__ Y, const float* __restrict__ Wt,
+    const double* __restrict__ Aterm, const double* __restrict__ quat,
+    const float* __restrict__ pRf, const int* __restrict__ rOff, const int* __restrict__ nR,
+    const double* __restrict__ trans, const float* __restrict__ pTf,
+    const int* __restrict__ tOff, const int* __restrict__ nT, const int2* __restrict__ blocks,
+    float* __restrict__ bMax, float* __restrict__ bWC, int* __restrict__ bArg,
+    float* __restrict__ bWT, float* __restrict__ wRrel) {
+    constexpr bool F32 = (PREC == THX_FP32);
+    constexpr int LD = F32 ? 65 : 72;
+    using E = typename std::conditional<F32, float, unsigned short>::type;
+    constexpr int MT = TPAD / 32;                   // 32-row MFMA tiles
+    __shared__ __attribute__((aligned(16))) E As[TPAD * LD];
+    __shared__ __attribute__((aligned(16))) E Bs[kNBL * LD];
+    __shared__ double rotM[kNBL][6];
+    __shared__ float trs[TPAD][2];
+    __shared__ float Cs[kNBL];
+    __shared__ float redMax[4];
+    __shared__ int redArg[4];
+    __shared__ float rowPart[4][TPAD];
+
+    const int2 bi = blocks[blockIdx.x];
+    const int p = bi.x, r0 = bi.y;
+    const int Rp = nR[p], Tp = nT[p];
+    const int nr = min(kNBL, Rp - r0);
+    const int rbase = rOff[p] + r0, tbase = tOff[p];
+    const float2* vol = vols[pcls ? pcls[p] : 0];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hl = lane >> 5;
+
+    if (tid < kNBL) {
+        double m[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+        if (tid < nr) quat_cols01(quat + 4 * (size_t)(rbase + tid), m);
+        rotM[tid][0] = m[0][0]; rotM[tid][1] = m[0][1];
+        rotM[tid][2] = m[1][0]; rotM[tid][3] = m[1][1];
+        rotM[tid][4] = m[2][0]; rotM[tid][5] = m[2][1];
+    }
+    if (tid < TPAD) {
+        float tx = 0.f, ty = 0.f;
+        if (tid < Tp) {
+            tx = (float)trans[2 * (size_t)(tbase + tid)] / N;
+            ty = (float)trans[2 * (size_t)(tbase + tid) + 1] / N;
+        }
+        trs[tid][0] = tx;
+        trs[tid][1] = ty;
+    }
+    __syncthreads();
+
+    floatx16 acc[MT];
+#pragma unroll
+    for (int a = 0; a < MT; a++) acc[a] = floatx16{0};
+    float cacc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) cacc[q] = 0.f;
+    const float2* yrow = Y + (size_t)p * Kp;
+    const float* wrow = Wt + (size_t)p * Kp;
+    const int grp = tid >> 5;            // 8 groups of 32 lanes
+
+    for (int k0 = 0; k0 < Kp; k0 += kChunk) {
+        const int k = k0 + l32;
+        const int ic = iCol[k], ir = iRow[k];
+        // ---- A: Z = Y conj(S) ----
+        {
+            const float2 y = yrow[k];
+#pragma unroll
+            for (int j = 0; j < TPAD / 8; j++) {
+                const int t = grp + 8 * j;
+                const float ph = (float)(2 * M_PI) * (ic * trs[t][0] + ir * trs[t][1]);
+                float sn, cs;
+                sincosf(ph, &sn, &cs);
+                const float zr = y.x * cs - y.y * sn, zi = y.x * sn + y.y * cs;
+                if constexpr (F32) {
+                    As[t * LD + 2 * l32] = zr;
+                    As[t * LD + 2 * l32 + 1] = zi;
+                } else {
+                    *reinterpret_cast<uint32_t*>(As + t * LD + 2 * l32) = pack16<PREC>(zr, zi);
+                }
+            }
+        }
+        // ---- B: slices of the block's rotations at pixel k ----
+        {
+            const float wk = wrow[k];
+            const bool kok = k < K;
+            const double c0 = (double)(ic * pf), c1 = (double)(ir * pf);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int rr = grp + 8 * q;
+                float2 v = make_float2(0.f, 0.f);
+                if (kok && rr < nr) {
+                    const double* m = rotM[rr];
+                    const float x = (float)(m[0] * c0 + m[1] * c1);
+                    const float yy = (float)(m[2] * c0 + m[3] * c1);
+                    const float z = (float)(m[4] * c0 + m[5] * c1);
+                    v = sample_trilinear(vol, V, x, yy, z);
+                }
+                cacc[q] += wk * (v.x * v.x + v.y * v.y);
+                if constexpr (F32) {
+                    Bs[rr * LD + 2 * l32] = v.x;
+                    Bs[rr * LD + 2 * l32 + 1] = v.y;
+                } else {
+                    *reinterpret_cast<uint32_t*>(Bs + rr * LD + 2 * l32) = pack16<PREC>(v.x, v.y);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- MFMA: wave owns columns 32 wave .. +32 ----
+        const int nrow = 32 * wave + l32;
+        if constexpr (F32) {
+#pragma unroll 4
+            for (int kk = 0; kk < 64; kk += 2) {
+                const float b = Bs[nrow * LD + kk + hl];
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++) {
+                    const float a = As[(32 * mt + l32) * LD + kk + hl];
+                    acc[mt] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[mt], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < 64; kk += 16) {
+                const int kc = kk + 8 * hl;
+                const uint4 rb4 = *reinterpret_cast<const uint4*>(Bs + nrow * LD + kc);
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++) {
+                    const uint4 ra = *reinterpret_cast<const uint4*>(As + (32 * mt + l32) * LD + kc);
+                    if constexpr (PREC == THX_BF16)
+                        acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, ra), __builtin_bit_cast(bf16x8, rb4), acc[mt], 0, 0, 0);
+                    else
+                        acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                            __builtin_bit_cast(halfx8, ra), __builtin_bit_cast(halfx8, rb4), acc[mt], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- norm term per rotation: sum of cacc over the 32 pixel lanes ----
+#pragma unroll
+    for (int q = 0; q < 16; q++) cacc[q] += __shfl_xor(cacc[q], 16);
+#pragma unroll
+    for (int s = 8; s >= 1; s >>= 1) {
+        const bool up = (l32 & s) != 0;
+#pragma unroll
+        for (int j = 0; j < s; j++) {
+            const float send = up ? cacc[j] : cacc[j + s];
+            const float keep = up ? cacc[j + s] : cacc[j];
+            cacc[j] = keep + __shfl_xor(send, s);
+        }
+    }
+    if (l32 < 16) Cs[grp + 8 * (l32 & 15)] = cacc[0];
+    __syncthreads();
+
+    // ---- dvp, block maximum ----
+    const float a = (float)Aterm[p];
+    const int ncol = 32 * wave + l32;
+    const bool cok = ncol < nr;
+    const float cterm = Cs[ncol];
+    float lmax = -INFINITY;
+    int larg = -1;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int t = 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+            float v = -INFINITY;
+            if (cok && t < Tp) v = a + cterm - 2.f * acc[mt][i];
+            acc[mt][i] = v;
+            if (v > lmax) { lmax = v; larg = ncol * TPAD + t; }
+        }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(lmax, off);
+        const int oa = __shfl_xor(larg, off);
+        if (o > lmax || (o == lmax && oa >= 0 && (larg < 0 || oa < larg))) { lmax = o; larg = oa; }
+    }
+    if (lane == 0) { redMax[wave] = lmax; redArg[wave] = larg; }
+    __syncthreads();
+    float M = redMax[0];
+    int arg = redArg[0];
+    for (int w = 1; w < 4; w++)
+        if (redMax[w] > M || (redMax[w] == M && redArg[w] >= 0 && (arg < 0 || redArg[w] < arg))) {
+            M = redMax[w];
+            arg = redArg[w];
+        }
+    const bool live = M > -INFINITY;
+
+    // ---- weights, wR (column sums x pT), wT (row sums x pR) ----
+    constexpr int NV = MT * 16 < 32 ? 32 : MT * 16;   // row values per lane (padded)
+    float rowv[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) rowv[j] = 0.f;
+    const float pr = cok ? pRf[rbase + ncol] : 0.f;
+    float cs = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int t = 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+            const float w = live ? __expf(acc[mt][i] - M) : 0.f;
+            cs += w * (t < Tp ? pTf[tbase + t] : 0.f);
+            rowv[mt * 16 + i] = w * pr;
+        }
+    cs += __shfl_xor(cs, 32);
+    if (hl == 0 && cok) wRrel[rbase + ncol] = cs;
+    // butterfly: groups of 32 values; lane l32 ends with value index g*32 + l32 of each group
+    constexpr int NG = NV / 32;
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+#pragma unroll
+        for (int s = 16; s >= 1; s >>= 1) {
+            const bool up = (l32 & s) != 0;
+#pragma unroll
+            for (int j = 0; j < s; j++) {
+                const float send = up ? rowv[g * 32 + j] : rowv[g * 32 + j + s];
+                const float keep = up ? rowv[g * 32 + j + s] : rowv[g * 32 + j];
+                rowv[g * 32 + j] = keep + __shfl_xor(send, s);
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int jv = g * 32 + l32;          // value index = mt * 16 + i
+        const int mt = jv >> 4, i = jv & 15;
+        if (mt < MT) {
+            const int t = 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+            rowPart[wave][t] = rowv[g * 32];
+        }
+    }
+    __syncthreads();
+    __shared__ float rowW[TPAD];
+    if (tid < TPAD) {
+        const float rsum = rowPart[0][tid] + rowPart[1][tid] + rowPart[2][tid] + rowPart[3][tid];
+        bWT[(size_t)blockIdx.x * TPAD + tid] = rsum;
+        rowW[tid] = tid < Tp ? rsum * pTf[tbase + tid] : 0.f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float wc = 0.f;
+        for (int t = 0; t < TPAD; t++) wc += rowW[t];
+        bMax[blockIdx.x] = M;
+        bWC[blockIdx.x] = wc;
+        bArg[blockIdx.x] = arg < 0 ? -1 : ((r0 + arg / TPAD) * kNBL + arg % TPAD);
+    }
+}
+
+template <int PREC>
+static void launch_local_prec(hipStream_t st, int TPAD, int nblk, const float2* const* vols,
+                              const int* pcls, int V, int pf, const int* iCol, const int* iRow,
+                              int K, int Kp, int N, const float2* Y, const float* Wt,
+                              const double* Aterm, const double* quat, const float* pRf,
+                              const int* rOff, const int* nR, const double* trans,
+                              const float* pTf, const int* tOff, const int* nT,
+                              const int2* blocks, float* bMax, float* bWC, int* bArg, float* bWT,
+                              float* wRrel) {
+#define THX_LOCAL_ARGS vols, pcls, V, pf, iCol, iRow, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, \
+                       nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel
+    if (TPAD == 32)
+        hipLaunchKernelGGL((k_score_local<PREC, 32>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+    else if (TPAD == 64)
+        hipLaunchKernelGGL((k_score_local<PREC, 64>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+    else
+        hipLaunchKernelGGL((k_score_local<PREC, 128>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+#undef THX_LOCAL_ARGS
+}
+
+int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b) {
+    if (precision == THX_FP32)
+        launch_local_prec<THX_FP32>(c->stream, TPAD, nblk, b.vols, b.pcls, c->vdim[0], c->pf,
+                                    c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->Kp,
+                                    c->idim, c->Y.as<const float2>(), c->Wt.as<const float>(),
+                                    c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR,
+                                    b.trans, b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg,
+                                    b.bWT, b.wRrel);
+    else if (precision == THX_BF16)
+        launch_local_prec<THX_BF16>(c->stream, TPAD, nblk, b.vols, b.pcls, c->vdim[0], c->pf,
+                                    c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->Kp,
+                                    c->idim, c->Y.as<const float2>(), c->Wt.as<const float>(),
+                                    c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR,
+                                    b.trans, b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg,
+                                    b.bWT, b.wRrel);
+    else {
+        set_error("local scorer supports FP32 and BF16");
+        return THX_ERR_ARG;
+    }
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// FP64 merge of a particle's local blocks (one block per particle).
+__global__ __launch_bounds__(256) void k_merge_local(
+    const float* __restrict__ bMax, const float* __restrict__ bWC, const int* __restrict__ bArg,
+    const float* __restrict__ bWT, const float* __restrict__ wRrel, const int* __restrict__ bStart,
+    const int* __restrict__ rOff, const int* __restrict__ nR, const int* __restrict__ tOff,
+    const int* __restrict__ nT, int TPAD, double* __restrict__ wC, double* __restrict__ wR,
+    double* __restrict__ wT, double* __restrict__ base, int* __restrict__ bestR,
+    int* __restrict__ bestT) {
+    const int p = blockIdx.x;
+    const int b0 = bStart[p], b1 = bStart[p + 1], nb = b1 - b0;
+    __shared__ double f[64];
+    __shared__ double red[256];
+    __shared__ int redi[256];
+    double mx = -INFINITY;
+    int ai = -1;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x)
+        if ((double)bMax[b0 + i] > mx) { mx = bMax[b0 + i]; ai = i; }
+    red[threadIdx.x] = mx;
+    redi[threadIdx.x] = ai;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double o = red[threadIdx.x + s];
+            const int oi = redi[threadIdx.x + s];
+            if (o > red[threadIdx.x] || (o == red[threadIdx.x] && oi >= 0 &&
+                                         (redi[threadIdx.x] < 0 || oi < redi[threadIdx.x]))) {
+                red[threadIdx.x] = o;
+                redi[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const double M = red[0];
+    const int bb = redi[0];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) f[i] = exp((double)bMax[b0 + i] - M);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double wc = 0.0;
+        for (int i = 0; i < nb; i++) wc += f[i] * bWC[b0 + i];
+        wC[p] = wc;
+        base[p] = M;
+        const int a = bb >= 0 ? bArg[b0 + bb] : -1;
+        bestR[p] = a < 0 ? -1 : a / kNBL;
+        bestT[p] = a < 0 ? -1 : a % kNBL;
+    }
+    const int Rp = nR[p], Tp = nT[p];
+    for (int r = threadIdx.x; r < Rp; r += blockDim.x)
+        wR[rOff[p] + r] = f[r / kNBL] * (double)wRrel[rOff[p] + r];
+    for (int t = threadIdx.x; t < Tp; t += blockDim.x) {
+        double s = 0.0;
+        for (int i = 0; i < nb; i++) s += f[i] * bWT[(size_t)(b0 + i) * TPAD + t];
+        wT[tOff[p] + t] = s;
+    }
+}
+
+int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
+                       double* wT, double* base, int* bestR, int* bestT) {
+    hipLaunchKernelGGL(k_merge_local, dim3(P), dim3(256), 0, c->stream, b.bMax, b.bWC, b.bArg,
+                       b.bWT, b.wRrel, b.bStart, b.rOff, b.nR, b.tOff, b.nT, TPAD, wC, wR, wT,
+                       base, bestR, bestT);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Direct-form scores of one particle (checking aid): one block per (r, t),
 // dvp = sum_k sig |X - ctf S P|^2 in FP32 (the kernel_logDataVS form).
 // ---------------------------------------------------------------------------

@@ -43,6 +43,8 @@ _SIGS = {
     "thx_ctx_get_best": [_p, _i, _p, _p, _p],
     "thx_ctx_score_one": [_p, _i, _i, _p],
     "thx_ctx_sync": [_p],
+    "thx_ctx_score_local": [_p, _p, _p, _p, _p, _p, _p, _p, _i],
+    "thx_ctx_get_local": [_p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
     "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
@@ -237,6 +239,29 @@ class Context:
         out = np.zeros((self.R, self.T), np.float32)
         check(load().thx_ctx_score_one(self._h, cls, p, ptr(out)), "score_one")
         return out
+
+    def score_local(self, quat, nR, trans, nT, pR=None, pT=None, cls=None, precision=FP32):
+        quat = c32(quat, np.float64).reshape(-1, 4)
+        trans = c32(trans, np.float64).reshape(-1, 2)
+        nR, nT = c32(nR, np.int32), c32(nT, np.int32)
+        if pR is None:
+            pR = np.concatenate([np.full(n, 1.0 / n) for n in nR])
+        if pT is None:
+            pT = np.concatenate([np.full(n, 1.0 / n) for n in nT])
+        pR, pT = c32(pR, np.float64), c32(pT, np.float64)
+        cls = None if cls is None else c32(cls, np.int32)
+        check(load().thx_ctx_score_local(self._h, ptr(cls), ptr(quat), ptr(nR), ptr(trans),
+                                         ptr(nT), ptr(pR), ptr(pT), precision), "score_local")
+        self._loc = (int(nR.sum()), int(nT.sum()))
+
+    def get_local(self):
+        Rt, Tt = self._loc
+        wC, base = np.zeros(self.P), np.zeros(self.P)
+        wR, wT = np.zeros(Rt), np.zeros(Tt)
+        bR, bT = np.zeros(self.P, np.int32), np.zeros(self.P, np.int32)
+        check(load().thx_ctx_get_local(self._h, ptr(wC), ptr(wR), ptr(wT), ptr(base), ptr(bR),
+                                       ptr(bT)), "get_local")
+        return wC, wR, wT, base, bR, bT
 
     def last_timing(self):
         buf = C.create_string_buffer(1024)
