@@ -194,6 +194,48 @@ int thx_ctx_last_timing(thx_ctx* ctx, char* names, int cap, double* ms,
                         int maxn, int* n);
 
 /* ------------------------------------------------------------------------ */
+/* Particle filter (host buffers, ragged per-particle counts; FP64).         */
+/* ------------------------------------------------------------------------ */
+
+/* Systematic resampling for P particles (Particle::resample(n, pt),
+ * src/Particle.cpp:1275-1462, one sample type): particle p has nIn[p] <= 8192
+ * samples with prior w and likelihood u (concatenated); W = w (.) u
+ * normalised, FP64 cumulative sum, nOut[p] systematic draws u0 + j / nOut
+ * with u0 uniform in [0, 1/nOut) (u0 != NULL: given per particle), new
+ * weight 1 / u (PARTICLE_PRIOR_ONE) normalised.  idx (within the particle),
+ * wNew: concatenated over nOut.  ess[p] = 1 / sum W^2 (optional; the
+ * commented-out Particle::neff, src/Particle.cpp:1647-1651).  The random
+ * shuffle before resampling (src/Particle.cpp:2173-2271) is not applied. */
+int thx_pf_resample(thx_ctx* ctx, const double* w, const double* u,
+                    const int* nIn, const int* nOut, int P, uint64_t seed,
+                    const double* u0, int* idx, double* wNew, double* ess);
+
+/* Weighted mean rotation per particle: principal eigenvector of the scatter
+ * sum w q q^T (the ACG mean inferACG estimates, src/Geometry/
+ * DirectionalStat.cpp:87-244; sign chosen with w >= 0) and the eigenvalue
+ * ratios k[3] (Particle::calVari, src/Particle.cpp:1002-1126).  w may be NULL. */
+int thx_pf_rot_mean(thx_ctx* ctx, const double* quat, const double* w,
+                    const int* cnt, int P, double* mean, double* k);
+
+/* Perturbation (Particle::perturb, src/Particle.cpp:1133-1273) in place:
+ * rotations q <- mean d conj(mean) q with d ~ ACG(1, pf^2 k1, pf^2 k2,
+ * pf^2 k3) (or the given d[4 * sum nR] when d != NULL); translations
+ * t += pf (x, y), (x, y) bivariate Gaussian (s0, s1, rho) per particle.
+ * Either part may be skipped by passing NULL quat or NULL trans. */
+int thx_pf_perturb(thx_ctx* ctx, double* quat, const int* nR, double* trans,
+                   const int* nT, int P, const double* mean, const double* k,
+                   const double* s01rho, double pf, uint64_t seed, int phase,
+                   const double* d);
+
+/* Samples of the learned proposal: per particle, mix holds C components
+ * (m0 m1 m2 m3 k pi) and then (mu_x mu_y sigma) of the shift Gaussian.
+ * Writes nRot quaternions quat[P*nRot*4] drawn from the ACG mixture, their
+ * mixture density qdens[P*nRot] and nTr shifts trans[P*nTr*2]. */
+int thx_proposal_sample(thx_ctx* ctx, const double* mix, int C, int P,
+                        int nRot, int nTr, uint64_t seed, double* quat,
+                        double* qdens, double* trans);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic workloads (seeded; Philox4x32-10 counter streams).              */
 /* ------------------------------------------------------------------------ */
 

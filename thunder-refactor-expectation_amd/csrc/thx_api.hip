@@ -632,6 +632,159 @@ int thx_ctx_get_local(thx_ctx* c, double* wC, double* wR, double* wT, double* ba
 }
 
 // ---------------------------------------------------------------------------
+// Particle-filter entry points (host buffers; ragged counts per particle)
+// ---------------------------------------------------------------------------
+
+static std::vector<int> offsets(const int* cnt, int P, size_t& total) {
+    std::vector<int> o(P);
+    total = 0;
+    for (int p = 0; p < P; p++) {
+        o[p] = (int)total;
+        total += cnt[p];
+    }
+    return o;
+}
+
+int thx_pf_resample(thx_ctx* c, const double* w, const double* u, const int* nIn,
+                    const int* nOut, int P, uint64_t seed, const double* u0, int* idx,
+                    double* wNew, double* ess) {
+    clear_error();
+    THX_CHECK(c && w && u && nIn && nOut && idx && wNew && P > 0, THX_ERR_ARG,
+              "pf_resample: bad argument");
+    for (int p = 0; p < P; p++)
+        THX_CHECK(nIn[p] >= 1 && nIn[p] <= kResampleMaxN && nOut[p] >= 1, THX_ERR_RANGE,
+                  "pf_resample: counts must be in [1, 8192]");
+    THX_TRY(set_device(c));
+    size_t tin = 0, tout = 0;
+    std::vector<int> oin = offsets(nIn, P, tin), oout = offsets(nOut, P, tout);
+    DevBuf dw, du, doi, dni, doo, dno, du0, didx, dwn, dess;
+    int s = THX_OK;
+    hipStream_t st = c->stream;
+    if ((s = upload(dw, w, tin, st)) || (s = upload(du, u, tin, st)) ||
+        (s = upload(doi, oin.data(), P, st)) || (s = upload(dni, nIn, P, st)) ||
+        (s = upload(doo, oout.data(), P, st)) || (s = upload(dno, nOut, P, st)) ||
+        (u0 && (s = upload(du0, u0, P, st))) || (s = didx.ensure(sizeof(int) * tout)) ||
+        (s = dwn.ensure(sizeof(double) * tout)) || (s = dess.ensure(sizeof(double) * P)))
+        return s;
+    s = pf_resample(st, dw.as<double>(), du.as<double>(), doi.as<int>(), dni.as<int>(),
+                    doo.as<int>(), dno.as<int>(), P, seed, 0, u0 ? du0.as<double>() : nullptr,
+                    didx.as<int>(), dwn.as<double>(), dess.as<double>());
+    if (s == THX_OK) {
+        THX_HIP(hipStreamSynchronize(st));
+        THX_HIP(hipMemcpy(idx, didx.p, sizeof(int) * tout, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(wNew, dwn.p, sizeof(double) * tout, hipMemcpyDeviceToHost));
+        if (ess) THX_HIP(hipMemcpy(ess, dess.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    }
+    DevBuf* all[] = {&dw, &du, &doi, &dni, &doo, &dno, &du0, &didx, &dwn, &dess};
+    for (DevBuf* b : all) b->release();
+    return s;
+}
+
+int thx_pf_rot_mean(thx_ctx* c, const double* quat, const double* w, const int* cnt, int P,
+                    double* mean, double* k) {
+    clear_error();
+    THX_CHECK(c && quat && cnt && mean && k && P > 0, THX_ERR_ARG, "pf_rot_mean: bad argument");
+    THX_TRY(set_device(c));
+    size_t tot = 0;
+    std::vector<int> off = offsets(cnt, P, tot);
+    DevBuf dq, dw, doff, dcnt, dm, dk;
+    hipStream_t st = c->stream;
+    int s = THX_OK;
+    if ((s = upload(dq, quat, 4 * tot, st)) || (w && (s = upload(dw, w, tot, st))) ||
+        (s = upload(doff, off.data(), P, st)) || (s = upload(dcnt, cnt, P, st)) ||
+        (s = dm.ensure(sizeof(double) * 4 * P)) || (s = dk.ensure(sizeof(double) * 3 * P)))
+        return s;
+    s = pf_rot_mean(st, dq.as<double>(), w ? dw.as<double>() : nullptr, doff.as<int>(),
+                    dcnt.as<int>(), P, dm.as<double>(), dk.as<double>());
+    if (s == THX_OK) {
+        THX_HIP(hipStreamSynchronize(st));
+        THX_HIP(hipMemcpy(mean, dm.p, sizeof(double) * 4 * P, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(k, dk.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost));
+    }
+    DevBuf* all[] = {&dq, &dw, &doff, &dcnt, &dm, &dk};
+    for (DevBuf* b : all) b->release();
+    return s;
+}
+
+int thx_pf_perturb(thx_ctx* c, double* quat, const int* nR, double* trans, const int* nT, int P,
+                   const double* mean, const double* k, const double* s01rho, double pf,
+                   uint64_t seed, int phase, const double* d) {
+    clear_error();
+    THX_CHECK(c && P > 0 && ((quat && nR && mean && k) || (trans && nT && s01rho)), THX_ERR_ARG,
+              "pf_perturb: bad argument");
+    THX_TRY(set_device(c));
+    hipStream_t st = c->stream;
+    int s = THX_OK;
+    if (quat) {
+        size_t tot = 0;
+        std::vector<int> off = offsets(nR, P, tot);
+        int mx = 0;
+        for (int p = 0; p < P; p++) mx = std::max(mx, nR[p]);
+        DevBuf dq, doff, dcnt, dm, dk, dd;
+        if ((s = upload(dq, quat, 4 * tot, st)) || (s = upload(doff, off.data(), P, st)) ||
+            (s = upload(dcnt, nR, P, st)) || (s = upload(dm, mean, 4 * (size_t)P, st)) ||
+            (s = upload(dk, k, 3 * (size_t)P, st)) || (d && (s = upload(dd, d, 4 * tot, st))))
+            return s;
+        s = pf_perturb_rot(st, dq.as<double>(), doff.as<int>(), dcnt.as<int>(), mx,
+                           dm.as<double>(), dk.as<double>(), pf, P, seed, (uint32_t)phase,
+                           d ? dd.as<double>() : nullptr);
+        if (s == THX_OK) {
+            THX_HIP(hipStreamSynchronize(st));
+            THX_HIP(hipMemcpy(quat, dq.p, sizeof(double) * 4 * tot, hipMemcpyDeviceToHost));
+        }
+        DevBuf* all[] = {&dq, &doff, &dcnt, &dm, &dk, &dd};
+        for (DevBuf* b : all) b->release();
+        if (s != THX_OK) return s;
+    }
+    if (trans) {
+        size_t tot = 0;
+        std::vector<int> off = offsets(nT, P, tot);
+        int mx = 0;
+        for (int p = 0; p < P; p++) mx = std::max(mx, nT[p]);
+        DevBuf dt, doff, dcnt, ds;
+        if ((s = upload(dt, trans, 2 * tot, st)) || (s = upload(doff, off.data(), P, st)) ||
+            (s = upload(dcnt, nT, P, st)) || (s = upload(ds, s01rho, 3 * (size_t)P, st)))
+            return s;
+        s = pf_perturb_trans(st, dt.as<double>(), doff.as<int>(), dcnt.as<int>(), mx,
+                             ds.as<double>(), pf, P, seed, (uint32_t)phase);
+        if (s == THX_OK) {
+            THX_HIP(hipStreamSynchronize(st));
+            THX_HIP(hipMemcpy(trans, dt.p, sizeof(double) * 2 * tot, hipMemcpyDeviceToHost));
+        }
+        DevBuf* all[] = {&dt, &doff, &dcnt, &ds};
+        for (DevBuf* b : all) b->release();
+    }
+    return s;
+}
+
+int thx_proposal_sample(thx_ctx* c, const double* mix, int C, int P, int nRot, int nTr,
+                        uint64_t seed, double* quat, double* qdens, double* trans) {
+    clear_error();
+    THX_CHECK(c && mix && quat && qdens && trans && C > 0 && P > 0 && nRot > 0 && nTr > 0,
+              THX_ERR_ARG, "proposal_sample: bad argument");
+    THX_TRY(set_device(c));
+    hipStream_t st = c->stream;
+    DevBuf dm, dq, dd, dt;
+    int s = THX_OK;
+    if ((s = upload(dm, mix, (size_t)P * (6 * C + 3), st)) ||
+        (s = dq.ensure(sizeof(double) * 4 * (size_t)P * nRot)) ||
+        (s = dd.ensure(sizeof(double) * (size_t)P * nRot)) ||
+        (s = dt.ensure(sizeof(double) * 2 * (size_t)P * nTr)))
+        return s;
+    s = pf_sample_mixture(st, dm.as<double>(), C, P, nRot, nTr, seed, 0, dq.as<double>(),
+                          dd.as<double>(), dt.as<double>());
+    if (s == THX_OK) {
+        THX_HIP(hipStreamSynchronize(st));
+        THX_HIP(hipMemcpy(quat, dq.p, sizeof(double) * 4 * (size_t)P * nRot, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(qdens, dd.p, sizeof(double) * (size_t)P * nRot, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(trans, dt.p, sizeof(double) * 2 * (size_t)P * nTr, hipMemcpyDeviceToHost));
+    }
+    DevBuf* all[] = {&dm, &dq, &dd, &dt};
+    for (DevBuf* b : all) b->release();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
 // Synthetic workloads
 // ---------------------------------------------------------------------------
 

@@ -162,6 +162,22 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
                        double* wT, double* base, int* bestR, int* bestT);
 
+// Particle-filter launchers (thx_pf.hip).
+int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
+                   const double* mean, const double* kk, double pf, int P, uint64_t seed,
+                   uint32_t phase, const double* dGiven);
+int pf_perturb_trans(hipStream_t st, double* trans, const int* off, const int* cnt, int maxCnt,
+                     const double* s01rho, double pf, int P, uint64_t seed, uint32_t phase);
+int pf_sample_mixture(hipStream_t st, const double* mix, int C, int P, int nRot, int nTr,
+                      uint64_t seed, uint32_t phase, double* quat, double* qdens,
+                      double* trans);
+int pf_resample(hipStream_t st, const double* w, const double* u, const int* offIn,
+                const int* nIn, const int* offOut, const int* nOut, int P, uint64_t seed,
+                uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess);
+int pf_rot_mean(hipStream_t st, const double* quat, const double* w, const int* off,
+                const int* cnt, int P, double* mean, double* kk);
+constexpr int kResampleMaxN = 8192;
+
 // Philox4x32-10 (Salmon et al., SC'11), shared by host and device code.
 struct Philox {
     __host__ __device__ static inline void mulhilo(uint32_t a, uint32_t b,

@@ -1,0 +1,368 @@
+// Particle-filter kernels of the expectation step: weight normalisation,
+// systematic resampling with FP64 cumulative sums, effective sample size,
+// angular-central-Gaussian (ACG) rotation sampling and perturbation, and
+// Gaussian translation sampling.  All batched over particles with ragged
+// per-particle sample counts (offsets / counts arrays).
+//
+// Random numbers: Philox4x32-10 counter streams keyed by (seed, particle,
+// sample, stream), so results do not depend on launch shape.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "thx_internal.hpp"
+
+namespace thx {
+
+// Hamilton product (quaternion_mul, src/Geometry/Euler.cpp:13-26).
+__device__ inline void qmul(const double a[4], const double b[4], double o[4]) {
+    const double w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    const double x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    const double y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    const double z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = w; o[1] = x; o[2] = y; o[3] = z;
+}
+
+__device__ inline void normals4(uint64_t seed, uint32_t a, uint32_t b, uint32_t stream,
+                                double g[4]) {
+    uint32_t w[4], v[4];
+    Philox::draw(seed, a, 2 * b, stream, w);
+    Philox::draw(seed, a, 2 * b + 1, stream, v);
+    const double u1 = Philox::u01(w[0], w[1]), u2 = Philox::u01(w[2], w[3]);
+    const double u3 = Philox::u01(v[0], v[1]), u4 = Philox::u01(v[2], v[3]);
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    double s1, c1, s2, c2;
+    sincospi(2.0 * u2, &s1, &c1);
+    sincospi(2.0 * u4, &s2, &c2);
+    g[0] = r1 * c1; g[1] = r1 * s1; g[2] = r2 * c2; g[3] = r2 * s2;
+}
+
+// ACG(diag(1, k1, k2, k3)) sample (sampleACG, src/Geometry/DirectionalStat.cpp:
+// 33-85): v ~ N(0, I), v = L v with L = chol(diag), normalise.
+__device__ inline void acg_identity(const double g[4], double k1, double k2, double k3,
+                                    double d[4]) {
+    d[0] = g[0];
+    d[1] = sqrt(k1) * g[1];
+    d[2] = sqrt(k2) * g[2];
+    d[3] = sqrt(k3) * g[3];
+    const double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3]);
+    for (int i = 0; i < 4; i++) d[i] /= n;
+}
+
+// ---------------------------------------------------------------------------
+// Rotation perturbation (Particle::perturb(pf, PAR_R), src/Particle.cpp:
+// 1133-1273, 3-D branch): d ~ ACG(1, pf^2 k1, pf^2 k2, pf^2 k3) and
+// q <- mean * d * conj(mean) * q.  One thread per sample.
+// ---------------------------------------------------------------------------
+__global__ void k_perturb_rot(double* __restrict__ quat, const int* __restrict__ off,
+                              const int* __restrict__ cnt, const double* __restrict__ mean,
+                              const double* __restrict__ kk, double pf, int P, uint64_t seed,
+                              uint32_t phase, const double* __restrict__ dGiven) {
+    const int p = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || j >= cnt[p]) return;
+    const size_t i = (size_t)off[p] + j;
+    double d[4];
+    if (dGiven) {
+        for (int a = 0; a < 4; a++) d[a] = dGiven[4 * i + a];
+    } else {
+        double g[4];
+        normals4(seed, (uint32_t)i, phase, kStreamPose + 16, g);
+        const double s = pf * pf;
+        acg_identity(g, s * kk[3 * p], s * kk[3 * p + 1], s * kk[3 * p + 2], d);
+    }
+    const double m[4] = {mean[4 * p], mean[4 * p + 1], mean[4 * p + 2], mean[4 * p + 3]};
+    const double mc[4] = {m[0], -m[1], -m[2], -m[3]};
+    double q[4] = {quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]};
+    double t1[4], t2[4], t3[4];
+    qmul(mc, q, t1);      // conj(mean) q
+    qmul(d, t1, t2);      // d conj(mean) q
+    qmul(m, t2, t3);      // mean d conj(mean) q
+    for (int a = 0; a < 4; a++) quat[4 * i + a] = t3[a];
+}
+
+// Translation perturbation (Particle::perturb(pf, PAR_T)): bivariate Gaussian
+// (gsl_ran_bivariate_gaussian(s0, s1, rho)) scaled by pf, added to t.
+__global__ void k_perturb_trans(double* __restrict__ trans, const int* __restrict__ off,
+                                const int* __restrict__ cnt, const double* __restrict__ s01rho,
+                                double pf, int P, uint64_t seed, uint32_t phase) {
+    const int p = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || j >= cnt[p]) return;
+    const size_t i = (size_t)off[p] + j;
+    double g[4];
+    normals4(seed, (uint32_t)i, phase, kStreamShift + 16, g);
+    const double s0 = s01rho[3 * p], s1 = s01rho[3 * p + 1], rho = s01rho[3 * p + 2];
+    const double x = s0 * g[0];
+    const double y = s1 * (rho * g[0] + sqrt(fmax(0.0, 1.0 - rho * rho)) * g[1]);
+    trans[2 * i] += x * pf;
+    trans[2 * i + 1] += y * pf;
+}
+
+// ---------------------------------------------------------------------------
+// Mixture proposal sampling: particle p has C ACG components, each a mean
+// quaternion m_c, concentration k_c in (0, 1] (ACG(diag(1, k, k, k)) rotated
+// to m_c) and weight pi_c; plus a Gaussian over shifts (mu, sigma).  Writes
+// nRot rotations and nTr translations per particle and the proposal density
+// of each rotation, q(x) = sum_c pi_c ACG_c(x) with
+// ACG_c(x) = k_c^{-3/2} ((m_c.x)^2 + (1 - (m_c.x)^2) / k_c)^{-2} / (2 pi^2)
+// (the normalised form of pdfACG, src/Geometry/DirectionalStat.cpp:13-31,
+// on S^3 with q and -q identified).
+// ---------------------------------------------------------------------------
+__device__ inline double acg_pdf(const double* m, double k, const double* x) {
+    const double c = m[0] * x[0] + m[1] * x[1] + m[2] * x[2] + m[3] * x[3];
+    const double c2 = c * c;
+    const double s = c2 + (1.0 - c2) / k;
+    return pow(k, -1.5) / (s * s) / (2.0 * M_PI * M_PI);
+}
+
+__global__ void k_sample_mixture(const double* __restrict__ mix, int C, int P, int nRot,
+                                 int nTr, uint64_t seed, uint32_t phase,
+                                 double* __restrict__ quat, double* __restrict__ qdens,
+                                 double* __restrict__ trans) {
+    // mix per particle: C x (m0 m1 m2 m3 k pi) then (mu_x mu_y sigma)
+    const int p = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const double* mp = mix + (size_t)p * (6 * C + 3);
+    if (j < nRot) {
+        uint32_t w[4];
+        Philox::draw(seed, (uint32_t)(p * nRot + j), phase, kStreamPose + 32, w);
+        const double u = Philox::u01(w[0], w[1]);
+        double acc = 0.0;
+        int c = C - 1;
+        for (int i = 0; i < C; i++) {
+            acc += mp[6 * i + 5];
+            if (u <= acc) { c = i; break; }
+        }
+        double g[4], d[4], q[4];
+        normals4(seed, (uint32_t)(p * nRot + j), phase, kStreamPose + 48, g);
+        const double k = mp[6 * c + 4];
+        acg_identity(g, k, k, k, d);
+        qmul(mp + 6 * c, d, q);
+        const size_t o = (size_t)p * nRot + j;
+        for (int a = 0; a < 4; a++) quat[4 * o + a] = q[a];
+        double dens = 0.0;
+        for (int i = 0; i < C; i++) dens += mp[6 * i + 5] * acg_pdf(mp + 6 * i, mp[6 * i + 4], q);
+        qdens[o] = dens;
+    }
+    if (j < nTr) {
+        double g[4];
+        normals4(seed, (uint32_t)(p * nTr + j), phase, kStreamShift + 32, g);
+        const double* sh = mp + 6 * C;
+        const size_t o = (size_t)p * nTr + j;
+        trans[2 * o] = sh[0] + sh[2] * g[0];
+        trans[2 * o + 1] = sh[1] + sh[2] * g[1];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Systematic resampling (Particle::resample(n, pt), src/Particle.cpp:
+// 1275-1462), one block per particle: W = w (.) u normalised, FP64 inclusive
+// cumulative sum, cdf /= cdf[last], u_j = u0 + j / nOut, idx_j = first i with
+// cdf[i] >= u_j, new weight 1 / u_idx (PARTICLE_PRIOR_ONE) normalised.
+// u0 is uniform in [0, 1/nOut) from Philox, or given (u0Given >= 0).
+// ---------------------------------------------------------------------------
+constexpr int kResampleMax = 8192;
+
+__global__ __launch_bounds__(256) void k_resample(
+    const double* __restrict__ w, const double* __restrict__ u, const int* __restrict__ offIn,
+    const int* __restrict__ nIn, const int* __restrict__ offOut, const int* __restrict__ nOut,
+    int P, uint64_t seed, uint32_t phase, const double* __restrict__ u0Given,
+    int* __restrict__ idx, double* __restrict__ wNew, double* __restrict__ essOut) {
+    const int p = blockIdx.x;
+    if (p >= P) return;
+    const int n = nIn[p], m = nOut[p];
+    const double* wp = w + offIn[p];
+    const double* up = u + offIn[p];
+    __shared__ double cdf[kResampleMax];
+    __shared__ double part[256];
+    // per-thread contiguous ranges, then a scan of the 256 range sums
+    const int per = (n + blockDim.x - 1) / blockDim.x;
+    const int b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+    double s = 0.0, s2 = 0.0;
+    for (int i = b0; i < b1; i++) {
+        const double v = wp[i] * up[i];
+        s += v;
+        cdf[i] = s;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int t = 0; t < (int)blockDim.x; t++) {
+            const double v = part[t];
+            part[t] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    const double offs = part[threadIdx.x];
+    for (int i = b0; i < b1; i++) cdf[i] += offs;
+    __syncthreads();
+    const double total = cdf[n - 1];
+    // effective sample size of the normalised weights W
+    for (int i = b0; i < b1; i++) {
+        const double v = wp[i] * up[i] / total;
+        s2 += v * v;
+    }
+    part[threadIdx.x] = s2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && essOut) essOut[p] = 1.0 / part[0];
+    double u0;
+    if (u0Given) {
+        u0 = u0Given[p];
+    } else {
+        uint32_t r[4];
+        Philox::draw(seed, (uint32_t)p, phase, kStreamPose + 64, r);
+        u0 = Philox::u01(r[0], r[1]) / m;
+    }
+    __syncthreads();
+    double ws = 0.0;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        const double uj = (u0 + (double)j / m) * total;   // cdf not normalised: scale u
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (cdf[mid] >= uj) hi = mid; else lo = mid + 1;
+        }
+        idx[offOut[p] + j] = lo;
+        const double wn = 1.0 / up[lo];
+        wNew[offOut[p] + j] = wn;
+        ws += wn;
+    }
+    part[threadIdx.x] = ws;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+        __syncthreads();
+    }
+    const double tot = part[0];
+    for (int j = threadIdx.x; j < m; j += blockDim.x) wNew[offOut[p] + j] /= tot;
+}
+
+// ---------------------------------------------------------------------------
+// Weighted mean rotation of each particle's samples: principal eigenvector of
+// the weighted scatter sum_i w_i q_i q_i^T (the ACG mean that inferACG,
+// src/Geometry/DirectionalStat.cpp:87-244, estimates), by power iteration.
+// Also returns the eigenvalue ratios lambda_a / lambda_0 (a = 1..3) used as
+// the concentration k1..k3 of the perturbation (Particle::calVari,
+// src/Particle.cpp:1002-1126).
+// ---------------------------------------------------------------------------
+__global__ void k_rot_mean(const double* __restrict__ quat, const double* __restrict__ w,
+                           const int* __restrict__ off, const int* __restrict__ cnt, int P,
+                           double* __restrict__ mean, double* __restrict__ kk) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double A[4][4] = {};
+    double ws = 0.0;
+    for (int j = 0; j < cnt[p]; j++) {
+        const size_t i = (size_t)off[p] + j;
+        const double wi = w ? w[i] : 1.0;
+        const double* q = quat + 4 * i;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) A[a][b] += wi * q[a] * q[b];
+        ws += wi;
+    }
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) A[a][b] /= ws;
+    // eigen-decomposition of the symmetric 4x4 by cyclic Jacobi
+    double Vv[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+    for (int sweep = 0; sweep < 30; sweep++) {
+        double off2 = 0.0;
+        for (int a = 0; a < 4; a++)
+            for (int b = a + 1; b < 4; b++) off2 += A[a][b] * A[a][b];
+        if (off2 < 1e-30) break;
+        for (int a = 0; a < 4; a++)
+            for (int b = a + 1; b < 4; b++) {
+                if (fabs(A[a][b]) < 1e-300) continue;
+                const double th = (A[b][b] - A[a][a]) / (2 * A[a][b]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                const double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 4; k++) {
+                    const double akp = A[k][a], akq = A[k][b];
+                    A[k][a] = c * akp - s * akq;
+                    A[k][b] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 4; k++) {
+                    const double apk = A[a][k], aqk = A[b][k];
+                    A[a][k] = c * apk - s * aqk;
+                    A[b][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 4; k++) {
+                    const double vkp = Vv[k][a], vkq = Vv[k][b];
+                    Vv[k][a] = c * vkp - s * vkq;
+                    Vv[k][b] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int order[4] = {0, 1, 2, 3};
+    for (int a = 0; a < 4; a++)
+        for (int b = a + 1; b < 4; b++)
+            if (A[order[b]][order[b]] > A[order[a]][order[a]]) {
+                const int t = order[a]; order[a] = order[b]; order[b] = t;
+            }
+    const int top = order[0];
+    double sgn = Vv[0][top] < 0 ? -1.0 : 1.0;   // canonical sign: w >= 0
+    for (int a = 0; a < 4; a++) mean[4 * p + a] = sgn * Vv[a][top];
+    const double l0 = fmax(A[top][top], 1e-300);
+    for (int a = 1; a < 4; a++) kk[3 * p + a - 1] = fmax(A[order[a]][order[a]], 0.0) / l0;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
+                   const double* mean, const double* kk, double pf, int P, uint64_t seed,
+                   uint32_t phase, const double* dGiven) {
+    dim3 grid((maxCnt + 255) / 256, P);
+    hipLaunchKernelGGL(k_perturb_rot, grid, dim3(256), 0, st, quat, off, cnt, mean, kk, pf, P,
+                       seed, phase, dGiven);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+int pf_perturb_trans(hipStream_t st, double* trans, const int* off, const int* cnt, int maxCnt,
+                     const double* s01rho, double pf, int P, uint64_t seed, uint32_t phase) {
+    dim3 grid((maxCnt + 255) / 256, P);
+    hipLaunchKernelGGL(k_perturb_trans, grid, dim3(256), 0, st, trans, off, cnt, s01rho, pf, P,
+                       seed, phase);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+int pf_sample_mixture(hipStream_t st, const double* mix, int C, int P, int nRot, int nTr,
+                      uint64_t seed, uint32_t phase, double* quat, double* qdens,
+                      double* trans) {
+    const int n = nRot > nTr ? nRot : nTr;
+    dim3 grid((n + 255) / 256, P);
+    hipLaunchKernelGGL(k_sample_mixture, grid, dim3(256), 0, st, mix, C, P, nRot, nTr, seed, phase,
+                       quat, qdens, trans);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+int pf_resample(hipStream_t st, const double* w, const double* u, const int* offIn,
+                const int* nIn, const int* offOut, const int* nOut, int P, uint64_t seed,
+                uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess) {
+    hipLaunchKernelGGL(k_resample, dim3(P), dim3(256), 0, st, w, u, offIn, nIn, offOut, nOut, P,
+                       seed, phase, u0Given, idx, wNew, ess);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+int pf_rot_mean(hipStream_t st, const double* quat, const double* w, const int* off,
+                const int* cnt, int P, double* mean, double* kk) {
+    hipLaunchKernelGGL(k_rot_mean, dim3((P + 127) / 128), dim3(128), 0, st, quat, w, off, cnt, P,
+                       mean, kk);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+}  // namespace thx

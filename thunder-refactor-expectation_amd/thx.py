@@ -46,6 +46,10 @@ _SIGS = {
     "thx_ctx_score_local": [_p, _p, _p, _p, _p, _p, _p, _p, _i],
     "thx_ctx_get_local": [_p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
+    "thx_pf_resample": [_p, _p, _p, _p, _p, _i, _u64, _p, _p, _p, _p],
+    "thx_pf_rot_mean": [_p, _p, _p, _p, _i, _p, _p],
+    "thx_pf_perturb": [_p, _p, _p, _p, _p, _i, _p, _p, _p, _d, _u64, _i, _p],
+    "thx_proposal_sample": [_p, _p, _i, _i, _i, _i, _u64, _p, _p, _p],
     "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
                                 _u64, _p, _p, _p],
@@ -262,6 +266,51 @@ class Context:
         check(load().thx_ctx_get_local(self._h, ptr(wC), ptr(wR), ptr(wT), ptr(base), ptr(bR),
                                        ptr(bT)), "get_local")
         return wC, wR, wT, base, bR, bT
+
+    # ---- particle filter ----
+    def pf_resample(self, w, u, nIn, nOut, seed=1, u0=None):
+        w, u = c32(w, np.float64), c32(u, np.float64)
+        nIn, nOut = c32(nIn, np.int32), c32(nOut, np.int32)
+        P = nIn.size
+        idx = np.zeros(int(nOut.sum()), np.int32)
+        wn = np.zeros(int(nOut.sum()))
+        ess = np.zeros(P)
+        u0 = None if u0 is None else c32(u0, np.float64)
+        check(load().thx_pf_resample(self._h, ptr(w), ptr(u), ptr(nIn), ptr(nOut), P, seed,
+                                     ptr(u0), ptr(idx), ptr(wn), ptr(ess)), "pf_resample")
+        return idx, wn, ess
+
+    def pf_rot_mean(self, quat, cnt, w=None):
+        quat = c32(quat, np.float64)
+        cnt = c32(cnt, np.int32)
+        w = None if w is None else c32(w, np.float64)
+        P = cnt.size
+        mean, k = np.zeros((P, 4)), np.zeros((P, 3))
+        check(load().thx_pf_rot_mean(self._h, ptr(quat), ptr(w), ptr(cnt), P, ptr(mean), ptr(k)),
+              "pf_rot_mean")
+        return mean, k
+
+    def pf_perturb(self, quat=None, nR=None, trans=None, nT=None, mean=None, k=None, s01rho=None,
+                   pf=1.0, seed=1, phase=0, d=None):
+        P = len(nR) if nR is not None else len(nT)
+        q = None if quat is None else c32(quat, np.float64).copy()
+        t = None if trans is None else c32(trans, np.float64).copy()
+        arr = lambda a, dt: None if a is None else c32(a, dt)
+        check(load().thx_pf_perturb(self._h, ptr(q), ptr(arr(nR, np.int32)), ptr(t),
+                                    ptr(arr(nT, np.int32)), P, ptr(arr(mean, np.float64)),
+                                    ptr(arr(k, np.float64)), ptr(arr(s01rho, np.float64)), pf,
+                                    seed, phase, ptr(arr(d, np.float64))), "pf_perturb")
+        return q, t
+
+    def proposal_sample(self, mix, C, nRot, nTr, seed=1):
+        mix = c32(mix, np.float64)
+        P = mix.size // (6 * C + 3)
+        q = np.zeros((P * nRot, 4))
+        dens = np.zeros(P * nRot)
+        t = np.zeros((P * nTr, 2))
+        check(load().thx_proposal_sample(self._h, ptr(mix), C, P, nRot, nTr, seed, ptr(q),
+                                         ptr(dens), ptr(t)), "proposal_sample")
+        return q, dens, t
 
     def last_timing(self):
         buf = C.create_string_buffer(1024)
