@@ -236,6 +236,41 @@ int thx_proposal_sample(thx_ctx* ctx, const double* mix, int C, int P,
                         double* qdens, double* trans);
 
 /* ------------------------------------------------------------------------ */
+/* Learned pose proposal (no counterpart in the reference, which always     */
+/* scans a uniform grid first): a ~0.9M-parameter CNN on 64 x 64 phase-     */
+/* flipped particle images predicting a C-component ACG mixture over        */
+/* rotations and a Gaussian over shifts.  Training, inference and gradients */
+/* run in hand-written HIP kernels.                                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct thx_cnn thx_cnn;
+
+int thx_cnn_create(int device, int components, uint64_t seed, thx_cnn** out);
+int thx_cnn_destroy(thx_cnn* net);
+int thx_cnn_num_params(thx_cnn* net, size_t* count);
+/* Flattened parameters: conv1..4 (weights [co][ci][ky][kx], biases), fc1
+ * [384][2048] + bias, fc2 [6C+3][384] + bias. */
+int thx_cnn_get_params(thx_cnn* net, float* p);
+int thx_cnn_set_params(thx_cnn* net, const float* p);
+/* out[B][6C+3]: raw network outputs for images [B][64][64], B <= 512. */
+int thx_cnn_forward(thx_cnn* net, const float* images, int B, float* out);
+/* Mean mixture negative log-likelihood of true quaternions q[B][4] and shifts
+ * t[B][2]; grad (optional) receives d loss / d params. */
+int thx_cnn_loss_grad(thx_cnn* net, const float* images, int B, const double* q,
+                      const double* t, double* loss, float* grad);
+/* One Adam step (lr) on a batch; loss is the batch loss before the step. */
+int thx_cnn_train_step(thx_cnn* net, const float* images, int B, const double* q,
+                       const double* t, float lr, double* loss);
+/* 64 x 64 images of the context's particles (device; optional host copy). */
+int thx_ctx_images(thx_ctx* ctx, float* host);
+/* Train on the context's particles with known poses q[P][4], t[P][2]. */
+int thx_cnn_train_ctx(thx_cnn* net, thx_ctx* ctx, const double* q, const double* t,
+                      int B, int epochs, float lr, double* lastLoss);
+/* Mixture parameters per particle of the context, mix[P][6C+3]: C x (m0 m1
+ * m2 m3 k pi) then (mu_x mu_y sigma); host and/or device destination. */
+int thx_cnn_propose_ctx(thx_cnn* net, thx_ctx* ctx, double* mixHost, double* mixDev);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic workloads (seeded; Philox4x32-10 counter streams).              */
 /* ------------------------------------------------------------------------ */
 

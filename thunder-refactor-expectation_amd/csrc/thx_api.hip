@@ -231,7 +231,7 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->locNR, &c->locTOff, &c->locNT, &c->locBlocks, &c->locBStart,
                        &c->locPcls, &c->locVols, &c->locBMax, &c->locBWC, &c->locBArg,
                        &c->locBWT, &c->locWRrel, &c->locWC, &c->locWR, &c->locWT, &c->locBase,
-                       &c->locBestR, &c->locBestT};
+                       &c->locBestR, &c->locBestT, &c->img};
     for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();
     for (auto& r : c->res) {

@@ -129,6 +129,8 @@ struct thx_ctx {
         locBStart, locPcls, locVols, locBMax, locBWC, locBArg, locBWT, locWRrel;
     thx::DevBuf locWC, locWR, locWT, locBase, locBestR, locBestT;
     bool locValid = false;
+    // 64 x 64 particle images of the learned proposal
+    thx::DevBuf img;
     // timing
     hipEvent_t ev[16] = {};
     int nev = 0;

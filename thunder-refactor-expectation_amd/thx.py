@@ -50,6 +50,17 @@ _SIGS = {
     "thx_pf_rot_mean": [_p, _p, _p, _p, _i, _p, _p],
     "thx_pf_perturb": [_p, _p, _p, _p, _p, _i, _p, _p, _p, _d, _u64, _i, _p],
     "thx_proposal_sample": [_p, _p, _i, _i, _i, _i, _u64, _p, _p, _p],
+    "thx_cnn_create": [_i, _i, _u64, _p],
+    "thx_cnn_destroy": [_p],
+    "thx_cnn_num_params": [_p, _p],
+    "thx_cnn_get_params": [_p, _p],
+    "thx_cnn_set_params": [_p, _p],
+    "thx_cnn_forward": [_p, _p, _i, _p],
+    "thx_cnn_loss_grad": [_p, _p, _i, _p, _p, _p, _p],
+    "thx_cnn_train_step": [_p, _p, _i, _p, _p, C.c_float, _p],
+    "thx_ctx_images": [_p, _p],
+    "thx_cnn_train_ctx": [_p, _p, _p, _p, _i, _i, C.c_float, _p],
+    "thx_cnn_propose_ctx": [_p, _p, _p, _p],
     "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
                                 _u64, _p, _p, _p],
@@ -337,3 +348,79 @@ class Context:
                                              ptr(t), ptr(cp)), "synth_particles")
         self.P = P
         return q, t, cp
+
+
+class Proposal:
+    """The learned pose proposal (CNN with hand-written HIP kernels)."""
+
+    def __init__(self, components=8, seed=1, device=0):
+        self._h = C.c_void_p()
+        self.C = components
+        self.NOUT = 6 * components + 3
+        check(load().thx_cnn_create(device, components, seed, C.byref(self._h)), "cnn_create")
+        n = C.c_size_t(0)
+        check(load().thx_cnn_num_params(self._h, C.byref(n)), "cnn_num_params")
+        self.nparam = n.value
+
+    def close(self):
+        if self._h:
+            load().thx_cnn_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        p = np.zeros(self.nparam, np.float32)
+        check(load().thx_cnn_get_params(self._h, ptr(p)), "cnn_get_params")
+        return p
+
+    def set_params(self, p):
+        p = c32(p)
+        assert p.size == self.nparam
+        check(load().thx_cnn_set_params(self._h, ptr(p)), "cnn_set_params")
+
+    def forward(self, images):
+        images = c32(images)
+        B = images.shape[0]
+        out = np.zeros((B, self.NOUT), np.float32)
+        check(load().thx_cnn_forward(self._h, ptr(images), B, ptr(out)), "cnn_forward")
+        return out
+
+    def loss_grad(self, images, q, t, want_grad=True):
+        images = c32(images)
+        q, t = c32(q, np.float64), c32(t, np.float64)
+        loss = C.c_double(0)
+        g = np.zeros(self.nparam, np.float32) if want_grad else None
+        check(load().thx_cnn_loss_grad(self._h, ptr(images), images.shape[0], ptr(q), ptr(t),
+                                       C.byref(loss), ptr(g)), "cnn_loss_grad")
+        return loss.value, g
+
+    def train_step(self, images, q, t, lr):
+        images = c32(images)
+        q, t = c32(q, np.float64), c32(t, np.float64)
+        loss = C.c_double(0)
+        check(load().thx_cnn_train_step(self._h, ptr(images), images.shape[0], ptr(q), ptr(t),
+                                        lr, C.byref(loss)), "cnn_train_step")
+        return loss.value
+
+    def train_ctx(self, ctx, q, t, batch=256, epochs=1, lr=1e-3):
+        q, t = c32(q, np.float64), c32(t, np.float64)
+        loss = C.c_double(0)
+        check(load().thx_cnn_train_ctx(self._h, ctx._h, ptr(q), ptr(t), batch, epochs, lr,
+                                       C.byref(loss)), "cnn_train_ctx")
+        return loss.value
+
+    def propose(self, ctx):
+        mix = np.zeros((ctx.P, self.NOUT))
+        check(load().thx_cnn_propose_ctx(self._h, ctx._h, ptr(mix), None), "cnn_propose_ctx")
+        return mix
+
+
+def ctx_images(ctx):
+    out = np.zeros((ctx.P, 64, 64), np.float32)
+    check(load().thx_ctx_images(ctx._h, ptr(out)), "ctx_images")
+    return out
