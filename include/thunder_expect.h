@@ -269,6 +269,22 @@ int thx_cnn_train_ctx(thx_cnn* net, thx_ctx* ctx, const double* q, const double*
 /* Mixture parameters per particle of the context, mix[P][6C+3]: C x (m0 m1
  * m2 m3 k pi) then (mu_x mu_y sigma); host and/or device destination. */
 int thx_cnn_propose_ctx(thx_cnn* net, thx_ctx* ctx, double* mixHost, double* mixDev);
+int thx_cnn_components(thx_cnn* net, int* C);
+
+/* Expectation with the learned proposal over the context's grid: the grid's
+ * slices are projected once (BF16 table + FP32 |P|^2), the network's mixture
+ * ranks the grid for each particle, the nSelR most probable rotations and
+ * nSelT (<= 128) most probable translations are scored (BF16 operands, FP32
+ * accumulation, FP64 merge), and particles whose importance-weight effective
+ * sample size over the scored rotations, (sum w)^2 / sum w^2 with w = wR / q,
+ * is below essMin are rescored over the whole grid (the uniform fallback).
+ * *nFallback receives the number of such particles. */
+int thx_ctx_expect_learned(thx_ctx* ctx, thx_cnn* net, int cls, int nSelR,
+                           int nSelT, double essMin, int* nFallback);
+/* Per particle: best grid rotation / translation indices, best
+ * log-likelihood, ESS and whether it fell back (any output may be NULL). */
+int thx_ctx_get_learned(thx_ctx* ctx, int* bestR, int* bestT, double* score,
+                        double* ess, int* fellBack);
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic workloads (seeded; Philox4x32-10 counter streams).              */

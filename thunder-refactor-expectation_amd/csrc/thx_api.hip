@@ -231,7 +231,9 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->locNR, &c->locTOff, &c->locNT, &c->locBlocks, &c->locBStart,
                        &c->locPcls, &c->locVols, &c->locBMax, &c->locBWC, &c->locBArg,
                        &c->locBWT, &c->locWRrel, &c->locWC, &c->locWR, &c->locWT, &c->locBase,
-                       &c->locBestR, &c->locBestT, &c->img};
+                       &c->locBestR, &c->locBestT, &c->img, &c->lpMix, &c->lpRidx,
+                       &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpPmap, &c->lpFullR,
+                       &c->lpFullT};
     for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();
     for (auto& r : c->res) {
@@ -628,6 +630,165 @@ int thx_ctx_get_local(thx_ctx* c, double* wC, double* wR, double* wT, double* ba
     if (baseL) THX_HIP(hipMemcpy(baseL, c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost));
     if (bestR) THX_HIP(hipMemcpy(bestR, c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost));
     if (bestT) THX_HIP(hipMemcpy(bestT, c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned-proposal expectation (grid-restricted)
+// ---------------------------------------------------------------------------
+
+// Scores nP particles (rows pmap or 0..nP-1) against per-particle selections
+// (ridx / tidx, null = whole grid) with the gather scorer and merges in FP64
+// into the loc* result buffers.
+static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, int nSelR,
+                          int nSelT, const int* pmap) {
+    const int TPAD = nSelT <= 32 ? 32 : (nSelT <= 64 ? 64 : 128);
+    const int nbp = (nSelR + kLocalRotBlock - 1) / kLocalRotBlock;
+    THX_CHECK(nbp <= 64, THX_ERR_RANGE, "learned: too many selected rotations");
+    const size_t nblk = (size_t)nP * nbp;
+    THX_CHECK(nblk < (1u << 31), THX_ERR_RANGE, "learned: grid too large");
+    std::vector<int> rOff(nP), nR(nP, nSelR), tOff(nP), nT(nP, nSelT), bStart(nP + 1);
+    for (int p = 0; p < nP; p++) {
+        rOff[p] = p * nSelR;
+        tOff[p] = p * nSelT;
+        bStart[p] = p * nbp;
+    }
+    bStart[nP] = (int)nblk;
+    hipStream_t st = c->stream;
+    THX_TRY(upload(c->locROff, rOff.data(), nP, st));
+    THX_TRY(upload(c->locNR, nR.data(), nP, st));
+    THX_TRY(upload(c->locTOff, tOff.data(), nP, st));
+    THX_TRY(upload(c->locNT, nT.data(), nP, st));
+    THX_TRY(upload(c->locBStart, bStart.data(), nP + 1, st));
+    THX_TRY(c->locBMax.ensure(sizeof(float) * nblk));
+    THX_TRY(c->locBWC.ensure(sizeof(float) * nblk));
+    THX_TRY(c->locBArg.ensure(sizeof(int) * nblk));
+    THX_TRY(c->locBWT.ensure(sizeof(float) * nblk * TPAD));
+    THX_TRY(c->locWRrel.ensure(sizeof(float) * (size_t)nP * nSelR));
+    THX_TRY(c->locWC.ensure(sizeof(double) * nP));
+    THX_TRY(c->locWR.ensure(sizeof(double) * (size_t)nP * nSelR));
+    THX_TRY(c->locWT.ensure(sizeof(double) * (size_t)nP * nSelT));
+    THX_TRY(c->locBase.ensure(sizeof(double) * nP));
+    THX_TRY(c->locBestR.ensure(sizeof(int) * nP));
+    THX_TRY(c->locBestT.ensure(sizeof(int) * nP));
+    LocalBatch b;
+    b.rOff = c->locROff.as<const int>();
+    b.nR = c->locNR.as<const int>();
+    b.tOff = c->locTOff.as<const int>();
+    b.nT = c->locNT.as<const int>();
+    b.bStart = c->locBStart.as<const int>();
+    b.bMax = c->locBMax.as<float>();
+    b.bWC = c->locBWC.as<float>();
+    b.bArg = c->locBArg.as<int>();
+    b.bWT = c->locBWT.as<float>();
+    b.wRrel = c->locWRrel.as<float>();
+    THX_TRY(launch_score_sel(c, TPAD, (int)nblk, ridx, tidx, nSelR, nSelT, 1.f / c->R, 1.f / c->T,
+                             pmap, b));
+    THX_TRY(launch_merge_local(c, nP, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
+                               c->locWT.as<double>(), c->locBase.as<double>(),
+                               c->locBestR.as<int>(), c->locBestT.as<int>()));
+    return THX_OK;
+}
+
+int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSelT, double essMin,
+                           int* nFallback) {
+    clear_error();
+    THX_CHECK(c && net && nSelR > 0 && nSelT > 0, THX_ERR_ARG, "expect_learned: bad argument");
+    THX_CHECK(c->K > 0 && c->P > 0 && c->R > 0 && c->T > 0, THX_ERR_STATE,
+              "expect_learned: pixels, particles and grid must be set");
+    THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] == c->idim * c->pf,
+              THX_ERR_STATE, "expect_learned: volume of this class not set");
+    THX_CHECK(nSelR <= c->R && nSelT <= c->T && c->R <= 16384 && c->T <= 128,
+              THX_ERR_RANGE, "expect_learned: selection sizes out of range");
+    int C = 0;
+    THX_TRY(thx_cnn_components(net, &C));
+    THX_TRY(set_device(c));
+    const int P = c->P;
+    hipStream_t st = c->stream;
+    c->nev = 0;
+    c->evNames.clear();
+    THX_TRY(record(c, "start"));
+    // slices of the grid, once for all particles; BF16 table + FP32 |P|^2
+    THX_TRY(launch_project(c, cls, c->quat.as<const double>(), c->R, c->Rpad,
+                           c->rotP.as<float2>(), c->normP.as<float>()));
+    THX_TRY(c->rotLo.ensure(sizeof(uint32_t) * (size_t)c->Rpad * c->Kp));
+    THX_TRY(launch_convert_lo(c, THX_BF16));
+    THX_TRY(record(c, "project"));
+    THX_TRY(launch_prep_particles(c, THX_FP32));
+    THX_TRY(record(c, "prep"));
+    // proposal network (synchronous on its own stream)
+    THX_TRY(c->lpMix.ensure(sizeof(double) * (size_t)P * (6 * C + 3)));
+    THX_TRY(thx_cnn_propose_ctx(net, c, nullptr, c->lpMix.as<double>()));
+    THX_TRY(record(c, "propose"));
+    THX_TRY(c->lpRidx.ensure(sizeof(int) * (size_t)P * nSelR));
+    THX_TRY(c->lpRlogq.ensure(sizeof(float) * (size_t)P * nSelR));
+    THX_TRY(c->lpTidx.ensure(sizeof(int) * (size_t)P * nSelT));
+    THX_TRY(pf_select_grid(st, c->lpMix.as<double>(), C, P, c->quat.as<double>(), c->R,
+                           c->trans.as<double>(), c->T, nSelR, nSelT, c->lpRidx.as<int>(),
+                           c->lpRlogq.as<float>(), c->lpTidx.as<int>()));
+    THX_TRY(record(c, "select"));
+    THX_TRY(score_selected(c, P, c->lpRidx.as<int>(), c->lpTidx.as<int>(), nSelR, nSelT, nullptr));
+    THX_TRY(c->lpESS.ensure(sizeof(double) * P));
+    THX_TRY(launch_ess_sel(st, c->locWR.as<double>(), c->lpRlogq.as<float>(), nSelR, P,
+                           c->lpESS.as<double>()));
+    THX_TRY(record(c, "score_sel"));
+    // results in grid indices
+    std::vector<int> bR(P), bT(P), ridx((size_t)P * nSelR), tidx((size_t)P * nSelT);
+    c->lpScore.assign(P, 0.0);
+    c->lpEssH.assign(P, 0.0);
+    THX_HIP(hipStreamSynchronize(st));
+    THX_HIP(hipMemcpy(bR.data(), c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(bT.data(), c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(c->lpScore.data(), c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(c->lpEssH.data(), c->lpESS.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(ridx.data(), c->lpRidx.p, sizeof(int) * ridx.size(), hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(tidx.data(), c->lpTidx.p, sizeof(int) * tidx.size(), hipMemcpyDeviceToHost));
+    c->lpBestR.assign(P, -1);
+    c->lpBestT.assign(P, -1);
+    c->lpFell.assign(P, 0);
+    std::vector<int> flagged;
+    for (int p = 0; p < P; p++) {
+        if (bR[p] >= 0) c->lpBestR[p] = ridx[(size_t)p * nSelR + bR[p]];
+        if (bT[p] >= 0) c->lpBestT[p] = tidx[(size_t)p * nSelT + bT[p]];
+        if (!(c->lpEssH[p] >= essMin)) flagged.push_back(p);
+    }
+    // fallback: uniform search over the whole grid for collapsed particles
+    const int nf = (int)flagged.size();
+    if (nf > 0) {
+        THX_TRY(upload(c->lpPmap, flagged.data(), nf, st));
+        THX_TRY(score_selected(c, nf, nullptr, nullptr, c->R, c->T, c->lpPmap.as<int>()));
+        std::vector<int> fR(nf), fT(nf);
+        std::vector<double> fS(nf);
+        THX_HIP(hipStreamSynchronize(st));
+        THX_HIP(hipMemcpy(fR.data(), c->locBestR.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(fT.data(), c->locBestT.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(fS.data(), c->locBase.p, sizeof(double) * nf, hipMemcpyDeviceToHost));
+        for (int i = 0; i < nf; i++) {
+            const int p = flagged[i];
+            c->lpBestR[p] = fR[i];
+            c->lpBestT[p] = fT[i];
+            c->lpScore[p] = fS[i];
+            c->lpFell[p] = 1;
+        }
+    }
+    THX_TRY(record(c, "fallback"));
+    THX_HIP(hipStreamSynchronize(st));
+    if (nFallback) *nFallback = nf;
+    c->lpValid = true;
+    c->locValid = false;   // loc* buffers now hold learned-mode data
+    return THX_OK;
+}
+
+int thx_ctx_get_learned(thx_ctx* c, int* bestR, int* bestT, double* score, double* ess,
+                        int* fellBack) {
+    clear_error();
+    THX_CHECK(c && c->lpValid, THX_ERR_STATE, "get_learned: no learned expectation done");
+    const int P = (int)c->lpBestR.size();
+    if (bestR) std::memcpy(bestR, c->lpBestR.data(), sizeof(int) * P);
+    if (bestT) std::memcpy(bestT, c->lpBestT.data(), sizeof(int) * P);
+    if (score) std::memcpy(score, c->lpScore.data(), sizeof(double) * P);
+    if (ess) std::memcpy(ess, c->lpEssH.data(), sizeof(double) * P);
+    if (fellBack) std::memcpy(fellBack, c->lpFell.data(), sizeof(int) * P);
     return THX_OK;
 }
 

@@ -559,6 +559,12 @@ int thx_cnn_destroy(thx_cnn* n) {
     return THX_OK;
 }
 
+int thx_cnn_components(thx_cnn* n, int* C) {
+    THX_CHECK(n && C, THX_ERR_ARG, "cnn_components: null argument");
+    *C = n->C;
+    return THX_OK;
+}
+
 int thx_cnn_num_params(thx_cnn* n, size_t* count) {
     THX_CHECK(n && count, THX_ERR_ARG, "cnn_num_params: null argument");
     *count = n->nparam;

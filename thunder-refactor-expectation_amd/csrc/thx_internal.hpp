@@ -131,6 +131,11 @@ struct thx_ctx {
     bool locValid = false;
     // 64 x 64 particle images of the learned proposal
     thx::DevBuf img;
+    // grid-restricted learned proposal: mixture, selections, results
+    thx::DevBuf lpMix, lpRidx, lpRlogq, lpTidx, lpESS, lpPmap, lpFullR, lpFullT;
+    std::vector<int> lpBestR, lpBestT, lpFell;
+    std::vector<double> lpScore, lpEssH;
+    bool lpValid = false;
     // timing
     hipEvent_t ev[16] = {};
     int nev = 0;
@@ -179,6 +184,13 @@ int pf_resample(hipStream_t st, const double* w, const double* u, const int* off
 int pf_rot_mean(hipStream_t st, const double* quat, const double* w, const int* off,
                 const int* cnt, int P, double* mean, double* kk);
 constexpr int kResampleMaxN = 8192;
+int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
+                   const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
+                   int* tidx);
+int launch_score_sel(thx_ctx* c, int TPAD, int nblk, const int* ridx, const int* tidx, int nSelR,
+                     int nSelT, float pRu, float pTu, const int* pmap, const LocalBatch& b);
+int launch_ess_sel(hipStream_t st, const double* wR, const float* rlogq, int nSel, int P,
+                   double* ess);
 
 // Philox4x32-10 (Salmon et al., SC'11), shared by host and device code.
 struct Philox {

@@ -1490,11 +1490,256 @@ __global__ __launch_bounds__(256) void k_merge_local(
     }
 }
 
+// importance-weight ESS per particle over its selected rotations:
+// w_r = wR_r / q_r (translation-marginal likelihood over the proposal density)
+__global__ void k_ess_sel(const double* __restrict__ wR, const float* __restrict__ rlogq, int nSel,
+                          int P, double* __restrict__ ess) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double mx = -INFINITY;
+    for (int r = 0; r < nSel; r++) mx = fmax(mx, -(double)rlogq[(size_t)p * nSel + r]);
+    double s1 = 0.0, s2 = 0.0;
+    for (int r = 0; r < nSel; r++) {
+        const double w = wR[(size_t)p * nSel + r] * exp(-(double)rlogq[(size_t)p * nSel + r] - mx);
+        s1 += w;
+        s2 += w * w;
+    }
+    ess[p] = s2 > 0.0 ? s1 * s1 / s2 : 0.0;
+}
+
+int launch_ess_sel(hipStream_t st, const double* wR, const float* rlogq, int nSel, int P,
+                   double* ess) {
+    hipLaunchKernelGGL(k_ess_sel, dim3((P + 127) / 128), dim3(128), 0, st, wR, rlogq, nSel, P, ess);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
                        double* wT, double* base, int* bestR, int* bestT) {
     hipLaunchKernelGGL(k_merge_local, dim3(P), dim3(256), 0, c->stream, b.bMax, b.bWC, b.bArg,
                        b.bWT, b.wRrel, b.bStart, b.rOff, b.nR, b.tOff, b.nT, TPAD, wC, wR, wT,
                        base, bestR, bestT);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Selected-pose scorer of the grid-restricted proposal.  Particle p scores
+// its nSelR selected grid rotations (rows of the shared slice table rotLo,
+// BF16) against its nSelT selected grid translations (rows of the phase
+// table S); the norm term is accumulated in FP32 from the |P|^2 table.  A
+// block owns one particle x up to 128 of its selected rotations x all its
+// selected translations; epilogue and merge are those of the local scorer.
+// ---------------------------------------------------------------------------
+template <int TPAD>
+__global__ __launch_bounds__(256, 2) void k_score_sel(
+    const uint32_t* __restrict__ rotLo, const float* __restrict__ normP,
+    const float2* __restrict__ S, int Kp, const float2* __restrict__ Y,
+    const float* __restrict__ Wt, const double* __restrict__ Aterm, const int* __restrict__ ridx,
+    const int* __restrict__ tidx, int nSelR, int nSelT, float pRu, float pTu,
+    const int* __restrict__ pmap, float* __restrict__ bMax, float* __restrict__ bWC,
+    int* __restrict__ bArg, float* __restrict__ bWT, float* __restrict__ wRrel) {
+    constexpr int LD = 72;
+    constexpr int MT = TPAD / 32;
+    constexpr int RB = kNBL;
+    __shared__ __attribute__((aligned(16))) unsigned short As[TPAD * LD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[RB * LD];
+    __shared__ int rrow[RB];
+    __shared__ int trow[TPAD];
+    __shared__ float Cs[RB];
+    __shared__ float redMax[4];
+    __shared__ int redArg[4];
+    __shared__ float rowPart[4][TPAD];
+    __shared__ float rowW[TPAD];
+
+    const int nbp = (nSelR + RB - 1) / RB;
+    const int p = blockIdx.x / nbp, r0 = (blockIdx.x % nbp) * RB;
+    const int pd = pmap ? pmap[p] : p;   // particle row of Y / Wt / Aterm
+    const int nr = min(RB, nSelR - r0);
+    const int Tp = nSelT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hl = lane >> 5;
+    const int grp = tid >> 5;
+    // null selections mean the whole grid (rotation / translation index = position)
+    if (tid < RB) rrow[tid] = tid < nr ? (ridx ? ridx[(size_t)p * nSelR + r0 + tid] : r0 + tid) : -1;
+    if (tid < TPAD) trow[tid] = tid < Tp ? (tidx ? tidx[(size_t)p * nSelT + tid] : tid) : -1;
+    __syncthreads();
+
+    floatx16 acc[MT];
+#pragma unroll
+    for (int a = 0; a < MT; a++) acc[a] = floatx16{0};
+    float cacc[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) cacc[q] = 0.f;
+    const float2* yrow = Y + (size_t)pd * Kp;
+    const float* wrow = Wt + (size_t)pd * Kp;
+
+    for (int k0 = 0; k0 < Kp; k0 += kChunk) {
+        const int k = k0 + l32;
+        // ---- A: Z = Y conj(S_t) ----
+        {
+            const float2 y = yrow[k];
+#pragma unroll
+            for (int j = 0; j < TPAD / 8; j++) {
+                const int t = grp + 8 * j;
+                float zr = 0.f, zi = 0.f;
+                if (trow[t] >= 0) {
+                    const float2 sv = S[(size_t)trow[t] * Kp + k];
+                    zr = y.x * sv.x + y.y * sv.y;
+                    zi = y.y * sv.x - y.x * sv.y;
+                }
+                *reinterpret_cast<uint32_t*>(As + t * LD + 2 * l32) = pack16<THX_BF16>(zr, zi);
+            }
+        }
+        // ---- B: gathered slice rows, norm term ----
+        {
+            const float wk = wrow[k];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int rr = grp + 8 * q;
+                const int row = rrow[rr];
+                uint32_t v = 0u;
+                float nv = 0.f;
+                if (row >= 0) {
+                    v = rotLo[(size_t)row * Kp + k];
+                    nv = normP[(size_t)row * Kp + k];
+                }
+                cacc[q] += wk * nv;
+                *reinterpret_cast<uint32_t*>(Bs + rr * LD + 2 * l32) = v;
+            }
+        }
+        __syncthreads();
+        const int nrow = 32 * wave + l32;
+#pragma unroll
+        for (int kk = 0; kk < 64; kk += 16) {
+            const int kc = kk + 8 * hl;
+            const uint4 rb4 = *reinterpret_cast<const uint4*>(Bs + nrow * LD + kc);
+#pragma unroll
+            for (int mt = 0; mt < MT; mt++) {
+                const uint4 ra = *reinterpret_cast<const uint4*>(As + (32 * mt + l32) * LD + kc);
+                acc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bf16x8, ra), __builtin_bit_cast(bf16x8, rb4), acc[mt], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- norm term per rotation ----
+#pragma unroll
+    for (int q = 0; q < 16; q++) cacc[q] += __shfl_xor(cacc[q], 16);
+#pragma unroll
+    for (int s = 8; s >= 1; s >>= 1) {
+        const bool up = (l32 & s) != 0;
+#pragma unroll
+        for (int j = 0; j < s; j++) {
+            const float send = up ? cacc[j] : cacc[j + s];
+            const float keep = up ? cacc[j + s] : cacc[j];
+            cacc[j] = keep + __shfl_xor(send, s);
+        }
+    }
+    if (l32 < 16) Cs[grp + 8 * (l32 & 15)] = cacc[0];
+    __syncthreads();
+
+    const float a = (float)Aterm[pd];
+    const int ncol = 32 * wave + l32;
+    const bool cok = ncol < nr;
+    const float cterm = Cs[ncol];
+    float lmax = -INFINITY;
+    int larg = -1;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int t = 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+            float v = -INFINITY;
+            if (cok && t < Tp) v = a + cterm - 2.f * acc[mt][i];
+            acc[mt][i] = v;
+            if (v > lmax) { lmax = v; larg = ncol * TPAD + t; }
+        }
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(lmax, off);
+        const int oa = __shfl_xor(larg, off);
+        if (o > lmax || (o == lmax && oa >= 0 && (larg < 0 || oa < larg))) { lmax = o; larg = oa; }
+    }
+    if (lane == 0) { redMax[wave] = lmax; redArg[wave] = larg; }
+    __syncthreads();
+    float M = redMax[0];
+    int arg = redArg[0];
+    for (int w = 1; w < 4; w++)
+        if (redMax[w] > M || (redMax[w] == M && redArg[w] >= 0 && (arg < 0 || redArg[w] < arg))) {
+            M = redMax[w];
+            arg = redArg[w];
+        }
+    const bool live = M > -INFINITY;
+    constexpr int NV = MT * 16 < 32 ? 32 : MT * 16;
+    float rowv[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) rowv[j] = 0.f;
+    const float pr = cok ? pRu : 0.f;
+    float cs = 0.f;
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int t = 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+            const float w = live ? __expf(acc[mt][i] - M) : 0.f;
+            cs += w * (t < Tp ? pTu : 0.f);
+            rowv[mt * 16 + i] = w * pr;
+        }
+    cs += __shfl_xor(cs, 32);
+    if (hl == 0 && cok) wRrel[(size_t)p * nSelR + r0 + ncol] = cs;
+    constexpr int NG = NV / 32;
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+#pragma unroll
+        for (int s = 16; s >= 1; s >>= 1) {
+            const bool up = (l32 & s) != 0;
+#pragma unroll
+            for (int j = 0; j < s; j++) {
+                const float send = up ? rowv[g * 32 + j] : rowv[g * 32 + j + s];
+                const float keep = up ? rowv[g * 32 + j + s] : rowv[g * 32 + j];
+                rowv[g * 32 + j] = keep + __shfl_xor(send, s);
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        const int jv = g * 32 + l32;
+        const int mt = jv >> 4, i = jv & 15;
+        if (mt < MT) {
+            const int t = 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+            rowPart[wave][t] = rowv[g * 32];
+        }
+    }
+    __syncthreads();
+    if (tid < TPAD) {
+        const float rsum = rowPart[0][tid] + rowPart[1][tid] + rowPart[2][tid] + rowPart[3][tid];
+        bWT[(size_t)blockIdx.x * TPAD + tid] = rsum;
+        rowW[tid] = tid < Tp ? rsum * pTu : 0.f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float wc = 0.f;
+        for (int t = 0; t < TPAD; t++) wc += rowW[t];
+        bMax[blockIdx.x] = M;
+        bWC[blockIdx.x] = wc;
+        bArg[blockIdx.x] = arg < 0 ? -1 : ((r0 + arg / TPAD) * kNBL + arg % TPAD);
+    }
+}
+
+int launch_score_sel(thx_ctx* c, int TPAD, int nblk, const int* ridx, const int* tidx, int nSelR,
+                     int nSelT, float pRu, float pTu, const int* pmap, const LocalBatch& b) {
+#define THX_SEL_ARGS c->rotLo.as<const uint32_t>(), c->normP.as<const float>(),            \
+                     c->S.as<const float2>(), c->Kp, c->Y.as<const float2>(),              \
+                     c->Wt.as<const float>(), c->Aterm.as<const double>(), ridx, tidx,     \
+                     nSelR, nSelT, pRu, pTu, pmap, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel
+    if (TPAD == 32)
+        hipLaunchKernelGGL((k_score_sel<32>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
+    else if (TPAD == 64)
+        hipLaunchKernelGGL((k_score_sel<64>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
+    else
+        hipLaunchKernelGGL((k_score_sel<128>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
+#undef THX_SEL_ARGS
     THX_HIP(hipGetLastError());
     return THX_OK;
 }

@@ -316,6 +316,139 @@ __global__ void k_rot_mean(const double* __restrict__ quat, const double* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Grid-restricted proposal: per particle, the log mixture density of every
+// grid rotation and the log Gaussian density of every grid translation; the
+// nSel highest of each are selected (threshold by bisection on the value,
+// ties by index order).  One block per particle.
+// ---------------------------------------------------------------------------
+__device__ inline float block_count_ge(const float* v, int n, float thr, int* red) {
+    int c = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) c += v[i] >= thr;
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const int r = red[0];
+    __syncthreads();
+    return (float)r;
+}
+
+// select the nSel largest of v[0..n) into out (indices), writes dens[j] = v
+__device__ void block_topk(const float* v, int n, int nSel, int* out, float* outv, float lo,
+                           float hi, int* red, int* cnt) {
+    // bisection for thr: count(v >= thr) >= nSel with thr as large as possible
+    for (int it = 0; it < 40; it++) {
+        const float mid = 0.5f * (lo + hi);
+        if (block_count_ge(v, n, mid, red) >= nSel) lo = mid; else hi = mid;
+    }
+    const float thr = lo;
+    if (threadIdx.x == 0) *cnt = 0;
+    __syncthreads();
+    // strictly above the threshold first, then ties at the threshold, in index order
+    for (int pass = 0; pass < 2; pass++) {
+        for (int base = 0; base < n; base += blockDim.x) {
+            const int i = base + threadIdx.x;
+            bool take = false;
+            if (i < n) take = pass == 0 ? v[i] > hi : (v[i] >= thr && v[i] <= hi);
+            // ordered compaction within the block
+            const unsigned long long m = __ballot(take);
+            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            const int before = __popcll(m & ((1ull << lane) - 1));
+            red[wave] = __popcll(m);
+            __syncthreads();
+            int off = *cnt;
+            for (int w = 0; w < wave; w++) off += red[w];
+            if (take && off + before < nSel) {
+                out[off + before] = i;
+                outv[off + before] = v[i];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int tot = 0;
+                for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += red[w];
+                *cnt = min(nSel, *cnt + tot);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+constexpr int kSelMaxR = 16384;
+
+__global__ __launch_bounds__(256) void k_select_grid(
+    const double* __restrict__ mix, int C, const double* __restrict__ gridQ, int R,
+    const double* __restrict__ gridT, int T, int nSelR, int nSelT, int* __restrict__ ridx,
+    float* __restrict__ rlogq, int* __restrict__ tidx) {
+    const int p = blockIdx.x;
+    const double* mp = mix + (size_t)p * (6 * C + 3);
+    __shared__ float v[kSelMaxR];
+    __shared__ int red[256];
+    __shared__ int cnt;
+    __shared__ float tv[512];
+    float mx = -INFINITY, mn = INFINITY;
+    for (int r = threadIdx.x; r < R; r += blockDim.x) {
+        const double* q = gridQ + 4 * (size_t)r;
+        double dens = 0.0;
+        for (int c = 0; c < C; c++) dens += mp[6 * c + 5] * acg_pdf(mp + 6 * c, mp[6 * c + 4], q);
+        const float lv = (float)log(fmax(dens, 1e-300));
+        v[r] = lv;
+        mx = fmaxf(mx, lv);
+        mn = fminf(mn, lv);
+    }
+    // block min / max for the bisection range
+    __shared__ float rmx[256], rmn[256];
+    rmx[threadIdx.x] = mx;
+    rmn[threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            rmx[threadIdx.x] = fmaxf(rmx[threadIdx.x], rmx[threadIdx.x + s]);
+            rmn[threadIdx.x] = fminf(rmn[threadIdx.x], rmn[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    const float hiR = rmx[0], loR = rmn[0];
+    __syncthreads();
+    block_topk(v, R, nSelR, ridx + (size_t)p * nSelR, rlogq + (size_t)p * nSelR, loR, hiR, red,
+               &cnt);
+    // translations: Gaussian log density (the constant does not matter)
+    const double* sh = mp + 6 * C;
+    float tmx = -INFINITY, tmn = INFINITY;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        const double dx = gridT[2 * t] - sh[0], dy = gridT[2 * t + 1] - sh[1];
+        const float lv = (float)(-(dx * dx + dy * dy) / (2 * sh[2] * sh[2]));
+        tv[t] = lv;
+        tmx = fmaxf(tmx, lv);
+        tmn = fminf(tmn, lv);
+    }
+    rmx[threadIdx.x] = tmx;
+    rmn[threadIdx.x] = tmn;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            rmx[threadIdx.x] = fmaxf(rmx[threadIdx.x], rmx[threadIdx.x + s]);
+            rmn[threadIdx.x] = fminf(rmn[threadIdx.x], rmn[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    const float hiT = rmx[0], loT = rmn[0];
+    __syncthreads();
+    __shared__ float tdummy[512];
+    block_topk(tv, T, nSelT, tidx + (size_t)p * nSelT, tdummy, loT, hiT, red, &cnt);
+}
+
+int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
+                   const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
+                   int* tidx) {
+    hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), 0, st, mix, C, gridQ, R, gridT, T, nSelR,
+                       nSelT, ridx, rlogq, tidx);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,

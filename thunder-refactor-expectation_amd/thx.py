@@ -61,6 +61,9 @@ _SIGS = {
     "thx_ctx_images": [_p, _p],
     "thx_cnn_train_ctx": [_p, _p, _p, _p, _i, _i, C.c_float, _p],
     "thx_cnn_propose_ctx": [_p, _p, _p, _p],
+    "thx_cnn_components": [_p, _p],
+    "thx_ctx_expect_learned": [_p, _p, _i, _i, _i, _d, _p],
+    "thx_ctx_get_learned": [_p, _p, _p, _p, _p, _p],
     "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
                                 _u64, _p, _p, _p],
@@ -277,6 +280,22 @@ class Context:
         check(load().thx_ctx_get_local(self._h, ptr(wC), ptr(wR), ptr(wT), ptr(base), ptr(bR),
                                        ptr(bT)), "get_local")
         return wC, wR, wT, base, bR, bT
+
+    # ---- learned proposal ----
+    def expect_learned(self, net, cls=0, n_rot=300, n_trans=25, ess_min=0.0):
+        nf = C.c_int(0)
+        check(load().thx_ctx_expect_learned(self._h, net._h, cls, n_rot, n_trans, ess_min,
+                                            C.byref(nf)), "expect_learned")
+        return nf.value
+
+    def get_learned(self):
+        P = self.P
+        bR, bT = np.zeros(P, np.int32), np.zeros(P, np.int32)
+        sc, ess = np.zeros(P), np.zeros(P)
+        fell = np.zeros(P, np.int32)
+        check(load().thx_ctx_get_learned(self._h, ptr(bR), ptr(bT), ptr(sc), ptr(ess),
+                                         ptr(fell)), "get_learned")
+        return bR, bT, sc, ess, fell
 
     # ---- particle filter ----
     def pf_resample(self, w, u, nIn, nOut, seed=1, u0=None):
