@@ -275,16 +275,26 @@ int thx_cnn_components(thx_cnn* net, int* C);
  * slices are projected once (BF16 table + FP32 |P|^2), the network's mixture
  * ranks the grid for each particle, the nSelR most probable rotations and
  * nSelT (<= 128) most probable translations are scored (BF16 operands, FP32
- * accumulation, FP64 merge), and particles whose importance-weight effective
- * sample size over the scored rotations, (sum w)^2 / sum w^2 with w = wR / q,
- * is below essMin are rescored over the whole grid (the uniform fallback).
- * *nFallback receives the number of such particles. */
+ * accumulation, FP64 merge).  A particle falls back to the uniform search of
+ * the whole grid (the tiled scan at fallbackPrecision) when any enabled test
+ * fires (a threshold <= 0 disables its test):
+ *   ESS of the importance weights w = wR / q over its scored rotations,
+ *     (sum w)^2 / sum w^2, below essMin;
+ *   spread of the proposal, the effective number of grid rotations under q,
+ *     (sum q)^2 / sum q^2, above spreadMax;
+ *   tail depth of its best rotation, max log q - log q(best), above tailMax;
+ *   best log-likelihood below the batch median by more than scoreDrop.
+ * *nFallback receives the number of particles that fell back. */
 int thx_ctx_expect_learned(thx_ctx* ctx, thx_cnn* net, int cls, int nSelR,
-                           int nSelT, double essMin, int* nFallback);
+                           int nSelT, double essMin, double spreadMax,
+                           double tailMax, double scoreDrop,
+                           int fallbackPrecision, int* nFallback);
 /* Per particle: best grid rotation / translation indices, best
- * log-likelihood, ESS and whether it fell back (any output may be NULL). */
+ * log-likelihood, ESS, spread, tail depth and whether it fell back (any
+ * output may be NULL). */
 int thx_ctx_get_learned(thx_ctx* ctx, int* bestR, int* bestT, double* score,
-                        double* ess, int* fellBack);
+                        double* ess, float* spread, float* tail,
+                        int* fellBack);
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic workloads (seeded; Philox4x32-10 counter streams).              */

@@ -232,8 +232,10 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->locPcls, &c->locVols, &c->locBMax, &c->locBWC, &c->locBArg,
                        &c->locBWT, &c->locWRrel, &c->locWC, &c->locWR, &c->locWT, &c->locBase,
                        &c->locBestR, &c->locBestT, &c->img, &c->lpMix, &c->lpRidx,
-                       &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpPmap, &c->lpFullR,
-                       &c->lpFullT};
+                       &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpSpread, &c->lpTail,
+                       &c->lpPmap, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpRes.wC,
+                       &c->lpRes.wR, &c->lpRes.wT, &c->lpRes.base, &c->lpRes.bestR,
+                       &c->lpRes.bestT, &c->lpRes.bestScore};
     for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();
     for (auto& r : c->res) {
@@ -690,15 +692,48 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
     return THX_OK;
 }
 
+// Whole-grid scan (run_scan) of the particles idx: their rows are gathered,
+// and the context's particle inputs and this class's result slot are
+// swapped for the gathered rows and lpRes for the duration of the scan.
+static int scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int>& idx) {
+    const int n = (int)idx.size();
+    const size_t nk = (size_t)n * c->K;
+    THX_TRY(upload(c->lpPmap, idx.data(), n, c->stream));
+    THX_TRY(c->lpDat.ensure(sizeof(float2) * nk));
+    THX_TRY(c->lpCtf.ensure(sizeof(float) * nk));
+    THX_TRY(c->lpSig.ensure(sizeof(float) * nk));
+    THX_TRY(launch_gather_particles(c, c->lpPmap.as<int>(), n, c->lpDat.as<float2>(),
+                                    c->lpCtf.as<float>(), c->lpSig.as<float>()));
+    if ((int)c->res.size() <= cls) c->res.resize(cls + 1);
+    const int P0 = c->P, Ppad0 = c->Ppad;
+    auto swap_in = [&]() {
+        std::swap(c->dat, c->lpDat);
+        std::swap(c->ctf, c->lpCtf);
+        std::swap(c->sig, c->lpSig);
+        std::swap(c->res[cls], c->lpRes);
+    };
+    swap_in();
+    c->P = n;
+    c->Ppad = round_up(n, 128);
+    const int s = run_scan(c, cls, precision);
+    swap_in();
+    c->P = P0;
+    c->Ppad = Ppad0;
+    return s;
+}
+
 int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSelT, double essMin,
-                           int* nFallback) {
+                           double spreadMax, double tailMax, double scoreDrop,
+                           int fallbackPrecision, int* nFallback) {
     clear_error();
     THX_CHECK(c && net && nSelR > 0 && nSelT > 0, THX_ERR_ARG, "expect_learned: bad argument");
+    THX_CHECK(fallbackPrecision >= THX_FP32 && fallbackPrecision <= THX_FP16, THX_ERR_ARG,
+              "expect_learned: fallback precision must be FP32, BF16 or FP16");
     THX_CHECK(c->K > 0 && c->P > 0 && c->R > 0 && c->T > 0, THX_ERR_STATE,
               "expect_learned: pixels, particles and grid must be set");
     THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] == c->idim * c->pf,
               THX_ERR_STATE, "expect_learned: volume of this class not set");
-    THX_CHECK(nSelR <= c->R && nSelT <= c->T && c->R <= 16384 && c->T <= 128,
+    THX_CHECK(nSelR <= c->R && nSelT <= c->T && nSelT <= 128 && c->R <= 16384 && c->T <= 512,
               THX_ERR_RANGE, "expect_learned: selection sizes out of range");
     int C = 0;
     THX_TRY(thx_cnn_components(net, &C));
@@ -723,17 +758,22 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     THX_TRY(c->lpRidx.ensure(sizeof(int) * (size_t)P * nSelR));
     THX_TRY(c->lpRlogq.ensure(sizeof(float) * (size_t)P * nSelR));
     THX_TRY(c->lpTidx.ensure(sizeof(int) * (size_t)P * nSelT));
+    THX_TRY(c->lpSpread.ensure(sizeof(float) * P));
     THX_TRY(pf_select_grid(st, c->lpMix.as<double>(), C, P, c->quat.as<double>(), c->R,
                            c->trans.as<double>(), c->T, nSelR, nSelT, c->lpRidx.as<int>(),
-                           c->lpRlogq.as<float>(), c->lpTidx.as<int>()));
+                           c->lpRlogq.as<float>(), c->lpTidx.as<int>(), c->lpSpread.as<float>()));
     THX_TRY(record(c, "select"));
     THX_TRY(score_selected(c, P, c->lpRidx.as<int>(), c->lpTidx.as<int>(), nSelR, nSelT, nullptr));
     THX_TRY(c->lpESS.ensure(sizeof(double) * P));
-    THX_TRY(launch_ess_sel(st, c->locWR.as<double>(), c->lpRlogq.as<float>(), nSelR, P,
-                           c->lpESS.as<double>()));
+    THX_TRY(c->lpTail.ensure(sizeof(float) * P));
+    THX_TRY(launch_sel_stats(st, c->locWR.as<double>(), c->lpRlogq.as<float>(),
+                             c->lpRidx.as<int>(), c->lpTidx.as<int>(), c->locBestR.as<int>(),
+                             c->locBestT.as<int>(), nSelR, nSelT, P, c->lpESS.as<double>(),
+                             c->lpTail.as<float>()));
     THX_TRY(record(c, "score_sel"));
-    // results in grid indices
-    std::vector<int> bR(P), bT(P), ridx((size_t)P * nSelR), tidx((size_t)P * nSelT);
+    // best pose in grid indices (mapped by the stats kernel)
+    std::vector<int> bR(P), bT(P);
+    std::vector<float> spread(P), tail(P);
     c->lpScore.assign(P, 0.0);
     c->lpEssH.assign(P, 0.0);
     THX_HIP(hipStreamSynchronize(st));
@@ -741,28 +781,44 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     THX_HIP(hipMemcpy(bT.data(), c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost));
     THX_HIP(hipMemcpy(c->lpScore.data(), c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost));
     THX_HIP(hipMemcpy(c->lpEssH.data(), c->lpESS.p, sizeof(double) * P, hipMemcpyDeviceToHost));
-    THX_HIP(hipMemcpy(ridx.data(), c->lpRidx.p, sizeof(int) * ridx.size(), hipMemcpyDeviceToHost));
-    THX_HIP(hipMemcpy(tidx.data(), c->lpTidx.p, sizeof(int) * tidx.size(), hipMemcpyDeviceToHost));
-    c->lpBestR.assign(P, -1);
-    c->lpBestT.assign(P, -1);
+    THX_HIP(hipMemcpy(spread.data(), c->lpSpread.p, sizeof(float) * P, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(tail.data(), c->lpTail.p, sizeof(float) * P, hipMemcpyDeviceToHost));
+    c->lpBestR = bR;
+    c->lpBestT = bT;
     c->lpFell.assign(P, 0);
+    c->lpSpreadH.assign(spread.begin(), spread.end());
+    c->lpTailH.assign(tail.begin(), tail.end());
+    // the fallback rule: collapsed ESS, a diffuse proposal, a best pose in
+    // the proposal's tail, or a best score far below the batch median
+    double med = 0.0;
+    if (scoreDrop > 0.0) {
+        std::vector<double> sc(c->lpScore);
+        std::nth_element(sc.begin(), sc.begin() + P / 2, sc.end());
+        med = sc[P / 2];
+    }
     std::vector<int> flagged;
     for (int p = 0; p < P; p++) {
-        if (bR[p] >= 0) c->lpBestR[p] = ridx[(size_t)p * nSelR + bR[p]];
-        if (bT[p] >= 0) c->lpBestT[p] = tidx[(size_t)p * nSelT + bT[p]];
-        if (!(c->lpEssH[p] >= essMin)) flagged.push_back(p);
+        const bool fb = c->lpBestR[p] < 0 || c->lpBestT[p] < 0 ||
+                        (essMin > 0.0 && !(c->lpEssH[p] >= essMin)) ||
+                        (spreadMax > 0.0 && !(spread[p] <= spreadMax)) ||
+                        (tailMax > 0.0 && !(tail[p] <= tailMax)) ||
+                        (scoreDrop > 0.0 && !(c->lpScore[p] >= med - scoreDrop));
+        if (fb) flagged.push_back(p);
     }
-    // fallback: uniform search over the whole grid for collapsed particles
+    // uniform whole-grid search of the flagged particles (tiled scan)
     const int nf = (int)flagged.size();
     if (nf > 0) {
-        THX_TRY(upload(c->lpPmap, flagged.data(), nf, st));
-        THX_TRY(score_selected(c, nf, nullptr, nullptr, c->R, c->T, c->lpPmap.as<int>()));
+        const int nev0 = c->nev;
+        THX_TRY(scan_subset(c, cls, fallbackPrecision, flagged));
+        c->nev = nev0;   // keep the stage events of this call
+        c->evNames.resize(nev0);
         std::vector<int> fR(nf), fT(nf);
         std::vector<double> fS(nf);
         THX_HIP(hipStreamSynchronize(st));
-        THX_HIP(hipMemcpy(fR.data(), c->locBestR.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
-        THX_HIP(hipMemcpy(fT.data(), c->locBestT.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
-        THX_HIP(hipMemcpy(fS.data(), c->locBase.p, sizeof(double) * nf, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(fR.data(), c->lpRes.bestR.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(fT.data(), c->lpRes.bestT.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(fS.data(), c->lpRes.bestScore.p, sizeof(double) * nf,
+                          hipMemcpyDeviceToHost));
         for (int i = 0; i < nf; i++) {
             const int p = flagged[i];
             c->lpBestR[p] = fR[i];
@@ -780,7 +836,7 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
 }
 
 int thx_ctx_get_learned(thx_ctx* c, int* bestR, int* bestT, double* score, double* ess,
-                        int* fellBack) {
+                        float* spread, float* tail, int* fellBack) {
     clear_error();
     THX_CHECK(c && c->lpValid, THX_ERR_STATE, "get_learned: no learned expectation done");
     const int P = (int)c->lpBestR.size();
@@ -788,6 +844,10 @@ int thx_ctx_get_learned(thx_ctx* c, int* bestR, int* bestT, double* score, doubl
     if (bestT) std::memcpy(bestT, c->lpBestT.data(), sizeof(int) * P);
     if (score) std::memcpy(score, c->lpScore.data(), sizeof(double) * P);
     if (ess) std::memcpy(ess, c->lpEssH.data(), sizeof(double) * P);
+    for (int p = 0; p < P; p++) {
+        if (spread) spread[p] = (float)c->lpSpreadH[p];
+        if (tail) tail[p] = (float)c->lpTailH[p];
+    }
     if (fellBack) std::memcpy(fellBack, c->lpFell.data(), sizeof(int) * P);
     return THX_OK;
 }

@@ -28,7 +28,8 @@
 namespace thx {
 
 constexpr int kImg = 64;
-constexpr int kMaxBatch = 512;
+constexpr int kMaxBatch = 512;     // training minibatch limit
+constexpr int kInferBatch = 4096;  // images per inference pass
 
 struct ConvShape {
     int ci, hi, co, ho, ks, st, pd;
@@ -40,8 +41,11 @@ constexpr int kFcIn = 128 * 4 * 4, kFcHid = 384;
 
 // ---------------------------------------------------------------------------
 // Images: 64 x 64 inverse DFT of the phase-flipped coefficients with
-// |iCol|, |iRow| < 32 (Hermitian pairs included), then normalisation.
-// One block per particle; the twiddle exp(2 pi i n / 64) is a table.
+// 0 <= iCol < 32 and -32 <= iRow < 32 (each standing for itself and its
+// Hermitian partner), then normalisation.  One block per particle; the
+// transform is separable: H[j][x] = sum_i G[j][i] e^{2 pi i i x / 64} over
+// the 32 columns, then img[y][x] = Re sum_j H[j][x] e^{2 pi i j y / 64} over
+// the 64 rows, both in LDS with a 64-entry twiddle table.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_images(const float2* __restrict__ dat,
                                                 const float* __restrict__ ctf,
@@ -49,57 +53,70 @@ __global__ __launch_bounds__(256) void k_images(const float2* __restrict__ dat,
                                                 const int* __restrict__ iRow, int K, int P,
                                                 float* __restrict__ img) {
     const int p = blockIdx.x;
-    __shared__ float2 coef[2048];
-    __shared__ int ci[2048], cj[2048];
-    __shared__ float tw[2][64];
-    __shared__ int nc;
+    __shared__ float2 G[64 * 33];    // [row j + 32][col i], padded
+    __shared__ float2 H[64 * 65];    // [row j + 32][x], padded
+    __shared__ float2 tw[64];
     __shared__ float red[2][256];
-    if (threadIdx.x < 64) {
-        float s, c;
-        sincospif(threadIdx.x / 32.f, &s, &c);
-        tw[0][threadIdx.x] = c;
-        tw[1][threadIdx.x] = s;
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        float sn, cs;
+        sincospif(tid / 32.f, &sn, &cs);
+        tw[tid] = make_float2(cs, sn);
     }
-    if (threadIdx.x == 0) nc = 0;
+    for (int e = tid; e < 64 * 33; e += 256) G[e] = make_float2(0.f, 0.f);
     __syncthreads();
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    for (int k = tid; k < K; k += 256) {
         const int i = iCol[k], j = iRow[k];
         if (i < 32 && j >= -32 && j < 32 && !(i == 0 && j == 0)) {
-            const int slot = atomicAdd(&nc, 1);
-            if (slot < 2048) {
-                const float2 x = dat[(size_t)p * K + k];
-                const float sg = ctf[(size_t)p * K + k] >= 0.f ? 1.f : -1.f;
-                // a half-plane coefficient stands for itself and its conjugate
-                const float f = 2.f;
-                coef[slot] = make_float2(f * sg * x.x, f * sg * x.y);
-                ci[slot] = i;
-                cj[slot] = j;
-            }
+            const float2 x = dat[(size_t)p * K + k];
+            const float f = ctf[(size_t)p * K + k] >= 0.f ? 2.f : -2.f;
+            G[(j + 32) * 33 + i] = make_float2(f * x.x, f * x.y);
         }
     }
     __syncthreads();
-    const int n = min(nc, 2048);
+    // columns: thread owns x = tid & 63 and rows (tid >> 6) + 4 q
+    {
+        const int x = tid & 63;
+        for (int q = 0; q < 16; q++) {
+            const int jr = (tid >> 6) + 4 * q;
+            float hr = 0.f, hi = 0.f;
+#pragma unroll 8
+            for (int i = 0; i < 32; i++) {
+                const float2 g = G[jr * 33 + i];
+                const float2 w = tw[(i * x) & 63];
+                hr += g.x * w.x - g.y * w.y;
+                hi += g.x * w.y + g.y * w.x;
+            }
+            H[jr * 65 + x] = make_float2(hr, hi);
+        }
+    }
+    __syncthreads();
+    // rows: thread owns x = tid & 63 and y = (tid >> 6) + 4 q
     float v[16];
     float s1 = 0.f, s2 = 0.f;
-    for (int q = 0; q < 16; q++) {
-        const int pix = threadIdx.x + 256 * q;
-        const int y = pix >> 6, x = pix & 63;
-        float acc = 0.f;
-        for (int c = 0; c < n; c++) {
-            const int ph = (ci[c] * x + cj[c] * y) & 63;
-            acc += coef[c].x * tw[0][ph] - coef[c].y * tw[1][ph];
+    {
+        const int x = tid & 63;
+        for (int q = 0; q < 16; q++) {
+            const int y = (tid >> 6) + 4 * q;
+            float acc = 0.f;
+#pragma unroll 8
+            for (int jr = 0; jr < 64; jr++) {
+                const float2 h = H[jr * 65 + x];
+                const float2 w = tw[((jr - 32) * y) & 63];
+                acc += h.x * w.x - h.y * w.y;
+            }
+            v[q] = acc;
+            s1 += acc;
+            s2 += acc * acc;
         }
-        v[q] = acc;
-        s1 += acc;
-        s2 += acc * acc;
     }
-    red[0][threadIdx.x] = s1;
-    red[1][threadIdx.x] = s2;
+    red[0][tid] = s1;
+    red[1][tid] = s2;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + s];
-            red[1][threadIdx.x] += red[1][threadIdx.x + s];
+        if (tid < s) {
+            red[0][tid] += red[0][tid + s];
+            red[1][tid] += red[1][tid + s];
         }
         __syncthreads();
     }
@@ -107,112 +124,214 @@ __global__ __launch_bounds__(256) void k_images(const float2* __restrict__ dat,
     const float var = fmaxf(red[1][0] / 4096.f - mean * mean, 1e-20f);
     const float inv = rsqrtf(var);
     for (int q = 0; q < 16; q++) {
-        const int pix = threadIdx.x + 256 * q;
+        const int pix = ((tid >> 6) + 4 * q) * 64 + (tid & 63);
         img[(size_t)p * 4096 + pix] = (v[q] - mean) * inv;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Layers
+// Layers as GEMMs on FP32 MFMA.
+// C[m][n] = sum_k A(m, k) B(n, k): a block of 256 threads holds WM x WN wave
+// tiles of 32 x 32 (v_mfma_f32_32x32x2f32); k advances 16 at a time through
+// LDS while the next step's operands load into registers.  A is loaded with
+// lanes along k; B with lanes along n (BKF false, for gathers such as
+// im2col) or along k (BKF true, for k-contiguous rows).  blockIdx.z splits
+// the k range (partial sums go to the store functor, which then adds
+// atomically).  Operand functors return A(m, k) / B(n, k) in range only.
 // ---------------------------------------------------------------------------
-__global__ void k_conv_fwd(const float* __restrict__ in, const float* __restrict__ w,
-                           const float* __restrict__ b, float* __restrict__ out, int B, int Ci,
-                           int Hi, int Co, int Ho, int KS, int S, int Pd) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n = (size_t)B * Co * Ho * Ho;
-    if (idx >= n) return;
-    const int ox = idx % Ho, oy = (idx / Ho) % Ho, co = (idx / ((size_t)Ho * Ho)) % Co;
-    const int bb = idx / ((size_t)Ho * Ho * Co);
-    float acc = b[co];
-    for (int c = 0; c < Ci; c++) {
-        const float* ip = in + ((size_t)bb * Ci + c) * Hi * Hi;
-        const float* wp = w + ((size_t)co * Ci + c) * KS * KS;
-        for (int ky = 0; ky < KS; ky++) {
-            const int iy = oy * S - Pd + ky;
-            if (iy < 0 || iy >= Hi) continue;
-            for (int kx = 0; kx < KS; kx++) {
-                const int ix = ox * S - Pd + kx;
-                if (ix < 0 || ix >= Hi) continue;
-                acc += wp[ky * KS + kx] * ip[iy * Hi + ix];
-            }
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+template <int WM, int WN, bool BKF, class LA, class LB, class ST>
+__global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, ST st, int M, int N, int Kd,
+                                              int kPer) {
+    static_assert(WM * WN == 4, "four waves per block");
+    constexpr int BM = 32 * WM, BN = 32 * WN, BK = 16;
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+    __shared__ float As[BK][BM + 1];
+    __shared__ float Bs[BK][BN + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kb = blockIdx.z * kPer, ke = min(Kd, kb + kPer);
+    float ra[NA], rb[NB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + 256 * i, k = k0 + (e & 15), m = m0 + (e >> 4);
+            ra[i] = (m < M && k < ke) ? la(m, k) : 0.f;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int e = tid + 256 * i;
+            const int k = k0 + (BKF ? (e & 15) : e / BN), n = n0 + (BKF ? (e >> 4) : e % BN);
+            rb[i] = (n < N && k < ke) ? lb(n, k) : 0.f;
+        }
+    };
+    floatx16 acc = {0};
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + 256 * i;
+            As[e & 15][e >> 4] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int e = tid + 256 * i;
+            if constexpr (BKF) Bs[e & 15][e >> 4] = rb[i];
+            else Bs[e / BN][e % BN] = rb[i];
+        }
+        __syncthreads();
+        if (k0 + BK < ke) load(k0 + BK);
+#pragma unroll
+        for (int s = 0; s < BK; s += 2) {
+            const float a = As[s + (lane >> 5)][32 * wm + (lane & 31)];
+            const float b = Bs[s + (lane >> 5)][32 * wn + (lane & 31)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
         }
     }
-    out[idx] = fmaxf(acc, 0.f);   // every conv is followed by ReLU
+    const int n = n0 + 32 * wn + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = m0 + 32 * wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (m < M && n < N) st(m, n, acc[r]);
+    }
 }
 
-// gradient w.r.t. the input of a conv (g = dL/d(pre-activation) of the output)
-__global__ void k_conv_bwd_in(const float* __restrict__ g, const float* __restrict__ w,
-                              float* __restrict__ din, int B, int Ci, int Hi, int Co, int Ho,
-                              int KS, int S, int Pd) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t n = (size_t)B * Ci * Hi * Hi;
-    if (idx >= n) return;
-    const int ix = idx % Hi, iy = (idx / Hi) % Hi, c = (idx / ((size_t)Hi * Hi)) % Ci;
-    const int bb = idx / ((size_t)Hi * Hi * Ci);
-    float acc = 0.f;
-    for (int ky = 0; ky < KS; ky++) {
-        const int ty = iy + Pd - ky;
-        if (ty < 0 || ty % S) continue;
-        const int oy = ty / S;
-        if (oy >= Ho) continue;
-        for (int kx = 0; kx < KS; kx++) {
-            const int tx = ix + Pd - kx;
-            if (tx < 0 || tx % S) continue;
-            const int ox = tx / S;
-            if (ox >= Ho) continue;
-            for (int co = 0; co < Co; co++)
-                acc += w[((size_t)co * Ci + c) * KS * KS + ky * KS + kx] *
-                       g[(((size_t)bb * Co + co) * Ho + oy) * Ho + ox];
-        }
-    }
-    din[idx] = acc;
-}
+// row-major operand: X(r, k) = p[r * ld + k]
+struct RowOp {
+    const float* p;
+    int ld;
+    __device__ float operator()(int r, int k) const { return p[(size_t)r * ld + k]; }
+};
+// transposed operand: X(r, k) = p[k * ld + r]
+struct ColOp {
+    const float* p;
+    int ld;
+    __device__ float operator()(int r, int k) const { return p[(size_t)k * ld + r]; }
+};
 
-// gradient w.r.t. weights: one block per weight, reduction over batch x output
-__global__ __launch_bounds__(256) void k_conv_bwd_w(const float* __restrict__ g,
-                                                    const float* __restrict__ in,
-                                                    float* __restrict__ dw, int B, int Ci,
-                                                    int Hi, int Co, int Ho, int KS, int S,
-                                                    int Pd) {
-    const int widx = blockIdx.x;
-    const int kx = widx % KS, ky = (widx / KS) % KS, c = (widx / (KS * KS)) % Ci;
-    const int co = widx / (KS * KS * Ci);
-    const int per = Ho * Ho;
-    float acc = 0.f;
-    for (int t = threadIdx.x; t < B * per; t += blockDim.x) {
-        const int bb = t / per, o = t % per, oy = o / Ho, ox = o % Ho;
-        const int iy = oy * S - Pd + ky, ix = ox * S - Pd + kx;
-        if (iy < 0 || iy >= Hi || ix < 0 || ix >= Hi) continue;
-        acc += g[(((size_t)bb * Co + co) * Ho + oy) * Ho + ox] *
-               in[(((size_t)bb * Ci + c) * Hi + iy) * Hi + ix];
-    }
-    __shared__ float red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) dw[widx] = red[0];
-}
+template <int L> struct Geo;
+template <> struct Geo<0> { static constexpr int CI = 1, HI = 64, CO = 16, HO = 32, KS = 5, S = 2, PD = 2; };
+template <> struct Geo<1> { static constexpr int CI = 16, HI = 32, CO = 32, HO = 16, KS = 3, S = 2, PD = 1; };
+template <> struct Geo<2> { static constexpr int CI = 32, HI = 16, CO = 64, HO = 8, KS = 3, S = 2, PD = 1; };
+template <> struct Geo<3> { static constexpr int CI = 64, HI = 8, CO = 128, HO = 4, KS = 3, S = 2, PD = 1; };
 
-// bias gradient: sum of g over batch and positions, one block per channel
-__global__ __launch_bounds__(256) void k_bias_grad(const float* __restrict__ g,
-                                                   float* __restrict__ db, int B, int C, int HW) {
-    const int c = blockIdx.x;
-    float acc = 0.f;
-    for (int t = threadIdx.x; t < B * HW; t += blockDim.x) {
-        const int bb = t / HW, o = t % HW;
-        acc += g[((size_t)bb * C + c) * HW + o];
+// im2col of a conv input: X(n, k) with n = (b, oy, ox), k = (c, ky, kx)
+template <int L> struct Im2col {
+    using G = Geo<L>;
+    const float* in;
+    __device__ float operator()(int n, int k) const {
+        constexpr int HW = G::HO * G::HO, KK = G::KS * G::KS;
+        const int b = n / HW, pos = n % HW, oy = pos / G::HO, ox = pos % G::HO;
+        const int c = k / KK, r = k % KK, ky = r / G::KS, kx = r % G::KS;
+        const int iy = oy * G::S - G::PD + ky, ix = ox * G::S - G::PD + kx;
+        if (iy < 0 || iy >= G::HI || ix < 0 || ix >= G::HI) return 0.f;
+        return in[((size_t)(b * G::CI + c) * G::HI + iy) * G::HI + ix];
     }
-    __shared__ float red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+};
+// im2col with the two indices swapped (weight-gradient GEMM: n = k_conv,
+// k = (b, pos)), plus a column of ones at n = CI KS KS for the bias gradient
+template <int L> struct Im2colT {
+    Im2col<L> x;
+    __device__ float operator()(int n, int k) const {
+        return n == Geo<L>::CI * Geo<L>::KS * Geo<L>::KS ? 1.f : x(k, n);
     }
-    if (threadIdx.x == 0) db[c] = red[0];
+};
+// output gradient as the A operand of the weight gradient: G(co, (b, pos))
+template <int L> struct GradRows {
+    using G = Geo<L>;
+    const float* g;
+    __device__ float operator()(int co, int k) const {
+        constexpr int HW = G::HO * G::HO;
+        const int b = k / HW, pos = k % HW;
+        return g[((size_t)b * G::CO + co) * HW + pos];
+    }
+};
+// transposed-conv gather for the input gradient: X(n, k) with
+// n = (b, iy, ix) and k = (co, ky, kx); zero where no output tap lands
+template <int L> struct GradCols {
+    using G = Geo<L>;
+    const float* g;
+    __device__ float operator()(int n, int k) const {
+        constexpr int HWI = G::HI * G::HI, KK = G::KS * G::KS;
+        const int b = n / HWI, pos = n % HWI, iy = pos / G::HI, ix = pos % G::HI;
+        const int co = k / KK, r = k % KK, ky = r / G::KS, kx = r % G::KS;
+        const int ty = iy + G::PD - ky, tx = ix + G::PD - kx;
+        if (ty < 0 || tx < 0 || ty % G::S || tx % G::S) return 0.f;
+        const int oy = ty / G::S, ox = tx / G::S;
+        if (oy >= G::HO || ox >= G::HO) return 0.f;
+        return g[((size_t)(b * G::CO + co) * G::HO + oy) * G::HO + ox];
+    }
+};
+// weights as the A operand of the input gradient: W(c, (co, ky, kx))
+template <int L> struct WeightT {
+    using G = Geo<L>;
+    const float* w;
+    __device__ float operator()(int c, int k) const {
+        constexpr int KK = G::KS * G::KS;
+        const int co = k / KK, r = k % KK;
+        return w[((size_t)co * G::CI + c) * KK + r];
+    }
+};
+
+// conv output (m = co, n = (b, pos)) + bias, ReLU
+template <int L> struct ConvStore {
+    using G = Geo<L>;
+    float* out;
+    const float* bias;
+    __device__ void operator()(int m, int n, float v) const {
+        constexpr int HW = G::HO * G::HO;
+        const int b = n / HW, pos = n % HW;
+        out[((size_t)b * G::CO + m) * HW + pos] = fmaxf(v + bias[m], 0.f);
+    }
+};
+// input gradient store (m = c, n = (b, pos))
+template <int L> struct GradInStore {
+    using G = Geo<L>;
+    float* din;
+    __device__ void operator()(int m, int n, float v) const {
+        constexpr int HW = G::HI * G::HI;
+        const int b = n / HW, pos = n % HW;
+        din[((size_t)b * G::CI + m) * HW + pos] = v;
+    }
+};
+// fully-connected output y[m][n] (+ bias[n], optional ReLU)
+struct FcStore {
+    float* y;
+    const float* bias;
+    int O, relu;
+    __device__ void operator()(int m, int n, float v) const {
+        v += bias[n];
+        y[(size_t)m * O + n] = relu ? fmaxf(v, 0.f) : v;
+    }
+};
+// plain store y[m][n] = v (ld O)
+struct PlainStore {
+    float* y;
+    int O;
+    __device__ void operator()(int m, int n, float v) const { y[(size_t)m * O + n] = v; }
+};
+// weight and bias gradient partial sums: column KC of the GEMM is the bias
+struct WgradStore {
+    float* dw;
+    float* db;
+    int KC;
+    __device__ void operator()(int m, int n, float v) const {
+        atomicAdd(n == KC ? db + m : dw + (size_t)m * KC + n, v);
+    }
+};
+template <int WM, int WN, bool BKF, class LA, class LB, class ST>
+static int gemm(hipStream_t st, LA la, LB lb, ST sto, int M, int N, int Kd, int splits = 1) {
+    constexpr int BM = 32 * WM, BN = 32 * WN;
+    const int kPer = ((Kd + splits - 1) / splits + 15) / 16 * 16;
+    const int nz = (Kd + kPer - 1) / kPer;
+    hipLaunchKernelGGL((k_gemm<WM, WN, BKF, LA, LB, ST>),
+                       dim3((N + BN - 1) / BN, (M + BM - 1) / BM, nz), dim3(256), 0, st, la, lb,
+                       sto, M, N, Kd, kPer);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
 }
 
 // g *= (act > 0): back through a ReLU whose output is act
@@ -221,45 +340,13 @@ __global__ void k_relu_back(float* __restrict__ g, const float* __restrict__ act
     if (i < n && act[i] <= 0.f) g[i] = 0.f;
 }
 
-// y[b][o] = sum_i W[o][i] x[b][i] + bias[o], optional ReLU
-__global__ void k_fc_fwd(const float* __restrict__ x, const float* __restrict__ W,
-                         const float* __restrict__ bias, float* __restrict__ y, int B, int I,
-                         int O, int relu) {
+// db[o] = sum_b g[b][o]
+__global__ void k_col_sum(const float* __restrict__ g, float* __restrict__ db, int B, int O) {
     const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    const int bb = blockIdx.y;
     if (o >= O) return;
-    const float* xr = x + (size_t)bb * I;
-    const float* wr = W + (size_t)o * I;
-    float acc = bias[o];
-    for (int i = 0; i < I; i++) acc += wr[i] * xr[i];
-    y[(size_t)bb * O + o] = relu ? fmaxf(acc, 0.f) : acc;
-}
-
-// dx[b][i] = sum_o W[o][i] g[b][o]
-__global__ void k_fc_bwd_x(const float* __restrict__ g, const float* __restrict__ W,
-                           float* __restrict__ dx, int B, int I, int O) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int bb = blockIdx.y;
-    if (i >= I) return;
-    float acc = 0.f;
-    for (int o = 0; o < O; o++) acc += W[(size_t)o * I + i] * g[(size_t)bb * O + o];
-    dx[(size_t)bb * I + i] = acc;
-}
-
-// dW[o][i] = sum_b g[b][o] x[b][i]; db[o] = sum_b g[b][o]
-__global__ void k_fc_bwd_w(const float* __restrict__ g, const float* __restrict__ x,
-                           float* __restrict__ dW, float* __restrict__ db, int B, int I, int O) {
-    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (size_t)O * I) return;
-    const int o = idx / I, i = idx % I;
-    float acc = 0.f;
-    for (int bb = 0; bb < B; bb++) acc += g[(size_t)bb * O + o] * x[(size_t)bb * I + i];
-    dW[idx] = acc;
-    if (i == 0) {
-        float s = 0.f;
-        for (int bb = 0; bb < B; bb++) s += g[(size_t)bb * O + o];
-        db[o] = s;
-    }
+    float s = 0.f;
+    for (int bb = 0; bb < B; bb++) s += g[(size_t)bb * O + o];
+    db[o] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,65 +526,97 @@ static int cnn_alloc(thx_cnn* n) {
     return THX_OK;
 }
 
+// activation buffers for forward-only passes of up to B images
+static int ensure_infer(thx_cnn* n, int B) {
+    const size_t sz[7] = {(size_t)kImg * kImg, 16 * 32 * 32, 32 * 16 * 16, 64 * 8 * 8,
+                          128 * 4 * 4, kFcHid, (size_t)n->NOUT};
+    for (int i = 1; i < 7; i++) THX_TRY(n->act[i].ensure(sizeof(float) * sz[i] * B));
+    THX_TRY(n->mix.ensure(sizeof(double) * n->NOUT * B));
+    return THX_OK;
+}
+
 static float* P_(thx_cnn* n, int i) { return n->params.as<float>() + n->off[i]; }
 static float* G_(thx_cnn* n, int i) { return n->grads.as<float>() + n->off[i]; }
 
 static unsigned nblk(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-// forward of B images already in act[0]
-static int cnn_forward(thx_cnn* n, int B) {
+template <int L, int WM, int WN>
+static int conv_fwd(hipStream_t st, const float* in, const float* w, const float* bias, float* out,
+                    int B) {
+    using G = Geo<L>;
+    return gemm<WM, WN, false>(st, RowOp{w, G::CI * G::KS * G::KS}, Im2col<L>{in},
+                               ConvStore<L>{out, bias}, G::CO, B * G::HO * G::HO,
+                               G::CI * G::KS * G::KS);
+}
+
+// forward of B images at `in` (act[0] when null)
+static int cnn_forward(thx_cnn* n, int B, const float* in = nullptr) {
     hipStream_t st = n->stream;
-    for (int l = 0; l < 4; l++) {
-        const ConvShape& c = kConv[l];
-        const size_t no = (size_t)B * c.co * c.ho * c.ho;
-        hipLaunchKernelGGL(k_conv_fwd, dim3(nblk(no)), dim3(256), 0, st, n->act[l].as<float>(),
-                           P_(n, 2 * l), P_(n, 2 * l + 1), n->act[l + 1].as<float>(), B, c.ci, c.hi,
-                           c.co, c.ho, c.ks, c.st, c.pd);
-    }
-    hipLaunchKernelGGL(k_fc_fwd, dim3(nblk(kFcHid, 128), B), dim3(128), 0, st,
-                       n->act[4].as<float>(), P_(n, 8), P_(n, 9), n->act[5].as<float>(), B, kFcIn,
-                       kFcHid, 1);
-    hipLaunchKernelGGL(k_fc_fwd, dim3(nblk(n->NOUT, 64), B), dim3(64), 0, st,
-                       n->act[5].as<float>(), P_(n, 10), P_(n, 11), n->act[6].as<float>(), B,
-                       kFcHid, n->NOUT, 0);
-    THX_HIP(hipGetLastError());
+    const float* x = in ? in : n->act[0].as<const float>();
+    THX_TRY((conv_fwd<0, 1, 4>(st, x, P_(n, 0), P_(n, 1), n->act[1].as<float>(), B)));
+    THX_TRY((conv_fwd<1, 1, 4>(st, n->act[1].as<float>(), P_(n, 2), P_(n, 3), n->act[2].as<float>(), B)));
+    THX_TRY((conv_fwd<2, 2, 2>(st, n->act[2].as<float>(), P_(n, 4), P_(n, 5), n->act[3].as<float>(), B)));
+    THX_TRY((conv_fwd<3, 2, 2>(st, n->act[3].as<float>(), P_(n, 6), P_(n, 7), n->act[4].as<float>(), B)));
+    THX_TRY((gemm<2, 2, true>(st, RowOp{n->act[4].as<float>(), kFcIn}, RowOp{P_(n, 8), kFcIn},
+                              FcStore{n->act[5].as<float>(), P_(n, 9), kFcHid, 1}, B, kFcHid,
+                              kFcIn)));
+    THX_TRY((gemm<2, 2, true>(st, RowOp{n->act[5].as<float>(), kFcHid}, RowOp{P_(n, 10), kFcHid},
+                              FcStore{n->act[6].as<float>(), P_(n, 11), n->NOUT, 0}, B, n->NOUT,
+                              kFcHid)));
+    return THX_OK;
+}
+
+template <int L>
+static int conv_bwd(thx_cnn* n, int B, const float* in) {
+    using G = Geo<L>;
+    hipStream_t st = n->stream;
+    constexpr int KC = G::CI * G::KS * G::KS, HW = G::HO * G::HO;
+    float* g = n->gact[L + 1].as<float>();
+    const size_t no = (size_t)B * G::CO * HW;
+    hipLaunchKernelGGL(k_relu_back, dim3(nblk(no)), dim3(256), 0, st, g,
+                       n->act[L + 1].as<const float>(), no);
+    // dW[co][k] = sum over (b, pos) of g[co] im2col[k] and db[co] = sum of g[co]
+    // (the extra ones column), split over the long reduction; the bias
+    // follows the weights in the parameter vector, so one clear covers both
+    THX_HIP(hipMemsetAsync(G_(n, 2 * L), 0, sizeof(float) * G::CO * (KC + 1), st));
+    const int red = B * HW;
+    const int splits = std::max(1, std::min(256, red / 1024));
+    THX_TRY((gemm<1, 4, false>(st, GradRows<L>{g}, Im2colT<L>{Im2col<L>{in}},
+                               WgradStore{G_(n, 2 * L), G_(n, 2 * L + 1), KC}, G::CO, KC + 1, red,
+                               splits)));
+    if (L > 0)
+        THX_TRY((gemm<1, 4, false>(st, WeightT<L>{P_(n, 2 * L)}, GradCols<L>{g},
+                                   GradInStore<L>{n->gact[L].as<float>()}, G::CI,
+                                   B * G::HI * G::HI, G::CO * G::KS * G::KS)));
     return THX_OK;
 }
 
 // backward from gact[6] (dL/dout) to parameter gradients
-static int cnn_backward(thx_cnn* n, int B) {
+static int cnn_backward(thx_cnn* n, int B, const float* in = nullptr) {
     hipStream_t st = n->stream;
-    hipLaunchKernelGGL(k_fc_bwd_w, dim3(nblk((size_t)n->NOUT * kFcHid)), dim3(256), 0, st,
-                       n->gact[6].as<float>(), n->act[5].as<float>(), G_(n, 10), G_(n, 11), B,
-                       kFcHid, n->NOUT);
-    hipLaunchKernelGGL(k_fc_bwd_x, dim3(nblk(kFcHid, 128), B), dim3(128), 0, st,
-                       n->gact[6].as<float>(), P_(n, 10), n->gact[5].as<float>(), B, kFcHid,
-                       n->NOUT);
+    const int O = n->NOUT;
+    // fc2: dW6[o][i] = sum_b g[b][o] z[b][i]; db6; dz = g W6
+    THX_TRY((gemm<1, 4, false>(st, ColOp{n->gact[6].as<float>(), O}, ColOp{n->act[5].as<float>(), kFcHid},
+                               PlainStore{G_(n, 10), kFcHid}, O, kFcHid, B)));
+    hipLaunchKernelGGL(k_col_sum, dim3(nblk(O, 64)), dim3(64), 0, st, n->gact[6].as<float>(),
+                       G_(n, 11), B, O);
+    THX_TRY((gemm<2, 2, false>(st, RowOp{n->gact[6].as<float>(), O}, ColOp{P_(n, 10), kFcHid},
+                               PlainStore{n->gact[5].as<float>(), kFcHid}, B, kFcHid, O)));
     hipLaunchKernelGGL(k_relu_back, dim3(nblk((size_t)B * kFcHid)), dim3(256), 0, st,
                        n->gact[5].as<float>(), n->act[5].as<const float>(), (size_t)B * kFcHid);
-    hipLaunchKernelGGL(k_fc_bwd_w, dim3(nblk((size_t)kFcHid * kFcIn)), dim3(256), 0, st,
-                       n->gact[5].as<float>(), n->act[4].as<float>(), G_(n, 8), G_(n, 9), B,
-                       kFcIn, kFcHid);
-    hipLaunchKernelGGL(k_fc_bwd_x, dim3(nblk(kFcIn, 128), B), dim3(128), 0, st,
-                       n->gact[5].as<float>(), P_(n, 8), n->gact[4].as<float>(), B, kFcIn,
-                       kFcHid);
-    for (int l = 3; l >= 0; l--) {
-        const ConvShape& c = kConv[l];
-        const size_t no = (size_t)B * c.co * c.ho * c.ho;
-        hipLaunchKernelGGL(k_relu_back, dim3(nblk(no)), dim3(256), 0, st,
-                           n->gact[l + 1].as<float>(), n->act[l + 1].as<const float>(), no);
-        hipLaunchKernelGGL(k_conv_bwd_w, dim3((unsigned)conv_w(l)), dim3(256), 0, st,
-                           n->gact[l + 1].as<float>(), n->act[l].as<float>(), G_(n, 2 * l), B,
-                           c.ci, c.hi, c.co, c.ho, c.ks, c.st, c.pd);
-        hipLaunchKernelGGL(k_bias_grad, dim3(c.co), dim3(256), 0, st, n->gact[l + 1].as<float>(),
-                           G_(n, 2 * l + 1), B, c.co, c.ho * c.ho);
-        if (l > 0) {
-            const size_t ni = (size_t)B * c.ci * c.hi * c.hi;
-            hipLaunchKernelGGL(k_conv_bwd_in, dim3(nblk(ni)), dim3(256), 0, st,
-                               n->gact[l + 1].as<float>(), P_(n, 2 * l), n->gact[l].as<float>(),
-                               B, c.ci, c.hi, c.co, c.ho, c.ks, c.st, c.pd);
-        }
-    }
+    // fc1
+    THX_TRY((gemm<2, 2, false>(st, ColOp{n->gact[5].as<float>(), kFcHid},
+                               ColOp{n->act[4].as<float>(), kFcIn}, PlainStore{G_(n, 8), kFcIn},
+                               kFcHid, kFcIn, B)));
+    hipLaunchKernelGGL(k_col_sum, dim3(nblk(kFcHid, 64)), dim3(64), 0, st, n->gact[5].as<float>(),
+                       G_(n, 9), B, kFcHid);
+    THX_TRY((gemm<2, 2, false>(st, RowOp{n->gact[5].as<float>(), kFcHid}, ColOp{P_(n, 8), kFcIn},
+                               PlainStore{n->gact[4].as<float>(), kFcIn}, B, kFcIn, kFcHid)));
+    const float* x0 = in ? in : n->act[0].as<const float>();
+    THX_TRY(conv_bwd<3>(n, B, n->act[3].as<float>()));
+    THX_TRY(conv_bwd<2>(n, B, n->act[2].as<float>()));
+    THX_TRY(conv_bwd<1>(n, B, n->act[1].as<float>()));
+    THX_TRY(conv_bwd<0>(n, B, x0));
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
@@ -601,10 +720,8 @@ static int cnn_run(thx_cnn* n, const float* images, const float* dimages, int B,
     if (images)
         THX_HIP(hipMemcpyAsync(n->act[0].p, images, sizeof(float) * B * 4096, hipMemcpyHostToDevice,
                                st));
-    else
-        THX_HIP(hipMemcpyAsync(n->act[0].p, dimages, sizeof(float) * B * 4096,
-                               hipMemcpyDeviceToDevice, st));
-    THX_TRY(cnn_forward(n, B));
+    const float* in = images ? n->act[0].as<const float>() : dimages;
+    THX_TRY(cnn_forward(n, B, in));
     if (outHost) {
         THX_HIP(hipStreamSynchronize(st));
         THX_HIP(hipMemcpy(outHost, n->act[6].p, sizeof(float) * B * n->NOUT, hipMemcpyDeviceToHost));
@@ -617,7 +734,7 @@ static int cnn_run(thx_cnn* n, const float* images, const float* dimages, int B,
                        n->loss.as<float>(), n->gact[6].as<float>());
     THX_HIP(hipGetLastError());
     if (grad || lr > 0.f) {
-        THX_TRY(cnn_backward(n, B));
+        THX_TRY(cnn_backward(n, B, in));
         if (lr > 0.f) {
             n->step++;
             const float b1 = 0.9f, b2 = 0.999f;
@@ -700,10 +817,11 @@ int thx_cnn_propose_ctx(thx_cnn* n, thx_ctx* c, double* mixHost, double* mixDev)
     THX_CHECK(n && c && (mixHost || mixDev), THX_ERR_ARG, "cnn_propose_ctx: bad argument");
     THX_TRY(thx_ctx_images(c, nullptr));
     THX_HIP(hipStreamSynchronize(c->stream));
-    for (int b0 = 0; b0 < c->P; b0 += kMaxBatch) {
-        const int B = std::min(kMaxBatch, c->P - b0);
-        THX_TRY(cnn_run(n, nullptr, c->img.as<float>() + (size_t)b0 * 4096, B, nullptr, nullptr,
-                        nullptr, nullptr, 0.f, nullptr));
+    const int BI = std::min(kInferBatch, c->P);
+    THX_TRY(ensure_infer(n, BI));
+    for (int b0 = 0; b0 < c->P; b0 += BI) {
+        const int B = std::min(BI, c->P - b0);
+        THX_TRY(cnn_forward(n, B, c->img.as<float>() + (size_t)b0 * 4096));
         hipLaunchKernelGGL(k_head, dim3(nblk(B, 64)), dim3(64), 0, n->stream,
                            n->act[6].as<const float>(), B, n->C, n->mix.as<double>());
         THX_HIP(hipGetLastError());

@@ -132,9 +132,11 @@ struct thx_ctx {
     // 64 x 64 particle images of the learned proposal
     thx::DevBuf img;
     // grid-restricted learned proposal: mixture, selections, results
-    thx::DevBuf lpMix, lpRidx, lpRlogq, lpTidx, lpESS, lpPmap, lpFullR, lpFullT;
+    thx::DevBuf lpMix, lpRidx, lpRlogq, lpTidx, lpESS, lpSpread, lpTail, lpPmap;
+    thx::DevBuf lpDat, lpCtf, lpSig;      // gathered rows of fallen-back particles
+    thx::ClassResult lpRes;               // their whole-grid scan
     std::vector<int> lpBestR, lpBestT, lpFell;
-    std::vector<double> lpScore, lpEssH;
+    std::vector<double> lpScore, lpEssH, lpSpreadH, lpTailH;
     bool lpValid = false;
     // timing
     hipEvent_t ev[16] = {};
@@ -186,11 +188,14 @@ int pf_rot_mean(hipStream_t st, const double* quat, const double* w, const int* 
 constexpr int kResampleMaxN = 8192;
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
                    const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
-                   int* tidx);
+                   int* tidx, float* spread);
 int launch_score_sel(thx_ctx* c, int TPAD, int nblk, const int* ridx, const int* tidx, int nSelR,
                      int nSelT, float pRu, float pTu, const int* pmap, const LocalBatch& b);
-int launch_ess_sel(hipStream_t st, const double* wR, const float* rlogq, int nSel, int P,
-                   double* ess);
+int launch_sel_stats(hipStream_t st, const double* wR, const float* rlogq, const int* ridx,
+                     const int* tidx, int* bestR, int* bestT, int nSelR, int nSelT, int P,
+                     double* ess, float* tail);
+int launch_gather_particles(thx_ctx* c, const int* idx, int n, float2* dat2, float* ctf2,
+                            float* sig2);
 
 // Philox4x32-10 (Salmon et al., SC'11), shared by host and device code.
 struct Philox {

@@ -1492,24 +1492,64 @@ __global__ __launch_bounds__(256) void k_merge_local(
 
 // importance-weight ESS per particle over its selected rotations:
 // w_r = wR_r / q_r (translation-marginal likelihood over the proposal density)
-__global__ void k_ess_sel(const double* __restrict__ wR, const float* __restrict__ rlogq, int nSel,
-                          int P, double* __restrict__ ess) {
+// Per particle over its scored selection: the effective sample size of the
+// importance weights w_r = wR_r / q_r, (sum w)^2 / sum w^2, the tail depth of
+// the best rotation, max_r log q_r - log q_best, and the best selected
+// rotation / translation mapped to grid indices (in place).
+__global__ void k_sel_stats(const double* __restrict__ wR, const float* __restrict__ rlogq,
+                            const int* __restrict__ ridx, const int* __restrict__ tidx,
+                            int* __restrict__ bestR, int* __restrict__ bestT, int nSelR,
+                            int nSelT, int P, double* __restrict__ ess, float* __restrict__ tail) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
+    const float* lq = rlogq + (size_t)p * nSelR;
     double mx = -INFINITY;
-    for (int r = 0; r < nSel; r++) mx = fmax(mx, -(double)rlogq[(size_t)p * nSel + r]);
+    float lqmax = -INFINITY;
+    for (int r = 0; r < nSelR; r++) {
+        mx = fmax(mx, -(double)lq[r]);
+        lqmax = fmaxf(lqmax, lq[r]);
+    }
     double s1 = 0.0, s2 = 0.0;
-    for (int r = 0; r < nSel; r++) {
-        const double w = wR[(size_t)p * nSel + r] * exp(-(double)rlogq[(size_t)p * nSel + r] - mx);
+    for (int r = 0; r < nSelR; r++) {
+        const double w = wR[(size_t)p * nSelR + r] * exp(-(double)lq[r] - mx);
         s1 += w;
         s2 += w * w;
     }
     ess[p] = s2 > 0.0 ? s1 * s1 / s2 : 0.0;
+    const int b = bestR[p], bt = bestT[p];
+    tail[p] = b >= 0 ? lqmax - lq[b] : INFINITY;
+    bestR[p] = b >= 0 ? ridx[(size_t)p * nSelR + b] : -1;
+    bestT[p] = bt >= 0 ? tidx[(size_t)p * nSelT + bt] : -1;
 }
 
-int launch_ess_sel(hipStream_t st, const double* wR, const float* rlogq, int nSel, int P,
-                   double* ess) {
-    hipLaunchKernelGGL(k_ess_sel, dim3((P + 127) / 128), dim3(128), 0, st, wR, rlogq, nSel, P, ess);
+int launch_sel_stats(hipStream_t st, const double* wR, const float* rlogq, const int* ridx,
+                     const int* tidx, int* bestR, int* bestT, int nSelR, int nSelT, int P,
+                     double* ess, float* tail) {
+    hipLaunchKernelGGL(k_sel_stats, dim3((P + 127) / 128), dim3(128), 0, st, wR, rlogq, ridx,
+                       tidx, bestR, bestT, nSelR, nSelT, P, ess, tail);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// Rows idx[i] of the particle inputs (data, CTF, sigma) copied to row i.
+__global__ __launch_bounds__(256) void k_gather_particles(
+    const float2* __restrict__ dat, const float* __restrict__ ctf, const float* __restrict__ sig,
+    const int* __restrict__ idx, int K, float2* __restrict__ dat2, float* __restrict__ ctf2,
+    float* __restrict__ sig2) {
+    const size_t src = (size_t)idx[blockIdx.x] * K, dst = (size_t)blockIdx.x * K;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        dat2[dst + k] = dat[src + k];
+        ctf2[dst + k] = ctf[src + k];
+        sig2[dst + k] = sig[src + k];
+    }
+}
+
+int launch_gather_particles(thx_ctx* c, const int* idx, int n, float2* dat2, float* ctf2,
+                            float* sig2) {
+    if (n <= 0) return THX_OK;
+    hipLaunchKernelGGL(k_gather_particles, dim3(n), dim3(256), 0, c->stream,
+                       c->dat.as<const float2>(), c->ctf.as<const float>(),
+                       c->sig.as<const float>(), idx, c->K, dat2, ctf2, sig2);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }

@@ -380,7 +380,7 @@ constexpr int kSelMaxR = 16384;
 __global__ __launch_bounds__(256) void k_select_grid(
     const double* __restrict__ mix, int C, const double* __restrict__ gridQ, int R,
     const double* __restrict__ gridT, int T, int nSelR, int nSelT, int* __restrict__ ridx,
-    float* __restrict__ rlogq, int* __restrict__ tidx) {
+    float* __restrict__ rlogq, int* __restrict__ tidx, float* __restrict__ spread) {
     const int p = blockIdx.x;
     const double* mp = mix + (size_t)p * (6 * C + 3);
     __shared__ float v[kSelMaxR];
@@ -411,6 +411,27 @@ __global__ __launch_bounds__(256) void k_select_grid(
     }
     const float hiR = rmx[0], loR = rmn[0];
     __syncthreads();
+    if (spread) {
+        // effective number of grid rotations under q: (sum q)^2 / sum q^2
+        float s1 = 0.f, s2 = 0.f;
+        for (int r = threadIdx.x; r < R; r += blockDim.x) {
+            const float e = __expf(v[r] - hiR);
+            s1 += e;
+            s2 += e * e;
+        }
+        rmx[threadIdx.x] = s1;
+        rmn[threadIdx.x] = s2;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                rmx[threadIdx.x] += rmx[threadIdx.x + s];
+                rmn[threadIdx.x] += rmn[threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) spread[p] = rmx[0] * rmx[0] / rmn[0];
+        __syncthreads();
+    }
     block_topk(v, R, nSelR, ridx + (size_t)p * nSelR, rlogq + (size_t)p * nSelR, loR, hiR, red,
                &cnt);
     // translations: Gaussian log density (the constant does not matter)
@@ -441,9 +462,9 @@ __global__ __launch_bounds__(256) void k_select_grid(
 
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
                    const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
-                   int* tidx) {
+                   int* tidx, float* spread) {
     hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), 0, st, mix, C, gridQ, R, gridT, T, nSelR,
-                       nSelT, ridx, rlogq, tidx);
+                       nSelT, ridx, rlogq, tidx, spread);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }

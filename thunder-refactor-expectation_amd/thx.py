@@ -62,8 +62,8 @@ _SIGS = {
     "thx_cnn_train_ctx": [_p, _p, _p, _p, _i, _i, C.c_float, _p],
     "thx_cnn_propose_ctx": [_p, _p, _p, _p],
     "thx_cnn_components": [_p, _p],
-    "thx_ctx_expect_learned": [_p, _p, _i, _i, _i, _d, _p],
-    "thx_ctx_get_learned": [_p, _p, _p, _p, _p, _p],
+    "thx_ctx_expect_learned": [_p, _p, _i, _i, _i, _d, _d, _d, _d, _i, _p],
+    "thx_ctx_get_learned": [_p, _p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
                                 _u64, _p, _p, _p],
@@ -282,20 +282,26 @@ class Context:
         return wC, wR, wT, base, bR, bT
 
     # ---- learned proposal ----
-    def expect_learned(self, net, cls=0, n_rot=300, n_trans=25, ess_min=0.0):
+    def expect_learned(self, net, cls=0, n_rot=300, n_trans=25, ess_min=0.0, spread_max=0.0,
+                       tail_max=0.0, score_drop=0.0, fallback_precision=BF16):
+        """Learned-proposal expectation; returns the number of particles that
+        fell back to the whole-grid search (thresholds <= 0 disable a test)."""
         nf = C.c_int(0)
         check(load().thx_ctx_expect_learned(self._h, net._h, cls, n_rot, n_trans, ess_min,
-                                            C.byref(nf)), "expect_learned")
+                                            spread_max, tail_max, score_drop,
+                                            fallback_precision, C.byref(nf)), "expect_learned")
         return nf.value
 
     def get_learned(self):
+        """(bestR, bestT, score, ess, spread, tail, fell_back) per particle."""
         P = self.P
         bR, bT = np.zeros(P, np.int32), np.zeros(P, np.int32)
         sc, ess = np.zeros(P), np.zeros(P)
+        spread, tail = np.zeros(P, np.float32), np.zeros(P, np.float32)
         fell = np.zeros(P, np.int32)
-        check(load().thx_ctx_get_learned(self._h, ptr(bR), ptr(bT), ptr(sc), ptr(ess),
-                                         ptr(fell)), "get_learned")
-        return bR, bT, sc, ess, fell
+        check(load().thx_ctx_get_learned(self._h, ptr(bR), ptr(bT), ptr(sc), ptr(ess), ptr(spread),
+                                         ptr(tail), ptr(fell)), "get_learned")
+        return bR, bT, sc, ess, spread, tail, fell
 
     # ---- particle filter ----
     def pf_resample(self, w, u, nIn, nOut, seed=1, u0=None):
