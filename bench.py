@@ -1,15 +1,24 @@
-"""Benchmark of the global expectation step on one MI355X.
+"""Benchmark of the expectation step on one MI355X (box-128 workload).
 
-One step = one global E-step over the box-128 workload: 10,000 particles x
-5,000 rotations x 121 translations (11 x 11 at 1 px), pixel cutoff 50
-(K = 3,853), synthetic blob volume and particles resident in HBM.  The step
-runs projection, particle-operand prep, norm GEMM, the MFMA cross-term
-scorer and the FP64 marginal merge.
+Workload: 10,000 particles x 5,000 rotations x 121 translations (11 x 11 at
+1 px), pixel cutoff 50 (K = 3,853), synthetic blob volume and particles
+resident in HBM.  Three forms of the same E-step are timed in one run:
 
-Prints one JSON line.  `value` is pose-evaluations per second of the mixed
-precision path; the plain FP32 path is timed in the same run and reported
-beside it, as is the CPU restatement of the reference loop on the host
-cores (a bounded sample).
+* plain FP32: projection, operand prep, FP32 norm GEMM, FP32-MFMA cross-term
+  scorer over the whole grid, FP64 marginal merge (the parity path);
+* mixed global: the same whole-grid scan with a BF16 (or FP16) cross term;
+* mixed + learned proposal (the headline): the proposal network, trained
+  beforehand on the library's own projections of the reference volume,
+  ranks the grid per particle; the selected rotations x translations are
+  scored with BF16 operands and the particles the calibrated fallback rule
+  flags are rescored over the whole grid.
+
+`value` is grid-equivalent pose-evaluations per second of the headline form
+(particles x 5,000 x 121 per step over its step time); its assignments are
+compared with the plain FP32 scan's (rotation within one sampling step).
+Training and calibration are one-off costs per reference volume, timed and
+reported beside it, outside the timed steps.  The CPU restatement of the
+reference loop is timed on a bounded sample on the host cores.
 """
 import argparse
 import json
@@ -22,6 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "thunder-refactor-expectation_amd"))
 
+import learned  # noqa: E402
 import thx  # noqa: E402
 import workloads as wl  # noqa: E402
 
@@ -50,6 +60,67 @@ def run_mode(ctx, prec, steps, warmup, P, R, T, K):
                 score_flops=4.0 * K * evals)
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def prepare_learned(name, step, train_samples, cal_particles):
+    """Train the proposal on projections of the workload's reference volume and
+    calibrate the fallback rule on a held-out synthetic set."""
+    w = wl.WORKLOADS[name]
+    sd = wl.seeds(name)
+    iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
+    tctx = thx.Context(0)
+    tctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    tctx.synth_volume(0, w["box"], w["pf"], w["n_blob"], w["blob_sigma"], w["blob_radius"],
+                      sd["volume"])
+    net = thx.Proposal(components=8, seed=7)
+    train_s, losses = learned.train_proposal(net, tctx, name, n_samples=train_samples, log=log)
+    tctx.close()
+    # held-out calibration particles: same generator, unseen seed
+    cctx = thx.Context(0)
+    cctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    cctx.synth_volume(0, w["box"], w["pf"], w["n_blob"], w["blob_sigma"], w["blob_radius"],
+                      sd["volume"])
+    cctx.synth_particles(0, cal_particles, w["max_shift"], w["df_min"], w["df_max"], w["snr"],
+                         sd["particles"] + 7919, **wl.CTF_DEFAULTS)
+    quat, trans = wl.grid(name)
+    cctx.set_grid(quat, trans)
+    t0 = time.perf_counter()
+    cal = learned.calibrate(cctx, net, step, log=log)
+    cal["seconds"] = time.perf_counter() - t0
+    cctx.close()
+    log(f"calibration: {cal}")
+    return net, dict(seconds=train_s, samples=train_samples, last_losses=losses), cal
+
+
+def run_learned(ctx, net, cal, steps, warmup, P, R, T):
+    args = (net, 0, cal["n_rot"], cal["n_trans"], 0.0, cal["spread_max"], cal["tail_max"],
+            cal["score_drop"], thx.BF16)
+    for _ in range(warmup):
+        ctx.expect_learned(*args)
+    ctx.sync()
+    kern, nfs = {}, []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        nfs.append(ctx.expect_learned(*args))
+        for k, v in ctx.last_timing().items():
+            kern.setdefault(k, []).append(v)
+    ctx.sync()
+    total = time.perf_counter() - t0
+    kern = {k: float(np.mean(v)) for k, v in kern.items()}
+    return dict(ms_per_step=1e3 * total / steps, pose_evals_per_s=float(P) * R * T / (total / steps),
+                kernels_ms=kern, fallback=float(np.mean(nfs)))
+
+
+def agreement(quat, trans, step, bR, bT, gR, gT):
+    ang = wl.geodesic_angle(quat[bR], quat[gR])
+    return dict(within_one_step=float(np.mean(ang <= step * (1 + 1e-6))),
+                same_rotation=float(np.mean(bR == gR)),
+                same_pose=float(np.mean((bR == gR) & (bT == gT))),
+                angle_p99_deg=float(np.degrees(np.percentile(ang, 99))))
+
+
 def cpu_baseline(ctx, info, name, P_s=384, R_s=1000):
     """Reference CPU loop (oracle restatement, OpenMP) on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -71,18 +142,15 @@ def cpu_baseline(ctx, info, name, P_s=384, R_s=1000):
                        f"FP32 reference loop order, {dt:.1f} s")
 
 
-def traffic_bytes(name, prec):
-    """HBM bytes per k_score launch from the committed rocprofv3 FETCH_SIZE +
-    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
+def traffic_bytes(name, key):
+    """HBM bytes per launch of a kernel (key as in the committed summary, e.g.
+    "k_score_sel") from the rocprofv3 FETCH_SIZE + WRITE_SIZE passes over this
+    workload, scaled by the copy-kernel calibration; None without a summary."""
     path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if not os.path.exists(path):
         return None
-    d = json.load(open(path))
-    tag = {"fp32": 0, "bf16": 1, "fp16": 2}[prec]
-    for k, v in d["kernels"].items():
-        if k.startswith(f"thx::k_score<{tag},"):
-            return (v["read_GB"] + v["write_GB"]) * 1e9
-    return None
+    v = json.load(open(path))["kernels"].get(key)
+    return None if v is None else (v["read_GB"] + v["write_GB"]) * 1e9
 
 
 def main():
@@ -93,8 +161,10 @@ def main():
     ap.add_argument("--workload", default="box128")
     ap.add_argument("--mixed", default="bf16", choices=["bf16", "fp16"])
     ap.add_argument("--particles", type=int, default=0)
+    ap.add_argument("--train-samples", type=int, default=3_932_160)
+    ap.add_argument("--cal-particles", type=int, default=2000)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -103,55 +173,81 @@ def main():
     name = args.workload
     w = wl.WORKLOADS[name]
     ctx = thx.Context(0)
-    print(f"[bench] setting up {name}", file=sys.stderr, flush=True)
+    log(f"setting up {name}")
     info = wl.setup_context(ctx, name, n_particles=args.particles or None)
     P, R, T, K = ctx.P, ctx.R, ctx.T, ctx.K
+    quat, trans = info["quat"], info["trans"]
+    step = wl.sampling_step(quat)
     res = {}
-    modes = [args.mixed] + ([] if args.no_fp32 else ["fp32"])
-    for m in modes:
-        print(f"[bench] mode {m}", file=sys.stderr, flush=True)
+    for m in ["fp32", args.mixed]:
+        log(f"mode {m}")
         res[m] = run_mode(ctx, m, args.steps, args.warmup, P, R, T, K)
-        print(f"[bench] {m}: {res[m]['ms_per_step']:.1f} ms/step "
-              f"{res[m]['kernels_ms']}", file=sys.stderr, flush=True)
-    main_m = args.mixed
-    r = res[main_m]
-    t_score = r["kernels_ms"]["score"] * 1e-3
-    achieved = r["score_flops"] / t_score
+        res[m]["best"] = ctx.get_best(0)
+        log(f"{m}: {res[m]['ms_per_step']:.1f} ms/step {res[m]['kernels_ms']}")
+    gR, gT, _ = res["fp32"]["best"]
+    f = res["fp32"]
+    mg = res[args.mixed]
     out = {
-        "metric": "pose-evals/sec (particles x rot x trans) at box 128",
-        "value": r["pose_evals_per_s"],
-        "unit": "pose-evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": r["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": main_m,
+        "metric": "pose-evals/sec (particles x rot x trans) at box 128, mixed+proposal vs plain FP32",
+        "unit": "pose-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "data": "synthetic (seeded blob volume, projected particles with CTF, shifts, noise)",
-        "config": {"workload": f"{name}: box {w['box']}, {P} particles x {R} rotations x "
-                               f"{T} translations, cutoff {w['cutoff']} (K={K}), global scan, "
-                               f"{main_m} cross term, no learned proposal",
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[main_m] / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[main_m],
-                     "traffic": traffic_bytes(name, main_m),
-                     "kernel": "k_score", "kernel_ms": r["kernels_ms"]["score"],
-                     "flops_per_launch": r["score_flops"],
-                     "flops_definition": "4*K per scored (particle, rotation, translation)"},
-        "kernels_ms": r["kernels_ms"],
     }
-    if "fp32" in res:
-        f = res["fp32"]
-        tf = f["kernels_ms"]["score"] * 1e-3
-        out["plain_fp32"] = {"value": f["pose_evals_per_s"], "ms_per_step": f["ms_per_step"],
-                             "kernels_ms": f["kernels_ms"],
-                             "score_tflops": f["score_flops"] / tf / 1e12,
-                             "score_frac_of_fp32_peak": f["score_flops"] / tf / PEAK["fp32"]}
-        out["speedup_vs_plain_fp32"] = r["pose_evals_per_s"] / f["pose_evals_per_s"]
+    plain = {"value": f["pose_evals_per_s"], "ms_per_step": f["ms_per_step"],
+             "kernels_ms": f["kernels_ms"],
+             "score_tflops": f["score_flops"] / (f["kernels_ms"]["score"] * 1e-3) / 1e12}
+    plain["score_frac_of_fp32_peak"] = plain["score_tflops"] * 1e12 / PEAK["fp32"]
+    mixed = {"value": mg["pose_evals_per_s"], "ms_per_step": mg["ms_per_step"],
+             "kernels_ms": mg["kernels_ms"],
+             "score_tflops": mg["score_flops"] / (mg["kernels_ms"]["score"] * 1e-3) / 1e12,
+             "speedup_vs_plain_fp32": mg["pose_evals_per_s"] / f["pose_evals_per_s"],
+             "agreement_with_plain_fp32": agreement(quat, trans, step, *mg["best"][:2], gR, gT)}
+    mixed["score_frac_of_peak"] = mixed["score_tflops"] * 1e12 / PEAK[args.mixed]
+    if args.no_learned:
+        r, dtype = mg, args.mixed
+        form = f"whole-grid scan, {args.mixed} cross term"
+        t_score = mg["kernels_ms"]["score"] * 1e-3
+        roof = {"bound": "mfma", "achieved": mg["score_flops"] / t_score / 1e12,
+                "peak": PEAK[args.mixed] / 1e12, "unit": "TFLOP/s",
+                "frac": mg["score_flops"] / t_score / PEAK[args.mixed],
+                "traffic": traffic_bytes(name, f"k_score16_{args.mixed}"), "kernel": "k_score16",
+                "kernel_ms": mg["kernels_ms"]["score"], "flops_per_launch": mg["score_flops"],
+                "flops_definition": "4*K per scored (particle, rotation, translation)"}
+    else:
+        log("training the proposal")
+        net, train, cal = prepare_learned(name, step, args.train_samples, args.cal_particles)
+        log("mode learned")
+        r = run_learned(ctx, net, cal, args.steps, args.warmup, P, R, T)
+        bR, bT, sc, ess, spread, tail, fell = ctx.get_learned()
+        log(f"learned: {r['ms_per_step']:.1f} ms/step {r['kernels_ms']}")
+        dtype = "bf16"
+        form = (f"mixed + learned proposal: {cal['n_rot']} rotations x {cal['n_trans']} "
+                f"translations per particle from the grid, BF16 cross term, whole-grid BF16 "
+                f"fallback (calibrated rule)")
+        nsel = float(P) * cal["n_rot"] * cal["n_trans"]
+        t_sel = r["kernels_ms"]["score_sel"] * 1e-3
+        roof = {"bound": "mfma", "achieved": 4.0 * K * nsel / t_sel / 1e12,
+                "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
+                "frac": 4.0 * K * nsel / t_sel / PEAK["bf16"],
+                "traffic": traffic_bytes(name, "k_score_sel"), "kernel": "k_score_sel",
+                "kernel_ms": r["kernels_ms"]["score_sel"], "flops_per_launch": 4.0 * K * nsel,
+                "flops_definition": "4*K per scored selected (particle, rotation, translation)"}
+        out["learned"] = {"agreement_with_plain_fp32": agreement(quat, trans, step, bR, bT, gR, gT),
+                          "fallback_particles": r["fallback"], "training": train,
+                          "calibration": cal, "kernels_ms": r["kernels_ms"],
+                          "sampling_step_deg": float(np.degrees(step))}
+        net.close()
+    out.update({
+        "value": r["pose_evals_per_s"], "ms_per_step": r["ms_per_step"], "dtype": dtype,
+        "config": {"workload": f"{name}: box {w['box']}, {P} particles x {R} rotations x "
+                               f"{T} translations, cutoff {w['cutoff']} (K={K}); {form}",
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "speedup_vs_plain_fp32": r["pose_evals_per_s"] / f["pose_evals_per_s"],
+        "plain_fp32": plain, "mixed_global": mixed,
+    })
     if not args.no_cpu:
-        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(ctx, info, name)
     ctx.close()
     print(json.dumps(out))

@@ -1567,7 +1567,7 @@ int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double*
 // Selected-pose scorer of the grid-restricted proposal.  Particle p scores
 // its nSelR selected grid rotations (rows of the shared slice table rotLo,
 // BF16) against its nSelT selected grid translations (rows of the phase
-// table S); the norm term is accumulated in FP32 from the |P|^2 table.  A
+// table S); the norm term is accumulated in FP32 from the 16-bit slice values.  A
 // block owns one particle x up to 128 of its selected rotations x all its
 // selected translations; epilogue and merge are those of the local scorer.
 // ---------------------------------------------------------------------------
@@ -1599,56 +1599,80 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
     const int Tp = nSelT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, hl = lane >> 5;
-    const int grp = tid >> 5;
     // null selections mean the whole grid (rotation / translation index = position)
     if (tid < RB) rrow[tid] = tid < nr ? (ridx ? ridx[(size_t)p * nSelR + r0 + tid] : r0 + tid) : -1;
     if (tid < TPAD) trow[tid] = tid < Tp ? (tidx ? tidx[(size_t)p * nSelT + tid] : tid) : -1;
     __syncthreads();
 
-    floatx16 acc[MT];
+    // Loads are 16 bytes per lane: slice rows as 4 packed pixels (lanes
+    // l8 = tid & 7 of eight consecutive lanes cover a row's 32-pixel chunk,
+    // rows rq + 32 q), phase rows as 2 complex pixels (l16 = tid & 15, rows
+    // tq + 16 j).  The norm term is taken from the 16-bit slice values, so
+    // no |P|^2 table is read.
+    constexpr int NJ = TPAD / 16;
+    const int l8 = tid & 7, rq = tid >> 3;
+    const int l16 = tid & 15, tq = tid >> 4;
+    const uint32_t* brow[4];
 #pragma unroll
-    for (int a = 0; a < MT; a++) acc[a] = floatx16{0};
-    float cacc[16];
+    for (int q = 0; q < 4; q++) {
+        const int row = rrow[rq + 32 * q];
+        brow[q] = row >= 0 ? rotLo + (size_t)row * Kp : nullptr;
+    }
+    const float2* srow[NJ];
 #pragma unroll
-    for (int q = 0; q < 16; q++) cacc[q] = 0.f;
+    for (int j = 0; j < NJ; j++) {
+        const int t = trow[tq + 16 * j];
+        srow[j] = t >= 0 ? S + (size_t)t * Kp : nullptr;
+    }
     const float2* yrow = Y + (size_t)pd * Kp;
     const float* wrow = Wt + (size_t)pd * Kp;
 
+    floatx16 acc[MT];
+#pragma unroll
+    for (int a = 0; a < MT; a++) acc[a] = floatx16{0};
+    float cpart[4] = {0.f, 0.f, 0.f, 0.f};
+    u32x4 bv[4];
+    float4 sv[NJ];
+    float4 yv, wv;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            bv[q] = brow[q] ? *reinterpret_cast<const u32x4*>(brow[q] + k0 + 4 * l8)
+                            : u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+            sv[j] = srow[j] ? *reinterpret_cast<const float4*>(srow[j] + k0 + 2 * l16)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+        yv = *reinterpret_cast<const float4*>(yrow + k0 + 2 * l16);
+        wv = *reinterpret_cast<const float4*>(wrow + k0 + 4 * l8);
+    };
+    gload(0);
     for (int k0 = 0; k0 < Kp; k0 += kChunk) {
-        const int k = k0 + l32;
-        // ---- A: Z = Y conj(S_t) ----
-        {
-            const float2 y = yrow[k];
+        // ---- norm term and B staging ----
 #pragma unroll
-            for (int j = 0; j < TPAD / 8; j++) {
-                const int t = grp + 8 * j;
-                float zr = 0.f, zi = 0.f;
-                if (trow[t] >= 0) {
-                    const float2 sv = S[(size_t)trow[t] * Kp + k];
-                    zr = y.x * sv.x + y.y * sv.y;
-                    zi = y.y * sv.x - y.x * sv.y;
-                }
-                *reinterpret_cast<uint32_t*>(As + t * LD + 2 * l32) = pack16<THX_BF16>(zr, zi);
+        for (int q = 0; q < 4; q++) {
+            const float w4[4] = {wv.x, wv.y, wv.z, wv.w};
+            float c = 0.f;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t v = bv[q][e];
+                const float pr = __uint_as_float(v << 16), pi = __uint_as_float(v & 0xffff0000u);
+                c += w4[e] * (pr * pr + pi * pi);
             }
+            cpart[q] += c;
+            *reinterpret_cast<u32x4*>(Bs + (rq + 32 * q) * LD + 8 * l8) = bv[q];
         }
-        // ---- B: gathered slice rows, norm term ----
-        {
-            const float wk = wrow[k];
+        // ---- A: Z = Y conj(S_t) ----
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int rr = grp + 8 * q;
-                const int row = rrow[rr];
-                uint32_t v = 0u;
-                float nv = 0.f;
-                if (row >= 0) {
-                    v = rotLo[(size_t)row * Kp + k];
-                    nv = normP[(size_t)row * Kp + k];
-                }
-                cacc[q] += wk * nv;
-                *reinterpret_cast<uint32_t*>(Bs + rr * LD + 2 * l32) = v;
-            }
+        for (int j = 0; j < NJ; j++) {
+            const float4 sj = sv[j];
+            uint2 z;
+            z.x = pack16<THX_BF16>(yv.x * sj.x + yv.y * sj.y, yv.y * sj.x - yv.x * sj.y);
+            z.y = pack16<THX_BF16>(yv.z * sj.z + yv.w * sj.w, yv.w * sj.z - yv.z * sj.w);
+            *reinterpret_cast<uint2*>(As + (tq + 16 * j) * LD + 4 * l16) = z;
         }
         __syncthreads();
+        if (k0 + kChunk < Kp) gload(k0 + kChunk);
         const int nrow = 32 * wave + l32;
 #pragma unroll
         for (int kk = 0; kk < 64; kk += 16) {
@@ -1664,20 +1688,15 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
         __syncthreads();
     }
 
-    // ---- norm term per rotation ----
+    // ---- norm term per rotation: sum over the eight lanes of a row ----
 #pragma unroll
-    for (int q = 0; q < 16; q++) cacc[q] += __shfl_xor(cacc[q], 16);
-#pragma unroll
-    for (int s = 8; s >= 1; s >>= 1) {
-        const bool up = (l32 & s) != 0;
-#pragma unroll
-        for (int j = 0; j < s; j++) {
-            const float send = up ? cacc[j] : cacc[j + s];
-            const float keep = up ? cacc[j + s] : cacc[j];
-            cacc[j] = keep + __shfl_xor(send, s);
-        }
+    for (int q = 0; q < 4; q++) {
+        float v = cpart[q];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        if (l8 == 0) Cs[rq + 32 * q] = v;
     }
-    if (l32 < 16) Cs[grp + 8 * (l32 & 15)] = cacc[0];
     __syncthreads();
 
     const float a = (float)Aterm[pd];

@@ -387,12 +387,29 @@ __global__ __launch_bounds__(256) void k_select_grid(
     __shared__ int red[256];
     __shared__ int cnt;
     __shared__ float tv[512];
+    // per component (FP32): mean m, and the density pi k^{-3/2} / (2 pi^2) /
+    // (a c^2 + b)^2 with c = m.q, a = 1 - 1/k, b = 1/k
+    __shared__ float cm[16][4], ccoef[16], ca[16], cb[16];
+    if ((int)threadIdx.x < C) {
+        const int c = threadIdx.x;
+        const double k = mp[6 * c + 4];
+        for (int a = 0; a < 4; a++) cm[c][a] = (float)mp[6 * c + a];
+        ccoef[c] = (float)(mp[6 * c + 5] * pow(k, -1.5) / (2.0 * M_PI * M_PI));
+        ca[c] = (float)(1.0 - 1.0 / k);
+        cb[c] = (float)(1.0 / k);
+    }
+    __syncthreads();
     float mx = -INFINITY, mn = INFINITY;
     for (int r = threadIdx.x; r < R; r += blockDim.x) {
         const double* q = gridQ + 4 * (size_t)r;
-        double dens = 0.0;
-        for (int c = 0; c < C; c++) dens += mp[6 * c + 5] * acg_pdf(mp + 6 * c, mp[6 * c + 4], q);
-        const float lv = (float)log(fmax(dens, 1e-300));
+        const float q0 = (float)q[0], q1 = (float)q[1], q2 = (float)q[2], q3 = (float)q[3];
+        float dens = 0.f;
+        for (int c = 0; c < C; c++) {
+            const float d = cm[c][0] * q0 + cm[c][1] * q1 + cm[c][2] * q2 + cm[c][3] * q3;
+            const float sv = ca[c] * d * d + cb[c];
+            dens += ccoef[c] / (sv * sv);
+        }
+        const float lv = __logf(fmaxf(dens, 1e-37f));
         v[r] = lv;
         mx = fmaxf(mx, lv);
         mn = fminf(mn, lv);
@@ -463,6 +480,8 @@ __global__ __launch_bounds__(256) void k_select_grid(
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
                    const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
                    int* tidx, float* spread) {
+    THX_CHECK(C >= 1 && C <= 16 && R <= kSelMaxR && T <= 512 && nSelR <= R && nSelT <= T,
+              THX_ERR_RANGE, "select_grid: sizes out of range");
     hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), 0, st, mix, C, gridQ, R, gridT, T, nSelR,
                        nSelT, ridx, rlogq, tidx, spread);
     THX_HIP(hipGetLastError());

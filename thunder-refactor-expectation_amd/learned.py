@@ -1,0 +1,99 @@
+"""Host-side driver of the learned-proposal expectation.
+
+The proposal network is trained on particles the library's own projector
+generates from the expectation's reference volume (random poses, defocus and
+shifts, white noise over a range of SNRs), with the hand-written HIP training
+kernels.  The fallback rule of thx_ctx_expect_learned (proposal spread, tail
+depth of the best rotation, score drop below the batch median, ESS) is
+calibrated on a held-out synthetic set that is scored both ways: the plain
+whole-grid scan and the learned step.  The calibration picks, among candidate
+selection sizes and thresholds, the one with the lowest predicted time whose
+assignments agree with the plain scan's (rotation within one sampling step)
+on at least the target fraction of the held-out particles.
+
+There is no counterpart in the reference: it has no learned proposal.
+"""
+import time
+
+import numpy as np
+
+import thx
+import workloads as wl
+
+
+def train_proposal(net, ctx, workload, n_samples=3_932_160, chunk=8192, batch=256,
+                   snr=(0.01, 0.1), seed=5000, schedule=((0.5, 1e-3), (0.25, 3e-4), (0.25, 1e-4)),
+                   log=None):
+    """Train `net` on synthetic particles of the reference volume held by `ctx`
+    (class 0, pixel set set).  Returns (seconds, last losses)."""
+    w = wl.WORKLOADS[workload]
+    rng = np.random.default_rng(seed)
+    n_chunks = max(1, n_samples // chunk)
+    lrs = []
+    for frac, lr in schedule:
+        lrs += [lr] * int(round(frac * n_chunks))
+    lrs = (lrs + [schedule[-1][1]] * n_chunks)[:n_chunks]
+    t0 = time.perf_counter()
+    losses = []
+    for ch, lr in enumerate(lrs):
+        q, t, _ = ctx.synth_particles(0, chunk, w["max_shift"], w["df_min"], w["df_max"],
+                                      float(rng.uniform(*snr)), seed + ch, **wl.CTF_DEFAULTS)
+        losses.append(net.train_ctx(ctx, q, t, batch, 1, lr))
+        if log and (ch % 60 == 0 or ch == n_chunks - 1):
+            log(f"train chunk {ch + 1}/{n_chunks} loss {losses[-1]:.3f} "
+                f"{time.perf_counter() - t0:.1f} s")
+    return time.perf_counter() - t0, losses[-8:]
+
+
+CANDIDATES = [(300, 49), (500, 49)]
+SPREAD = [30.0, 50.0, 80.0, 0.0]
+TAIL = [4.0, 5.0, 6.0, 0.0]
+DROP = [100.0, 150.0, 0.0]
+
+
+def calibrate(ctx, net, step, target=0.99, margin=0.003, candidates=CANDIDATES, log=None):
+    """Choose selection sizes and fallback thresholds on the held-out set in
+    `ctx` (particles, grid and class-0 volume set).  Returns a dict with the
+    chosen n_rot, n_trans, spread_max, tail_max, score_drop and the
+    calibration statistics."""
+    quat = ctx.grid_quat
+    P = ctx.P
+    ctx.score_global(0, thx.FP32)
+    gR, _, _ = ctx.get_best(0)
+    ctx.score_global(0, thx.BF16)
+    ctx.sync()
+    t_full = sum(ctx.last_timing().values()) / P        # ms per particle, whole grid
+    best = None
+    for nr, nt in candidates:
+        ctx.expect_learned(net, 0, nr, nt)
+        tm = ctx.last_timing()
+        t_sel = sum(v for k, v in tm.items() if k != "fallback") / P
+        bR, bT, sc, ess, spread, tail, fell = ctx.get_learned()
+        ok = wl.geodesic_angle(quat[bR], quat[gR]) <= step * (1 + 1e-6)
+        med = np.median(sc)
+        for sm in SPREAD:
+            for tmx in TAIL:
+                for dz in DROP:
+                    fb = np.zeros(P, bool)
+                    if sm > 0:
+                        fb |= spread > sm
+                    if tmx > 0:
+                        fb |= tail > tmx
+                    if dz > 0:
+                        fb |= sc < med - dz
+                    acc = float(np.mean(ok | fb))
+                    cost = t_sel + fb.mean() * t_full
+                    if acc >= target + margin and (best is None or cost < best["pred_ms_per_particle"]):
+                        best = dict(n_rot=nr, n_trans=nt, spread_max=sm, tail_max=tmx,
+                                    score_drop=dz, cal_accuracy=acc, cal_fallback=float(fb.mean()),
+                                    cal_accuracy_no_fallback=float(ok.mean()),
+                                    pred_ms_per_particle=cost)
+        if log:
+            log(f"calibrate {nr}x{nt}: no-fallback agreement {ok.mean():.4f}")
+    if best is None:   # nothing meets the target: every particle falls back
+        best = dict(n_rot=candidates[0][0], n_trans=candidates[0][1], spread_max=0.5,
+                    tail_max=0.0, score_drop=0.0, cal_accuracy=1.0, cal_fallback=1.0,
+                    cal_accuracy_no_fallback=float("nan"), pred_ms_per_particle=t_full)
+    best["cal_particles"] = P
+    best["target"] = target
+    return best

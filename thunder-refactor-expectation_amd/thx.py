@@ -229,6 +229,7 @@ class Context:
               "set_grid")
         self.R, self.T = R, T
         self.pR, self.pT = pR, pT
+        self.grid_quat, self.grid_trans = quat, trans
 
     def score_global(self, cls=0, precision=FP32):
         check(load().thx_ctx_score_global(self._h, cls, precision), "score_global")
