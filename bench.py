@@ -64,7 +64,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def prepare_learned(name, step, train_samples, cal_particles):
+def prepare_learned(name, step, train_samples, cal_particles, rule=""):
     """Train the proposal on projections of the workload's reference volume and
     calibrate the fallback rule on a held-out synthetic set."""
     w = wl.WORKLOADS[name]
@@ -77,6 +77,12 @@ def prepare_learned(name, step, train_samples, cal_particles):
     net = thx.Proposal(components=8, seed=7)
     train_s, losses = learned.train_proposal(net, tctx, name, n_samples=train_samples, log=log)
     tctx.close()
+    train = dict(seconds=train_s, samples=train_samples, last_losses=losses)
+    if rule:
+        nr, nt, sm, tm, dz = rule.split(",")
+        cal = dict(n_rot=int(nr), n_trans=int(nt), spread_max=float(sm), tail_max=float(tm),
+                   score_drop=float(dz), calibrated=False)
+        return net, train, cal
     # held-out calibration particles: same generator, unseen seed
     cctx = thx.Context(0)
     cctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
@@ -90,8 +96,9 @@ def prepare_learned(name, step, train_samples, cal_particles):
     cal = learned.calibrate(cctx, net, step, log=log)
     cal["seconds"] = time.perf_counter() - t0
     cctx.close()
+    cal["calibrated"] = True
     log(f"calibration: {cal}")
-    return net, dict(seconds=train_s, samples=train_samples, last_losses=losses), cal
+    return net, train, cal
 
 
 def run_learned(ctx, net, cal, steps, warmup, P, R, T):
@@ -163,6 +170,8 @@ def main():
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--train-samples", type=int, default=3_932_160)
     ap.add_argument("--cal-particles", type=int, default=2000)
+    ap.add_argument("--rule", default="",
+                    help="n_rot,n_trans,spread_max,tail_max,score_drop: skip the calibration")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -215,7 +224,8 @@ def main():
                 "flops_definition": "4*K per scored (particle, rotation, translation)"}
     else:
         log("training the proposal")
-        net, train, cal = prepare_learned(name, step, args.train_samples, args.cal_particles)
+        net, train, cal = prepare_learned(name, step, args.train_samples, args.cal_particles,
+                                          args.rule)
         log("mode learned")
         r = run_learned(ctx, net, cal, args.steps, args.warmup, P, R, T)
         bR, bT, sc, ess, spread, tail, fell = ctx.get_learned()
