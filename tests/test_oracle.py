@@ -330,6 +330,45 @@ def test_global_scan_f32_streaming_matches_f64(scan_case):
         np.testing.assert_allclose(w32[2], w64[2], rtol=1e-4, atol=1e-6 * w64[2].max())
 
 
+def test_local_scan_reduces_to_global(scan_case):
+    """One local phase whose per-particle lists are the whole grid equals the
+    global scan (pinned above by brute force); with ragged sub-lists it equals
+    the marginals of the brute-force dvp restricted to each particle's poses."""
+    c = scan_case
+    args = (c["hv"], c["V"], c["pf"], c["iCol"], c["iRow"], c["N"])
+    wC0, wR0, wT0, base0, dvp = oracle.global_scan(*args, c["quat"], c["trans"], c["dat"],
+                                                   c["ctf"], c["sig"], c["pR"], c["pT"],
+                                                   want_dvp=True)
+    P, R, T = dvp.shape
+    full = oracle.local_scan(*args, np.tile(c["quat"], (P, 1)), np.full(P, R),
+                             np.tile(c["trans"], (P, 1)), np.full(P, T), np.tile(c["pR"], P),
+                             np.tile(c["pT"], P), c["dat"], c["ctf"], c["sig"])
+    np.testing.assert_allclose(full[3], base0, rtol=1e-12)
+    np.testing.assert_allclose(full[0], wC0, rtol=1e-10)
+    np.testing.assert_allclose(full[1].reshape(P, R), wR0, rtol=1e-10, atol=1e-300)
+    np.testing.assert_allclose(full[2].reshape(P, T), wT0, rtol=1e-10, atol=1e-300)
+    rng = np.random.default_rng(9)
+    rs = [np.sort(rng.choice(R, rng.integers(1, R + 1), replace=False)) for _ in range(P)]
+    ts = [np.sort(rng.choice(T, rng.integers(1, T + 1), replace=False)) for _ in range(P)]
+    rs[2], ts[2] = np.array([3, 7, 11]), np.array([0, 4])
+    wC, wR, wT, base, bR, bT = oracle.local_scan(
+        *args, np.concatenate([c["quat"][r] for r in rs]), [len(r) for r in rs],
+        np.concatenate([c["trans"][t] for t in ts]), [len(t) for t in ts],
+        np.concatenate([c["pR"][r] for r in rs]), np.concatenate([c["pT"][t] for t in ts]),
+        c["dat"], c["ctf"], c["sig"])
+    ro = np.r_[0, np.cumsum([len(r) for r in rs])]
+    to = np.r_[0, np.cumsum([len(t) for t in ts])]
+    for l in range(P):
+        sub = dvp[l][np.ix_(rs[l], ts[l])]
+        assert base[l] == pytest.approx(sub.max(), rel=1e-12)
+        w = np.exp(sub - sub.max())
+        np.testing.assert_allclose(wR[ro[l]:ro[l + 1]], w @ c["pT"][ts[l]], rtol=1e-10)
+        np.testing.assert_allclose(wT[to[l]:to[l + 1]], c["pR"][rs[l]] @ w, rtol=1e-10)
+        assert wC[l] == pytest.approx(c["pR"][rs[l]] @ w @ c["pT"][ts[l]], rel=1e-10)
+        assert (bR[l], bT[l]) == np.unravel_index(sub.argmax(), sub.shape)
+    assert (rs[2][bR[2]], ts[2][bT[2]]) == (7, 4)
+
+
 # ---------------------------------------------------------------------------
 # a10 systematic resampling
 # ---------------------------------------------------------------------------
