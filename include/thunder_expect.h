@@ -236,6 +236,64 @@ int thx_proposal_sample(thx_ctx* ctx, const double* mix, int C, int P,
                         double* qdens, double* trans);
 
 /* ------------------------------------------------------------------------ */
+/* Local search with the particle filter on the GPU (state resident in HBM). */
+/* Uniform counts: mLR rotations and mLT <= 128 translations per particle    */
+/* (_para.mLR / _para.mLT).  The state is FP64.                              */
+/* ------------------------------------------------------------------------ */
+
+/* Particle states of the context's P particles: quat[P*mLR*4], trans[P*mLT*2]
+ * (pixels), concentrations k[P*3], shift deviations s01[P*2], the rotation
+ * peak factor peakR[P] and the translation search scale transS[P]
+ * (Particle::load / the state Particle keeps, src/Particle.cpp:401-554). */
+int thx_pf_set_state(thx_ctx* ctx, int mLR, int mLT, const double* quat,
+                     const double* trans, const double* k, const double* s01,
+                     const double* peakR, const double* transS);
+
+/* Particle states from the last global scan of class cls (src/Optimiser.cpp:
+ * 926-1084): the rotation marginal gets its peak factor and half-height peak
+ * (setPeakFactor + keepHalfHeightPeak), mLR rotations and mLT shifts are
+ * resampled from the grid (prior = the grid's), calVari gives k and s, and
+ * the scanning minima apply (k >= (minStdR / pfS)^2, s >= minStdT / pfS). */
+int thx_pf_from_global(thx_ctx* ctx, int cls, int mLR, int mLT, double minStdR,
+                       double minStdT, double pfS, const double* transS,
+                       uint64_t seed);
+
+/* Local search phases until every particle converges or maxPhase (the
+ * expectation loop, src/Optimiser.cpp:1475-1660, with updateSP :1139-1437):
+ * per phase, perturbation (pfL at phase 0, else pfS) with balanced weights,
+ * scoring of every (rotation, shift) sample against the particle's class
+ * (cls[P] or NULL: class 0) at precision THX_FP32 or THX_BF16, half-height
+ * peak of the rotation marginal, top samples, calVari, resampling, and the
+ * variance test (from phase 10 after a global scan, afterGlobal != 0, where
+ * phases start at 1; else from phase 3).  *phasesRun: phases executed. */
+int thx_pf_refine(thx_ctx* ctx, const int* cls, int precision, int afterGlobal,
+                  double pfL, double pfS, int maxPhase, uint64_t seed,
+                  int* phasesRun);
+
+/* Current state (any output may be NULL): samples, weights, k, s01, the top
+ * rotation / shift of the last phase, peak factors and the phase at which
+ * each particle converged (-1: not converged). */
+int thx_pf_get_state(thx_ctx* ctx, double* quat, double* trans, double* wR,
+                     double* wT, double* k, double* s01, double* topR,
+                     double* topT, double* peakR, int* nPhase);
+
+/* Stage kernels on host buffers (checking): per set of n rotations, the
+ * inferACG matrix A[16], mean[4], calVari k[3], balance weights[n] and the
+ * fixed-point iteration count (src/Geometry/DirectionalStat.cpp:87-244,
+ * src/Particle.cpp:1002-1081, 2301-2311). */
+int thx_pf_rot_stats(thx_ctx* ctx, const double* quat, int n, int P, double* A,
+                     double* mean, double* k, double* wBal, int* iters);
+/* Per set of n >= 2 shifts: means[2], n - 1 deviations[2], balance weights[n]
+ * (src/Particle.cpp:1083-1097, 2313-2344). */
+int thx_pf_trans_stats(thx_ctx* ctx, const double* trans, int n, int P,
+                       double* m, double* s01, double* wBal);
+/* setPeakFactor (setFactor != 0) + keepHalfHeightPeak of P sets of n values
+ * (in place), factor[P] in/out, top[P] = first maximum (src/Particle.cpp:
+ * 988-1000, 1864-1954). */
+int thx_pf_keep_peak(thx_ctx* ctx, double* u, int n, int P, double* factor,
+                     int setFactor, int* top);
+
+/* ------------------------------------------------------------------------ */
 /* Learned pose proposal (no counterpart in the reference, which always     */
 /* scans a uniform grid first): a ~0.9M-parameter CNN on 64 x 64 phase-     */
 /* flipped particle images predicting a C-component ACG mixture over        */

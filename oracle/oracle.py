@@ -40,6 +40,12 @@ def load():
         "thxo_local_scan_f64": ([_p, _i, _i, _p, _p, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p,
                                  _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _i], _i),
         "thxo_blob_ft": ([_p, _i, _i, _p, _p, _i, _d, _p, _p, _p, _i], None),
+        "thxo_infer_acg": ([_p, _i, _p, _d, _i, _p], _i),
+        "thxo_sym4_top": ([_p, _p, _p], None),
+        "thxo_cal_vari_rot": ([_p, _i, _d, _i, _p, _p], _i),
+        "thxo_balance_rot": ([_p, _i, _d, _i, _p], _i),
+        "thxo_trans_stats": ([_p, _i, _p, _p, _p], None),
+        "thxo_keep_peak": ([_p, _i, _d, _i, _p], _d),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -203,3 +209,59 @@ def blob_ft(V, N, centres, amps, sigma, ix, iy, iz):
     load().thxo_blob_ft(_ptr(out), V, N, _ptr(centres), _ptr(amps), amps.size, sigma,
                         _ptr(ix), _ptr(iy), _ptr(iz), ix.size)
     return out
+
+
+def infer_acg(q, pre=None, tol=1e-3, max_iter=10000):
+    """inferACG(dmat44&, const dmat4&), src/Geometry/DirectionalStat.cpp:87-139.
+    Returns (A, iterations)."""
+    q = _c(q, np.float64).reshape(-1, 4)
+    A = np.zeros(16)
+    pre_a = None if pre is None else _c(pre, np.float64)
+    it = load().thxo_infer_acg(_ptr(q), q.shape[0], _ptr(pre_a), C.c_double(tol), max_iter,
+                               _ptr(A))
+    assert it > 0, it
+    return A.reshape(4, 4), it
+
+
+def sym4_top(A):
+    v, ev = np.zeros(4), np.zeros(4)
+    load().thxo_sym4_top(_ptr(_c(A, np.float64).reshape(16)), _ptr(v), _ptr(ev))
+    return v
+
+
+def cal_vari_rot(q, tol=1e-3, max_iter=10000):
+    """Particle::calVari(PAR_R), src/Particle.cpp:1002-1081: (mean, k[3])."""
+    q = _c(q, np.float64).reshape(-1, 4)
+    mean, k = np.zeros(4), np.zeros(3)
+    st = load().thxo_cal_vari_rot(_ptr(q), q.shape[0], C.c_double(tol), max_iter, _ptr(mean),
+                                  _ptr(k))
+    assert st == 0, st
+    return mean, k
+
+
+def balance_rot(q, tol=1e-3, max_iter=10000):
+    """Particle::balanceWeight(PAR_R), src/Particle.cpp:2301-2311 (+ normW)."""
+    q = _c(q, np.float64).reshape(-1, 4)
+    w = np.zeros(q.shape[0])
+    st = load().thxo_balance_rot(_ptr(q), q.shape[0], C.c_double(tol), max_iter, _ptr(w))
+    assert st == 0, st
+    return w
+
+
+def trans_stats(t):
+    """Means, n-1 standard deviations and balance weights of shifts t[n][2]
+    (Particle::calVari(PAR_T) :1083-1097, balanceWeight(PAR_T) :2313-2344)."""
+    t = _c(t, np.float64).reshape(-1, 2)
+    m, s, w = np.zeros(2), np.zeros(2), np.zeros(t.shape[0])
+    load().thxo_trans_stats(_ptr(t), t.shape[0], _ptr(m), _ptr(s), _ptr(w))
+    return m, s, w
+
+
+def keep_peak(u, factor=1e-3, set_factor=False):
+    """setPeakFactor(PAR_R) + keepHalfHeightPeak(PAR_R), src/Particle.cpp:1864-1954.
+    Returns (u after, factor, index of the first maximum)."""
+    u = _c(u, np.float64).copy()
+    top = C.c_int(0)
+    f = load().thxo_keep_peak(_ptr(u), u.size, C.c_double(factor), int(set_factor),
+                              C.byref(top))
+    return u, f, top.value

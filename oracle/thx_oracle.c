@@ -566,3 +566,240 @@ void thxo_blob_ft(float* dst, int V, int N, const double* centres,
         dst[2 * q + 1] = (float)(g * im);
     }
 }
+
+/* ------------------------------------------------------------------------
+ * Particle update of one local phase (a9 / a11 / a13).
+ * ---------------------------------------------------------------------- */
+
+/* 4x4 inverse and determinant by Gauss-Jordan with partial pivoting. */
+static int inv4(const double* A, double* inv, double* det)
+{
+    double m[4][8];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 8; j++)
+            m[i][j] = j < 4 ? A[4 * i + j] : (j - 4 == i ? 1.0 : 0.0);
+    double d = 1.0;
+    for (int c = 0; c < 4; c++) {
+        int p = c;
+        for (int r = c + 1; r < 4; r++)
+            if (fabs(m[r][c]) > fabs(m[p][c])) p = r;
+        if (m[p][c] == 0.0) return -1;
+        if (p != c) {
+            for (int j = 0; j < 8; j++) { double t = m[p][j]; m[p][j] = m[c][j]; m[c][j] = t; }
+            d = -d;
+        }
+        d *= m[c][c];
+        double pv = m[c][c];
+        for (int j = 0; j < 8; j++) m[c][j] /= pv;
+        for (int r = 0; r < 4; r++)
+            if (r != c) {
+                double f = m[r][c];
+                for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+            }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) inv[4 * i + j] = m[i][j + 4];
+    if (det) *det = d;
+    return 0;
+}
+
+static void qmul_o(const double* a, const double* b, double* o)
+{
+    double w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    double x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    double y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    double z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = w; o[1] = x; o[2] = y; o[3] = z;
+}
+
+/*
+ * inferACG(dmat44&, const dmat4&), src/Geometry/DirectionalStat.cpp:87-139:
+ * A <- I; repeat B = (4 / sum 1/u_i) sum q_i q_i^T / u_i with u_i = q_i^T A^-1 q_i,
+ * until sum |A - B| <= tol (1e-3), returning the last A (the one before B).
+ * pre (optional): every sample is first left-multiplied by it (the frame
+ * change of Particle::calVari, src/Particle.cpp:1043-1052).  Returns the
+ * iteration count, or -1 (singular) / -2 (maxIter reached).
+ */
+int thxo_infer_acg(const double* q, int n, const double* pre, double tol, int maxIter,
+                   double* Aout)
+{
+    double A[16], B[16], Ai[16];
+    for (int i = 0; i < 16; i++) B[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    int it = 0;
+    for (;;) {
+        memcpy(A, B, sizeof(A));
+        if (inv4(A, Ai, NULL)) return -1;
+        memset(B, 0, sizeof(B));
+        double nf = 0;
+        for (int i = 0; i < n; i++) {
+            double x[4];
+            if (pre) qmul_o(pre, q + 4 * i, x);
+            else memcpy(x, q + 4 * i, sizeof(x));
+            double v[4] = {0, 0, 0, 0};
+            for (int a = 0; a < 4; a++)
+                for (int b = 0; b < 4; b++) v[a] += Ai[4 * a + b] * x[b];
+            double u = x[0] * v[0] + x[1] * v[1] + x[2] * v[2] + x[3] * v[3];
+            for (int a = 0; a < 4; a++)
+                for (int b = 0; b < 4; b++) B[4 * a + b] += x[a] * x[b] / u;
+            nf += 1.0 / u;
+        }
+        for (int i = 0; i < 16; i++) B[i] *= 4.0 / nf;
+        it++;
+        double diff = 0;
+        for (int i = 0; i < 16; i++) diff += fabs(A[i] - B[i]);
+        if (diff <= tol) break;
+        if (it >= maxIter) { memcpy(Aout, A, sizeof(A)); return -2; }
+    }
+    memcpy(Aout, A, sizeof(A));
+    return it;
+}
+
+/* Eigenvector of the largest eigenvalue of a symmetric 4x4 (cyclic Jacobi),
+ * normalised, sign fixed so that its largest-magnitude entry is positive
+ * (inferACG(dvec4&, ...), src/Geometry/DirectionalStat.cpp:218-244; the sign
+ * of an eigenvector is arbitrary and q, -q are the same rotation). */
+void thxo_sym4_top(const double* Ain, double* vec, double* evals)
+{
+    double A[4][4], V[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) { A[i][j] = Ain[4 * i + j]; V[i][j] = i == j; }
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0;
+        for (int a = 0; a < 4; a++)
+            for (int b = a + 1; b < 4; b++) off += A[a][b] * A[a][b];
+        if (off < 1e-40) break;
+        for (int a = 0; a < 4; a++)
+            for (int b = a + 1; b < 4; b++) {
+                if (fabs(A[a][b]) < 1e-300) continue;
+                double th = (A[b][b] - A[a][a]) / (2 * A[a][b]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 4; k++) {
+                    double x = A[k][a], y = A[k][b];
+                    A[k][a] = c * x - s * y; A[k][b] = s * x + c * y;
+                }
+                for (int k = 0; k < 4; k++) {
+                    double x = A[a][k], y = A[b][k];
+                    A[a][k] = c * x - s * y; A[b][k] = s * x + c * y;
+                }
+                for (int k = 0; k < 4; k++) {
+                    double x = V[k][a], y = V[k][b];
+                    V[k][a] = c * x - s * y; V[k][b] = s * x + c * y;
+                }
+            }
+    }
+    int top = 0;
+    for (int a = 1; a < 4; a++) if (A[a][a] > A[top][top]) top = a;
+    int big = 0;
+    for (int a = 1; a < 4; a++) if (fabs(V[a][top]) > fabs(V[big][top])) big = a;
+    double sg = V[big][top] < 0 ? -1.0 : 1.0, nrm = 0;
+    for (int a = 0; a < 4; a++) nrm += V[a][top] * V[a][top];
+    nrm = sqrt(nrm);
+    for (int a = 0; a < 4; a++) vec[a] = sg * V[a][top] / nrm;
+    if (evals) for (int a = 0; a < 4; a++) evals[a] = A[a][a];
+}
+
+/*
+ * Particle::calVari(PAR_R), 3-D branch with PARTICLE_ROT_MEAN_USING_STAT_CAL_VARI
+ * (src/Particle.cpp:1002-1081, include/Config.h:77): mean = inferACG mean of the
+ * samples; k_a = A'(a,a) / A'(0,0) of inferACG of conj(mean) q
+ * (inferACG(k1,k2,k3,...), src/Geometry/DirectionalStat.cpp:178-216).
+ */
+int thxo_cal_vari_rot(const double* q, int n, double tol, int maxIter, double* mean, double* k)
+{
+    double A[16];
+    int it = thxo_infer_acg(q, n, NULL, tol, maxIter, A);
+    if (it < 0) return it;
+    thxo_sym4_top(A, mean, NULL);
+    double mc[4] = {mean[0], -mean[1], -mean[2], -mean[3]};
+    it = thxo_infer_acg(q, n, mc, tol, maxIter, A);
+    if (it < 0) return it;
+    for (int a = 0; a < 3; a++) k[a] = A[5 * (a + 1)] / A[0];
+    return 0;
+}
+
+/*
+ * Particle::balanceWeight(PAR_R), 3-D (src/Particle.cpp:2301-2311) with normW
+ * (:813-819): w_i = 1 / pdfACG(q_i, A), pdfACG(x, A) = det(A)^-1/2 (x^T A^-1 x)^-2
+ * (src/Geometry/DirectionalStat.cpp:13-18), A = inferACG of the samples; then
+ * normalised to sum 1.
+ */
+int thxo_balance_rot(const double* q, int n, double tol, int maxIter, double* w)
+{
+    double A[16], Ai[16], det;
+    int it = thxo_infer_acg(q, n, NULL, tol, maxIter, A);
+    if (it < 0) return it;
+    if (inv4(A, Ai, &det)) return -1;
+    double s = 0;
+    for (int i = 0; i < n; i++) {
+        const double* x = q + 4 * i;
+        double u = 0;
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) u += x[a] * Ai[4 * a + b] * x[b];
+        w[i] = sqrt(det) * u * u;
+        s += w[i];
+    }
+    for (int i = 0; i < n; i++) w[i] /= s;
+    return 0;
+}
+
+/*
+ * Translation statistics of one particle's samples t[n][2]: means m, sample
+ * standard deviations s (n - 1 denominators, gsl_stats_sd / gsl_stats_sd_m) as
+ * Particle::calVari(PAR_T) (src/Particle.cpp:1083-1097, zero-mean switch off)
+ * and balanceWeight(PAR_T) (:2313-2344, rho = 0): w_i = 1 / N2(t_i - m; s0, s1)
+ * normalised.  w may be NULL.
+ */
+void thxo_trans_stats(const double* t, int n, double* m, double* s, double* w)
+{
+    double m0 = 0, m1 = 0;
+    for (int i = 0; i < n; i++) { m0 += t[2 * i]; m1 += t[2 * i + 1]; }
+    m0 /= n; m1 /= n;
+    double v0 = 0, v1 = 0;
+    for (int i = 0; i < n; i++) {
+        v0 += (t[2 * i] - m0) * (t[2 * i] - m0);
+        v1 += (t[2 * i + 1] - m1) * (t[2 * i + 1] - m1);
+    }
+    double s0 = sqrt(v0 / (n - 1)), s1 = sqrt(v1 / (n - 1));
+    m[0] = m0; m[1] = m1; s[0] = s0; s[1] = s1;
+    if (!w) return;
+    double sum = 0;
+    for (int i = 0; i < n; i++) {
+        double x = (t[2 * i] - m0) / s0, y = (t[2 * i + 1] - m1) / s1;
+        double pdf = exp(-0.5 * (x * x + y * y)) / (2 * M_PI * s0 * s1);
+        w[i] = 1.0 / pdf;
+        sum += w[i];
+    }
+    for (int i = 0; i < n; i++) w[i] /= sum;
+}
+
+/*
+ * Particle::setPeakFactor(PAR_R), 3-D (src/Particle.cpp:1893-1907):
+ * f = clamp(u_(n/8) / u_(0), 1e-3, 0.5) over u sorted in descending order
+ * (PEAK_FACTOR_BASE^3 = 8, include/Particle.h:44-49), when setFactor != 0;
+ * then keepHalfHeightPeak (:1935-1954): hh = max(u) f, u_i <- u_i < hh ? 0 :
+ * u_i - hh.  Returns f and the index of the first maximum (calRank1st).
+ */
+double thxo_keep_peak(double* u, int n, double factor, int setFactor, int* top)
+{
+    int best = 0;
+    for (int i = 1; i < n; i++) if (u[i] > u[best]) best = i;
+    if (setFactor) {
+        double* s = (double*)malloc(sizeof(double) * n);
+        memcpy(s, u, sizeof(double) * n);
+        /* descending selection of rank n/8 by partial sort */
+        int k = n / 8;
+        for (int a = 0; a <= k; a++) {
+            int m = a;
+            for (int b = a + 1; b < n; b++) if (s[b] > s[m]) m = b;
+            double tt = s[a]; s[a] = s[m]; s[m] = tt;
+        }
+        double f = s[k] / s[0];
+        free(s);
+        factor = f > 0.5 ? 0.5 : (f < 1e-3 ? 1e-3 : f);
+    }
+    double hh = u[best] * factor;
+    for (int i = 0; i < n; i++) u[i] = u[i] < hh ? 0.0 : u[i] - hh;
+    if (top) *top = best;
+    return factor;
+}

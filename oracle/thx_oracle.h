@@ -94,7 +94,17 @@ void thxo_blob_ft(float* dst, int V, int N, const double* centres,
                   const double* amps, int nBlob, double sigma,
                   const int* ix, const int* iy, const int* iz, int n);
 
+/* Particle update of a local phase (a9 / a11). */
+int thxo_infer_acg(const double* q, int n, const double* pre, double tol, int maxIter,
+                   double* Aout);
+void thxo_sym4_top(const double* A, double* vec, double* evals);
+int thxo_cal_vari_rot(const double* q, int n, double tol, int maxIter, double* mean, double* k);
+int thxo_balance_rot(const double* q, int n, double tol, int maxIter, double* w);
+void thxo_trans_stats(const double* t, int n, double* m, double* s, double* w);
+double thxo_keep_peak(double* u, int n, double factor, int setFactor, int* top);
+
 #ifdef __cplusplus
 }
 #endif
 #endif
+

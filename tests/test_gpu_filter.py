@@ -1,0 +1,110 @@
+"""Particle filter of the local search on the GPU (needs gfx950).
+
+Stage kernels against the oracle on the same samples:
+  * inferACG matrix, mean, calVari concentrations and balance weights: the
+    same fixed-point iteration count and agreement to 1e-9 (FP64 on both
+    sides; only the summation order differs);
+  * shift means, deviations and balance weights to 1e-12 relative;
+  * peak factor, half-height peak and top index exactly.
+The whole local search (perturb, balance, score, peak, calVari, resample,
+convergence) is checked by what it must achieve on synthetic particles: from
+a global scan on a 10-degree grid it converges and refines the poses below
+the grid's spacing.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import thx
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _qmat(m):
+    w, x, y, z = m
+    return np.array([[w, -x, -y, -z], [x, w, -z, y], [y, z, w, -x], [z, -y, x, w]])
+
+
+def _acg_sets(rng, P, n):
+    out = np.zeros((P, n, 4))
+    for p in range(P):
+        lam = np.r_[1.0, rng.uniform(0.005, 0.5, 3)]
+        m = rng.normal(size=4)
+        m /= np.linalg.norm(m)
+        v = rng.normal(size=(n, 4)) * np.sqrt(lam)
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        out[p] = v @ _qmat(m).T
+    return out
+
+
+@pytest.mark.parametrize("n", [40, 300, 2000])
+def test_rot_stats_match_oracle(gpu_ctx, n):
+    rng = np.random.default_rng(n)
+    q = _acg_sets(rng, 6, n)
+    A, mean, k, w, it = gpu_ctx.pf_rot_stats(q)
+    for p in range(6):
+        A0, it0 = oracle.infer_acg(q[p])
+        assert it[p] == it0, (p, it[p], it0)
+        np.testing.assert_allclose(A[p], A0, atol=1e-9)
+        m0, k0 = oracle.cal_vari_rot(q[p])
+        assert abs(abs(mean[p] @ m0) - 1) < 1e-9
+        np.testing.assert_allclose(k[p], k0, rtol=1e-8)
+        np.testing.assert_allclose(w[p], oracle.balance_rot(q[p]), rtol=1e-9)
+
+
+def test_trans_stats_match_oracle(gpu_ctx):
+    rng = np.random.default_rng(5)
+    t = rng.normal(0, 1.5, size=(7, 33, 2)) + rng.normal(0, 2, size=(7, 1, 2))
+    m, s, w = gpu_ctx.pf_trans_stats(t)
+    for p in range(7):
+        m0, s0, w0 = oracle.trans_stats(t[p])
+        np.testing.assert_allclose(m[p], m0, rtol=1e-12)
+        np.testing.assert_allclose(s[p], s0, rtol=1e-12)
+        np.testing.assert_allclose(w[p], w0, rtol=1e-10)
+
+
+@pytest.mark.parametrize("set_factor", [False, True])
+def test_keep_peak_matches_oracle(gpu_ctx, set_factor):
+    rng = np.random.default_rng(9)
+    u = np.exp(rng.normal(0, 3, size=(5, 777)))
+    u[1, 10] = u[1, 20] = u[1].max() * 2          # tied maxima: the first one wins
+    f_in = np.array([1e-3, 0.2, 0.5, 0.01, 0.3])
+    out, f, top = gpu_ctx.pf_keep_peak(u, f_in, set_factor)
+    for p in range(5):
+        ref, f0, top0 = oracle.keep_peak(u[p], f_in[p], set_factor)
+        assert f[p] == f0 and top[p] == top0
+        np.testing.assert_array_equal(out[p], ref)
+
+
+def test_local_search_refines_global_assignments(gpu_ctx):
+    """Global scan on a coarse grid, particle states from it, local phases to
+    convergence: the final poses must beat the grid's own resolution."""
+    print("\n[setup] box64 P=64 R=1000 T=49 local search", flush=True)
+    info = wl.setup_context(gpu_ctx, "box64", n_particles=64, n_rot=1000)
+    quat, trans = info["quat"], info["trans"]
+    step = wl.sampling_step(quat)
+    gpu_ctx.score_global(0, thx.FP32)
+    gR, gT, _ = gpu_ctx.get_best(0)
+    grid_err = wl.geodesic_angle(quat[gR], info["true_quat"])
+    ms = wl.WORKLOADS["box64"]["n_rot"]
+    gpu_ctx.pf_from_global(0, 300, 25, ms ** (-1.0 / 3), 0.3, 0.5, 1.0, seed=3)
+    st0 = gpu_ctx.pf_get_state()
+    assert np.all(st0["k"] > 0) and np.all(st0["s01"] > 0)
+    assert np.all((st0["peakR"] >= 1e-3) & (st0["peakR"] <= 0.5))
+    ran = gpu_ctx.pf_refine(None, thx.FP32, after_global=True, max_phase=40, seed=4)
+    st = gpu_ctx.pf_get_state()
+    err = wl.geodesic_angle(st["topR"], info["true_quat"])
+    conv = st["nPhase"] >= 0
+    terr = np.linalg.norm(st["topT"] - info["true_shift"], axis=1)
+    print(f"phases {ran}, converged {conv.mean():.2f}, grid error median "
+          f"{np.degrees(np.median(grid_err)):.2f} deg, refined median "
+          f"{np.degrees(np.median(err)):.2f} deg (step {np.degrees(step):.1f}), shift error "
+          f"median {np.median(terr):.2f} px")
+    assert conv.mean() >= 0.9
+    assert np.all(st["nPhase"][conv] >= 10)
+    assert np.median(err) < 0.5 * np.median(grid_err)
+    assert np.median(terr) < 0.5
+    # weights are normalised per particle, samples stay unit quaternions
+    np.testing.assert_allclose(st["wR"].sum(1), 1.0, rtol=1e-9)
+    np.testing.assert_allclose(np.linalg.norm(st["quat"], axis=2), 1.0, rtol=1e-9)

@@ -428,3 +428,96 @@ def test_blob_ft_matches_numpy_fft():
     ref = F[iz % V, iy % V, ix]
     scale = np.abs(F).max()
     np.testing.assert_allclose(got[:, 0] + 1j * got[:, 1], ref, atol=1e-5 * scale)
+
+
+# ---------------------------------------------------------------------------
+# a9 / a11 particle update: inferACG, calVari, balanceWeight, peaks, shifts
+# ---------------------------------------------------------------------------
+
+def _qmat(m):
+    """4x4 matrix of left multiplication by quaternion m (x -> m x)."""
+    w, x, y, z = m
+    return np.array([[w, -x, -y, -z], [x, w, -z, y], [y, z, w, -x], [z, -y, x, w]])
+
+
+def _acg_samples(rng, lam, n, m=None):
+    v = rng.normal(size=(n, 4)) * np.sqrt(lam)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    if m is not None:
+        v = v @ _qmat(m).T
+    return v
+
+
+def test_infer_acg_recovers_shape_and_is_equivariant():
+    rng = np.random.default_rng(21)
+    lam = np.array([1.0, 0.3, 0.1, 0.05])
+    q = _acg_samples(rng, lam, 40000)
+    A, it = oracle.infer_acg(q)
+    # Tyler's fixed point estimates the ACG matrix up to scale (trace 4)
+    assert np.trace(A) == pytest.approx(4.0, abs=2e-3)
+    np.testing.assert_allclose(np.diag(A) / A[0, 0], lam / lam[0], rtol=0.05, atol=2e-3)
+    assert np.abs(A - np.diag(np.diag(A))).max() < 0.02
+    # equivariance: samples m q give M A M^T
+    m = np.array([0.3, -0.5, 0.7, 0.4])
+    m /= np.linalg.norm(m)
+    M = _qmat(m)
+    A2, _ = oracle.infer_acg(q @ M.T)
+    np.testing.assert_allclose(A2, M @ A @ M.T, atol=5e-4)
+    # the same frame change through the pre-multiplication argument
+    A3, _ = oracle.infer_acg(q, pre=m)
+    np.testing.assert_allclose(A3, A2, atol=1e-12)
+    # q and -q are the same rotation
+    sgn = np.where(rng.uniform(size=(len(q), 1)) < 0.5, -1.0, 1.0)
+    A4, _ = oracle.infer_acg(q * sgn)
+    np.testing.assert_allclose(A4, A, atol=1e-12)
+
+
+def test_cal_vari_rot_recovers_mean_and_concentrations():
+    rng = np.random.default_rng(22)
+    k = np.array([0.02, 0.05, 0.01])
+    m = np.array([0.1, 0.8, -0.3, 0.5])
+    m /= np.linalg.norm(m)
+    q = _acg_samples(rng, np.r_[1.0, k], 30000, m)
+    mean, kk = oracle.cal_vari_rot(q)
+    assert abs(abs(mean @ m) - 1) < 1e-4
+    np.testing.assert_allclose(kk, k, rtol=0.05)
+    ev = np.linalg.eigvalsh(oracle.infer_acg(q)[0])
+    A, _ = oracle.infer_acg(q)
+    v = oracle.sym4_top(A)
+    np.testing.assert_allclose(A @ v, ev[-1] * v, atol=1e-10)
+
+
+def test_balance_rot_makes_samples_uniform():
+    rng = np.random.default_rng(23)
+    q = _acg_samples(rng, np.array([1.0, 0.2, 0.1, 0.3]), 60000)
+    w = oracle.balance_rot(q)
+    assert w.sum() == pytest.approx(1.0, rel=1e-12)
+    # weights 1/pdf turn ACG samples into uniform ones on S^3: E[q q^T] = I / 4
+    S = (q * w[:, None]).T @ q
+    np.testing.assert_allclose(S, np.eye(4) / 4, atol=0.02)
+
+
+def test_trans_stats_against_numpy_scipy():
+    from scipy.stats import multivariate_normal
+    rng = np.random.default_rng(24)
+    t = rng.normal([0.5, -1.0], [2.0, 0.7], size=(53, 2))
+    m, s, w = oracle.trans_stats(t)
+    np.testing.assert_allclose(m, t.mean(0), rtol=1e-12)
+    np.testing.assert_allclose(s, t.std(0, ddof=1), rtol=1e-12)
+    pdf = multivariate_normal(m, np.diag(s ** 2)).pdf(t)
+    np.testing.assert_allclose(w, (1 / pdf) / np.sum(1 / pdf), rtol=1e-10)
+
+
+def test_keep_peak_hand_worked():
+    u = np.array([0.1, 4.0, 2.0, 1.0, 3.0, 0.5, 4.0, 0.2, 1.5, 0.05, 2.5, 0.3, 0.7, 1.2, 0.9, 0.8])
+    # 16 samples: rank 16 / 8 = 2 of the descending order is 3.0, so the
+    # factor is clamp(3.0 / 4.0) = 0.5 and the half height is 2.0
+    out, f, top = oracle.keep_peak(u, set_factor=True)
+    assert f == 0.5 and top == 1
+    np.testing.assert_allclose(out, np.where(u < 2.0, 0.0, u - 2.0))
+    out, f, _ = oracle.keep_peak(u, factor=1e-3)
+    assert f == 1e-3
+    np.testing.assert_allclose(out, np.where(u < 4e-3, 0.0, u - 4e-3))
+    # a flat set clamps at 0.5; a single spike at 1e-3
+    assert oracle.keep_peak(np.ones(40), set_factor=True)[1] == 0.5
+    assert oracle.keep_peak(np.r_[1.0, np.zeros(39)], set_factor=True)[1] == 1e-3

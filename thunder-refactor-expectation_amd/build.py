@@ -14,7 +14,7 @@ ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libthunder_expect.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_thx.so")
 
-HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_api.hip"]
+HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_filter.hip", "thx_api.hip"]
 
 HIPCC_FLAGS = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",

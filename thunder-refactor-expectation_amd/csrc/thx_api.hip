@@ -156,12 +156,6 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
     return THX_OK;
 }
 
-template <class T>
-static int upload(DevBuf& d, const T* h, size_t n, hipStream_t st) {
-    THX_TRY(d.ensure(sizeof(T) * (n ? n : 1)));
-    if (n) THX_HIP(hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, st));
-    return THX_OK;
-}
 
 static std::mutex g_default_mu;
 static thx_ctx* g_default = nullptr;
@@ -235,7 +229,11 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpSpread, &c->lpTail,
                        &c->lpPmap, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpRes.wC,
                        &c->lpRes.wR, &c->lpRes.wT, &c->lpRes.base, &c->lpRes.bestR,
-                       &c->lpRes.bestT, &c->lpRes.bestScore};
+                       &c->lpRes.bestT, &c->lpRes.bestScore, &c->pfQ, &c->pfQ2, &c->pfTr,
+                       &c->pfTr2, &c->pfWR, &c->pfWT, &c->pfWRf, &c->pfWTf, &c->pfIdx,
+                       &c->pfWNew, &c->pfK, &c->pfS, &c->pfMean, &c->pfTopR, &c->pfTopT,
+                       &c->pfPeak, &c->pfTransS, &c->pfBest, &c->pfActive, &c->pfNPhase,
+                       &c->pfOffR, &c->pfOffT, &c->pfNR, &c->pfNT, &c->pfNRa, &c->pfNTa};
     for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();
     for (auto& r : c->res) {
@@ -369,6 +367,11 @@ static int upload_priors(thx_ctx* c, const double* pR, const double* pT) {
     THX_TRY(c->pTf.ensure(sizeof(float) * c->TpadS));
     THX_HIP(hipMemcpy(c->pRf.p, fr.data(), sizeof(float) * c->Rpad, hipMemcpyHostToDevice));
     THX_HIP(hipMemcpy(c->pTf.p, ft.data(), sizeof(float) * c->TpadS, hipMemcpyHostToDevice));
+    // FP64 copies: the resampling prior of the particle filter
+    THX_TRY(c->pR.ensure(sizeof(double) * c->R));
+    THX_TRY(c->pT.ensure(sizeof(double) * c->T));
+    THX_HIP(hipMemcpy(c->pR.p, pR, sizeof(double) * c->R, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pT.p, pT, sizeof(double) * c->T, hipMemcpyHostToDevice));
     return THX_OK;
 }
 
@@ -947,7 +950,7 @@ int thx_pf_perturb(thx_ctx* c, double* quat, const int* nR, double* trans, const
             (s = upload(dk, k, 3 * (size_t)P, st)) || (d && (s = upload(dd, d, 4 * tot, st))))
             return s;
         s = pf_perturb_rot(st, dq.as<double>(), doff.as<int>(), dcnt.as<int>(), mx,
-                           dm.as<double>(), dk.as<double>(), pf, P, seed, (uint32_t)phase,
+                           dm.as<double>(), dk.as<double>(), 0.0, pf, P, seed, (uint32_t)phase,
                            d ? dd.as<double>() : nullptr);
         if (s == THX_OK) {
             THX_HIP(hipStreamSynchronize(st));

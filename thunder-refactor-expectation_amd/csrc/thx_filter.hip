@@ -1,0 +1,914 @@
+// Particle filter of the local search on the GPU: the per-particle update
+// around each local-search scoring phase, batched over all particles with
+// uniform sample counts (mLR rotations, mLT translations per particle, as
+// the reference's _para.mLR / _para.mLT).
+//
+// One phase (Optimiser::updateSP, src/Optimiser.cpp:1139-1437, 3-D, no CTF
+// search) is: perturb rotations about the inferACG mean with ACG(1, pf^2
+// min(1, k)) and rebalance their weights to 1 / pdfACG; perturb shifts with
+// N(0, diag(s0^2, s1^2)) pf, re-centre outliers and rebalance to 1 / N2 pdf;
+// score every (rotation, shift) sample pair (the local scorer); keep the
+// half-height peak of the rotation marginal; record the top samples; infer
+// the new concentrations k1..k3 (calVari) and shift deviations; resample
+// both sets; and test convergence (src/Optimiser.cpp:1475-1660).
+//
+// Every stage kernel is one block (or one thread) per particle and skips the
+// particles already converged.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "thx_internal.hpp"
+
+namespace thx {
+
+// 4x4 inverse and determinant, Gauss-Jordan with partial pivoting (FP64).
+__device__ bool inv4(const double* A, double* inv, double* det) {
+    double m[4][8];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 8; j++) m[i][j] = j < 4 ? A[4 * i + j] : (j - 4 == i ? 1.0 : 0.0);
+    double d = 1.0;
+    for (int c = 0; c < 4; c++) {
+        int piv = c;
+        for (int r = c + 1; r < 4; r++)
+            if (fabs(m[r][c]) > fabs(m[piv][c])) piv = r;
+        if (m[piv][c] == 0.0) return false;
+        if (piv != c) {
+            for (int j = 0; j < 8; j++) {
+                const double t = m[piv][j];
+                m[piv][j] = m[c][j];
+                m[c][j] = t;
+            }
+            d = -d;
+        }
+        d *= m[c][c];
+        const double pv = m[c][c];
+        for (int j = 0; j < 8; j++) m[c][j] /= pv;
+        for (int r = 0; r < 4; r++)
+            if (r != c) {
+                const double f = m[r][c];
+                for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+            }
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) inv[4 * i + j] = m[i][j + 4];
+    if (det) *det = d;
+    return true;
+}
+
+// Eigenvector of the largest eigenvalue of a symmetric 4x4 (cyclic Jacobi),
+// unit length, largest-magnitude entry positive.
+__device__ void sym4_top(const double* Ain, double* vec) {
+    double A[4][4], V[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            A[i][j] = Ain[4 * i + j];
+            V[i][j] = i == j ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0;
+        for (int a = 0; a < 4; a++)
+            for (int b = a + 1; b < 4; b++) off += A[a][b] * A[a][b];
+        if (off < 1e-40) break;
+        for (int a = 0; a < 4; a++)
+            for (int b = a + 1; b < 4; b++) {
+                if (fabs(A[a][b]) < 1e-300) continue;
+                const double th = (A[b][b] - A[a][a]) / (2 * A[a][b]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                const double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 4; k++) {
+                    const double x = A[k][a], y = A[k][b];
+                    A[k][a] = c * x - s * y;
+                    A[k][b] = s * x + c * y;
+                }
+                for (int k = 0; k < 4; k++) {
+                    const double x = A[a][k], y = A[b][k];
+                    A[a][k] = c * x - s * y;
+                    A[b][k] = s * x + c * y;
+                }
+                for (int k = 0; k < 4; k++) {
+                    const double x = V[k][a], y = V[k][b];
+                    V[k][a] = c * x - s * y;
+                    V[k][b] = s * x + c * y;
+                }
+            }
+    }
+    int top = 0;
+    for (int a = 1; a < 4; a++)
+        if (A[a][a] > A[top][top]) top = a;
+    int big = 0;
+    for (int a = 1; a < 4; a++)
+        if (fabs(V[a][top]) > fabs(V[big][top])) big = a;
+    const double sg = V[big][top] < 0 ? -1.0 : 1.0;
+    double n = 0.0;
+    for (int a = 0; a < 4; a++) n += V[a][top] * V[a][top];
+    n = sqrt(n);
+    for (int a = 0; a < 4; a++) vec[a] = sg * V[a][top] / n;
+}
+
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block-wide sum (256 threads); every thread gets the total.
+__device__ double block_sum(double v, double* part) {
+    v = wave_sum(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return part[0] + part[1] + part[2] + part[3];
+}
+
+// Fixed point of inferACG(dmat44&, const dmat4&) (src/Geometry/
+// DirectionalStat.cpp:87-139) over n samples, each optionally left-multiplied
+// by the quaternion pre: A <- I; repeat B = (4 / sum 1/u) sum x x^T / u,
+// u = x^T A^-1 x, until sum |A - B| <= tol; the result is the last A.
+// A, Ai (shared, 16 each), part (shared, 44), flag (shared).  Returns the
+// iteration count (negative when maxIter stopped it or A became singular).
+__device__ int block_infer_acg(const double* __restrict__ q, int n, const double* pre,
+                               double tol, int maxIter, double* A, double* Ai, double* part,
+                               int* flag) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < 16) A[tid] = (tid % 5 == 0) ? 1.0 : 0.0;
+    __syncthreads();
+    int it = 0, status = 0;
+    for (;;) {
+        if (tid == 0) *flag = inv4(A, Ai, nullptr) ? 0 : 2;
+        __syncthreads();
+        if (*flag == 2) { status = -1; break; }
+        double acc[11];
+        for (int j = 0; j < 11; j++) acc[j] = 0.0;
+        for (int i = tid; i < n; i += blockDim.x) {
+            double x[4] = {q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]};
+            if (pre) {
+                double y[4];
+                qmul(pre, x, y);
+                for (int a = 0; a < 4; a++) x[a] = y[a];
+            }
+            double u = 0.0;
+            for (int a = 0; a < 4; a++) {
+                double v = 0.0;
+                for (int b = 0; b < 4; b++) v += Ai[4 * a + b] * x[b];
+                u += x[a] * v;
+            }
+            const double iu = 1.0 / u;
+            int j = 0;
+            for (int a = 0; a < 4; a++)
+                for (int b = a; b < 4; b++) acc[j++] += x[a] * x[b] * iu;
+            acc[10] += iu;
+        }
+        for (int j = 0; j < 11; j++) {
+            const double v = wave_sum(acc[j]);
+            if (lane == 0) part[wave * 11 + j] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double sm[11];
+            for (int j = 0; j < 11; j++)
+                sm[j] = part[j] + part[11 + j] + part[22 + j] + part[33 + j];
+            const double f = 4.0 / sm[10];
+            double B[16];
+            int j = 0;
+            for (int a = 0; a < 4; a++)
+                for (int b = a; b < 4; b++) {
+                    B[4 * a + b] = B[4 * b + a] = f * sm[j];
+                    j++;
+                }
+            double diff = 0.0;
+            for (int e = 0; e < 16; e++) diff += fabs(A[e] - B[e]);
+            if (diff <= tol) {
+                *flag = 1;
+            } else if (it + 1 >= maxIter) {
+                *flag = 3;
+            } else {
+                for (int e = 0; e < 16; e++) A[e] = B[e];
+                *flag = 0;
+            }
+        }
+        __syncthreads();
+        it++;
+        if (*flag == 1) break;
+        if (*flag == 3) { status = -2; break; }
+    }
+    __syncthreads();
+    return status < 0 ? status : it;
+}
+
+// Per particle over its n rotation samples (flags): 1 = balance weights
+// w_i = 1 / pdfACG(q_i, A) normalised (Particle::balanceWeight(PAR_R),
+// src/Particle.cpp:2301-2311; pdfACG = det(A)^-1/2 (x^T A^-1 x)^-2,
+// src/Geometry/DirectionalStat.cpp:13-18); 2 = mean (inferACG(dvec4&, ...),
+// :218-244); 4 = calVari concentrations k_a = A'(a,a) / A'(0,0) of inferACG
+// of conj(mean) q (src/Particle.cpp:1043-1067, DirectionalStat.cpp:178-216).
+// 8 = also write the balance weights as FP32 priors for the scorer.
+__global__ __launch_bounds__(256) void k_rot_stats(
+    const double* __restrict__ quat, int n, const int* __restrict__ active, int flags, double tol,
+    int maxIter, double* __restrict__ Aout, double* __restrict__ meanOut,
+    double* __restrict__ kOut, double* __restrict__ wBal, float* __restrict__ wBalF,
+    int* __restrict__ itOut) {
+    const int p = blockIdx.x;
+    if (active && !active[p]) return;
+    __shared__ double A[16], Ai[16], part[44], pre[4], det;
+    __shared__ int flag;
+    const double* q = quat + (size_t)p * n * 4;
+    int it = block_infer_acg(q, n, nullptr, tol, maxIter, A, Ai, part, &flag);
+    if (Aout && threadIdx.x < 16) Aout[(size_t)p * 16 + threadIdx.x] = A[threadIdx.x];
+    if (flags & 1) {
+        if (threadIdx.x == 0) {
+            double d = 1.0;
+            if (!inv4(A, Ai, &d)) d = 0.0;
+            det = d;
+        }
+        __syncthreads();
+        double s = 0.0;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const double* x = q + 4 * i;
+            double u = 0.0;
+            for (int a = 0; a < 4; a++) {
+                double v = 0.0;
+                for (int b = 0; b < 4; b++) v += Ai[4 * a + b] * x[b];
+                u += x[a] * v;
+            }
+            s += sqrt(fmax(det, 0.0)) * u * u;
+        }
+        const double tot = block_sum(s, part);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const double* x = q + 4 * i;
+            double u = 0.0;
+            for (int a = 0; a < 4; a++) {
+                double v = 0.0;
+                for (int b = 0; b < 4; b++) v += Ai[4 * a + b] * x[b];
+                u += x[a] * v;
+            }
+            const double w = sqrt(fmax(det, 0.0)) * u * u / tot;
+            wBal[(size_t)p * n + i] = w;
+            if (flags & 8) wBalF[(size_t)p * n + i] = (float)w;
+        }
+        __syncthreads();
+    }
+    if (flags & 6) {
+        if (threadIdx.x == 0) {
+            double m[4];
+            sym4_top(A, m);
+            pre[0] = m[0];
+            pre[1] = -m[1];
+            pre[2] = -m[2];
+            pre[3] = -m[3];
+            if (meanOut)
+                for (int a = 0; a < 4; a++) meanOut[4 * (size_t)p + a] = m[a];
+        }
+        __syncthreads();
+        if (flags & 4) {
+            const int it2 = block_infer_acg(q, n, pre, tol, maxIter, A, Ai, part, &flag);
+            if (threadIdx.x == 0)
+                for (int a = 0; a < 3; a++) kOut[3 * (size_t)p + a] = A[5 * (a + 1)] / A[0];
+            if (it2 < 0) it = it2;
+        }
+    }
+    if (itOut && threadIdx.x == 0) itOut[p] = it;
+}
+
+// Shifts of one particle (one thread per particle, n samples): optional
+// perturbation t += pf (s0 g0, s1 g1) (Particle::perturb(pf, PAR_T),
+// src/Particle.cpp:1227-1262, rho = 0) with re-centring of samples farther
+// than transM from the origin to N(0, transS^2 I) (reCentre, :2427-2449);
+// then means, n - 1 standard deviations (calVari(PAR_T), :1083-1097) and the
+// balance weights 1 / N2(t - m; s0, s1) normalised (balanceWeight(PAR_T),
+// :2313-2344).  s01 (in: the perturbation's deviations; out: the new ones).
+__global__ void k_trans_stats(double* __restrict__ trans, int n, const int* __restrict__ active,
+                              int P, int perturb, double* __restrict__ s01, double pf,
+                              const double* __restrict__ transS, double transQ, uint64_t seed,
+                              uint32_t phase, double* __restrict__ mOut, double* __restrict__ wBal,
+                              float* __restrict__ wBalF) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || (active && !active[p])) return;
+    double* t = trans + (size_t)p * n * 2;
+    if (perturb) {
+        const double s0 = s01[3 * p], s1 = s01[3 * p + 1];
+        const double tS = transS[p];
+        const double transM = tS * (-2.0 * log(transQ));   // tS * chi2_2 Qinv(transQ)
+        for (int i = 0; i < n; i++) {
+            double g[4];
+            normals4(seed, (uint32_t)((size_t)p * n + i), phase, kStreamShift + 48, g);
+            t[2 * i] += pf * s0 * g[0];
+            t[2 * i + 1] += pf * s1 * g[1];
+            if (sqrt(t[2 * i] * t[2 * i] + t[2 * i + 1] * t[2 * i + 1]) > transM) {
+                t[2 * i] = tS * g[2];
+                t[2 * i + 1] = tS * g[3];
+            }
+        }
+    }
+    double m0 = 0.0, m1 = 0.0;
+    for (int i = 0; i < n; i++) {
+        m0 += t[2 * i];
+        m1 += t[2 * i + 1];
+    }
+    m0 /= n;
+    m1 /= n;
+    double v0 = 0.0, v1 = 0.0;
+    for (int i = 0; i < n; i++) {
+        v0 += (t[2 * i] - m0) * (t[2 * i] - m0);
+        v1 += (t[2 * i + 1] - m1) * (t[2 * i + 1] - m1);
+    }
+    const double s0 = sqrt(v0 / (n - 1)), s1 = sqrt(v1 / (n - 1));
+    s01[3 * p] = s0;
+    s01[3 * p + 1] = s1;
+    s01[3 * p + 2] = 0.0;
+    if (mOut) {
+        mOut[2 * p] = m0;
+        mOut[2 * p + 1] = m1;
+    }
+    if (!wBal) return;
+    // 1 / pdf up to the common factor 2 pi s0 s1, normalised
+    double sum = 0.0;
+    for (int i = 0; i < n; i++) {
+        const double x = (t[2 * i] - m0) / s0, y = (t[2 * i + 1] - m1) / s1;
+        sum += exp(0.5 * (x * x + y * y));
+    }
+    for (int i = 0; i < n; i++) {
+        const double x = (t[2 * i] - m0) / s0, y = (t[2 * i + 1] - m1) / s1;
+        const double w = exp(0.5 * (x * x + y * y)) / sum;
+        wBal[(size_t)p * n + i] = w;
+        if (wBalF) wBalF[(size_t)p * n + i] = (float)w;
+    }
+}
+
+// setPeakFactor (setFactor) + keepHalfHeightPeak + calRank1st of one sample
+// type (src/Particle.cpp:988-1000, 1864-1954): u (n per particle, >= 0) in
+// place.  The factor is clamp(u_(n/8) / u_(0), 1e-3, 0.5) over the descending
+// order (rotations, 3-D); hh = max(u) factor; u_i <- u_i < hh ? 0 : u_i - hh.
+// topOut receives the sample (dim values) of the first maximum.  One block per
+// particle; the rank statistic is found by bisection on the FP64 bit pattern.
+__global__ __launch_bounds__(256) void k_keep_peak(
+    double* __restrict__ u, int n, const int* __restrict__ active, double* __restrict__ factor,
+    int setFactor, int keep, const double* __restrict__ samples, int dim,
+    double* __restrict__ topOut, int* __restrict__ topIdx) {
+    const int p = blockIdx.x;
+    if (active && !active[p]) return;
+    double* up = u + (size_t)p * n;
+    __shared__ double rmax[256];
+    __shared__ int ridx[256];
+    __shared__ double part[4];
+    __shared__ double sh[2];
+    double mx = -1.0;
+    int mi = 0x7fffffff;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (up[i] > mx) { mx = up[i]; mi = i; }
+    rmax[threadIdx.x] = mx;
+    ridx[threadIdx.x] = mi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double o = rmax[threadIdx.x + s];
+            const int oi = ridx[threadIdx.x + s];
+            if (o > rmax[threadIdx.x] || (o == rmax[threadIdx.x] && oi < ridx[threadIdx.x])) {
+                rmax[threadIdx.x] = o;
+                ridx[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const double umax = rmax[0];
+    const int best = ridx[0];
+    if (setFactor) {
+        // largest key K with count(key >= K) >= n/8 + 1 (keys of u >= 0 order as u)
+        const unsigned long long need = (unsigned long long)(n / 8 + 1);
+        unsigned long long lo = 0ull, hi = (unsigned long long)__double_as_longlong(umax);
+        while (lo < hi) {
+            const unsigned long long mid = lo + (hi - lo + 1) / 2;
+            double c = 0.0;
+            for (int i = threadIdx.x; i < n; i += blockDim.x)
+                c += (unsigned long long)__double_as_longlong(up[i]) >= mid ? 1.0 : 0.0;
+            const double tot = block_sum(c, part);
+            if (tot >= (double)need) lo = mid; else hi = mid - 1;
+        }
+        if (threadIdx.x == 0) {
+            const double f = umax > 0 ? __longlong_as_double((long long)lo) / umax : 0.0;
+            factor[p] = fmin(0.5, fmax(1e-3, f));
+        }
+        __syncthreads();
+    }
+    if (keep) {
+        if (threadIdx.x == 0) sh[0] = umax * factor[p];
+        __syncthreads();
+        const double hh = sh[0];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) up[i] = up[i] < hh ? 0.0 : up[i] - hh;
+    }
+    if (threadIdx.x < dim && topOut)
+        topOut[(size_t)p * dim + threadIdx.x] = samples[((size_t)p * n + best) * dim + threadIdx.x];
+    if (threadIdx.x == 0 && topIdx) topIdx[p] = best;
+}
+
+// After systematic resampling: particle p's new samples are its (or the
+// shared grid's, srcShared) samples idx[p][j]; the new weights wNew are its
+// prior (and its FP32 copy for the scorer).  Converged particles are left.
+__global__ void k_gather_samples(const double* __restrict__ src, int nSrc, int srcShared,
+                                 const int* __restrict__ idx, const double* __restrict__ wNew,
+                                 int n, int dim, const int* __restrict__ active, int P,
+                                 double* __restrict__ dst, double* __restrict__ w,
+                                 float* __restrict__ wF) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)P * n) return;
+    const int p = (int)(g / n);
+    if (active && !active[p]) return;
+    const size_t s = (srcShared ? 0 : (size_t)p * nSrc) + idx[g];
+    for (int d = 0; d < dim; d++) dst[g * dim + d] = src[s * dim + d];
+    w[g] = wNew[g];
+    if (wF) wF[g] = (float)wNew[g];
+}
+
+// Rows of converged particles copied into the other buffer of a pair.
+__global__ void k_keep_rows(const double* __restrict__ src, double* __restrict__ dst, int rowLen,
+                            const int* __restrict__ active, int P) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)P * rowLen) return;
+    if (active[g / rowLen]) return;
+    dst[g] = src[g];
+}
+
+// Convergence of the local search (src/Optimiser.cpp:1511-1620 with
+// OPTIMISER_COMPRESS_CRITERIA, include/Config.h:222): variR = (k1 k2 k3)^(1/6)
+// (Particle::variR, src/Particle.cpp:609-625), variT = s0 s1 (variT, :627-638,
+// rho = 0).  From phase minPhase on, a phase with neither below 0.95 x the
+// best so far counts; the first such phase ends the particle's search
+// (N_PHASE_WITH_NO_VARI_DECREASE = 1, include/Optimiser.h:56-63).
+__global__ void k_converge(const double* __restrict__ kk, const double* __restrict__ s01,
+                           int* __restrict__ active, double* __restrict__ best,
+                           int* __restrict__ nPhase, int P, int phase, int minPhase) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || !active[p]) return;
+    if (phase < minPhase) return;
+    const double vR = pow(kk[3 * p] * kk[3 * p + 1] * kk[3 * p + 2], 1.0 / 6);
+    const double vT = s01[3 * p] * s01[3 * p + 1];
+    const double bR = best[2 * p], bT = best[2 * p + 1];
+    // the defocus term of the reference (no CTF search) only counts on the
+    // first check, where every best is still DBL_MAX
+    const bool first = bR == 1.7976931348623157e308;
+    const bool dec = first || vR < bR * 0.95 || vT < bT * 0.95;
+    if (vR < bR) best[2 * p] = vR;
+    if (vT < bT) best[2 * p + 1] = vT;
+    if (!dec) {
+        active[p] = 0;
+        nPhase[p] = phase;
+    }
+}
+
+// min-std clamps after the global scan (src/Optimiser.cpp:1031-1084 with
+// OPTIMISER_SCAN_SET_MIN_STD_WITH_PERTURB): k_a >= (minR / pfS)^2,
+// s_a >= minT / pfS.
+__global__ void k_min_std(double* __restrict__ kk, double* __restrict__ s01, int P, double kMin,
+                          double sMin) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    for (int a = 0; a < 3; a++) kk[3 * p + a] = fmax(kMin, kk[3 * p + a]);
+    for (int a = 0; a < 2; a++) s01[3 * p + a] = fmax(sMin, s01[3 * p + a]);
+}
+
+constexpr double kAcgTol = 1e-3;     // inferACG's convergence bound
+constexpr int kAcgMaxIter = 1000;
+constexpr double kTransQ = 0.05;     // TRANS_Q, include/Optimiser.h:67
+
+static unsigned nb(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace thx
+
+using namespace thx;
+
+namespace {
+
+int pf_alloc(thx_ctx* c, int P, int mLR, int mLT) {
+    c->pfP = P;
+    c->pfR = mLR;
+    c->pfT = mLT;
+    const size_t nr = (size_t)P * mLR, nt = (size_t)P * mLT;
+    THX_TRY(c->pfQ.ensure(sizeof(double) * 4 * nr));
+    THX_TRY(c->pfQ2.ensure(sizeof(double) * 4 * nr));
+    THX_TRY(c->pfTr.ensure(sizeof(double) * 2 * nt));
+    THX_TRY(c->pfTr2.ensure(sizeof(double) * 2 * nt));
+    THX_TRY(c->pfWR.ensure(sizeof(double) * nr));
+    THX_TRY(c->pfWT.ensure(sizeof(double) * nt));
+    THX_TRY(c->pfWRf.ensure(sizeof(float) * nr));
+    THX_TRY(c->pfWTf.ensure(sizeof(float) * nt));
+    THX_TRY(c->pfIdx.ensure(sizeof(int) * std::max(nr, nt)));
+    THX_TRY(c->pfWNew.ensure(sizeof(double) * std::max(nr, nt)));
+    THX_TRY(c->pfK.ensure(sizeof(double) * 3 * P));
+    THX_TRY(c->pfS.ensure(sizeof(double) * 3 * P));
+    THX_TRY(c->pfMean.ensure(sizeof(double) * 4 * P));
+    THX_TRY(c->pfTopR.ensure(sizeof(double) * 4 * P));
+    THX_TRY(c->pfTopT.ensure(sizeof(double) * 2 * P));
+    THX_TRY(c->pfPeak.ensure(sizeof(double) * P));
+    THX_TRY(c->pfTransS.ensure(sizeof(double) * P));
+    THX_TRY(c->pfBest.ensure(sizeof(double) * 2 * P));
+    THX_TRY(c->pfActive.ensure(sizeof(int) * P));
+    THX_TRY(c->pfNPhase.ensure(sizeof(int) * P));
+    std::vector<int> offR(P), offT(P), nR(P, mLR), nT(P, mLT);
+    for (int p = 0; p < P; p++) {
+        offR[p] = p * mLR;
+        offT[p] = p * mLT;
+    }
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const std::vector<int> ones(P, 1);
+    THX_HIP(hipMemcpy(c->pfActive.p, ones.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+    std::vector<double> best(2 * (size_t)P, 1.7976931348623157e308);
+    std::vector<int> np(P, -1);
+    THX_HIP(hipMemcpy(c->pfBest.p, best.data(), sizeof(double) * 2 * P, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pfNPhase.p, np.data(), sizeof(int) * P, hipMemcpyHostToDevice));
+    auto put = [&](DevBuf& d, const std::vector<int>& v) -> int {
+        THX_TRY(d.ensure(sizeof(int) * v.size()));
+        THX_HIP(hipMemcpy(d.p, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+        return THX_OK;
+    };
+    THX_TRY(put(c->pfOffR, offR));
+    THX_TRY(put(c->pfOffT, offT));
+    THX_TRY(put(c->pfNR, nR));
+    THX_TRY(put(c->pfNT, nT));
+    c->pfValid = true;
+    return THX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int thx_pf_set_state(thx_ctx* c, int mLR, int mLT, const double* quat, const double* trans,
+                     const double* k, const double* s01, const double* peakR,
+                     const double* transS) {
+    clear_error();
+    THX_CHECK(c && quat && trans && k && s01 && peakR && transS, THX_ERR_ARG,
+              "pf_set_state: null argument");
+    THX_CHECK(c->P > 0, THX_ERR_STATE, "pf_set_state: particles must be set first");
+    THX_CHECK(mLR >= 2 && mLR <= kResampleMaxN && mLT >= 2 && mLT <= 128, THX_ERR_RANGE,
+              "pf_set_state: mLR must be in [2, 8192] and mLT in [2, 128]");
+    THX_HIP(hipSetDevice(c->device));
+    const int P = c->P;
+    THX_TRY(pf_alloc(c, P, mLR, mLT));
+    THX_HIP(hipMemcpy(c->pfQ.p, quat, sizeof(double) * 4 * P * (size_t)mLR, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pfTr.p, trans, sizeof(double) * 2 * P * (size_t)mLT, hipMemcpyHostToDevice));
+    std::vector<double> s3(3 * (size_t)P);
+    for (int p = 0; p < P; p++) {
+        s3[3 * p] = s01[2 * p];
+        s3[3 * p + 1] = s01[2 * p + 1];
+        s3[3 * p + 2] = 0.0;
+    }
+    THX_HIP(hipMemcpy(c->pfK.p, k, sizeof(double) * 3 * P, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pfS.p, s3.data(), sizeof(double) * 3 * P, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pfPeak.p, peakR, sizeof(double) * P, hipMemcpyHostToDevice));
+    THX_HIP(hipMemcpy(c->pfTransS.p, transS, sizeof(double) * P, hipMemcpyHostToDevice));
+    return THX_OK;
+}
+
+int thx_pf_from_global(thx_ctx* c, int cls, int mLR, int mLT, double minStdR, double minStdT,
+                       double pfS, const double* transS, uint64_t seed) {
+    clear_error();
+    THX_CHECK(c && transS && pfS > 0, THX_ERR_ARG, "pf_from_global: bad argument");
+    THX_CHECK(cls >= 0 && cls < (int)c->res.size() && c->res[cls].valid, THX_ERR_STATE,
+              "pf_from_global: no global scan of this class");
+    THX_CHECK(mLR >= 2 && mLR <= kResampleMaxN && mLT >= 2 && mLT <= 128 &&
+                  c->R <= kResampleMaxN && c->T <= kResampleMaxN,
+              THX_ERR_RANGE, "pf_from_global: sizes out of range");
+    THX_CHECK(c->pR.p && c->pT.p && c->pR.bytes >= sizeof(double) * c->R &&
+                  c->pT.bytes >= sizeof(double) * c->T && c->quat.p && c->trans.p,
+              THX_ERR_STATE, "pf_from_global: grid and its priors must be set");
+    THX_CHECK(c->res[cls].wR.bytes >= sizeof(double) * (size_t)c->P * c->R &&
+                  c->res[cls].wT.bytes >= sizeof(double) * (size_t)c->P * c->T,
+              THX_ERR_STATE, "pf_from_global: marginals of the last scan do not match the grid");
+    THX_HIP(hipSetDevice(c->device));
+    const int P = c->P, R = c->R, T = c->T;
+    hipStream_t st = c->stream;
+    THX_TRY(pf_alloc(c, P, mLR, mLT));
+    THX_HIP(hipMemcpy(c->pfTransS.p, transS, sizeof(double) * P, hipMemcpyHostToDevice));
+    ClassResult& res = c->res[cls];
+    // marginals of the grid: copies (the peak keeping works in place)
+    DevBuf uR, uT, offR, offT, nRg, nTg;
+    std::vector<int> oR(P), oT(P), cR(P, R), cT(P, T);
+    for (int p = 0; p < P; p++) {
+        oR[p] = p * R;
+        oT[p] = p * T;
+    }
+    int s = THX_OK;
+    auto cleanup = [&]() {
+        DevBuf* b[] = {&uR, &uT, &offR, &offT, &nRg, &nTg};
+        for (DevBuf* x : b) x->release();
+    };
+    if ((s = uR.ensure(sizeof(double) * (size_t)P * R)) || (s = uT.ensure(sizeof(double) * (size_t)P * T)) ||
+        (s = offR.ensure(sizeof(int) * P)) || (s = offT.ensure(sizeof(int) * P)) ||
+        (s = nRg.ensure(sizeof(int) * P)) || (s = nTg.ensure(sizeof(int) * P))) {
+        cleanup();
+        return s;
+    }
+    THX_HIP(hipMemcpyAsync(uR.p, res.wR.p, sizeof(double) * (size_t)P * R, hipMemcpyDeviceToDevice, st));
+    THX_HIP(hipMemcpyAsync(uT.p, res.wT.p, sizeof(double) * (size_t)P * T, hipMemcpyDeviceToDevice, st));
+    THX_HIP(hipMemcpyAsync(offR.p, oR.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+    THX_HIP(hipMemcpyAsync(offT.p, oT.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+    THX_HIP(hipMemcpyAsync(nRg.p, cR.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+    THX_HIP(hipMemcpyAsync(nTg.p, cT.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+    THX_HIP(hipStreamSynchronize(st));
+    // setUR + setPeakFactor + keepHalfHeightPeak (rotations), rank of both
+    hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, uR.as<double>(), R, nullptr,
+                       c->pfPeak.as<double>(), 1, 1, nullptr, 4, nullptr, nullptr);
+    THX_HIP(hipGetLastError());
+    // resample mLR rotations / mLT shifts from the grid (prior = the grid's)
+    s = pf_resample(st, c->pR.as<double>(), uR.as<double>(), offR.as<int>(), nRg.as<int>(),
+                    c->pfOffR.as<int>(), c->pfNR.as<int>(), P, seed, 0u, nullptr,
+                    c->pfIdx.as<int>(), c->pfWNew.as<double>(), nullptr, 1);
+    if (s) { cleanup(); return s; }
+    hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLR)), dim3(256), 0, st,
+                       c->quat.as<const double>(), R, 1, c->pfIdx.as<const int>(),
+                       c->pfWNew.as<const double>(), mLR, 4, nullptr, P, c->pfQ.as<double>(),
+                       c->pfWR.as<double>(), c->pfWRf.as<float>());
+    s = pf_resample(st, c->pT.as<double>(), uT.as<double>(), offT.as<int>(), nTg.as<int>(),
+                    c->pfOffT.as<int>(), c->pfNT.as<int>(), P, seed, 1u, nullptr,
+                    c->pfIdx.as<int>(), c->pfWNew.as<double>(), nullptr, 1);
+    if (s) { cleanup(); return s; }
+    hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLT)), dim3(256), 0, st,
+                       c->trans.as<const double>(), T, 1, c->pfIdx.as<const int>(),
+                       c->pfWNew.as<const double>(), mLT, 2, nullptr, P, c->pfTr.as<double>(),
+                       c->pfWT.as<double>(), c->pfWTf.as<float>());
+    // calVari of both sets, then the scanning minimum deviations
+    hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
+                       nullptr, 6, kAcgTol, kAcgMaxIter, nullptr, c->pfMean.as<double>(),
+                       c->pfK.as<double>(), nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, c->pfTr.as<double>(), mLT,
+                       nullptr, P, 0, c->pfS.as<double>(), 0.0, c->pfTransS.as<const double>(),
+                       kTransQ, seed, 0u, nullptr, nullptr, nullptr);
+    const double kMin = (minStdR / pfS) * (minStdR / pfS), sMin = minStdT / pfS;
+    hipLaunchKernelGGL(k_min_std, dim3(nb(P)), dim3(256), 0, st, c->pfK.as<double>(),
+                       c->pfS.as<double>(), P, kMin, sMin);
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipStreamSynchronize(st));
+    cleanup();
+    return THX_OK;
+}
+
+int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, double pfL,
+                  double pfS, int maxPhase, uint64_t seed, int* phasesRun) {
+    clear_error();
+    THX_CHECK(c && c->pfValid && c->pfP == c->P, THX_ERR_STATE,
+              "pf_refine: particle-filter state not set for the context's particles");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
+              "pf_refine: precision must be FP32 or BF16");
+    THX_CHECK(!c->vol.empty() && c->K > 0, THX_ERR_STATE, "pf_refine: volume and pixels needed");
+    const int nCls = (int)c->vol.size();
+    for (int k = 0; k < nCls; k++)
+        THX_CHECK(c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
+                  "pf_refine: every class volume must be set with vdim = idim * pf");
+    THX_HIP(hipSetDevice(c->device));
+    const int P = c->P, mLR = c->pfR, mLT = c->pfT;
+    if (cls)
+        for (int p = 0; p < P; p++)
+            THX_CHECK(cls[p] >= 0 && cls[p] < nCls, THX_ERR_ARG, "pf_refine: bad class");
+    hipStream_t st = c->stream;
+    // scorer inputs that stay fixed over the phases
+    std::vector<const float2*> vp(nCls);
+    for (int k = 0; k < nCls; k++) vp[k] = c->vol[k].as<const float2>();
+    THX_TRY(upload(c->locVols, vp.data(), vp.size(), st));
+    if (cls) THX_TRY(upload(c->locPcls, cls, P, st));
+    const int TPAD = mLT <= 32 ? 32 : (mLT <= 64 ? 64 : 128);
+    const int nbp = (mLR + kLocalRotBlock - 1) / kLocalRotBlock;
+    THX_TRY(c->locBMax.ensure(sizeof(float) * (size_t)P * nbp));
+    THX_TRY(c->locBWC.ensure(sizeof(float) * (size_t)P * nbp));
+    THX_TRY(c->locBArg.ensure(sizeof(int) * (size_t)P * nbp));
+    THX_TRY(c->locBWT.ensure(sizeof(float) * (size_t)P * nbp * TPAD));
+    THX_TRY(c->locWRrel.ensure(sizeof(float) * (size_t)P * mLR));
+    THX_TRY(c->locWC.ensure(sizeof(double) * P));
+    THX_TRY(c->locWR.ensure(sizeof(double) * (size_t)P * mLR));
+    THX_TRY(c->locWT.ensure(sizeof(double) * (size_t)P * mLT));
+    THX_TRY(c->locBase.ensure(sizeof(double) * P));
+    THX_TRY(c->locBestR.ensure(sizeof(int) * P));
+    THX_TRY(c->locBestT.ensure(sizeof(int) * P));
+    THX_TRY(c->pfNRa.ensure(sizeof(int) * P));
+    THX_TRY(c->pfNTa.ensure(sizeof(int) * P));
+    THX_TRY(launch_prep_particles(c, THX_FP32));
+    const int start = afterGlobal ? 1 : 0;
+    const int minPhase = afterGlobal ? 10 : 3;   // MIN_N_PHASE_PER_ITER_{GLOBAL,LOCAL}
+    std::vector<int> act(P), nRa(P), nTa(P), bStart(P + 1);
+    std::vector<int2> blocks;
+    THX_HIP(hipMemcpy(act.data(), c->pfActive.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    int phase = start, ran = 0;
+    for (; phase < maxPhase; phase++) {
+        int nAct = 0;
+        blocks.clear();
+        for (int p = 0; p < P; p++) {
+            bStart[p] = (int)blocks.size();
+            nRa[p] = act[p] ? mLR : 0;
+            nTa[p] = act[p] ? mLT : 0;
+            if (act[p]) {
+                nAct++;
+                for (int r0 = 0; r0 < mLR; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
+            }
+        }
+        bStart[P] = (int)blocks.size();
+        if (nAct == 0) break;
+        ran++;
+        const double pf = phase == 0 ? pfL : pfS;
+        const int* dAct = c->pfActive.as<const int>();
+        // rotations: mean of the current samples, perturbation, balance + calVari
+        hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
+                           dAct, 2, kAcgTol, kAcgMaxIter, nullptr, c->pfMean.as<double>(),
+                           nullptr, nullptr, nullptr, nullptr);
+        THX_HIP(hipMemcpyAsync(c->pfNRa.p, nRa.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+        THX_HIP(hipMemcpyAsync(c->pfNTa.p, nTa.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+        THX_TRY(pf_perturb_rot(st, c->pfQ.as<double>(), c->pfOffR.as<int>(), c->pfNRa.as<int>(),
+                               mLR, c->pfMean.as<double>(), c->pfK.as<double>(), 1.0, pf, P, seed,
+                               (uint32_t)(2 * phase), nullptr));
+        hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
+                           dAct, 1 | 4 | 8, kAcgTol, kAcgMaxIter, nullptr, nullptr,
+                           c->pfK.as<double>(), c->pfWR.as<double>(), c->pfWRf.as<float>(), nullptr);
+        // shifts: perturbation, re-centring, balance + calVari
+        hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, c->pfTr.as<double>(), mLT,
+                           dAct, P, 1, c->pfS.as<double>(), pf, c->pfTransS.as<const double>(),
+                           kTransQ, seed, (uint32_t)(2 * phase + 1), nullptr,
+                           c->pfWT.as<double>(), c->pfWTf.as<float>());
+        THX_HIP(hipGetLastError());
+        // scoring of every active particle's samples (projection fused)
+        THX_TRY(upload(c->locBlocks, blocks.data(), blocks.size(), st));
+        THX_TRY(upload(c->locBStart, bStart.data(), P + 1, st));
+        LocalBatch b;
+        b.vols = c->locVols.as<const float2* const>();
+        b.pcls = cls ? c->locPcls.as<const int>() : nullptr;
+        b.quat = c->pfQ.as<const double>();
+        b.pRf = c->pfWRf.as<const float>();
+        b.rOff = c->pfOffR.as<const int>();
+        b.nR = c->pfNRa.as<const int>();
+        b.trans = c->pfTr.as<const double>();
+        b.pTf = c->pfWTf.as<const float>();
+        b.tOff = c->pfOffT.as<const int>();
+        b.nT = c->pfNTa.as<const int>();
+        b.blocks = c->locBlocks.as<const int2>();
+        b.bStart = c->locBStart.as<const int>();
+        b.bMax = c->locBMax.as<float>();
+        b.bWC = c->locBWC.as<float>();
+        b.bArg = c->locBArg.as<int>();
+        b.bWT = c->locBWT.as<float>();
+        b.wRrel = c->locWRrel.as<float>();
+        THX_TRY(launch_score_local(c, precision, TPAD, (int)blocks.size(), b));
+        THX_TRY(launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
+                                   c->locWT.as<double>(), c->locBase.as<double>(),
+                                   c->locBestR.as<int>(), c->locBestT.as<int>()));
+        // half-height peak of the rotation marginal, top samples
+        hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWR.as<double>(), mLR,
+                           dAct, c->pfPeak.as<double>(), 0, 1, c->pfQ.as<const double>(), 4,
+                           c->pfTopR.as<double>(), nullptr);
+        hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWT.as<double>(), mLT,
+                           dAct, c->pfPeak.as<double>(), 0, 0, c->pfTr.as<const double>(), 2,
+                           c->pfTopT.as<double>(), nullptr);
+        // resampling of both sets
+        THX_TRY(pf_resample(st, c->pfWR.as<double>(), c->locWR.as<double>(), c->pfOffR.as<int>(),
+                            c->pfNRa.as<int>(), c->pfOffR.as<int>(), c->pfNRa.as<int>(), P, seed,
+                            (uint32_t)(1000 + 2 * phase), nullptr, c->pfIdx.as<int>(),
+                            c->pfWNew.as<double>(), nullptr));
+        hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLR)), dim3(256), 0, st,
+                           c->pfQ.as<const double>(), mLR, 0, c->pfIdx.as<const int>(),
+                           c->pfWNew.as<const double>(), mLR, 4, dAct, P, c->pfQ2.as<double>(),
+                           c->pfWR.as<double>(), c->pfWRf.as<float>());
+        THX_TRY(pf_resample(st, c->pfWT.as<double>(), c->locWT.as<double>(), c->pfOffT.as<int>(),
+                            c->pfNTa.as<int>(), c->pfOffT.as<int>(), c->pfNTa.as<int>(), P, seed,
+                            (uint32_t)(1001 + 2 * phase), nullptr, c->pfIdx.as<int>(),
+                            c->pfWNew.as<double>(), nullptr));
+        hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLT)), dim3(256), 0, st,
+                           c->pfTr.as<const double>(), mLT, 0, c->pfIdx.as<const int>(),
+                           c->pfWNew.as<const double>(), mLT, 2, dAct, P, c->pfTr2.as<double>(),
+                           c->pfWT.as<double>(), c->pfWTf.as<float>());
+        // the gathered sets become current (converged particles were copied
+        // neither way: keep their rows by copying them back first)
+        hipLaunchKernelGGL(k_keep_rows, dim3(nb((size_t)P * mLR * 4)), dim3(256), 0, st,
+                           c->pfQ.as<const double>(), c->pfQ2.as<double>(), mLR * 4, dAct, P);
+        hipLaunchKernelGGL(k_keep_rows, dim3(nb((size_t)P * mLT * 2)), dim3(256), 0, st,
+                           c->pfTr.as<const double>(), c->pfTr2.as<double>(), mLT * 2, dAct, P);
+        std::swap(c->pfQ, c->pfQ2);
+        std::swap(c->pfTr, c->pfTr2);
+        hipLaunchKernelGGL(k_converge, dim3(nb(P)), dim3(256), 0, st, c->pfK.as<const double>(),
+                           c->pfS.as<const double>(), c->pfActive.as<int>(), c->pfBest.as<double>(),
+                           c->pfNPhase.as<int>(), P, phase, minPhase);
+        THX_HIP(hipGetLastError());
+        THX_HIP(hipMemcpyAsync(act.data(), c->pfActive.p, sizeof(int) * P, hipMemcpyDeviceToHost, st));
+        THX_HIP(hipStreamSynchronize(st));
+    }
+    if (phasesRun) *phasesRun = ran;
+    c->pfLastPhase = phase;
+    return THX_OK;
+}
+
+int thx_pf_get_state(thx_ctx* c, double* quat, double* trans, double* wR, double* wT, double* k,
+                     double* s01, double* topR, double* topT, double* peakR, int* nPhase) {
+    clear_error();
+    THX_CHECK(c && c->pfValid, THX_ERR_STATE, "pf_get_state: no particle-filter state");
+    THX_HIP(hipSetDevice(c->device));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const size_t P = c->pfP, nr = P * c->pfR, nt = P * c->pfT;
+    if (quat) THX_HIP(hipMemcpy(quat, c->pfQ.p, sizeof(double) * 4 * nr, hipMemcpyDeviceToHost));
+    if (trans) THX_HIP(hipMemcpy(trans, c->pfTr.p, sizeof(double) * 2 * nt, hipMemcpyDeviceToHost));
+    if (wR) THX_HIP(hipMemcpy(wR, c->pfWR.p, sizeof(double) * nr, hipMemcpyDeviceToHost));
+    if (wT) THX_HIP(hipMemcpy(wT, c->pfWT.p, sizeof(double) * nt, hipMemcpyDeviceToHost));
+    if (k) THX_HIP(hipMemcpy(k, c->pfK.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost));
+    if (s01) {
+        std::vector<double> s3(3 * P);
+        THX_HIP(hipMemcpy(s3.data(), c->pfS.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost));
+        for (size_t p = 0; p < P; p++) {
+            s01[2 * p] = s3[3 * p];
+            s01[2 * p + 1] = s3[3 * p + 1];
+        }
+    }
+    if (topR) THX_HIP(hipMemcpy(topR, c->pfTopR.p, sizeof(double) * 4 * P, hipMemcpyDeviceToHost));
+    if (topT) THX_HIP(hipMemcpy(topT, c->pfTopT.p, sizeof(double) * 2 * P, hipMemcpyDeviceToHost));
+    if (peakR) THX_HIP(hipMemcpy(peakR, c->pfPeak.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    if (nPhase) THX_HIP(hipMemcpy(nPhase, c->pfNPhase.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    return THX_OK;
+}
+
+// Stage kernels on host buffers, for checking: rotation statistics of P sets
+// of n samples (A[16], mean[4], k[3], balance weights[n], iterations).
+int thx_pf_rot_stats(thx_ctx* c, const double* quat, int n, int P, double* A, double* mean,
+                     double* k, double* wBal, int* iters) {
+    clear_error();
+    THX_CHECK(c && quat && n >= 1 && P >= 1, THX_ERR_ARG, "pf_rot_stats: bad argument");
+    THX_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    DevBuf dq, dA, dm, dk, dw, di;
+    const size_t N = (size_t)P * n;
+    int s = THX_OK;
+    if ((s = upload(dq, quat, 4 * N, st)) || (s = dA.ensure(sizeof(double) * 16 * P)) ||
+        (s = dm.ensure(sizeof(double) * 4 * P)) || (s = dk.ensure(sizeof(double) * 3 * P)) ||
+        (s = dw.ensure(sizeof(double) * N)) || (s = di.ensure(sizeof(int) * P)))
+        return s;
+    hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, dq.as<const double>(), n, nullptr,
+                       1 | 2 | 4, kAcgTol, kAcgMaxIter, dA.as<double>(), dm.as<double>(),
+                       dk.as<double>(), dw.as<double>(), nullptr, di.as<int>());
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipStreamSynchronize(st));
+    if (A) THX_HIP(hipMemcpy(A, dA.p, sizeof(double) * 16 * P, hipMemcpyDeviceToHost));
+    if (mean) THX_HIP(hipMemcpy(mean, dm.p, sizeof(double) * 4 * P, hipMemcpyDeviceToHost));
+    if (k) THX_HIP(hipMemcpy(k, dk.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost));
+    if (wBal) THX_HIP(hipMemcpy(wBal, dw.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    if (iters) THX_HIP(hipMemcpy(iters, di.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    DevBuf* all[] = {&dq, &dA, &dm, &dk, &dw, &di};
+    for (DevBuf* b : all) b->release();
+    return THX_OK;
+}
+
+// Shift statistics of P sets of n samples: means[2], deviations[2] (n - 1)
+// and balance weights[n].
+int thx_pf_trans_stats(thx_ctx* c, const double* trans, int n, int P, double* m, double* s01,
+                       double* wBal) {
+    clear_error();
+    THX_CHECK(c && trans && n >= 2 && P >= 1, THX_ERR_ARG, "pf_trans_stats: bad argument");
+    THX_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    DevBuf dt, dm, ds, dw;
+    const size_t N = (size_t)P * n;
+    int s = THX_OK;
+    if ((s = upload(dt, trans, 2 * N, st)) || (s = dm.ensure(sizeof(double) * 2 * P)) ||
+        (s = ds.ensure(sizeof(double) * 3 * P)) || (s = dw.ensure(sizeof(double) * N)))
+        return s;
+    hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, dt.as<double>(), n, nullptr,
+                       P, 0, ds.as<double>(), 0.0, nullptr, kTransQ, 0ull, 0u, dm.as<double>(),
+                       dw.as<double>(), nullptr);
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipStreamSynchronize(st));
+    std::vector<double> s3(3 * (size_t)P);
+    THX_HIP(hipMemcpy(s3.data(), ds.p, sizeof(double) * 3 * P, hipMemcpyDeviceToHost));
+    if (s01)
+        for (int p = 0; p < P; p++) {
+            s01[2 * p] = s3[3 * p];
+            s01[2 * p + 1] = s3[3 * p + 1];
+        }
+    if (m) THX_HIP(hipMemcpy(m, dm.p, sizeof(double) * 2 * P, hipMemcpyDeviceToHost));
+    if (wBal) THX_HIP(hipMemcpy(wBal, dw.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    DevBuf* all[] = {&dt, &dm, &ds, &dw};
+    for (DevBuf* b : all) b->release();
+    return THX_OK;
+}
+
+// Peak keeping of P sets of n values u (in place): factor[P] in/out
+// (setFactor: computed), top[P] the first maximum.
+int thx_pf_keep_peak(thx_ctx* c, double* u, int n, int P, double* factor, int setFactor,
+                     int* top) {
+    clear_error();
+    THX_CHECK(c && u && factor && n >= 1 && P >= 1, THX_ERR_ARG, "pf_keep_peak: bad argument");
+    THX_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    DevBuf du, df, dtop;
+    const size_t N = (size_t)P * n;
+    int s = THX_OK;
+    if ((s = upload(du, u, N, st)) || (s = upload(df, factor, P, st)) ||
+        (s = dtop.ensure(sizeof(int) * P)))
+        return s;
+    hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, du.as<double>(), n, nullptr,
+                       df.as<double>(), setFactor, 1, nullptr, 0, nullptr, dtop.as<int>());
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipStreamSynchronize(st));
+    THX_HIP(hipMemcpy(u, du.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(factor, df.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    if (top) THX_HIP(hipMemcpy(top, dtop.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    DevBuf* all[] = {&du, &df, &dtop};
+    for (DevBuf* b : all) b->release();
+    return THX_OK;
+}
+
+}  // extern "C"

@@ -49,6 +49,18 @@ struct DevBuf {
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// Host array -> device buffer (grown as needed), asynchronous on st.
+template <class T>
+int upload(DevBuf& d, const T* h, size_t n, hipStream_t st) {
+    int s = d.ensure(sizeof(T) * (n ? n : 1));
+    if (s) return s;
+    if (n && hipMemcpyAsync(d.p, h, sizeof(T) * n, hipMemcpyHostToDevice, st) != hipSuccess) {
+        set_error("upload: hipMemcpyAsync failed");
+        return 2;
+    }
+    return 0;
+}
+
 // Tile geometry of the scoring kernel: a block covers 128 (particle,
 // translation) rows x 128 rotations.  TC rows belong to one particle.
 constexpr int kTileM = 128;
@@ -138,6 +150,13 @@ struct thx_ctx {
     std::vector<int> lpBestR, lpBestT, lpFell;
     std::vector<double> lpScore, lpEssH, lpSpreadH, lpTailH;
     bool lpValid = false;
+    // particle-filter state of the local search: mLR rotations and mLT shifts
+    // per particle (FP64), weights, concentrations, deviations, tops
+    int pfP = 0, pfR = 0, pfT = 0, pfLastPhase = 0;
+    bool pfValid = false;
+    thx::DevBuf pfQ, pfQ2, pfTr, pfTr2, pfWR, pfWT, pfWRf, pfWTf, pfIdx, pfWNew, pfK, pfS,
+        pfMean, pfTopR, pfTopT, pfPeak, pfTransS, pfBest, pfActive, pfNPhase, pfOffR, pfOffT,
+        pfNR, pfNT, pfNRa, pfNTa;
     // timing
     hipEvent_t ev[16] = {};
     int nev = 0;
@@ -173,8 +192,8 @@ int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double*
 
 // Particle-filter launchers (thx_pf.hip).
 int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
-                   const double* mean, const double* kk, double pf, int P, uint64_t seed,
-                   uint32_t phase, const double* dGiven);
+                   const double* mean, const double* kk, double kmax, double pf, int P,
+                   uint64_t seed, uint32_t phase, const double* dGiven);
 int pf_perturb_trans(hipStream_t st, double* trans, const int* off, const int* cnt, int maxCnt,
                      const double* s01rho, double pf, int P, uint64_t seed, uint32_t phase);
 int pf_sample_mixture(hipStream_t st, const double* mix, int C, int P, int nRot, int nTr,
@@ -182,7 +201,8 @@ int pf_sample_mixture(hipStream_t st, const double* mix, int C, int P, int nRot,
                       double* trans);
 int pf_resample(hipStream_t st, const double* w, const double* u, const int* offIn,
                 const int* nIn, const int* offOut, const int* nOut, int P, uint64_t seed,
-                uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess);
+                uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess,
+                int wShared = 0);
 int pf_rot_mean(hipStream_t st, const double* quat, const double* w, const int* off,
                 const int* cnt, int P, double* mean, double* kk);
 constexpr int kResampleMaxN = 8192;
@@ -239,5 +259,41 @@ enum Stream : uint32_t {
     kStreamCtf = 4,
     kStreamNoise = 5
 };
+
+// Hamilton product (quaternion_mul, src/Geometry/Euler.cpp:13-26).
+__device__ inline void qmul(const double a[4], const double b[4], double o[4]) {
+    const double w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    const double x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    const double y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    const double z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = w; o[1] = x; o[2] = y; o[3] = z;
+}
+
+__device__ inline void normals4(uint64_t seed, uint32_t a, uint32_t b, uint32_t stream,
+                                double g[4]) {
+    uint32_t w[4], v[4];
+    Philox::draw(seed, a, 2 * b, stream, w);
+    Philox::draw(seed, a, 2 * b + 1, stream, v);
+    const double u1 = Philox::u01(w[0], w[1]), u2 = Philox::u01(w[2], w[3]);
+    const double u3 = Philox::u01(v[0], v[1]), u4 = Philox::u01(v[2], v[3]);
+    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+    double s1, c1, s2, c2;
+    sincospi(2.0 * u2, &s1, &c1);
+    sincospi(2.0 * u4, &s2, &c2);
+    g[0] = r1 * c1; g[1] = r1 * s1; g[2] = r2 * c2; g[3] = r2 * s2;
+}
+
+// ACG(diag(1, k1, k2, k3)) sample (sampleACG, src/Geometry/DirectionalStat.cpp:
+// 33-85): v ~ N(0, I), v = L v with L = chol(diag), normalise.
+__device__ inline void acg_identity(const double g[4], double k1, double k2, double k3,
+                                    double d[4]) {
+    d[0] = g[0];
+    d[1] = sqrt(k1) * g[1];
+    d[2] = sqrt(k2) * g[2];
+    d[3] = sqrt(k3) * g[3];
+    const double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3]);
+    for (int i = 0; i < 4; i++) d[i] /= n;
+}
+
 
 }  // namespace thx

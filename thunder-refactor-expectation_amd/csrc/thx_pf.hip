@@ -16,41 +16,6 @@
 
 namespace thx {
 
-// Hamilton product (quaternion_mul, src/Geometry/Euler.cpp:13-26).
-__device__ inline void qmul(const double a[4], const double b[4], double o[4]) {
-    const double w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
-    const double x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
-    const double y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
-    const double z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
-    o[0] = w; o[1] = x; o[2] = y; o[3] = z;
-}
-
-__device__ inline void normals4(uint64_t seed, uint32_t a, uint32_t b, uint32_t stream,
-                                double g[4]) {
-    uint32_t w[4], v[4];
-    Philox::draw(seed, a, 2 * b, stream, w);
-    Philox::draw(seed, a, 2 * b + 1, stream, v);
-    const double u1 = Philox::u01(w[0], w[1]), u2 = Philox::u01(w[2], w[3]);
-    const double u3 = Philox::u01(v[0], v[1]), u4 = Philox::u01(v[2], v[3]);
-    const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
-    double s1, c1, s2, c2;
-    sincospi(2.0 * u2, &s1, &c1);
-    sincospi(2.0 * u4, &s2, &c2);
-    g[0] = r1 * c1; g[1] = r1 * s1; g[2] = r2 * c2; g[3] = r2 * s2;
-}
-
-// ACG(diag(1, k1, k2, k3)) sample (sampleACG, src/Geometry/DirectionalStat.cpp:
-// 33-85): v ~ N(0, I), v = L v with L = chol(diag), normalise.
-__device__ inline void acg_identity(const double g[4], double k1, double k2, double k3,
-                                    double d[4]) {
-    d[0] = g[0];
-    d[1] = sqrt(k1) * g[1];
-    d[2] = sqrt(k2) * g[2];
-    d[3] = sqrt(k3) * g[3];
-    const double n = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + d[3] * d[3]);
-    for (int i = 0; i < 4; i++) d[i] /= n;
-}
-
 // ---------------------------------------------------------------------------
 // Rotation perturbation (Particle::perturb(pf, PAR_R), src/Particle.cpp:
 // 1133-1273, 3-D branch): d ~ ACG(1, pf^2 k1, pf^2 k2, pf^2 k3) and
@@ -58,8 +23,8 @@ __device__ inline void acg_identity(const double g[4], double k1, double k2, dou
 // ---------------------------------------------------------------------------
 __global__ void k_perturb_rot(double* __restrict__ quat, const int* __restrict__ off,
                               const int* __restrict__ cnt, const double* __restrict__ mean,
-                              const double* __restrict__ kk, double pf, int P, uint64_t seed,
-                              uint32_t phase, const double* __restrict__ dGiven) {
+                              const double* __restrict__ kk, double kmax, double pf, int P,
+                              uint64_t seed, uint32_t phase, const double* __restrict__ dGiven) {
     const int p = blockIdx.y;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P || j >= cnt[p]) return;
@@ -70,8 +35,12 @@ __global__ void k_perturb_rot(double* __restrict__ quat, const int* __restrict__
     } else {
         double g[4];
         normals4(seed, (uint32_t)i, phase, kStreamPose + 16, g);
+        // k clamped to kmax when kmax > 0 (PERTURB_K_MAX, include/Particle.h:56)
         const double s = pf * pf;
-        acg_identity(g, s * kk[3 * p], s * kk[3 * p + 1], s * kk[3 * p + 2], d);
+        double k3[3] = {kk[3 * p], kk[3 * p + 1], kk[3 * p + 2]};
+        if (kmax > 0)
+            for (int a = 0; a < 3; a++) k3[a] = fmin(kmax, k3[a]);
+        acg_identity(g, s * k3[0], s * k3[1], s * k3[2], d);
     }
     const double m[4] = {mean[4 * p], mean[4 * p + 1], mean[4 * p + 2], mean[4 * p + 3]};
     const double mc[4] = {m[0], -m[1], -m[2], -m[3]};
@@ -171,11 +140,12 @@ __global__ __launch_bounds__(256) void k_resample(
     const double* __restrict__ w, const double* __restrict__ u, const int* __restrict__ offIn,
     const int* __restrict__ nIn, const int* __restrict__ offOut, const int* __restrict__ nOut,
     int P, uint64_t seed, uint32_t phase, const double* __restrict__ u0Given,
-    int* __restrict__ idx, double* __restrict__ wNew, double* __restrict__ essOut) {
+    int* __restrict__ idx, double* __restrict__ wNew, double* __restrict__ essOut, int wShared) {
     const int p = blockIdx.x;
     if (p >= P) return;
     const int n = nIn[p], m = nOut[p];
-    const double* wp = w + offIn[p];
+    if (n <= 0 || m <= 0) return;
+    const double* wp = wShared ? w : w + offIn[p];   // wShared: one prior for every particle
     const double* up = u + offIn[p];
     __shared__ double cdf[kResampleMax];
     __shared__ double part[256];
@@ -492,11 +462,11 @@ int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double
 // Launchers
 // ---------------------------------------------------------------------------
 int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
-                   const double* mean, const double* kk, double pf, int P, uint64_t seed,
-                   uint32_t phase, const double* dGiven) {
+                   const double* mean, const double* kk, double kmax, double pf, int P,
+                   uint64_t seed, uint32_t phase, const double* dGiven) {
     dim3 grid((maxCnt + 255) / 256, P);
-    hipLaunchKernelGGL(k_perturb_rot, grid, dim3(256), 0, st, quat, off, cnt, mean, kk, pf, P,
-                       seed, phase, dGiven);
+    hipLaunchKernelGGL(k_perturb_rot, grid, dim3(256), 0, st, quat, off, cnt, mean, kk, kmax, pf,
+                       P, seed, phase, dGiven);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
@@ -523,9 +493,10 @@ int pf_sample_mixture(hipStream_t st, const double* mix, int C, int P, int nRot,
 
 int pf_resample(hipStream_t st, const double* w, const double* u, const int* offIn,
                 const int* nIn, const int* offOut, const int* nOut, int P, uint64_t seed,
-                uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess) {
+                uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess,
+                int wShared) {
     hipLaunchKernelGGL(k_resample, dim3(P), dim3(256), 0, st, w, u, offIn, nIn, offOut, nOut, P,
-                       seed, phase, u0Given, idx, wNew, ess);
+                       seed, phase, u0Given, idx, wNew, ess, wShared);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }

@@ -64,6 +64,13 @@ _SIGS = {
     "thx_cnn_components": [_p, _p],
     "thx_ctx_expect_learned": [_p, _p, _i, _i, _i, _d, _d, _d, _d, _i, _p],
     "thx_ctx_get_learned": [_p, _p, _p, _p, _p, _p, _p, _p],
+    "thx_pf_set_state": [_p, _i, _i, _p, _p, _p, _p, _p, _p],
+    "thx_pf_from_global": [_p, _i, _i, _i, _d, _d, _d, _p, _u64],
+    "thx_pf_refine": [_p, _p, _i, _i, _d, _d, _i, _u64, _p],
+    "thx_pf_get_state": [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    "thx_pf_rot_stats": [_p, _p, _i, _i, _p, _p, _p, _p, _p],
+    "thx_pf_trans_stats": [_p, _p, _i, _i, _p, _p, _p],
+    "thx_pf_keep_peak": [_p, _p, _i, _i, _p, _i, _p],
     "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
                                 _u64, _p, _p, _p],
@@ -281,6 +288,67 @@ class Context:
         check(load().thx_ctx_get_local(self._h, ptr(wC), ptr(wR), ptr(wT), ptr(base), ptr(bR),
                                        ptr(bT)), "get_local")
         return wC, wR, wT, base, bR, bT
+
+    # ---- particle-filter local search ----
+    def pf_set_state(self, quat, trans, k, s01, peakR, transS):
+        quat = c32(quat, np.float64).reshape(self.P, -1, 4)
+        trans = c32(trans, np.float64).reshape(self.P, -1, 2)
+        self.mLR, self.mLT = quat.shape[1], trans.shape[1]
+        check(load().thx_pf_set_state(self._h, self.mLR, self.mLT, ptr(quat), ptr(trans),
+                                      ptr(c32(k, np.float64)), ptr(c32(s01, np.float64)),
+                                      ptr(c32(peakR, np.float64)), ptr(c32(transS, np.float64))),
+              "pf_set_state")
+
+    def pf_from_global(self, cls, mLR, mLT, min_std_r, min_std_t, pf_s, transS, seed=1):
+        self.mLR, self.mLT = mLR, mLT
+        transS = np.broadcast_to(np.asarray(transS, np.float64), (self.P,)).copy()
+        check(load().thx_pf_from_global(self._h, cls, mLR, mLT, min_std_r, min_std_t, pf_s,
+                                        ptr(transS), seed), "pf_from_global")
+
+    def pf_refine(self, cls=None, precision=FP32, after_global=False, pf_l=2.0, pf_s=0.5,
+                  max_phase=100, seed=1):
+        n = C.c_int(0)
+        cl = None if cls is None else c32(cls, np.int32)
+        check(load().thx_pf_refine(self._h, ptr(cl), precision, int(after_global), pf_l, pf_s,
+                                   max_phase, seed, C.byref(n)), "pf_refine")
+        return n.value
+
+    def pf_get_state(self):
+        P, R, T = self.P, self.mLR, self.mLT
+        out = dict(quat=np.zeros((P, R, 4)), trans=np.zeros((P, T, 2)), wR=np.zeros((P, R)),
+                   wT=np.zeros((P, T)), k=np.zeros((P, 3)), s01=np.zeros((P, 2)),
+                   topR=np.zeros((P, 4)), topT=np.zeros((P, 2)), peakR=np.zeros(P),
+                   nPhase=np.zeros(P, np.int32))
+        check(load().thx_pf_get_state(self._h, *[ptr(out[k]) for k in
+                                                 ("quat", "trans", "wR", "wT", "k", "s01", "topR",
+                                                  "topT", "peakR", "nPhase")]), "pf_get_state")
+        return out
+
+    def pf_rot_stats(self, quat):
+        quat = c32(quat, np.float64)
+        P, n = quat.shape[0], quat.shape[1]
+        A, mean, k = np.zeros((P, 4, 4)), np.zeros((P, 4)), np.zeros((P, 3))
+        w, it = np.zeros((P, n)), np.zeros(P, np.int32)
+        check(load().thx_pf_rot_stats(self._h, ptr(quat), n, P, ptr(A), ptr(mean), ptr(k),
+                                      ptr(w), ptr(it)), "pf_rot_stats")
+        return A, mean, k, w, it
+
+    def pf_trans_stats(self, trans):
+        trans = c32(trans, np.float64)
+        P, n = trans.shape[0], trans.shape[1]
+        m, s, w = np.zeros((P, 2)), np.zeros((P, 2)), np.zeros((P, n))
+        check(load().thx_pf_trans_stats(self._h, ptr(trans), n, P, ptr(m), ptr(s), ptr(w)),
+              "pf_trans_stats")
+        return m, s, w
+
+    def pf_keep_peak(self, u, factor, set_factor):
+        u = c32(u, np.float64).copy()
+        P, n = u.shape
+        f = np.broadcast_to(np.asarray(factor, np.float64), (P,)).copy()
+        top = np.zeros(P, np.int32)
+        check(load().thx_pf_keep_peak(self._h, ptr(u), n, P, ptr(f), int(set_factor), ptr(top)),
+              "pf_keep_peak")
+        return u, f, top
 
     # ---- learned proposal ----
     def expect_learned(self, net, cls=0, n_rot=300, n_trans=25, ess_min=0.0, spread_max=0.0,
