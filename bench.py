@@ -120,6 +120,32 @@ def run_learned(ctx, net, cal, steps, warmup, P, R, T):
                 kernels_ms=kern, fallback=float(np.mean(nfs)))
 
 
+def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
+    """Particle states from the last global scan, then `phases` local-search
+    phases (perturb, balance, BF16 scoring with fused projection, peak,
+    calVari, resample, convergence test) over every particle."""
+    quat = info["quat"]
+    w = wl.WORKLOADS[name]
+    ctx.pf_from_global(0, mLR, mLT, w["n_rot"] ** (-1.0 / 3), 0.3, 0.5, 1.0, seed=3)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ran = ctx.pf_refine(None, thx.BF16, after_global=True, max_phase=1 + phases, seed=4)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    st = ctx.pf_get_state()
+    err = wl.geodesic_angle(st["topR"], info["true_quat"])
+    P, K = ctx.P, ctx.K
+    poses = float(P) * mLR * mLT
+    samples = float(P) * mLR * K
+    return {"phases": ran, "ms_per_phase": 1e3 * dt / max(ran, 1),
+            "samples_per_particle": [mLR, mLT],
+            "pose_evals_per_s": poses * ran / dt,
+            "trilinear_samples_per_s": samples * ran / dt,
+            "median_error_deg_vs_truth": float(np.degrees(np.median(err))),
+            "note": "raw pose-evals (each particle's own samples); projection fused into the "
+                    "scorer, one trilinear slice per (particle, rotation)"}
+
+
 def agreement(quat, trans, step, bR, bT, gR, gT):
     ang = wl.geodesic_angle(quat[bR], quat[gR])
     return dict(within_one_step=float(np.mean(ang <= step * (1 + 1e-6))),
@@ -173,6 +199,7 @@ def main():
     ap.add_argument("--rule", default="",
                     help="n_rot,n_trans,spread_max,tail_max,score_drop: skip the calibration")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--local-phases", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -247,6 +274,9 @@ def main():
                           "calibration": cal, "kernels_ms": r["kernels_ms"],
                           "sampling_step_deg": float(np.degrees(step))}
         net.close()
+    log("local search")
+    out["local_search"] = run_local_search(ctx, info, name, args.local_phases)
+    log(f"local search: {out['local_search']}")
     out.update({
         "value": r["pose_evals_per_s"], "ms_per_step": r["ms_per_step"], "dtype": dtype,
         "config": {"workload": f"{name}: box {w['box']}, {P} particles x {R} rotations x "
