@@ -220,6 +220,12 @@ def main():
         res[m] = run_mode(ctx, m, args.steps, args.warmup, P, R, T, K)
         res[m]["best"] = ctx.get_best(0)
         log(f"{m}: {res[m]['ms_per_step']:.1f} ms/step {res[m]['kernels_ms']}")
+    log("mode fp8")
+    res["fp8"] = run_mode(ctx, "fp8", args.steps, args.warmup, P, R, T, K)
+    res["fp8"]["best"] = ctx.get_best(0)
+    log(f"fp8: {res['fp8']['ms_per_step']:.1f} ms/step {res['fp8']['kernels_ms']}")
+    # the mixed mode's marginals stay in the context for the local search
+    run_mode(ctx, args.mixed, 1, 0, P, R, T, K)
     gR, gT, _ = res["fp32"]["best"]
     f = res["fp32"]
     mg = res[args.mixed]
@@ -239,6 +245,15 @@ def main():
              "speedup_vs_plain_fp32": mg["pose_evals_per_s"] / f["pose_evals_per_s"],
              "agreement_with_plain_fp32": agreement(quat, trans, step, *mg["best"][:2], gR, gT)}
     mixed["score_frac_of_peak"] = mixed["score_tflops"] * 1e12 / PEAK[args.mixed]
+    f8 = res["fp8"]
+    fp8 = {"value": f8["pose_evals_per_s"], "ms_per_step": f8["ms_per_step"],
+           "kernels_ms": f8["kernels_ms"],
+           "score_tflops": f8["score_flops"] / (f8["kernels_ms"]["score"] * 1e-3) / 1e12,
+           "speedup_vs_plain_fp32": f8["pose_evals_per_s"] / f["pose_evals_per_s"],
+           "agreement_with_plain_fp32": agreement(quat, trans, step, *f8["best"][:2], gR, gT)}
+    fp8["score_frac_of_peak"] = fp8["score_tflops"] * 1e12 / PEAK["fp8"]
+    fp8["best_score_rel_err_vs_fp32"] = float(np.median(np.abs(f8["best"][2] - res["fp32"]["best"][2])
+                                                        / np.abs(res["fp32"]["best"][2])))
     if args.no_learned:
         r, dtype = mg, args.mixed
         form = f"whole-grid scan, {args.mixed} cross term"
@@ -284,7 +299,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roof,
         "speedup_vs_plain_fp32": r["pose_evals_per_s"] / f["pose_evals_per_s"],
-        "plain_fp32": plain, "mixed_global": mixed,
+        "plain_fp32": plain, "mixed_global": mixed, "fp8_global": fp8,
     })
     if not args.no_cpu:
         log("cpu baseline")

@@ -134,7 +134,7 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
     THX_TRY(choose_scan_shape(c, precision));
     THX_TRY(ensure_scan_buffers(c, cls));
     if (precision == THX_FP16 || precision == THX_FP8) {
-        THX_TRY(c->shellScale.ensure(sizeof(float) * (c->maxShell + 1)));
+        THX_TRY(c->shellScale.ensure(sizeof(float) * (c->maxShell + 2)));   // + global factor
         THX_TRY(c->shellMax.ensure(sizeof(float) * 2 * (c->maxShell + 1)));
         THX_TRY(launch_shell_scale(c, precision));
         THX_TRY(record(c, "shell_scale"));
@@ -396,8 +396,8 @@ int thx_ctx_set_grid(thx_ctx* c, const double* quat, int R, const double* trans,
 int thx_ctx_score_global(thx_ctx* c, int cls, int precision) {
     clear_error();
     THX_CHECK(c, THX_ERR_ARG, "score_global: null context");
-    THX_CHECK(precision >= THX_FP32 && precision <= THX_FP16, THX_ERR_ARG,
-              "score_global: precision must be FP32, BF16 or FP16");
+    THX_CHECK(precision >= THX_FP32 && precision <= THX_FP8, THX_ERR_ARG,
+              "score_global: unknown precision");
     THX_CHECK(c->K > 0 && c->P > 0 && c->R > 0 && c->T > 0, THX_ERR_STATE,
               "score_global: pixels, particles and grid must be set");
     THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] > 0, THX_ERR_STATE,
@@ -730,8 +730,8 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
                            int fallbackPrecision, int* nFallback) {
     clear_error();
     THX_CHECK(c && net && nSelR > 0 && nSelT > 0, THX_ERR_ARG, "expect_learned: bad argument");
-    THX_CHECK(fallbackPrecision >= THX_FP32 && fallbackPrecision <= THX_FP16, THX_ERR_ARG,
-              "expect_learned: fallback precision must be FP32, BF16 or FP16");
+    THX_CHECK(fallbackPrecision >= THX_FP32 && fallbackPrecision <= THX_FP8, THX_ERR_ARG,
+              "expect_learned: unknown fallback precision");
     THX_CHECK(c->K > 0 && c->P > 0 && c->R > 0 && c->T > 0, THX_ERR_STATE,
               "expect_learned: pixels, particles and grid must be set");
     THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vdim[cls] == c->idim * c->pf,

@@ -34,6 +34,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------
 // Geometry helpers (device)
@@ -251,35 +252,61 @@ __global__ void k_shell_max(const float2* __restrict__ src, int rows, int Kp, in
     atomic_max_pos(&shellMax[iSig[k]], m);
 }
 
+// Per-shell power-of-two scales a_s = 2^rint(log2 sqrt(maxP_s / maxY_s)): Y is
+// multiplied by a_s and the slices divided by it, so every shell's largest
+// operand magnitude on both sides is sqrt(maxP_s maxY_s) and the products are
+// unchanged.  FP8 (e4m3, largest finite 448) also takes one global power of
+// two g that puts the largest balanced magnitude at or below 224 on both
+// sides: scale[s] = a_s g, the slices get g / a_s, and the accumulator is
+// multiplied by 1 / g^2.  scale[nShell] = g (1 for FP16).  One block.
 __global__ void k_shell_finish(const float* __restrict__ maxP, const float* __restrict__ maxY,
-                               int nShell, float target, float* __restrict__ scale) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nShell) return;
-    const float mp = maxP[s], my = maxY[s];
-    float a = 1.f;
-    if (mp > 0.f && my > 0.f) a = sqrtf(mp / my);
-    // round to a power of two so scaling is exact
-    a = exp2f(rintf(log2f(a)));
-    (void)target;
-    scale[s] = a;
+                               int nShell, int fp8, float* __restrict__ scale) {
+    __shared__ float red[256];
+    float m = 0.f;
+    for (int s = threadIdx.x; s < nShell; s += blockDim.x) {
+        const float mp = maxP[s], my = maxY[s];
+        float a = 1.f;
+        if (mp > 0.f && my > 0.f) a = sqrtf(mp / my);
+        a = exp2f(rintf(log2f(a)));   // a power of two keeps the scaling exact
+        scale[s] = a;
+        m = fmaxf(m, fmaxf(mp / a, my * a));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + st]);
+        __syncthreads();
+    }
+    float g = 1.f;
+    if (fp8 && red[0] > 0.f) g = exp2f(floorf(log2f(224.f / red[0])));
+    for (int s = threadIdx.x; s < nShell; s += blockDim.x) scale[s] *= g;
+    if (threadIdx.x == 0) scale[nShell] = g;
 }
 
 // ---------------------------------------------------------------------------
 // Low-precision B operand: rotLo[r][k] = P_rk / a_shell(k) in bf16 / fp16.
 // ---------------------------------------------------------------------------
+// FP8: two e4m3 bytes (re, im) per pixel into a 16-bit word of dst, the
+// slice multiplied by g^2 / scale[s] = g / a_s.
 template <int PREC>
 __global__ void k_convert_lo(const float2* __restrict__ src, int rows, int Kp,
                              const int* __restrict__ iSig, int K,
-                             const float* __restrict__ shellScale,
+                             const float* __restrict__ shellScale, int nShell,
                              uint32_t* __restrict__ dst) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)rows * Kp) return;
     const int k = (int)(i % Kp);
     float2 v = src[i];
     if (shellScale && k < K) {
-        const float inv = 1.f / shellScale[iSig[k]];
+        float inv = 1.f / shellScale[iSig[k]];
+        if (PREC == THX_FP8) inv *= shellScale[nShell] * shellScale[nShell];
         v.x *= inv;
         v.y *= inv;
+    }
+    if constexpr (PREC == THX_FP8) {
+        const int w = __builtin_amdgcn_cvt_pk_fp8_f32(v.x, v.y, 0, false);
+        reinterpret_cast<unsigned short*>(dst)[i] = (unsigned short)(w & 0xffff);
+        return;
     }
     uint32_t lo, hi;
     if (PREC == THX_BF16) {
@@ -315,8 +342,8 @@ int launch_shell_scale(thx_ctx* c, int precision) {
     hipLaunchKernelGGL(k_shell_max, gridY, dim3(256), 0, c->stream, c->Y.as<const float2>(),
                        c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), maxY);
     THX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_shell_finish, dim3((nShell + 63) / 64), dim3(64), 0, c->stream, maxP,
-                       maxY, nShell, 1.f, c->shellScale.as<float>());
+    hipLaunchKernelGGL(k_shell_finish, dim3(1), dim3(256), 0, c->stream, maxP, maxY, nShell,
+                       precision == THX_FP8 ? 1 : 0, c->shellScale.as<float>());
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
@@ -325,15 +352,18 @@ int launch_convert_lo(thx_ctx* c, int precision) {
     const size_t n = (size_t)c->Rpad * c->Kp;
     const float* scale = (precision == THX_FP16 || precision == THX_FP8)
                              ? c->shellScale.as<const float>() : nullptr;
+    const int nShell = c->maxShell + 1;
     dim3 grid((unsigned)((n + 255) / 256));
+#define THX_CONV_ARGS                                                                        \
+    c->rotP.as<const float2>(), c->Rpad, c->Kp, c->iSig.as<const int>(), c->K, scale, nShell, \
+        c->rotLo.as<uint32_t>()
     if (precision == THX_BF16)
-        hipLaunchKernelGGL(k_convert_lo<THX_BF16>, grid, dim3(256), 0, c->stream,
-                           c->rotP.as<const float2>(), c->Rpad, c->Kp, c->iSig.as<const int>(),
-                           c->K, scale, c->rotLo.as<uint32_t>());
+        hipLaunchKernelGGL(k_convert_lo<THX_BF16>, grid, dim3(256), 0, c->stream, THX_CONV_ARGS);
+    else if (precision == THX_FP16)
+        hipLaunchKernelGGL(k_convert_lo<THX_FP16>, grid, dim3(256), 0, c->stream, THX_CONV_ARGS);
     else
-        hipLaunchKernelGGL(k_convert_lo<THX_FP16>, grid, dim3(256), 0, c->stream,
-                           c->rotP.as<const float2>(), c->Rpad, c->Kp, c->iSig.as<const int>(),
-                           c->K, scale, c->rotLo.as<uint32_t>());
+        hipLaunchKernelGGL(k_convert_lo<THX_FP8>, grid, dim3(256), 0, c->stream, THX_CONV_ARGS);
+#undef THX_CONV_ARGS
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
@@ -720,14 +750,17 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     const float* __restrict__ pTf, int P, int R, int T, int Kp, int Rpad,
     int Tpad, int nRb, int nTc, int nPg, float* __restrict__ partMax,
     float* __restrict__ partWC, int* __restrict__ partArg,
-    float* __restrict__ partWR, float* __restrict__ partWT) {
+    float* __restrict__ partWR, float* __restrict__ partWT,
+    const float* __restrict__ gScale) {
     constexpr int PB = 128 / TC;
     constexpr int NB = kNB16;
-    constexpr int LD = kLD16;
+    constexpr bool F8 = PREC == THX_FP8;
+    constexpr int LD = kLD16;                 // 16-bit elements per LDS row
+    constexpr int LDB = F8 ? 80 : 2 * LD;     // bytes per LDS row (64 or 128 + 16 pad)
     constexpr int NTH = 512;
     // two LDS stages: chunk c + 1 is staged while chunk c feeds the MFMAs
-    __shared__ __attribute__((aligned(16))) unsigned short As[2][128 * LD];
-    __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NB * LD];
+    __shared__ __attribute__((aligned(16))) unsigned char AsB[2][128 * LDB];
+    __shared__ __attribute__((aligned(16))) unsigned char BsB[2][NB * LDB];
     __shared__ float redMax[4][4];        // [32-row tile][column quarter]
     __shared__ int redArg[4][4];
     __shared__ float tileMax[PB];
@@ -759,16 +792,20 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     //      step; rows m = tid / 16 + 32 v (v = 0..3).  Row m is particle
     //      m / TC, translation m % TC: a thread needs NT = TC / 32 phase rows
     //      and 4 / NT particle rows, one float4 each.
-    //   B: lane part8 = tid % 8 owns 4 complex pixels of rows tid / 8 + 64 v.
+    //   B: lane part8 = tid % 8 owns 4 complex pixels of rows tid / 8 + 64 v
+    //      (FP8: lane part4 = tid % 4 owns 8 pixels of rows tid / 4 + 128 v).
     constexpr int NT = TC / 32;
     constexpr int NY = 4 / NT;
     const int part = tid & 15, rbase = tid >> 4;      // rbase 0..31
     const int part8 = tid & 7, rb8 = tid >> 3;        // rb8 0..63
     const float4* yp = reinterpret_cast<const float4*>(Y + (size_t)(p0 + rbase / TC) * Kp) + part;
     const float4* sp = reinterpret_cast<const float4*>(S + (size_t)(t0 + rbase % TC) * Kp) + part;
-    const u32x4* bp = reinterpret_cast<const u32x4*>(Bop + (size_t)(r0 + rb8) * Kp) + part8;
+    const int part4 = tid & 3, rb4 = tid >> 2;        // rb4 0..127 (FP8)
+    const u32x4* bp = F8 ? reinterpret_cast<const u32x4*>(
+                               reinterpret_cast<const unsigned char*>(Bop) + (size_t)(r0 + rb4) * 2 * Kp) + part4
+                         : reinterpret_cast<const u32x4*>(Bop + (size_t)(r0 + rb8) * Kp) + part8;
     const size_t rowF4 = (size_t)Kp / 2;
-    const size_t rowU4 = (size_t)Kp / 4;
+    const size_t rowU4 = F8 ? (size_t)Kp / 8 : (size_t)Kp / 4;   // B row length in 16-byte units
     // row m = rbase + 32 v: particle (rbase + 32 v) / TC, translation (rbase + 32 v) % TC
     //   TC = 32:  particle v,        translation rbase           (NY = 4, NT = 1)
     //   TC = 64:  particle v / 2,    translation rbase + 32 (v%2) (NY = 2, NT = 2)
@@ -781,7 +818,11 @@ __global__ __launch_bounds__(512, 1) void k_score16(
         const int f4_ = (K0) / 2, u4_ = (K0) / 4;                                \
         _Pragma("unroll") for (int q = 0; q < NY; q++) ry[q] = yp[(size_t)q * rowF4 + f4_]; \
         _Pragma("unroll") for (int u = 0; u < NT; u++) rs[u] = sp[(size_t)32 * u * rowF4 + f4_]; \
-        _Pragma("unroll") for (int v = 0; v < 4; v++) rbv[v] = bp[(size_t)64 * v * rowU4 + u4_]; \
+        if constexpr (F8) {                                                      \
+            _Pragma("unroll") for (int v = 0; v < 2; v++) rbv[v] = bp[(size_t)128 * v * rowU4 + (K0) / 8]; \
+        } else {                                                                 \
+            _Pragma("unroll") for (int v = 0; v < 4; v++) rbv[v] = bp[(size_t)64 * v * rowU4 + u4_]; \
+        }                                                                        \
     } while (0)
     float4 ry[NY], rs[NT];
     u32x4 rbv[4];
@@ -792,13 +833,24 @@ __global__ __launch_bounds__(512, 1) void k_score16(
             const float4 y = ry[v / NT], sv = rs[v % NT];                                      \
             const float z0r = y.x * sv.x + y.y * sv.y, z0i = y.y * sv.x - y.x * sv.y;         \
             const float z1r = y.z * sv.z + y.w * sv.w, z1i = y.w * sv.z - y.z * sv.w;         \
-            uint2 z;                                                                           \
-            z.x = pack16<PREC>(z0r, z0i);                                                      \
-            z.y = pack16<PREC>(z1r, z1i);                                                      \
-            *reinterpret_cast<uint2*>(As[BUF] + (rbase + 32 * v) * LD + 4 * part) = z;         \
+            if constexpr (F8) {                                                                \
+                int z8 = __builtin_amdgcn_cvt_pk_fp8_f32(z0r, z0i, 0, false);                  \
+                z8 = __builtin_amdgcn_cvt_pk_fp8_f32(z1r, z1i, z8, true);                      \
+                *reinterpret_cast<int*>(AsB[BUF] + (rbase + 32 * v) * LDB + 4 * part) = z8;   \
+            } else {                                                                           \
+                uint2 z;                                                                       \
+                z.x = pack16<PREC>(z0r, z0i);                                                  \
+                z.y = pack16<PREC>(z1r, z1i);                                                  \
+                *reinterpret_cast<uint2*>(AsB[BUF] + (rbase + 32 * v) * LDB + 8 * part) = z;  \
+            }                                                                                  \
         }                                                                                      \
-        _Pragma("unroll") for (int v = 0; v < 4; v++)                                          \
-            *reinterpret_cast<u32x4*>(Bs[BUF] + (rb8 + 64 * v) * LD + 8 * part8) = rbv[v];     \
+        if constexpr (F8) {                                                                    \
+            _Pragma("unroll") for (int v = 0; v < 2; v++)                                      \
+                *reinterpret_cast<u32x4*>(BsB[BUF] + (rb4 + 128 * v) * LDB + 16 * part4) = rbv[v]; \
+        } else {                                                                               \
+            _Pragma("unroll") for (int v = 0; v < 4; v++)                                      \
+                *reinterpret_cast<u32x4*>(BsB[BUF] + (rb8 + 64 * v) * LDB + 16 * part8) = rbv[v]; \
+        }                                                                                      \
     } while (0)
     THX_GLOAD16(0);
     THX_STAGE16(0);
@@ -811,8 +863,35 @@ __global__ __launch_bounds__(512, 1) void k_score16(
             if (k0 + 2 * kChunk < Kp) THX_GLOAD16(k0 + 2 * kChunk);
         }
         // ---- MFMA on the current buffer ----
-        const unsigned short* Ab = As[buf];
-        const unsigned short* Bb = Bs[buf];
+        if constexpr (F8) {
+            // one 32x32x64 e4m3 step per tile: lane half hl takes bytes 32 hl .. 32 hl + 31
+            // of its row in A and in B (the same k order on both sides)
+            intx8 ra[2], rb8v[2];
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) {
+                const unsigned char* src = AsB[buf] + (wm + 32 * mt + l32) * LDB + 32 * hl;
+                const uint4 lo = *reinterpret_cast<const uint4*>(src);
+                const uint4 hi = *reinterpret_cast<const uint4*>(src + 16);
+                ra[mt] = intx8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w,
+                               (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+            }
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                const unsigned char* src = BsB[buf] + (wn + 32 * nt + l32) * LDB + 32 * hl;
+                const uint4 lo = *reinterpret_cast<const uint4*>(src);
+                const uint4 hi = *reinterpret_cast<const uint4*>(src + 16);
+                rb8v[nt] = intx8{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w,
+                                 (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                        ra[mt], rb8v[nt], acc[mt][nt], 0, 0, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f);
+        } else {
+        const unsigned short* Ab = reinterpret_cast<const unsigned short*>(AsB[buf]);
+        const unsigned short* Bb = reinterpret_cast<const unsigned short*>(BsB[buf]);
 #pragma unroll
         for (int kk = 0; kk < 64; kk += 16) {
             const int kc = kk + 8 * hl;
@@ -837,6 +916,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
                             acc[mt][nt], 0, 0, 0);
                 }
         }
+        }
         __syncthreads();
         buf ^= 1;
     }
@@ -846,6 +926,8 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     // ---- epilogue: dvp in registers ----
     // lane owns rows m = wm + 32 mt + 8 (i / 4) + 4 hl + (i % 4), columns n = wn + 32 nt + l32.
     // TC >= 32, so all rows of one 32-row tile belong to one particle.
+    // FP8: undo the global operand scale g on both sides
+    const float accScale = F8 ? 1.f / (gScale[0] * gScale[0]) : 1.f;
     float lmax[2];
     int larg[2];
 #pragma unroll
@@ -867,7 +949,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
                 const int m = mrow0 + 8 * (i >> 2) + 4 * hl + (i & 3);
                 const int t = t0 + m % TC;
                 float v = -INFINITY;
-                if (rok && t < T) v = a + cterm - 2.f * acc[mt][nt][i];
+                if (rok && t < T) v = a + cterm - 2.f * accScale * acc[mt][nt][i];
                 acc[mt][nt][i] = v;
                 if (v > lmax[mt]) { lmax[mt] = v; larg[mt] = m * NB + n; }
             }
@@ -998,7 +1080,8 @@ static void launch_score_prec(thx_ctx* c, const void* B, int nPg, int nblk) {
     c->Y.as<const float2>(), c->S.as<const float2>(), B16, c->Cterm.as<const float>(),        \
         c->Aterm.as<const double>(), c->pRf.as<const float>(), c->pTf.as<const float>(), c->P, \
         c->R, c->T, c->Kp, c->Rpad, c->Tpad, c->nRb, c->nTc, nPg, c->partMax.as<float>(),      \
-        c->partWC.as<float>(), c->partArg.as<int>(), c->partWR.as<float>(), c->partWT.as<float>()
+        c->partWC.as<float>(), c->partArg.as<int>(), c->partWR.as<float>(), c->partWT.as<float>(), \
+        PREC == THX_FP8 ? c->shellScale.as<const float>() + c->maxShell + 1 : nullptr
         if (c->TC == 128)
             hipLaunchKernelGGL((k_score16<PREC, 128>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
         else if (c->TC == 64)
@@ -1025,6 +1108,8 @@ int launch_score(thx_ctx* c, int precision) {
         launch_score_prec<THX_BF16>(c, c->rotLo.p, nPg, (int)nblk);
     else if (precision == THX_FP16)
         launch_score_prec<THX_FP16>(c, c->rotLo.p, nPg, (int)nblk);
+    else if (precision == THX_FP8)
+        launch_score_prec<THX_FP8>(c, c->rotLo.p, nPg, (int)nblk);
     else {
         set_error("precision not supported by the global scorer");
         return THX_ERR_ARG;
