@@ -73,3 +73,17 @@ def test_training_reduces_loss(net):
         l = net.train_step(x, q, t, 1e-3)
     print(f"loss {l0:.3f} -> {l:.3f}")
     assert l < l0 - 1.0
+
+
+def test_training_is_deterministic():
+    """Same seed, same batches: bit-identical parameters (no atomics in the
+    gradient reductions)."""
+    x, q, t = _batch(64, 2)
+    params = []
+    for _ in range(2):
+        n = thx.Proposal(components=C, seed=5)
+        for _ in range(3):
+            n.train_step(x, q, t, 1e-3)
+        params.append(n.get_params())
+        n.close()
+    np.testing.assert_array_equal(params[0], params[1])

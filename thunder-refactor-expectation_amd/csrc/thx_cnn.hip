@@ -313,15 +313,29 @@ struct PlainStore {
     int O;
     __device__ void operator()(int m, int n, float v) const { y[(size_t)m * O + n] = v; }
 };
-// weight and bias gradient partial sums: column KC of the GEMM is the bias
+// weight and bias gradient partial sums, one slab per k split (blockIdx.z):
+// part[z][m][n] with column KC of the GEMM the bias; reduced in split order
+// afterwards, so training is deterministic
 struct WgradStore {
-    float* dw;
-    float* db;
-    int KC;
+    float* part;
+    int ld, slab;
     __device__ void operator()(int m, int n, float v) const {
-        atomicAdd(n == KC ? db + m : dw + (size_t)m * KC + n, v);
+        part[(size_t)blockIdx.z * slab + (size_t)m * ld + n] = v;
     }
 };
+
+// dw[m][n] (n < KC) and db[m] (n == KC) = sum over splits z of part[z][m][n]
+__global__ void k_split_reduce(const float* __restrict__ part, int nz, int CO, int KC,
+                               float* __restrict__ dw, float* __restrict__ db) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ld = KC + 1;
+    if (i >= CO * ld) return;
+    float s = 0.f;
+    for (int z = 0; z < nz; z++) s += part[(size_t)z * CO * ld + i];
+    const int m = i / ld, n = i % ld;
+    if (n == KC) db[m] = s;
+    else dw[(size_t)m * KC + n] = s;
+}
 template <int WM, int WN, bool BKF, class LA, class LB, class ST>
 static int gemm(hipStream_t st, LA la, LB lb, ST sto, int M, int N, int Kd, int splits = 1) {
     constexpr int BM = 32 * WM, BN = 32 * WN;
@@ -484,6 +498,7 @@ struct thx_cnn {
     DevBuf act[7];           // input, conv1..4 outputs, fc1 output, head output
     DevBuf gact[7];
     DevBuf loss, qt, tt, mix, imgs;
+    DevBuf wpart;            // split-K partial weight gradients
     long step = 0;
 };
 
@@ -576,14 +591,20 @@ static int conv_bwd(thx_cnn* n, int B, const float* in) {
     hipLaunchKernelGGL(k_relu_back, dim3(nblk(no)), dim3(256), 0, st, g,
                        n->act[L + 1].as<const float>(), no);
     // dW[co][k] = sum over (b, pos) of g[co] im2col[k] and db[co] = sum of g[co]
-    // (the extra ones column), split over the long reduction; the bias
-    // follows the weights in the parameter vector, so one clear covers both
-    THX_HIP(hipMemsetAsync(G_(n, 2 * L), 0, sizeof(float) * G::CO * (KC + 1), st));
+    // (the extra ones column), split over the long reduction into per-split
+    // slabs, then summed in split order
     const int red = B * HW;
     const int splits = std::max(1, std::min(256, red / 1024));
+    const int kPer = ((red + splits - 1) / splits + 15) / 16 * 16;
+    const int nz = (red + kPer - 1) / kPer;
+    const int slab = G::CO * (KC + 1);
+    THX_TRY(n->wpart.ensure(sizeof(float) * (size_t)nz * slab));
     THX_TRY((gemm<1, 4, false>(st, GradRows<L>{g}, Im2colT<L>{Im2col<L>{in}},
-                               WgradStore{G_(n, 2 * L), G_(n, 2 * L + 1), KC}, G::CO, KC + 1, red,
-                               splits)));
+                               WgradStore{n->wpart.as<float>(), KC + 1, slab}, G::CO, KC + 1,
+                               red, splits)));
+    hipLaunchKernelGGL(k_split_reduce, dim3(nblk(slab)), dim3(256), 0, st,
+                       n->wpart.as<const float>(), nz, G::CO, KC, G_(n, 2 * L),
+                       G_(n, 2 * L + 1));
     if (L > 0)
         THX_TRY((gemm<1, 4, false>(st, WeightT<L>{P_(n, 2 * L)}, GradCols<L>{g},
                                    GradInStore<L>{n->gact[L].as<float>()}, G::CI,
@@ -667,7 +688,7 @@ int thx_cnn_destroy(thx_cnn* n) {
     (void)hipSetDevice(n->device);
     (void)hipStreamSynchronize(n->stream);
     DevBuf* b[] = {&n->params, &n->grads, &n->adam_m, &n->adam_v, &n->loss, &n->qt, &n->tt,
-                   &n->mix, &n->imgs};
+                   &n->mix, &n->imgs, &n->wpart};
     for (DevBuf* x : b) x->release();
     for (int i = 0; i < 7; i++) {
         n->act[i].release();
