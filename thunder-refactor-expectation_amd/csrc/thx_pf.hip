@@ -294,13 +294,11 @@ __global__ void k_rot_mean(const double* __restrict__ quat, const double* __rest
 __device__ inline float block_count_ge(const float* v, int n, float thr, int* red) {
     int c = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) c += v[i] >= thr;
-    red[threadIdx.x] = c;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    const int r = red[0];
+    int r = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += red[w];
     __syncthreads();
     return (float)r;
 }
@@ -309,9 +307,12 @@ __device__ inline float block_count_ge(const float* v, int n, float thr, int* re
 __device__ void block_topk(const float* v, int n, int nSel, int* out, float* outv, float lo,
                            float hi, int* red, int* cnt) {
     // bisection for thr: count(v >= thr) >= nSel with thr as large as possible
+    // (the count is block-uniform, so the early exit is too)
     for (int it = 0; it < 40; it++) {
         const float mid = 0.5f * (lo + hi);
-        if (block_count_ge(v, n, mid, red) >= nSel) lo = mid; else hi = mid;
+        const float cnt0 = block_count_ge(v, n, mid, red);
+        if (cnt0 >= nSel) lo = mid; else hi = mid;
+        if (cnt0 == nSel) break;
     }
     const float thr = lo;
     if (threadIdx.x == 0) *cnt = 0;
