@@ -108,3 +108,76 @@ def test_local_matches_global_on_shared_grid(gpu_ctx, local_case):
     np.testing.assert_array_equal(bR, gbR)
     np.testing.assert_array_equal(bT, gbT)
     np.testing.assert_allclose(wC, gC[:, 0], rtol=1e-3)
+
+
+def test_local_box160_three_classes(gpu_ctx):
+    """The ragged three-class batch of the box-160 local search (cutoff 64,
+    K = 6,323), at reduced counts: per-particle rotations drawn about the
+    true pose (spread of a von Mises-Fisher kappa = 50), 25-81 translations,
+    each particle scored against its own class volume in one launch.  FP32
+    against the FP64 oracle run per class; BF16 at its bar."""
+    name = "box160"
+    w = wl.WORKLOADS[name]
+    sd = wl.seeds(name)
+    iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
+    assert len(iCol) == 6323
+    gpu_ctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    Vb = w["box"] * w["pf"]
+    per, nCls = 3, 3
+    parts, truth, vols = [], [], []
+    for c in range(nCls):
+        gpu_ctx.synth_volume(c, w["box"], w["pf"], w["n_blob"], w["blob_sigma"],
+                             w["blob_radius"], sd["volume"] + c)
+        q, t, _ = gpu_ctx.synth_particles(c, per, w["max_shift"], w["df_min"], w["df_max"],
+                                          w["snr"], sd["particles"] + c, **wl.CTF_DEFAULTS)
+        parts.append(gpu_ctx.get_particles())
+        truth.append((q, t))
+        vols.append(gpu_ctx.get_volume(c, Vb))
+    dat = np.concatenate([x[0] for x in parts])
+    ctf = np.concatenate([x[1] for x in parts])
+    sig = np.concatenate([x[2] for x in parts])
+    gpu_ctx.set_particles(dat, ctf, sig)
+    P = per * nCls
+    cls = np.repeat(np.arange(nCls), per).astype(np.int32)
+    tq = np.concatenate([x[0] for x in truth])
+    ts = np.concatenate([x[1] for x in truth])
+    rng = np.random.default_rng(160)
+    nR = rng.integers(20, 250, P).astype(np.int32)
+    nT = rng.integers(25, 82, P).astype(np.int32)
+    quats, trans = [], []
+    for p in range(P):
+        q = tq[p] + rng.normal(0, 1 / np.sqrt(2 * 50), size=(nR[p], 4))
+        quats.append(q / np.linalg.norm(q, axis=1, keepdims=True))
+        trans.append(ts[p] + rng.uniform(-3, 3, size=(nT[p], 2)))
+    quat, tr = np.concatenate(quats), np.concatenate(trans)
+    pR = rng.uniform(0.5, 1.5, int(nR.sum()))
+    pT = rng.uniform(0.5, 1.5, int(nT.sum()))
+    print(f"\n[case] box160 P={P} Rtot={nR.sum()} Ttot={nT.sum()}", flush=True)
+    ro = np.r_[0, np.cumsum(nR)]
+    to = np.r_[0, np.cumsum(nT)]
+    ref = [None] * P
+    for c in range(nCls):
+        ps = np.where(cls == c)[0]
+        sel_r = np.concatenate([np.arange(ro[p], ro[p + 1]) for p in ps])
+        sel_t = np.concatenate([np.arange(to[p], to[p + 1]) for p in ps])
+        out = oracle.local_scan(vols[c], Vb, w["pf"], iCol, iRow, w["box"], quat[sel_r],
+                                nR[ps], tr[sel_t], nT[ps], pR[sel_r], pT[sel_t], dat[ps],
+                                ctf[ps], sig[ps])
+        r0 = np.r_[0, np.cumsum(nR[ps])]
+        t0 = np.r_[0, np.cumsum(nT[ps])]
+        for i, p in enumerate(ps):
+            ref[p] = (out[0][i], out[1][r0[i]:r0[i + 1]], out[2][t0[i]:t0[i + 1]], out[3][i],
+                      out[4][i], out[5][i])
+    for prec in (thx.FP32, thx.BF16):
+        gpu_ctx.score_local(quat, nR, tr, nT, pR, pT, cls=cls, precision=prec)
+        wC, wR, wT, base, bR, bT = gpu_ctx.get_local()
+        rel = np.array([abs(base[p] - ref[p][3]) / abs(ref[p][3]) for p in range(P)])
+        same = np.mean([(bR[p], bT[p]) == (ref[p][4], ref[p][5]) for p in range(P)])
+        print(f"precision {prec}: max rel base err {rel.max():.2e}, same best pose {same:.2f}")
+        if prec == thx.FP32:
+            assert rel.max() <= 1e-4 and same == 1.0
+            for p in range(P):
+                assert np.abs(wR[ro[p]:ro[p + 1]] - ref[p][1]).max() <= 3e-3 * ref[p][1].max()
+                assert np.abs(wT[to[p]:to[p + 1]] - ref[p][2]).max() <= 3e-3 * ref[p][2].max()
+        else:
+            assert rel.max() <= 2e-3

@@ -648,7 +648,9 @@ int thx_ctx_get_local(thx_ctx* c, double* wC, double* wR, double* wT, double* ba
 static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, int nSelR,
                           int nSelT, const int* pmap) {
     const int TPAD = nSelT <= 32 ? 32 : (nSelT <= 64 ? 64 : 128);
-    const int nbp = (nSelR + kLocalRotBlock - 1) / kLocalRotBlock;
+    // 256 rotations per block share one staged Z tile when there are enough
+    const int RB = nSelR > kLocalRotBlock ? 2 * kLocalRotBlock : kLocalRotBlock;
+    const int nbp = (nSelR + RB - 1) / RB;
     THX_CHECK(nbp <= 64, THX_ERR_RANGE, "learned: too many selected rotations");
     const size_t nblk = (size_t)nP * nbp;
     THX_CHECK(nblk < (1u << 31), THX_ERR_RANGE, "learned: grid too large");
@@ -687,11 +689,11 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
     b.bArg = c->locBArg.as<int>();
     b.bWT = c->locBWT.as<float>();
     b.wRrel = c->locWRrel.as<float>();
-    THX_TRY(launch_score_sel(c, TPAD, (int)nblk, ridx, tidx, nSelR, nSelT, 1.f / c->R, 1.f / c->T,
-                             pmap, b));
+    THX_TRY(launch_score_sel(c, TPAD, RB, (int)nblk, ridx, tidx, nSelR, nSelT, 1.f / c->R,
+                             1.f / c->T, pmap, b));
     THX_TRY(launch_merge_local(c, nP, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
                                c->locWT.as<double>(), c->locBase.as<double>(),
-                               c->locBestR.as<int>(), c->locBestT.as<int>()));
+                               c->locBestR.as<int>(), c->locBestT.as<int>(), RB));
     return THX_OK;
 }
 

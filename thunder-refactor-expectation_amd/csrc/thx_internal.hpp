@@ -188,7 +188,8 @@ int launch_score_one(thx_ctx* c, int cls, int p, float* dOut);
 constexpr int kLocalRotBlock = 128;   // rotations per local-search block
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
-                       double* wT, double* base, int* bestR, int* bestT);
+                       double* wT, double* base, int* bestR, int* bestT,
+                       int rb = kLocalRotBlock);
 
 // Particle-filter launchers (thx_pf.hip).
 int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
@@ -209,8 +210,9 @@ constexpr int kResampleMaxN = 8192;
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
                    const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
                    int* tidx, float* spread);
-int launch_score_sel(thx_ctx* c, int TPAD, int nblk, const int* ridx, const int* tidx, int nSelR,
-                     int nSelT, float pRu, float pTu, const int* pmap, const LocalBatch& b);
+int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, const int* tidx,
+                     int nSelR, int nSelT, float pRu, float pTu, const int* pmap,
+                     const LocalBatch& b);
 int launch_sel_stats(hipStream_t st, const double* wR, const float* rlogq, const int* ridx,
                      const int* tidx, int* bestR, int* bestT, int nSelR, int nSelT, int P,
                      double* ess, float* tail);

@@ -1524,9 +1524,9 @@ __global__ __launch_bounds__(256) void k_merge_local(
     const float* __restrict__ bMax, const float* __restrict__ bWC, const int* __restrict__ bArg,
     const float* __restrict__ bWT, const float* __restrict__ wRrel, const int* __restrict__ bStart,
     const int* __restrict__ rOff, const int* __restrict__ nR, const int* __restrict__ tOff,
-    const int* __restrict__ nT, int TPAD, double* __restrict__ wC, double* __restrict__ wR,
-    double* __restrict__ wT, double* __restrict__ base, int* __restrict__ bestR,
-    int* __restrict__ bestT) {
+    const int* __restrict__ nT, int TPAD, int rb, double* __restrict__ wC,
+    double* __restrict__ wR, double* __restrict__ wT, double* __restrict__ base,
+    int* __restrict__ bestR, int* __restrict__ bestT) {
     const int p = blockIdx.x;
     const int b0 = bStart[p], b1 = bStart[p + 1], nb = b1 - b0;
     __shared__ double f[64];
@@ -1567,7 +1567,7 @@ __global__ __launch_bounds__(256) void k_merge_local(
     }
     const int Rp = nR[p], Tp = nT[p];
     for (int r = threadIdx.x; r < Rp; r += blockDim.x)
-        wR[rOff[p] + r] = f[r / kNBL] * (double)wRrel[rOff[p] + r];
+        wR[rOff[p] + r] = f[r / rb] * (double)wRrel[rOff[p] + r];
     for (int t = threadIdx.x; t < Tp; t += blockDim.x) {
         double s = 0.0;
         for (int i = 0; i < nb; i++) s += f[i] * bWT[(size_t)(b0 + i) * TPAD + t];
@@ -1640,9 +1640,9 @@ int launch_gather_particles(thx_ctx* c, const int* idx, int n, float2* dat2, flo
 }
 
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
-                       double* wT, double* base, int* bestR, int* bestT) {
+                       double* wT, double* base, int* bestR, int* bestT, int rb) {
     hipLaunchKernelGGL(k_merge_local, dim3(P), dim3(256), 0, c->stream, b.bMax, b.bWC, b.bArg,
-                       b.bWT, b.wRrel, b.bStart, b.rOff, b.nR, b.tOff, b.nT, TPAD, wC, wR, wT,
+                       b.bWT, b.wRrel, b.bStart, b.rOff, b.nR, b.tOff, b.nT, TPAD, rb, wC, wR, wT,
                        base, bestR, bestT);
     THX_HIP(hipGetLastError());
     return THX_OK;
@@ -1656,8 +1656,8 @@ int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double*
 // block owns one particle x up to 128 of its selected rotations x all its
 // selected translations; epilogue and merge are those of the local scorer.
 // ---------------------------------------------------------------------------
-template <int TPAD>
-__global__ __launch_bounds__(256, 2) void k_score_sel(
+template <int TPAD, int RB>
+__global__ __launch_bounds__(2 * RB) void k_score_sel(
     const uint32_t* __restrict__ rotLo, const float* __restrict__ normP,
     const float2* __restrict__ S, int Kp, const float2* __restrict__ Y,
     const float* __restrict__ Wt, const double* __restrict__ Aterm, const int* __restrict__ ridx,
@@ -1666,15 +1666,15 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
     int* __restrict__ bArg, float* __restrict__ bWT, float* __restrict__ wRrel) {
     constexpr int LD = 72;
     constexpr int MT = TPAD / 32;
-    constexpr int RB = kNBL;
+    constexpr int NTH = 2 * RB, NW = NTH / 64;   // 128 rotations: 4 waves; 256: 8
     __shared__ __attribute__((aligned(16))) unsigned short As[TPAD * LD];
     __shared__ __attribute__((aligned(16))) unsigned short Bs[RB * LD];
     __shared__ int rrow[RB];
     __shared__ int trow[TPAD];
     __shared__ float Cs[RB];
-    __shared__ float redMax[4];
-    __shared__ int redArg[4];
-    __shared__ float rowPart[4][TPAD];
+    __shared__ float redMax[NW];
+    __shared__ int redArg[NW];
+    __shared__ float rowPart[NW][TPAD];
     __shared__ float rowW[TPAD];
 
     const int nbp = (nSelR + RB - 1) / RB;
@@ -1691,22 +1691,22 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
 
     // Loads are 16 bytes per lane: slice rows as 4 packed pixels (lanes
     // l8 = tid & 7 of eight consecutive lanes cover a row's 32-pixel chunk,
-    // rows rq + 32 q), phase rows as 2 complex pixels (l16 = tid & 15, rows
-    // tq + 16 j).  The norm term is taken from the 16-bit slice values, so
+    // rows rq + NTH/8 q), phase rows as 2 complex pixels (l16 = tid & 15,
+    // rows tq + NTH/16 j).  The norm term is taken from the 16-bit slice values, so
     // no |P|^2 table is read.
-    constexpr int NJ = TPAD / 16;
+    constexpr int NJ = TPAD / (NTH / 16);
     const int l8 = tid & 7, rq = tid >> 3;
     const int l16 = tid & 15, tq = tid >> 4;
     const uint32_t* brow[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const int row = rrow[rq + 32 * q];
+        const int row = rrow[rq + (NTH / 8) * q];
         brow[q] = row >= 0 ? rotLo + (size_t)row * Kp : nullptr;
     }
     const float2* srow[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int t = trow[tq + 16 * j];
+        const int t = trow[tq + (NTH / 16) * j];
         srow[j] = t >= 0 ? S + (size_t)t * Kp : nullptr;
     }
     const float2* yrow = Y + (size_t)pd * Kp;
@@ -1745,7 +1745,7 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
                 c += w4[e] * (pr * pr + pi * pi);
             }
             cpart[q] += c;
-            *reinterpret_cast<u32x4*>(Bs + (rq + 32 * q) * LD + 8 * l8) = bv[q];
+            *reinterpret_cast<u32x4*>(Bs + (rq + (NTH / 8) * q) * LD + 8 * l8) = bv[q];
         }
         // ---- A: Z = Y conj(S_t) ----
 #pragma unroll
@@ -1754,7 +1754,7 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
             uint2 z;
             z.x = pack16<THX_BF16>(yv.x * sj.x + yv.y * sj.y, yv.y * sj.x - yv.x * sj.y);
             z.y = pack16<THX_BF16>(yv.z * sj.z + yv.w * sj.w, yv.w * sj.z - yv.z * sj.w);
-            *reinterpret_cast<uint2*>(As + (tq + 16 * j) * LD + 4 * l16) = z;
+            *reinterpret_cast<uint2*>(As + (tq + (NTH / 16) * j) * LD + 4 * l16) = z;
         }
         __syncthreads();
         if (k0 + kChunk < Kp) gload(k0 + kChunk);
@@ -1780,7 +1780,7 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
         v += __shfl_xor(v, 1);
         v += __shfl_xor(v, 2);
         v += __shfl_xor(v, 4);
-        if (l8 == 0) Cs[rq + 32 * q] = v;
+        if (l8 == 0) Cs[rq + (NTH / 8) * q] = v;
     }
     __syncthreads();
 
@@ -1809,7 +1809,7 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
     __syncthreads();
     float M = redMax[0];
     int arg = redArg[0];
-    for (int w = 1; w < 4; w++)
+    for (int w = 1; w < NW; w++)
         if (redMax[w] > M || (redMax[w] == M && redArg[w] >= 0 && (arg < 0 || redArg[w] < arg))) {
             M = redMax[w];
             arg = redArg[w];
@@ -1857,7 +1857,8 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
     }
     __syncthreads();
     if (tid < TPAD) {
-        const float rsum = rowPart[0][tid] + rowPart[1][tid] + rowPart[2][tid] + rowPart[3][tid];
+        float rsum = 0.f;
+        for (int w = 0; w < NW; w++) rsum += rowPart[w][tid];
         bWT[(size_t)blockIdx.x * TPAD + tid] = rsum;
         rowW[tid] = tid < Tp ? rsum * pTu : 0.f;
     }
@@ -1871,18 +1872,29 @@ __global__ __launch_bounds__(256, 2) void k_score_sel(
     }
 }
 
-int launch_score_sel(thx_ctx* c, int TPAD, int nblk, const int* ridx, const int* tidx, int nSelR,
-                     int nSelT, float pRu, float pTu, const int* pmap, const LocalBatch& b) {
+int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, const int* tidx,
+                     int nSelR, int nSelT, float pRu, float pTu, const int* pmap,
+                     const LocalBatch& b) {
 #define THX_SEL_ARGS c->rotLo.as<const uint32_t>(), c->normP.as<const float>(),            \
                      c->S.as<const float2>(), c->Kp, c->Y.as<const float2>(),              \
                      c->Wt.as<const float>(), c->Aterm.as<const double>(), ridx, tidx,     \
                      nSelR, nSelT, pRu, pTu, pmap, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel
-    if (TPAD == 32)
-        hipLaunchKernelGGL((k_score_sel<32>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
-    else if (TPAD == 64)
-        hipLaunchKernelGGL((k_score_sel<64>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
-    else
-        hipLaunchKernelGGL((k_score_sel<128>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
+    THX_CHECK(RB == 128 || RB == 256, THX_ERR_ARG, "score_sel: rotation block must be 128 or 256");
+    if (RB == 128) {
+        if (TPAD == 32)
+            hipLaunchKernelGGL((k_score_sel<32, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
+        else if (TPAD == 64)
+            hipLaunchKernelGGL((k_score_sel<64, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
+        else
+            hipLaunchKernelGGL((k_score_sel<128, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
+    } else {
+        if (TPAD == 32)
+            hipLaunchKernelGGL((k_score_sel<32, 256>), dim3(nblk), dim3(512), 0, c->stream, THX_SEL_ARGS);
+        else if (TPAD == 64)
+            hipLaunchKernelGGL((k_score_sel<64, 256>), dim3(nblk), dim3(512), 0, c->stream, THX_SEL_ARGS);
+        else
+            hipLaunchKernelGGL((k_score_sel<128, 256>), dim3(nblk), dim3(512), 0, c->stream, THX_SEL_ARGS);
+    }
 #undef THX_SEL_ARGS
     THX_HIP(hipGetLastError());
     return THX_OK;

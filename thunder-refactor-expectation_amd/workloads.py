@@ -67,6 +67,12 @@ WORKLOADS = {
                    n_particles=10000, n_rot=5000, shift_side=11, shift_step=1.0,
                    max_shift=5.0, df_min=8000.0, df_max=35000.0, snr=0.03, cutoff=50,
                    n_class=1),
+    # Ragged local search over three classes: box 160 (blob width and radius
+    # of box 128 scaled by 160 / 128).
+    "box160": dict(box=160, pf=2, n_blob=200, blob_sigma=3.75, blob_radius=60.0,
+                   n_particles=50000, n_rot=1000, shift_side=9, shift_step=1.0,
+                   max_shift=5.0, df_min=8000.0, df_max=35000.0, snr=0.03, cutoff=64,
+                   n_class=3),
     # FP8 scoring at large box: box 256.
     "box256": dict(box=256, pf=2, n_blob=500, blob_sigma=6.0, blob_radius=96.0,
                    n_particles=20000, n_rot=10000, shift_side=9, shift_step=2.0,
