@@ -184,6 +184,17 @@ int thx_ctx_score_local(thx_ctx* ctx, const int* cls, const double* quat,
 int thx_ctx_get_local(thx_ctx* ctx, double* wC, double* wR, double* wT,
                       double* baseL, int* bestR, int* bestT);
 
+/* Launch shape of the local scorer (no reference counterpart): mode 0 =
+ * autotuned (both shapes are timed on the first batch of each size class and
+ * the faster is kept), 1 = one block per work item (particle x 128
+ * rotations), 2 = a persistent grid sized to the resident capacity pulling
+ * work items from a counter.  graph != 0 captures the local phase (operand
+ * prep, scorer, merge) in a hipGraph, replayed while the batch shape and the
+ * context's buffers stay the same. */
+int thx_ctx_set_local_options(thx_ctx* ctx, int mode, int graph);
+/* Launch shape used by the last local phase (1 or 2). */
+int thx_ctx_local_mode(thx_ctx* ctx, int* mode);
+
 /* Waits for all work queued on the context stream. */
 int thx_ctx_sync(thx_ctx* ctx);
 

@@ -181,3 +181,48 @@ def test_local_box160_three_classes(gpu_ctx):
                 assert np.abs(wT[to[p]:to[p + 1]] - ref[p][2]).max() <= 3e-3 * ref[p][2].max()
         else:
             assert rel.max() <= 2e-3
+
+
+def test_local_launch_shapes_and_graph_replays(gpu_ctx, local_case):
+    """Direct and persistent launches give bit-identical results (the same
+    per-item code); the autotuned shape is one of them; a captured graph,
+    replayed on new batch contents of the same shape with eager work in
+    between, reproduces the eager results every time."""
+    c = local_case
+    wl.setup_context(gpu_ctx, "box64", n_particles=24, n_rot=64)   # same seeds as local_case
+    rng = np.random.default_rng(31)
+    first = _batch(c, 24, rng, 300, 49)
+    nR0, nT0 = first[1], first[3]
+    same = []
+    for _ in range(3):
+        q = rng.normal(size=(int(nR0.sum()), 4))
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
+        t = rng.uniform(-3, 3, size=(int(nT0.sum()), 2))
+        same.append((q, nR0, t, nT0, rng.uniform(0.5, 1.5, int(nR0.sum())),
+                     rng.uniform(0.5, 1.5, int(nT0.sum()))))
+    batches = [first]
+    outs = {}
+    for mode in (1, 2, 0):
+        gpu_ctx.set_local_options(mode, False)
+        gpu_ctx.score_local(*batches[0], precision=thx.FP32)
+        outs[mode] = gpu_ctx.get_local()
+        assert gpu_ctx.local_mode() in (1, 2)
+    for a, b in zip(outs[1], outs[2]):
+        np.testing.assert_array_equal(a, b)
+    for a, b in zip(outs[1], outs[0]):
+        np.testing.assert_array_equal(a, b)
+    eager = []
+    gpu_ctx.set_local_options(0, False)
+    for b in same:
+        gpu_ctx.score_local(*b, precision=thx.BF16)
+        eager.append(gpu_ctx.get_local())
+    gpu_ctx.set_local_options(0, True)
+    for rep in range(2):
+        for i, b in enumerate(same):
+            gpu_ctx.score_global(0, thx.FP32)      # eager work between replays
+            gpu_ctx.sync()
+            gpu_ctx.score_local(*b, precision=thx.BF16)
+            got = gpu_ctx.get_local()
+            for a, e in zip(got, eager[i]):
+                np.testing.assert_array_equal(a, e)
+    gpu_ctx.set_local_options(0, False)

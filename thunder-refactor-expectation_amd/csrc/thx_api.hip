@@ -157,6 +157,40 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
 }
 
 
+// Launch shape of the local scorer for this batch: the requested one, or the
+// faster of the two timed once per (precision, padding, item-count bucket).
+int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b) {
+    if (c->locModeReq == kLocalDirect || c->locModeReq == kLocalQueue) return c->locModeReq;
+    int lg = 0;
+    while ((1 << lg) < nblk) lg++;
+    const long long key = (long long)precision * 1000000 + TPAD * 1000 + lg;
+    for (auto& kv : c->locTuned)
+        if (kv.first == key) return kv.second;
+    hipEvent_t e[3] = {};
+    for (int i = 0; i < 3; i++) THX_HIP(hipEventCreate(&e[i]));
+    int s = THX_OK;
+    float tD = 0.f, tQ = 0.f;
+    if ((s = (hipEventRecord(e[0], c->stream) == hipSuccess ? THX_OK : THX_ERR_HIP)) == THX_OK &&
+        (s = launch_score_local(c, precision, TPAD, nblk, b, kLocalDirect)) == THX_OK &&
+        (s = (hipEventRecord(e[1], c->stream) == hipSuccess ? THX_OK : THX_ERR_HIP)) == THX_OK &&
+        (s = launch_score_local(c, precision, TPAD, nblk, b, kLocalQueue)) == THX_OK &&
+        (s = (hipEventRecord(e[2], c->stream) == hipSuccess ? THX_OK : THX_ERR_HIP)) == THX_OK &&
+        hipEventSynchronize(e[2]) == hipSuccess) {
+        (void)hipEventElapsedTime(&tD, e[0], e[1]);
+        (void)hipEventElapsedTime(&tQ, e[1], e[2]);
+    } else if (s == THX_OK) {
+        s = THX_ERR_HIP;
+    }
+    for (int i = 0; i < 3; i++) (void)hipEventDestroy(e[i]);
+    if (s != THX_OK) {
+        set_error("local scorer autotuning failed");
+        return -1;
+    }
+    const int mode = tQ < tD ? kLocalQueue : kLocalDirect;
+    c->locTuned.push_back({key, mode});
+    return mode;
+}
+
 static std::mutex g_default_mu;
 static thx_ctx* g_default = nullptr;
 
@@ -215,6 +249,7 @@ int thx_ctx_destroy(thx_ctx* c) {
     if (!c) return THX_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (c->locGraph) (void)hipGraphExecDestroy(c->locGraph);
     DevBuf* bufs[] = {&c->iCol, &c->iRow, &c->iSig, &c->dat, &c->ctf, &c->sig, &c->Aterm,
                       &c->Y, &c->Wt, &c->quat, &c->trans, &c->pR, &c->pT, &c->pRf, &c->pTf,
                       &c->S, &c->rotP, &c->rotLo, &c->normP, &c->Cterm, &c->shellScale,
@@ -233,7 +268,8 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->pfTr2, &c->pfWR, &c->pfWT, &c->pfWRf, &c->pfWTf, &c->pfIdx,
                        &c->pfWNew, &c->pfK, &c->pfS, &c->pfMean, &c->pfTopR, &c->pfTopT,
                        &c->pfPeak, &c->pfTransS, &c->pfBest, &c->pfActive, &c->pfNPhase,
-                       &c->pfOffR, &c->pfOffT, &c->pfNR, &c->pfNT, &c->pfNRa, &c->pfNTa};
+                       &c->pfOffR, &c->pfOffT, &c->pfNR, &c->pfNT, &c->pfNRa, &c->pfNTa,
+                       &c->locCounter};
     for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();
     for (auto& r : c->res) {
@@ -529,9 +565,19 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     THX_CHECK(!c->vol.empty(), THX_ERR_STATE, "score_local: no volume set");
     const int P = c->P;
     const int nCls = (int)c->vol.size();
-    for (int k = 0; k < nCls; k++)
-        THX_CHECK(c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
-                  "score_local: every class volume must be set with vdim = idim * pf");
+    // every class the particles use must hold a volume of this pixel set's size
+    {
+        std::vector<char> used(nCls, 0);
+        if (cls) {
+            for (int q = 0; q < c->P; q++)
+                if (cls[q] >= 0 && cls[q] < nCls) used[cls[q]] = 1;
+        } else {
+            used[0] = 1;
+        }
+        for (int k = 0; k < nCls; k++)
+            THX_CHECK(!used[k] || c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
+                      "score_local: a class volume in use does not match idim * pf");
+    }
     std::vector<int> rOff(P), tOff(P), bStart(P + 1);
     std::vector<int2> blocks;
     size_t Rtot = 0, Ttot = 0;
@@ -609,16 +655,84 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     b.wRrel = c->locWRrel.as<float>();
     c->nev = 0;
     c->evNames.clear();
+    THX_TRY(launch_prep_particles(c, THX_FP32));
+    const int mode = choose_local_mode(c, precision, TPAD, nblk, b);
+    if (mode < 0) return THX_ERR_HIP;
+    c->locModeUsed = mode;
+    if (c->locGraphOn) {
+        THX_TRY(c->locCounter.ensure(sizeof(int)));   // no allocation inside the capture
+        // one captured graph per batch shape and buffer set: operand prep,
+        // scorer, merge; replayed while they stay the same
+        auto ptrs = [](std::initializer_list<const void*> l) {
+            std::vector<long long> v;
+            for (const void* x : l) v.push_back((long long)(uintptr_t)x);
+            return v;
+        };
+        std::vector<long long> key = ptrs(
+            {c->Y.p, c->Wt.p, c->Aterm.p, c->dat.p, c->ctf.p, c->sig.p, c->iCol.p, c->iRow.p,
+             c->iSig.p, c->locQuat.p, c->locTrans.p, c->locPRf.p, c->locPTf.p, c->locROff.p,
+             c->locNR.p, c->locTOff.p, c->locNT.p, c->locBlocks.p, c->locBStart.p, c->locVols.p,
+             cls ? c->locPcls.p : nullptr, c->locBMax.p, c->locBWC.p, c->locBArg.p, c->locBWT.p,
+             c->locWRrel.p, c->locWC.p, c->locWR.p, c->locWT.p, c->locBase.p, c->locBestR.p,
+             c->locBestT.p, c->locCounter.p});
+        const long long shape[] = {P, nblk, TPAD, precision, mode, c->K, c->Kp, c->idim};
+        key.insert(key.end(), shape, shape + 8);
+        if (!c->locGraph || key != c->locGraphKey) {
+            if (c->locGraph) {
+                (void)hipGraphExecDestroy(c->locGraph);
+                c->locGraph = nullptr;
+            }
+            THX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int cs = launch_prep_particles(c, THX_FP32);
+            if (cs == THX_OK) cs = launch_score_local(c, precision, TPAD, nblk, b, mode);
+            if (cs == THX_OK)
+                cs = launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(),
+                                        c->locWR.as<double>(), c->locWT.as<double>(),
+                                        c->locBase.as<double>(), c->locBestR.as<int>(),
+                                        c->locBestT.as<int>());
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(st, &g);
+            if (cs != THX_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return cs;
+            }
+            THX_CHECK(ee == hipSuccess && g, THX_ERR_HIP, "score_local: graph capture failed");
+            const hipError_t ie = hipGraphInstantiate(&c->locGraph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            THX_CHECK(ie == hipSuccess, THX_ERR_HIP, "score_local: graph instantiation failed");
+            c->locGraphKey = key;
+        }
+        THX_TRY(record(c, "start"));
+        THX_HIP(hipGraphLaunch(c->locGraph, st));
+        THX_TRY(record(c, "local_graph"));
+        c->locValid = true;
+        return THX_OK;
+    }
     THX_TRY(record(c, "start"));
     THX_TRY(launch_prep_particles(c, THX_FP32));
     THX_TRY(record(c, "prep"));
-    THX_TRY(launch_score_local(c, precision, TPAD, nblk, b));
+    THX_TRY(launch_score_local(c, precision, TPAD, nblk, b, mode));
     THX_TRY(record(c, "score_local"));
     THX_TRY(launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
                                c->locWT.as<double>(), c->locBase.as<double>(),
                                c->locBestR.as<int>(), c->locBestT.as<int>()));
     THX_TRY(record(c, "merge_local"));
     c->locValid = true;
+    return THX_OK;
+}
+
+int thx_ctx_set_local_options(thx_ctx* c, int mode, int graph) {
+    clear_error();
+    THX_CHECK(c && mode >= 0 && mode <= 2, THX_ERR_ARG, "set_local_options: bad argument");
+    c->locModeReq = mode;
+    c->locGraphOn = graph != 0;
+    return THX_OK;
+}
+
+int thx_ctx_local_mode(thx_ctx* c, int* mode) {
+    clear_error();
+    THX_CHECK(c && mode, THX_ERR_ARG, "local_mode: bad argument");
+    *mode = c->locModeUsed;
     return THX_OK;
 }
 

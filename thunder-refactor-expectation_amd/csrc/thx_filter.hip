@@ -655,9 +655,19 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
               "pf_refine: precision must be FP32 or BF16");
     THX_CHECK(!c->vol.empty() && c->K > 0, THX_ERR_STATE, "pf_refine: volume and pixels needed");
     const int nCls = (int)c->vol.size();
-    for (int k = 0; k < nCls; k++)
-        THX_CHECK(c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
-                  "pf_refine: every class volume must be set with vdim = idim * pf");
+    // every class the particles use must hold a volume of this pixel set's size
+    {
+        std::vector<char> used(nCls, 0);
+        if (cls) {
+            for (int q = 0; q < c->P; q++)
+                if (cls[q] >= 0 && cls[q] < nCls) used[cls[q]] = 1;
+        } else {
+            used[0] = 1;
+        }
+        for (int k = 0; k < nCls; k++)
+            THX_CHECK(!used[k] || c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
+                      "pf_refine: a class volume in use does not match idim * pf");
+    }
     THX_HIP(hipSetDevice(c->device));
     const int P = c->P, mLR = c->pfR, mLT = c->pfT;
     if (cls)
@@ -747,7 +757,10 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
         b.bArg = c->locBArg.as<int>();
         b.bWT = c->locBWT.as<float>();
         b.wRrel = c->locWRrel.as<float>();
-        THX_TRY(launch_score_local(c, precision, TPAD, (int)blocks.size(), b));
+        const int mode = choose_local_mode(c, precision, TPAD, (int)blocks.size(), b);
+        if (mode < 0) return THX_ERR_HIP;
+        c->locModeUsed = mode;
+        THX_TRY(launch_score_local(c, precision, TPAD, (int)blocks.size(), b, mode));
         THX_TRY(launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
                                    c->locWT.as<double>(), c->locBase.as<double>(),
                                    c->locBestR.as<int>(), c->locBestT.as<int>()));

@@ -141,6 +141,15 @@ struct thx_ctx {
         locBStart, locPcls, locVols, locBMax, locBWC, locBArg, locBWT, locWRrel;
     thx::DevBuf locWC, locWR, locWT, locBase, locBestR, locBestT;
     bool locValid = false;
+    // launch shape of the local scorer: requested (0 = autotuned), the tuned
+    // choice per (precision, translation padding, work-item count bucket),
+    // the last one used, and the captured graph of a local phase
+    int locModeReq = 0, locModeUsed = 0;
+    bool locGraphOn = false;
+    std::vector<std::pair<long long, int>> locTuned;
+    thx::DevBuf locCounter;
+    hipGraphExec_t locGraph = nullptr;
+    std::vector<long long> locGraphKey;
     // 64 x 64 particle images of the learned proposal
     thx::DevBuf img;
     // grid-restricted learned proposal: mixture, selections, results
@@ -186,7 +195,13 @@ int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams,
 int launch_synth_noise(thx_ctx* c, int P, double noiseSigma, uint64_t seed);
 int launch_score_one(thx_ctx* c, int cls, int p, float* dOut);
 constexpr int kLocalRotBlock = 128;   // rotations per local-search block
-int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);
+// local scorer launch shapes: one block per work item, or a persistent grid
+// pulling items from a counter
+constexpr int kLocalDirect = 1;
+constexpr int kLocalQueue = 2;
+int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
+                       int mode = kLocalDirect);
+int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
                        double* wT, double* base, int* bestR, int* bestT,
                        int rb = kLocalRotBlock);

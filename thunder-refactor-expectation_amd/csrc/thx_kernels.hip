@@ -1229,7 +1229,7 @@ int launch_merge(thx_ctx* c, int cls) {
 constexpr int kNBL = 128;
 
 template <int PREC, int TPAD>
-__global__ __launch_bounds__(256, 2) void k_score_local(
+__device__ __forceinline__ void score_local_item(int item, 
     const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
     const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int Kp, int N,
     const float2* __restrict__ Y, const float* __restrict__ Wt,
@@ -1252,7 +1252,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
     __shared__ int redArg[4];
     __shared__ float rowPart[4][TPAD];
 
-    const int2 bi = blocks[blockIdx.x];
+    const int2 bi = blocks[item];
     const int p = bi.x, r0 = bi.y;
     const int Rp = nR[p], Tp = nT[p];
     const int nr = min(kNBL, Rp - r0);
@@ -1463,58 +1463,135 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
     __shared__ float rowW[TPAD];
     if (tid < TPAD) {
         const float rsum = rowPart[0][tid] + rowPart[1][tid] + rowPart[2][tid] + rowPart[3][tid];
-        bWT[(size_t)blockIdx.x * TPAD + tid] = rsum;
+        bWT[(size_t)item * TPAD + tid] = rsum;
         rowW[tid] = tid < Tp ? rsum * pTf[tbase + tid] : 0.f;
     }
     __syncthreads();
     if (tid == 0) {
         float wc = 0.f;
         for (int t = 0; t < TPAD; t++) wc += rowW[t];
-        bMax[blockIdx.x] = M;
-        bWC[blockIdx.x] = wc;
-        bArg[blockIdx.x] = arg < 0 ? -1 : ((r0 + arg / TPAD) * kNBL + arg % TPAD);
+        bMax[item] = M;
+        bWC[item] = wc;
+        bArg[item] = arg < 0 ? -1 : ((r0 + arg / TPAD) * kNBL + arg % TPAD);
     }
+}
+
+
+// Direct launch: one block per work item (particle, 128 rotations).
+template <int PREC, int TPAD>
+__global__ __launch_bounds__(256, 2) void k_score_local(
+    const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int Kp, int N,
+    const float2* __restrict__ Y, const float* __restrict__ Wt,
+    const double* __restrict__ Aterm, const double* __restrict__ quat,
+    const float* __restrict__ pRf, const int* __restrict__ rOff, const int* __restrict__ nR,
+    const double* __restrict__ trans, const float* __restrict__ pTf,
+    const int* __restrict__ tOff, const int* __restrict__ nT, const int2* __restrict__ blocks,
+    float* __restrict__ bMax, float* __restrict__ bWC, int* __restrict__ bArg,
+    float* __restrict__ bWT, float* __restrict__ wRrel) {
+    score_local_item<PREC, TPAD>(blockIdx.x, vols, pcls, V, pf, iCol, iRow, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+}
+
+// Persistent launch: a grid sized to the resident capacity pulls work items
+// from an atomic counter until the ragged list is exhausted.
+template <int PREC, int TPAD>
+__global__ __launch_bounds__(256, 2) void k_score_local_q(
+    const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int Kp, int N,
+    const float2* __restrict__ Y, const float* __restrict__ Wt,
+    const double* __restrict__ Aterm, const double* __restrict__ quat,
+    const float* __restrict__ pRf, const int* __restrict__ rOff, const int* __restrict__ nR,
+    const double* __restrict__ trans, const float* __restrict__ pTf,
+    const int* __restrict__ tOff, const int* __restrict__ nT, const int2* __restrict__ blocks,
+    float* __restrict__ bMax, float* __restrict__ bWC, int* __restrict__ bArg,
+    float* __restrict__ bWT, float* __restrict__ wRrel, int nItems,
+    int* __restrict__ counter) {
+    __shared__ int item;
+    for (;;) {
+        __syncthreads();   // the previous item's LDS traffic is over
+        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = item;
+        if (it >= nItems) break;
+        score_local_item<PREC, TPAD>(it, vols, pcls, V, pf, iCol, iRow, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+    }
+}
+
+// counter = 0 (a kernel, so a captured graph holds no memset node)
+__global__ void k_zero_int(int* __restrict__ counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
 template <int PREC>
-static void launch_local_prec(hipStream_t st, int TPAD, int nblk, const float2* const* vols,
-                              const int* pcls, int V, int pf, const int* iCol, const int* iRow,
-                              int K, int Kp, int N, const float2* Y, const float* Wt,
-                              const double* Aterm, const double* quat, const float* pRf,
-                              const int* rOff, const int* nR, const double* trans,
-                              const float* pTf, const int* tOff, const int* nT,
-                              const int2* blocks, float* bMax, float* bWC, int* bArg, float* bWT,
-                              float* wRrel) {
+static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int grid, int* counter,
+                              const float2* const* vols, const int* pcls, int V, int pf,
+                              const int* iCol, const int* iRow, int K, int Kp, int N,
+                              const float2* Y, const float* Wt, const double* Aterm,
+                              const double* quat, const float* pRf, const int* rOff,
+                              const int* nR, const double* trans, const float* pTf,
+                              const int* tOff, const int* nT, const int2* blocks, float* bMax,
+                              float* bWC, int* bArg, float* bWT, float* wRrel) {
 #define THX_LOCAL_ARGS vols, pcls, V, pf, iCol, iRow, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, \
                        nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel
-    if (TPAD == 32)
-        hipLaunchKernelGGL((k_score_local<PREC, 32>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
-    else if (TPAD == 64)
-        hipLaunchKernelGGL((k_score_local<PREC, 64>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
-    else
-        hipLaunchKernelGGL((k_score_local<PREC, 128>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+    if (mode == kLocalQueue) {
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, counter);
+        const int g = std::min(grid, nblk);
+        if (TPAD == 32)
+            hipLaunchKernelGGL((k_score_local_q<PREC, 32>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter);
+        else if (TPAD == 64)
+            hipLaunchKernelGGL((k_score_local_q<PREC, 64>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter);
+        else
+            hipLaunchKernelGGL((k_score_local_q<PREC, 128>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter);
+    } else {
+        if (TPAD == 32)
+            hipLaunchKernelGGL((k_score_local<PREC, 32>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+        else if (TPAD == 64)
+            hipLaunchKernelGGL((k_score_local<PREC, 64>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+        else
+            hipLaunchKernelGGL((k_score_local<PREC, 128>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+    }
 #undef THX_LOCAL_ARGS
 }
 
-int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b) {
+// resident work-item capacity of the persistent launch: blocks per CU x CUs
+static int local_queue_grid(int precision, int TPAD) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 1024;
+    const void* fn = nullptr;
     if (precision == THX_FP32)
-        launch_local_prec<THX_FP32>(c->stream, TPAD, nblk, b.vols, b.pcls, c->vdim[0], c->pf,
-                                    c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->Kp,
-                                    c->idim, c->Y.as<const float2>(), c->Wt.as<const float>(),
-                                    c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR,
-                                    b.trans, b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg,
-                                    b.bWT, b.wRrel);
-    else if (precision == THX_BF16)
-        launch_local_prec<THX_BF16>(c->stream, TPAD, nblk, b.vols, b.pcls, c->vdim[0], c->pf,
-                                    c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->Kp,
-                                    c->idim, c->Y.as<const float2>(), c->Wt.as<const float>(),
-                                    c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR,
-                                    b.trans, b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg,
-                                    b.bWT, b.wRrel);
-    else {
-        set_error("local scorer supports FP32 and BF16");
-        return THX_ERR_ARG;
+        fn = TPAD == 32 ? (const void*)k_score_local_q<THX_FP32, 32>
+             : TPAD == 64 ? (const void*)k_score_local_q<THX_FP32, 64>
+                          : (const void*)k_score_local_q<THX_FP32, 128>;
+    else
+        fn = TPAD == 32 ? (const void*)k_score_local_q<THX_BF16, 32>
+             : TPAD == 64 ? (const void*)k_score_local_q<THX_BF16, 64>
+                          : (const void*)k_score_local_q<THX_BF16, 128>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0) != hipSuccess || per < 1)
+        per = 2;
+    return cus * per;
+}
+
+int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
+                       int mode) {
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
+              "local scorer supports FP32 and BF16");
+    int grid = 0;
+    if (mode == kLocalQueue) {
+        THX_TRY(c->locCounter.ensure(sizeof(int)));
+        grid = local_queue_grid(precision, TPAD);
     }
+#define THX_LOCAL_CALL(PR)                                                                      \
+    launch_local_prec<PR>(c->stream, TPAD, nblk, mode, grid, c->locCounter.as<int>(), b.vols,  \
+                          b.pcls, c->idim * c->pf, c->pf, c->iCol.as<const int>(),             \
+                          c->iRow.as<const int>(), c->K, c->Kp, c->idim,                       \
+                          c->Y.as<const float2>(), c->Wt.as<const float>(),                    \
+                          c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR, b.trans,   \
+                          b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel)
+    if (precision == THX_FP32) THX_LOCAL_CALL(THX_FP32);
+    else THX_LOCAL_CALL(THX_BF16);
+#undef THX_LOCAL_CALL
     THX_HIP(hipGetLastError());
     return THX_OK;
 }

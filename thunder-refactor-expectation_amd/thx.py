@@ -43,6 +43,8 @@ _SIGS = {
     "thx_ctx_get_best": [_p, _i, _p, _p, _p],
     "thx_ctx_score_one": [_p, _i, _i, _p],
     "thx_ctx_sync": [_p],
+    "thx_ctx_set_local_options": [_p, _i, _i],
+    "thx_ctx_local_mode": [_p, _p],
     "thx_ctx_score_local": [_p, _p, _p, _p, _p, _p, _p, _p, _i],
     "thx_ctx_get_local": [_p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
@@ -288,6 +290,16 @@ class Context:
         check(load().thx_ctx_get_local(self._h, ptr(wC), ptr(wR), ptr(wT), ptr(base), ptr(bR),
                                        ptr(bT)), "get_local")
         return wC, wR, wT, base, bR, bT
+
+    def set_local_options(self, mode=0, graph=False):
+        """Local scorer launch shape (0 autotuned, 1 direct, 2 persistent) and
+        hipGraph capture of the local phase."""
+        check(load().thx_ctx_set_local_options(self._h, mode, int(graph)), "set_local_options")
+
+    def local_mode(self):
+        m = C.c_int(0)
+        check(load().thx_ctx_local_mode(self._h, C.byref(m)), "local_mode")
+        return m.value
 
     # ---- particle-filter local search ----
     def pf_set_state(self, quat, trans, k, s01, peakR, transS):
