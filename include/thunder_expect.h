@@ -123,6 +123,18 @@ int thx_ctx_set_pixels(thx_ctx* ctx, const int* iCol, const int* iRow,
  * Replaces ExpectLocalV3D (gpu/interface/Interface.h:45-48) and the volume
  * upload inside ExpectProject. */
 int thx_ctx_set_volume(thx_ctx* ctx, int cls, const float* vol, int vdim);
+
+/* Projectee preparation on the GPU: the N^3 half-complex map volFT
+ * [(N/2+1)*N*N complex, x fastest] is transformed to real space (scaled by
+ * 1/N^3), zero-padded around the origin to V = pf*N, divided by
+ * TIK_RL(r / (pf*V)) = j0(pi r / (pf*V))^2 and transformed back; the
+ * (V/2+1)*V*V half-complex result becomes class cls's volume (vdim V).
+ * Replaces Projector::setProjectee(Volume, nThread) (src/Projector.cpp:123-148)
+ * with gridCorrection (src/Projector.cpp:524-606) and the 3-D FFTs of
+ * FFT::bw / FFT::fw (src/FFT.cpp:172-230).  msOut (optional, 3 floats):
+ * inverse FFT, pad + correction kernel and forward FFT times in ms. */
+int thx_ctx_set_projectee(thx_ctx* ctx, int cls, const float* volFT, int N, int pf,
+                          float* msOut);
 int thx_ctx_get_volume(thx_ctx* ctx, int cls, float* vol);
 
 /* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)

@@ -265,3 +265,28 @@ def keep_peak(u, factor=1e-3, set_factor=False):
     f = load().thxo_keep_peak(_ptr(u), u.size, C.c_double(factor), int(set_factor),
                               C.byref(top))
     return u, f, top.value
+
+
+def projectee(vol_ft, N, pf):
+    """Projector::setProjectee(Volume, nThread), src/Projector.cpp:123-148, with the
+    LINEAR gridCorrection (src/Projector.cpp:524-606), in numpy FP64:
+    FFT::bw (c2r, scaled by 1/N^3, src/FFT.cpp:201-230); VOL_PAD_RL (zero padding to
+    V = pf N around the origin, negative indices wrapped, include/Image/ImageFunctions.h:
+    176-192 with Volume::iRL, include/Image/Volume.h:519-527); division by
+    TIK_RL(r / (pf V)) = j0(pi r / (pf V))^2 (src/Functions/Functions.cpp:217-220),
+    r = |(i, j, k)| of the padded voxel; FFT::fw (r2c, unnormalised, src/FFT.cpp:172-199).
+    vol_ft: complex [N][N][N/2+1] (z, y, x); returns complex [V][V][V/2+1]."""
+    V = pf * N
+    real = np.fft.irfftn(np.asarray(vol_ft, np.complex128), s=(N, N, N), axes=(0, 1, 2))
+    c = np.r_[0:N // 2, -N // 2:0]
+    idx = np.where(c >= 0, c, c + V)
+    pad = np.zeros((V, V, V))
+    pad[np.ix_(idx, idx, idx)] = real
+    s = np.r_[0:V // 2, -V // 2:0].astype(np.float64)
+    r = np.sqrt(s[:, None, None] ** 2 + s[None, :, None] ** 2 + s[None, None, :] ** 2)
+    a = np.pi * r / (pf * V)
+    j0 = np.ones_like(a)
+    nz = a > 0
+    j0[nz] = np.sin(a[nz]) / a[nz]
+    pad /= j0 ** 2
+    return np.fft.rfftn(pad, axes=(0, 1, 2))

@@ -12,6 +12,10 @@
   the best rotation ranks within the nSelR most probable grid rotations; the
   tail depth (max log q - log q(best)) agrees to 1e-3 absolute and the spread
   (sum q)^2 / sum q^2 to 1e-3 relative (the kernel keeps log q in FP32).
+* Gather-scorer shapes: whole selections of 77 .. 600 rotations and 9, 49
+  and 81 translations run every rotation block (128, 256, 512 per block,
+  ragged last blocks) and translation tile (32, 64, 128 rows); each result
+  is held to the whole-selection bars against the FP64 oracle.
 * What training buys: at a fixed small selection the trained network's
   assignments agree with the plain FP32 scan on >= 80% of particles and at
   least 0.3 more often than an untrained network's.
@@ -140,3 +144,36 @@ def test_trained_network_gains(gpu_ctx):
     print(f"agreement within one sampling step: {agree}")
     assert agree["trained"] >= 0.8
     assert agree["trained"] - agree["untrained"] >= 0.3
+
+
+@pytest.mark.parametrize("R,side", [(77, 3), (200, 7), (600, 7), (600, 3), (300, 9)])
+def test_whole_selection_shapes(gpu_ctx, R, side):
+    Pn = 16
+    print(f"\n[setup] box64 P={Pn} R={R} T={side * side} learned whole selection", flush=True)
+    info = wl.setup_context(gpu_ctx, "box64", n_particles=Pn, n_rot=R)
+    trans = wl.shift_grid(side, 1.0)
+    gpu_ctx.set_grid(info["quat"], trans)
+    T = len(trans)
+    vol = gpu_ctx.get_volume(0, V)
+    dat, ctf, sig = gpu_ctx.get_particles()
+    dvp = oracle.global_scan(vol, V, PF, info["iCol"], info["iRow"], BOX, info["quat"], trans,
+                             dat, ctf, sig, np.full(R, 1.0 / R), np.full(T, 1.0 / T), f64=True,
+                             want_dvp=True)[4]
+    net = thx.Proposal(components=4, seed=3)
+    try:
+        nf = gpu_ctx.expect_learned(net, 0, R, T)
+        bR, bT, sc, *_ = gpu_ctx.get_learned()
+    finally:
+        net.close()
+    assert nf == 0
+    flat = dvp.reshape(Pn, -1)
+    best = flat.max(1)
+    chosen = flat[np.arange(Pn), bR * T + bT]
+    agree = np.mean(flat.argmax(1) == bR * T + bT)
+    rel_gap = (best - chosen) / np.abs(best)
+    rel_score = np.abs(sc - best) / np.abs(best)
+    print(f"agreement {agree:.3f}, max FP64 gap {rel_gap.max():.2e}, "
+          f"max score err {rel_score.max():.2e}")
+    assert agree >= 0.85
+    assert rel_gap.max() <= 2e-3
+    assert rel_score.max() <= 2e-3

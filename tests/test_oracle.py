@@ -521,3 +521,64 @@ def test_keep_peak_hand_worked():
     # a flat set clamps at 0.5; a single spike at 1e-3
     assert oracle.keep_peak(np.ones(40), set_factor=True)[1] == 0.5
     assert oracle.keep_peak(np.r_[1.0, np.zeros(39)], set_factor=True)[1] == 1e-3
+
+
+# ---- projectee preparation (Projector::setProjectee + gridCorrection) ----
+
+def _tik(r, pf, V):
+    a = math.pi * r / (pf * V)
+    return 1.0 if a == 0 else (math.sin(a) / a) ** 2
+
+
+def test_projectee_flat_spectrum_stays_flat():
+    """A flat half spectrum is a unit delta at the origin after the scaled
+    inverse transform; the origin is not moved by padding and its correction
+    is j0(0)^2 = 1, so the padded spectrum is flat too (pins the 1/N^3 scale
+    and the origin of the padding)."""
+    N, pf = 8, 2
+    out = oracle.projectee(np.ones((N, N, N // 2 + 1)), N, pf)
+    assert out.shape == (16, 16, 9)
+    np.testing.assert_allclose(out, 1.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("x0", [(2, -3, 1), (-4, 3, -2)])
+def test_projectee_shifted_delta_closed_form(x0):
+    """A unit delta at signed voxel x0 = (i, j, k): input spectrum
+    exp(-2 pi i k.x0 / N), output exp(-2 pi i k'.x0 / V) / j0(pi |x0| / (pf V))^2.
+    Pins the transform sign, the wrap of negative indices at both sizes and
+    the correction's argument r / (pf V) with V the padded size."""
+    N, pf = 8, 2
+    V = pf * N
+    i0, j0_, k0 = x0
+    kz, ky, kx = np.meshgrid(np.arange(N), np.arange(N), np.arange(N // 2 + 1), indexing="ij")
+    ft = np.exp(-2j * np.pi * (kx * i0 + ky * j0_ + kz * k0) / N)
+    out = oracle.projectee(ft, N, pf)
+    Kz, Ky, Kx = np.meshgrid(np.arange(V), np.arange(V), np.arange(V // 2 + 1), indexing="ij")
+    ref = np.exp(-2j * np.pi * (Kx * i0 + Ky * j0_ + Kz * k0) / V)
+    ref /= _tik(math.sqrt(i0 * i0 + j0_ * j0_ + k0 * k0), pf, V)
+    np.testing.assert_allclose(out, ref, atol=1e-12)
+
+
+def test_projectee_brute_force_dft():
+    """N = 4, pf = 2 on an asymmetric random map: every step by explicit sums
+    (DFT definitions, padding loop, TIK_RL), independent of numpy's FFT
+    conventions; pins the axis order (z, y, x) and the half dimension (x)."""
+    N, pf = 4, 2
+    V = pf * N
+    rng = np.random.default_rng(17)
+    f = rng.normal(size=(N, N, N))               # f[z][y][x]
+    n = np.arange(N)
+    E = np.exp(-2j * np.pi * np.outer(n, n) / N)
+    ft = np.einsum("zyx,az,by,cx->abc", f, E, E, E)[:, :, :N // 2 + 1]
+    pad = np.zeros((V, V, V))
+    for z in range(N):
+        for y in range(N):
+            for x in range(N):
+                k, j, i = (z if z < N // 2 else z - N), (y if y < N // 2 else y - N), \
+                    (x if x < N // 2 else x - N)
+                r = math.sqrt(i * i + j * j + k * k)
+                pad[k % V, j % V, i % V] = f[z, y, x] / _tik(r, pf, V)
+    m = np.arange(V)
+    EV = np.exp(-2j * np.pi * np.outer(m, m) / V)
+    ref = np.einsum("zyx,az,by,cx->abc", pad, EV, EV, EV)[:, :, :V // 2 + 1]
+    np.testing.assert_allclose(oracle.projectee(ft, N, pf), ref, atol=1e-10)

@@ -14,7 +14,10 @@ ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libthunder_expect.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_thx.so")
 
-HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_filter.hip", "thx_api.hip"]
+HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_filter.hip",
+               "thx_projectee.hip", "thx_api.hip"]
+# hipFFT: the two plain 3-D transforms of the projectee preparation
+HIP_LIBS = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
 
 HIPCC_FLAGS = [
     "-std=c++17", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",
@@ -34,7 +37,7 @@ def build_library():
     srcs = [os.path.join(PKG, "csrc", s) for s in HIP_SOURCES]
     tmp = LIB + ".tmp"
     _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"),
-                                  "-I", os.path.join(PKG, "csrc"), "-o", tmp] + srcs)
+                                  "-I", os.path.join(PKG, "csrc"), "-o", tmp] + srcs + HIP_LIBS)
     os.replace(tmp, LIB)
     return LIB
 

@@ -250,6 +250,7 @@ int thx_ctx_destroy(thx_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->locGraph) (void)hipGraphExecDestroy(c->locGraph);
+    projectee_release(c);
     DevBuf* bufs[] = {&c->iCol, &c->iRow, &c->iSig, &c->dat, &c->ctf, &c->sig, &c->Aterm,
                       &c->Y, &c->Wt, &c->quat, &c->trans, &c->pR, &c->pT, &c->pRf, &c->pTf,
                       &c->S, &c->rotP, &c->rotLo, &c->normP, &c->Cterm, &c->shellScale,

@@ -112,6 +112,11 @@ struct thx_ctx {
     // volumes per class
     std::vector<thx::DevBuf> vol;
     std::vector<int> vdim;
+    // projectee preparation: FFT plans (hipfftHandle) for one (N, pf), scratch
+    void* prjPlanBw = nullptr;
+    void* prjPlanFw = nullptr;
+    int prjN = 0, prjPf = 0;
+    thx::DevBuf prjIn, prjReal, prjPad;
     // particles (image-major, as uploaded) and derived operands
     int P = 0, Ppad = 0;
     thx::DevBuf dat, ctf, sig;
@@ -201,6 +206,8 @@ constexpr int kLocalDirect = 1;
 constexpr int kLocalQueue = 2;
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
                        int mode = kLocalDirect);
+// frees the projectee FFT plans and scratch
+void projectee_release(thx_ctx* c);
 int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
                        double* wT, double* base, int* bestR, int* bestT,

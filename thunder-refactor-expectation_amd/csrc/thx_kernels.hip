@@ -1770,7 +1770,9 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     // l8 = tid & 7 of eight consecutive lanes cover a row's 32-pixel chunk,
     // rows rq + NTH/8 q), phase rows as 2 complex pixels (l16 = tid & 15,
     // rows tq + NTH/16 j).  The norm term is taken from the 16-bit slice values, so
-    // no |P|^2 table is read.
+    // no |P|^2 table is read.  Rows past the selection read row 0 of their
+    // table: their columns (rotations) and rows (translations) of the tile are
+    // masked in the epilogue, so the loads need no predicate.
     constexpr int NJ = TPAD / (NTH / 16);
     const int l8 = tid & 7, rq = tid >> 3;
     const int l16 = tid & 15, tq = tid >> 4;
@@ -1778,13 +1780,13 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int row = rrow[rq + (NTH / 8) * q];
-        brow[q] = row >= 0 ? rotLo + (size_t)row * Kp : nullptr;
+        brow[q] = rotLo + (size_t)(row >= 0 ? row : 0) * Kp + 4 * l8;
     }
     const float2* srow[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int t = trow[tq + (NTH / 16) * j];
-        srow[j] = t >= 0 ? S + (size_t)t * Kp : nullptr;
+        srow[j] = S + (size_t)(t >= 0 ? t : 0) * Kp + 2 * l16;
     }
     const float2* yrow = Y + (size_t)pd * Kp;
     const float* wrow = Wt + (size_t)pd * Kp;
@@ -1798,13 +1800,9 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     float4 yv, wv;
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            bv[q] = brow[q] ? *reinterpret_cast<const u32x4*>(brow[q] + k0 + 4 * l8)
-                            : u32x4{0u, 0u, 0u, 0u};
+        for (int q = 0; q < 4; q++) bv[q] = *reinterpret_cast<const u32x4*>(brow[q] + k0);
 #pragma unroll
-        for (int j = 0; j < NJ; j++)
-            sv[j] = srow[j] ? *reinterpret_cast<const float4*>(srow[j] + k0 + 2 * l16)
-                            : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int j = 0; j < NJ; j++) sv[j] = *reinterpret_cast<const float4*>(srow[j] + k0);
         yv = *reinterpret_cast<const float4*>(yrow + k0 + 2 * l16);
         wv = *reinterpret_cast<const float4*>(wrow + k0 + 4 * l8);
     };
@@ -1817,9 +1815,12 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
             float c = 0.f;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const uint32_t v = bv[q][e];
-                const float pr = __uint_as_float(v << 16), pi = __uint_as_float(v & 0xffff0000u);
-                c += w4[e] * (pr * pr + pi * pi);
+                // |P|^2 of one packed BF16 pixel: re^2 + im^2 (exact products;
+                // the element goes through a scalar: a bit_cast of a vector
+                // element expression reads element 0 with this compiler)
+                const uint32_t u = bv[q][e];
+                const bf16x2 v = __builtin_bit_cast(bf16x2, u);
+                c += w4[e] * __builtin_amdgcn_fdot2_f32_bf16(v, v, 0.f, false);
             }
             cpart[q] += c;
             *reinterpret_cast<u32x4*>(Bs + (rq + (NTH / 8) * q) * LD + 8 * l8) = bv[q];

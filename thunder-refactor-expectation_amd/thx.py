@@ -35,6 +35,7 @@ _SIGS = {
     "thx_ctx_set_pixels": [_p, _p, _p, _p, _i, _i, _i],
     "thx_ctx_set_volume": [_p, _i, _p, _i],
     "thx_ctx_get_volume": [_p, _i, _p],
+    "thx_ctx_set_projectee": [_p, _i, _p, _i, _i, _p],
     "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
     "thx_ctx_get_particles": [_p, _p, _p, _p],
     "thx_ctx_set_grid": [_p, _p, _i, _p, _i, _p, _p],
@@ -207,6 +208,18 @@ class Context:
     def set_volume(self, cls, vol, vdim):
         vol = c32(vol)
         check(load().thx_ctx_set_volume(self._h, cls, ptr(vol), vdim), "set_volume")
+
+    def set_projectee(self, cls, vol_ft, N, pf=2):
+        """Projectee preparation on the GPU (Projector::setProjectee): vol_ft is
+        the N^3 map's half-complex FT [N][N][N/2+1]; class cls then holds the
+        padded, grid-corrected (pf N)^3 projectee.  Returns the inverse FFT,
+        pad + correction and forward FFT times (ms)."""
+        ft = np.ascontiguousarray(np.asarray(vol_ft).astype(np.complex64))
+        assert ft.shape == (N, N, N // 2 + 1), ft.shape
+        ms = np.zeros(3, np.float32)
+        check(load().thx_ctx_set_projectee(self._h, cls, ptr(ft), N, pf, ptr(ms)),
+              "set_projectee")
+        return ms
 
     def get_volume(self, cls, vdim):
         out = np.zeros((vdim, vdim, vdim // 2 + 1, 2), np.float32)
