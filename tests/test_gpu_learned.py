@@ -13,7 +13,7 @@
   tail depth (max log q - log q(best)) agrees to 1e-3 absolute and the spread
   (sum q)^2 / sum q^2 to 1e-3 relative (the kernel keeps log q in FP32).
 * Gather-scorer shapes: whole selections of 77 .. 600 rotations and 9, 49
-  and 81 translations run every rotation block (128, 256, 512 per block,
+  and 81 translations run every rotation block (128 and 256 per block,
   ragged last blocks) and translation tile (32, 64, 128 rows); each result
   is held to the whole-selection bars against the FP64 oracle.
 * What training buys: at a fixed small selection the trained network's

@@ -73,6 +73,17 @@ WORKLOADS = {
                    n_particles=50000, n_rot=1000, shift_side=9, shift_step=1.0,
                    max_shift=5.0, df_min=8000.0, df_max=35000.0, snr=0.03, cutoff=64,
                    n_class=3),
+    # Learned proposal at box 192: trained on particles of n_train_vol random
+    # blob maps (SNR 0.01-0.1), scored on a held-out map; 300 rotations x 25
+    # translations per particle from the proposal, uniform 5000-rotation
+    # fallback.  Not fixed by the workload statement, and chosen here: blob
+    # width and radius of box 128 scaled by 192 / 128, cutoff 75 px (box 128's
+    # 50 scaled), a 5 x 5 shift grid at 1 px (the 25 translations), shifts
+    # within +-2 px, box 128's defocus range and SNR 0.03 for the held-out set.
+    "box192": dict(box=192, pf=2, n_blob=200, blob_sigma=4.5, blob_radius=72.0,
+                   n_particles=100000, n_rot=5000, shift_side=5, shift_step=1.0,
+                   max_shift=2.0, df_min=8000.0, df_max=35000.0, snr=0.03, cutoff=75,
+                   n_class=1, n_train_vol=8),
     # FP8 scoring at large box: box 256.
     "box256": dict(box=256, pf=2, n_blob=500, blob_sigma=6.0, blob_radius=96.0,
                    n_particles=20000, n_rot=10000, shift_side=9, shift_step=2.0,
