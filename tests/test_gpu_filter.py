@@ -77,9 +77,11 @@ def test_keep_peak_matches_oracle(gpu_ctx, set_factor):
         np.testing.assert_array_equal(out[p], ref)
 
 
-def test_local_search_refines_global_assignments(gpu_ctx):
+@pytest.mark.parametrize("prec", [thx.FP32, thx.FP16])
+def test_local_search_refines_global_assignments(gpu_ctx, prec):
     """Global scan on a coarse grid, particle states from it, local phases to
-    convergence: the final poses must beat the grid's own resolution."""
+    convergence (FP32 or FP16 cross term): the final poses must beat the
+    grid's own resolution."""
     print("\n[setup] box64 P=64 R=1000 T=49 local search", flush=True)
     info = wl.setup_context(gpu_ctx, "box64", n_particles=64, n_rot=1000)
     quat, trans = info["quat"], info["trans"]
@@ -92,7 +94,7 @@ def test_local_search_refines_global_assignments(gpu_ctx):
     st0 = gpu_ctx.pf_get_state()
     assert np.all(st0["k"] > 0) and np.all(st0["s01"] > 0)
     assert np.all((st0["peakR"] >= 1e-3) & (st0["peakR"] <= 0.5))
-    ran = gpu_ctx.pf_refine(None, thx.FP32, after_global=True, max_phase=40, seed=4)
+    ran = gpu_ctx.pf_refine(None, prec, after_global=True, max_phase=40, seed=4)
     st = gpu_ctx.pf_get_state()
     err = wl.geodesic_angle(st["topR"], info["true_quat"])
     conv = st["nPhase"] >= 0

@@ -4,8 +4,9 @@ Tolerances: FP32 baseline (best log-likelihood) relative error <= 1e-4 and the
 same best (rotation, translation) per particle; marginals within 3e-3 of each
 particle's largest entry (an FP32 log-likelihood of magnitude ~1e3 carries an
 absolute rounding error ~1e-3, which is the relative error of exp(dvp - max)).
-BF16: best-score relative error <= 2e-3 and >= 80% identical best poses on
-this small noisy case.
+BF16 and FP16 (FP16 with per-shell operand scales taken from the class
+volumes and the particles): best-score relative error <= 2e-3 and >= 80%
+identical best poses on this small noisy case.
 """
 import numpy as np
 import pytest
@@ -75,17 +76,19 @@ def test_local_fp32_matches_oracle(gpu_ctx, local_case, tmax):
         assert np.abs(a - b).max() <= 3e-3 * b.max()
 
 
-def test_local_bf16_close_to_oracle(gpu_ctx, local_case):
+@pytest.mark.parametrize("prec", [thx.BF16, thx.FP16])
+def test_local_16bit_close_to_oracle(gpu_ctx, local_case, prec):
     c = local_case
     rng = np.random.default_rng(5)
     quat, nR, trans, nT, pR, pT = _batch(c, 24, rng, 260, 49)
-    gpu_ctx.score_local(quat, nR, trans, nT, pR, pT, precision=thx.BF16)
+    gpu_ctx.score_local(quat, nR, trans, nT, pR, pT, precision=prec)
     wC, wR, wT, base, bR, bT = gpu_ctx.get_local()
     ref = oracle.local_scan(c["vol"], V, PF, c["iCol"], c["iRow"], BOX, quat, nR, trans, nT,
                             pR, pT, c["dat"], c["ctf"], c["sig"])
     rel = np.abs(base - ref[3]) / np.abs(ref[3])
     agree = np.mean((bR == ref[4]) & (bT == ref[5]))
-    print(f"BF16 local: max rel base err {rel.max():.2e}, best-pose agreement {agree:.3f}")
+    print(f"precision {prec} local: max rel base err {rel.max():.2e}, best-pose agreement "
+          f"{agree:.3f}")
     assert rel.max() <= 2e-3
     assert agree >= 0.8
 
@@ -115,7 +118,8 @@ def test_local_box160_three_classes(gpu_ctx):
     K = 6,323), at reduced counts: per-particle rotations drawn about the
     true pose (spread of a von Mises-Fisher kappa = 50), 25-81 translations,
     each particle scored against its own class volume in one launch.  FP32
-    against the FP64 oracle run per class; BF16 at its bar."""
+    against the FP64 oracle run per class; BF16 and FP16 (the workload's
+    cross-term format) at their bar."""
     name = "box160"
     w = wl.WORKLOADS[name]
     sd = wl.seeds(name)
@@ -168,7 +172,7 @@ def test_local_box160_three_classes(gpu_ctx):
         for i, p in enumerate(ps):
             ref[p] = (out[0][i], out[1][r0[i]:r0[i + 1]], out[2][t0[i]:t0[i + 1]], out[3][i],
                       out[4][i], out[5][i])
-    for prec in (thx.FP32, thx.BF16):
+    for prec in (thx.FP32, thx.BF16, thx.FP16):
         gpu_ctx.score_local(quat, nR, tr, nT, pR, pT, cls=cls, precision=prec)
         wC, wR, wT, base, bR, bT = gpu_ctx.get_local()
         rel = np.array([abs(base[p] - ref[p][3]) / abs(ref[p][3]) for p in range(P)])
@@ -211,18 +215,19 @@ def test_local_launch_shapes_and_graph_replays(gpu_ctx, local_case):
         np.testing.assert_array_equal(a, b)
     for a, b in zip(outs[1], outs[0]):
         np.testing.assert_array_equal(a, b)
-    eager = []
-    gpu_ctx.set_local_options(0, False)
-    for b in same:
-        gpu_ctx.score_local(*b, precision=thx.BF16)
-        eager.append(gpu_ctx.get_local())
-    gpu_ctx.set_local_options(0, True)
-    for rep in range(2):
-        for i, b in enumerate(same):
-            gpu_ctx.score_global(0, thx.FP32)      # eager work between replays
-            gpu_ctx.sync()
-            gpu_ctx.score_local(*b, precision=thx.BF16)
-            got = gpu_ctx.get_local()
-            for a, e in zip(got, eager[i]):
-                np.testing.assert_array_equal(a, e)
+    for prec in (thx.BF16, thx.FP16):
+        eager = []
+        gpu_ctx.set_local_options(0, False)
+        for b in same:
+            gpu_ctx.score_local(*b, precision=prec)
+            eager.append(gpu_ctx.get_local())
+        gpu_ctx.set_local_options(0, True)
+        for rep in range(2):
+            for i, b in enumerate(same):
+                gpu_ctx.score_global(0, thx.FP32)      # eager work between replays
+                gpu_ctx.sync()
+                gpu_ctx.score_local(*b, precision=prec)
+                got = gpu_ctx.get_local()
+                for a, e in zip(got, eager[i]):
+                    np.testing.assert_array_equal(a, e)
     gpu_ctx.set_local_options(0, False)

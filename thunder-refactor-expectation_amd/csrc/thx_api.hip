@@ -560,25 +560,23 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     clear_error();
     THX_CHECK(c && quat && nR && trans && nT && pR && pT, THX_ERR_ARG,
               "score_local: null argument");
-    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
-              "score_local: precision must be FP32 or BF16");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16 || precision == THX_FP16,
+              THX_ERR_ARG, "score_local: precision must be FP32, BF16 or FP16");
     THX_CHECK(c->K > 0 && c->P > 0, THX_ERR_STATE, "score_local: pixels and particles must be set");
     THX_CHECK(!c->vol.empty(), THX_ERR_STATE, "score_local: no volume set");
     const int P = c->P;
     const int nCls = (int)c->vol.size();
     // every class the particles use must hold a volume of this pixel set's size
-    {
-        std::vector<char> used(nCls, 0);
-        if (cls) {
-            for (int q = 0; q < c->P; q++)
-                if (cls[q] >= 0 && cls[q] < nCls) used[cls[q]] = 1;
-        } else {
-            used[0] = 1;
-        }
-        for (int k = 0; k < nCls; k++)
-            THX_CHECK(!used[k] || c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
-                      "score_local: a class volume in use does not match idim * pf");
+    std::vector<char> used(nCls, 0);
+    if (cls) {
+        for (int q = 0; q < c->P; q++)
+            if (cls[q] >= 0 && cls[q] < nCls) used[cls[q]] = 1;
+    } else {
+        used[0] = 1;
     }
+    for (int k = 0; k < nCls; k++)
+        THX_CHECK(!used[k] || c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
+                  "score_local: a class volume in use does not match idim * pf");
     std::vector<int> rOff(P), tOff(P), bStart(P + 1);
     std::vector<int2> blocks;
     size_t Rtot = 0, Ttot = 0;
@@ -656,7 +654,11 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     b.wRrel = c->locWRrel.as<float>();
     c->nev = 0;
     c->evNames.clear();
-    THX_TRY(launch_prep_particles(c, THX_FP32));
+    // FP16: per-shell operand scales from the class volumes in use and the
+    // particles (eager: no memset node may enter a captured graph)
+    const int prepPrec = precision == THX_FP16 ? THX_FP16 : THX_FP32;
+    if (precision == THX_FP16) THX_TRY(launch_local_shell_scale(c, used));
+    THX_TRY(launch_prep_particles(c, prepPrec));
     const int mode = choose_local_mode(c, precision, TPAD, nblk, b);
     if (mode < 0) return THX_ERR_HIP;
     c->locModeUsed = mode;
@@ -684,7 +686,7 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
                 c->locGraph = nullptr;
             }
             THX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            int cs = launch_prep_particles(c, THX_FP32);
+            int cs = launch_prep_particles(c, prepPrec);
             if (cs == THX_OK) cs = launch_score_local(c, precision, TPAD, nblk, b, mode);
             if (cs == THX_OK)
                 cs = launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(),
@@ -710,7 +712,7 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
         return THX_OK;
     }
     THX_TRY(record(c, "start"));
-    THX_TRY(launch_prep_particles(c, THX_FP32));
+    THX_TRY(launch_prep_particles(c, prepPrec));
     THX_TRY(record(c, "prep"));
     THX_TRY(launch_score_local(c, precision, TPAD, nblk, b, mode));
     THX_TRY(record(c, "score_local"));

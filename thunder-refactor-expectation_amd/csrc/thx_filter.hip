@@ -651,23 +651,21 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     clear_error();
     THX_CHECK(c && c->pfValid && c->pfP == c->P, THX_ERR_STATE,
               "pf_refine: particle-filter state not set for the context's particles");
-    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
-              "pf_refine: precision must be FP32 or BF16");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16 || precision == THX_FP16,
+              THX_ERR_ARG, "pf_refine: precision must be FP32, BF16 or FP16");
     THX_CHECK(!c->vol.empty() && c->K > 0, THX_ERR_STATE, "pf_refine: volume and pixels needed");
     const int nCls = (int)c->vol.size();
     // every class the particles use must hold a volume of this pixel set's size
-    {
-        std::vector<char> used(nCls, 0);
-        if (cls) {
-            for (int q = 0; q < c->P; q++)
-                if (cls[q] >= 0 && cls[q] < nCls) used[cls[q]] = 1;
-        } else {
-            used[0] = 1;
-        }
-        for (int k = 0; k < nCls; k++)
-            THX_CHECK(!used[k] || c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
-                      "pf_refine: a class volume in use does not match idim * pf");
+    std::vector<char> used(nCls, 0);
+    if (cls) {
+        for (int q = 0; q < c->P; q++)
+            if (cls[q] >= 0 && cls[q] < nCls) used[cls[q]] = 1;
+    } else {
+        used[0] = 1;
     }
+    for (int k = 0; k < nCls; k++)
+        THX_CHECK(!used[k] || c->vdim[k] == c->idim * c->pf, THX_ERR_STATE,
+                  "pf_refine: a class volume in use does not match idim * pf");
     THX_HIP(hipSetDevice(c->device));
     const int P = c->P, mLR = c->pfR, mLT = c->pfT;
     if (cls)
@@ -694,7 +692,9 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     THX_TRY(c->locBestT.ensure(sizeof(int) * P));
     THX_TRY(c->pfNRa.ensure(sizeof(int) * P));
     THX_TRY(c->pfNTa.ensure(sizeof(int) * P));
-    THX_TRY(launch_prep_particles(c, THX_FP32));
+    // FP16: per-shell operand scales (particles and volumes stay fixed over the phases)
+    if (precision == THX_FP16) THX_TRY(launch_local_shell_scale(c, used));
+    THX_TRY(launch_prep_particles(c, precision == THX_FP16 ? THX_FP16 : THX_FP32));
     const int start = afterGlobal ? 1 : 0;
     const int minPhase = afterGlobal ? 10 : 3;   // MIN_N_PHASE_PER_ITER_{GLOBAL,LOCAL}
     std::vector<int> act(P), nRa(P), nTa(P), bStart(P + 1);

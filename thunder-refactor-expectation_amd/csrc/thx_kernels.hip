@@ -348,6 +348,68 @@ int launch_shell_scale(thx_ctx* c, int precision) {
     return THX_OK;
 }
 
+// Per-shell bound of |P| for slices projected on the fly from a half-complex
+// volume (the local scorer): a trilinear sample at pixel radius r reads
+// voxels within sqrt(3) of pf r, so each voxel's magnitude bounds shells
+// rint(rho / pf) - 1 .. + 1 (rho its padded radius; sqrt(3) / pf < 1 for
+// pf >= 2).  Block maxima in LDS, then one atomic per shell and block.
+__global__ __launch_bounds__(256) void k_vol_shell_max(const float2* __restrict__ vol, int V,
+                                                       int pf, int nShell,
+                                                       float* __restrict__ maxP) {
+    __shared__ float sm[1024];
+    for (int s = threadIdx.x; s < nShell; s += blockDim.x) sm[s] = 0.f;
+    __syncthreads();
+    const int nx = V / 2 + 1;
+    const size_t n = (size_t)nx * V * V;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % nx);
+        const int y0 = (int)((i / nx) % V), z0 = (int)(i / ((size_t)nx * V));
+        const int y = y0 < V / 2 ? y0 : y0 - V, z = z0 < V / 2 ? z0 : z0 - V;
+        const int s = (int)rintf(sqrtf((float)(x * x + y * y + z * z)) / pf);
+        if (s > nShell) continue;
+        const float2 v = vol[i];
+        const float m = fmaxf(fabsf(v.x), fabsf(v.y));
+        for (int t = max(s - 1, 0); t <= min(s + 1, nShell - 1); t++)
+            atomicMax(reinterpret_cast<int*>(&sm[t]), __float_as_int(m));
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < nShell; s += blockDim.x)
+        if (sm[s] > 0.f) atomic_max_pos(&maxP[s], sm[s]);
+}
+
+// Shell scales of the FP16 local scorer from the class volumes in use and
+// the particles (as launch_shell_scale does from the grid's slices).
+int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used) {
+    const int nShell = c->maxShell + 1;
+    THX_CHECK(nShell <= 1024, THX_ERR_RANGE, "local FP16: too many shells");
+    THX_TRY(c->shellMax.ensure(sizeof(float) * 2 * nShell));
+    THX_TRY(c->shellScale.ensure(sizeof(float) * (nShell + 1)));
+    THX_HIP(hipMemsetAsync(c->shellMax.p, 0, sizeof(float) * 2 * nShell, c->stream));
+    float* maxP = c->shellMax.as<float>();
+    float* maxY = maxP + nShell;
+    for (size_t k = 0; k < used.size(); k++) {
+        if (!used[k]) continue;
+        hipLaunchKernelGGL(k_vol_shell_max, dim3(1024), dim3(256), 0, c->stream,
+                           c->vol[k].as<const float2>(), c->vdim[k], c->pf, nShell, maxP);
+        THX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_prep, dim3(c->Ppad), dim3(256), 0, c->stream,
+                       c->dat.as<const float2>(), c->ctf.as<const float>(),
+                       c->sig.as<const float>(), c->P, c->K, c->Kp, c->iSig.as<const int>(),
+                       (const float*)nullptr, c->Aterm.as<double>(), c->Y.as<float2>(),
+                       c->Wt.as<float>());
+    THX_HIP(hipGetLastError());
+    dim3 gridY((c->K + 255) / 256, 64);
+    hipLaunchKernelGGL(k_shell_max, gridY, dim3(256), 0, c->stream, c->Y.as<const float2>(),
+                       c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), maxY);
+    THX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_shell_finish, dim3(1), dim3(256), 0, c->stream, maxP, maxY, nShell, 0,
+                       c->shellScale.as<float>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
 int launch_convert_lo(thx_ctx* c, int precision) {
     const size_t n = (size_t)c->Rpad * c->Kp;
     const float* scale = (precision == THX_FP16 || precision == THX_FP8)
@@ -1231,7 +1293,8 @@ constexpr int kNBL = 128;
 template <int PREC, int TPAD>
 __device__ __forceinline__ void score_local_item(int item, 
     const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
-    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int Kp, int N,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, const int* __restrict__ iSig,
+    const float* __restrict__ shellScale, int K, int Kp, int N,
     const float2* __restrict__ Y, const float* __restrict__ Wt,
     const double* __restrict__ Aterm, const double* __restrict__ quat,
     const float* __restrict__ pRf, const int* __restrict__ rOff, const int* __restrict__ nR,
@@ -1314,6 +1377,9 @@ __device__ __forceinline__ void score_local_item(int item,
         {
             const float wk = wrow[k];
             const bool kok = k < K;
+            // FP16: the slice is divided by its shell's scale (Y carries it)
+            float inv = 1.f;
+            if constexpr (PREC == THX_FP16) inv = 1.f / shellScale[iSig[k]];
             const double c0 = (double)(ic * pf), c1 = (double)(ir * pf);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
@@ -1331,7 +1397,8 @@ __device__ __forceinline__ void score_local_item(int item,
                     Bs[rr * LD + 2 * l32] = v.x;
                     Bs[rr * LD + 2 * l32 + 1] = v.y;
                 } else {
-                    *reinterpret_cast<uint32_t*>(Bs + rr * LD + 2 * l32) = pack16<PREC>(v.x, v.y);
+                    *reinterpret_cast<uint32_t*>(Bs + rr * LD + 2 * l32) =
+                        pack16<PREC>(v.x * inv, v.y * inv);
                 }
             }
         }
@@ -1481,7 +1548,8 @@ __device__ __forceinline__ void score_local_item(int item,
 template <int PREC, int TPAD>
 __global__ __launch_bounds__(256, 2) void k_score_local(
     const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
-    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int Kp, int N,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, const int* __restrict__ iSig,
+    const float* __restrict__ shellScale, int K, int Kp, int N,
     const float2* __restrict__ Y, const float* __restrict__ Wt,
     const double* __restrict__ Aterm, const double* __restrict__ quat,
     const float* __restrict__ pRf, const int* __restrict__ rOff, const int* __restrict__ nR,
@@ -1489,7 +1557,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
     const int* __restrict__ tOff, const int* __restrict__ nT, const int2* __restrict__ blocks,
     float* __restrict__ bMax, float* __restrict__ bWC, int* __restrict__ bArg,
     float* __restrict__ bWT, float* __restrict__ wRrel) {
-    score_local_item<PREC, TPAD>(blockIdx.x, vols, pcls, V, pf, iCol, iRow, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+    score_local_item<PREC, TPAD>(blockIdx.x, vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
 }
 
 // Persistent launch: a grid sized to the resident capacity pulls work items
@@ -1497,7 +1565,8 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
 template <int PREC, int TPAD>
 __global__ __launch_bounds__(256, 2) void k_score_local_q(
     const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
-    const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int Kp, int N,
+    const int* __restrict__ iCol, const int* __restrict__ iRow, const int* __restrict__ iSig,
+    const float* __restrict__ shellScale, int K, int Kp, int N,
     const float2* __restrict__ Y, const float* __restrict__ Wt,
     const double* __restrict__ Aterm, const double* __restrict__ quat,
     const float* __restrict__ pRf, const int* __restrict__ rOff, const int* __restrict__ nR,
@@ -1513,7 +1582,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local_q(
         __syncthreads();
         const int it = item;
         if (it >= nItems) break;
-        score_local_item<PREC, TPAD>(it, vols, pcls, V, pf, iCol, iRow, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+        score_local_item<PREC, TPAD>(it, vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
     }
 }
 
@@ -1525,13 +1594,15 @@ __global__ void k_zero_int(int* __restrict__ counter) {
 template <int PREC>
 static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int grid, int* counter,
                               const float2* const* vols, const int* pcls, int V, int pf,
-                              const int* iCol, const int* iRow, int K, int Kp, int N,
+                              const int* iCol, const int* iRow, const int* iSig,
+                              const float* shellScale, int K, int Kp, int N,
                               const float2* Y, const float* Wt, const double* Aterm,
                               const double* quat, const float* pRf, const int* rOff,
                               const int* nR, const double* trans, const float* pTf,
                               const int* tOff, const int* nT, const int2* blocks, float* bMax,
                               float* bWC, int* bArg, float* bWT, float* wRrel) {
-#define THX_LOCAL_ARGS vols, pcls, V, pf, iCol, iRow, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, \
+#define THX_LOCAL_ARGS vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, \
+                       quat, pRf, rOff, \
                        nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel
     if (mode == kLocalQueue) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, counter);
@@ -1564,10 +1635,14 @@ static int local_queue_grid(int precision, int TPAD) {
         fn = TPAD == 32 ? (const void*)k_score_local_q<THX_FP32, 32>
              : TPAD == 64 ? (const void*)k_score_local_q<THX_FP32, 64>
                           : (const void*)k_score_local_q<THX_FP32, 128>;
-    else
+    else if (precision == THX_BF16)
         fn = TPAD == 32 ? (const void*)k_score_local_q<THX_BF16, 32>
              : TPAD == 64 ? (const void*)k_score_local_q<THX_BF16, 64>
                           : (const void*)k_score_local_q<THX_BF16, 128>;
+    else
+        fn = TPAD == 32 ? (const void*)k_score_local_q<THX_FP16, 32>
+             : TPAD == 64 ? (const void*)k_score_local_q<THX_FP16, 64>
+                          : (const void*)k_score_local_q<THX_FP16, 128>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0) != hipSuccess || per < 1)
         per = 2;
     return cus * per;
@@ -1575,8 +1650,8 @@ static int local_queue_grid(int precision, int TPAD) {
 
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
                        int mode) {
-    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
-              "local scorer supports FP32 and BF16");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16 || precision == THX_FP16,
+              THX_ERR_ARG, "local scorer supports FP32, BF16 and FP16");
     int grid = 0;
     if (mode == kLocalQueue) {
         THX_TRY(c->locCounter.ensure(sizeof(int)));
@@ -1585,12 +1660,15 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
 #define THX_LOCAL_CALL(PR)                                                                      \
     launch_local_prec<PR>(c->stream, TPAD, nblk, mode, grid, c->locCounter.as<int>(), b.vols,  \
                           b.pcls, c->idim * c->pf, c->pf, c->iCol.as<const int>(),             \
-                          c->iRow.as<const int>(), c->K, c->Kp, c->idim,                       \
+                          c->iRow.as<const int>(), c->iSig.as<const int>(),                    \
+                          PR == THX_FP16 ? c->shellScale.as<const float>() : nullptr, c->K,    \
+                          c->Kp, c->idim,                                                      \
                           c->Y.as<const float2>(), c->Wt.as<const float>(),                    \
                           c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR, b.trans,   \
                           b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel)
     if (precision == THX_FP32) THX_LOCAL_CALL(THX_FP32);
-    else THX_LOCAL_CALL(THX_BF16);
+    else if (precision == THX_BF16) THX_LOCAL_CALL(THX_BF16);
+    else THX_LOCAL_CALL(THX_FP16);
 #undef THX_LOCAL_CALL
     THX_HIP(hipGetLastError());
     return THX_OK;
