@@ -183,7 +183,8 @@ int thx_ctx_score_one(thx_ctx* ctx, int cls, int p, float* out);
  * <= 128 translations (pixels, concatenated) with prior weights pR, pT (same
  * concatenation), against the volume of class cls[p] (cls may be NULL: class
  * 0).  Slices are projected on the fly; no R x K intermediate is stored.
- * precision: THX_FP32 or THX_BF16.  Marginals as in the global scan:
+ * precision: THX_FP32, THX_BF16 or THX_FP16 (per-shell operand scales).
+ * Marginals as in the global scan:
  *   wC[p] = sum w pR pT, wR[r] = sum_t w pT, wT[t] = sum_r w pR,
  *   w = exp(dvp - baseL[p]), baseL[p] = max dvp. */
 int thx_ctx_score_local(thx_ctx* ctx, const int* cls, const double* quat,
@@ -285,7 +286,8 @@ int thx_pf_from_global(thx_ctx* ctx, int cls, int mLR, int mLT, double minStdR,
  * expectation loop, src/Optimiser.cpp:1475-1660, with updateSP :1139-1437):
  * per phase, perturbation (pfL at phase 0, else pfS) with balanced weights,
  * scoring of every (rotation, shift) sample against the particle's class
- * (cls[P] or NULL: class 0) at precision THX_FP32 or THX_BF16, half-height
+ * (cls[P] or NULL: class 0) at precision THX_FP32, THX_BF16 or THX_FP16
+ * (per-shell operand scales bounded from the class volumes), half-height
  * peak of the rotation marginal, top samples, calVari, resampling, and the
  * variance test (from phase 10 after a global scan, afterGlobal != 0, where
  * phases start at 1; else from phase 3).  *phasesRun: phases executed. */
