@@ -217,3 +217,38 @@ def test_fp8_box256_against_oracle(gpu_ctx):
     assert res["fp32"][0].max() <= 1e-4 and res["fp32"][2] == 1.0
     assert res["fp8"][0].max() <= 5e-3
     assert res["fp8"][1].max() <= 3e-3
+
+
+def test_box128_against_oracle(gpu_ctx):
+    """Box 128, cutoff 50 (K = 3,853), the 11 x 11 shift grid: plain FP32 within
+    1e-4 relative of the FP64 oracle's best log-likelihood with the identical
+    top pose; BF16 and FP16 at their bars (best-score error, chosen pose's FP64
+    gap)."""
+    print("\n[setup] box128 P=16 R=64 T=121", flush=True)
+    info = wl.setup_context(gpu_ctx, "box128", n_particles=16, n_rot=64)
+    w = wl.WORKLOADS["box128"]
+    Vb = w["box"] * w["pf"]
+    vol = gpu_ctx.get_volume(0, Vb)
+    dat, ctf, sig = gpu_ctx.get_particles()
+    R, T = len(info["quat"]), len(info["trans"])
+    assert len(info["iCol"]) == 3853 and T == 121
+    dvp = oracle.global_scan(vol, Vb, w["pf"], info["iCol"], info["iRow"], w["box"],
+                             info["quat"], info["trans"], dat, ctf, sig, np.full(R, 1.0 / R),
+                             np.full(T, 1.0 / T), f64=True, want_dvp=True)[4]
+    del vol
+    flat = dvp.reshape(len(dvp), -1)
+    best = flat.max(1)
+    for prec in ("fp32", "bf16", "fp16"):
+        gpu_ctx.score_global(0, thx.PRECISIONS[prec])
+        bR, bT, bS = gpu_ctx.get_best(0)
+        chosen = flat[np.arange(len(flat)), bR * T + bT]
+        rel = np.abs(bS - best) / np.abs(best)
+        gap = (best - chosen) / np.abs(best)
+        agree = np.mean(flat.argmax(1) == bR * T + bT)
+        print(f"{prec}: max rel best err {rel.max():.2e}, max rel FP64 gap {gap.max():.2e}, "
+              f"agreement {agree:.3f}")
+        if prec == "fp32":
+            assert rel.max() <= 1e-4 and agree == 1.0
+        else:
+            r_max, a_min, g_max = LOW_BARS[prec]
+            assert rel.max() <= r_max and gap.max() <= g_max and agree >= a_min

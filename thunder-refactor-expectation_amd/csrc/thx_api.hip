@@ -157,6 +157,33 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
 }
 
 
+// Particles sorted by the pose of one rotation each (quat + 4 off[p]): by
+// class, then along a Morton curve over (x, y, z) of the quaternion taken with
+// w >= 0, so neighbours in the list project nearby slices of the same volume.
+std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P) {
+    std::vector<std::pair<uint64_t, int>> key(P);
+    for (int p = 0; p < P; p++) {
+        const double* q = quat + 4 * (size_t)off[p];
+        const double s = q[0] < 0 ? -1.0 : 1.0;
+        uint64_t m = 0;
+        uint32_t c[3];
+        for (int a = 0; a < 3; a++) {
+            const double v = std::min(1.0, std::max(-1.0, s * q[a + 1]));
+            c[a] = (uint32_t)std::min(1023.0, std::floor((v + 1.0) * 512.0));
+        }
+        for (int b = 9; b >= 0; b--)
+            for (int a = 0; a < 3; a++) m = (m << 1) | ((c[a] >> b) & 1u);
+        key[p] = {((uint64_t)(cls ? cls[p] : 0) << 32) | m, p};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint64_t, int>& x, const std::pair<uint64_t, int>& y) {
+                         return x.first < y.first;
+                     });
+    std::vector<int> order(P);
+    for (int i = 0; i < P; i++) order[i] = key[i].second;
+    return order;
+}
+
 // Launch shape of the local scorer for this batch: the requested one, or the
 // faster of the two timed once per (precision, padding, item-count bucket).
 int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b) {
@@ -589,11 +616,14 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
         if (cls) THX_CHECK(cls[p] >= 0 && cls[p] < nCls, THX_ERR_ARG, "score_local: bad class");
         rOff[p] = (int)Rtot;
         tOff[p] = (int)Ttot;
-        bStart[p] = (int)blocks.size();
-        for (int r0 = 0; r0 < nR[p]; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
         Rtot += nR[p];
         Ttot += nT[p];
         maxT = std::max(maxT, nT[p]);
+    }
+    // work items particle by particle in pose order (class, then first rotation)
+    for (int p : pose_order(quat, rOff.data(), cls, P)) {
+        bStart[p] = (int)blocks.size();
+        for (int r0 = 0; r0 < nR[p]; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
     }
     bStart[P] = (int)blocks.size();
     THX_CHECK(Rtot < (1u << 31) && Ttot < (1u << 31), THX_ERR_RANGE, "score_local: batch too large");
@@ -663,7 +693,7 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     if (mode < 0) return THX_ERR_HIP;
     c->locModeUsed = mode;
     if (c->locGraphOn) {
-        THX_TRY(c->locCounter.ensure(sizeof(int)));   // no allocation inside the capture
+        THX_TRY(c->locCounter.ensure(8 * sizeof(int)));   // no allocation inside the capture
         // one captured graph per batch shape and buffer set: operand prep,
         // scorer, merge; replayed while they stay the same
         auto ptrs = [](std::initializer_list<const void*> l) {

@@ -700,11 +700,19 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     std::vector<int> act(P), nRa(P), nTa(P), bStart(P + 1);
     std::vector<int2> blocks;
     THX_HIP(hipMemcpy(act.data(), c->pfActive.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    // work-list order by pose (the particles move little over the phases):
+    // the first sample of each particle
+    std::vector<double> q0(4 * (size_t)P);
+    std::vector<int> off0(P);
+    THX_HIP(hipMemcpy2D(q0.data(), 4 * sizeof(double), c->pfQ.p, 4 * sizeof(double) * mLR,
+                        4 * sizeof(double), P, hipMemcpyDeviceToHost));
+    for (int p = 0; p < P; p++) off0[p] = p;
+    const std::vector<int> order = pose_order(q0.data(), off0.data(), cls, P);
     int phase = start, ran = 0;
     for (; phase < maxPhase; phase++) {
         int nAct = 0;
         blocks.clear();
-        for (int p = 0; p < P; p++) {
+        for (int p : order) {
             bStart[p] = (int)blocks.size();
             nRa[p] = act[p] ? mLR : 0;
             nTa[p] = act[p] ? mLT : 0;

@@ -206,6 +206,8 @@ constexpr int kLocalDirect = 1;
 constexpr int kLocalQueue = 2;
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
                        int mode = kLocalDirect);
+// particle order for the local scorer's work list (pose locality)
+std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P);
 // FP16 shell scales of the local scorer from the used class volumes + particles
 int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used);
 // frees the projectee FFT plans and scratch

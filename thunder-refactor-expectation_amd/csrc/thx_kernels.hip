@@ -1557,11 +1557,16 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
     const int* __restrict__ tOff, const int* __restrict__ nT, const int2* __restrict__ blocks,
     float* __restrict__ bMax, float* __restrict__ bWC, int* __restrict__ bArg,
     float* __restrict__ bWT, float* __restrict__ wRrel) {
-    score_local_item<PREC, TPAD>(blockIdx.x, vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+    // the list is in pose order: each XCD walks a contiguous range of it, so
+    // concurrent blocks of one XCD gather from overlapping volume regions
+    score_local_item<PREC, TPAD>(xcd_logical(blockIdx.x, gridDim.x), vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
 }
 
 // Persistent launch: a grid sized to the resident capacity pulls work items
-// from an atomic counter until the ragged list is exhausted.
+// from atomic counters until the ragged list is exhausted.  The list (in pose
+// order) is cut into 8 contiguous ranges, one per XCD (workgroups are dealt
+// to the XCDs round-robin, so block b starts on range b % 8); a block that
+// drains its range moves on to the next ones.
 template <int PREC, int TPAD>
 __global__ __launch_bounds__(256, 2) void k_score_local_q(
     const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
@@ -1576,19 +1581,27 @@ __global__ __launch_bounds__(256, 2) void k_score_local_q(
     float* __restrict__ bWT, float* __restrict__ wRrel, int nItems,
     int* __restrict__ counter) {
     __shared__ int item;
+    const int q = nItems / 8, rmd = nItems % 8;
+    int h = 0, x = blockIdx.x % 8;
     for (;;) {
+        const int lo = x * q + min(x, rmd), n = q + (x < rmd ? 1 : 0);
         __syncthreads();   // the previous item's LDS traffic is over
-        if (threadIdx.x == 0) item = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) item = atomicAdd(&counter[x], 1);
         __syncthreads();
-        const int it = item;
-        if (it >= nItems) break;
+        const int j = item;
+        if (j >= n) {      // this range is drained: the next XCD's
+            if (++h == 8) break;
+            x = (x + 1) % 8;
+            continue;
+        }
+        const int it = lo + j;
         score_local_item<PREC, TPAD>(it, vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
     }
 }
 
-// counter = 0 (a kernel, so a captured graph holds no memset node)
+// counter[0..7] = 0 (a kernel, so a captured graph holds no memset node)
 __global__ void k_zero_int(int* __restrict__ counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
+    if (threadIdx.x < 8 && blockIdx.x == 0) counter[threadIdx.x] = 0;
 }
 
 template <int PREC>
@@ -1654,7 +1667,7 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
               THX_ERR_ARG, "local scorer supports FP32, BF16 and FP16");
     int grid = 0;
     if (mode == kLocalQueue) {
-        THX_TRY(c->locCounter.ensure(sizeof(int)));
+        THX_TRY(c->locCounter.ensure(8 * sizeof(int)));
         grid = local_queue_grid(precision, TPAD);
     }
 #define THX_LOCAL_CALL(PR)                                                                      \
@@ -1683,7 +1696,9 @@ __global__ __launch_bounds__(256) void k_merge_local(
     double* __restrict__ wR, double* __restrict__ wT, double* __restrict__ base,
     int* __restrict__ bestR, int* __restrict__ bestT) {
     const int p = blockIdx.x;
-    const int b0 = bStart[p], b1 = bStart[p + 1], nb = b1 - b0;
+    // a particle's blocks are contiguous from bStart[p] (particles themselves
+    // may be listed in any order)
+    const int b0 = bStart[p], nb = (nR[p] + rb - 1) / rb;
     __shared__ double f[64];
     __shared__ double red[256];
     __shared__ int redi[256];
