@@ -282,6 +282,18 @@ int thx_pf_from_global(thx_ctx* ctx, int cls, int mLR, int mLT, double minStdR,
                        double minStdT, double pfS, const double* transS,
                        uint64_t seed);
 
+/* As thx_pf_from_global over the classes 0..nCls-1 scanned last
+ * (src/Optimiser.cpp:926-966): per particle the class marginals on a common
+ * baseline, u_k = wC_k exp(base_k - max base), keep their half-height peak
+ * with the fixed factor 1 - 1e-2 (setPeakFactor / keepHalfHeightPeak(PAR_C),
+ * src/Particle.cpp:1879-1886, 1935-1944), are resampled to k systematic
+ * draws and one draw is taken uniformly (resample(k, PAR_C) + rand(cls),
+ * src/Particle.cpp:1275-1462, 2080-2089); the particle's rotations and
+ * shifts then come from that class's marginals.  clsOut[P]: the classes. */
+int thx_pf_from_global_classes(thx_ctx* ctx, int nCls, int mLR, int mLT, double minStdR,
+                               double minStdT, double pfS, const double* transS,
+                               uint64_t seed, int* clsOut);
+
 /* Local search phases until every particle converges or maxPhase (the
  * expectation loop, src/Optimiser.cpp:1475-1660, with updateSP :1139-1437):
  * per phase, perturbation (pfL at phase 0, else pfS) with balanced weights,

@@ -110,3 +110,47 @@ def test_local_search_refines_global_assignments(gpu_ctx, prec):
     # weights are normalised per particle, samples stay unit quaternions
     np.testing.assert_allclose(st["wR"].sum(1), 1.0, rtol=1e-9)
     np.testing.assert_allclose(np.linalg.norm(st["quat"], axis=2), 1.0, rtol=1e-9)
+
+
+def test_class_draw_and_per_class_states(gpu_ctx):
+    """Two classes (independently seeded box-64 maps, 24 particles of each):
+    the drawn class of every particle is among the classes that survive the
+    class half-height peak (marginal >= 0.99 of the particle's largest, the
+    merged marginals of both scans on one baseline); >= 90% of particles get
+    the class they were made from; and each particle's state equals, bit for
+    bit, the single-class update from its drawn class with the same seed."""
+    w = wl.WORKLOADS["box64"]
+    sd = wl.seeds("box64")
+    iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
+    gpu_ctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    per, parts = 24, []
+    for c in range(2):
+        gpu_ctx.synth_volume(c, w["box"], w["pf"], w["n_blob"], w["blob_sigma"],
+                             w["blob_radius"], sd["volume"] + 17 * c)
+        gpu_ctx.synth_particles(c, per, w["max_shift"], w["df_min"], w["df_max"], w["snr"],
+                                sd["particles"] + c, **wl.CTF_DEFAULTS)
+        parts.append(gpu_ctx.get_particles())
+    gpu_ctx.set_particles(*[np.concatenate([x[i] for x in parts]) for i in range(3)])
+    truth = np.repeat([0, 1], per)
+    quat, trans = wl.grid("box64", 1000)
+    gpu_ctx.set_grid(quat, trans)
+    for c in range(2):
+        gpu_ctx.score_global(c, thx.FP32)
+    wC, _, _, _ = gpu_ctx.get_global(2)
+    args = (300, 25, 1000 ** (-1.0 / 3), 0.3, 0.5, 1.0)
+    cls = gpu_ctx.pf_from_global_classes(2, *args, seed=3)
+    surv = wC >= 0.99 * wC.max(1, keepdims=True)
+    assert np.all(surv[np.arange(2 * per), cls])
+    print(f"drawn class = true class for {np.mean(cls == truth):.3f}")
+    assert np.mean(cls == truth) >= 0.9
+    st = gpu_ctx.pf_get_state()
+    for c in range(2):
+        gpu_ctx.pf_from_global(c, *args, seed=3)
+        ref = gpu_ctx.pf_get_state()
+        m = cls == c
+        for key in ("quat", "trans", "wR", "wT", "k", "s01", "peakR"):
+            np.testing.assert_array_equal(st[key][m], ref[key][m])
+    # the local search takes the drawn classes
+    gpu_ctx.pf_from_global_classes(2, *args, seed=3)
+    ran = gpu_ctx.pf_refine(cls, thx.BF16, after_global=True, max_phase=12, seed=4)
+    assert ran >= 1

@@ -563,27 +563,40 @@ int thx_pf_set_state(thx_ctx* c, int mLR, int mLT, const double* quat, const dou
     return THX_OK;
 }
 
-int thx_pf_from_global(thx_ctx* c, int cls, int mLR, int mLT, double minStdR, double minStdT,
-                       double pfS, const double* transS, uint64_t seed) {
-    clear_error();
-    THX_CHECK(c && transS && pfS > 0, THX_ERR_ARG, "pf_from_global: bad argument");
-    THX_CHECK(cls >= 0 && cls < (int)c->res.size() && c->res[cls].valid, THX_ERR_STATE,
+// dst[p][0..n) = rows[cls[p]][p][0..n): each particle's marginal from its class
+__global__ void k_gather_class_rows(const double* const* __restrict__ rows,
+                                    const int* __restrict__ cls, int n,
+                                    double* __restrict__ dst) {
+    const int p = blockIdx.x;
+    const double* src = rows[cls[p]] + (size_t)p * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[(size_t)p * n + i] = src[i];
+}
+
+static int check_scan(thx_ctx* c, int k) {
+    THX_CHECK(k >= 0 && k < (int)c->res.size() && c->res[k].valid, THX_ERR_STATE,
               "pf_from_global: no global scan of this class");
+    THX_CHECK(c->res[k].wR.bytes >= sizeof(double) * (size_t)c->P * c->R &&
+                  c->res[k].wT.bytes >= sizeof(double) * (size_t)c->P * c->T,
+              THX_ERR_STATE, "pf_from_global: marginals of the last scan do not match the grid");
+    return THX_OK;
+}
+
+// Particle states from the rotation / translation marginals of class cls0
+// (pcls == null) or of each particle's own class pcls[p] (host array).
+static int from_global_impl(thx_ctx* c, int cls0, const int* pcls, int nCls, int mLR, int mLT,
+                            double minStdR, double minStdT, double pfS, const double* transS,
+                            uint64_t seed) {
     THX_CHECK(mLR >= 2 && mLR <= kResampleMaxN && mLT >= 2 && mLT <= 128 &&
                   c->R <= kResampleMaxN && c->T <= kResampleMaxN,
               THX_ERR_RANGE, "pf_from_global: sizes out of range");
     THX_CHECK(c->pR.p && c->pT.p && c->pR.bytes >= sizeof(double) * c->R &&
                   c->pT.bytes >= sizeof(double) * c->T && c->quat.p && c->trans.p,
               THX_ERR_STATE, "pf_from_global: grid and its priors must be set");
-    THX_CHECK(c->res[cls].wR.bytes >= sizeof(double) * (size_t)c->P * c->R &&
-                  c->res[cls].wT.bytes >= sizeof(double) * (size_t)c->P * c->T,
-              THX_ERR_STATE, "pf_from_global: marginals of the last scan do not match the grid");
     THX_HIP(hipSetDevice(c->device));
     const int P = c->P, R = c->R, T = c->T;
     hipStream_t st = c->stream;
     THX_TRY(pf_alloc(c, P, mLR, mLT));
     THX_HIP(hipMemcpy(c->pfTransS.p, transS, sizeof(double) * P, hipMemcpyHostToDevice));
-    ClassResult& res = c->res[cls];
     // marginals of the grid: copies (the peak keeping works in place)
     DevBuf uR, uT, offR, offT, nRg, nTg;
     std::vector<int> oR(P), oT(P), cR(P, R), cT(P, T);
@@ -602,8 +615,44 @@ int thx_pf_from_global(thx_ctx* c, int cls, int mLR, int mLT, double minStdR, do
         cleanup();
         return s;
     }
-    THX_HIP(hipMemcpyAsync(uR.p, res.wR.p, sizeof(double) * (size_t)P * R, hipMemcpyDeviceToDevice, st));
-    THX_HIP(hipMemcpyAsync(uT.p, res.wT.p, sizeof(double) * (size_t)P * T, hipMemcpyDeviceToDevice, st));
+    if (!pcls) {
+        const ClassResult& res = c->res[cls0];
+        THX_HIP(hipMemcpyAsync(uR.p, res.wR.p, sizeof(double) * (size_t)P * R,
+                               hipMemcpyDeviceToDevice, st));
+        THX_HIP(hipMemcpyAsync(uT.p, res.wT.p, sizeof(double) * (size_t)P * T,
+                               hipMemcpyDeviceToDevice, st));
+    } else {
+        std::vector<const double*> rowsR(nCls), rowsT(nCls);
+        for (int k = 0; k < nCls; k++) {
+            rowsR[k] = c->res[k].wR.as<const double>();
+            rowsT[k] = c->res[k].wT.as<const double>();
+        }
+        DevBuf dR, dT, dC;
+        int e = THX_OK;
+        if ((e = dR.ensure(sizeof(void*) * nCls)) || (e = dT.ensure(sizeof(void*) * nCls)) ||
+            (e = dC.ensure(sizeof(int) * P))) {
+            cleanup();
+            return e;
+        }
+        bool ok = hipMemcpy(dR.p, rowsR.data(), sizeof(void*) * nCls, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(dT.p, rowsT.data(), sizeof(void*) * nCls, hipMemcpyHostToDevice) == hipSuccess &&
+                  hipMemcpy(dC.p, pcls, sizeof(int) * P, hipMemcpyHostToDevice) == hipSuccess;
+        if (ok) {
+            hipLaunchKernelGGL(k_gather_class_rows, dim3(P), dim3(256), 0, st,
+                               dR.as<const double* const>(), dC.as<const int>(), R, uR.as<double>());
+            hipLaunchKernelGGL(k_gather_class_rows, dim3(P), dim3(256), 0, st,
+                               dT.as<const double* const>(), dC.as<const int>(), T, uT.as<double>());
+            ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+        }
+        dR.release();
+        dT.release();
+        dC.release();
+        if (!ok) {
+            cleanup();
+            set_error("pf_from_global: gathering the class marginals failed");
+            return THX_ERR_HIP;
+        }
+    }
     THX_HIP(hipMemcpyAsync(offR.p, oR.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
     THX_HIP(hipMemcpyAsync(offT.p, oT.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
     THX_HIP(hipMemcpyAsync(nRg.p, cR.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
@@ -644,6 +693,72 @@ int thx_pf_from_global(thx_ctx* c, int cls, int mLR, int mLT, double minStdR, do
     THX_HIP(hipStreamSynchronize(st));
     cleanup();
     return THX_OK;
+}
+
+int thx_pf_from_global(thx_ctx* c, int cls, int mLR, int mLT, double minStdR, double minStdT,
+                       double pfS, const double* transS, uint64_t seed) {
+    clear_error();
+    THX_CHECK(c && transS && pfS > 0, THX_ERR_ARG, "pf_from_global: bad argument");
+    THX_TRY(check_scan(c, cls));
+    return from_global_impl(c, cls, nullptr, 0, mLR, mLT, minStdR, minStdT, pfS, transS, seed);
+}
+
+int thx_pf_from_global_classes(thx_ctx* c, int nCls, int mLR, int mLT, double minStdR,
+                               double minStdT, double pfS, const double* transS, uint64_t seed,
+                               int* clsOut) {
+    clear_error();
+    THX_CHECK(c && transS && clsOut && pfS > 0 && nCls >= 1, THX_ERR_ARG,
+              "pf_from_global_classes: bad argument");
+    for (int k = 0; k < nCls; k++) THX_TRY(check_scan(c, k));
+    THX_HIP(hipSetDevice(c->device));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    const int P = c->P;
+    // class marginals on a common baseline: u_k = wC_k exp(base_k - max base)
+    std::vector<double> wC((size_t)nCls * P), base((size_t)nCls * P);
+    for (int k = 0; k < nCls; k++) {
+        THX_HIP(hipMemcpy(wC.data() + (size_t)k * P, c->res[k].wC.p, sizeof(double) * P,
+                          hipMemcpyDeviceToHost));
+        THX_HIP(hipMemcpy(base.data() + (size_t)k * P, c->res[k].base.p, sizeof(double) * P,
+                          hipMemcpyDeviceToHost));
+    }
+    std::vector<double> u(nCls), cum(nCls);
+    std::vector<int> pick(nCls);
+    for (int p = 0; p < P; p++) {
+        double bmax = -INFINITY;
+        for (int k = 0; k < nCls; k++) bmax = std::max(bmax, base[(size_t)k * P + p]);
+        double umax = 0.0;
+        for (int k = 0; k < nCls; k++) {
+            u[k] = wC[(size_t)k * P + p] * std::exp(base[(size_t)k * P + p] - bmax);
+            umax = std::max(umax, u[k]);
+        }
+        // keepHalfHeightPeak(PAR_C) with the fixed factor PEAK_FACTOR_C = 1 - 1e-2
+        // (src/Particle.cpp:1879-1886, 1935-1944; include/Particle.h:47)
+        const double hh = umax * (1.0 - 1e-2);
+        double sum = 0.0;
+        for (int k = 0; k < nCls; k++) {
+            u[k] = u[k] < hh ? 0.0 : u[k] - hh;
+            sum += u[k];
+        }
+        // resample(k, PAR_C) from uniform class weights: systematic draws
+        // u0 + j / k over the cumulative weights (src/Particle.cpp:1275-1462),
+        // then rand(cls): one of the k draws, uniformly (:2080-2089)
+        uint32_t w[4];
+        Philox::draw(seed, (uint32_t)p, 0, kStreamClass, w);
+        const double u0 = Philox::u01(w[0], w[1]) / nCls, u1 = Philox::u01(w[2], w[3]);
+        double acc = 0.0;
+        for (int k = 0; k < nCls; k++) {
+            acc += sum > 0.0 ? u[k] / sum : 1.0 / nCls;
+            cum[k] = acc;
+        }
+        int k = 0;
+        for (int j = 0; j < nCls; j++) {
+            const double t = u0 + (double)j / nCls;
+            while (k < nCls - 1 && cum[k] <= t) k++;
+            pick[j] = k;
+        }
+        clsOut[p] = pick[std::min(nCls - 1, (int)(u1 * nCls))];
+    }
+    return from_global_impl(c, -1, clsOut, nCls, mLR, mLT, minStdR, minStdT, pfS, transS, seed);
 }
 
 int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, double pfL,

@@ -285,7 +285,8 @@ enum Stream : uint32_t {
     kStreamPose = 2,
     kStreamShift = 3,
     kStreamCtf = 4,
-    kStreamNoise = 5
+    kStreamNoise = 5,
+    kStreamClass = 6
 };
 
 // Hamilton product (quaternion_mul, src/Geometry/Euler.cpp:13-26).

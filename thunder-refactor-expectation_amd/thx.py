@@ -69,6 +69,7 @@ _SIGS = {
     "thx_ctx_get_learned": [_p, _p, _p, _p, _p, _p, _p, _p],
     "thx_pf_set_state": [_p, _i, _i, _p, _p, _p, _p, _p, _p],
     "thx_pf_from_global": [_p, _i, _i, _i, _d, _d, _d, _p, _u64],
+    "thx_pf_from_global_classes": [_p, _i, _i, _i, _d, _d, _d, _p, _u64, _p],
     "thx_pf_refine": [_p, _p, _i, _i, _d, _d, _i, _u64, _p],
     "thx_pf_get_state": [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "thx_pf_rot_stats": [_p, _p, _i, _i, _p, _p, _p, _p, _p],
@@ -329,6 +330,18 @@ class Context:
         transS = np.broadcast_to(np.asarray(transS, np.float64), (self.P,)).copy()
         check(load().thx_pf_from_global(self._h, cls, mLR, mLT, min_std_r, min_std_t, pf_s,
                                         ptr(transS), seed), "pf_from_global")
+
+    def pf_from_global_classes(self, n_cls, mLR, mLT, min_std_r, min_std_t, pf_s, transS,
+                               seed=1):
+        """Class draw per particle from the scanned classes 0..n_cls-1, then
+        particle states from that class's marginals; returns the classes."""
+        self.mLR, self.mLT = mLR, mLT
+        transS = np.broadcast_to(np.asarray(transS, np.float64), (self.P,)).copy()
+        out = np.zeros(self.P, np.int32)
+        check(load().thx_pf_from_global_classes(self._h, n_cls, mLR, mLT, min_std_r, min_std_t,
+                                                pf_s, ptr(transS), seed, ptr(out)),
+              "pf_from_global_classes")
+        return out
 
     def pf_refine(self, cls=None, precision=FP32, after_global=False, pf_l=2.0, pf_s=0.5,
                   max_phase=100, seed=1):
