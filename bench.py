@@ -233,7 +233,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(particles shard across GPUs, see DESIGN.md)"}))
+                                   "(the particles are independent: each GPU would run this path on its share)"}))
         sys.exit(2)
     name = args.workload
     w = wl.WORKLOADS[name]

@@ -6,7 +6,7 @@
  * algorithms (THUNDER 1.4.11) in plain C; each function cites the reference
  * file:line it follows.  Arithmetic types follow the default reference build
  * (SINGLE_PRECISION=ON, so RFLOAT = float; rotations and particle weights are
- * double), except the *_f64 variants, which are the FP64 judges.
+ * double), except the *_f64 variants, which are the FP64 references.
  *
  * Parity pinning: the reference ships no golden vectors or tests for this
  * path, so each function is pinned by closed forms, invariants and brute
@@ -217,7 +217,7 @@ void thxo_logdatavs_pm(const float* dat, const float* pri, const float* ctf,
         }
 }
 
-/* FP64 judge of the same sum for one image-major particle. */
+/* FP64 reference of the same sum for one image-major particle. */
 double thxo_logdatavs_f64(const float* dat, const float* pri, const float* ctf,
                           const float* sigRcp, int K)
 {
@@ -339,7 +339,7 @@ int thxo_global_scan_f32(const float* vol, int V, int pf,
 }
 
 /*
- * FP64 judge of the same marginals: dvp[l][m][n] in FP64 from the FP64
+ * FP64 reference of the same marginals: dvp[l][m][n] in FP64 from the FP64
  * projection and phases, baseline = max over (m, n) (two passes, so the
  * result does not depend on the streaming order of :835-895), then
  * wC = sum w pR pT, wR[m] = sum_n w pT, wT[n] = sum_m w pR.
@@ -415,7 +415,7 @@ int thxo_global_scan_f64(const float* vol, int V, int pf,
 }
 
 /*
- * One local-search phase for a batch of particles, FP64 judge
+ * One local-search phase for a batch of particles, FP64 reference
  * (Optimiser::updateSP, src/Optimiser.cpp:1139-1437, one class, no CTF
  * search): particle l scores its own rotations quat[rOff[l] .. + nR[l]) and
  * translations trans[tOff[l] .. + nT[l]) with priors pR, pT (same offsets):

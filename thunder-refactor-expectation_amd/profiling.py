@@ -76,7 +76,7 @@ def copy_scale(d, per_launch_bytes):
 def summarize(a):
     rows = list(csv.DictReader(open(os.path.join(a.trace, "run_kernel_stats.csv"))))
     lines = [f"# Kernel trace: {a.name}", "", "Command (repository root, `cd /tmp && export "
-             "TMPDIR=/tmp` first, rocprofv3 writing under gpurun_out/):", "", "```",
+             "TMPDIR=/tmp` first; rocprofv3 writes its CSV files under the -d directory):", "", "```",
              a.command, "```", "", "| Kernel | Calls | Average ms | Min ms | Max ms | % |",
              "|---|---|---|---|---|---|"]
     for r in rows:
