@@ -151,6 +151,59 @@ def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
                     "scorer, one trilinear slice per (particle, rotation)"}
 
 
+def run_local_box160(steps=1):
+    """Ragged local-search scoring at the box-160 workload's full size: 50,000
+    particles over 3 class maps, R_p uniform in 200..2000 rotations drawn about
+    each particle's true pose (von Mises-Fisher kappa 50), T_p uniform in 25..81
+    shifts, FP16 cross term, every particle in one launch.  The particle images
+    come from class 0's map; each particle is routed to the class p % 3, so the
+    projections gather from all three 132 MB volumes as the workload does (the
+    timing does not depend on which map made the image)."""
+    name = "box160"
+    w = wl.WORKLOADS[name]
+    sd = wl.seeds(name)
+    lctx = thx.Context(0)
+    iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
+    lctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    for c in range(3):
+        lctx.synth_volume(c, w["box"], w["pf"], w["n_blob"], w["blob_sigma"], w["blob_radius"],
+                          sd["volume"] + c)
+    P = w["n_particles"]
+    q, t, _ = lctx.synth_particles(0, P, w["max_shift"], w["df_min"], w["df_max"], w["snr"],
+                                   sd["particles"], **wl.CTF_DEFAULTS)
+    rng = np.random.default_rng(160)
+    nR = rng.integers(200, 2001, P).astype(np.int32)
+    nT = rng.integers(25, 82, P).astype(np.int32)
+    quat = np.repeat(q, nR, axis=0)
+    quat += rng.normal(0, 1 / np.sqrt(2 * 50), size=quat.shape)
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    trans = np.repeat(t, nT, axis=0) + rng.uniform(-3, 3, size=(int(nT.sum()), 2))
+    pR, pT = np.repeat(1.0 / nR, nR), np.repeat(1.0 / nT, nT)
+    cls = (np.arange(P) % 3).astype(np.int32)
+    args = (quat, nR, trans, nT, pR, pT, cls, thx.FP16)
+    # the direct launch (what the tuner picks at this size), without the
+    # tuner's extra timing runs: each call takes about 13 s
+    lctx.set_local_options(1, False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lctx.score_local(*args)
+    lctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    kern = lctx.last_timing()
+    mode = lctx.local_mode()
+    K = len(iCol)
+    lctx.close()
+    samples = float(nR.sum()) * K
+    t_k = sum(kern.values()) * 1e-3
+    return {"particles": P, "classes": 3, "rotations": int(nR.sum()),
+            "pose_evals": float(np.sum(nR.astype(np.int64) * nT)), "K": K,
+            "ms_per_call_with_upload": 1e3 * dt, "kernels_ms": kern,
+            "pose_evals_per_s": float(np.sum(nR.astype(np.int64) * nT)) / t_k,
+            "trilinear_samples_per_s": samples / t_k,
+            "launch": ["", "direct", "persistent work queue"][max(0, mode)],
+            "precision": "fp16"}
+
+
 def run_projectee(name, steps):
     """Projectee preparation (Projector::setProjectee) of a workload-sized map:
     inverse FFT, pad + sinc^2 correction kernel, forward FFT, each timed on the
@@ -230,6 +283,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--local-phases", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-box160", action="store_true",
+                    help="skip the full-size box-160 ragged local scoring")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -325,6 +380,10 @@ def main():
     log("local search")
     out["local_search"] = run_local_search(ctx, info, name, args.local_phases)
     log(f"local search: {out['local_search']}")
+    if not args.no_box160:
+        log("box-160 ragged local scoring")
+        out["local_box160"] = run_local_box160()
+        log(f"box-160 local: {out['local_box160']}")
     log("projectee preparation")
     out["projectee"] = run_projectee(name, args.steps)
     log(f"projectee: {out['projectee']}")
