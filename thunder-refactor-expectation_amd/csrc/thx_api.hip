@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -160,21 +161,23 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
 // Particles sorted by the pose of one rotation each (quat + 4 off[p]): by
 // class, then along a Morton curve over (x, y, z) of the quaternion taken with
 // w >= 0, so neighbours in the list project nearby slices of the same volume.
+// 30-bit Morton code of (x, y, z) of a unit quaternion taken with w >= 0
+uint32_t quat_morton(const double* q) {
+    const double s = q[0] < 0 ? -1.0 : 1.0;
+    uint32_t m = 0, c[3];
+    for (int a = 0; a < 3; a++) {
+        const double v = std::min(1.0, std::max(-1.0, s * q[a + 1]));
+        c[a] = (uint32_t)std::min(1023.0, std::floor((v + 1.0) * 512.0));
+    }
+    for (int b = 9; b >= 0; b--)
+        for (int a = 0; a < 3; a++) m = (m << 1) | ((c[a] >> b) & 1u);
+    return m;
+}
+
 std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P) {
     std::vector<std::pair<uint64_t, int>> key(P);
-    for (int p = 0; p < P; p++) {
-        const double* q = quat + 4 * (size_t)off[p];
-        const double s = q[0] < 0 ? -1.0 : 1.0;
-        uint64_t m = 0;
-        uint32_t c[3];
-        for (int a = 0; a < 3; a++) {
-            const double v = std::min(1.0, std::max(-1.0, s * q[a + 1]));
-            c[a] = (uint32_t)std::min(1023.0, std::floor((v + 1.0) * 512.0));
-        }
-        for (int b = 9; b >= 0; b--)
-            for (int a = 0; a < 3; a++) m = (m << 1) | ((c[a] >> b) & 1u);
-        key[p] = {((uint64_t)(cls ? cls[p] : 0) << 32) | m, p};
-    }
+    for (int p = 0; p < P; p++)
+        key[p] = {((uint64_t)(cls ? cls[p] : 0) << 32) | quat_morton(quat + 4 * (size_t)off[p]), p};
     std::stable_sort(key.begin(), key.end(),
                      [](const std::pair<uint64_t, int>& x, const std::pair<uint64_t, int>& y) {
                          return x.first < y.first;
@@ -629,14 +632,45 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     THX_CHECK(Rtot < (1u << 31) && Ttot < (1u << 31), THX_ERR_RANGE, "score_local: batch too large");
     const int TPAD = maxT <= 32 ? 32 : (maxT <= 64 ? 64 : 128);
     const int nblk = (int)blocks.size();
+    // each particle's rotations in pose order, so the 128 rotations of a work
+    // item are a compact cluster and their gathers share cache lines; wR and
+    // the best rotation are mapped back by thx_ctx_get_local
+    c->locPerm.resize(Rtot);
+    c->locROffH = rOff;
+    std::vector<double> qP(4 * Rtot);
     std::vector<float> fR(Rtot), fT(Ttot);
-    for (size_t i = 0; i < Rtot; i++) fR[i] = (float)pR[i];
+    {
+        // particles split over host threads (large batches take seconds serially)
+        auto sort_range = [&](int pa, int pb) {
+            std::vector<std::pair<uint32_t, int>> key;
+            for (int p = pa; p < pb; p++) {
+                key.resize(nR[p]);
+                for (int r = 0; r < nR[p]; r++)
+                    key[r] = {quat_morton(quat + 4 * ((size_t)rOff[p] + r)), rOff[p] + r};
+                std::sort(key.begin(), key.end());
+                for (int r = 0; r < nR[p]; r++) {
+                    const size_t dst = (size_t)rOff[p] + r, src = (size_t)key[r].second;
+                    c->locPerm[dst] = (int)src;
+                    for (int e = 0; e < 4; e++) qP[4 * dst + e] = quat[4 * src + e];
+                    fR[dst] = (float)pR[src];
+                }
+            }
+        };
+        const int nth = Rtot < (1u << 20) ? 1
+                        : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (int i = 1; i < nth; i++)
+            pool.emplace_back(sort_range, (int)((long long)P * i / nth),
+                              (int)((long long)P * (i + 1) / nth));
+        sort_range(0, (int)((long long)P / nth));
+        for (auto& t : pool) t.join();
+    }
     for (size_t i = 0; i < Ttot; i++) fT[i] = (float)pT[i];
     std::vector<const float2*> vp(nCls);
     for (int k = 0; k < nCls; k++) vp[k] = c->vol[k].as<const float2>();
     THX_TRY(set_device(c));
     hipStream_t st = c->stream;
-    THX_TRY(upload(c->locQuat, quat, 4 * Rtot, st));
+    THX_TRY(upload(c->locQuat, qP.data(), 4 * Rtot, st));
     THX_TRY(upload(c->locTrans, trans, 2 * Ttot, st));
     THX_TRY(upload(c->locPRf, fR.data(), Rtot, st));
     THX_TRY(upload(c->locPTf, fT.data(), Ttot, st));
@@ -777,10 +811,22 @@ int thx_ctx_get_local(thx_ctx* c, double* wC, double* wR, double* wT, double* ba
     THX_HIP(hipStreamSynchronize(c->stream));
     const int P = c->locP;
     if (wC) THX_HIP(hipMemcpy(wC, c->locWC.p, sizeof(double) * P, hipMemcpyDeviceToHost));
-    if (wR) THX_HIP(hipMemcpy(wR, c->locWR.p, sizeof(double) * c->locRtot, hipMemcpyDeviceToHost));
+    if (wR) {
+        // the scorer's rotations were in pose order: back to the caller's
+        std::vector<double> w(c->locRtot);
+        THX_HIP(hipMemcpy(w.data(), c->locWR.p, sizeof(double) * c->locRtot, hipMemcpyDeviceToHost));
+        const bool perm = c->locPerm.size() == c->locRtot;
+        for (size_t i = 0; i < c->locRtot; i++) wR[perm ? c->locPerm[i] : i] = w[i];
+    }
     if (wT) THX_HIP(hipMemcpy(wT, c->locWT.p, sizeof(double) * c->locTtot, hipMemcpyDeviceToHost));
     if (baseL) THX_HIP(hipMemcpy(baseL, c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost));
-    if (bestR) THX_HIP(hipMemcpy(bestR, c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    if (bestR) {
+        THX_HIP(hipMemcpy(bestR, c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+        if (c->locPerm.size() == c->locRtot && (int)c->locROffH.size() == P)
+            for (int p = 0; p < P; p++)
+                if (bestR[p] >= 0)
+                    bestR[p] = c->locPerm[(size_t)c->locROffH[p] + bestR[p]] - c->locROffH[p];
+    }
     if (bestT) THX_HIP(hipMemcpy(bestT, c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost));
     return THX_OK;
 }

@@ -761,6 +761,80 @@ int thx_pf_from_global_classes(thx_ctx* c, int nCls, int mLR, int mLT, double mi
     return from_global_impl(c, -1, clsOut, nCls, mLR, mLT, minStdR, minStdT, pfS, transS, seed);
 }
 
+// Each active particle's n rotation samples and their weights, reordered in
+// place along a Morton curve over (x, y, z) of the quaternion (w >= 0): the
+// samples are an unordered set for the filter, and in this order the local
+// scorer's 128-rotation work items are compact clusters whose gathers share
+// cache lines.  One block per particle; bitonic sort of (key, index) in LDS;
+// sets larger than kSortMaxN are left as they are.
+constexpr int kSortMaxN = 1024;
+
+__device__ inline uint32_t quat_morton_dev(const double* q) {
+    const double s = q[0] < 0 ? -1.0 : 1.0;
+    uint32_t m = 0, c[3];
+    for (int a = 0; a < 3; a++) {
+        const double v = fmin(1.0, fmax(-1.0, s * q[a + 1]));
+        c[a] = (uint32_t)fmin(1023.0, floor((v + 1.0) * 512.0));
+    }
+    for (int b = 9; b >= 0; b--)
+        for (int a = 0; a < 3; a++) m = (m << 1) | ((c[a] >> b) & 1u);
+    return m;
+}
+
+__global__ __launch_bounds__(256) void k_sort_samples(double* __restrict__ q,
+                                                      double* __restrict__ w,
+                                                      float* __restrict__ wf, int n,
+                                                      const int* __restrict__ active) {
+    const int p = blockIdx.x, tid = threadIdx.x;
+    if (n > kSortMaxN || (active && !active[p])) return;
+    __shared__ uint64_t key[kSortMaxN];
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    double* qp = q + (size_t)p * n * 4;
+    double* wp = w + (size_t)p * n;
+    float* fp = wf + (size_t)p * n;
+    for (int i = tid; i < n2; i += 256)
+        key[i] = i < n ? ((uint64_t)quat_morton_dev(qp + 4 * i) << 32) | (uint32_t)i : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = tid; i < n2; i += 256) {
+                const int x = i ^ j;
+                if (x > i) {
+                    const uint64_t a = key[i], b = key[x];
+                    if ((a > b) == ((i & k) == 0)) {
+                        key[i] = b;
+                        key[x] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    constexpr int S = kSortMaxN / 256;
+    double qv[S][4], wv[S];
+    float fv[S];
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int i = tid + 256 * s;
+        if (i < n) {
+            const int src = (int)(key[i] & 0xffffffffu);
+            for (int e = 0; e < 4; e++) qv[s][e] = qp[4 * src + e];
+            wv[s] = wp[src];
+            fv[s] = fp[src];
+        }
+    }
+    __syncthreads();   // every source read before any write
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+        const int i = tid + 256 * s;
+        if (i < n) {
+            for (int e = 0; e < 4; e++) qp[4 * i + e] = qv[s][e];
+            wp[i] = wv[s];
+            fp[i] = fv[s];
+        }
+    }
+}
+
 int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, double pfL,
                   double pfS, int maxPhase, uint64_t seed, int* phasesRun) {
     clear_error();
@@ -807,6 +881,7 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     THX_TRY(c->locBestT.ensure(sizeof(int) * P));
     THX_TRY(c->pfNRa.ensure(sizeof(int) * P));
     THX_TRY(c->pfNTa.ensure(sizeof(int) * P));
+    c->locValid = false;   // the loc* buffers now hold this search's phases
     // FP16: per-shell operand scales (particles and volumes stay fixed over the phases)
     if (precision == THX_FP16) THX_TRY(launch_local_shell_scale(c, used));
     THX_TRY(launch_prep_particles(c, precision == THX_FP16 ? THX_FP16 : THX_FP32));
@@ -853,6 +928,8 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
         hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
                            dAct, 1 | 4 | 8, kAcgTol, kAcgMaxIter, nullptr, nullptr,
                            c->pfK.as<double>(), c->pfWR.as<double>(), c->pfWRf.as<float>(), nullptr);
+        hipLaunchKernelGGL(k_sort_samples, dim3(P), dim3(256), 0, st, c->pfQ.as<double>(),
+                           c->pfWR.as<double>(), c->pfWRf.as<float>(), mLR, dAct);
         // shifts: perturbation, re-centring, balance + calVari
         hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, c->pfTr.as<double>(), mLT,
                            dAct, P, 1, c->pfS.as<double>(), pf, c->pfTransS.as<const double>(),

@@ -142,6 +142,9 @@ struct thx_ctx {
     // ragged local-search batch (inputs, partials, results)
     int locP = 0, locTPAD = 0, locNblk = 0;
     size_t locRtot = 0, locTtot = 0;
+    // rotation order of the last thx_ctx_score_local batch: scorer position ->
+    // caller's index (empty: identity), and the per-particle offsets
+    std::vector<int> locPerm, locROffH;
     thx::DevBuf locQuat, locTrans, locPRf, locPTf, locROff, locNR, locTOff, locNT, locBlocks,
         locBStart, locPcls, locVols, locBMax, locBWC, locBArg, locBWT, locWRrel;
     thx::DevBuf locWC, locWR, locWT, locBase, locBestR, locBestT;
@@ -208,6 +211,7 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
                        int mode = kLocalDirect);
 // particle order for the local scorer's work list (pose locality)
 std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P);
+uint32_t quat_morton(const double* q);
 // FP16 shell scales of the local scorer from the used class volumes + particles
 int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used);
 // frees the projectee FFT plans and scratch
