@@ -141,12 +141,31 @@ def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
     P, K = ctx.P, ctx.K
     poses = float(P) * mLR * mLT
     samples = float(P) * mLR * K
+    # the scorer alone on one phase's batch (the final samples), same launch shape
+    mode = ctx.local_mode()
+    ctx.set_local_options(mode, False)
+    ctx.score_local(st["quat"].reshape(-1, 4), np.full(P, mLR, np.int32),
+                    st["trans"].reshape(-1, 2), np.full(P, mLT, np.int32), precision=thx.BF16)
+    ctx.sync()
+    t_sc = ctx.last_timing()["score_local"] * 1e-3
+    ctx.set_local_options(0, False)
+    key = "k_score_local_q" if mode == 2 else "k_score_local"
+    traffic = traffic_bytes(name, key)
+    roof = None
+    if traffic:
+        roof = {"bound": "hbm", "achieved": traffic / t_sc / 1e9, "peak": 8000.0, "unit": "GB/s",
+                "frac": traffic / t_sc / 8e12, "traffic": traffic, "kernel": key,
+                "kernel_ms": 1e3 * t_sc,
+                "note": "fabric bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of "
+                        "the trilinear gathers; the algorithmic bytes are one volume read "
+                        "(67.6 MB), so this measures cache misses"}
     return {"phases": ran, "ms_per_phase": 1e3 * dt / max(ran, 1),
             "samples_per_particle": [mLR, mLT],
             "pose_evals_per_s": poses * ran / dt,
             "trilinear_samples_per_s": samples * ran / dt,
             "median_error_deg_vs_truth": float(np.degrees(np.median(err))),
-            "launch": ["", "direct", "persistent work queue"][max(0, ctx.local_mode())],
+            "launch": ["", "direct", "persistent work queue"][max(0, mode)],
+            "scorer_ms_per_phase": 1e3 * t_sc, "roofline": roof,
             "note": "raw pose-evals (each particle's own samples); projection fused into the "
                     "scorer, one trilinear slice per (particle, rotation)"}
 

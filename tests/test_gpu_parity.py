@@ -252,3 +252,20 @@ def test_box128_against_oracle(gpu_ctx):
         else:
             r_max, a_min, g_max = LOW_BARS[prec]
             assert rel.max() <= r_max and gap.max() <= g_max and agree >= a_min
+
+
+def test_synth_particles_deterministic(gpu_ctx):
+    """The same seed gives bit-identical particles (images, CTFs, noise
+    levels): the signal energy that sets the noise level is summed in a fixed
+    order, so training data, calibration sets and benchmarks repeat exactly."""
+    w = wl.WORKLOADS["box128"]
+    iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
+    gpu_ctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    gpu_ctx.synth_volume(0, w["box"], w["pf"], w["n_blob"], w["blob_sigma"], w["blob_radius"], 3)
+    runs = []
+    for _ in range(2):
+        gpu_ctx.synth_particles(0, 300, w["max_shift"], w["df_min"], w["df_max"], w["snr"], 11,
+                                **wl.CTF_DEFAULTS)
+        runs.append(gpu_ctx.get_particles())
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)

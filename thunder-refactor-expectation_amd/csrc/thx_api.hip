@@ -1298,14 +1298,16 @@ int thx_ctx_synth_particles(thx_ctx* c, int cls, int P, double maxShift, double 
     THX_TRY(alloc_particles(c, P));
     DevBuf dp, de;
     THX_TRY(dp.ensure(sizeof(double) * prm.size()));
-    THX_TRY(de.ensure(sizeof(double) * P));
+    const size_t nbx = (size_t)(c->K + 255) / 256;   // signal-energy partials per particle
+    THX_TRY(de.ensure(sizeof(double) * P * nbx));
     THX_HIP(hipMemcpy(dp.p, prm.data(), sizeof(double) * prm.size(), hipMemcpyHostToDevice));
-    THX_HIP(hipMemsetAsync(de.p, 0, sizeof(double) * P, c->stream));
     THX_TRY(launch_synth_signal(c, cls, P, dp.as<const double>(), de.as<double>(), voltage, Cs,
                                 ampContrast, pixelSize));
-    std::vector<double> en(P);
+    std::vector<double> part((size_t)P * nbx), en(P, 0.0);
     THX_HIP(hipStreamSynchronize(c->stream));
-    THX_HIP(hipMemcpy(en.data(), de.p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(part.data(), de.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost));
+    for (int p = 0; p < P; p++)
+        for (size_t b = 0; b < nbx; b++) en[p] += part[(size_t)p * nbx + b];
     dp.release();
     de.release();
     // real-space variance of the band-limited signal (Parseval over the

@@ -2201,7 +2201,8 @@ __global__ __launch_bounds__(256) void k_synth_signal(
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&energy[p], red[0]);
+    // one partial per block, summed in order on the host (deterministic)
+    if (threadIdx.x == 0) energy[(size_t)p * gridDim.x + blockIdx.x] = red[0];
 }
 
 int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams, double* dEnergy,

@@ -53,6 +53,10 @@ def simple_key(s):
         return f"k_score_{PREC[m.group(1)]}"
     if s.startswith("k_score_sel<"):
         return "k_score_sel"
+    if s.startswith("k_score_local_q<"):
+        return "k_score_local_q"
+    if s.startswith("k_score_local<"):
+        return "k_score_local"
     return None
 
 
