@@ -141,31 +141,42 @@ def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
     P, K = ctx.P, ctx.K
     poses = float(P) * mLR * mLT
     samples = float(P) * mLR * K
-    # the scorer alone on one phase's batch (the final samples), same launch shape
+    # the scorer alone, direct launch, on a fixed batch of its own size (256
+    # rotations about each particle's true pose, 4 degrees spread, 25 shifts):
+    # 2 work items per particle, a launch size no other local launch here has,
+    # so the profile's FETCH / WRITE figure for that grid is this launch's
     mode = ctx.local_mode()
-    ctx.set_local_options(mode, False)
-    ctx.score_local(st["quat"].reshape(-1, 4), np.full(P, mLR, np.int32),
-                    st["trans"].reshape(-1, 2), np.full(P, mLT, np.int32), precision=thx.BF16)
+    rng = np.random.default_rng(256)
+    nRr = 256
+    qr = np.repeat(info["true_quat"], nRr, axis=0) + rng.normal(0, 0.035, size=(P * nRr, 4))
+    qr /= np.linalg.norm(qr, axis=1, keepdims=True)
+    tr = np.repeat(info["true_shift"], mLT, axis=0) + rng.uniform(-2, 2, size=(P * mLT, 2))
+    ctx.set_local_options(1, False)
+    ctx.score_local(qr, np.full(P, nRr, np.int32), tr, np.full(P, mLT, np.int32),
+                    precision=thx.BF16)
     ctx.sync()
     t_sc = ctx.last_timing()["score_local"] * 1e-3
     ctx.set_local_options(0, False)
-    key = "k_score_local_q" if mode == 2 else "k_score_local"
-    traffic = traffic_bytes(name, key)
+    key = "k_score_local"
+    traffic = traffic_bytes(name, key, grid=P * (nRr // 128) * 256)
+    samples_r = float(P) * nRr * K
     roof = None
     if traffic:
         roof = {"bound": "hbm", "achieved": traffic / t_sc / 1e9, "peak": 8000.0, "unit": "GB/s",
                 "frac": traffic / t_sc / 8e12, "traffic": traffic, "kernel": key,
-                "kernel_ms": 1e3 * t_sc,
+                "kernel_ms": 1e3 * t_sc, "samples_per_s": samples_r / t_sc,
                 "note": "fabric bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of "
-                        "the trilinear gathers; the algorithmic bytes are one volume read "
-                        "(67.6 MB), so this measures cache misses"}
+                        "the trilinear gathers of the BF16 volume copy; the algorithmic bytes "
+                        "are one volume read (34 MB), so this measures L2 misses"}
     return {"phases": ran, "ms_per_phase": 1e3 * dt / max(ran, 1),
             "samples_per_particle": [mLR, mLT],
             "pose_evals_per_s": poses * ran / dt,
             "trilinear_samples_per_s": samples * ran / dt,
             "median_error_deg_vs_truth": float(np.degrees(np.median(err))),
             "launch": ["", "direct", "persistent work queue"][max(0, mode)],
-            "scorer_ms_per_phase": 1e3 * t_sc, "roofline": roof,
+            "roofline_batch": f"{P} particles x {nRr} rotations (4 deg about the true pose) "
+                              f"x {mLT} shifts, BF16, direct launch",
+            "roofline": roof,
             "note": "raw pose-evals (each particle's own samples); projection fused into the "
                     "scorer, one trilinear slice per (particle, rotation)"}
 
@@ -276,14 +287,17 @@ def cpu_baseline(ctx, info, name, P_s=384, R_s=1000):
                        f"FP32 reference loop order, {dt:.1f} s")
 
 
-def traffic_bytes(name, key):
+def traffic_bytes(name, key, grid=None):
     """HBM bytes per launch of a kernel (key as in the committed summary, e.g.
     "k_score_sel") from the rocprofv3 FETCH_SIZE + WRITE_SIZE passes over this
-    workload, scaled by the copy-kernel calibration; None without a summary."""
+    workload, scaled by the copy-kernel calibration; with `grid` (threads of
+    the launch), that launch size's figure.  None without a summary."""
     path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if not os.path.exists(path):
         return None
     v = json.load(open(path))["kernels"].get(key)
+    if v is not None and grid is not None:
+        v = v.get("by_grid", {}).get(str(grid))
     return None if v is None else (v["read_GB"] + v["write_GB"]) * 1e9
 
 

@@ -231,6 +231,29 @@ static int default_ctx(thx_ctx** out) {
     return THX_OK;
 }
 
+// BF16 copies of the class volumes for the 16-bit local scorers: rebuilt on
+// their next use after the volume changes.
+void invalidate_vol_lo(thx_ctx* c, int cls) {
+    if ((int)c->volLoOk.size() <= cls) c->volLoOk.resize(cls + 1, 0);
+    c->volLoOk[cls] = 0;
+}
+
+int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used) {
+    const int nCls = (int)c->vol.size();
+    if ((int)c->volLo.size() < nCls) c->volLo.resize(nCls);
+    if ((int)c->volLoOk.size() < nCls) c->volLoOk.resize(nCls, 0);
+    for (int k = 0; k < nCls; k++) {
+        if (!used[k] || c->volLoOk[k]) continue;
+        const int V = c->vdim[k];
+        const size_t n = (size_t)(V / 2 + 1) * V * V;
+        THX_TRY(c->volLo[k].ensure(sizeof(uint32_t) * n));
+        THX_TRY(launch_vol_to_bf16(c->stream, c->vol[k].as<const float2>(), n,
+                                   c->volLo[k].as<uint32_t>()));
+        c->volLoOk[k] = 1;
+    }
+    return THX_OK;
+}
+
 }  // namespace thx
 
 using namespace thx;
@@ -303,6 +326,8 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->locCounter};
     for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();
+    for (auto& v : c->volLo) v.release();
+    c->locVolsLo.release();
     for (auto& r : c->res) {
         r.wC.release(); r.wR.release(); r.wT.release(); r.base.release();
         r.bestR.release(); r.bestT.release(); r.bestScore.release();
@@ -365,6 +390,7 @@ int thx_ctx_set_volume(thx_ctx* c, int cls, const float* vol, int vdim) {
     THX_TRY(c->vol[cls].ensure(sizeof(float2) * n));
     THX_HIP(hipMemcpy(c->vol[cls].p, vol, sizeof(float2) * n, hipMemcpyHostToDevice));
     c->vdim[cls] = vdim;
+    invalidate_vol_lo(c, cls);
     return THX_OK;
 }
 
@@ -681,6 +707,13 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     THX_TRY(upload(c->locBlocks, blocks.data(), blocks.size(), st));
     THX_TRY(upload(c->locBStart, bStart.data(), P + 1, st));
     THX_TRY(upload(c->locVols, vp.data(), vp.size(), st));
+    if (precision != THX_FP32) {
+        THX_TRY(ensure_vol_lo(c, used));
+        std::vector<const uint32_t*> vl(nCls, nullptr);
+        for (int k = 0; k < nCls; k++)
+            if (used[k]) vl[k] = c->volLo[k].as<const uint32_t>();
+        THX_TRY(upload(c->locVolsLo, vl.data(), vl.size(), st));
+    }
     if (cls) THX_TRY(upload(c->locPcls, cls, P, st));
     THX_TRY(c->locBMax.ensure(sizeof(float) * nblk));
     THX_TRY(c->locBWC.ensure(sizeof(float) * nblk));
@@ -700,6 +733,7 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     c->locTtot = Ttot;
     LocalBatch b;
     b.vols = c->locVols.as<const float2* const>();
+    b.volsLo = precision != THX_FP32 ? c->locVolsLo.as<const uint32_t* const>() : nullptr;
     b.pcls = cls ? c->locPcls.as<const int>() : nullptr;
     b.quat = c->locQuat.as<const double>();
     b.pRf = c->locPRf.as<const float>();
@@ -739,6 +773,7 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
             {c->Y.p, c->Wt.p, c->Aterm.p, c->dat.p, c->ctf.p, c->sig.p, c->iCol.p, c->iRow.p,
              c->iSig.p, c->locQuat.p, c->locTrans.p, c->locPRf.p, c->locPTf.p, c->locROff.p,
              c->locNR.p, c->locTOff.p, c->locNT.p, c->locBlocks.p, c->locBStart.p, c->locVols.p,
+             precision != THX_FP32 ? c->locVolsLo.p : nullptr,
              cls ? c->locPcls.p : nullptr, c->locBMax.p, c->locBWC.p, c->locBArg.p, c->locBWT.p,
              c->locWRrel.p, c->locWC.p, c->locWR.p, c->locWT.p, c->locBase.p, c->locBestR.p,
              c->locBestT.p, c->locCounter.p});
@@ -1250,6 +1285,7 @@ int thx_ctx_synth_volume(thx_ctx* c, int cls, int idim, int pf, int nBlob, doubl
     }
     THX_TRY(c->vol[cls].ensure(sizeof(float2) * (size_t)(V / 2 + 1) * V * V));
     c->vdim[cls] = V;
+    invalidate_vol_lo(c, cls);
     DevBuf dc;
     THX_TRY(dc.ensure(sizeof(double) * cen.size()));
     THX_HIP(hipMemcpy(dc.p, cen.data(), sizeof(double) * cen.size(), hipMemcpyHostToDevice));

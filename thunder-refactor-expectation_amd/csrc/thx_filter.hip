@@ -865,6 +865,13 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     std::vector<const float2*> vp(nCls);
     for (int k = 0; k < nCls; k++) vp[k] = c->vol[k].as<const float2>();
     THX_TRY(upload(c->locVols, vp.data(), vp.size(), st));
+    if (precision != THX_FP32) {
+        THX_TRY(ensure_vol_lo(c, used));
+        std::vector<const uint32_t*> vl(nCls, nullptr);
+        for (int k = 0; k < nCls; k++)
+            if (used[k]) vl[k] = c->volLo[k].as<const uint32_t>();
+        THX_TRY(upload(c->locVolsLo, vl.data(), vl.size(), st));
+    }
     if (cls) THX_TRY(upload(c->locPcls, cls, P, st));
     const int TPAD = mLT <= 32 ? 32 : (mLT <= 64 ? 64 : 128);
     const int nbp = (mLR + kLocalRotBlock - 1) / kLocalRotBlock;
@@ -941,6 +948,7 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
         THX_TRY(upload(c->locBStart, bStart.data(), P + 1, st));
         LocalBatch b;
         b.vols = c->locVols.as<const float2* const>();
+        b.volsLo = precision != THX_FP32 ? c->locVolsLo.as<const uint32_t* const>() : nullptr;
         b.pcls = cls ? c->locPcls.as<const int>() : nullptr;
         b.quat = c->pfQ.as<const double>();
         b.pRf = c->pfWRf.as<const float>();

@@ -78,6 +78,7 @@ struct ClassResult {
 // Device pointers of one ragged local-search batch.
 struct LocalBatch {
     const float2* const* vols = nullptr;   // per-class volume base pointers
+    const uint32_t* const* volsLo = nullptr;   // per-class BF16 copies (16-bit scorers)
     const int* pcls = nullptr;             // class per particle (or null: class 0)
     const double* quat = nullptr;          // [Rtot][4]
     const float* pRf = nullptr;            // [Rtot]
@@ -109,9 +110,13 @@ struct thx_ctx {
     // pixel set
     int K = 0, Kp = 0, idim = 0, pf = 2, maxShell = 0;
     thx::DevBuf iCol, iRow, iSig;
-    // volumes per class
+    // volumes per class, and their BF16 copies for the 16-bit local scorers
+    // (built on first use after a volume changes)
     std::vector<thx::DevBuf> vol;
     std::vector<int> vdim;
+    std::vector<thx::DevBuf> volLo;
+    std::vector<char> volLoOk;
+    thx::DevBuf locVolsLo;
     // projectee preparation: FFT plans (hipfftHandle) for one (N, pf), scratch
     void* prjPlanBw = nullptr;
     void* prjPlanFw = nullptr;
@@ -214,6 +219,10 @@ std::vector<int> pose_order(const double* quat, const int* off, const int* cls, 
 uint32_t quat_morton(const double* q);
 // FP16 shell scales of the local scorer from the used class volumes + particles
 int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used);
+// BF16 copies of the used class volumes (built where missing or stale)
+int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used);
+void invalidate_vol_lo(thx_ctx* c, int cls);
+int launch_vol_to_bf16(hipStream_t st, const float2* vol, size_t n, uint32_t* lo);
 // frees the projectee FFT plans and scratch
 void projectee_release(thx_ctx* c);
 int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);

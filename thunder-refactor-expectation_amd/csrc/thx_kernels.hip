@@ -87,6 +87,56 @@ __device__ inline float2 sample_trilinear(const float2* __restrict__ vol, int V,
     return conj ? make_float2(re, -im) : make_float2(re, im);
 }
 
+// The same interpolation from the BF16 copy of the volume (one 4-byte word
+// per voxel: re in the low half, im in the high half); corners are widened to
+// FP32 and weighted in FP32.
+__device__ inline float2 sample_trilinear_lo(const uint32_t* __restrict__ vol, int V, float x,
+                                             float y, float z) {
+    bool conj = false;
+    if (x < 0) { x = -x; y = -y; z = -z; conj = true; }
+    const int x0 = (int)floorf(x), y0 = (int)floorf(y), z0 = (int)floorf(z);
+    const float xd = x - x0, yd = y - y0, zd = z - z0;
+    const float v0[2] = {1 - xd, xd}, v1[2] = {1 - yd, yd}, v2[2] = {1 - zd, zd};
+    const int nColFT = V / 2 + 1;
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        int zz = z0 + k;
+        zz = zz < 0 ? zz + V : zz;
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int yy = y0 + j;
+            yy = yy < 0 ? yy + V : yy;
+            const uint32_t* row = vol + ((size_t)zz * V + yy) * nColFT;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const uint32_t d = row[x0 + i];
+                const float w = v0[i] * v1[j] * v2[k];
+                re += w * __uint_as_float(d << 16);
+                im += w * __uint_as_float(d & 0xffff0000u);
+            }
+        }
+    }
+    return conj ? make_float2(re, -im) : make_float2(re, im);
+}
+
+// lo[i] = (bf16(vol[i].x), bf16(vol[i].y)), round to nearest even
+__global__ void k_vol_to_bf16(const float2* __restrict__ vol, size_t n, uint32_t* __restrict__ lo) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 v = vol[i];
+    const __bf16 a = (__bf16)v.x, b = (__bf16)v.y;
+    lo[i] = (uint32_t)__builtin_bit_cast(unsigned short, a) |
+            ((uint32_t)__builtin_bit_cast(unsigned short, b) << 16);
+}
+
+int launch_vol_to_bf16(hipStream_t st, const float2* vol, size_t n, uint32_t* lo) {
+    hipLaunchKernelGGL(k_vol_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vol, n,
+                       lo);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
 // CTF(RFLOAT* dst, ...) closed form, src/CTF.cpp:109-145, in float.
 __device__ inline float ctf_value(int ic, int ir, float pixelSize, float lambda,
                                   float dU, float dV, float theta, float Cs,
@@ -1292,7 +1342,8 @@ constexpr int kNBL = 128;
 
 template <int PREC, int TPAD>
 __device__ __forceinline__ void score_local_item(int item, 
-    const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
+    const float2* const* __restrict__ vols, const uint32_t* const* __restrict__ volsLo,
+    const int* __restrict__ pcls, int V, int pf,
     const int* __restrict__ iCol, const int* __restrict__ iRow, const int* __restrict__ iSig,
     const float* __restrict__ shellScale, int K, int Kp, int N,
     const float2* __restrict__ Y, const float* __restrict__ Wt,
@@ -1320,7 +1371,9 @@ __device__ __forceinline__ void score_local_item(int item,
     const int Rp = nR[p], Tp = nT[p];
     const int nr = min(kNBL, Rp - r0);
     const int rbase = rOff[p] + r0, tbase = tOff[p];
-    const float2* vol = vols[pcls ? pcls[p] : 0];
+    const int cl = pcls ? pcls[p] : 0;
+    const float2* vol = vols[cl];
+    const uint32_t* volLo = F32 ? nullptr : volsLo[cl];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, hl = lane >> 5;
 
@@ -1390,7 +1443,8 @@ __device__ __forceinline__ void score_local_item(int item,
                     const float x = (float)(m[0] * c0 + m[1] * c1);
                     const float yy = (float)(m[2] * c0 + m[3] * c1);
                     const float z = (float)(m[4] * c0 + m[5] * c1);
-                    v = sample_trilinear(vol, V, x, yy, z);
+                    if constexpr (F32) v = sample_trilinear(vol, V, x, yy, z);
+                    else v = sample_trilinear_lo(volLo, V, x, yy, z);
                 }
                 cacc[q] += wk * (v.x * v.x + v.y * v.y);
                 if constexpr (F32) {
@@ -1547,7 +1601,8 @@ __device__ __forceinline__ void score_local_item(int item,
 // Direct launch: one block per work item (particle, 128 rotations).
 template <int PREC, int TPAD>
 __global__ __launch_bounds__(256, 2) void k_score_local(
-    const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
+    const float2* const* __restrict__ vols, const uint32_t* const* __restrict__ volsLo,
+    const int* __restrict__ pcls, int V, int pf,
     const int* __restrict__ iCol, const int* __restrict__ iRow, const int* __restrict__ iSig,
     const float* __restrict__ shellScale, int K, int Kp, int N,
     const float2* __restrict__ Y, const float* __restrict__ Wt,
@@ -1559,7 +1614,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
     float* __restrict__ bWT, float* __restrict__ wRrel) {
     // the list is in pose order: each XCD walks a contiguous range of it, so
     // concurrent blocks of one XCD gather from overlapping volume regions
-    score_local_item<PREC, TPAD>(xcd_logical(blockIdx.x, gridDim.x), vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+    score_local_item<PREC, TPAD>(xcd_logical(blockIdx.x, gridDim.x), vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
 }
 
 // Persistent launch: a grid sized to the resident capacity pulls work items
@@ -1569,7 +1624,8 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
 // drains its range moves on to the next ones.
 template <int PREC, int TPAD>
 __global__ __launch_bounds__(256, 2) void k_score_local_q(
-    const float2* const* __restrict__ vols, const int* __restrict__ pcls, int V, int pf,
+    const float2* const* __restrict__ vols, const uint32_t* const* __restrict__ volsLo,
+    const int* __restrict__ pcls, int V, int pf,
     const int* __restrict__ iCol, const int* __restrict__ iRow, const int* __restrict__ iSig,
     const float* __restrict__ shellScale, int K, int Kp, int N,
     const float2* __restrict__ Y, const float* __restrict__ Wt,
@@ -1595,7 +1651,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local_q(
             continue;
         }
         const int it = lo + j;
-        score_local_item<PREC, TPAD>(it, vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+        score_local_item<PREC, TPAD>(it, vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
     }
 }
 
@@ -1606,7 +1662,8 @@ __global__ void k_zero_int(int* __restrict__ counter) {
 
 template <int PREC>
 static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int grid, int* counter,
-                              const float2* const* vols, const int* pcls, int V, int pf,
+                              const float2* const* vols, const uint32_t* const* volsLo,
+                              const int* pcls, int V, int pf,
                               const int* iCol, const int* iRow, const int* iSig,
                               const float* shellScale, int K, int Kp, int N,
                               const float2* Y, const float* Wt, const double* Aterm,
@@ -1614,7 +1671,7 @@ static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int 
                               const int* nR, const double* trans, const float* pTf,
                               const int* tOff, const int* nT, const int2* blocks, float* bMax,
                               float* bWC, int* bArg, float* bWT, float* wRrel) {
-#define THX_LOCAL_ARGS vols, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, \
+#define THX_LOCAL_ARGS vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, \
                        quat, pRf, rOff, \
                        nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel
     if (mode == kLocalQueue) {
@@ -1665,6 +1722,8 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
                        int mode) {
     THX_CHECK(precision == THX_FP32 || precision == THX_BF16 || precision == THX_FP16,
               THX_ERR_ARG, "local scorer supports FP32, BF16 and FP16");
+    THX_CHECK(precision == THX_FP32 || b.volsLo, THX_ERR_STATE,
+              "local scorer: the 16-bit forms need the BF16 volume copies");
     int grid = 0;
     if (mode == kLocalQueue) {
         THX_TRY(c->locCounter.ensure(8 * sizeof(int)));
@@ -1672,7 +1731,7 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
     }
 #define THX_LOCAL_CALL(PR)                                                                      \
     launch_local_prec<PR>(c->stream, TPAD, nblk, mode, grid, c->locCounter.as<int>(), b.vols,  \
-                          b.pcls, c->idim * c->pf, c->pf, c->iCol.as<const int>(),             \
+                          b.volsLo, b.pcls, c->idim * c->pf, c->pf, c->iCol.as<const int>(),             \
                           c->iRow.as<const int>(), c->iSig.as<const int>(),                    \
                           PR == THX_FP16 ? c->shellScale.as<const float>() : nullptr, c->K,    \
                           c->Kp, c->idim,                                                      \

@@ -114,6 +114,7 @@ int thx_ctx_set_projectee(thx_ctx* c, int cls, const float* volFT, int N, int pf
     THX_TRY(c->prjPad.ensure(sizeof(float) * nPad));
     THX_TRY(c->vol[cls].ensure(sizeof(float2) * nOut));
     c->vdim[cls] = V;
+    invalidate_vol_lo(c, cls);
     THX_HIP(hipMemcpyAsync(c->prjIn.p, volFT, sizeof(float2) * nIn, hipMemcpyHostToDevice, st));
     THX_HIP(hipEventRecord(c->ev[0], st));
     // the inverse transform may overwrite its input: prjIn is scratch

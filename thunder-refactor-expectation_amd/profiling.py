@@ -108,12 +108,22 @@ def summarize(a):
           f"WRITE_SIZE by {ws:.3f}.  Per kernel: the dispatches of its largest grid, averaged.", "",
           "| Kernel | grid | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB | write GB |",
           "|---|---|---|---|---|---|"]
+    # every grid size of every kernel, for lookups of one specific launch
+    by_grid = {}
+    for (k, g), vs in fc.items():
+        if (k, g) not in wc:
+            continue
+        e = {"read_GB": sum(vs) / len(vs) * 1024 * fs / 1e9,
+             "write_GB": sum(wc[(k, g)]) / len(wc[(k, g)]) * 1024 * ws / 1e9}
+        for name in (k, simple_key(k)):
+            if name:
+                by_grid.setdefault(name, {})[str(g)] = e
     for k, (g, f, w) in sorted(best.items(), key=lambda x: -(x[1][1] * fs + x[1][2] * ws)):
         e = {"grid": g, "read_GB": f * 1024 * fs / 1e9, "write_GB": w * 1024 * ws / 1e9}
-        kern[k] = e
+        kern[k] = dict(e, by_grid=by_grid.get(k, {}))
         sk = simple_key(k)
         if sk and (sk not in kern or kern[sk]["grid"] < g):
-            kern[sk] = e
+            kern[sk] = dict(e, by_grid=by_grid.get(sk, {}))
         md.append(f"| `{k}` | {g} | {f:.0f} | {w:.0f} | {e['read_GB']:.3f} | {e['write_GB']:.3f} |")
     json.dump({"workload": a.name, "fetch_scale": fs, "write_scale": ws, "kernels": kern},
               open(a.out + "_traffic.json", "w"), indent=1)
