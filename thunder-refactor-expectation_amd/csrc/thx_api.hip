@@ -303,6 +303,8 @@ int thx_ctx_destroy(thx_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->locGraph) (void)hipGraphExecDestroy(c->locGraph);
+    if (c->lpReady) (void)hipEventDestroy(c->lpReady);
+    if (c->lpDone) (void)hipEventDestroy(c->lpDone);
     projectee_release(c);
     DevBuf* bufs[] = {&c->iCol, &c->iRow, &c->iSig, &c->dat, &c->ctf, &c->sig, &c->Aterm,
                       &c->Y, &c->Wt, &c->quat, &c->trans, &c->pR, &c->pT, &c->pRf, &c->pTf,
@@ -975,18 +977,24 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     hipStream_t st = c->stream;
     c->nev = 0;
     c->evNames.clear();
+    THX_TRY(c->lpMix.ensure(sizeof(double) * (size_t)P * (6 * C + 3)));
+    THX_TRY(c->rotLo.ensure(sizeof(uint32_t) * (size_t)c->Rpad * c->Kp));
+    if (!c->lpReady) THX_HIP(hipEventCreateWithFlags(&c->lpReady, hipEventDisableTiming));
+    if (!c->lpDone) THX_HIP(hipEventCreateWithFlags(&c->lpDone, hipEventDisableTiming));
     THX_TRY(record(c, "start"));
+    // the proposal network (images, layers, head) on its own stream, while the
+    // context stream projects the grid and prepares the particle operands
+    THX_HIP(hipEventRecord(c->lpReady, st));
+    THX_TRY(cnn_propose_async(net, c, c->lpMix.as<double>(), c->lpReady, c->lpDone));
     // slices of the grid, once for all particles; BF16 table + FP32 |P|^2
     THX_TRY(launch_project(c, cls, c->quat.as<const double>(), c->R, c->Rpad,
                            c->rotP.as<float2>(), c->normP.as<float>()));
-    THX_TRY(c->rotLo.ensure(sizeof(uint32_t) * (size_t)c->Rpad * c->Kp));
     THX_TRY(launch_convert_lo(c, THX_BF16));
     THX_TRY(record(c, "project"));
     THX_TRY(launch_prep_particles(c, THX_FP32));
     THX_TRY(record(c, "prep"));
-    // proposal network (synchronous on its own stream)
-    THX_TRY(c->lpMix.ensure(sizeof(double) * (size_t)P * (6 * C + 3)));
-    THX_TRY(thx_cnn_propose_ctx(net, c, nullptr, c->lpMix.as<double>()));
+    // the rest of the network's time (what the projection did not cover)
+    THX_HIP(hipStreamWaitEvent(st, c->lpDone, 0));
     THX_TRY(record(c, "propose"));
     THX_TRY(c->lpRidx.ensure(sizeof(int) * (size_t)P * nSelR));
     THX_TRY(c->lpRlogq.ensure(sizeof(float) * (size_t)P * nSelR));

@@ -642,6 +642,41 @@ static int cnn_backward(thx_cnn* n, int B, const float* in = nullptr) {
     return THX_OK;
 }
 
+namespace thx {
+
+// Proposal parameters of every particle into mixDev, enqueued on the
+// network's stream (images included) with no host synchronisation: the
+// stream first waits for `ready` (the context stream's work so far), and
+// `done` is recorded after the last copy for the context stream to wait on.
+int cnn_propose_async(thx_cnn* n, thx_ctx* c, double* mixDev, hipEvent_t ready,
+                      hipEvent_t done) {
+    THX_CHECK(n && c && mixDev && c->P > 0 && c->K > 0, THX_ERR_STATE,
+              "propose: particles and output needed");
+    THX_CHECK(n->device == c->device, THX_ERR_ARG,
+              "propose: the network and the context must be on the same device");
+    THX_TRY(c->img.ensure(sizeof(float) * 4096 * (size_t)c->P));
+    const int BI = std::min(kInferBatch, c->P);
+    THX_TRY(ensure_infer(n, BI));
+    THX_HIP(hipStreamWaitEvent(n->stream, ready, 0));
+    hipLaunchKernelGGL(k_images, dim3(c->P), dim3(256), 0, n->stream, c->dat.as<const float2>(),
+                       c->ctf.as<const float>(), c->iCol.as<const int>(), c->iRow.as<const int>(),
+                       c->K, c->P, c->img.as<float>());
+    THX_HIP(hipGetLastError());
+    for (int b0 = 0; b0 < c->P; b0 += BI) {
+        const int B = std::min(BI, c->P - b0);
+        THX_TRY(cnn_forward(n, B, c->img.as<float>() + (size_t)b0 * 4096));
+        hipLaunchKernelGGL(k_head, dim3(nblk(B, 64)), dim3(64), 0, n->stream,
+                           n->act[6].as<const float>(), B, n->C, n->mix.as<double>());
+        THX_HIP(hipGetLastError());
+        THX_HIP(hipMemcpyAsync(mixDev + (size_t)b0 * n->NOUT, n->mix.p,
+                               sizeof(double) * B * n->NOUT, hipMemcpyDeviceToDevice, n->stream));
+    }
+    THX_HIP(hipEventRecord(done, n->stream));
+    return THX_OK;
+}
+
+}  // namespace thx
+
 extern "C" {
 
 int thx_cnn_create(int device, int components, uint64_t seed, thx_cnn** out) {

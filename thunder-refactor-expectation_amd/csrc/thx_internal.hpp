@@ -117,6 +117,8 @@ struct thx_ctx {
     std::vector<thx::DevBuf> volLo;
     std::vector<char> volLoOk;
     thx::DevBuf locVolsLo;
+    // ordering events between the context stream and the proposal network's
+    hipEvent_t lpReady = nullptr, lpDone = nullptr;
     // projectee preparation: FFT plans (hipfftHandle) for one (N, pf), scratch
     void* prjPlanBw = nullptr;
     void* prjPlanFw = nullptr;
@@ -217,6 +219,9 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
 // particle order for the local scorer's work list (pose locality)
 std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P);
 uint32_t quat_morton(const double* q);
+// proposal network on its own stream after `ready`, `done` recorded at the end
+int cnn_propose_async(thx_cnn* n, thx_ctx* c, double* mixDev, hipEvent_t ready,
+                      hipEvent_t done);
 // FP16 shell scales of the local scorer from the used class volumes + particles
 int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used);
 // BF16 copies of the used class volumes (built where missing or stale)
