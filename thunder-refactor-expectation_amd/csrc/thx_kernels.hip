@@ -1850,8 +1850,9 @@ __global__ __launch_bounds__(256) void k_gather_particles(
     const float2* __restrict__ dat, const float* __restrict__ ctf, const float* __restrict__ sig,
     const int* __restrict__ idx, int K, float2* __restrict__ dat2, float* __restrict__ ctf2,
     float* __restrict__ sig2) {
+    // blockIdx.y splits the row so a few hundred rows still fill the machine
     const size_t src = (size_t)idx[blockIdx.x] * K, dst = (size_t)blockIdx.x * K;
-    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    for (int k = blockIdx.y * blockDim.x + threadIdx.x; k < K; k += gridDim.y * blockDim.x) {
         dat2[dst + k] = dat[src + k];
         ctf2[dst + k] = ctf[src + k];
         sig2[dst + k] = sig[src + k];
@@ -1861,7 +1862,7 @@ __global__ __launch_bounds__(256) void k_gather_particles(
 int launch_gather_particles(thx_ctx* c, const int* idx, int n, float2* dat2, float* ctf2,
                             float* sig2) {
     if (n <= 0) return THX_OK;
-    hipLaunchKernelGGL(k_gather_particles, dim3(n), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_gather_particles, dim3(n, 8), dim3(256), 0, c->stream,
                        c->dat.as<const float2>(), c->ctf.as<const float>(),
                        c->sig.as<const float>(), idx, c->K, dat2, ctf2, sig2);
     THX_HIP(hipGetLastError());
