@@ -29,7 +29,7 @@ namespace thx {
 
 constexpr int kImg = 64;
 constexpr int kMaxBatch = 512;     // training minibatch limit
-constexpr int kInferBatch = 4096;  // images per inference pass
+constexpr int kInferBatch = 16384;  // images per inference pass
 
 struct ConvShape {
     int ci, hi, co, ho, ks, st, pd;
