@@ -318,7 +318,7 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->locBWT, &c->locWRrel, &c->locWC, &c->locWR, &c->locWT, &c->locBase,
                        &c->locBestR, &c->locBestT, &c->img, &c->lpMix, &c->lpRidx,
                        &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpSpread, &c->lpTail,
-                       &c->lpPmap, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpRes.wC,
+                       &c->lpPmap, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpQf, &c->lpRes.wC,
                        &c->lpRes.wR, &c->lpRes.wT, &c->lpRes.base, &c->lpRes.bestR,
                        &c->lpRes.bestT, &c->lpRes.bestScore, &c->pfQ, &c->pfQ2, &c->pfTr,
                        &c->pfTr2, &c->pfWR, &c->pfWT, &c->pfWRf, &c->pfWTf, &c->pfIdx,
@@ -1000,9 +1000,11 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     THX_TRY(c->lpRlogq.ensure(sizeof(float) * (size_t)P * nSelR));
     THX_TRY(c->lpTidx.ensure(sizeof(int) * (size_t)P * nSelT));
     THX_TRY(c->lpSpread.ensure(sizeof(float) * P));
+    THX_TRY(c->lpQf.ensure(sizeof(float4) * c->R));
     THX_TRY(pf_select_grid(st, c->lpMix.as<double>(), C, P, c->quat.as<double>(), c->R,
                            c->trans.as<double>(), c->T, nSelR, nSelT, c->lpRidx.as<int>(),
-                           c->lpRlogq.as<float>(), c->lpTidx.as<int>(), c->lpSpread.as<float>()));
+                           c->lpRlogq.as<float>(), c->lpTidx.as<int>(), c->lpSpread.as<float>(),
+                           c->lpQf.as<float4>()));
     THX_TRY(record(c, "select"));
     THX_TRY(score_selected(c, P, c->lpRidx.as<int>(), c->lpTidx.as<int>(), nSelR, nSelT, nullptr));
     THX_TRY(c->lpESS.ensure(sizeof(double) * P));

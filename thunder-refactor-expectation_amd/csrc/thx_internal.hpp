@@ -117,6 +117,7 @@ struct thx_ctx {
     std::vector<thx::DevBuf> volLo;
     std::vector<char> volLoOk;
     thx::DevBuf locVolsLo;
+    thx::DevBuf lpQf;        // grid quaternions as float4 for the selection
     // ordering events between the context stream and the proposal network's
     hipEvent_t lpReady = nullptr, lpDone = nullptr;
     // projectee preparation: FFT plans (hipfftHandle) for one (N, pf), scratch
@@ -253,7 +254,8 @@ int pf_rot_mean(hipStream_t st, const double* quat, const double* w, const int* 
 constexpr int kResampleMaxN = 8192;
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
                    const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
-                   int* tidx, float* spread);
+                   int* tidx, float* spread,
+                   float4* gridQf);
 int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, const int* tidx,
                      int nSelR, int nSelT, float pRu, float pTu, const int* pmap,
                      const LocalBatch& b);

@@ -348,8 +348,16 @@ __device__ void block_topk(const float* v, int n, int nSel, int* out, float* out
 
 constexpr int kSelMaxR = 16384;
 
+// grid quaternions in FP32, one 16-byte load each in the selection kernel
+__global__ void k_quat_to_f4(const double* __restrict__ q, int R, float4* __restrict__ qf) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R)
+        qf[r] = make_float4((float)q[4 * r], (float)q[4 * r + 1], (float)q[4 * r + 2],
+                            (float)q[4 * r + 3]);
+}
+
 __global__ __launch_bounds__(256) void k_select_grid(
-    const double* __restrict__ mix, int C, const double* __restrict__ gridQ, int R,
+    const double* __restrict__ mix, int C, const float4* __restrict__ gridQ, int R,
     const double* __restrict__ gridT, int T, int nSelR, int nSelT, int* __restrict__ ridx,
     float* __restrict__ rlogq, int* __restrict__ tidx, float* __restrict__ spread) {
     const int p = blockIdx.x;
@@ -372,13 +380,13 @@ __global__ __launch_bounds__(256) void k_select_grid(
     __syncthreads();
     float mx = -INFINITY, mn = INFINITY;
     for (int r = threadIdx.x; r < R; r += blockDim.x) {
-        const double* q = gridQ + 4 * (size_t)r;
-        const float q0 = (float)q[0], q1 = (float)q[1], q2 = (float)q[2], q3 = (float)q[3];
+        const float4 q = gridQ[r];
         float dens = 0.f;
         for (int c = 0; c < C; c++) {
-            const float d = cm[c][0] * q0 + cm[c][1] * q1 + cm[c][2] * q2 + cm[c][3] * q3;
+            const float d = cm[c][0] * q.x + cm[c][1] * q.y + cm[c][2] * q.z + cm[c][3] * q.w;
             const float sv = ca[c] * d * d + cb[c];
-            dens += ccoef[c] / (sv * sv);
+            // hardware reciprocal (1 ulp): the densities only rank and compare
+            dens += ccoef[c] * __builtin_amdgcn_rcpf(sv * sv);
         }
         const float lv = __logf(fmaxf(dens, 1e-37f));
         v[r] = lv;
@@ -450,11 +458,12 @@ __global__ __launch_bounds__(256) void k_select_grid(
 
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
                    const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
-                   int* tidx, float* spread) {
+                   int* tidx, float* spread, float4* gridQf) {
     THX_CHECK(C >= 1 && C <= 16 && R <= kSelMaxR && T <= 512 && nSelR <= R && nSelT <= T,
               THX_ERR_RANGE, "select_grid: sizes out of range");
-    hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), 0, st, mix, C, gridQ, R, gridT, T, nSelR,
-                       nSelT, ridx, rlogq, tidx, spread);
+    hipLaunchKernelGGL(k_quat_to_f4, dim3((R + 255) / 256), dim3(256), 0, st, gridQ, R, gridQf);
+    hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), 0, st, mix, C, gridQf, R, gridT, T,
+                       nSelR, nSelT, ridx, rlogq, tidx, spread);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
