@@ -687,10 +687,17 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
         const int nth = Rtot < (1u << 20) ? 1
                         : (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::vector<std::thread> pool;
-        for (int i = 1; i < nth; i++)
-            pool.emplace_back(sort_range, (int)((long long)P * i / nth),
-                              (int)((long long)P * (i + 1) / nth));
+        int started = 1;   // ranges handed to threads: 1 .. started-1
+        try {
+            for (; started < nth; started++)
+                pool.emplace_back(sort_range, (int)((long long)P * started / nth),
+                                  (int)((long long)P * (started + 1) / nth));
+        } catch (...) {
+            // no more threads: the remaining ranges run here
+        }
         sort_range(0, (int)((long long)P / nth));
+        for (int i = started; i < nth; i++)
+            sort_range((int)((long long)P * i / nth), (int)((long long)P * (i + 1) / nth));
         for (auto& t : pool) t.join();
     }
     for (size_t i = 0; i < Ttot; i++) fT[i] = (float)pT[i];
