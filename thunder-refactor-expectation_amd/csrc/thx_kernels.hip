@@ -831,8 +831,9 @@ __global__ __launch_bounds__(256, 2) void k_score(
 
 // ---------------------------------------------------------------------------
 // 16-bit scoring kernel (BF16 / FP16 operands, FP32 accumulation).
-// Block tile: 128 rows (PB = 128 / TC particles x TC translations) x 256
-// rotations; 4 waves of 64 x 128 (2 x 4 tiles of v_mfma_f32_32x32x16).  The
+// Block tile: MROWS rows (PB = MROWS / TC particles x TC translations) x 256
+// rotations; 8 waves, each 64 rows x 256 / (8 / (MROWS / 64)) columns of
+// v_mfma_f32_32x32x16 tiles (MROWS = 256: 4 x 2 waves of 64 x 128).  The
 // next K step's global loads are issued before the current step's MFMAs so
 // their latency hides behind matrix work; the epilogue runs on registers
 // (butterfly reductions) with only small LDS exchanges.
@@ -854,7 +855,7 @@ __device__ inline uint32_t pack16(float lo, float hi) {
         return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, halfx2));
 }
 
-template <int PREC, int TC>
+template <int PREC, int TC, int MROWS>
 __global__ __launch_bounds__(512, 1) void k_score16(
     const float2* __restrict__ Y, const float2* __restrict__ S,
     const uint32_t* __restrict__ Bop, const float* __restrict__ Cterm,
@@ -864,22 +865,28 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     float* __restrict__ partWC, int* __restrict__ partArg,
     float* __restrict__ partWR, float* __restrict__ partWT,
     const float* __restrict__ gScale) {
-    constexpr int PB = 128 / TC;
+    constexpr int PB = MROWS / TC;
     constexpr int NB = kNB16;
+    // wave grid: NWM x NWN waves, each 64 rows x WNC columns (NTW 32-column tiles)
+    constexpr int NWM = MROWS / 64, NWN = 8 / NWM;
+    constexpr int WNC = NB / NWN, NTW = WNC / 32;
+    constexpr int NV = MROWS / 32;            // 32-row A loader groups
     constexpr bool F8 = PREC == THX_FP8;
     constexpr int LD = kLD16;                 // 16-bit elements per LDS row
     constexpr int LDB = F8 ? 80 : 2 * LD;     // bytes per LDS row (64 or 128 + 16 pad)
     constexpr int NTH = 512;
     // two LDS stages: chunk c + 1 is staged while chunk c feeds the MFMAs
-    __shared__ __attribute__((aligned(16))) unsigned char AsB[2][128 * LDB];
+    __shared__ __attribute__((aligned(16))) unsigned char AsB[2][MROWS * LDB];
     __shared__ __attribute__((aligned(16))) unsigned char BsB[2][NB * LDB];
-    __shared__ float redMax[4][4];        // [32-row tile][column quarter]
-    __shared__ int redArg[4][4];
+    __shared__ float redMax[NV][NWN];     // [32-row tile][column group]
+    __shared__ int redArg[NV][NWN];
     __shared__ float tileMax[PB];
     __shared__ int tileArg[PB];
-    __shared__ float colPart[4][NB];      // [32-row tile][column]
-    __shared__ float rowPart[4][128];     // [column quarter][row]
-    __shared__ float rowW[128];
+    __shared__ float rowW[MROWS];
+    // epilogue partials in the A stages (free once the main loop is over)
+    static_assert(sizeof(float) * (NV * NB + NWN * MROWS) <= sizeof(AsB), "epilogue LDS");
+    float (*colPart)[NB] = reinterpret_cast<float (*)[NB]>(&AsB[0][0]);             // [NV][NB]
+    float (*rowPart)[MROWS] = reinterpret_cast<float (*)[MROWS]>(&AsB[0][0] + sizeof(float) * NV * NB);
 
     const int Lg = xcd_logical(blockIdx.x, gridDim.x);
     const int pg = Lg % nPg;
@@ -889,15 +896,15 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     const int p0 = pg * PB, r0 = rb * NB, t0 = tc * TC;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wmi = wave >> 2, wni = wave & 3;       // 2 x 4 wave grid, 64 x 64 each
-    const int wm = wmi * 64, wn = wni * 64;
+    const int wmi = wave / NWN, wni = wave % NWN;
+    const int wm = wmi * 64, wn = wni * WNC;
     const int l32 = lane & 31, hl = lane >> 5;
 
-    floatx16 acc[2][2];
+    floatx16 acc[2][NTW];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = floatx16{0};
+        for (int b = 0; b < NTW; b++) acc[a][b] = floatx16{0};
 
     // Loader roles (lanes of one instruction read whole contiguous rows):
     //   A: lane part = tid % 16 owns complex pixels 2 part, 2 part + 1 of the
@@ -907,7 +914,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     //   B: lane part8 = tid % 8 owns 4 complex pixels of rows tid / 8 + 64 v
     //      (FP8: lane part4 = tid % 4 owns 8 pixels of rows tid / 4 + 128 v).
     constexpr int NT = TC / 32;
-    constexpr int NY = 4 / NT;
+    constexpr int NY = NV / NT;
     const int part = tid & 15, rbase = tid >> 4;      // rbase 0..31
     const int part8 = tid & 7, rb8 = tid >> 3;        // rb8 0..63
     const float4* yp = reinterpret_cast<const float4*>(Y + (size_t)(p0 + rbase / TC) * Kp) + part;
@@ -941,7 +948,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     // stage registers of one chunk into LDS buffer BUF: A = Z = Y conj(S), B as loaded
 #define THX_STAGE16(BUF)                                                                       \
     do {                                                                                       \
-        _Pragma("unroll") for (int v = 0; v < 4; v++) {                                        \
+        _Pragma("unroll") for (int v = 0; v < NV; v++) {                                       \
             const float4 y = ry[v / NT], sv = rs[v % NT];                                      \
             const float z0r = y.x * sv.x + y.y * sv.y, z0i = y.y * sv.x - y.x * sv.y;         \
             const float z1r = y.z * sv.z + y.w * sv.w, z1i = y.w * sv.z - y.z * sv.w;         \
@@ -978,7 +985,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
         if constexpr (F8) {
             // one 32x32x64 e4m3 step per tile: lane half hl takes bytes 32 hl .. 32 hl + 31
             // of its row in A and in B (the same k order on both sides)
-            intx8 ra[2], rb8v[2];
+            intx8 ra[2], rb8v[NTW];
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
                 const unsigned char* src = AsB[buf] + (wm + 32 * mt + l32) * LDB + 32 * hl;
@@ -988,7 +995,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
                                (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
             }
 #pragma unroll
-            for (int nt = 0; nt < 2; nt++) {
+            for (int nt = 0; nt < NTW; nt++) {
                 const unsigned char* src = BsB[buf] + (wn + 32 * nt + l32) * LDB + 32 * hl;
                 const uint4 lo = *reinterpret_cast<const uint4*>(src);
                 const uint4 hi = *reinterpret_cast<const uint4*>(src + 16);
@@ -998,7 +1005,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-                for (int nt = 0; nt < 2; nt++)
+                for (int nt = 0; nt < NTW; nt++)
                     acc[mt][nt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                         ra[mt], rb8v[nt], acc[mt][nt], 0, 0, 0, 0x7f7f7f7f, 0, 0x7f7f7f7f);
         } else {
@@ -1007,17 +1014,17 @@ __global__ __launch_bounds__(512, 1) void k_score16(
 #pragma unroll
         for (int kk = 0; kk < 64; kk += 16) {
             const int kc = kk + 8 * hl;
-            uint4 ra[2], rb4[2];
+            uint4 ra[2], rb4[NTW];
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
                 ra[mt] = *reinterpret_cast<const uint4*>(Ab + (wm + 32 * mt + l32) * LD + kc);
 #pragma unroll
-            for (int nt = 0; nt < 2; nt++)
+            for (int nt = 0; nt < NTW; nt++)
                 rb4[nt] = *reinterpret_cast<const uint4*>(Bb + (wn + 32 * nt + l32) * LD + kc);
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
 #pragma unroll
-                for (int nt = 0; nt < 2; nt++) {
+                for (int nt = 0; nt < NTW; nt++) {
                     if constexpr (PREC == THX_BF16)
                         acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
                             __builtin_bit_cast(bf16x8, ra[mt]), __builtin_bit_cast(bf16x8, rb4[nt]),
@@ -1051,7 +1058,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
         lmax[mt] = -INFINITY;
         larg[mt] = -1;
 #pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
+        for (int nt = 0; nt < NTW; nt++) {
             const int n = wn + 32 * nt + l32;
             const int r = r0 + n;
             const bool rok = pok && r < R;
@@ -1088,7 +1095,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
         float mx = -INFINITY;
         int arg = -1;
         for (int q = tid * TPP; q < (tid + 1) * TPP; q++)
-            for (int h = 0; h < 4; h++) {
+            for (int h = 0; h < NWN; h++) {
                 const float v = redMax[q][h];
                 const int a = redArg[q][h];
                 if (v > mx || (v == mx && a >= 0 && (arg < 0 || a < arg))) { mx = v; arg = a; }
@@ -1108,7 +1115,7 @@ __global__ __launch_bounds__(512, 1) void k_score16(
         const float mx = tileMax[mrow0 / TC];
         const bool live = mx > -INFINITY;
 #pragma unroll
-        for (int nt = 0; nt < 2; nt++) {
+        for (int nt = 0; nt < NTW; nt++) {
             const int n = wn + 32 * nt + l32;
             const float pr = pRf[r0 + n];
             float cs = 0.f;
@@ -1150,9 +1157,11 @@ __global__ __launch_bounds__(512, 1) void k_score16(
         for (int u = 0; u < TC / 32; u++) cs += colPart[pi * (TC / 32) + u][n];
         if (p < P) partWR[((size_t)p * nTc + tc) * Rpad + r0 + n] = cs;
     }
-    if (tid < 128) {
+    if (tid < MROWS) {
         const int m = tid;
-        const float rsum = rowPart[0][m] + rowPart[1][m] + rowPart[2][m] + rowPart[3][m];
+        float rsum = 0.f;
+#pragma unroll
+        for (int h = 0; h < NWN; h++) rsum += rowPart[h][m];
         const int p = p0 + m / TC, t = t0 + m % TC;
         if (p < P) partWT[((size_t)p * nRb + rb) * Tpad + t] = rsum;
         rowW[m] = rsum * pTf[t];
@@ -1171,6 +1180,12 @@ __global__ __launch_bounds__(512, 1) void k_score16(
         }
     }
 }
+
+// rows (particles x translations) per block of the 16-bit / FP8 scorer: 256,
+// 8 waves of 64 x 128 (a 128-row block, 8 waves of 64 x 64, measured 1.2x
+// slower for BF16 and 1.3x for FP8: every A and B fragment read from LDS now
+// feeds twice the MFMAs)
+constexpr int score16_rows(int) { return 256; }
 
 template <int PREC>
 static void launch_score_prec(thx_ctx* c, const void* B, int nPg, int nblk) {
@@ -1194,12 +1209,13 @@ static void launch_score_prec(thx_ctx* c, const void* B, int nPg, int nblk) {
         c->R, c->T, c->Kp, c->Rpad, c->Tpad, c->nRb, c->nTc, nPg, c->partMax.as<float>(),      \
         c->partWC.as<float>(), c->partArg.as<int>(), c->partWR.as<float>(), c->partWT.as<float>(), \
         PREC == THX_FP8 ? c->shellScale.as<const float>() + c->maxShell + 1 : nullptr
+        constexpr int MR = score16_rows(PREC);
         if (c->TC == 128)
-            hipLaunchKernelGGL((k_score16<PREC, 128>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
+            hipLaunchKernelGGL((k_score16<PREC, 128, MR>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
         else if (c->TC == 64)
-            hipLaunchKernelGGL((k_score16<PREC, 64>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
+            hipLaunchKernelGGL((k_score16<PREC, 64, MR>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
         else
-            hipLaunchKernelGGL((k_score16<PREC, 32>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
+            hipLaunchKernelGGL((k_score16<PREC, 32, MR>), dim3(nblk), dim3(512), 0, c->stream, THX_SCORE16_ARGS);
 #undef THX_SCORE16_ARGS
     }
 #undef THX_SCORE_ARGS
@@ -1208,7 +1224,9 @@ static void launch_score_prec(thx_ctx* c, const void* B, int nPg, int nblk) {
 int score_tile_cols(int precision) { return precision == THX_FP32 ? 128 : kNB16; }
 
 int launch_score(thx_ctx* c, int precision) {
-    const int PB = 128 / c->TC;
+    const int PB = (precision == THX_FP32 ? 128 : score16_rows(precision)) / c->TC;
+    THX_CHECK(c->Ppad % 128 == 0 && (size_t)((c->P + PB - 1) / PB) * PB <= (size_t)c->Ppad,
+              THX_ERR_STATE, "score: particle rows not padded for this tile");
     const int nPg = (c->P + PB - 1) / PB;
     const long long nblk = (long long)nPg * c->nTc * c->nRb;
     THX_CHECK(nblk < (1ll << 31), THX_ERR_RANGE, "score grid too large");
