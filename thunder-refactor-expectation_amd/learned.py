@@ -45,7 +45,10 @@ def train_proposal(net, ctx, workload, n_samples=7_864_320, chunk=8192, batch=51
     return time.perf_counter() - t0, losses[-8:]
 
 
-CANDIDATES = [(300, 49), (500, 49)]
+# (rotations, translations) selected per particle.  The scorer pads
+# translations to 32 / 64 / 128 rows and rotations to blocks of 128 / 256, so
+# the 64-translation and 256 / 512-rotation sizes fill the tiles they pay for.
+CANDIDATES = [(300, 49), (500, 49), (256, 64), (512, 64), (256, 121)]
 SPREAD = [30.0, 50.0, 80.0, 0.0]
 TAIL = [4.0, 5.0, 6.0, 0.0]
 DROP = [100.0, 150.0, 0.0]
