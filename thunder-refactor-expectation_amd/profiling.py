@@ -9,7 +9,9 @@
         counter runs, PREFIX_traffic.md / PREFIX_traffic.json: per-launch HBM
         read and write bytes of each kernel (average over its dispatches of
         the largest grid), each counter scaled so the copy reads and writes
-        exactly its size.
+        exactly its size; with --mfma (a run of the MFMA_COUNTERS below)
+        also PREFIX_mfma.md: executed matrix-core flops, rate and busy
+        fraction of each kernel that issues MFMA instructions.
 
 Counter directories are rocprofv3 `--output-format csv` runs
 (run_counter_collection.csv); the trace directory holds run_kernel_stats.csv.
@@ -77,6 +79,65 @@ def copy_scale(d, per_launch_bytes):
     return per_launch_bytes / (kib * 1024.0), kib
 
 
+# one counter run: 5 SQ counters and 1 GRBM counter (within the per-run limits)
+MFMA_COUNTERS = ["SQ_INSTS_VALU_MFMA_MOPS_F32", "SQ_INSTS_VALU_MFMA_MOPS_BF16",
+                 "SQ_INSTS_VALU_MFMA_MOPS_F8", "SQ_INSTS_VALU_MFMA_MOPS_F6F4",
+                 "SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE"]
+# operand type of each MOPS counter and MI355X's dense peak for it, TFLOP/s
+# (the FP8 operands of v_mfma_scale_*_f8f6f4 count under F8 or F6F4)
+MFMA_PEAK = {"SQ_INSTS_VALU_MFMA_MOPS_F32": ("fp32", 157.3),
+             "SQ_INSTS_VALU_MFMA_MOPS_BF16": ("bf16", 2500.0),
+             "SQ_INSTS_VALU_MFMA_MOPS_F8": ("fp8", 5000.0),
+             "SQ_INSTS_VALU_MFMA_MOPS_F6F4": ("fp8", 5000.0)}
+N_XCC, N_SIMD = 8, 256 * 4
+
+
+def mfma_summary(d, name, command):
+    """Markdown table of the MFMA counters of one run, per (kernel, grid),
+    averaged over its dispatches.  MOPS counters count 512 flops each;
+    GRBM_GUI_ACTIVE is reported summed over the 8 XCDs, so one XCD's cycles of
+    the dispatch are GRBM / 8; MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES (summed over
+    the 1024 SIMDs) / (1024 x GRBM / 8)."""
+    disp = {}
+    with open(os.path.join(d, "run_counter_collection.csv")) as f:
+        for row in csv.DictReader(f):
+            e = disp.setdefault(row["Dispatch_Id"], {
+                "key": (short(row["Kernel_Name"]), int(row["Grid_Size"])),
+                "ns": float(row["End_Timestamp"]) - float(row["Start_Timestamp"]), "c": {}})
+            e["c"][row["Counter_Name"]] = e["c"].get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    groups = {}
+    for e in disp.values():
+        groups.setdefault(e["key"], []).append(e)
+    md = [f"# MFMA utilisation per launch: {name}", "",
+          "Command (as for the kernel trace):", "", "```", command, "```", "",
+          "Per kernel and grid, averaged over its dispatches in this run: executed matrix-core "
+          "flops = MOPS x 512 (padding rows and columns included), the dispatch time of the "
+          "counter run, the executed rate against MI355X's dense peak for the operand type, "
+          "and MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (1024 SIMDs x GRBM_GUI_ACTIVE / 8 XCDs); "
+          "`clock` = GRBM_GUI_ACTIVE / 8 / time, a check on the normalisation.", "",
+          "| Kernel | grid | launches | type | executed TFLOP | ms | TFLOP/s | % of peak | "
+          "MFMA busy % | clock GHz |", "|---|---|---|---|---|---|---|---|---|---|"]
+    rows = []
+    for (k, g), es in groups.items():
+        n = len(es)
+        avg = {c: sum(e["c"].get(c, 0.0) for e in es) / n for c in MFMA_COUNTERS}
+        mops = {c: avg[c] for c in MFMA_PEAK if avg[c] > 0}
+        if not mops:
+            continue
+        c_main = max(mops, key=mops.get)
+        typ, peak = MFMA_PEAK[c_main]
+        flops = 512.0 * sum(mops.values())
+        ns = sum(e["ns"] for e in es) / n
+        cyc = avg["GRBM_GUI_ACTIVE"] / N_XCC
+        busy = 100.0 * avg["SQ_VALU_MFMA_BUSY_CYCLES"] / (N_SIMD * cyc) if cyc > 0 else float("nan")
+        rate = flops / (ns * 1e-9) / 1e12 if ns > 0 else float("nan")
+        rows.append((flops, f"| `{k}` | {g} | {n} | {typ} | {flops / 1e12:.3f} | {ns / 1e6:.3f} | "
+                            f"{rate:.1f} | {100.0 * rate / peak:.1f} | {busy:.1f} | "
+                            f"{cyc / ns:.2f} |"))
+    md += [r for _, r in sorted(rows, key=lambda x: -x[0])]
+    return "\n".join(md) + "\n"
+
+
 def summarize(a):
     rows = list(csv.DictReader(open(os.path.join(a.trace, "run_kernel_stats.csv"))))
     lines = [f"# Kernel trace: {a.name}", "", "Command (repository root, `cd /tmp && export "
@@ -88,6 +149,8 @@ def summarize(a):
                      f"{float(r['MinNs']) / 1e6:.3f} | {float(r['MaxNs']) / 1e6:.3f} | "
                      f"{float(r['Percentage']):.2f} |")
     open(a.out + "_trace.md", "w").write("\n".join(lines) + "\n")
+    if a.mfma:
+        open(a.out + "_mfma.md", "w").write(mfma_summary(a.mfma, a.name, a.command_mfma))
     if not a.fetch:
         return
     fs, fkib = copy_scale(a.cal_fetch, COPY_BYTES)
@@ -144,6 +207,8 @@ def main():
     ap.add_argument("--command-write", default="")
     ap.add_argument("--command-cal-fetch", default="")
     ap.add_argument("--command-cal-write", default="")
+    ap.add_argument("--mfma")
+    ap.add_argument("--command-mfma", default="")
     ap.add_argument("--out")
     a = ap.parse_args()
     if a.what == "copy":
