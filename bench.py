@@ -234,6 +234,38 @@ def run_local_box160(steps=1):
             "precision": "fp16"}
 
 
+def run_box256_fp8(steps):
+    """FP8 global scan at the box-256 workload's full size (20,000 particles x
+    10,000 rotations x 81 translations, cutoff 100 px, K = 15,559) against one
+    plain FP32 scan of the same context: times, rates and agreement."""
+    name = "box256"
+    gctx = thx.Context(0)
+    info = wl.setup_context(gctx, name)
+    P, R, T, K = gctx.P, gctx.R, gctx.T, gctx.K
+    quat, trans = info["quat"], info["trans"]
+    step = wl.sampling_step(quat)
+    f = run_mode(gctx, "fp32", 1, 0, P, R, T, K)
+    gR, gT, gS = gctx.get_best(0)
+    e = run_mode(gctx, "fp8", steps, 1, P, R, T, K)
+    eR, eT, eS = gctx.get_best(0)
+    gctx.close()
+    t8 = e["kernels_ms"]["score"] * 1e-3
+    rel = np.abs(eS - gS) / np.abs(gS)
+    return {"particles": P, "rotations": R, "translations": T, "K": K,
+            "fp8": {"ms_per_step": e["ms_per_step"], "pose_evals_per_s": e["pose_evals_per_s"],
+                    "kernels_ms": e["kernels_ms"],
+                    "score_tflops": e["score_flops"] / t8 / 1e12,
+                    "score_frac_of_peak": e["score_flops"] / t8 / PEAK["fp8"]},
+            "fp32": {"ms_per_step": f["ms_per_step"], "pose_evals_per_s": f["pose_evals_per_s"],
+                     "kernels_ms": f["kernels_ms"],
+                     "score_tflops": f["score_flops"] / (f["kernels_ms"]["score"] * 1e-3) / 1e12},
+            "speedup_vs_fp32": e["pose_evals_per_s"] / f["pose_evals_per_s"],
+            "agreement_with_fp32": agreement(quat, trans, step, eR, eT, gR, gT),
+            "best_score_rel_err_vs_fp32": {"median": float(np.median(rel)),
+                                           "max": float(np.max(rel))},
+            "sampling_step_deg": float(np.degrees(step))}
+
+
 def run_projectee(name, steps):
     """Projectee preparation (Projector::setProjectee) of a workload-sized map:
     inverse FFT, pad + sinc^2 correction kernel, forward FFT, each timed on the
@@ -318,6 +350,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-box160", action="store_true",
                     help="skip the full-size box-160 ragged local scoring")
+    ap.add_argument("--no-box256", action="store_true",
+                    help="skip the full-size box-256 FP8 global scan")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -417,6 +451,10 @@ def main():
         log("box-160 ragged local scoring")
         out["local_box160"] = run_local_box160()
         log(f"box-160 local: {out['local_box160']}")
+    if not args.no_box256:
+        log("box-256 FP8 global scan")
+        out["fp8_box256"] = run_box256_fp8(2)
+        log(f"box-256 FP8: {out['fp8_box256']}")
     log("projectee preparation")
     out["projectee"] = run_projectee(name, args.steps)
     log(f"projectee: {out['projectee']}")
