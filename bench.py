@@ -341,7 +341,7 @@ def main():
     ap.add_argument("--workload", default="box128")
     ap.add_argument("--mixed", default="bf16", choices=["bf16", "fp16"])
     ap.add_argument("--particles", type=int, default=0)
-    ap.add_argument("--train-samples", type=int, default=15_728_640)
+    ap.add_argument("--train-samples", type=int, default=31_457_280)
     ap.add_argument("--cal-particles", type=int, default=2000)
     ap.add_argument("--rule", default="",
                     help="n_rot,n_trans,spread_max,tail_max,score_drop: skip the calibration")

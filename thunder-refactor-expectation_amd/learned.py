@@ -21,7 +21,7 @@ import thx
 import workloads as wl
 
 
-def train_proposal(net, ctx, workload, n_samples=15_728_640, chunk=8192, batch=512,
+def train_proposal(net, ctx, workload, n_samples=31_457_280, chunk=8192, batch=512,
                    snr=(0.01, 0.1), seed=5000, schedule=((0.5, 1e-3), (0.25, 3e-4), (0.25, 1e-4)),
                    log=None):
     """Train `net` on synthetic particles of the reference volume held by `ctx`
