@@ -50,6 +50,27 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+int HostBuf::ensure(size_t need) {
+    if (need <= bytes && p) return THX_OK;
+    release();
+    if (need == 0) return THX_OK;
+    hipError_t e = hipHostMalloc(&p, need, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        set_error("hipHostMalloc of " + std::to_string(need) + " bytes failed: " + hipGetErrorString(e));
+        return THX_ERR_OOM;
+    }
+    bytes = need;
+    return THX_OK;
+}
+
+void HostBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
 // Translation chunking of a scan: rows per particle TC in {32, 64, 128}.
 // The plain kernel minimises the padded count ceil(T / TC) * TC (ties to the
 // larger TC); the 16-bit kernel prefers TC = 32 at equal padding, which lets
@@ -143,8 +164,12 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
     THX_TRY(launch_prep_particles(c, precision));
     THX_TRY(record(c, "prep"));
     if (precision != THX_FP32) {
+        const void* was = c->rotLo.p;
         THX_TRY(c->rotLo.ensure(sizeof(uint32_t) * (size_t)c->Rpad * c->Kp));
-        THX_TRY(launch_convert_lo(c, precision));
+        if (c->rotLo.p != was) c->rotLoPrec = -1;
+        // the BF16 table depends on the projections only (no particle-derived
+        // scales): kept when it already holds the current ones
+        if (precision != THX_BF16 || c->rotLoPrec != THX_BF16) THX_TRY(launch_convert_lo(c, precision));
         THX_TRY(record(c, "convert"));
     }
     THX_TRY(launch_norm_gemm(c));
@@ -306,9 +331,10 @@ int thx_ctx_destroy(thx_ctx* c) {
     if (c->lpReady) (void)hipEventDestroy(c->lpReady);
     if (c->lpDone) (void)hipEventDestroy(c->lpDone);
     projectee_release(c);
+    c->lpStage.release();
     DevBuf* bufs[] = {&c->iCol, &c->iRow, &c->iSig, &c->dat, &c->ctf, &c->sig, &c->Aterm,
                       &c->Y, &c->Wt, &c->quat, &c->trans, &c->pR, &c->pT, &c->pRf, &c->pTf,
-                      &c->S, &c->rotP, &c->rotLo, &c->normP, &c->Cterm, &c->shellScale,
+                      &c->S, &c->rotP, &c->rotLo, &c->normP, &c->Cterm, &c->normPart, &c->shellScale,
                       &c->shellMax, &c->partMax, &c->partWC, &c->partArg, &c->partWR,
                       &c->partWT};
     for (DevBuf* b : bufs) b->release();
@@ -891,19 +917,6 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
     THX_CHECK(nbp <= 64, THX_ERR_RANGE, "learned: too many selected rotations");
     const size_t nblk = (size_t)nP * nbp;
     THX_CHECK(nblk < (1u << 31), THX_ERR_RANGE, "learned: grid too large");
-    std::vector<int> rOff(nP), nR(nP, nSelR), tOff(nP), nT(nP, nSelT), bStart(nP + 1);
-    for (int p = 0; p < nP; p++) {
-        rOff[p] = p * nSelR;
-        tOff[p] = p * nSelT;
-        bStart[p] = p * nbp;
-    }
-    bStart[nP] = (int)nblk;
-    hipStream_t st = c->stream;
-    THX_TRY(upload(c->locROff, rOff.data(), nP, st));
-    THX_TRY(upload(c->locNR, nR.data(), nP, st));
-    THX_TRY(upload(c->locTOff, tOff.data(), nP, st));
-    THX_TRY(upload(c->locNT, nT.data(), nP, st));
-    THX_TRY(upload(c->locBStart, bStart.data(), nP + 1, st));
     THX_TRY(c->locBMax.ensure(sizeof(float) * nblk));
     THX_TRY(c->locBWC.ensure(sizeof(float) * nblk));
     THX_TRY(c->locBArg.ensure(sizeof(int) * nblk));
@@ -915,12 +928,12 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
     THX_TRY(c->locBase.ensure(sizeof(double) * nP));
     THX_TRY(c->locBestR.ensure(sizeof(int) * nP));
     THX_TRY(c->locBestT.ensure(sizeof(int) * nP));
+    // every particle has nSelR rotations and nSelT translations: the merge
+    // takes the affine offsets p * nSelR, p * nSelT and blocks p * nbp
     LocalBatch b;
-    b.rOff = c->locROff.as<const int>();
-    b.nR = c->locNR.as<const int>();
-    b.tOff = c->locTOff.as<const int>();
-    b.nT = c->locNT.as<const int>();
-    b.bStart = c->locBStart.as<const int>();
+    b.affR = nSelR;
+    b.affT = nSelT;
+    b.affB = nbp;
     b.bMax = c->locBMax.as<float>();
     b.bWC = c->locBWC.as<float>();
     b.bArg = c->locBArg.as<int>();
@@ -1022,17 +1035,26 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
                              c->lpTail.as<float>()));
     THX_TRY(record(c, "score_sel"));
     // best pose in grid indices (mapped by the stats kernel)
-    std::vector<int> bR(P), bT(P);
-    std::vector<float> spread(P), tail(P);
-    c->lpScore.assign(P, 0.0);
-    c->lpEssH.assign(P, 0.0);
+    // the six per-particle results in one pinned batch: [score, ess] as
+    // doubles, then [bestR, bestT] as ints and [spread, tail] as floats
+    THX_TRY(c->lpStage.ensure((2 * sizeof(double) + 4 * sizeof(int)) * (size_t)P));
+    double* hScore = c->lpStage.as<double>();
+    double* hEss = hScore + P;
+    int* hR = reinterpret_cast<int*>(hEss + P);
+    int* hT = hR + P;
+    float* hSpread = reinterpret_cast<float*>(hT + P);
+    float* hTail = hSpread + P;
+    THX_HIP(hipMemcpyAsync(hScore, c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+    THX_HIP(hipMemcpyAsync(hEss, c->lpESS.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+    THX_HIP(hipMemcpyAsync(hR, c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost, st));
+    THX_HIP(hipMemcpyAsync(hT, c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost, st));
+    THX_HIP(hipMemcpyAsync(hSpread, c->lpSpread.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
+    THX_HIP(hipMemcpyAsync(hTail, c->lpTail.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
     THX_HIP(hipStreamSynchronize(st));
-    THX_HIP(hipMemcpy(bR.data(), c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost));
-    THX_HIP(hipMemcpy(bT.data(), c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost));
-    THX_HIP(hipMemcpy(c->lpScore.data(), c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost));
-    THX_HIP(hipMemcpy(c->lpEssH.data(), c->lpESS.p, sizeof(double) * P, hipMemcpyDeviceToHost));
-    THX_HIP(hipMemcpy(spread.data(), c->lpSpread.p, sizeof(float) * P, hipMemcpyDeviceToHost));
-    THX_HIP(hipMemcpy(tail.data(), c->lpTail.p, sizeof(float) * P, hipMemcpyDeviceToHost));
+    c->lpScore.assign(hScore, hScore + P);
+    c->lpEssH.assign(hEss, hEss + P);
+    std::vector<int> bR(hR, hR + P), bT(hT, hT + P);
+    std::vector<float> spread(hSpread, hSpread + P), tail(hTail, hTail + P);
     c->lpBestR = bR;
     c->lpBestT = bT;
     c->lpFell.assign(P, 0);
@@ -1062,13 +1084,15 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
         THX_TRY(scan_subset(c, cls, fallbackPrecision, flagged));
         c->nev = nev0;   // keep the stage events of this call
         c->evNames.resize(nev0);
-        std::vector<int> fR(nf), fT(nf);
-        std::vector<double> fS(nf);
+        // the staging holds P >= nf entries of each kind
+        double* fS = c->lpStage.as<double>();
+        int* fR = reinterpret_cast<int*>(fS + nf);
+        int* fT = fR + nf;
+        THX_HIP(hipMemcpyAsync(fS, c->lpRes.bestScore.p, sizeof(double) * nf,
+                               hipMemcpyDeviceToHost, st));
+        THX_HIP(hipMemcpyAsync(fR, c->lpRes.bestR.p, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
+        THX_HIP(hipMemcpyAsync(fT, c->lpRes.bestT.p, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
         THX_HIP(hipStreamSynchronize(st));
-        THX_HIP(hipMemcpy(fR.data(), c->lpRes.bestR.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
-        THX_HIP(hipMemcpy(fT.data(), c->lpRes.bestT.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
-        THX_HIP(hipMemcpy(fS.data(), c->lpRes.bestScore.p, sizeof(double) * nf,
-                          hipMemcpyDeviceToHost));
         for (int i = 0; i < nf; i++) {
             const int p = flagged[i];
             c->lpBestR[p] = fR[i];
@@ -1552,6 +1576,7 @@ int thx_ExpectGlobal3D(const float* rotP, const float* traP, const float* datP, 
     THX_TRY(upload_priors(c, pR, pT));
     // projections and phases into the padded layouts (zero padding)
     THX_HIP(hipMemset(c->rotP.p, 0, sizeof(float2) * (size_t)c->Rpad * c->Kp));
+    c->rotLoPrec = -1;
     THX_HIP(hipMemset(c->S.p, 0, sizeof(float2) * (size_t)c->TpadS * c->Kp));
     THX_HIP(hipMemcpy2D(c->rotP.p, sizeof(float2) * c->Kp, rotP, sizeof(float2) * npxl,
                         sizeof(float2) * npxl, nR, hipMemcpyHostToDevice));

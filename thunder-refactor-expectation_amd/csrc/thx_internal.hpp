@@ -47,6 +47,16 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Pinned host staging (hipHostMalloc), grown as needed: device-to-host
+// copies into it are asynchronous and can share one stream synchronisation.
+struct HostBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
 // Host array -> device buffer (grown as needed), asynchronous on st.
@@ -95,6 +105,10 @@ struct LocalBatch {
     int* bArg = nullptr;                   // [nblk]
     float* bWT = nullptr;                  // [nblk][TPAD]
     float* wRrel = nullptr;                // [Rtot]
+    // equal counts for every particle (rOff / nR / tOff / nT / bStart null):
+    // particle p has affR rotations from p * affR, affT translations from
+    // p * affT and blocks from p * affB
+    int affR = 0, affT = 0, affB = 0;
 };
 
 struct Timing {
@@ -118,6 +132,7 @@ struct thx_ctx {
     std::vector<char> volLoOk;
     thx::DevBuf locVolsLo;
     thx::DevBuf lpQf;        // grid quaternions as float4 for the selection
+    thx::HostBuf lpStage;    // pinned: per-particle results read back in one batch
     // ordering events between the context stream and the proposal network's
     hipEvent_t lpReady = nullptr, lpDone = nullptr;
     // projectee preparation: FFT plans (hipfftHandle) for one (N, pf), scratch
@@ -141,8 +156,10 @@ struct thx_ctx {
     // scan workspaces
     thx::DevBuf rotP;       // float2 [Rpad][Kp]
     thx::DevBuf rotLo;      // 2 x 16-bit (or 2 x 8-bit) [Rpad][Kp]
+    int rotLoPrec = -1;     // precision rotLo holds for the current rotP; -1: stale
     thx::DevBuf normP;      // float [Rpad][Kp]  |P|^2
     thx::DevBuf Cterm;      // float [Ppad][Rpad]
+    thx::DevBuf normPart;   // float [ns][Ppad][Rpad]: pixel-range slabs of Cterm
     thx::DevBuf shellScale; // float [maxShell+1]
     thx::DevBuf shellMax;   // float [2][maxShell+1] (scratch of the scale pass)
     thx::DevBuf partMax, partWC, partArg, partWR, partWT;

@@ -185,6 +185,7 @@ __global__ __launch_bounds__(256) void k_project(
 
 int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
                    float2* out, float* outNorm) {
+    if (out == c->rotP.as<float2>()) c->rotLoPrec = -1;   // the 16-bit table is stale
     dim3 grid((c->Kp + 255) / 256, Rpad);
     hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c->stream,
                        c->vol[cls].as<const float2>(), c->vdim[cls], c->pf, dQuat, R,
@@ -477,6 +478,7 @@ int launch_convert_lo(thx_ctx* c, int precision) {
         hipLaunchKernelGGL(k_convert_lo<THX_FP8>, grid, dim3(256), 0, c->stream, THX_CONV_ARGS);
 #undef THX_CONV_ARGS
     THX_HIP(hipGetLastError());
+    c->rotLoPrec = precision;
     return THX_OK;
 }
 
@@ -494,17 +496,22 @@ __device__ inline int xcd_logical(int b, int nb) {
 
 // ---------------------------------------------------------------------------
 // Norm term: Cterm[p][r] = sum_k Wt[p][k] normP[r][k] (FP32 MFMA 32x32x2).
-// Block tile 128 x 128, 4 waves of 64 x 64, K step 64.
+// Block tile 128 x 128, 4 waves of 64 x 64, K step 64.  blockIdx.y splits
+// the pixels into ranges of kLen (a multiple of 64) when there are too few
+// tiles to fill the machine; each range then writes its own slab of Cterm,
+// Ppad x Rpad floats apart, summed afterwards in a fixed order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256, 2) void k_norm_gemm(
     const float* __restrict__ Wt, const float* __restrict__ normP, int Kp, int Rpad,
-    int nPb, int nRb, float* __restrict__ Cterm) {
+    int nPb, int nRb, int kLen, float* __restrict__ Cterm) {
     constexpr int LD = 65;
     __shared__ float As[128 * LD];
     __shared__ float Bs[128 * LD];
     const int L = xcd_logical(blockIdx.x, gridDim.x);
     const int pb = L % nPb, rb = L / nPb;
     const int p0 = pb * 128, r0 = rb * 128;
+    const int kBeg = blockIdx.y * kLen, kEnd = min(Kp, kBeg + kLen);
+    Cterm += (size_t)blockIdx.y * nPb * 128 * Rpad;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     floatx16 acc[2][2];
@@ -513,11 +520,13 @@ __global__ __launch_bounds__(256, 2) void k_norm_gemm(
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = floatx16{0};
     const int lr = tid >> 1, lh = (tid & 1) * 32;
-    for (int k0 = 0; k0 < Kp; k0 += 64) {
+    for (int k0 = kBeg; k0 < kEnd; k0 += 64) {
         // each thread loads 32 consecutive floats of one row of A and of B
+        // (Kp and kLen are multiples of 32 and 64: a segment lies wholly
+        // inside or outside the range)
         const float4* ga = reinterpret_cast<const float4*>(Wt + (size_t)(p0 + lr) * Kp + k0 + lh);
         const float4* gb = reinterpret_cast<const float4*>(normP + (size_t)(r0 + lr) * Kp + k0 + lh);
-        const bool okA = (k0 + lh) < Kp, okB = okA;
+        const bool okA = (k0 + lh) < kEnd, okB = okA;
 #pragma unroll
         for (int v = 0; v < 8; v++) {
             float4 x = okA ? ga[v] : make_float4(0, 0, 0, 0);
@@ -554,12 +563,39 @@ __global__ __launch_bounds__(256, 2) void k_norm_gemm(
             }
 }
 
+// Cterm[i] = sum over the ns slabs of part, in slab order.
+__global__ void k_sum_slabs(const float* __restrict__ part, int ns, size_t n,
+                            float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = part[i];
+    for (int j = 1; j < ns; j++) s += part[(size_t)j * n + i];
+    out[i] = s;
+}
+
 int launch_norm_gemm(thx_ctx* c) {
     const int nPb = c->Ppad / 128, nRb = c->Rpad / 128;
-    hipLaunchKernelGGL(k_norm_gemm, dim3(nPb * nRb), dim3(256), 0, c->stream,
+    // fewer tiles than two per CU: split the pixels (at least 256 per range)
+    const int tiles = nPb * nRb;
+    int ns = tiles >= 512 ? 1 : std::min(8, (1024 + tiles - 1) / tiles);
+    ns = std::max(1, std::min(ns, c->Kp / 256));
+    const int kLen = round_up((c->Kp + ns - 1) / ns, 64);
+    ns = (c->Kp + kLen - 1) / kLen;
+    const size_t n = (size_t)c->Ppad * c->Rpad;
+    float* dst = c->Cterm.as<float>();
+    if (ns > 1) {
+        THX_TRY(c->normPart.ensure(sizeof(float) * n * ns));
+        dst = c->normPart.as<float>();
+    }
+    hipLaunchKernelGGL(k_norm_gemm, dim3(tiles, ns), dim3(256), 0, c->stream,
                        c->Wt.as<const float>(), c->normP.as<const float>(), c->Kp, c->Rpad,
-                       nPb, nRb, c->Cterm.as<float>());
+                       nPb, nRb, kLen, dst);
     THX_HIP(hipGetLastError());
+    if (ns > 1) {
+        hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           c->normPart.as<const float>(), ns, n, c->Cterm.as<float>());
+        THX_HIP(hipGetLastError());
+    }
     return THX_OK;
 }
 
@@ -1769,13 +1805,16 @@ __global__ __launch_bounds__(256) void k_merge_local(
     const float* __restrict__ bMax, const float* __restrict__ bWC, const int* __restrict__ bArg,
     const float* __restrict__ bWT, const float* __restrict__ wRrel, const int* __restrict__ bStart,
     const int* __restrict__ rOff, const int* __restrict__ nR, const int* __restrict__ tOff,
-    const int* __restrict__ nT, int TPAD, int rb, double* __restrict__ wC,
-    double* __restrict__ wR, double* __restrict__ wT, double* __restrict__ base,
-    int* __restrict__ bestR, int* __restrict__ bestT) {
+    const int* __restrict__ nT, int affR, int affT, int affB, int TPAD, int rb,
+    double* __restrict__ wC, double* __restrict__ wR, double* __restrict__ wT,
+    double* __restrict__ base, int* __restrict__ bestR, int* __restrict__ bestT) {
     const int p = blockIdx.x;
     // a particle's blocks are contiguous from bStart[p] (particles themselves
-    // may be listed in any order)
-    const int b0 = bStart[p], nb = (nR[p] + rb - 1) / rb;
+    // may be listed in any order); null tables mean equal counts for all
+    const int Rp = nR ? nR[p] : affR, Tp = nT ? nT[p] : affT;
+    const size_t ro = rOff ? (size_t)rOff[p] : (size_t)p * affR;
+    const size_t to = tOff ? (size_t)tOff[p] : (size_t)p * affT;
+    const int b0 = bStart ? bStart[p] : p * affB, nb = (Rp + rb - 1) / rb;
     __shared__ double f[64];
     __shared__ double red[256];
     __shared__ int redi[256];
@@ -1812,13 +1851,12 @@ __global__ __launch_bounds__(256) void k_merge_local(
         bestR[p] = a < 0 ? -1 : a / kNBL;
         bestT[p] = a < 0 ? -1 : a % kNBL;
     }
-    const int Rp = nR[p], Tp = nT[p];
     for (int r = threadIdx.x; r < Rp; r += blockDim.x)
-        wR[rOff[p] + r] = f[r / rb] * (double)wRrel[rOff[p] + r];
+        wR[ro + r] = f[r / rb] * (double)wRrel[ro + r];
     for (int t = threadIdx.x; t < Tp; t += blockDim.x) {
         double s = 0.0;
         for (int i = 0; i < nb; i++) s += f[i] * bWT[(size_t)(b0 + i) * TPAD + t];
-        wT[tOff[p] + t] = s;
+        wT[to + t] = s;
     }
 }
 
@@ -1890,8 +1928,8 @@ int launch_gather_particles(thx_ctx* c, const int* idx, int n, float2* dat2, flo
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
                        double* wT, double* base, int* bestR, int* bestT, int rb) {
     hipLaunchKernelGGL(k_merge_local, dim3(P), dim3(256), 0, c->stream, b.bMax, b.bWC, b.bArg,
-                       b.bWT, b.wRrel, b.bStart, b.rOff, b.nR, b.tOff, b.nT, TPAD, rb, wC, wR, wT,
-                       base, bestR, bestT);
+                       b.bWT, b.wRrel, b.bStart, b.rOff, b.nR, b.tOff, b.nT, b.affR, b.affT,
+                       b.affB, TPAD, rb, wC, wR, wT, base, bestR, bestT);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
