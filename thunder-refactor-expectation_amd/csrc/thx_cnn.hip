@@ -42,90 +42,152 @@ constexpr int kFcIn = 128 * 4 * 4, kFcHid = 384;
 // ---------------------------------------------------------------------------
 // Images: 64 x 64 inverse DFT of the phase-flipped coefficients with
 // 0 <= iCol < 32 and -32 <= iRow < 32 (each standing for itself and its
-// Hermitian partner), then normalisation.  One block per particle; the
-// transform is separable: H[j][x] = sum_i G[j][i] e^{2 pi i i x / 64} over
-// the 32 columns, then img[y][x] = Re sum_j H[j][x] e^{2 pi i j y / 64} over
-// the 64 rows, both in LDS with a 64-entry twiddle table.
+// Hermitian partner), then normalisation.  One wave per particle, both
+// passes of the separable transform as real products on FP32 MFMA
+// (v_mfma_f32_32x32x2f32) with w(t) = exp(2 pi i t / 64):
+//   pass 1  H[j][x] = sum_i G[j][i] w(i x)                (j = row + 32)
+//           as [Hr | Hi] (64 x 128) = [Gr | Gi] (64 x 64) B1, with
+//           B1 = [[cos, sin], [-sin, cos]] of (i x), G staged in LDS;
+//   pass 2  img[y][x] = Re sum_j H[j][x] w((j - 32) y)
+//           as img (64 x 64) = [cos | -sin] of ((j - 32) y) (64 x 128) [Hr; Hi],
+// where pass 2 takes H straight from pass 1's accumulators: its k order is
+// the accumulator row order, so no data moves between the passes.
 // ---------------------------------------------------------------------------
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+constexpr int kImgLD = 65;   // LDS row stride of G (floats): conflict-free column reads
+
 __global__ __launch_bounds__(256) void k_images(const float2* __restrict__ dat,
                                                 const float* __restrict__ ctf,
                                                 const int* __restrict__ iCol,
                                                 const int* __restrict__ iRow, int K, int P,
                                                 float* __restrict__ img) {
-    const int p = blockIdx.x;
-    __shared__ float2 G[64 * 33];    // [row j + 32][col i], padded
-    __shared__ float2 H[64 * 65];    // [row j + 32][x], padded
+    __shared__ __attribute__((aligned(16))) float G[4][64 * kImgLD];   // per wave: [row j][Re i | Im i]
     __shared__ float2 tw[64];
-    __shared__ float red[2][256];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hl = lane >> 5;
+    const int p = blockIdx.x * 4 + wave;
     if (tid < 64) {
         float sn, cs;
         sincospif(tid / 32.f, &sn, &cs);
         tw[tid] = make_float2(cs, sn);
     }
-    for (int e = tid; e < 64 * 33; e += 256) G[e] = make_float2(0.f, 0.f);
+    float* g = G[wave];
+    for (int e = lane; e < 64 * kImgLD; e += 64) g[e] = 0.f;
     __syncthreads();
-    for (int k = tid; k < K; k += 256) {
-        const int i = iCol[k], j = iRow[k];
-        if (i < 32 && j >= -32 && j < 32 && !(i == 0 && j == 0)) {
-            const float2 x = dat[(size_t)p * K + k];
-            const float f = ctf[(size_t)p * K + k] >= 0.f ? 2.f : -2.f;
-            G[(j + 32) * 33 + i] = make_float2(f * x.x, f * x.y);
-        }
-    }
-    __syncthreads();
-    // columns: thread owns x = tid & 63 and rows (tid >> 6) + 4 q
-    {
-        const int x = tid & 63;
-        for (int q = 0; q < 16; q++) {
-            const int jr = (tid >> 6) + 4 * q;
-            float hr = 0.f, hi = 0.f;
-#pragma unroll 8
-            for (int i = 0; i < 32; i++) {
-                const float2 g = G[jr * 33 + i];
-                const float2 w = tw[(i * x) & 63];
-                hr += g.x * w.x - g.y * w.y;
-                hi += g.x * w.y + g.y * w.x;
+    if (p < P) {
+        // eight coefficients per lane in flight at a time
+        constexpr int U = 8;
+        for (int k0 = 0; k0 < K; k0 += 64 * U) {
+            float2 x[U];
+            float cv[U];
+            int ij[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = k0 + 64 * u + lane;
+                ij[u] = -1;
+                if (k < K) {
+                    const int i = iCol[k], j = iRow[k];
+                    if (i < 32 && j >= -32 && j < 32 && !(i == 0 && j == 0)) {
+                        ij[u] = (j + 32) * kImgLD + i;
+                        x[u] = dat[(size_t)p * K + k];
+                        cv[u] = ctf[(size_t)p * K + k];
+                    }
+                }
             }
-            H[jr * 65 + x] = make_float2(hr, hi);
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (ij[u] >= 0) {
+                    const float f = cv[u] >= 0.f ? 2.f : -2.f;
+                    g[ij[u]] = f * x[u].x;
+                    g[ij[u] + 32] = f * x[u].y;
+                }
         }
     }
     __syncthreads();
-    // rows: thread owns x = tid & 63 and y = (tid >> 6) + 4 q
-    float v[16];
+    if (p >= P) return;   // no block-wide synchronisation follows
+
+    // ---- pass 1: acc[mt][nt], rows j = 32 mt + .., columns n = 32 nt + l32
+    // (nt 0, 1: Hr at x = n; nt 2, 3: Hi at x = n - 64)
+    floatx16 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = floatx16{0};
+    for (int s = 0; s < 32; s++) {
+        const int k = 2 * s + hl;   // this lane's k: Re (k < 32) or Im (k >= 32) of i = k & 31
+        const int i = k & 31;
+        const float a0 = g[l32 * kImgLD + k];
+        const float a1 = g[(32 + l32) * kImgLD + k];
+        const float2 w0 = tw[(i * l32) & 63];          // x = l32
+        const float2 w1 = tw[(i * (32 + l32)) & 63];   // x = 32 + l32
+        // B1[k][n]: Hr columns cos (k < 32) / -sin (k >= 32); Hi columns sin / cos
+        const bool re = k < 32;
+        const float b0 = re ? w0.x : -w0.y, b1 = re ? w1.x : -w1.y;
+        const float b2 = re ? w0.y : w0.x, b3 = re ? w1.y : w1.x;
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+        acc[0][2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b2, acc[0][2], 0, 0, 0);
+        acc[0][3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b3, acc[0][3], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+        acc[1][2] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b2, acc[1][2], 0, 0, 0);
+        acc[1][3] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b3, acc[1][3], 0, 0, 0);
+    }
+
+    // ---- pass 2, one half of the rows y = 32 my + .. at a time: out[nx],
+    // columns x = 32 nx + l32.  Step (mt, e): this lane's k is row
+    // j = 32 mt + 8 (e / 4) + 4 hl + e % 4 of pass 1, whose Hr / Hi at
+    // x = 32 nx + l32 it already holds in acc[mt][nx] / acc[mt][2 + nx]
+    // element e.  The unnormalised image goes to the wave's LDS rows (G is
+    // no longer read), 64 floats per row.
     float s1 = 0.f, s2 = 0.f;
-    {
-        const int x = tid & 63;
-        for (int q = 0; q < 16; q++) {
-            const int y = (tid >> 6) + 4 * q;
-            float acc = 0.f;
-#pragma unroll 8
-            for (int jr = 0; jr < 64; jr++) {
-                const float2 h = H[jr * 65 + x];
-                const float2 w = tw[((jr - 32) * y) & 63];
-                acc += h.x * w.x - h.y * w.y;
+#pragma unroll
+    for (int my = 0; my < 2; my++) {
+        floatx16 out[2];
+        out[0] = floatx16{0};
+        out[1] = floatx16{0};
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int j = 32 * mt + 8 * (e >> 2) + 4 * hl + (e & 3);
+                const float2 v = tw[((j - 32) * (32 * my + l32)) & 63];
+#pragma unroll
+                for (int nx = 0; nx < 2; nx++) {
+                    out[nx] = __builtin_amdgcn_mfma_f32_32x32x2f32(v.x, acc[mt][nx][e], out[nx], 0, 0, 0);
+                    out[nx] = __builtin_amdgcn_mfma_f32_32x32x2f32(-v.y, acc[mt][2 + nx][e], out[nx], 0, 0, 0);
+                }
             }
-            v[q] = acc;
-            s1 += acc;
-            s2 += acc * acc;
-        }
+#pragma unroll
+        for (int nx = 0; nx < 2; nx++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const float v = out[nx][e];
+                s1 += v;
+                s2 += v * v;
+                const int y = 32 * my + 8 * (e >> 2) + 4 * hl + (e & 3);
+                g[y * 64 + 32 * nx + l32] = v;
+            }
     }
-    red[0][tid] = s1;
-    red[1][tid] = s2;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) {
-            red[0][tid] += red[0][tid + s];
-            red[1][tid] += red[1][tid + s];
-        }
-        __syncthreads();
+
+    // ---- normalisation over the 4096 pixels of the wave's image
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
     }
-    const float mean = red[0][0] / 4096.f;
-    const float var = fmaxf(red[1][0] / 4096.f - mean * mean, 1e-20f);
+    const float mean = s1 / 4096.f;
+    const float var = fmaxf(s2 / 4096.f - mean * mean, 1e-20f);
     const float inv = rsqrtf(var);
+    float4* im = reinterpret_cast<float4*>(img + (size_t)p * 4096);
+    const float4* src = reinterpret_cast<const float4*>(g);
+#pragma unroll
     for (int q = 0; q < 16; q++) {
-        const int pix = ((tid >> 6) + 4 * q) * 64 + (tid & 63);
-        img[(size_t)p * 4096 + pix] = (v[q] - mean) * inv;
+        float4 v = src[64 * q + lane];
+        v.x = (v.x - mean) * inv;
+        v.y = (v.y - mean) * inv;
+        v.z = (v.z - mean) * inv;
+        v.w = (v.w - mean) * inv;
+        im[64 * q + lane] = v;
     }
 }
 
@@ -139,7 +201,6 @@ __global__ __launch_bounds__(256) void k_images(const float2* __restrict__ dat,
 // the k range (partial sums go to the store functor, which then adds
 // atomically).  Operand functors return A(m, k) / B(n, k) in range only.
 // ---------------------------------------------------------------------------
-typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 template <int WM, int WN, bool BKF, class LA, class LB, class ST>
 __global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, ST st, int M, int N, int Kd,
@@ -658,7 +719,7 @@ int cnn_propose_async(thx_cnn* n, thx_ctx* c, double* mixDev, hipEvent_t ready,
     const int BI = std::min(kInferBatch, c->P);
     THX_TRY(ensure_infer(n, BI));
     THX_HIP(hipStreamWaitEvent(n->stream, ready, 0));
-    hipLaunchKernelGGL(k_images, dim3(c->P), dim3(256), 0, n->stream, c->dat.as<const float2>(),
+    hipLaunchKernelGGL(k_images, dim3((c->P + 3) / 4), dim3(256), 0, n->stream, c->dat.as<const float2>(),
                        c->ctf.as<const float>(), c->iCol.as<const int>(), c->iRow.as<const int>(),
                        c->K, c->P, c->img.as<float>());
     THX_HIP(hipGetLastError());
@@ -838,7 +899,7 @@ int thx_ctx_images(thx_ctx* c, float* host) {
     THX_CHECK(c && c->P > 0 && c->K > 0, THX_ERR_STATE, "ctx_images: no particles");
     THX_HIP(hipSetDevice(c->device));
     THX_TRY(c->img.ensure(sizeof(float) * 4096 * (size_t)c->P));
-    hipLaunchKernelGGL(k_images, dim3(c->P), dim3(256), 0, c->stream, c->dat.as<const float2>(),
+    hipLaunchKernelGGL(k_images, dim3((c->P + 3) / 4), dim3(256), 0, c->stream, c->dat.as<const float2>(),
                        c->ctf.as<const float>(), c->iCol.as<const int>(), c->iRow.as<const int>(),
                        c->K, c->P, c->img.as<float>());
     THX_HIP(hipGetLastError());
