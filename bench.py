@@ -79,9 +79,11 @@ def prepare_learned(name, step, train_samples, cal_particles, rule=""):
     tctx.close()
     train = dict(seconds=train_s, samples=train_samples, last_losses=losses)
     if rule:
-        nr, nt, sm, tm, dz = rule.split(",")
+        f = rule.split(",")
+        nr, nt, sm, tm, dz = f[:5]
         cal = dict(n_rot=int(nr), n_trans=int(nt), spread_max=float(sm), tail_max=float(tm),
-                   score_drop=float(dz), calibrated=False)
+                   score_drop=float(dz), fallback=f[5] if len(f) > 5 else "bf16",
+                   calibrated=False)
         return net, train, cal
     # held-out calibration particles: same generator, unseen seed
     cctx = thx.Context(0)
@@ -103,7 +105,7 @@ def prepare_learned(name, step, train_samples, cal_particles, rule=""):
 
 def run_learned(ctx, net, cal, steps, warmup, P, R, T):
     args = (net, 0, cal["n_rot"], cal["n_trans"], 0.0, cal["spread_max"], cal["tail_max"],
-            cal["score_drop"], thx.BF16)
+            cal["score_drop"], thx.PRECISIONS[cal["fallback"]])
     for _ in range(warmup):
         ctx.expect_learned(*args)
     ctx.sync()
@@ -342,9 +344,10 @@ def main():
     ap.add_argument("--mixed", default="bf16", choices=["bf16", "fp16"])
     ap.add_argument("--particles", type=int, default=0)
     ap.add_argument("--train-samples", type=int, default=31_457_280)
-    ap.add_argument("--cal-particles", type=int, default=2000)
+    ap.add_argument("--cal-particles", type=int, default=8000)
     ap.add_argument("--rule", default="",
-                    help="n_rot,n_trans,spread_max,tail_max,score_drop: skip the calibration")
+                    help="n_rot,n_trans,spread_max,tail_max,score_drop[,bf16|fp8 fallback]: "
+                         "skip the calibration")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--local-phases", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -423,10 +426,10 @@ def main():
         r = run_learned(ctx, net, cal, args.steps, args.warmup, P, R, T)
         bR, bT, sc, ess, spread, tail, fell = ctx.get_learned()
         log(f"learned: {r['ms_per_step']:.1f} ms/step {r['kernels_ms']}")
-        dtype = "bf16"
+        dtype = "bf16" if cal["fallback"] == "bf16" else f"bf16 ({cal['fallback']} fallback scan)"
         form = (f"mixed + learned proposal: {cal['n_rot']} rotations x {cal['n_trans']} "
-                f"translations per particle from the grid, BF16 cross term, whole-grid BF16 "
-                f"fallback (calibrated rule)")
+                f"translations per particle from the grid, BF16 cross term, whole-grid "
+                f"{cal['fallback'].upper()} fallback (calibrated rule)")
         nsel = float(P) * cal["n_rot"] * cal["n_trans"]
         # cross term 4K per scored pose + norm term 2K per (particle, rotation),
         # both computed inside the kernel

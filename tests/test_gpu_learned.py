@@ -80,13 +80,17 @@ def test_whole_selection_matches_oracle(gpu_ctx, setup):
     assert not fell.any()
 
 
-def test_forced_fallback_equals_global_bf16(gpu_ctx, setup):
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_forced_fallback_equals_global_scan(gpu_ctx, setup, prec):
+    """Every particle flagged: the fallback is the whole-grid scan of that
+    precision, bit for bit."""
     info, dvp, net = setup
-    nf = gpu_ctx.expect_learned(net, 0, 32, 16, spread_max=0.5, fallback_precision=thx.BF16)
+    nf = gpu_ctx.expect_learned(net, 0, 32, 16, spread_max=0.5,
+                                fallback_precision=thx.PRECISIONS[prec])
     assert nf == P
     bR, bT, sc, ess, spread, tail, fell = gpu_ctx.get_learned()
     assert fell.all()
-    gpu_ctx.score_global(0, thx.BF16)
+    gpu_ctx.score_global(0, thx.PRECISIONS[prec])
     gR, gT, gS = gpu_ctx.get_best(0)
     np.testing.assert_array_equal(bR, gR)
     np.testing.assert_array_equal(bT, gT)

@@ -157,7 +157,9 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
     THX_TRY(ensure_scan_buffers(c, cls));
     if (precision == THX_FP16 || precision == THX_FP8) {
         THX_TRY(c->shellScale.ensure(sizeof(float) * (c->maxShell + 2)));   // + global factor
+        const void* was = c->shellMax.p;
         THX_TRY(c->shellMax.ensure(sizeof(float) * 2 * (c->maxShell + 1)));
+        if (c->shellMax.p != was) c->shellMaxPOk = false;
         THX_TRY(launch_shell_scale(c, precision));
         THX_TRY(record(c, "shell_scale"));
     }
@@ -402,6 +404,8 @@ int thx_ctx_set_pixels(thx_ctx* c, const int* iCol, const int* iRow, const int* 
     c->maxShell = maxShell;
     c->P = 0;   // particle arrays depend on K
     c->R = c->T = 0;
+    c->rotLoPrec = -1;   // tables derived from the projections are stale
+    c->shellMaxPOk = false;
     return THX_OK;
 }
 
@@ -1577,6 +1581,7 @@ int thx_ExpectGlobal3D(const float* rotP, const float* traP, const float* datP, 
     // projections and phases into the padded layouts (zero padding)
     THX_HIP(hipMemset(c->rotP.p, 0, sizeof(float2) * (size_t)c->Rpad * c->Kp));
     c->rotLoPrec = -1;
+    c->shellMaxPOk = false;
     THX_HIP(hipMemset(c->S.p, 0, sizeof(float2) * (size_t)c->TpadS * c->Kp));
     THX_HIP(hipMemcpy2D(c->rotP.p, sizeof(float2) * c->Kp, rotP, sizeof(float2) * npxl,
                         sizeof(float2) * npxl, nR, hipMemcpyHostToDevice));

@@ -162,6 +162,7 @@ struct thx_ctx {
     thx::DevBuf normPart;   // float [ns][Ppad][Rpad]: pixel-range slabs of Cterm
     thx::DevBuf shellScale; // float [maxShell+1]
     thx::DevBuf shellMax;   // float [2][maxShell+1] (scratch of the scale pass)
+    bool shellMaxPOk = false;   // shellMax[0..maxShell] holds the current slices' maxima
     thx::DevBuf partMax, partWC, partArg, partWR, partWT;
     std::vector<thx::ClassResult> res;
     // ragged local-search batch (inputs, partials, results)

@@ -185,7 +185,10 @@ __global__ __launch_bounds__(256) void k_project(
 
 int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
                    float2* out, float* outNorm) {
-    if (out == c->rotP.as<float2>()) c->rotLoPrec = -1;   // the 16-bit table is stale
+    if (out == c->rotP.as<float2>()) {   // the 16-bit table and the slice maxima are stale
+        c->rotLoPrec = -1;
+        c->shellMaxPOk = false;
+    }
     dim3 grid((c->Kp + 255) / 256, Rpad);
     hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c->stream,
                        c->vol[cls].as<const float2>(), c->vdim[cls], c->pf, dQuat, R,
@@ -291,16 +294,29 @@ __device__ inline void atomic_max_pos(float* addr, float v) {
     atomicMax(reinterpret_cast<int*>(addr), __float_as_int(v));
 }
 
-__global__ void k_shell_max(const float2* __restrict__ src, int rows, int Kp, int K,
-                            const int* __restrict__ iSig, float* __restrict__ shellMax) {
+// shellMax[s] = max |re|, |im| over rows and the pixels of shell s: each
+// thread a pixel column over every gridDim.y-th row, then per-shell maxima
+// in LDS and one global atomic per touched shell and block.
+constexpr int kMaxShellLds = 1024;
+__global__ __launch_bounds__(256) void k_shell_max(const float2* __restrict__ src, int rows,
+                                                   int Kp, int K, const int* __restrict__ iSig,
+                                                   int nShell, float* __restrict__ shellMax) {
+    __shared__ float sm[kMaxShellLds];
+    for (int e = threadIdx.x; e < nShell; e += blockDim.x) sm[e] = 0.f;
+    __syncthreads();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= K) return;
-    float m = 0.f;
-    for (int r = blockIdx.y; r < rows; r += gridDim.y) {
-        const float2 v = src[(size_t)r * Kp + k];
-        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    if (k < K) {
+        float m = 0.f;
+#pragma unroll 8
+        for (int r = blockIdx.y; r < rows; r += gridDim.y) {
+            const float2 v = src[(size_t)r * Kp + k];
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+        atomic_max_pos(&sm[iSig[k]], m);
     }
-    atomic_max_pos(&shellMax[iSig[k]], m);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nShell; e += blockDim.x)
+        if (sm[e] > 0.f) atomic_max_pos(&shellMax[e], sm[e]);
 }
 
 // Per-shell power-of-two scales a_s = 2^rint(log2 sqrt(maxP_s / maxY_s)): Y is
@@ -375,13 +391,19 @@ __global__ void k_convert_lo(const float2* __restrict__ src, int rows, int Kp,
 int launch_shell_scale(thx_ctx* c, int precision) {
     const int nShell = c->maxShell + 1;
     if (precision != THX_FP16 && precision != THX_FP8) return THX_OK;
-    THX_HIP(hipMemsetAsync(c->shellMax.p, 0, sizeof(float) * 2 * nShell, c->stream));
+    THX_CHECK(nShell <= kMaxShellLds, THX_ERR_RANGE, "shell scale: too many shells");
     float* maxP = c->shellMax.as<float>();
     float* maxY = maxP + nShell;
-    dim3 gridP((c->K + 255) / 256, 64);
-    hipLaunchKernelGGL(k_shell_max, gridP, dim3(256), 0, c->stream, c->rotP.as<const float2>(),
-                       c->Rpad, c->Kp, c->K, c->iSig.as<const int>(), maxP);
-    THX_HIP(hipGetLastError());
+    // the slices' maxima depend on the projections only: kept while current
+    if (!c->shellMaxPOk) {
+        THX_HIP(hipMemsetAsync(maxP, 0, sizeof(float) * nShell, c->stream));
+        dim3 gridP((c->K + 255) / 256, 16);
+        hipLaunchKernelGGL(k_shell_max, gridP, dim3(256), 0, c->stream, c->rotP.as<const float2>(),
+                           c->Rpad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxP);
+        THX_HIP(hipGetLastError());
+        c->shellMaxPOk = true;
+    }
+    THX_HIP(hipMemsetAsync(maxY, 0, sizeof(float) * nShell, c->stream));
     // Y with unit scale: run the prep once without scales, take its max.
     hipLaunchKernelGGL(k_prep, dim3(c->Ppad), dim3(256), 0, c->stream,
                        c->dat.as<const float2>(), c->ctf.as<const float>(),
@@ -389,9 +411,9 @@ int launch_shell_scale(thx_ctx* c, int precision) {
                        (const float*)nullptr, c->Aterm.as<double>(), c->Y.as<float2>(),
                        c->Wt.as<float>());
     THX_HIP(hipGetLastError());
-    dim3 gridY((c->K + 255) / 256, 64);
+    dim3 gridY((c->K + 255) / 256, 16);
     hipLaunchKernelGGL(k_shell_max, gridY, dim3(256), 0, c->stream, c->Y.as<const float2>(),
-                       c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), maxY);
+                       c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxY);
     THX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_shell_finish, dim3(1), dim3(256), 0, c->stream, maxP, maxY, nShell,
                        precision == THX_FP8 ? 1 : 0, c->shellScale.as<float>());
@@ -437,6 +459,7 @@ int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used) {
     THX_TRY(c->shellMax.ensure(sizeof(float) * 2 * nShell));
     THX_TRY(c->shellScale.ensure(sizeof(float) * (nShell + 1)));
     THX_HIP(hipMemsetAsync(c->shellMax.p, 0, sizeof(float) * 2 * nShell, c->stream));
+    c->shellMaxPOk = false;   // maxP now holds volume bounds, not the slices' maxima
     float* maxP = c->shellMax.as<float>();
     float* maxY = maxP + nShell;
     for (size_t k = 0; k < used.size(); k++) {
@@ -451,9 +474,9 @@ int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used) {
                        (const float*)nullptr, c->Aterm.as<double>(), c->Y.as<float2>(),
                        c->Wt.as<float>());
     THX_HIP(hipGetLastError());
-    dim3 gridY((c->K + 255) / 256, 64);
+    dim3 gridY((c->K + 255) / 256, 16);
     hipLaunchKernelGGL(k_shell_max, gridY, dim3(256), 0, c->stream, c->Y.as<const float2>(),
-                       c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), maxY);
+                       c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxY);
     THX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_shell_finish, dim3(1), dim3(256), 0, c->stream, maxP, maxY, nShell, 0,
                        c->shellScale.as<float>());

@@ -54,18 +54,25 @@ TAIL = [4.0, 5.0, 6.0, 0.0]
 DROP = [100.0, 150.0, 0.0]
 
 
-def calibrate(ctx, net, step, target=0.99, margin=0.003, candidates=CANDIDATES, log=None):
-    """Choose selection sizes and fallback thresholds on the held-out set in
-    `ctx` (particles, grid and class-0 volume set).  Returns a dict with the
-    chosen n_rot, n_trans, spread_max, tail_max, score_drop and the
-    calibration statistics."""
+def calibrate(ctx, net, step, target=0.99, margin=0.003, candidates=CANDIDATES, log=None,
+              fallbacks=("bf16", "fp8")):
+    """Choose selection sizes, fallback thresholds and the fallback scan's
+    precision on the held-out set in `ctx` (particles, grid and class-0
+    volume set).  A particle that falls back is counted correct when that
+    precision's whole-grid scan agrees with the plain FP32 scan on it.
+    Returns a dict with the chosen n_rot, n_trans, spread_max, tail_max,
+    score_drop, fallback and the calibration statistics."""
     quat = ctx.grid_quat
     P = ctx.P
     ctx.score_global(0, thx.FP32)
     gR, _, _ = ctx.get_best(0)
-    ctx.score_global(0, thx.BF16)
-    ctx.sync()
-    t_full = sum(ctx.last_timing().values()) / P        # ms per particle, whole grid
+    full = {}   # precision -> (ms per particle of the whole-grid scan, agreement per particle)
+    for name in fallbacks:
+        ctx.score_global(0, thx.PRECISIONS[name])
+        ctx.sync()
+        t = sum(ctx.last_timing().values()) / P
+        fR, _, _ = ctx.get_best(0)
+        full[name] = (t, wl.geodesic_angle(quat[fR], quat[gR]) <= step * (1 + 1e-6))
     best = None
     for nr, nt in candidates:
         ctx.expect_learned(net, 0, nr, nt)
@@ -84,19 +91,23 @@ def calibrate(ctx, net, step, target=0.99, margin=0.003, candidates=CANDIDATES, 
                         fb |= tail > tmx
                     if dz > 0:
                         fb |= sc < med - dz
-                    acc = float(np.mean(ok | fb))
-                    cost = t_sel + fb.mean() * t_full
-                    if acc >= target + margin and (best is None or cost < best["pred_ms_per_particle"]):
-                        best = dict(n_rot=nr, n_trans=nt, spread_max=sm, tail_max=tmx,
-                                    score_drop=dz, cal_accuracy=acc, cal_fallback=float(fb.mean()),
-                                    cal_accuracy_no_fallback=float(ok.mean()),
-                                    pred_ms_per_particle=cost)
+                    for name, (t_full, ok_full) in full.items():
+                        acc = float(np.mean(np.where(fb, ok_full, ok)))
+                        cost = t_sel + fb.mean() * t_full
+                        if acc >= target + margin and (best is None or cost < best["pred_ms_per_particle"]):
+                            best = dict(n_rot=nr, n_trans=nt, spread_max=sm, tail_max=tmx,
+                                        score_drop=dz, fallback=name, cal_accuracy=acc,
+                                        cal_fallback=float(fb.mean()),
+                                        cal_accuracy_no_fallback=float(ok.mean()),
+                                        pred_ms_per_particle=cost)
         if log:
             log(f"calibrate {nr}x{nt}: no-fallback agreement {ok.mean():.4f}")
     if best is None:   # nothing meets the target: every particle falls back
         best = dict(n_rot=candidates[0][0], n_trans=candidates[0][1], spread_max=0.5,
-                    tail_max=0.0, score_drop=0.0, cal_accuracy=1.0, cal_fallback=1.0,
-                    cal_accuracy_no_fallback=float("nan"), pred_ms_per_particle=t_full)
+                    tail_max=0.0, score_drop=0.0, fallback=fallbacks[0], cal_accuracy=float("nan"),
+                    cal_fallback=1.0, cal_accuracy_no_fallback=float("nan"),
+                    pred_ms_per_particle=full[fallbacks[0]][0])
+    best["whole_grid_agreement"] = {k: float(np.mean(v[1])) for k, v in full.items()}
     best["cal_particles"] = P
     best["target"] = target
     return best
