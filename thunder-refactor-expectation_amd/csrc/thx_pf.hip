@@ -362,7 +362,7 @@ __global__ __launch_bounds__(256) void k_select_grid(
     float* __restrict__ rlogq, int* __restrict__ tidx, float* __restrict__ spread) {
     const int p = blockIdx.x;
     const double* mp = mix + (size_t)p * (6 * C + 3);
-    __shared__ float v[kSelMaxR];
+    extern __shared__ float v[];   // [R]: sized at launch
     __shared__ int red[256];
     __shared__ int cnt;
     __shared__ float tv[512];
@@ -432,28 +432,37 @@ __global__ __launch_bounds__(256) void k_select_grid(
                &cnt);
     // translations: Gaussian log density (the constant does not matter)
     const double* sh = mp + 6 * C;
-    float tmx = -INFINITY, tmn = INFINITY;
     for (int t = threadIdx.x; t < T; t += blockDim.x) {
         const double dx = gridT[2 * t] - sh[0], dy = gridT[2 * t + 1] - sh[1];
-        const float lv = (float)(-(dx * dx + dy * dy) / (2 * sh[2] * sh[2]));
-        tv[t] = lv;
-        tmx = fmaxf(tmx, lv);
-        tmn = fminf(tmn, lv);
+        tv[t] = (float)(-(dx * dx + dy * dy) / (2 * sh[2] * sh[2]));
     }
-    rmx[threadIdx.x] = tmx;
-    rmn[threadIdx.x] = tmn;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            rmx[threadIdx.x] = fmaxf(rmx[threadIdx.x], rmx[threadIdx.x + s]);
-            rmn[threadIdx.x] = fminf(rmn[threadIdx.x], rmn[threadIdx.x + s]);
+    // exact top-nSelT by rank (value descending, ties by index), listed in
+    // index order: T is small, so each thread ranks its translations against
+    // all of them instead of bisecting
+    int base = 0;
+    for (int t0 = 0; t0 < T; t0 += blockDim.x) {
+        const int t = t0 + threadIdx.x;
+        bool take = false;
+        if (t < T) {
+            const float x = tv[t];
+            int rank = 0;
+            for (int u = 0; u < T; u++) {
+                const float y = tv[u];
+                rank += (y > x) || (y == x && u < t);
+            }
+            take = rank < nSelT;
         }
+        const unsigned long long m = __ballot(take);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        red[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; w++) off += red[w];
+        if (take) tidx[(size_t)p * nSelT + off + __popcll(m & ((1ull << lane) - 1))] = t;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) base += red[w];
         __syncthreads();
     }
-    const float hiT = rmx[0], loT = rmn[0];
-    __syncthreads();
-    __shared__ float tdummy[512];
-    block_topk(tv, T, nSelT, tidx + (size_t)p * nSelT, tdummy, loT, hiT, red, &cnt);
 }
 
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,
@@ -462,8 +471,12 @@ int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double
     THX_CHECK(C >= 1 && C <= 16 && R <= kSelMaxR && T <= 512 && nSelR <= R && nSelT <= T,
               THX_ERR_RANGE, "select_grid: sizes out of range");
     hipLaunchKernelGGL(k_quat_to_f4, dim3((R + 255) / 256), dim3(256), 0, st, gridQ, R, gridQf);
-    hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), 0, st, mix, C, gridQf, R, gridT, T,
-                       nSelR, nSelT, ridx, rlogq, tidx, spread);
+    if (sizeof(float) * R > 32768)
+        THX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_select_grid),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(sizeof(float) * R)));
+    hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), sizeof(float) * R, st, mix, C, gridQf,
+                       R, gridT, T, nSelR, nSelT, ridx, rlogq, tidx, spread);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
