@@ -6,8 +6,10 @@
   oracle's best pose, and agree with that pose for >= 90% of particles (the
   bar of the BF16 global scan).
 * Forced fallback (spread threshold below 1, which every proposal exceeds):
-  each particle is rescored by the tiled whole-grid BF16 scan on gathered
-  rows, whose results must equal those of the global BF16 scan bit for bit.
+  each particle is rescored by the tiled whole-grid scan (BF16 or FP8) on
+  gathered rows, whose results must equal those of the global scan bit for
+  bit; with about half the particles flagged, their results must equal a
+  fresh context's scan of just those particles, bit for bit.
 * Selection and statistics from the network's mixture, against numpy FP64:
   the best rotation ranks within the nSelR most probable grid rotations; the
   tail depth (max log q - log q(best)) agrees to 1e-3 absolute and the spread
@@ -95,6 +97,35 @@ def test_forced_fallback_equals_global_scan(gpu_ctx, setup, prec):
     np.testing.assert_array_equal(bR, gR)
     np.testing.assert_array_equal(bT, gT)
     np.testing.assert_array_equal(sc, gS)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_partial_fallback_equals_subset_scan(gpu_ctx, setup, prec):
+    """About half the particles flagged: their fallback results equal, bit for
+    bit, a whole-grid scan of a fresh context holding only those particles.
+    This holds the pieces the fallback reuses from the selection step (the
+    BF16 slice table, the slices' per-shell maxima) and its split-K norm term
+    to what a cold scan computes."""
+    info, dvp, net = setup
+    gpu_ctx.expect_learned(net, 0, 32, 16)
+    spread = gpu_ctx.get_learned()[4]
+    thr = float(np.median(spread))
+    nf = gpu_ctx.expect_learned(net, 0, 32, 16, spread_max=thr,
+                                fallback_precision=thx.PRECISIONS[prec])
+    bR, bT, sc, ess, spread2, tail, fell = gpu_ctx.get_learned()
+    idx = np.flatnonzero(fell)
+    assert nf == len(idx) and 0 < nf < P
+    dat, ctf, sig = gpu_ctx.get_particles()
+    with thx.Context(0) as sub:
+        sub.set_pixels(info["iCol"], info["iRow"], info["iSig"], BOX, PF)
+        sub.set_volume(0, gpu_ctx.get_volume(0, V), V)
+        sub.set_particles(dat[idx], ctf[idx], sig[idx])
+        sub.set_grid(info["quat"], info["trans"])
+        sub.score_global(0, thx.PRECISIONS[prec])
+        gR, gT, gS = sub.get_best(0)
+    np.testing.assert_array_equal(bR[idx], gR)
+    np.testing.assert_array_equal(bT[idx], gT)
+    np.testing.assert_array_equal(sc[idx], gS)
 
 
 def test_selection_and_statistics(gpu_ctx, setup):
