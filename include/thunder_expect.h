@@ -365,6 +365,11 @@ int thx_cnn_train_ctx(thx_cnn* net, thx_ctx* ctx, const double* q, const double*
  * m2 m3 k pi) then (mu_x mu_y sigma); host and/or device destination. */
 int thx_cnn_propose_ctx(thx_cnn* net, thx_ctx* ctx, double* mixHost, double* mixDev);
 int thx_cnn_components(thx_cnn* net, int* C);
+/* Operand precision of the proposal passes (thx_cnn_propose_ctx and the
+ * learned expectation): THX_BF16 (default; FP32 accumulation) or THX_FP32.
+ * Training and thx_cnn_forward always use FP32.  No counterpart in the
+ * reference (it has no learned proposal). */
+int thx_cnn_set_precision(thx_cnn* net, int precision);
 
 /* Expectation with the learned proposal over the context's grid: the grid's
  * slices are projected once (BF16 table + FP32 |P|^2), the network's mixture

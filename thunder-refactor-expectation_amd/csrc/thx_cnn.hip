@@ -200,9 +200,13 @@ __global__ __launch_bounds__(256) void k_images(const float2* __restrict__ dat,
 // im2col) or along k (BKF true, for k-contiguous rows).  blockIdx.z splits
 // the k range (partial sums go to the store functor, which then adds
 // atomically).  Operand functors return A(m, k) / B(n, k) in range only.
+// With BF (inference only) each 16-deep step is one v_mfma_f32_32x32x16_bf16
+// on operands rounded to BF16 (FP32 accumulation) instead of eight FP32
+// MFMAs: lane (m, h) takes k = 8 h .. 8 h + 7 of the step.
 // ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-template <int WM, int WN, bool BKF, class LA, class LB, class ST>
+template <int WM, int WN, bool BKF, class LA, class LB, class ST, bool BF = false>
 __global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, ST st, int M, int N, int Kd,
                                               int kPer) {
     static_assert(WM * WN == 4, "four waves per block");
@@ -245,11 +249,21 @@ __global__ __launch_bounds__(256) void k_gemm(LA la, LB lb, ST st, int M, int N,
         }
         __syncthreads();
         if (k0 + BK < ke) load(k0 + BK);
+        if constexpr (BF) {
+            bf16x8_t a, b;
 #pragma unroll
-        for (int s = 0; s < BK; s += 2) {
-            const float a = As[s + (lane >> 5)][32 * wm + (lane & 31)];
-            const float b = Bs[s + (lane >> 5)][32 * wn + (lane & 31)];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            for (int e = 0; e < 8; e++) {
+                a[e] = (__bf16)As[8 * (lane >> 5) + e][32 * wm + (lane & 31)];
+                b[e] = (__bf16)Bs[8 * (lane >> 5) + e][32 * wn + (lane & 31)];
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int s = 0; s < BK; s += 2) {
+                const float a = As[s + (lane >> 5)][32 * wm + (lane & 31)];
+                const float b = Bs[s + (lane >> 5)][32 * wn + (lane & 31)];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            }
         }
     }
     const int n = n0 + 32 * wn + (lane & 31);
@@ -397,12 +411,12 @@ __global__ void k_split_reduce(const float* __restrict__ part, int nz, int CO, i
     if (n == KC) db[m] = s;
     else dw[(size_t)m * KC + n] = s;
 }
-template <int WM, int WN, bool BKF, class LA, class LB, class ST>
+template <int WM, int WN, bool BKF, class LA, class LB, class ST, bool BF = false>
 static int gemm(hipStream_t st, LA la, LB lb, ST sto, int M, int N, int Kd, int splits = 1) {
     constexpr int BM = 32 * WM, BN = 32 * WN;
     const int kPer = ((Kd + splits - 1) / splits + 15) / 16 * 16;
     const int nz = (Kd + kPer - 1) / kPer;
-    hipLaunchKernelGGL((k_gemm<WM, WN, BKF, LA, LB, ST>),
+    hipLaunchKernelGGL((k_gemm<WM, WN, BKF, LA, LB, ST, BF>),
                        dim3((N + BN - 1) / BN, (M + BM - 1) / BM, nz), dim3(256), 0, st, la, lb,
                        sto, M, N, Kd, kPer);
     THX_HIP(hipGetLastError());
@@ -561,6 +575,7 @@ struct thx_cnn {
     DevBuf loss, qt, tt, mix, imgs;
     DevBuf wpart;            // split-K partial weight gradients
     long step = 0;
+    int inferPrec = THX_BF16;   // operand precision of the proposal passes
 };
 
 static size_t conv_w(int l) {
@@ -616,30 +631,35 @@ static float* G_(thx_cnn* n, int i) { return n->grads.as<float>() + n->off[i]; }
 
 static unsigned nblk(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-template <int L, int WM, int WN>
+template <int L, int WM, int WN, bool BF>
 static int conv_fwd(hipStream_t st, const float* in, const float* w, const float* bias, float* out,
                     int B) {
     using G = Geo<L>;
-    return gemm<WM, WN, false>(st, RowOp{w, G::CI * G::KS * G::KS}, Im2col<L>{in},
-                               ConvStore<L>{out, bias}, G::CO, B * G::HO * G::HO,
-                               G::CI * G::KS * G::KS);
+    return gemm<WM, WN, false, RowOp, Im2col<L>, ConvStore<L>, BF>(
+        st, RowOp{w, G::CI * G::KS * G::KS}, Im2col<L>{in}, ConvStore<L>{out, bias}, G::CO,
+        B * G::HO * G::HO, G::CI * G::KS * G::KS);
 }
 
-// forward of B images at `in` (act[0] when null)
-static int cnn_forward(thx_cnn* n, int B, const float* in = nullptr) {
+// forward of B images at `in` (act[0] when null); BF: BF16 operands
+template <bool BF>
+static int cnn_forward_p(thx_cnn* n, int B, const float* in) {
     hipStream_t st = n->stream;
     const float* x = in ? in : n->act[0].as<const float>();
-    THX_TRY((conv_fwd<0, 1, 4>(st, x, P_(n, 0), P_(n, 1), n->act[1].as<float>(), B)));
-    THX_TRY((conv_fwd<1, 1, 4>(st, n->act[1].as<float>(), P_(n, 2), P_(n, 3), n->act[2].as<float>(), B)));
-    THX_TRY((conv_fwd<2, 2, 2>(st, n->act[2].as<float>(), P_(n, 4), P_(n, 5), n->act[3].as<float>(), B)));
-    THX_TRY((conv_fwd<3, 2, 2>(st, n->act[3].as<float>(), P_(n, 6), P_(n, 7), n->act[4].as<float>(), B)));
-    THX_TRY((gemm<2, 2, true>(st, RowOp{n->act[4].as<float>(), kFcIn}, RowOp{P_(n, 8), kFcIn},
-                              FcStore{n->act[5].as<float>(), P_(n, 9), kFcHid, 1}, B, kFcHid,
-                              kFcIn)));
-    THX_TRY((gemm<2, 2, true>(st, RowOp{n->act[5].as<float>(), kFcHid}, RowOp{P_(n, 10), kFcHid},
-                              FcStore{n->act[6].as<float>(), P_(n, 11), n->NOUT, 0}, B, n->NOUT,
-                              kFcHid)));
+    THX_TRY((conv_fwd<0, 1, 4, BF>(st, x, P_(n, 0), P_(n, 1), n->act[1].as<float>(), B)));
+    THX_TRY((conv_fwd<1, 1, 4, BF>(st, n->act[1].as<float>(), P_(n, 2), P_(n, 3), n->act[2].as<float>(), B)));
+    THX_TRY((conv_fwd<2, 2, 2, BF>(st, n->act[2].as<float>(), P_(n, 4), P_(n, 5), n->act[3].as<float>(), B)));
+    THX_TRY((conv_fwd<3, 2, 2, BF>(st, n->act[3].as<float>(), P_(n, 6), P_(n, 7), n->act[4].as<float>(), B)));
+    THX_TRY((gemm<2, 2, true, RowOp, RowOp, FcStore, BF>(
+        st, RowOp{n->act[4].as<float>(), kFcIn}, RowOp{P_(n, 8), kFcIn},
+        FcStore{n->act[5].as<float>(), P_(n, 9), kFcHid, 1}, B, kFcHid, kFcIn)));
+    THX_TRY((gemm<2, 2, true, RowOp, RowOp, FcStore, BF>(
+        st, RowOp{n->act[5].as<float>(), kFcHid}, RowOp{P_(n, 10), kFcHid},
+        FcStore{n->act[6].as<float>(), P_(n, 11), n->NOUT, 0}, B, n->NOUT, kFcHid)));
     return THX_OK;
+}
+
+static int cnn_forward(thx_cnn* n, int B, const float* in = nullptr, bool bf16 = false) {
+    return bf16 ? cnn_forward_p<true>(n, B, in) : cnn_forward_p<false>(n, B, in);
 }
 
 template <int L>
@@ -725,7 +745,8 @@ int cnn_propose_async(thx_cnn* n, thx_ctx* c, double* mixDev, hipEvent_t ready,
     THX_HIP(hipGetLastError());
     for (int b0 = 0; b0 < c->P; b0 += BI) {
         const int B = std::min(BI, c->P - b0);
-        THX_TRY(cnn_forward(n, B, c->img.as<float>() + (size_t)b0 * 4096));
+        THX_TRY(cnn_forward(n, B, c->img.as<float>() + (size_t)b0 * 4096,
+                            n->inferPrec == THX_BF16));
         hipLaunchKernelGGL(k_head, dim3(nblk(B, 64)), dim3(64), 0, n->stream,
                            n->act[6].as<const float>(), B, n->C, n->mix.as<double>());
         THX_HIP(hipGetLastError());
@@ -798,6 +819,15 @@ int thx_cnn_destroy(thx_cnn* n) {
 int thx_cnn_components(thx_cnn* n, int* C) {
     THX_CHECK(n && C, THX_ERR_ARG, "cnn_components: null argument");
     *C = n->C;
+    return THX_OK;
+}
+
+int thx_cnn_set_precision(thx_cnn* n, int precision) {
+    clear_error();
+    THX_CHECK(n, THX_ERR_ARG, "cnn_set_precision: null network");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
+              "cnn_set_precision: FP32 or BF16");
+    n->inferPrec = precision;
     return THX_OK;
 }
 
@@ -938,7 +968,8 @@ int thx_cnn_propose_ctx(thx_cnn* n, thx_ctx* c, double* mixHost, double* mixDev)
     THX_TRY(ensure_infer(n, BI));
     for (int b0 = 0; b0 < c->P; b0 += BI) {
         const int B = std::min(BI, c->P - b0);
-        THX_TRY(cnn_forward(n, B, c->img.as<float>() + (size_t)b0 * 4096));
+        THX_TRY(cnn_forward(n, B, c->img.as<float>() + (size_t)b0 * 4096,
+                            n->inferPrec == THX_BF16));
         hipLaunchKernelGGL(k_head, dim3(nblk(B, 64)), dim3(64), 0, n->stream,
                            n->act[6].as<const float>(), B, n->C, n->mix.as<double>());
         THX_HIP(hipGetLastError());

@@ -65,6 +65,7 @@ _SIGS = {
     "thx_cnn_train_ctx": [_p, _p, _p, _p, _i, _i, C.c_float, _p],
     "thx_cnn_propose_ctx": [_p, _p, _p, _p],
     "thx_cnn_components": [_p, _p],
+    "thx_cnn_set_precision": [_p, C.c_int],
     "thx_ctx_expect_learned": [_p, _p, _i, _i, _i, _d, _d, _d, _d, _i, _p],
     "thx_ctx_get_learned": [_p, _p, _p, _p, _p, _p, _p, _p],
     "thx_pf_set_state": [_p, _i, _i, _p, _p, _p, _p, _p, _p],
@@ -545,6 +546,10 @@ class Proposal:
         check(load().thx_cnn_train_ctx(self._h, ctx._h, ptr(q), ptr(t), batch, epochs, lr,
                                        C.byref(loss)), "cnn_train_ctx")
         return loss.value
+
+    def set_precision(self, precision):
+        """Operand precision of the proposal passes: BF16 (default) or FP32."""
+        check(load().thx_cnn_set_precision(self._h, precision), "cnn_set_precision")
 
     def propose(self, ctx):
         mix = np.zeros((ctx.P, self.NOUT))
