@@ -631,6 +631,64 @@ static float* G_(thx_cnn* n, int i) { return n->grads.as<float>() + n->off[i]; }
 
 static unsigned nblk(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// ---------------------------------------------------------------------------
+// First layer of the proposal passes as a direct convolution (1 -> 16
+// channels, 5 x 5, stride 2, padding 2, bias + ReLU, FP32): one image per
+// block, staged in LDS with a zero border; thread t computes output row
+// t / 8, columns 4 (t % 8) .. + 3 for all 16 channels from the 5 x 11 input
+// patch in registers and the weights read from LDS (the same address for
+// every lane).  The implicit GEMM spends this layer on scattered im2col
+// loads for a 25-deep product.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv0_direct(const float* __restrict__ in,
+                                                      const float* __restrict__ w,
+                                                      const float* __restrict__ bias,
+                                                      float* __restrict__ out) {
+    using G = Geo<0>;
+    static_assert(G::CI == 1 && G::KS == 5 && G::S == 2 && G::PD == 2 && G::HI == 64 &&
+                  G::HO == 32 && G::CO == 16, "layer shape");
+    constexpr int W = G::HI + 2 * G::PD;   // 68
+    __shared__ __attribute__((aligned(16))) float im[W * W];
+    __shared__ float ws[G::CO * 25];
+    __shared__ float bs[G::CO];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    for (int e = tid; e < W * W; e += 256) im[e] = 0.f;
+    for (int e = tid; e < G::CO * 25; e += 256) ws[e] = w[e];
+    if (tid < G::CO) bs[tid] = bias[tid];
+    __syncthreads();
+    const float4* src = reinterpret_cast<const float4*>(in + (size_t)b * 4096);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int e = tid + 256 * q;            // float4 index: row e / 16, columns 4 (e % 16)
+        const float4 v = src[e];
+        float* d = im + (e / 16 + G::PD) * W + 4 * (e % 16) + G::PD;
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    __syncthreads();
+    const int oy = tid >> 3, ox0 = 4 * (tid & 7);
+    float x[5][11];
+#pragma unroll
+    for (int ky = 0; ky < 5; ky++)
+#pragma unroll
+        for (int c = 0; c < 11; c++) x[ky][c] = im[(2 * oy + ky) * W + 2 * ox0 + c];
+    float* o = out + (size_t)b * G::CO * 1024 + oy * 32 + ox0;
+    for (int co = 0; co < G::CO; co++) {
+        float a0 = bs[co], a1 = a0, a2 = a0, a3 = a0;
+#pragma unroll
+        for (int ky = 0; ky < 5; ky++)
+#pragma unroll
+            for (int kx = 0; kx < 5; kx++) {
+                const float wv = ws[co * 25 + ky * 5 + kx];
+                a0 += wv * x[ky][kx];
+                a1 += wv * x[ky][kx + 2];
+                a2 += wv * x[ky][kx + 4];
+                a3 += wv * x[ky][kx + 6];
+            }
+        *reinterpret_cast<float4*>(o + co * 1024) =
+            make_float4(fmaxf(a0, 0.f), fmaxf(a1, 0.f), fmaxf(a2, 0.f), fmaxf(a3, 0.f));
+    }
+}
+
 template <int L, int WM, int WN, bool BF>
 static int conv_fwd(hipStream_t st, const float* in, const float* w, const float* bias, float* out,
                     int B) {
@@ -640,12 +698,19 @@ static int conv_fwd(hipStream_t st, const float* in, const float* w, const float
         B * G::HO * G::HO, G::CI * G::KS * G::KS);
 }
 
-// forward of B images at `in` (act[0] when null); BF: BF16 operands
+// forward of B images at `in` (act[0] when null); BF (the proposal passes):
+// the first layer direct in FP32, the others with BF16 operands
 template <bool BF>
 static int cnn_forward_p(thx_cnn* n, int B, const float* in) {
     hipStream_t st = n->stream;
     const float* x = in ? in : n->act[0].as<const float>();
-    THX_TRY((conv_fwd<0, 1, 4, BF>(st, x, P_(n, 0), P_(n, 1), n->act[1].as<float>(), B)));
+    if constexpr (BF) {
+        hipLaunchKernelGGL(k_conv0_direct, dim3(B), dim3(256), 0, st, x, P_(n, 0), P_(n, 1),
+                           n->act[1].as<float>());
+        THX_HIP(hipGetLastError());
+    } else {
+        THX_TRY((conv_fwd<0, 1, 4, BF>(st, x, P_(n, 0), P_(n, 1), n->act[1].as<float>(), B)));
+    }
     THX_TRY((conv_fwd<1, 1, 4, BF>(st, n->act[1].as<float>(), P_(n, 2), P_(n, 3), n->act[2].as<float>(), B)));
     THX_TRY((conv_fwd<2, 2, 2, BF>(st, n->act[2].as<float>(), P_(n, 4), P_(n, 5), n->act[3].as<float>(), B)));
     THX_TRY((conv_fwd<3, 2, 2, BF>(st, n->act[3].as<float>(), P_(n, 6), P_(n, 7), n->act[4].as<float>(), B)));
