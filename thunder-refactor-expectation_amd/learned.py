@@ -49,8 +49,8 @@ def train_proposal(net, ctx, workload, n_samples=31_457_280, chunk=8192, batch=5
 # translations to 32 / 64 / 128 rows and rotations to blocks of 128 / 256, so
 # the 32 / 64-translation and 128 / 256 / 512-rotation sizes fill the tiles
 # they pay for.
-CANDIDATES = [(128, 32), (128, 64), (192, 64), (256, 64), (300, 49), (500, 49), (512, 64),
-              (256, 121)]
+CANDIDATES = [(128, 32), (256, 32), (512, 32), (128, 64), (192, 64), (256, 64), (300, 49),
+              (500, 49), (512, 64), (256, 121)]
 SPREAD = [30.0, 50.0, 80.0, 0.0]
 TAIL = [4.0, 5.0, 6.0, 0.0]
 DROP = [100.0, 150.0, 0.0]
