@@ -332,6 +332,7 @@ int thx_ctx_destroy(thx_ctx* c) {
     if (c->locGraph) (void)hipGraphExecDestroy(c->locGraph);
     if (c->lpReady) (void)hipEventDestroy(c->lpReady);
     if (c->lpDone) (void)hipEventDestroy(c->lpDone);
+    if (c->lpRead) (void)hipEventDestroy(c->lpRead);
     projectee_release(c);
     c->lpStage.release();
     DevBuf* bufs[] = {&c->iCol, &c->iRow, &c->iSig, &c->dat, &c->ctf, &c->sig, &c->Aterm,
@@ -954,10 +955,17 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
 // Whole-grid scan (run_scan) of the particles idx: their rows are gathered,
 // and the context's particle inputs and this class's result slot are
 // swapped for the gathered rows and lpRes for the duration of the scan.
-static int scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int>& idx) {
+static int scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int>& idx,
+                       const int* pinnedIdx = nullptr) {
     const int n = (int)idx.size();
     const size_t nk = (size_t)n * c->K;
-    THX_TRY(upload(c->lpPmap, idx.data(), n, c->stream));
+    if (pinnedIdx) {   // a pinned copy of idx: the upload does not block the host
+        THX_TRY(c->lpPmap.ensure(sizeof(int) * n));
+        THX_HIP(hipMemcpyAsync(c->lpPmap.p, pinnedIdx, sizeof(int) * n, hipMemcpyHostToDevice,
+                               c->stream));
+    } else {
+        THX_TRY(upload(c->lpPmap, idx.data(), n, c->stream));
+    }
     THX_TRY(c->lpDat.ensure(sizeof(float2) * nk));
     THX_TRY(c->lpCtf.ensure(sizeof(float) * nk));
     THX_TRY(c->lpSig.ensure(sizeof(float) * nk));
@@ -1054,7 +1062,13 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     THX_HIP(hipMemcpyAsync(hT, c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost, st));
     THX_HIP(hipMemcpyAsync(hSpread, c->lpSpread.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
     THX_HIP(hipMemcpyAsync(hTail, c->lpTail.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
-    THX_HIP(hipStreamSynchronize(st));
+    if (!c->lpRead) THX_HIP(hipEventCreateWithFlags(&c->lpRead, hipEventDisableTiming));
+    THX_HIP(hipEventRecord(c->lpRead, st));
+    // a 16- or 8-bit fallback scan needs the slices' per-shell maxima: queued
+    // behind the copies, they run while the host applies the rule
+    if (fallbackPrecision == THX_FP16 || fallbackPrecision == THX_FP8)
+        THX_TRY(ensure_slice_shell_max(c));
+    THX_HIP(hipEventSynchronize(c->lpRead));
     c->lpScore.assign(hScore, hScore + P);
     c->lpEssH.assign(hEss, hEss + P);
     std::vector<int> bR(hR, hR + P), bT(hT, hT + P);
@@ -1085,7 +1099,12 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     const int nf = (int)flagged.size();
     if (nf > 0) {
         const int nev0 = c->nev;
-        THX_TRY(scan_subset(c, cls, fallbackPrecision, flagged));
+        // the indices go up from the pinned staging (its read-back contents
+        // were copied out above; the stream finishes this upload before the
+        // fallback's results are read back into it)
+        int* pinned = c->lpStage.as<int>();
+        std::copy(flagged.begin(), flagged.end(), pinned);
+        THX_TRY(scan_subset(c, cls, fallbackPrecision, flagged, pinned));
         c->nev = nev0;   // keep the stage events of this call
         c->evNames.resize(nev0);
         // the staging holds P >= nf entries of each kind

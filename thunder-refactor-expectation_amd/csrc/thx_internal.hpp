@@ -134,7 +134,7 @@ struct thx_ctx {
     thx::DevBuf lpQf;        // grid quaternions as float4 for the selection
     thx::HostBuf lpStage;    // pinned: per-particle results read back in one batch
     // ordering events between the context stream and the proposal network's
-    hipEvent_t lpReady = nullptr, lpDone = nullptr;
+    hipEvent_t lpReady = nullptr, lpDone = nullptr, lpRead = nullptr;
     // projectee preparation: FFT plans (hipfftHandle) for one (N, pf), scratch
     void* prjPlanBw = nullptr;
     void* prjPlanFw = nullptr;
@@ -216,6 +216,7 @@ int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, c
                          const int* iRow, int K, int Kp, int N, float2* S);
 int launch_prep_particles(thx_ctx* c, int precision);
 int launch_shell_scale(thx_ctx* c, int precision);
+int ensure_slice_shell_max(thx_ctx* c);
 int launch_convert_lo(thx_ctx* c, int precision);
 int launch_norm_gemm(thx_ctx* c);
 int launch_score(thx_ctx* c, int precision);

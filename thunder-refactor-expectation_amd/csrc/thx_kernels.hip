@@ -388,21 +388,31 @@ __global__ void k_convert_lo(const float2* __restrict__ src, int rows, int Kp,
     dst[i] = lo | (hi << 16);
 }
 
+// Per-shell maxima of the current slices (rotP) into shellMax[0..maxShell],
+// unless already there: they depend on the projections only.
+int ensure_slice_shell_max(thx_ctx* c) {
+    const int nShell = c->maxShell + 1;
+    THX_CHECK(nShell <= kMaxShellLds, THX_ERR_RANGE, "shell scale: too many shells");
+    const void* was = c->shellMax.p;
+    THX_TRY(c->shellMax.ensure(sizeof(float) * 2 * nShell));
+    if (c->shellMax.p != was) c->shellMaxPOk = false;
+    if (c->shellMaxPOk) return THX_OK;
+    float* maxP = c->shellMax.as<float>();
+    THX_HIP(hipMemsetAsync(maxP, 0, sizeof(float) * nShell, c->stream));
+    dim3 gridP((c->K + 255) / 256, 16);
+    hipLaunchKernelGGL(k_shell_max, gridP, dim3(256), 0, c->stream, c->rotP.as<const float2>(),
+                       c->Rpad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxP);
+    THX_HIP(hipGetLastError());
+    c->shellMaxPOk = true;
+    return THX_OK;
+}
+
 int launch_shell_scale(thx_ctx* c, int precision) {
     const int nShell = c->maxShell + 1;
     if (precision != THX_FP16 && precision != THX_FP8) return THX_OK;
-    THX_CHECK(nShell <= kMaxShellLds, THX_ERR_RANGE, "shell scale: too many shells");
+    THX_TRY(ensure_slice_shell_max(c));
     float* maxP = c->shellMax.as<float>();
     float* maxY = maxP + nShell;
-    // the slices' maxima depend on the projections only: kept while current
-    if (!c->shellMaxPOk) {
-        THX_HIP(hipMemsetAsync(maxP, 0, sizeof(float) * nShell, c->stream));
-        dim3 gridP((c->K + 255) / 256, 16);
-        hipLaunchKernelGGL(k_shell_max, gridP, dim3(256), 0, c->stream, c->rotP.as<const float2>(),
-                           c->Rpad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxP);
-        THX_HIP(hipGetLastError());
-        c->shellMaxPOk = true;
-    }
     THX_HIP(hipMemsetAsync(maxY, 0, sizeof(float) * nShell, c->stream));
     // Y with unit scale: run the prep once without scales, take its max.
     hipLaunchKernelGGL(k_prep, dim3(c->Ppad), dim3(256), 0, c->stream,
