@@ -166,6 +166,33 @@ def test_bf16_proposal_matches_fp32(gpu_ctx, setup):
     assert dlk.max() <= 0.05
 
 
+def test_whole_selection_large_grid():
+    """A 9,000-rotation grid: the selection kernel's per-particle log
+    densities need 36 KB of dynamic LDS (beyond the default limit, raised at
+    launch).  With the whole grid selected, the gather scorer's best poses
+    agree with the BF16 global scan's for >= 90 % of particles and its best
+    scores to 2e-3 relative."""
+    R9 = 9000
+    print(f"\n[case] box64 P={P} R={R9} whole selection", flush=True)
+    with thx.Context(0) as ctx:
+        info = wl.setup_context(ctx, "box64", n_particles=P, n_rot=R9)
+        T = len(info["trans"])
+        net = thx.Proposal(components=4, seed=3)
+        try:
+            nf = ctx.expect_learned(net, 0, R9, T)
+        finally:
+            net.close()
+        assert nf == 0
+        bR, bT, sc = ctx.get_learned()[:3]
+        ctx.score_global(0, thx.BF16)
+        gR, gT, gS = ctx.get_best(0)
+    agree = np.mean((bR == gR) & (bT == gT))
+    rel = np.max(np.abs(sc - gS) / np.abs(gS))
+    print(f"agreement {agree:.3f}, max score difference {rel:.2e}")
+    assert agree >= 0.9
+    assert rel <= 2e-3
+
+
 def test_selection_and_statistics(gpu_ctx, setup):
     info, dvp, net = setup
     nsel = 24
