@@ -636,9 +636,9 @@ static unsigned nblk(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t)
 // channels, 5 x 5, stride 2, padding 2, bias + ReLU, FP32): one image per
 // block, staged in LDS with a zero border; thread t computes output row
 // t / 8, columns 4 (t % 8) .. + 3 for all 16 channels from the 5 x 11 input
-// patch in registers and the weights read from LDS (the same address for
-// every lane).  The implicit GEMM spends this layer on scattered im2col
-// loads for a 25-deep product.
+// patch in registers; the weights and biases are read at block-uniform
+// addresses (scalar loads).  The implicit GEMM spends this layer on
+// scattered im2col loads for a 25-deep product.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_conv0_direct(const float* __restrict__ in,
                                                       const float* __restrict__ w,
@@ -649,12 +649,8 @@ __global__ __launch_bounds__(256) void k_conv0_direct(const float* __restrict__ 
                   G::HO == 32 && G::CO == 16, "layer shape");
     constexpr int W = G::HI + 2 * G::PD;   // 68
     __shared__ __attribute__((aligned(16))) float im[W * W];
-    __shared__ float ws[G::CO * 25];
-    __shared__ float bs[G::CO];
     const int b = blockIdx.x, tid = threadIdx.x;
     for (int e = tid; e < W * W; e += 256) im[e] = 0.f;
-    for (int e = tid; e < G::CO * 25; e += 256) ws[e] = w[e];
-    if (tid < G::CO) bs[tid] = bias[tid];
     __syncthreads();
     const float4* src = reinterpret_cast<const float4*>(in + (size_t)b * 4096);
 #pragma unroll
@@ -673,12 +669,12 @@ __global__ __launch_bounds__(256) void k_conv0_direct(const float* __restrict__ 
         for (int c = 0; c < 11; c++) x[ky][c] = im[(2 * oy + ky) * W + 2 * ox0 + c];
     float* o = out + (size_t)b * G::CO * 1024 + oy * 32 + ox0;
     for (int co = 0; co < G::CO; co++) {
-        float a0 = bs[co], a1 = a0, a2 = a0, a3 = a0;
+        float a0 = bias[co], a1 = a0, a2 = a0, a3 = a0;
 #pragma unroll
         for (int ky = 0; ky < 5; ky++)
 #pragma unroll
             for (int kx = 0; kx < 5; kx++) {
-                const float wv = ws[co * 25 + ky * 5 + kx];
+                const float wv = w[co * 25 + ky * 5 + kx];
                 a0 += wv * x[ky][kx];
                 a1 += wv * x[ky][kx + 2];
                 a2 += wv * x[ky][kx + 4];
