@@ -174,7 +174,7 @@ static int run_scan(thx_ctx* c, int cls, int precision) {
         if (precision != THX_BF16 || c->rotLoPrec != THX_BF16) THX_TRY(launch_convert_lo(c, precision));
         THX_TRY(record(c, "convert"));
     }
-    THX_TRY(launch_norm_gemm(c));
+    THX_TRY(launch_norm_gemm(c, precision));
     THX_TRY(record(c, "norm_gemm"));
     THX_TRY(launch_score(c, precision));
     THX_TRY(record(c, "score"));

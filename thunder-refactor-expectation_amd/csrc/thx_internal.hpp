@@ -218,7 +218,7 @@ int launch_prep_particles(thx_ctx* c, int precision);
 int launch_shell_scale(thx_ctx* c, int precision);
 int ensure_slice_shell_max(thx_ctx* c);
 int launch_convert_lo(thx_ctx* c, int precision);
-int launch_norm_gemm(thx_ctx* c);
+int launch_norm_gemm(thx_ctx* c, int precision);
 int launch_score(thx_ctx* c, int precision);
 int score_tile_cols(int precision);
 int launch_merge(thx_ctx* c, int cls);

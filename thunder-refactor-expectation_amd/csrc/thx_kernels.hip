@@ -596,6 +596,114 @@ __global__ __launch_bounds__(256, 2) void k_norm_gemm(
             }
 }
 
+// The same norm term for the 16- and 8-bit scans on BF16 MFMA
+// (v_mfma_f32_32x32x16_bf16): each FP32 operand x is split into
+// hi = bf16(x) and lo = bf16(x - hi), and C = Wh.Nh + Wh.Nl + Wl.Nh, which
+// leaves out Wl.Nl and the split residuals, about 2^-16 of each (non-negative)
+// product: a relative error near 3e-5 of C, far below the error of the 16- or
+// 8-bit cross term it is combined with.  Tiles, split-K slabs and the
+// epilogue are those of k_norm_gemm; the hi and lo halves are staged in LDS
+// as BF16, 8 lanes per 64-pixel row segment.
+__device__ inline uint32_t bf16_hi_lo(float x, float& lo) {
+    const __bf16 h = (__bf16)x;
+    lo = x - (float)h;
+    return __builtin_bit_cast(unsigned short, h);
+}
+
+__global__ __launch_bounds__(256, 2) void k_norm_gemm3(
+    const float* __restrict__ Wt, const float* __restrict__ normP, int Kp, int Rpad,
+    int nPb, int nRb, int kLen, float* __restrict__ Cterm) {
+    constexpr int LD = 72;   // BF16 elements per LDS row (64 + 8: conflict-free 16-byte reads)
+    __shared__ __attribute__((aligned(16))) unsigned short Ah[128 * LD], Al[128 * LD];
+    __shared__ __attribute__((aligned(16))) unsigned short Bh[128 * LD], Bl[128 * LD];
+    const int L = xcd_logical(blockIdx.x, gridDim.x);
+    const int pb = L % nPb, rb = L / nPb;
+    const int p0 = pb * 128, r0 = rb * 128;
+    const int kBeg = blockIdx.y * kLen, kEnd = min(Kp, kBeg + kLen);
+    Cterm += (size_t)blockIdx.y * nPb * 128 * Rpad;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hl = lane >> 5;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = floatx16{0};
+    const int lr = tid >> 1, lh = (tid & 1) * 32;
+    for (int k0 = kBeg; k0 < kEnd; k0 += 64) {
+        // each thread splits 32 consecutive floats of one row of A and of B
+        // (Kp and kLen are multiples of 32 and 64: a segment lies wholly
+        // inside or outside the range)
+        const bool ok = (k0 + lh) < kEnd;
+        const float4* ga = reinterpret_cast<const float4*>(Wt + (size_t)(p0 + lr) * Kp + k0 + lh);
+        const float4* gb = reinterpret_cast<const float4*>(normP + (size_t)(r0 + lr) * Kp + k0 + lh);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            // 8 floats of each row: one 16-byte hi store and one lo store per operand
+            float x[8], y[8];
+            const float4 x0 = ok ? ga[2 * v] : make_float4(0, 0, 0, 0);
+            const float4 x1 = ok ? ga[2 * v + 1] : make_float4(0, 0, 0, 0);
+            const float4 y0 = ok ? gb[2 * v] : make_float4(0, 0, 0, 0);
+            const float4 y1 = ok ? gb[2 * v + 1] : make_float4(0, 0, 0, 0);
+            x[0] = x0.x; x[1] = x0.y; x[2] = x0.z; x[3] = x0.w;
+            x[4] = x1.x; x[5] = x1.y; x[6] = x1.z; x[7] = x1.w;
+            y[0] = y0.x; y[1] = y0.y; y[2] = y0.z; y[3] = y0.w;
+            y[4] = y1.x; y[5] = y1.y; y[6] = y1.z; y[7] = y1.w;
+            u32x4 ah, al, bh, bl;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                float lo0, lo1;
+                uint32_t h0 = bf16_hi_lo(x[2 * e], lo0), h1 = bf16_hi_lo(x[2 * e + 1], lo1);
+                ah[e] = h0 | (h1 << 16);
+                al[e] = (uint32_t)__builtin_bit_cast(unsigned short, (__bf16)lo0) |
+                        ((uint32_t)__builtin_bit_cast(unsigned short, (__bf16)lo1) << 16);
+                h0 = bf16_hi_lo(y[2 * e], lo0);
+                h1 = bf16_hi_lo(y[2 * e + 1], lo1);
+                bh[e] = h0 | (h1 << 16);
+                bl[e] = (uint32_t)__builtin_bit_cast(unsigned short, (__bf16)lo0) |
+                        ((uint32_t)__builtin_bit_cast(unsigned short, (__bf16)lo1) << 16);
+            }
+            const int off = lr * LD + lh + 8 * v;
+            *reinterpret_cast<u32x4*>(Ah + off) = ah;
+            *reinterpret_cast<u32x4*>(Al + off) = al;
+            *reinterpret_cast<u32x4*>(Bh + off) = bh;
+            *reinterpret_cast<u32x4*>(Bl + off) = bl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 64; kk += 16) {
+            const int kc = kk + 8 * hl;
+            bf16x8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                ah[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(Ah + (wm + 32 * t + l32) * LD + kc));
+                al[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(Al + (wm + 32 * t + l32) * LD + kc));
+                bh[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(Bh + (wn + 32 * t + l32) * LD + kc));
+                bl[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(Bl + (wn + 32 * t + l32) * LD + kc));
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++) {
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bl[nt], acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mt], bh[nt], acc[mt][nt], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = wm + 32 * mt + 8 * (i >> 2) + 4 * hl + (i & 3);
+                const int n = wn + 32 * nt + l32;
+                Cterm[(size_t)(p0 + m) * Rpad + r0 + n] = acc[mt][nt][i];
+            }
+}
+
 // Cterm[i] = sum over the ns slabs of part, in slab order.
 __global__ void k_sum_slabs(const float* __restrict__ part, int ns, size_t n,
                             float* __restrict__ out) {
@@ -606,7 +714,7 @@ __global__ void k_sum_slabs(const float* __restrict__ part, int ns, size_t n,
     out[i] = s;
 }
 
-int launch_norm_gemm(thx_ctx* c) {
+int launch_norm_gemm(thx_ctx* c, int precision) {
     const int nPb = c->Ppad / 128, nRb = c->Rpad / 128;
     // fewer tiles than two per CU: split the pixels (at least 256 per range)
     const int tiles = nPb * nRb;
@@ -620,9 +728,14 @@ int launch_norm_gemm(thx_ctx* c) {
         THX_TRY(c->normPart.ensure(sizeof(float) * n * ns));
         dst = c->normPart.as<float>();
     }
-    hipLaunchKernelGGL(k_norm_gemm, dim3(tiles, ns), dim3(256), 0, c->stream,
-                       c->Wt.as<const float>(), c->normP.as<const float>(), c->Kp, c->Rpad,
-                       nPb, nRb, kLen, dst);
+    if (precision == THX_FP32)
+        hipLaunchKernelGGL(k_norm_gemm, dim3(tiles, ns), dim3(256), 0, c->stream,
+                           c->Wt.as<const float>(), c->normP.as<const float>(), c->Kp, c->Rpad,
+                           nPb, nRb, kLen, dst);
+    else
+        hipLaunchKernelGGL(k_norm_gemm3, dim3(tiles, ns), dim3(256), 0, c->stream,
+                           c->Wt.as<const float>(), c->normP.as<const float>(), c->Kp, c->Rpad,
+                           nPb, nRb, kLen, dst);
     THX_HIP(hipGetLastError());
     if (ns > 1) {
         hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
