@@ -137,6 +137,22 @@ int thx_ctx_set_projectee(thx_ctx* ctx, int cls, const float* volFT, int N, int 
                           float* msOut);
 int thx_ctx_get_volume(thx_ctx* ctx, int cls, float* vol);
 
+/* Particle images to the context's particle data on the GPU: images [P][N][N]
+ * real, row-major with the particle centre at pixel (N/2, N/2) as stored in
+ * an MRC stack.  Each image's background (centred i^2 + j^2 > bgRadius^2)
+ * mean is subtracted and the image divided by the background's sample
+ * standard deviation, the image is moved to the centre-at-origin layout, FFT'd
+ * (unnormalised, forward) and its pixel set gathered image-major; ctfP and
+ * sigRcpP [P*K] are taken as in thx_ctx_set_particles.  Replaces the image
+ * path of Optimiser::initImg (substractBgImg and fwImg,
+ * src/Optimiser.cpp:4917-4951, 5003-5013; bgMeanStddev,
+ * src/Image/ImageFunctions.cpp:607-620) and the datP packing of
+ * Optimiser::allocPreCal (src/Optimiser.cpp:8005-8016), without the soft
+ * mask.  msOut (optional, 3 floats): normalisation, FFT and packing times
+ * in ms (the host upload not included). */
+int thx_ctx_set_particle_images(thx_ctx* ctx, const float* images, int P, int N, float bgRadius,
+                                const float* ctfP, const float* sigRcpP, float* msOut);
+
 /* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)
  * packs them (src/Optimiser.cpp:7994-8121).  Replaces ExpectLocalIn +
  * ExpectLocalP (gpu/interface/Interface.h:30-62) for all particles at once. */

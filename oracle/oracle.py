@@ -290,3 +290,33 @@ def projectee(vol_ft, N, pf):
     j0[nz] = np.sin(a[nz]) / a[nz]
     pad /= j0 ** 2
     return np.fft.rfftn(pad, axes=(0, 1, 2))
+
+
+def particle_images(images, bg_radius, iCol, iRow):
+    """The image path of Optimiser::initImg in numpy FP64, per image: the
+    background (centred i^2 + j^2 > r^2) mean and sample standard deviation
+    (bgMeanStddev, src/Image/ImageFunctions.cpp:607-620; GSL's sd uses n - 1)
+    and (x - mean) / sd (Optimiser::substractBgImg, src/Optimiser.cpp:
+    4917-4951); the centre moved to the origin as ImageFile's read stores an
+    image (MESH_IMAGE_INDEX, include/Image/ImageFile.h:199-200); the
+    unnormalised r2c FFT (Optimiser::fwImg, src/Optimiser.cpp:5003-5013);
+    the pixel set gathered at Image::iFTHalf(iCol, iRow)
+    (include/Image/Image.h:426-432) as allocPreCal packs datP
+    (src/Optimiser.cpp:8005-8016).  images: [P][N][N] with the centre at
+    pixel (N/2, N/2); returns complex [P][K]."""
+    images = np.asarray(images, np.float64)
+    P, N, _ = images.shape
+    h = N // 2
+    j, i = np.meshgrid(np.arange(N) - h, np.arange(N) - h, indexing="ij")
+    bg = (i * i + j * j) > bg_radius ** 2
+    iCol = np.asarray(iCol)
+    iRow = np.asarray(iRow)
+    rows = np.where(iRow >= 0, iRow, iRow + N)
+    out = np.empty((P, len(iCol)), np.complex128)
+    for p in range(P):
+        x = images[p]
+        v = x[bg]
+        y = (x - v.mean()) / v.std(ddof=1)
+        y = np.roll(y, (-h, -h), axis=(0, 1))   # stored[r][c] = file[(r + h) % N][(c + h) % N]
+        out[p] = np.fft.rfft2(y)[rows, iCol]
+    return out

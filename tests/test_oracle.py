@@ -582,3 +582,77 @@ def test_projectee_brute_force_dft():
     EV = np.exp(-2j * np.pi * np.outer(m, m) / V)
     ref = np.einsum("zyx,az,by,cx->abc", pad, EV, EV, EV)[:, :, :V // 2 + 1]
     np.testing.assert_allclose(oracle.projectee(ft, N, pf), ref, atol=1e-10)
+
+
+# ---- particle images (Optimiser::initImg image path + allocPreCal datP) ----
+
+def _bg_image(N, r):
+    """An image whose background (outside r) alternates +1 / -1 in raster
+    order, so its mean and sample standard deviation are known in closed
+    form, and whose inside is zero."""
+    h = N // 2
+    img = np.zeros((N, N))
+    sgn, n = 1.0, 0
+    for y in range(N):
+        for x in range(N):
+            if (x - h) ** 2 + (y - h) ** 2 > r * r:
+                img[y, x] = sgn
+                sgn = -sgn
+                n += 1
+    return img, n
+
+
+def test_particle_images_brute_force_dft():
+    """N = 8: the normalisation with hand-computed background statistics, the
+    centre-to-origin move by an explicit index loop and the DFT by explicit
+    sums; pins the roll direction, the transform sign, the missing
+    normalisation and the wrap of negative rows in the packing."""
+    N, r = 8, 3.0
+    h = N // 2
+    rng = np.random.default_rng(3)
+    img, n = _bg_image(N, r)
+    inner = rng.normal(size=(N, N))
+    yy, xx = np.mgrid[0:N, 0:N]
+    inside = (xx - h) ** 2 + (yy - h) ** 2 <= r * r
+    img[inside] = inner[inside]
+    bgv = img[~inside]
+    mean = bgv.sum() / n
+    sd = math.sqrt(((bgv - mean) ** 2).sum() / (n - 1))
+    stored = np.zeros((N, N))
+    for y in range(N):
+        for x in range(N):
+            stored[(y + h) % N, (x + h) % N] = (img[y, x] - mean) / sd
+    iCol = np.array([0, 1, 3, 4, 2, 0])
+    iRow = np.array([0, -1, 2, -4, 3, -2])
+    ref = np.zeros(len(iCol), complex)
+    for k in range(len(iCol)):
+        kx, ky = iCol[k], iRow[k]
+        for y in range(N):
+            for x in range(N):
+                ref[k] += stored[y, x] * np.exp(-2j * np.pi * (kx * x + ky * y) / N)
+    out = oracle.particle_images(img[None], r, iCol, iRow)[0]
+    np.testing.assert_allclose(out, ref, atol=1e-10)
+
+
+@pytest.mark.parametrize("a,b", [(2, -1), (-3, 2)])
+def test_particle_images_shifted_delta(a, b):
+    """A delta of height A at centred pixel (a, b) over the +1 / -1
+    background: after normalisation the stored image is the delta at (a, b)
+    (wrapped) plus the known background, so every coefficient is
+    ((A - m) / sd) exp(-2 pi i (kx a + ky b) / N) plus the background's own
+    transform; the background part is cancelled by subtracting the same
+    image without the delta."""
+    N, r, A = 16, 5.0, 7.5
+    h = N // 2
+    img, n = _bg_image(N, r)
+    bgv = img[img != 0]
+    m = bgv.mean()
+    sd = bgv.std(ddof=1)
+    with_delta = img.copy()
+    with_delta[h + b, h + a] = A
+    iCol, iRow = oracle.pixel_set(N, 6)[:2]
+    d = oracle.particle_images(np.stack([with_delta, img]), r, iCol, iRow)
+    diff = d[0] - d[1]
+    ref = (A - 0.0) / sd * np.exp(-2j * np.pi * (iCol * a + iRow * b) / N)
+    np.testing.assert_allclose(diff, ref, atol=1e-10)
+    assert abs(m) < 1 and n > 2

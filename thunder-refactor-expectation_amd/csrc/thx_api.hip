@@ -281,6 +281,19 @@ int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used) {
     return THX_OK;
 }
 
+// Particle input arrays for P particles (and the derived operands' rows).
+int alloc_particles(thx_ctx* c, int P) {
+    c->P = P;
+    c->Ppad = round_up(P, 128);
+    THX_TRY(c->dat.ensure(sizeof(float2) * (size_t)P * c->K));
+    THX_TRY(c->ctf.ensure(sizeof(float) * (size_t)P * c->K));
+    THX_TRY(c->sig.ensure(sizeof(float) * (size_t)P * c->K));
+    THX_TRY(c->Aterm.ensure(sizeof(double) * c->Ppad));
+    THX_TRY(c->Y.ensure(sizeof(float2) * (size_t)c->Ppad * c->Kp));
+    THX_TRY(c->Wt.ensure(sizeof(float) * (size_t)c->Ppad * c->Kp));
+    return THX_OK;
+}
+
 }  // namespace thx
 
 using namespace thx;
@@ -334,6 +347,7 @@ int thx_ctx_destroy(thx_ctx* c) {
     if (c->lpDone) (void)hipEventDestroy(c->lpDone);
     if (c->lpRead) (void)hipEventDestroy(c->lpRead);
     projectee_release(c);
+    images_release(c);
     c->lpStage.release();
     DevBuf* bufs[] = {&c->iCol, &c->iRow, &c->iSig, &c->dat, &c->ctf, &c->sig, &c->Aterm,
                       &c->Y, &c->Wt, &c->quat, &c->trans, &c->pR, &c->pT, &c->pRf, &c->pTf,
@@ -380,6 +394,7 @@ int thx_ctx_set_pixels(thx_ctx* c, const int* iCol, const int* iRow, const int* 
     std::vector<int> col(Kp, 0), row(Kp, 0), sg(Kp, 0);
     int maxShell = 0;
     double maxR = 0;
+    int maxCol = 0, maxAbsRow = 0;
     for (int k = 0; k < K; k++) {
         col[k] = iCol[k];
         row[k] = iRow[k];
@@ -389,6 +404,8 @@ int thx_ctx_set_pixels(thx_ctx* c, const int* iCol, const int* iRow, const int* 
         sg[k] = iSig ? iSig[k] : (int)std::rint(rr);
         THX_CHECK(sg[k] >= 0, THX_ERR_ARG, "set_pixels: negative shell index");
         maxShell = std::max(maxShell, sg[k]);
+        maxCol = std::max(maxCol, iCol[k]);
+        maxAbsRow = std::max(maxAbsRow, std::abs(iRow[k]));
     }
     THX_CHECK(maxR * pf + 2 <= (double)idim * pf / 2, THX_ERR_RANGE,
               "set_pixels: pixel radius too close to Nyquist of the padded volume");
@@ -403,6 +420,8 @@ int thx_ctx_set_pixels(thx_ctx* c, const int* iCol, const int* iRow, const int* 
     c->idim = idim;
     c->pf = pf;
     c->maxShell = maxShell;
+    c->pixMaxCol = maxCol;
+    c->pixMaxAbsRow = maxAbsRow;
     c->P = 0;   // particle arrays depend on K
     c->R = c->T = 0;
     c->rotLoPrec = -1;   // tables derived from the projections are stale
@@ -439,17 +458,6 @@ int thx_ctx_get_volume(thx_ctx* c, int cls, float* vol) {
     return THX_OK;
 }
 
-static int alloc_particles(thx_ctx* c, int P) {
-    c->P = P;
-    c->Ppad = round_up(P, 128);
-    THX_TRY(c->dat.ensure(sizeof(float2) * (size_t)P * c->K));
-    THX_TRY(c->ctf.ensure(sizeof(float) * (size_t)P * c->K));
-    THX_TRY(c->sig.ensure(sizeof(float) * (size_t)P * c->K));
-    THX_TRY(c->Aterm.ensure(sizeof(double) * c->Ppad));
-    THX_TRY(c->Y.ensure(sizeof(float2) * (size_t)c->Ppad * c->Kp));
-    THX_TRY(c->Wt.ensure(sizeof(float) * (size_t)c->Ppad * c->Kp));
-    return THX_OK;
-}
 
 int thx_ctx_set_particles(thx_ctx* c, const float* datP, const float* ctfP,
                           const float* sigRcpP, int P) {

@@ -1,0 +1,188 @@
+// Particle images to the packed Fourier data of the pixel set, on the GPU:
+// the producer side of the expectation (Optimiser::initImg,
+// src/Optimiser.cpp:4596-4794, without the file reading).
+//
+//   1. per image, the mean and sample standard deviation of the background,
+//      the pixels with centred coordinates i^2 + j^2 > r^2, and the image
+//      made (x - mean) / stddev (Optimiser::substractBgImg,
+//      src/Optimiser.cpp:4917-4951; bgMeanStddev,
+//      src/Image/ImageFunctions.cpp:607-620, GSL's sample standard deviation);
+//      the image is written in the stored layout, its centre at index (0, 0),
+//      as ImageFile's read puts it (MESH_IMAGE_INDEX,
+//      include/Image/ImageFile.h:199-200, 229-244);
+//   2. the unnormalised forward real FFT of every image (Optimiser::fwImg,
+//      src/Optimiser.cpp:5003-5013; FFT::fwExecutePlan(Image&),
+//      src/FFT.cpp:450-458), as one batched hipFFT transform (FP32, like the
+//      reference's single-precision FFTW build);
+//   3. the pixel set gathered image-major into the context's data array,
+//      dat[p][k] = FT_p[iFTHalf(iCol[k], iRow[k])] (Optimiser::allocPreCal,
+//      src/Optimiser.cpp:8005-8016; Image::iFTHalf, include/Image/Image.h:426-432).
+//
+// normaliseImg (src/Optimiser.cpp:4987-5001) then scales every image by the
+// inverse of the mean background standard deviation, which step 1 has made
+// exactly 1 when statImg measures the same region; the soft mask of maskImg
+// (compiled in under OPTIMISER_MASK_IMG) is not applied, which is the
+// unmasked (_imgOri) branch of allocPreCal.
+#include <hipfft/hipfft.h>
+
+#include "thx_internal.hpp"
+
+namespace thx {
+
+// One block per image.  Background statistics in FP64 (two passes: the mean,
+// then the sum of squared deviations), then the normalised image written
+// with its centre at the origin.
+__global__ __launch_bounds__(256) void k_img_normalise(const float* __restrict__ src, int N,
+                                                       float r2, float* __restrict__ dst) {
+    const float* s = src + (size_t)blockIdx.x * N * N;
+    float* d = dst + (size_t)blockIdx.x * N * N;
+    const int h = N / 2;
+    __shared__ double rs[256];
+    __shared__ int rc[256];
+    double sum = 0.0;
+    int cnt = 0;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int i = e % N - h, j = e / N - h;   // centred coordinates of file pixel e
+        if ((float)(i * i + j * j) > r2) {
+            sum += s[e];
+            cnt++;
+        }
+    }
+    rs[threadIdx.x] = sum;
+    rc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            rs[threadIdx.x] += rs[threadIdx.x + st];
+            rc[threadIdx.x] += rc[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    const double mean = rs[0] / rc[0];
+    const int n = rc[0];
+    __syncthreads();
+    double ss = 0.0;
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int i = e % N - h, j = e / N - h;
+        if ((float)(i * i + j * j) > r2) {
+            const double v = s[e] - mean;
+            ss += v * v;
+        }
+    }
+    rs[threadIdx.x] = ss;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) rs[threadIdx.x] += rs[threadIdx.x + st];
+        __syncthreads();
+    }
+    const double sd = sqrt(rs[0] / (n - 1));
+    const float m = (float)mean, inv = (float)(1.0 / sd);
+    for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
+        const int x = e % N, y = e / N;
+        d[((y + h) % N) * N + (x + h) % N] = (s[e] - m) * inv;
+    }
+}
+
+// dat[p][k] = FT_p[row(k)][iCol[k]], row = iRow wrapped to [0, N), FT_p the
+// (N/2 + 1)-column half spectrum of image p.
+__global__ void k_img_pack(const float2* __restrict__ ft, int N, const int* __restrict__ iCol,
+                           const int* __restrict__ iRow, int K, float2* __restrict__ dat) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int p = blockIdx.y;
+    const int j = iRow[k], row = j >= 0 ? j : j + N;
+    dat[(size_t)p * K + k] = ft[((size_t)p * N + row) * (N / 2 + 1) + iCol[k]];
+}
+
+static int fft_status(hipfftResult r, const char* what) {
+    if (r != HIPFFT_SUCCESS) {
+        set_error(std::string("set_particle_images: ") + what + " failed (hipFFT status " +
+                  std::to_string((int)r) + ")");
+        return THX_ERR_HIP;
+    }
+    return THX_OK;
+}
+
+void images_release(thx_ctx* c) {
+    if (c->imgPlan) (void)hipfftDestroy(static_cast<hipfftHandle>(c->imgPlan));
+    c->imgPlan = nullptr;
+    c->imgN = c->imgP = 0;
+    c->imgIn.release();
+    c->imgReal.release();
+    c->imgFT.release();
+}
+
+}  // namespace thx
+
+using namespace thx;
+
+int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, float bgRadius,
+                                const float* ctfP, const float* sigRcpP, float* msOut) {
+    clear_error();
+    THX_CHECK(c && images && ctfP && sigRcpP && P > 0, THX_ERR_ARG,
+              "set_particle_images: bad argument");
+    THX_CHECK(c->K > 0, THX_ERR_STATE, "set_particle_images: pixel set not defined");
+    THX_CHECK(N >= 4 && N % 2 == 0 && N <= 4096, THX_ERR_ARG,
+              "set_particle_images: N must be even, 4 .. 4096");
+    THX_CHECK(c->pixMaxCol <= N / 2 && c->pixMaxAbsRow < N / 2, THX_ERR_RANGE,
+              "set_particle_images: the pixel set reaches beyond the images' half spectrum");
+    // the sample standard deviation needs at least two background pixels
+    THX_CHECK(bgRadius >= 0.f, THX_ERR_ARG, "set_particle_images: negative background radius");
+    {
+        const float r2 = bgRadius * bgRadius;
+        long nbg = 0;
+        for (int j = -N / 2; j < N / 2; j++)
+            for (int i = -N / 2; i < N / 2; i++) nbg += (float)(i * i + j * j) > r2;
+        THX_CHECK(nbg >= 2, THX_ERR_ARG,
+                  "set_particle_images: fewer than two background pixels outside the radius");
+    }
+    THX_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const size_t nReal = (size_t)P * N * N, nFT = (size_t)P * N * (N / 2 + 1);
+    if (c->imgN != N || c->imgP != P) {
+        if (c->imgPlan) (void)hipfftDestroy(static_cast<hipfftHandle>(c->imgPlan));
+        c->imgPlan = nullptr;
+        c->imgN = c->imgP = 0;
+        hipfftHandle plan = nullptr;
+        THX_TRY(fft_status(hipfftCreate(&plan), "plan create"));
+        c->imgPlan = plan;
+        int n[2] = {N, N};
+        size_t work = 0;
+        THX_TRY(fft_status(hipfftMakePlanMany(plan, 2, n, nullptr, 1, N * N, nullptr, 1,
+                                              N * (N / 2 + 1), HIPFFT_R2C, P, &work),
+                           "batched plan"));
+        c->imgN = N;
+        c->imgP = P;
+    }
+    hipfftHandle plan = static_cast<hipfftHandle>(c->imgPlan);
+    THX_TRY(fft_status(hipfftSetStream(plan, st), "set stream"));
+    THX_TRY(alloc_particles(c, P));
+    THX_TRY(c->imgIn.ensure(sizeof(float) * nReal));
+    THX_TRY(c->imgReal.ensure(sizeof(float) * nReal));
+    THX_TRY(c->imgFT.ensure(sizeof(float2) * nFT));
+    THX_HIP(hipMemcpyAsync(c->imgIn.p, images, sizeof(float) * nReal, hipMemcpyHostToDevice, st));
+    const size_t nk = (size_t)P * c->K;
+    THX_HIP(hipMemcpyAsync(c->ctf.p, ctfP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
+    THX_HIP(hipMemcpyAsync(c->sig.p, sigRcpP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
+    THX_HIP(hipEventRecord(c->ev[0], st));
+    hipLaunchKernelGGL(k_img_normalise, dim3(P), dim3(256), 0, st, c->imgIn.as<const float>(), N,
+                       bgRadius * bgRadius, c->imgReal.as<float>());
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipEventRecord(c->ev[1], st));
+    THX_TRY(fft_status(hipfftExecR2C(plan, c->imgReal.as<hipfftReal>(), c->imgFT.as<hipfftComplex>()),
+                       "forward transform"));
+    THX_HIP(hipEventRecord(c->ev[2], st));
+    hipLaunchKernelGGL(k_img_pack, dim3((c->K + 255) / 256, P), dim3(256), 0, st,
+                       c->imgFT.as<const float2>(), N, c->iCol.as<const int>(),
+                       c->iRow.as<const int>(), c->K, c->dat.as<float2>());
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipEventRecord(c->ev[3], st));
+    THX_HIP(hipStreamSynchronize(st));
+    c->nev = 0;   // the events now time this call
+    c->evNames.clear();
+    if (msOut) {
+        // [0] normalise, [1] forward FFT, [2] pack
+        for (int s = 0; s < 3; s++) THX_HIP(hipEventElapsedTime(msOut + s, c->ev[s], c->ev[s + 1]));
+    }
+    return THX_OK;
+}

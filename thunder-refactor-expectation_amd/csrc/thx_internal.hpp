@@ -140,6 +140,11 @@ struct thx_ctx {
     void* prjPlanFw = nullptr;
     int prjN = 0, prjPf = 0;
     thx::DevBuf prjIn, prjReal, prjPad;
+    // particle images: batched R2C FFT plan (hipfftHandle) for (imgN, imgP), buffers
+    void* imgPlan = nullptr;
+    int imgN = 0, imgP = 0;
+    thx::DevBuf imgIn, imgReal, imgFT;
+    int pixMaxCol = 0, pixMaxAbsRow = 0;   // extent of the pixel set
     // particles (image-major, as uploaded) and derived operands
     int P = 0, Ppad = 0;
     thx::DevBuf dat, ctf, sig;
@@ -250,6 +255,8 @@ void invalidate_vol_lo(thx_ctx* c, int cls);
 int launch_vol_to_bf16(hipStream_t st, const float2* vol, size_t n, uint32_t* lo);
 // frees the projectee FFT plans and scratch
 void projectee_release(thx_ctx* c);
+void images_release(thx_ctx* c);
+int alloc_particles(thx_ctx* c, int P);
 int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);
 int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
                        double* wT, double* base, int* bestR, int* bestT,
