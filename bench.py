@@ -268,6 +268,40 @@ def run_box256_fp8(steps):
             "sampling_step_deg": float(np.degrees(step))}
 
 
+def run_images(name, steps):
+    """Producer side of the workload's particles: P images of N x N (seeded
+    noise with a blob), background-normalised, FFT'd and packed into the pixel
+    set on the GPU.  Algorithmic bytes of the normalisation kernel: each image
+    read once and written once (its two statistics passes re-read the image,
+    which the roofline does not credit)."""
+    w = wl.WORKLOADS[name]
+    N, P = w["box"], w["n_particles"]
+    iCol, iRow, iSig = wl.pixel_set(N, w["cutoff"])
+    K = len(iCol)
+    rng = np.random.default_rng(21)
+    imgs = rng.normal(size=(P, N, N)).astype(np.float32)
+    ctf = np.ones((P, K), np.float32)
+    sig = np.ones((P, K), np.float32)
+    ictx = thx.Context(0)
+    ictx.set_pixels(iCol, iRow, iSig, N, w["pf"])
+    ictx.set_particle_images(imgs, N / 2 - 2, ctf, sig)
+    t0 = time.perf_counter()
+    ms = np.array([ictx.set_particle_images(imgs, N / 2 - 2, ctf, sig) for _ in range(steps)],
+                  np.float64).mean(0)
+    wall = (time.perf_counter() - t0) / steps
+    ictx.close()
+    nb = 2.0 * P * N * N * 4   # one read and one write of the FP32 images
+    gbs = float(nb / (ms[0] * 1e-3) / 1e9)
+    return {"particles": P, "N": N, "K": K, "normalise_ms": float(ms[0]),
+            "fft_ms": float(ms[1]), "pack_ms": float(ms[2]),
+            "images_per_s_gpu": P / (float(ms.sum()) * 1e-3),
+            "ms_per_call_with_upload": 1e3 * wall,
+            "normalise_roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0,
+                                   "unit": "GB/s", "frac": gbs / 8000.0, "bytes_per_launch": nb},
+            "note": "FFT is hipFFT (FP32, batched); the upload of the images, CTF and sigma "
+                    "arrays is in ms_per_call_with_upload only"}
+
+
 def run_projectee(name, steps):
     """Projectee preparation (Projector::setProjectee) of a workload-sized map:
     inverse FFT, pad + sinc^2 correction kernel, forward FFT, each timed on the
@@ -461,6 +495,9 @@ def main():
     log("projectee preparation")
     out["projectee"] = run_projectee(name, args.steps)
     log(f"projectee: {out['projectee']}")
+    log("particle images (producer side)")
+    out["images"] = run_images(name, args.steps)
+    log(f"images: {out['images']}")
     out.update({
         "value": r["pose_evals_per_s"], "ms_per_step": r["ms_per_step"], "dtype": dtype,
         "config": {"workload": f"{name}: box {w['box']}, {P} particles x {R} rotations x "

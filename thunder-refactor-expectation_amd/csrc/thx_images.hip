@@ -83,6 +83,84 @@ __global__ __launch_bounds__(256) void k_img_normalise(const float* __restrict__
     }
 }
 
+// Block-wide FP64 sum over 256 threads (four waves): wave shuffles, then
+// one value per wave through LDS.  Every thread returns the total.
+__device__ inline double block_sum256(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();   // red may still be read by a previous call
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// The same normalisation with the image held in registers: V4 float4s per
+// thread cover N * N <= 1024 * V4 pixels (N a multiple of 4), so each image is
+// read from HBM once.  A group of four x-consecutive pixels starts at a
+// multiple of 4 and N / 2 is even, so the moved group stays contiguous and is
+// written with one 16-byte store.
+template <int V4>
+__global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restrict__ src, int N,
+                                                           float r2, float* __restrict__ dst) {
+    const float4* s = reinterpret_cast<const float4*>(src + (size_t)blockIdx.x * N * N);
+    float* d = dst + (size_t)blockIdx.x * N * N;
+    const int h = N / 2, n4 = N * N / 4;
+    __shared__ double red[4];
+    float4 x[V4];
+#pragma unroll
+    for (int v = 0; v < V4; v++) {
+        const int e4 = threadIdx.x + 256 * v;
+        x[v] = e4 < n4 ? s[e4] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // background membership of the 4 V4 pixels, bit 4 v + c
+    unsigned long long bgBits[(4 * V4 + 63) / 64] = {};
+    double sum = 0.0;
+    int cnt = 0;
+#pragma unroll
+    for (int v = 0; v < V4; v++) {
+        const int e4 = threadIdx.x + 256 * v;
+        if (e4 < n4) {
+            const int e = 4 * e4, y = e / N, x0 = e - y * N;
+            const int j = y - h;
+            const float xs[4] = {x[v].x, x[v].y, x[v].z, x[v].w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int i = x0 + c - h;
+                if ((float)(i * i + j * j) > r2) {
+                    sum += xs[c];
+                    cnt++;
+                    bgBits[(4 * v + c) >> 6] |= 1ull << ((4 * v + c) & 63);
+                }
+            }
+        }
+    }
+    const double n = block_sum256((double)cnt, red);
+    const double mean = block_sum256(sum, red) / n;
+    double ss = 0.0;
+#pragma unroll
+    for (int v = 0; v < V4; v++) {
+        const float xs[4] = {x[v].x, x[v].y, x[v].z, x[v].w};
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            if ((bgBits[(4 * v + c) >> 6] >> ((4 * v + c) & 63)) & 1ull) {
+                const double dv = xs[c] - mean;
+                ss += dv * dv;
+            }
+    }
+    const double sd = sqrt(block_sum256(ss, red) / (n - 1.0));
+    const float m = (float)mean, inv = (float)(1.0 / sd);
+#pragma unroll
+    for (int v = 0; v < V4; v++) {
+        const int e4 = threadIdx.x + 256 * v;
+        if (e4 < n4) {
+            const int e = 4 * e4, y = e / N, x0 = e - y * N;
+            const int yo = y + h < N ? y + h : y + h - N, xo = x0 + h < N ? x0 + h : x0 + h - N;
+            *reinterpret_cast<float4*>(d + (size_t)yo * N + xo) =
+                make_float4((x[v].x - m) * inv, (x[v].y - m) * inv, (x[v].z - m) * inv,
+                            (x[v].w - m) * inv);
+        }
+    }
+}
+
 // dat[p][k] = FT_p[row(k)][iCol[k]], row = iRow wrapped to [0, N), FT_p the
 // (N/2 + 1)-column half spectrum of image p.
 __global__ void k_img_pack(const float2* __restrict__ ft, int N, const int* __restrict__ iCol,
@@ -165,8 +243,18 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     THX_HIP(hipMemcpyAsync(c->ctf.p, ctfP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
     THX_HIP(hipMemcpyAsync(c->sig.p, sigRcpP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
     THX_HIP(hipEventRecord(c->ev[0], st));
-    hipLaunchKernelGGL(k_img_normalise, dim3(P), dim3(256), 0, st, c->imgIn.as<const float>(), N,
-                       bgRadius * bgRadius, c->imgReal.as<float>());
+    // images of up to 16 K (N <= 128) or 32 K pixels (N <= 180) are held in
+    // registers and read once; larger ones are re-read by the generic kernel
+    const float r2 = bgRadius * bgRadius;
+    if (N % 4 == 0 && N * N <= 1024 * 16)
+        hipLaunchKernelGGL(k_img_normalise_reg<16>, dim3(P), dim3(256), 0, st,
+                           c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>());
+    else if (N % 4 == 0 && N * N <= 1024 * 32)
+        hipLaunchKernelGGL(k_img_normalise_reg<32>, dim3(P), dim3(256), 0, st,
+                           c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>());
+    else
+        hipLaunchKernelGGL(k_img_normalise, dim3(P), dim3(256), 0, st, c->imgIn.as<const float>(),
+                           N, r2, c->imgReal.as<float>());
     THX_HIP(hipGetLastError());
     THX_HIP(hipEventRecord(c->ev[1], st));
     THX_TRY(fft_status(hipfftExecR2C(plan, c->imgReal.as<hipfftReal>(), c->imgFT.as<hipfftComplex>()),
