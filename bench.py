@@ -271,7 +271,8 @@ def run_box256_fp8(steps):
 def run_images(name, steps):
     """Producer side of the workload's particles: P images of N x N (seeded
     noise with a blob), background-normalised, FFT'd and packed into the pixel
-    set on the GPU.  Algorithmic bytes of the normalisation kernel: each image
+    set on the GPU, then the initial sigma spectrum estimated from the
+    resident spectra.  Algorithmic bytes of the normalisation kernel: each image
     read once and written once (its two statistics passes re-read the image,
     which the roofline does not credit)."""
     w = wl.WORKLOADS[name]
@@ -289,6 +290,8 @@ def run_images(name, steps):
     ms = np.array([ictx.set_particle_images(imgs, N / 2 - 2, ctf, sig) for _ in range(steps)],
                   np.float64).mean(0)
     wall = (time.perf_counter() - t0) / steps
+    ictx.init_sigma(N // 2 - 1)
+    sig_ms = float(np.mean([ictx.init_sigma(N // 2 - 1)[2] for _ in range(steps)]))
     ictx.close()
     nb = 2.0 * P * N * N * 4   # one read and one write of the FP32 images
     gbs = float(nb / (ms[0] * 1e-3) / 1e9)
@@ -296,6 +299,7 @@ def run_images(name, steps):
             "fft_ms": float(ms[1]), "pack_ms": float(ms[2]),
             "images_per_s_gpu": P / (float(ms.sum()) * 1e-3),
             "ms_per_call_with_upload": 1e3 * wall,
+            "init_sigma_ms": sig_ms,
             "normalise_roofline": {"bound": "hbm", "achieved": gbs, "peak": 8000.0,
                                    "unit": "GB/s", "frac": gbs / 8000.0, "bytes_per_launch": nb},
             "note": "FFT is hipFFT (FP32, batched); the upload of the images, CTF and sigma "

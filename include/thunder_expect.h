@@ -153,6 +153,22 @@ int thx_ctx_get_volume(thx_ctx* ctx, int cls, float* vol);
 int thx_ctx_set_particle_images(thx_ctx* ctx, const float* images, int P, int N, float bgRadius,
                                 const float* ctfP, const float* sigRcpP, float* msOut);
 
+/* Initial noise spectrum from the images of the last
+ * thx_ctx_set_particle_images call (their normalised, unmasked spectra):
+ * sigma^2(u) = (average over images of the power spectrum in shell u
+ * - (ring average of Re + Im of the average image)^2) / 2 and
+ * sigRcp(u) = -0.5 / sigma^2(u), for shells u = 0 .. nShell - 1 with
+ * nShell <= N/2 - 1, shell u holding the pixels with rint(|(i, j)|) == u.
+ * Replaces Optimiser::initSigma (src/Optimiser.cpp:5111-5209; powerSpectrum and
+ * ringAverage, src/Functions/Spectrum.cpp:47-62, 161-190) for one group;
+ * nShell = maxR() (src/Optimiser.cpp:4384-4387) gives its full output.  With apply
+ * nonzero the context's per-particle sigRcpP becomes sigRcp(iSig[k]) for every
+ * particle, as allocPreCal fills it (src/Optimiser.cpp:8017-8022).  sigOut,
+ * sigRcpOut (optional, nShell doubles each); msOut (optional, 1 float):
+ * kernel time in ms. */
+int thx_ctx_init_sigma(thx_ctx* ctx, int nShell, int apply, double* sigOut, double* sigRcpOut,
+                       float* msOut);
+
 /* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)
  * packs them (src/Optimiser.cpp:7994-8121).  Replaces ExpectLocalIn +
  * ExpectLocalP (gpu/interface/Interface.h:30-62) for all particles at once. */

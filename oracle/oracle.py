@@ -304,19 +304,53 @@ def particle_images(images, bg_radius, iCol, iRow):
     (include/Image/Image.h:426-432) as allocPreCal packs datP
     (src/Optimiser.cpp:8005-8016).  images: [P][N][N] with the centre at
     pixel (N/2, N/2); returns complex [P][K]."""
+    N = np.asarray(images).shape[1]
+    iCol = np.asarray(iCol)
+    iRow = np.asarray(iRow)
+    rows = np.where(iRow >= 0, iRow, iRow + N)
+    return image_spectra(images, bg_radius)[:, rows, iCol]
+
+
+def image_spectra(images, bg_radius):
+    """The stored half spectra [P][N][N/2 + 1] of the normalised images
+    (the steps of particle_images before the gather): row r is frequency
+    r (r < N/2) or r - N, column c frequency c."""
     images = np.asarray(images, np.float64)
     P, N, _ = images.shape
     h = N // 2
     j, i = np.meshgrid(np.arange(N) - h, np.arange(N) - h, indexing="ij")
     bg = (i * i + j * j) > bg_radius ** 2
-    iCol = np.asarray(iCol)
-    iRow = np.asarray(iRow)
-    rows = np.where(iRow >= 0, iRow, iRow + N)
-    out = np.empty((P, len(iCol)), np.complex128)
+    out = np.empty((P, N, h + 1), np.complex128)
     for p in range(P):
         x = images[p]
         v = x[bg]
         y = (x - v.mean()) / v.std(ddof=1)
         y = np.roll(y, (-h, -h), axis=(0, 1))   # stored[r][c] = file[(r + h) % N][(c + h) % N]
-        out[p] = np.fft.rfft2(y)[rows, iCol]
+        out[p] = np.fft.rfft2(y)
     return out
+
+
+def init_sigma(images, bg_radius, n_shell):
+    """Optimiser::initSigma (src/Optimiser.cpp:5111-5209) for one group, on
+    the normalised unmasked spectra (OPTIMISER_SIGMA_MASK is off,
+    include/Config.h:169): the average image avg = mean_p F_p; the average over
+    images of powerSpectrum (src/Functions/Spectrum.cpp:161-190: the mean of
+    |F|^2 over the pixels of shell u = rint(hypot(i, j)) < r, the loop of
+    IMAGE_FOR_EACH_PIXEL_FT, include/Image/Image.h:50-52: column i = 0 .. N/2,
+    row j = -N/2 .. N/2 - 1); psAvg(u) = (ringAverage of Re + Im of avg,
+    src/Functions/Spectrum.cpp:47-62)^2; sigma^2 = (avgPs - psAvg) / 2 and
+    sigRcp = -0.5 / sigma^2.  Returns (sigma^2, sigRcp) for u < n_shell."""
+    F = image_spectra(images, bg_radius)
+    P, N, W = F.shape
+    j = np.where(np.arange(N) < N // 2, np.arange(N), np.arange(N) - N)[:, None]
+    i = np.arange(W)[None, :]
+    shell = np.rint(np.hypot(i, j)).astype(np.int64)
+    avg = F.mean(axis=0)
+    pw = (F.real ** 2 + F.imag ** 2).mean(axis=0)
+    sig = np.empty(n_shell)
+    for u in range(n_shell):
+        m = shell == u
+        avg_ps = pw[m].mean()
+        ps_avg = (avg.real[m] + avg.imag[m]).mean() ** 2
+        sig[u] = 0.5 * (avg_ps - ps_avg)
+    return sig, -0.5 / sig

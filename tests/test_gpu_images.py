@@ -99,3 +99,41 @@ def test_images_score_as_uploaded_data(ictx):
         results.append(ictx.get_best(0))
     for a, b in zip(*results):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("N,cutoff,rbg,P", [(64, 24, 30.0, 48), (128, 50, 60.0, 300)])
+def test_init_sigma_matches_oracle(ictx, N, cutoff, rbg, P):
+    """Initial sigma^2 and sigRcp per shell from the last images
+    (Optimiser::initSigma) against the FP64 oracle on the same images: the
+    GPU sums FP32 spectra in FP64, so every shell agrees to 1e-4 relative;
+    with apply the particles' sigRcp becomes the shell value at iSig."""
+    print(f"\n[sigma] N={N} P={P}", flush=True)
+    iCol, iRow, iSig = wl.pixel_set(N, cutoff)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    K = len(iCol)
+    imgs = _images(P, N, 3 + N)
+    ones = np.ones((P, K), np.float32)
+    ictx.set_particle_images(imgs, rbg, ones, ones)
+    nS = N // 2 - 1
+    with pytest.raises(thx.THXError):
+        ictx.init_sigma(nS + 1)
+    sg, rcp, ms = ictx.init_sigma(nS, apply=True)
+    ref, ref_rcp = oracle.init_sigma(imgs, rbg, nS)
+    err = np.max(np.abs(sg - ref) / np.abs(ref))
+    print(f"max relative error {err:.2e}; kernels {ms:.3f} ms")
+    assert err <= 1e-4
+    np.testing.assert_allclose(rcp, -0.5 / sg, rtol=1e-12)
+    got = ictx.get_particles()[2]
+    want = np.broadcast_to(ref_rcp[np.asarray(iSig)].astype(np.float32), (P, K))
+    np.testing.assert_allclose(got, want, rtol=1e-4)
+
+
+def test_init_sigma_needs_images(gpu_ctx):
+    """Without particle images there is nothing to estimate from: a clear
+    error, not a result."""
+    ctx = thx.Context(0)
+    try:
+        with pytest.raises(thx.THXError, match="no particle images"):
+            ctx.init_sigma(10)
+    finally:
+        ctx.close()

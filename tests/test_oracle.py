@@ -656,3 +656,74 @@ def test_particle_images_shifted_delta(a, b):
     ref = (A - 0.0) / sd * np.exp(-2j * np.pi * (iCol * a + iRow * b) / N)
     np.testing.assert_allclose(diff, ref, atol=1e-10)
     assert abs(m) < 1 and n > 2
+
+
+def test_init_sigma_explicit_loops():
+    """N = 8, three images: initSigma written out with explicit loops
+    (normalisation, centre move, DFT sums over the stored half spectrum,
+    powerSpectrum with its radius test and counters, ringAverage of Re + Im
+    with its own counter) against the vectorised oracle."""
+    N, rbg, P = 8, 3.0, 3
+    h, r = N // 2, N // 2 - 1
+    rng = np.random.default_rng(8)
+    imgs = rng.normal(1.0, 2.0, size=(P, N, N))
+    spectra = []
+    for p in range(P):
+        bgv = [imgs[p, y, x] for y in range(N) for x in range(N)
+               if (x - h) ** 2 + (y - h) ** 2 > rbg * rbg]
+        mean = sum(bgv) / len(bgv)
+        sd = math.sqrt(sum((v - mean) ** 2 for v in bgv) / (len(bgv) - 1))
+        stored = np.zeros((N, N))
+        for y in range(N):
+            for x in range(N):
+                stored[(y + h) % N, (x + h) % N] = (imgs[p, y, x] - mean) / sd
+        F = {}
+        for j in range(-h, h):
+            for i in range(h + 1):
+                F[(i, j)] = sum(stored[y, x] * np.exp(-2j * np.pi * (i * x + j * y) / N)
+                                for y in range(N) for x in range(N))
+        spectra.append(F)
+    avg_ps = np.zeros(r)
+    for F in spectra:
+        ps, cnt = np.zeros(r), np.zeros(r)
+        for (i, j), f in F.items():
+            if i * i + j * j < r * r:
+                u = int(round(math.hypot(i, j)))
+                if u < r:
+                    ps[u] += abs(f) ** 2
+                    cnt[u] += 1
+        avg_ps += ps / cnt / P
+    avg = {key: sum(F[key] for F in spectra) / P for key in spectra[0]}
+    ps_avg = np.zeros(r)
+    for u in range(r):
+        acc, c = 0.0, 0
+        for (i, j), f in avg.items():
+            if int(round(math.hypot(i, j))) == u:
+                acc += f.real + f.imag
+                c += 1
+        ps_avg[u] = (acc / c) ** 2
+    sig_ref = (avg_ps - ps_avg) / 2
+    sig, rcp = oracle.init_sigma(imgs, rbg, r)
+    np.testing.assert_allclose(sig, sig_ref, rtol=1e-10)
+    np.testing.assert_allclose(rcp, -0.5 / sig_ref, rtol=1e-10)
+
+
+def test_init_sigma_white_noise():
+    """Images of white Gaussian noise (any mean and spread): after the
+    background normalisation every pixel has unit variance, so each
+    unnormalised coefficient has E|F|^2 = N^2 and the average image tends to
+    zero: sigma^2(u) = N^2 / 2 for every shell u >= 1 (shell 0 is the DC term,
+    reduced by the mean subtraction)."""
+    N, P = 32, 300
+    rng = np.random.default_rng(32)
+    imgs = rng.normal(5.0, 2.0, size=(P, N, N))
+    sig, rcp = oracle.init_sigma(imgs, 10.0, N // 2 - 1)
+    j = np.where(np.arange(N) < N // 2, np.arange(N), np.arange(N) - N)[:, None]
+    shell = np.rint(np.hypot(np.arange(N // 2 + 1)[None, :], j)).astype(int)
+    cnt = np.array([(shell == u).sum() for u in range(N // 2 - 1)])
+    rel = sig[1:] / (N * N / 2) - 1
+    # relative spread of a mean of P * cnt / 2 independent exponential powers
+    assert np.all(np.abs(rel) < 6 / np.sqrt(P * cnt[1:] / 2)), rel
+    pooled = (sig[1:] * cnt[1:]).sum() / cnt[1:].sum() / (N * N / 2)
+    assert abs(pooled - 1) < 6 / np.sqrt(P * cnt[1:].sum() / 2), pooled
+    np.testing.assert_allclose(rcp, -0.5 / sig)

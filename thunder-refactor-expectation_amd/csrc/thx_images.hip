@@ -25,6 +25,8 @@
 // unmasked (_imgOri) branch of allocPreCal.
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
+
 #include "thx_internal.hpp"
 
 namespace thx {
@@ -172,6 +174,91 @@ __global__ void k_img_pack(const float2* __restrict__ ft, int N, const int* __re
     dat[(size_t)p * K + k] = ft[((size_t)p * N + row) * (N / 2 + 1) + iCol[k]];
 }
 
+// Initial noise spectrum (Optimiser::initSigma, src/Optimiser.cpp:5111-5209)
+// from the stored half spectra F_p of the last images, nPix = N (N/2 + 1)
+// pixels each.  Pass 1: per pixel, over the images of slice blockIdx.y, the
+// sums of Re F, Im F and |F|^2 in FP64.
+__global__ __launch_bounds__(256) void k_sigma_partial(const float2* __restrict__ ft, int P,
+                                                       int nPix, double* __restrict__ part) {
+    const int pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= nPix) return;
+    const int G = gridDim.y, g = blockIdx.y;
+    const int p0 = (int)((long long)P * g / G), p1 = (int)((long long)P * (g + 1) / G);
+    double sr = 0.0, si = 0.0, sp = 0.0;
+#pragma unroll 8
+    for (int p = p0; p < p1; p++) {
+        const float2 f = ft[(size_t)p * nPix + pix];
+        sr += f.x;
+        si += f.y;
+        sp += (double)f.x * f.x + (double)f.y * f.y;
+    }
+    double* o = part + (size_t)g * 3 * nPix;
+    o[pix] = sr;
+    o[nPix + pix] = si;
+    o[2 * (size_t)nPix + pix] = sp;
+}
+
+// Pass 2: per pixel, the G slice sums added in slice order into slice 0 (in
+// place: each thread reads and writes only its own pixel).
+__global__ __launch_bounds__(256) void k_sigma_fold(double* __restrict__ part, int G, int nPix) {
+    const int pix = blockIdx.x * 256 + threadIdx.x;
+    if (pix >= nPix) return;
+    double sr = 0.0, si = 0.0, sp = 0.0;
+#pragma unroll 8
+    for (int g = 0; g < G; g++) {
+        const double* o = part + (size_t)g * 3 * nPix;
+        sr += o[pix];
+        si += o[nPix + pix];
+        sp += o[2 * (size_t)nPix + pix];
+    }
+    part[pix] = sr;
+    part[nPix + pix] = si;
+    part[2 * (size_t)nPix + pix] = sp;
+}
+
+// Pass 3: one block per shell u, the stored pixels (i, j) with
+// rint(hypot(i, j)) == u (i the column 0 .. N/2, j the signed row), from the
+// folded sums of slice 0.  The average power spectrum
+// (powerSpectrum, src/Functions/Spectrum.cpp:161-190, averaged over the
+// images) minus the square of the ring average of Re + Im of the average
+// image (ringAverage with gsl_real_imag_sum, src/Functions/Spectrum.cpp:47-62,
+// include/Complex.h:155-158), halved: sigma^2(u), and sigRcp(u) = -0.5 / sigma^2(u).
+// For u < N/2 - 1 every pixel of the ring lies inside powerSpectrum's radius,
+// so both averages run over the same pixels.
+__global__ __launch_bounds__(256) void k_sigma_shells(const double* __restrict__ part, int N, int P,
+                                                      double* __restrict__ out, int nShell) {
+    const int u = blockIdx.x, W = N / 2 + 1, nPix = N * W;
+    __shared__ double red[4];
+    double aP = 0.0, aA = 0.0;
+    int cnt = 0;
+    for (int pix = threadIdx.x; pix < nPix; pix += 256) {
+        const int row = pix / W, i = pix - row * W, j = row < N / 2 ? row : row - N;
+        if ((int)rint(sqrt((double)(i * i + j * j))) != u) continue;
+        aP += part[2 * (size_t)nPix + pix];
+        aA += part[pix] + part[nPix + pix];
+        cnt++;
+    }
+    const double n = block_sum256((double)cnt, red);
+    const double sumP = block_sum256(aP, red);
+    const double sumA = block_sum256(aA, red);
+    if (threadIdx.x == 0) {
+        const double avgPs = sumP / ((double)P * n);
+        const double ring = sumA / (double)P / n;
+        const double s2 = 0.5 * (avgPs - ring * ring);
+        out[u] = s2;
+        out[nShell + u] = -0.5 / s2;
+    }
+}
+
+// sigRcpP[p][k] = sigRcp(iSig[k]) for every particle, as allocPreCal fills
+// it for one group (src/Optimiser.cpp:8017-8022).
+__global__ void k_sigma_apply(const double* __restrict__ rcp, const int* __restrict__ iSig, int K,
+                              float* __restrict__ sig) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    sig[(size_t)blockIdx.y * K + k] = (float)rcp[iSig[k]];
+}
+
 static int fft_status(hipfftResult r, const char* what) {
     if (r != HIPFFT_SUCCESS) {
         set_error(std::string("set_particle_images: ") + what + " failed (hipFFT status " +
@@ -188,6 +275,9 @@ void images_release(thx_ctx* c) {
     c->imgIn.release();
     c->imgReal.release();
     c->imgFT.release();
+    c->imgReady = false;
+    c->sigPart.release();
+    c->sigShell.release();
 }
 
 }  // namespace thx
@@ -215,6 +305,7 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
                   "set_particle_images: fewer than two background pixels outside the radius");
     }
     THX_HIP(hipSetDevice(c->device));
+    c->imgReady = false;
     hipStream_t st = c->stream;
     const size_t nReal = (size_t)P * N * N, nFT = (size_t)P * N * (N / 2 + 1);
     if (c->imgN != N || c->imgP != P) {
@@ -266,11 +357,63 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     THX_HIP(hipGetLastError());
     THX_HIP(hipEventRecord(c->ev[3], st));
     THX_HIP(hipStreamSynchronize(st));
+    c->imgReady = true;
     c->nev = 0;   // the events now time this call
     c->evNames.clear();
     if (msOut) {
         // [0] normalise, [1] forward FFT, [2] pack
         for (int s = 0; s < 3; s++) THX_HIP(hipEventElapsedTime(msOut + s, c->ev[s], c->ev[s + 1]));
     }
+    return THX_OK;
+}
+
+int thx_ctx_init_sigma(thx_ctx* c, int nShell, int apply, double* sigOut, double* sigRcpOut,
+                       float* msOut) {
+    clear_error();
+    THX_CHECK(c, THX_ERR_ARG, "init_sigma: null context");
+    THX_CHECK(c->imgReady, THX_ERR_STATE,
+              "init_sigma: no particle images (call thx_ctx_set_particle_images first)");
+    const int N = c->imgN, P = c->imgP;
+    THX_CHECK(nShell >= 1 && nShell <= N / 2 - 1, THX_ERR_ARG,
+              "init_sigma: nShell must be 1 .. N/2 - 1 (Optimiser::maxR)");
+    if (apply) {
+        THX_CHECK(c->P == P && c->K > 0, THX_ERR_STATE,
+                  "init_sigma: the particles are not the last images");
+        THX_CHECK(c->maxShell < nShell, THX_ERR_RANGE,
+                  "init_sigma: the pixel set has shells beyond nShell");
+    }
+    THX_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int nPix = N * (N / 2 + 1), nb = (nPix + 255) / 256;
+    // enough image slices for about 2048 blocks in pass 1
+    const int G = std::max(1, std::min({P, 256, 2048 / nb}));
+    THX_TRY(c->sigPart.ensure(sizeof(double) * 3 * (size_t)G * nPix));
+    THX_TRY(c->sigShell.ensure(sizeof(double) * 2 * (size_t)nShell));
+    THX_HIP(hipEventRecord(c->ev[0], st));
+    hipLaunchKernelGGL(k_sigma_partial, dim3(nb, G), dim3(256), 0, st, c->imgFT.as<const float2>(), P,
+                       nPix, c->sigPart.as<double>());
+    THX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sigma_fold, dim3(nb), dim3(256), 0, st, c->sigPart.as<double>(), G, nPix);
+    THX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sigma_shells, dim3(nShell), dim3(256), 0, st, c->sigPart.as<const double>(),
+                       N, P, c->sigShell.as<double>(), nShell);
+    THX_HIP(hipGetLastError());
+    if (apply) {
+        hipLaunchKernelGGL(k_sigma_apply, dim3((c->K + 255) / 256, P), dim3(256), 0, st,
+                           c->sigShell.as<const double>() + nShell, c->iSig.as<const int>(), c->K,
+                           c->sig.as<float>());
+        THX_HIP(hipGetLastError());
+    }
+    THX_HIP(hipEventRecord(c->ev[1], st));
+    if (sigOut)
+        THX_HIP(hipMemcpyAsync(sigOut, c->sigShell.p, sizeof(double) * nShell,
+                               hipMemcpyDeviceToHost, st));
+    if (sigRcpOut)
+        THX_HIP(hipMemcpyAsync(sigRcpOut, c->sigShell.as<double>() + nShell,
+                               sizeof(double) * nShell, hipMemcpyDeviceToHost, st));
+    THX_HIP(hipStreamSynchronize(st));
+    c->nev = 0;
+    c->evNames.clear();
+    if (msOut) THX_HIP(hipEventElapsedTime(msOut, c->ev[0], c->ev[1]));
     return THX_OK;
 }

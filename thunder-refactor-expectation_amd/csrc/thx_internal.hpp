@@ -144,6 +144,8 @@ struct thx_ctx {
     void* imgPlan = nullptr;
     int imgN = 0, imgP = 0;
     thx::DevBuf imgIn, imgReal, imgFT;
+    bool imgReady = false;            // imgFT holds the spectra of the last images
+    thx::DevBuf sigPart, sigShell;    // initial sigma: per-pixel partial sums, shell results
     int pixMaxCol = 0, pixMaxAbsRow = 0;   // extent of the pixel set
     // particles (image-major, as uploaded) and derived operands
     int P = 0, Ppad = 0;

@@ -37,6 +37,7 @@ _SIGS = {
     "thx_ctx_get_volume": [_p, _i, _p],
     "thx_ctx_set_projectee": [_p, _i, _p, _i, _i, _p],
     "thx_ctx_set_particle_images": [_p, _p, _i, _i, C.c_float, _p, _p, _p],
+    "thx_ctx_init_sigma": [_p, _i, _i, _p, _p, _p],
     "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
     "thx_ctx_get_particles": [_p, _p, _p, _p],
     "thx_ctx_set_grid": [_p, _p, _i, _p, _i, _p, _p],
@@ -239,6 +240,18 @@ class Context:
               "set_particle_images")
         self.P = P
         return ms
+
+    def init_sigma(self, n_shell, apply=False):
+        """Initial sigma^2 and sigRcp = -0.5 / sigma^2 per shell (0 .. n_shell - 1)
+        from the last set_particle_images call (Optimiser::initSigma); with
+        apply, the particles' sigRcp becomes sigRcp[iSig].  Returns
+        (sigma^2, sigRcp, kernel ms)."""
+        sig = np.zeros(n_shell, np.float64)
+        rcp = np.zeros(n_shell, np.float64)
+        ms = np.zeros(1, np.float32)
+        check(load().thx_ctx_init_sigma(self._h, int(n_shell), int(bool(apply)), ptr(sig),
+                                         ptr(rcp), ptr(ms)), "init_sigma")
+        return sig, rcp, float(ms[0])
 
     def get_volume(self, cls, vdim):
         out = np.zeros((vdim, vdim, vdim // 2 + 1, 2), np.float32)
