@@ -147,11 +147,19 @@ int thx_ctx_get_volume(thx_ctx* ctx, int cls, float* vol);
  * path of Optimiser::initImg (substractBgImg and fwImg,
  * src/Optimiser.cpp:4917-4951, 5003-5013; bgMeanStddev,
  * src/Image/ImageFunctions.cpp:607-620) and the datP packing of
- * Optimiser::allocPreCal (src/Optimiser.cpp:8005-8016), without the soft
- * mask.  msOut (optional, 3 floats): normalisation, FFT and packing times
- * in ms (the host upload not included). */
+ * Optimiser::allocPreCal (src/Optimiser.cpp:8005-8016).  With softMask
+ * nonzero the packed data are those of the soft-masked images, as the
+ * expectation packs them (allocPreCal(mask = true), src/Optimiser.cpp:637,
+ * 1457; maskImg / reMaskImg with zeroMask, src/Optimiser.cpp:4953-4985,
+ * 6053-6105): each normalised image times the mask of radius bgRadius with
+ * a raised-cosine edge of 6 pixels (softMask, src/Functions/Mask.cpp:333-350;
+ * EDGE_WIDTH_RL, include/Macro.h:82); with softMask zero, those of the
+ * unmasked images (allocPreCal(mask = false)).  The unmasked spectra stay
+ * resident for thx_ctx_init_sigma either way.  msOut (optional, 3 floats):
+ * normalisation, FFT and packing times in ms (the host upload not included). */
 int thx_ctx_set_particle_images(thx_ctx* ctx, const float* images, int P, int N, float bgRadius,
-                                const float* ctfP, const float* sigRcpP, float* msOut);
+                                int softMask, const float* ctfP, const float* sigRcpP,
+                                float* msOut);
 
 /* Initial noise spectrum from the images of the last
  * thx_ctx_set_particle_images call (their normalised, unmasked spectra):

@@ -292,7 +292,22 @@ def projectee(vol_ft, N, pf):
     return np.fft.rfftn(pad, axes=(0, 1, 2))
 
 
-def particle_images(images, bg_radius, iCol, iRow):
+EDGE_WIDTH_RL = 6   # include/Macro.h:82
+
+
+def soft_mask(N, r, ew=EDGE_WIDTH_RL):
+    """softMask(Image&, r, ew) (src/Functions/Mask.cpp:333-350) over an N x N
+    image with its centre at pixel (N/2, N/2): 1 for u < r,
+    0.5 + 0.5 cos(pi (u - r) / ew) for r <= u <= r + ew, 0 beyond;
+    u = |(x - N/2, y - N/2)|."""
+    h = N // 2
+    y, x = np.mgrid[0:N, 0:N]
+    u = np.hypot(x - h, y - h)
+    m = np.where(u > r + ew, 0.0, 0.5 + 0.5 * np.cos((u - r) / ew * np.pi))
+    return np.where(u < r, 1.0, m)
+
+
+def particle_images(images, bg_radius, iCol, iRow, masked=False):
     """The image path of Optimiser::initImg in numpy FP64, per image: the
     background (centred i^2 + j^2 > r^2) mean and sample standard deviation
     (bgMeanStddev, src/Image/ImageFunctions.cpp:607-620; GSL's sd uses n - 1)
@@ -302,16 +317,19 @@ def particle_images(images, bg_radius, iCol, iRow):
     unnormalised r2c FFT (Optimiser::fwImg, src/Optimiser.cpp:5003-5013);
     the pixel set gathered at Image::iFTHalf(iCol, iRow)
     (include/Image/Image.h:426-432) as allocPreCal packs datP
-    (src/Optimiser.cpp:8005-8016).  images: [P][N][N] with the centre at
+    (src/Optimiser.cpp:8005-8016).  With masked, the normalised image is
+    multiplied by soft_mask(N, bg_radius) first (maskImg / reMaskImg with
+    zeroMask, src/Optimiser.cpp:4953-4985, 6053-6105): the images
+    allocPreCal(mask = true) packs.  images: [P][N][N] with the centre at
     pixel (N/2, N/2); returns complex [P][K]."""
     N = np.asarray(images).shape[1]
     iCol = np.asarray(iCol)
     iRow = np.asarray(iRow)
     rows = np.where(iRow >= 0, iRow, iRow + N)
-    return image_spectra(images, bg_radius)[:, rows, iCol]
+    return image_spectra(images, bg_radius, masked)[:, rows, iCol]
 
 
-def image_spectra(images, bg_radius):
+def image_spectra(images, bg_radius, masked=False):
     """The stored half spectra [P][N][N/2 + 1] of the normalised images
     (the steps of particle_images before the gather): row r is frequency
     r (r < N/2) or r - N, column c frequency c."""
@@ -320,11 +338,12 @@ def image_spectra(images, bg_radius):
     h = N // 2
     j, i = np.meshgrid(np.arange(N) - h, np.arange(N) - h, indexing="ij")
     bg = (i * i + j * j) > bg_radius ** 2
+    mask = soft_mask(N, bg_radius) if masked else 1.0
     out = np.empty((P, N, h + 1), np.complex128)
     for p in range(P):
         x = images[p]
         v = x[bg]
-        y = (x - v.mean()) / v.std(ddof=1)
+        y = (x - v.mean()) / v.std(ddof=1) * mask
         y = np.roll(y, (-h, -h), axis=(0, 1))   # stored[r][c] = file[(r + h) % N][(c + h) % N]
         out[p] = np.fft.rfft2(y)
     return out

@@ -10,7 +10,8 @@ computes the statistics in FP64 and the FFT in FP32: every packed coefficient
 must agree with the oracle to 1e-5 of the largest magnitude, at box 64 and
 128 with their pixel sets, at a size with factors 3 and 5 (box 160), and for
 a background radius that reaches past the edge midpoints (only the corners
-are background).  Boxes up to 180 with N a multiple of 4 hold each image in
+are background); each case unmasked and with the soft mask of radius equal to
+the background radius (6-pixel cosine edge) applied.  Boxes up to 180 with N a multiple of 4 hold each image in
 registers (two register budgets: box 64 / 128 and box 160); box 66 (not a
 multiple of 4) and box 192 (too large) take the generic kernel.  The CTF and sigma arrays pass through unchanged.  Data set
 this way scores exactly like the same data uploaded with set_particles.
@@ -45,11 +46,12 @@ def _images(P, N, seed):
     return out
 
 
+@pytest.mark.parametrize("masked", [False, True])
 @pytest.mark.parametrize("N,cutoff,rbg", [(64, 24, 30.0), (128, 50, 60.0), (160, 64, 76.0),
                                           (64, 24, 40.0), (66, 24, 31.0), (192, 70, 90.0)])
-def test_particle_images_match_oracle(ictx, N, cutoff, rbg):
+def test_particle_images_match_oracle(ictx, N, cutoff, rbg, masked):
     P = 48
-    print(f"\n[images] N={N} cutoff={cutoff} bg radius={rbg} P={P}", flush=True)
+    print(f"\n[images] N={N} cutoff={cutoff} bg radius={rbg} P={P} masked={masked}", flush=True)
     iCol, iRow, iSig = wl.pixel_set(N, cutoff)
     ictx.set_pixels(iCol, iRow, iSig, N, 2)
     K = len(iCol)
@@ -57,10 +59,10 @@ def test_particle_images_match_oracle(ictx, N, cutoff, rbg):
     rng = np.random.default_rng(N)
     ctf = rng.uniform(-1, 1, size=(P, K)).astype(np.float32)
     sig = rng.uniform(0.5, 2, size=(P, K)).astype(np.float32)
-    ms = ictx.set_particle_images(imgs, rbg, ctf, sig)
+    ms = ictx.set_particle_images(imgs, rbg, ctf, sig, soft_mask=masked)
     dat, ctf2, sig2 = ictx.get_particles()
     got = dat[..., 0] + 1j * dat[..., 1]
-    ref = oracle.particle_images(imgs, rbg, iCol, iRow)
+    ref = oracle.particle_images(imgs, rbg, iCol, iRow, masked=masked)
     err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
     print(f"max error {err:.2e} of the largest coefficient; times (ms) {ms}")
     assert err <= 1e-5
@@ -113,7 +115,8 @@ def test_init_sigma_matches_oracle(ictx, N, cutoff, rbg, P):
     K = len(iCol)
     imgs = _images(P, N, 3 + N)
     ones = np.ones((P, K), np.float32)
-    ictx.set_particle_images(imgs, rbg, ones, ones)
+    # masked particle data; the estimate still reads the unmasked spectra
+    ictx.set_particle_images(imgs, rbg, ones, ones, soft_mask=True)
     nS = N // 2 - 1
     with pytest.raises(thx.THXError):
         ictx.init_sigma(nS + 1)

@@ -727,3 +727,42 @@ def test_init_sigma_white_noise():
     pooled = (sig[1:] * cnt[1:]).sum() / cnt[1:].sum() / (N * N / 2)
     assert abs(pooled - 1) < 6 / np.sqrt(P * cnt[1:].sum() / 2), pooled
     np.testing.assert_allclose(rcp, -0.5 / sig)
+
+
+def test_soft_mask_profile():
+    """The mask at hand-worked radii: 1 inside r (and at u = r), 0.5 halfway
+    along the 6-pixel edge, 0 at and beyond r + 6, (1 + cos(pi / 3)) / 2 =
+    0.75 a third of the way; radially symmetric about pixel (N/2, N/2)."""
+    N, r = 32, 8.0
+    m = oracle.soft_mask(N, r)
+    h = N // 2
+    assert m[h, h] == 1 and m[h, h + 8] == 1 and m[h + 7, h] == 1
+    assert m[h, h + 11] == pytest.approx(0.5, abs=1e-15)
+    assert m[h + 10, h] == pytest.approx(0.75, abs=1e-15)
+    assert m[h, h + 14] == pytest.approx(0.0, abs=1e-15) and m[h, h + 15] == 0 and m[0, 0] == 0
+    assert m[h - 11, h] == m[h, h - 11] == m[h + 11, h]
+
+
+def test_masked_images_explicit_loops():
+    """N = 12, radius 2 (so the cosine edge and the zero region both fall
+    inside the image): the masked image written pixel by pixel with math.cos
+    and its DFT by explicit sums at packed coefficients, against the oracle."""
+    N, r = 12, 2.0
+    h = N // 2
+    rng = np.random.default_rng(12)
+    img = rng.normal(0.5, 1.5, size=(N, N))
+    bgv = [img[y, x] for y in range(N) for x in range(N) if (x - h) ** 2 + (y - h) ** 2 > r * r]
+    mean = sum(bgv) / len(bgv)
+    sd = math.sqrt(sum((v - mean) ** 2 for v in bgv) / (len(bgv) - 1))
+    stored = np.zeros((N, N))
+    for y in range(N):
+        for x in range(N):
+            u = math.hypot(x - h, y - h)
+            w = 1.0 if u < r else (0.0 if u > r + 6 else 0.5 + 0.5 * math.cos((u - r) / 6 * math.pi))
+            stored[(y + h) % N, (x + h) % N] = (img[y, x] - mean) / sd * w
+    iCol = np.array([0, 1, 2, 5, 6, 3])
+    iRow = np.array([0, -1, 4, -6, 2, -3])
+    ref = np.array([sum(stored[y, x] * np.exp(-2j * np.pi * (iCol[k] * x + iRow[k] * y) / N)
+                        for y in range(N) for x in range(N)) for k in range(len(iCol))])
+    out = oracle.particle_images(img[None], r, iCol, iRow, masked=True)[0]
+    np.testing.assert_allclose(out, ref, atol=1e-10)

@@ -20,9 +20,13 @@
 //
 // normaliseImg (src/Optimiser.cpp:4987-5001) then scales every image by the
 // inverse of the mean background standard deviation, which step 1 has made
-// exactly 1 when statImg measures the same region; the soft mask of maskImg
-// (compiled in under OPTIMISER_MASK_IMG) is not applied, which is the
-// unmasked (_imgOri) branch of allocPreCal.
+// exactly 1 when statImg measures the same region.  On request step 1 also
+// writes the soft-masked image (maskImg with zeroMask, src/Optimiser.cpp:
+// 4953-4985, and reMaskImg, :6053-6105: the image times the raised-cosine
+// mask, zero beyond it), the one the expectation packs (allocPreCal(mask =
+// true), src/Optimiser.cpp:637, 1457); the unmasked spectra stay resident for
+// the initial sigma, which reads the unmasked images (OPTIMISER_SIGMA_MASK
+// off, include/Config.h:169).
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
@@ -31,11 +35,34 @@
 
 namespace thx {
 
+// Soft mask of radius r with a raised-cosine edge of kMaskEdge pixels at the
+// centred pixel (i, j), as softMask(Image&, r, ew) builds it
+// (src/Functions/Mask.cpp:333-350): 1 inside r, 0.5 + 0.5 cos(pi (u - r) / ew)
+// on the edge, 0 beyond r + ew, u = |(i, j)| in single precision like the
+// reference's RFLOAT image.
+constexpr int kMaskEdge = 6;   // EDGE_WIDTH_RL, include/Macro.h:82
+__device__ inline float soft_mask(int i, int j, float r) {
+    const float u = (float)sqrt((double)(i * i + j * j));
+    if (u > r + (float)kMaskEdge) return 0.f;
+    if (u >= r) return (float)(0.5 + 0.5 * cos((double)((u - r) / (float)kMaskEdge) * M_PI));
+    return 1.f;
+}
+
+// The mask of every pixel in the file layout (centre at (N/2, N/2)), built
+// once per call and read by every image's block (it stays in L2).
+__global__ void k_mask_table(int N, float r, float* __restrict__ table) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * N) return;
+    table[e] = soft_mask(e % N - N / 2, e / N - N / 2, r);
+}
+
 // One block per image.  Background statistics in FP64 (two passes: the mean,
 // then the sum of squared deviations), then the normalised image written
 // with its centre at the origin.
 __global__ __launch_bounds__(256) void k_img_normalise(const float* __restrict__ src, int N,
-                                                       float r2, float* __restrict__ dst) {
+                                                       float r2, float* __restrict__ dst,
+                                                       const float* __restrict__ mask,
+                                                       float* __restrict__ dstMasked) {
     const float* s = src + (size_t)blockIdx.x * N * N;
     float* d = dst + (size_t)blockIdx.x * N * N;
     const int h = N / 2;
@@ -79,9 +106,13 @@ __global__ __launch_bounds__(256) void k_img_normalise(const float* __restrict__
     }
     const double sd = sqrt(rs[0] / (n - 1));
     const float m = (float)mean, inv = (float)(1.0 / sd);
+    float* dm = dstMasked ? dstMasked + (size_t)blockIdx.x * N * N : nullptr;
     for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
         const int x = e % N, y = e / N;
-        d[((y + h) % N) * N + (x + h) % N] = (s[e] - m) * inv;
+        const size_t o = (size_t)((y + h) % N) * N + (x + h) % N;
+        const float v = (s[e] - m) * inv;
+        d[o] = v;
+        if (dm) dm[o] = v * mask[e];
     }
 }
 
@@ -102,7 +133,9 @@ __device__ inline double block_sum256(double v, double* red) {
 // written with one 16-byte store.
 template <int V4>
 __global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restrict__ src, int N,
-                                                           float r2, float* __restrict__ dst) {
+                                                           float r2, float* __restrict__ dst,
+                                                           const float* __restrict__ mask,
+                                                           float* __restrict__ dstMasked) {
     const float4* s = reinterpret_cast<const float4*>(src + (size_t)blockIdx.x * N * N);
     float* d = dst + (size_t)blockIdx.x * N * N;
     const int h = N / 2, n4 = N * N / 4;
@@ -156,9 +189,15 @@ __global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restri
         if (e4 < n4) {
             const int e = 4 * e4, y = e / N, x0 = e - y * N;
             const int yo = y + h < N ? y + h : y + h - N, xo = x0 + h < N ? x0 + h : x0 + h - N;
-            *reinterpret_cast<float4*>(d + (size_t)yo * N + xo) =
-                make_float4((x[v].x - m) * inv, (x[v].y - m) * inv, (x[v].z - m) * inv,
-                            (x[v].w - m) * inv);
+            const float4 o = make_float4((x[v].x - m) * inv, (x[v].y - m) * inv,
+                                         (x[v].z - m) * inv, (x[v].w - m) * inv);
+            *reinterpret_cast<float4*>(d + (size_t)yo * N + xo) = o;
+            if (dstMasked) {
+                const float4 w = reinterpret_cast<const float4*>(mask)[e4];
+                *reinterpret_cast<float4*>(dstMasked + (size_t)blockIdx.x * N * N +
+                                           (size_t)yo * N + xo) =
+                    make_float4(o.x * w.x, o.y * w.y, o.z * w.z, o.w * w.w);
+            }
         }
     }
 }
@@ -271,10 +310,11 @@ static int fft_status(hipfftResult r, const char* what) {
 void images_release(thx_ctx* c) {
     if (c->imgPlan) (void)hipfftDestroy(static_cast<hipfftHandle>(c->imgPlan));
     c->imgPlan = nullptr;
-    c->imgN = c->imgP = 0;
+    c->imgN = c->imgP = c->imgB = 0;
     c->imgIn.release();
     c->imgReal.release();
     c->imgFT.release();
+    c->imgMask.release();
     c->imgReady = false;
     c->sigPart.release();
     c->sigShell.release();
@@ -285,7 +325,8 @@ void images_release(thx_ctx* c) {
 using namespace thx;
 
 int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, float bgRadius,
-                                const float* ctfP, const float* sigRcpP, float* msOut) {
+                                int softMask, const float* ctfP, const float* sigRcpP,
+                                float* msOut) {
     clear_error();
     THX_CHECK(c && images && ctfP && sigRcpP && P > 0, THX_ERR_ARG,
               "set_particle_images: bad argument");
@@ -307,29 +348,36 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     THX_HIP(hipSetDevice(c->device));
     c->imgReady = false;
     hipStream_t st = c->stream;
-    const size_t nReal = (size_t)P * N * N, nFT = (size_t)P * N * (N / 2 + 1);
-    if (c->imgN != N || c->imgP != P) {
+    // with the soft mask, the masked images follow the unmasked ones in one
+    // batch of 2 P transforms: the particle data come from the masked
+    // spectra, the initial sigma from the unmasked ones
+    const int B = softMask ? 2 * P : P;
+    const size_t nImg = (size_t)N * N, nPix = (size_t)N * (N / 2 + 1);
+    const size_t nReal = (size_t)B * nImg, nFT = (size_t)B * nPix;
+    if (c->imgN != N || c->imgP != P || c->imgB != B) {
         if (c->imgPlan) (void)hipfftDestroy(static_cast<hipfftHandle>(c->imgPlan));
         c->imgPlan = nullptr;
-        c->imgN = c->imgP = 0;
+        c->imgN = c->imgP = c->imgB = 0;
         hipfftHandle plan = nullptr;
         THX_TRY(fft_status(hipfftCreate(&plan), "plan create"));
         c->imgPlan = plan;
         int n[2] = {N, N};
         size_t work = 0;
         THX_TRY(fft_status(hipfftMakePlanMany(plan, 2, n, nullptr, 1, N * N, nullptr, 1,
-                                              N * (N / 2 + 1), HIPFFT_R2C, P, &work),
+                                              N * (N / 2 + 1), HIPFFT_R2C, B, &work),
                            "batched plan"));
         c->imgN = N;
         c->imgP = P;
+        c->imgB = B;
     }
     hipfftHandle plan = static_cast<hipfftHandle>(c->imgPlan);
     THX_TRY(fft_status(hipfftSetStream(plan, st), "set stream"));
     THX_TRY(alloc_particles(c, P));
-    THX_TRY(c->imgIn.ensure(sizeof(float) * nReal));
+    THX_TRY(c->imgIn.ensure(sizeof(float) * P * nImg));
     THX_TRY(c->imgReal.ensure(sizeof(float) * nReal));
     THX_TRY(c->imgFT.ensure(sizeof(float2) * nFT));
-    THX_HIP(hipMemcpyAsync(c->imgIn.p, images, sizeof(float) * nReal, hipMemcpyHostToDevice, st));
+    THX_HIP(hipMemcpyAsync(c->imgIn.p, images, sizeof(float) * P * nImg, hipMemcpyHostToDevice,
+                           st));
     const size_t nk = (size_t)P * c->K;
     THX_HIP(hipMemcpyAsync(c->ctf.p, ctfP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
     THX_HIP(hipMemcpyAsync(c->sig.p, sigRcpP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
@@ -337,22 +385,33 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     // images of up to 16 K (N <= 128) or 32 K pixels (N <= 180) are held in
     // registers and read once; larger ones are re-read by the generic kernel
     const float r2 = bgRadius * bgRadius;
+    float* masked = softMask ? c->imgReal.as<float>() + P * nImg : nullptr;
+    if (softMask) {
+        THX_TRY(c->imgMask.ensure(sizeof(float) * nImg));
+        hipLaunchKernelGGL(k_mask_table, dim3((unsigned)((nImg + 255) / 256)), dim3(256), 0, st, N,
+                           bgRadius, c->imgMask.as<float>());
+        THX_HIP(hipGetLastError());
+    }
+    const float* mask = c->imgMask.as<const float>();
     if (N % 4 == 0 && N * N <= 1024 * 16)
         hipLaunchKernelGGL(k_img_normalise_reg<16>, dim3(P), dim3(256), 0, st,
-                           c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>());
+                           c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
+                           masked);
     else if (N % 4 == 0 && N * N <= 1024 * 32)
         hipLaunchKernelGGL(k_img_normalise_reg<32>, dim3(P), dim3(256), 0, st,
-                           c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>());
+                           c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
+                           masked);
     else
         hipLaunchKernelGGL(k_img_normalise, dim3(P), dim3(256), 0, st, c->imgIn.as<const float>(),
-                           N, r2, c->imgReal.as<float>());
+                           N, r2, c->imgReal.as<float>(), mask, masked);
     THX_HIP(hipGetLastError());
     THX_HIP(hipEventRecord(c->ev[1], st));
     THX_TRY(fft_status(hipfftExecR2C(plan, c->imgReal.as<hipfftReal>(), c->imgFT.as<hipfftComplex>()),
                        "forward transform"));
     THX_HIP(hipEventRecord(c->ev[2], st));
     hipLaunchKernelGGL(k_img_pack, dim3((c->K + 255) / 256, P), dim3(256), 0, st,
-                       c->imgFT.as<const float2>(), N, c->iCol.as<const int>(),
+                       c->imgFT.as<const float2>() + (softMask ? P * nPix : 0), N,
+                       c->iCol.as<const int>(),
                        c->iRow.as<const int>(), c->K, c->dat.as<float2>());
     THX_HIP(hipGetLastError());
     THX_HIP(hipEventRecord(c->ev[3], st));

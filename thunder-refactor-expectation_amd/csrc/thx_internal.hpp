@@ -142,8 +142,8 @@ struct thx_ctx {
     thx::DevBuf prjIn, prjReal, prjPad;
     // particle images: batched R2C FFT plan (hipfftHandle) for (imgN, imgP), buffers
     void* imgPlan = nullptr;
-    int imgN = 0, imgP = 0;
-    thx::DevBuf imgIn, imgReal, imgFT;
+    int imgN = 0, imgP = 0, imgB = 0;   // imgB transforms: P, or 2 P with the soft mask
+    thx::DevBuf imgIn, imgReal, imgFT, imgMask;
     bool imgReady = false;            // imgFT holds the spectra of the last images
     thx::DevBuf sigPart, sigShell;    // initial sigma: per-pixel partial sums, shell results
     int pixMaxCol = 0, pixMaxAbsRow = 0;   // extent of the pixel set

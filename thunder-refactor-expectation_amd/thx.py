@@ -36,7 +36,7 @@ _SIGS = {
     "thx_ctx_set_volume": [_p, _i, _p, _i],
     "thx_ctx_get_volume": [_p, _i, _p],
     "thx_ctx_set_projectee": [_p, _i, _p, _i, _i, _p],
-    "thx_ctx_set_particle_images": [_p, _p, _i, _i, C.c_float, _p, _p, _p],
+    "thx_ctx_set_particle_images": [_p, _p, _i, _i, C.c_float, _i, _p, _p, _p],
     "thx_ctx_init_sigma": [_p, _i, _i, _p, _p, _p],
     "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
     "thx_ctx_get_particles": [_p, _p, _p, _p],
@@ -225,18 +225,21 @@ class Context:
               "set_projectee")
         return ms
 
-    def set_particle_images(self, images, bg_radius, ctf, sig):
+    def set_particle_images(self, images, bg_radius, ctf, sig, soft_mask=False):
         """Particle images [P][N][N] (centre at pixel (N/2, N/2)) normalised by
         their background outside bg_radius, FFT'd and packed into the pixel set
         on the GPU (Optimiser::initImg's image path + allocPreCal's datP);
-        ctf and sig [P][K] as for set_particles.  Returns the normalisation,
-        FFT and packing times (ms)."""
+        with soft_mask, the packed data are those of the images times the
+        soft mask of radius bg_radius (6-pixel cosine edge).  ctf and sig
+        [P][K] as for set_particles.  Returns the normalisation, FFT and
+        packing times (ms)."""
         img = c32(images)
         P, N = img.shape[0], img.shape[1]
         assert img.shape == (P, N, N), img.shape
         ms = np.zeros(3, np.float32)
         check(load().thx_ctx_set_particle_images(self._h, ptr(img), P, N, float(bg_radius),
-                                                  ptr(c32(ctf)), ptr(c32(sig)), ptr(ms)),
+                                                  int(bool(soft_mask)), ptr(c32(ctf)),
+                                                  ptr(c32(sig)), ptr(ms)),
               "set_particle_images")
         self.P = P
         return ms
