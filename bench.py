@@ -395,12 +395,17 @@ def main():
                     help="skip the full-size box-160 ragged local scoring")
     ap.add_argument("--no-box256", action="store_true",
                     help="skip the full-size box-256 FP8 global scan")
+    ap.add_argument("--images-only", action="store_true",
+                    help="only the producer side (particle images and initial sigma), for profiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(the particles are independent: each GPU would run this path on its share)"}))
         sys.exit(2)
     name = args.workload
+    if args.images_only:
+        print(json.dumps({"images": run_images(name, args.steps)}))
+        return
     w = wl.WORKLOADS[name]
     ctx = thx.Context(0)
     log(f"setting up {name}")
