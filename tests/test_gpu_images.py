@@ -140,3 +140,26 @@ def test_init_sigma_needs_images(gpu_ctx):
             ctx.init_sigma(10)
     finally:
         ctx.close()
+
+
+def test_more_images_than_a_y_grid(ictx):
+    """70,000 images of 16^2 (more than a 65,535-block grid dimension): every
+    particle packed, soft-masked, and given its sigRcp."""
+    N, P = 16, 70_000
+    print(f"\n[images] N={N} P={P} masked, with sigma", flush=True)
+    iCol, iRow, iSig = wl.pixel_set(N, 6)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    K = len(iCol)
+    rng = np.random.default_rng(70)
+    imgs = rng.normal(0.0, 1.0, size=(P, N, N)).astype(np.float32)
+    ones = np.ones((P, K), np.float32)
+    ictx.set_particle_images(imgs, 5.0, ones, ones, soft_mask=True)
+    sg, rcp, _ = ictx.init_sigma(N // 2 - 1, apply=True)
+    dat, _, sig = ictx.get_particles()
+    got = dat[..., 0] + 1j * dat[..., 1]
+    ref = oracle.particle_images(imgs, 5.0, iCol, iRow, masked=True)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"max error {err:.2e} of the largest coefficient", flush=True)
+    assert err <= 1e-5
+    np.testing.assert_array_equal(sig, np.broadcast_to(rcp[np.asarray(iSig)].astype(np.float32),
+                                                       (P, K)))

@@ -204,11 +204,12 @@ __global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restri
 
 // dat[p][k] = FT_p[row(k)][iCol[k]], row = iRow wrapped to [0, N), FT_p the
 // (N/2 + 1)-column half spectrum of image p.
+// Grid (P, ceil(K / 256)): the particle count can exceed a y dimension's limit.
 __global__ void k_img_pack(const float2* __restrict__ ft, int N, const int* __restrict__ iCol,
                            const int* __restrict__ iRow, int K, float2* __restrict__ dat) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    const int p = blockIdx.y;
+    const int p = blockIdx.x;
     const int j = iRow[k], row = j >= 0 ? j : j + N;
     dat[(size_t)p * K + k] = ft[((size_t)p * N + row) * (N / 2 + 1) + iCol[k]];
 }
@@ -290,12 +291,12 @@ __global__ __launch_bounds__(256) void k_sigma_shells(const double* __restrict__
 }
 
 // sigRcpP[p][k] = sigRcp(iSig[k]) for every particle, as allocPreCal fills
-// it for one group (src/Optimiser.cpp:8017-8022).
+// it for one group (src/Optimiser.cpp:8017-8022).  Grid (P, ceil(K / 256)).
 __global__ void k_sigma_apply(const double* __restrict__ rcp, const int* __restrict__ iSig, int K,
                               float* __restrict__ sig) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= K) return;
-    sig[(size_t)blockIdx.y * K + k] = (float)rcp[iSig[k]];
+    sig[(size_t)blockIdx.x * K + k] = (float)rcp[iSig[k]];
 }
 
 static int fft_status(hipfftResult r, const char* what) {
@@ -409,7 +410,7 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     THX_TRY(fft_status(hipfftExecR2C(plan, c->imgReal.as<hipfftReal>(), c->imgFT.as<hipfftComplex>()),
                        "forward transform"));
     THX_HIP(hipEventRecord(c->ev[2], st));
-    hipLaunchKernelGGL(k_img_pack, dim3((c->K + 255) / 256, P), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_img_pack, dim3(P, (c->K + 255) / 256), dim3(256), 0, st,
                        c->imgFT.as<const float2>() + (softMask ? P * nPix : 0), N,
                        c->iCol.as<const int>(),
                        c->iRow.as<const int>(), c->K, c->dat.as<float2>());
@@ -458,7 +459,7 @@ int thx_ctx_init_sigma(thx_ctx* c, int nShell, int apply, double* sigOut, double
                        N, P, c->sigShell.as<double>(), nShell);
     THX_HIP(hipGetLastError());
     if (apply) {
-        hipLaunchKernelGGL(k_sigma_apply, dim3((c->K + 255) / 256, P), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_sigma_apply, dim3(P, (c->K + 255) / 256), dim3(256), 0, st,
                            c->sigShell.as<const double>() + nShell, c->iSig.as<const int>(), c->K,
                            c->sig.as<float>());
         THX_HIP(hipGetLastError());
