@@ -143,8 +143,9 @@ def test_init_sigma_needs_images(gpu_ctx):
 
 
 def test_more_images_than_a_y_grid(ictx):
-    """70,000 images of 16^2 (more than a 65,535-block grid dimension): every
-    particle packed, soft-masked, and given its sigRcp."""
+    """70,000 images of 16^2 (more than 65,535; the kernels put particles on
+    the grid's x dimension): every particle packed, soft-masked, and given its
+    sigRcp."""
     N, P = 16, 70_000
     print(f"\n[images] N={N} P={P} masked, with sigma", flush=True)
     iCol, iRow, iSig = wl.pixel_set(N, 6)

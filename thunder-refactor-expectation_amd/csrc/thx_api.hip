@@ -1451,8 +1451,9 @@ __global__ void k_rotmat_colmajor(const double* __restrict__ q, int nR, double* 
 __global__ __launch_bounds__(256) void k_project_mat(
     const float2* __restrict__ vol, int V, int pf, const double* __restrict__ mat, int R,
     const int* __restrict__ iCol, const int* __restrict__ iRow, int K, float2* __restrict__ out) {
-    const int r = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    // grid (R, ceil(K / 256)): rotations on x, the dimension with the largest limit
+    const int r = blockIdx.x;
+    const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= K || r >= R) return;
     const double* m = mat + 9 * (size_t)r;   // column-major
     const double c0 = (double)(iCol[k] * pf), c1 = (double)(iRow[k] * pf);
@@ -1567,7 +1568,7 @@ int thx_ExpectProject(const float* volume, float* rotP, const double* rotMat, co
             s = THX_ERR_HIP;
             break;
         }
-        hipLaunchKernelGGL(k_project_mat, dim3((npxl + 255) / 256, nR), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(k_project_mat, dim3(nR, (npxl + 255) / 256), dim3(256), 0, c->stream,
                            dV.as<const float2>(), vdim, pf, dM.as<const double>(), nR,
                            dCol.as<const int>(), dRow.as<const int>(), npxl, dOut.as<float2>());
         if (hipStreamSynchronize(c->stream) != hipSuccess ||

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restri
 
 // dat[p][k] = FT_p[row(k)][iCol[k]], row = iRow wrapped to [0, N), FT_p the
 // (N/2 + 1)-column half spectrum of image p.
-// Grid (P, ceil(K / 256)): the particle count can exceed a y dimension's limit.
+// Grid (P, ceil(K / 256)): particles on x, the dimension with the largest limit.
 __global__ void k_img_pack(const float2* __restrict__ ft, int N, const int* __restrict__ iCol,
                            const int* __restrict__ iRow, int K, float2* __restrict__ dat) {
     const int k = blockIdx.y * blockDim.x + threadIdx.x;
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void k_sigma_shells(const double* __restrict__
 }
 
 // sigRcpP[p][k] = sigRcp(iSig[k]) for every particle, as allocPreCal fills
-// it for one group (src/Optimiser.cpp:8017-8022).  Grid (P, ceil(K / 256)).
+// it for one group (src/Optimiser.cpp:8017-8022).  Grid (P, ceil(K / 256)), particles on x.
 __global__ void k_sigma_apply(const double* __restrict__ rcp, const int* __restrict__ iSig, int K,
                               float* __restrict__ sig) {
     const int k = blockIdx.y * blockDim.x + threadIdx.x;
