@@ -11,9 +11,10 @@ must agree with the oracle to 1e-5 of the largest magnitude, at box 64 and
 128 with their pixel sets, at a size with factors 3 and 5 (box 160), and for
 a background radius that reaches past the edge midpoints (only the corners
 are background); each case unmasked and with the soft mask of radius equal to
-the background radius (6-pixel cosine edge) applied.  Boxes up to 180 with N a multiple of 4 hold each image in
-registers (two register budgets: box 64 / 128 and box 160); box 66 (not a
-multiple of 4) and box 192 (too large) take the generic kernel.  The CTF and sigma arrays pass through unchanged.  Data set
+the background radius (6-pixel cosine edge) applied.  Boxes up to 256 with N a multiple of 4 hold each image in
+registers (three register budgets: box 64 / 128, box 160, and box 192 / 256
+with 512-thread blocks); box 66 (not a multiple of 4) takes the generic
+kernel.  The CTF and sigma arrays pass through unchanged.  Data set
 this way scores exactly like the same data uploaded with set_particles.
 """
 import numpy as np
@@ -48,7 +49,8 @@ def _images(P, N, seed):
 
 @pytest.mark.parametrize("masked", [False, True])
 @pytest.mark.parametrize("N,cutoff,rbg", [(64, 24, 30.0), (128, 50, 60.0), (160, 64, 76.0),
-                                          (64, 24, 40.0), (66, 24, 31.0), (192, 70, 90.0)])
+                                          (64, 24, 40.0), (66, 24, 31.0), (192, 70, 90.0),
+                                          (256, 100, 120.0)])
 def test_particle_images_match_oracle(ictx, N, cutoff, rbg, masked):
     P = 48
     print(f"\n[images] N={N} cutoff={cutoff} bg radius={rbg} P={P} masked={masked}", flush=True)

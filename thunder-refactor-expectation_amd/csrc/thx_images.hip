@@ -116,34 +116,39 @@ __global__ __launch_bounds__(256) void k_img_normalise(const float* __restrict__
     }
 }
 
-// Block-wide FP64 sum over 256 threads (four waves): wave shuffles, then
-// one value per wave through LDS.  Every thread returns the total.
-__device__ inline double block_sum256(double v, double* red) {
+// Block-wide FP64 sum over NT threads (NT / 64 waves): wave shuffles, then
+// one value per wave through LDS, added in wave order.  Every thread returns
+// the total.
+template <int NT = 256>
+__device__ inline double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     __syncthreads();   // red may still be read by a previous call
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    return (red[0] + red[1]) + (red[2] + red[3]);
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) t += red[w];
+    return t;
 }
 
 // The same normalisation with the image held in registers: V4 float4s per
-// thread cover N * N <= 1024 * V4 pixels (N a multiple of 4), so each image is
-// read from HBM once.  A group of four x-consecutive pixels starts at a
+// thread of NT cover N * N <= 4 * NT * V4 pixels (N a multiple of 4), so each
+// image is read from HBM once.  A group of four x-consecutive pixels starts at a
 // multiple of 4 and N / 2 is even, so the moved group stays contiguous and is
 // written with one 16-byte store.
-template <int V4>
-__global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restrict__ src, int N,
+template <int V4, int NT>
+__global__ __launch_bounds__(NT) void k_img_normalise_reg(const float* __restrict__ src, int N,
                                                            float r2, float* __restrict__ dst,
                                                            const float* __restrict__ mask,
                                                            float* __restrict__ dstMasked) {
     const float4* s = reinterpret_cast<const float4*>(src + (size_t)blockIdx.x * N * N);
     float* d = dst + (size_t)blockIdx.x * N * N;
     const int h = N / 2, n4 = N * N / 4;
-    __shared__ double red[4];
+    __shared__ double red[NT / 64];
     float4 x[V4];
 #pragma unroll
     for (int v = 0; v < V4; v++) {
-        const int e4 = threadIdx.x + 256 * v;
+        const int e4 = threadIdx.x + NT * v;
         x[v] = e4 < n4 ? s[e4] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     // background membership of the 4 V4 pixels, bit 4 v + c
@@ -152,7 +157,7 @@ __global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restri
     int cnt = 0;
 #pragma unroll
     for (int v = 0; v < V4; v++) {
-        const int e4 = threadIdx.x + 256 * v;
+        const int e4 = threadIdx.x + NT * v;
         if (e4 < n4) {
             const int e = 4 * e4, y = e / N, x0 = e - y * N;
             const int j = y - h;
@@ -168,8 +173,8 @@ __global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restri
             }
         }
     }
-    const double n = block_sum256((double)cnt, red);
-    const double mean = block_sum256(sum, red) / n;
+    const double n = block_sum<NT>((double)cnt, red);
+    const double mean = block_sum<NT>(sum, red) / n;
     double ss = 0.0;
 #pragma unroll
     for (int v = 0; v < V4; v++) {
@@ -181,11 +186,11 @@ __global__ __launch_bounds__(256) void k_img_normalise_reg(const float* __restri
                 ss += dv * dv;
             }
     }
-    const double sd = sqrt(block_sum256(ss, red) / (n - 1.0));
+    const double sd = sqrt(block_sum<NT>(ss, red) / (n - 1.0));
     const float m = (float)mean, inv = (float)(1.0 / sd);
 #pragma unroll
     for (int v = 0; v < V4; v++) {
-        const int e4 = threadIdx.x + 256 * v;
+        const int e4 = threadIdx.x + NT * v;
         if (e4 < n4) {
             const int e = 4 * e4, y = e / N, x0 = e - y * N;
             const int yo = y + h < N ? y + h : y + h - N, xo = x0 + h < N ? x0 + h : x0 + h - N;
@@ -278,9 +283,9 @@ __global__ __launch_bounds__(256) void k_sigma_shells(const double* __restrict__
         aA += part[pix] + part[nPix + pix];
         cnt++;
     }
-    const double n = block_sum256((double)cnt, red);
-    const double sumP = block_sum256(aP, red);
-    const double sumA = block_sum256(aA, red);
+    const double n = block_sum<256>((double)cnt, red);
+    const double sumP = block_sum<256>(aP, red);
+    const double sumA = block_sum<256>(aA, red);
     if (threadIdx.x == 0) {
         const double avgPs = sumP / ((double)P * n);
         const double ring = sumA / (double)P / n;
@@ -383,8 +388,9 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     THX_HIP(hipMemcpyAsync(c->ctf.p, ctfP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
     THX_HIP(hipMemcpyAsync(c->sig.p, sigRcpP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
     THX_HIP(hipEventRecord(c->ev[0], st));
-    // images of up to 16 K (N <= 128) or 32 K pixels (N <= 180) are held in
-    // registers and read once; larger ones are re-read by the generic kernel
+    // images of up to 16 K (N <= 128), 32 K (N <= 180) or, with 512-thread
+    // blocks, 64 K pixels (N <= 256) are held in registers and read once;
+    // others are re-read by the generic kernel
     const float r2 = bgRadius * bgRadius;
     float* masked = softMask ? c->imgReal.as<float>() + P * nImg : nullptr;
     if (softMask) {
@@ -395,11 +401,15 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     }
     const float* mask = c->imgMask.as<const float>();
     if (N % 4 == 0 && N * N <= 1024 * 16)
-        hipLaunchKernelGGL(k_img_normalise_reg<16>, dim3(P), dim3(256), 0, st,
+        hipLaunchKernelGGL((k_img_normalise_reg<16, 256>), dim3(P), dim3(256), 0, st,
                            c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
                            masked);
     else if (N % 4 == 0 && N * N <= 1024 * 32)
-        hipLaunchKernelGGL(k_img_normalise_reg<32>, dim3(P), dim3(256), 0, st,
+        hipLaunchKernelGGL((k_img_normalise_reg<32, 256>), dim3(P), dim3(256), 0, st,
+                           c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
+                           masked);
+    else if (N % 4 == 0 && N * N <= 2048 * 32)
+        hipLaunchKernelGGL((k_img_normalise_reg<32, 512>), dim3(P), dim3(512), 0, st,
                            c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
                            masked);
     else
