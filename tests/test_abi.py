@@ -45,6 +45,11 @@ def test_argument_errors_are_status_codes():
     st = lib.thx_ctx_create(-1, C.byref(h))
     assert st != 0 and not h
     assert lib.thx_ctx_destroy(None) == 0
+    # producer side: a missing context is an argument error, before any device work
+    st = lib.thx_ctx_set_particle_images(None, None, 4, 64, 30.0, 1, None, None, None)
+    assert st == 1 and b"bad argument" in lib.thx_last_error()
+    st = lib.thx_ctx_init_sigma(None, 31, 0, None, None, None)
+    assert st == 1 and b"null context" in lib.thx_last_error()
 
 
 def test_device_count_without_gpu_does_not_crash():
