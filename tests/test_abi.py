@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 import thx
@@ -57,3 +58,21 @@ def test_device_count_without_gpu_does_not_crash():
     st = thx.load().thx_device_count(C.byref(n))
     assert st in (0, 2)
     assert n.value >= 0
+
+
+def test_particle_arrays_must_match_the_pixel_set():
+    """The library copies P * K values from each host array: the binding
+    rejects arrays of another shape before any call (a short array would be
+    read past its end)."""
+    P, K = 5, 7
+    ok = np.ones((P, K), np.float32)
+    ctf, sig, dat = thx.particle_arrays(P, K, ok, ok, np.zeros((P, K, 2)))
+    assert ctf.dtype == np.float32 and dat.shape == (P, K, 2)
+    with pytest.raises(ValueError, match="ctf"):
+        thx.particle_arrays(P, K, np.ones((P, K - 1)), ok)
+    with pytest.raises(ValueError, match="sig"):
+        thx.particle_arrays(P, K, ok, np.ones((P - 1, K)))
+    with pytest.raises(ValueError, match="sig"):
+        thx.particle_arrays(P, K, ok, np.ones(P * K))
+    with pytest.raises(ValueError, match="dat"):
+        thx.particle_arrays(P, K, ok, ok, np.zeros((P, K - 1, 2)))

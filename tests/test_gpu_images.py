@@ -12,9 +12,10 @@ must agree with the oracle to 1e-5 of the largest magnitude, at box 64 and
 a background radius that reaches past the edge midpoints (only the corners
 are background); each case unmasked and with the soft mask of radius equal to
 the background radius (6-pixel cosine edge) applied.  Boxes up to 256 with N a multiple of 4 hold each image in
-registers (three register budgets: box 64 / 128, box 160, and box 192 / 256
+registers (three register budgets: box 64 / 128, box 100 / 160 / 180, and box 192 / 256
 with 512-thread blocks); box 66 (not a multiple of 4) takes the generic
-kernel.  The CTF and sigma arrays pass through unchanged.  Data set
+kernel.  Boxes 100 and 180 (N / 2 not a multiple of 4) move one group of
+four pixels per row across the row's wrap.  The CTF and sigma arrays pass through unchanged.  Data set
 this way scores exactly like the same data uploaded with set_particles.
 """
 import numpy as np
@@ -50,7 +51,8 @@ def _images(P, N, seed):
 @pytest.mark.parametrize("masked", [False, True])
 @pytest.mark.parametrize("N,cutoff,rbg", [(64, 24, 30.0), (128, 50, 60.0), (160, 64, 76.0),
                                           (64, 24, 40.0), (66, 24, 31.0), (192, 70, 90.0),
-                                          (256, 100, 120.0)])
+                                          (256, 100, 120.0), (100, 40, 48.0),
+                                          (180, 70, 85.0)])
 def test_particle_images_match_oracle(ictx, N, cutoff, rbg, masked):
     P = 48
     print(f"\n[images] N={N} cutoff={cutoff} bg radius={rbg} P={P} masked={masked}", flush=True)
@@ -131,6 +133,36 @@ def test_init_sigma_matches_oracle(ictx, N, cutoff, rbg, P):
     got = ictx.get_particles()[2]
     want = np.broadcast_to(ref_rcp[np.asarray(iSig)].astype(np.float32), (P, K))
     np.testing.assert_allclose(got, want, rtol=1e-4)
+
+
+def test_init_sigma_apply_needs_image_particles(ictx):
+    """After set_particles replaces the particle arrays (same count), the
+    spectra of the last images no longer describe them: apply is refused,
+    the estimate alone is still available."""
+    N, P = 64, 16
+    print(f"\n[sigma] apply after set_particles N={N} P={P}", flush=True)
+    iCol, iRow, iSig = wl.pixel_set(N, 24)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    K = len(iCol)
+    ones = np.ones((P, K), np.float32)
+    ictx.set_particle_images(_images(P, N, 11), 30.0, ones, ones)
+    ictx.set_particles(np.zeros((P, K, 2), np.float32), ones, ones)
+    with pytest.raises(thx.THXError, match="not the last images"):
+        ictx.init_sigma(N // 2 - 1, apply=True)
+    ictx.init_sigma(N // 2 - 1, apply=False)
+
+
+def test_set_particles_rejects_wrong_shapes(ictx):
+    """ctf / sig of another shape than (P, K) raise before the library reads
+    the host arrays."""
+    N, P = 64, 4
+    iCol, iRow, iSig = wl.pixel_set(N, 24)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    K = len(iCol)
+    with pytest.raises(ValueError):
+        ictx.set_particles(np.zeros((P, K, 2), np.float32), np.ones((P, K)), np.ones((P, K - 1)))
+    with pytest.raises(ValueError):
+        ictx.set_particle_images(_images(P, N, 1), 30.0, np.ones((P, K - 3)), np.ones((P, K)))
 
 
 def test_init_sigma_needs_images(gpu_ctx):

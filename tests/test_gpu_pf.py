@@ -156,3 +156,38 @@ def test_proposal_sample_density_and_frequencies(gpu_ctx):
         sh = mp[6 * C:]
         assert np.abs(ts.mean(0) - sh[:2]).max() < 5 * sh[2] / np.sqrt(nTr)
         assert abs((ts - ts.mean(0)).std() / sh[2] - 1) < 0.1
+
+
+def test_more_particles_than_a_y_grid(gpu_ctx):
+    """70,000 particles (more than 65,535; the particle-filter kernels put
+    particles on the grid's x dimension): every particle's samples are
+    perturbed, resampled and drawn."""
+    rng = np.random.default_rng(70)
+    P = 70_000
+    print(f"\n[pf] P={P}: perturb, resample, proposal draws", flush=True)
+    nR = np.full(P, 2, np.int32)
+    quat = rng.normal(size=(2 * P, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    d = np.tile([1.0, 0, 0, 0], (2 * P, 1))          # identity perturbation
+    mean = np.tile([1.0, 0, 0, 0], (P, 1))
+    out, _ = gpu_ctx.pf_perturb(quat=quat, nR=nR, mean=mean, k=np.ones((P, 3)), d=d)
+    np.testing.assert_allclose(out, quat, atol=1e-15)
+    t = np.zeros((P, 2))
+    _, tout = gpu_ctx.pf_perturb(trans=t, nT=np.ones(P, np.int32),
+                                 s01rho=np.tile([1.0, 1.0, 0.0], (P, 1)), pf=1.0, seed=9)
+    assert np.all(tout != 0.0)           # every particle's shift was drawn
+    nIn = np.full(P, 4, np.int32)
+    nOut = np.full(P, 3, np.int32)
+    w = np.ones(4 * P)
+    u = rng.uniform(0.1, 1.0, 4 * P)
+    u0 = rng.uniform(0, 1, P) / 3
+    idx, wn, _ = gpu_ctx.pf_resample(w, u, nIn, nOut, u0=u0)
+    for p in list(range(0, P, 9973)) + [P - 1]:
+        ri, rw = oracle.resample(w[4 * p:4 * p + 4], u[4 * p:4 * p + 4], 3, u0[p])
+        np.testing.assert_array_equal(idx[3 * p:3 * p + 3], ri)
+        np.testing.assert_allclose(wn[3 * p:3 * p + 3], rw, rtol=1e-12)
+    np.testing.assert_allclose(wn.reshape(P, 3).sum(1), 1.0, rtol=1e-12)
+    mix = np.tile([1.0, 0, 0, 0, 0.01, 1.0, 0.0, 0.0, 1.0], (P, 1))
+    q, dens, tt = gpu_ctx.proposal_sample(mix, 1, 2, 1, seed=4)
+    np.testing.assert_allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-12)
+    assert np.all(dens > 0) and np.all(tt != 0.0)

@@ -283,6 +283,7 @@ int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used) {
 
 // Particle input arrays for P particles (and the derived operands' rows).
 int alloc_particles(thx_ctx* c, int P) {
+    c->partFromImages = false;   // set again by the images path once it has packed them
     c->P = P;
     c->Ppad = round_up(P, 128);
     THX_TRY(c->dat.ensure(sizeof(float2) * (size_t)P * c->K));

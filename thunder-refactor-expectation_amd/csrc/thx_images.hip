@@ -134,8 +134,11 @@ __device__ inline double block_sum(double v, double* red) {
 // The same normalisation with the image held in registers: V4 float4s per
 // thread of NT cover N * N <= 4 * NT * V4 pixels (N a multiple of 4), so each
 // image is read from HBM once.  A group of four x-consecutive pixels starts at a
-// multiple of 4 and N / 2 is even, so the moved group stays contiguous and is
-// written with one 16-byte store.
+// multiple of 4.  When N / 2 is a multiple of 4 the moved group stays
+// contiguous and leaves in one 16-byte store; when N / 2 is 2 more than a
+// multiple of 4 (N = 100, 180, ...) the group that the centre shift moves to
+// column N - 2 wraps within its row: its first two pixels go to columns
+// N - 2 and N - 1, its last two to columns 0 and 1, as two 8-byte stores.
 template <int V4, int NT>
 __global__ __launch_bounds__(NT) void k_img_normalise_reg(const float* __restrict__ src, int N,
                                                            float r2, float* __restrict__ dst,
@@ -196,12 +199,24 @@ __global__ __launch_bounds__(NT) void k_img_normalise_reg(const float* __restric
             const int yo = y + h < N ? y + h : y + h - N, xo = x0 + h < N ? x0 + h : x0 + h - N;
             const float4 o = make_float4((x[v].x - m) * inv, (x[v].y - m) * inv,
                                          (x[v].z - m) * inv, (x[v].w - m) * inv);
-            *reinterpret_cast<float4*>(d + (size_t)yo * N + xo) = o;
+            float* row = d + (size_t)yo * N;
+            float* rowM = dstMasked ? dstMasked + (size_t)blockIdx.x * N * N + (size_t)yo * N
+                                    : nullptr;
+            float4 om = o;
             if (dstMasked) {
                 const float4 w = reinterpret_cast<const float4*>(mask)[e4];
-                *reinterpret_cast<float4*>(dstMasked + (size_t)blockIdx.x * N * N +
-                                           (size_t)yo * N + xo) =
-                    make_float4(o.x * w.x, o.y * w.y, o.z * w.z, o.w * w.w);
+                om = make_float4(o.x * w.x, o.y * w.y, o.z * w.z, o.w * w.w);
+            }
+            if (xo + 4 <= N) {
+                *reinterpret_cast<float4*>(row + xo) = o;
+                if (rowM) *reinterpret_cast<float4*>(rowM + xo) = om;
+            } else {   // xo == N - 2: the group wraps to the start of its row
+                *reinterpret_cast<float2*>(row + xo) = make_float2(o.x, o.y);
+                *reinterpret_cast<float2*>(row) = make_float2(o.z, o.w);
+                if (rowM) {
+                    *reinterpret_cast<float2*>(rowM + xo) = make_float2(om.x, om.y);
+                    *reinterpret_cast<float2*>(rowM) = make_float2(om.z, om.w);
+                }
             }
         }
     }
@@ -428,6 +443,7 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     THX_HIP(hipEventRecord(c->ev[3], st));
     THX_HIP(hipStreamSynchronize(st));
     c->imgReady = true;
+    c->partFromImages = true;
     c->nev = 0;   // the events now time this call
     c->evNames.clear();
     if (msOut) {
@@ -447,7 +463,7 @@ int thx_ctx_init_sigma(thx_ctx* c, int nShell, int apply, double* sigOut, double
     THX_CHECK(nShell >= 1 && nShell <= N / 2 - 1, THX_ERR_ARG,
               "init_sigma: nShell must be 1 .. N/2 - 1 (Optimiser::maxR)");
     if (apply) {
-        THX_CHECK(c->P == P && c->K > 0, THX_ERR_STATE,
+        THX_CHECK(c->partFromImages && c->P == P && c->K > 0, THX_ERR_STATE,
                   "init_sigma: the particles are not the last images");
         THX_CHECK(c->maxShell < nShell, THX_ERR_RANGE,
                   "init_sigma: the pixel set has shells beyond nShell");

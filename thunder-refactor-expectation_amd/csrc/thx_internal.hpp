@@ -145,6 +145,7 @@ struct thx_ctx {
     int imgN = 0, imgP = 0, imgB = 0;   // imgB transforms: P, or 2 P with the soft mask
     thx::DevBuf imgIn, imgReal, imgFT, imgMask;
     bool imgReady = false;            // imgFT holds the spectra of the last images
+    bool partFromImages = false;      // the particle arrays were packed from those images
     thx::DevBuf sigPart, sigShell;    // initial sigma: per-pixel partial sums, shell results
     int pixMaxCol = 0, pixMaxAbsRow = 0;   // extent of the pixel set
     // particles (image-major, as uploaded) and derived operands

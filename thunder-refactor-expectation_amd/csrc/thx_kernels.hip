@@ -163,8 +163,8 @@ __global__ __launch_bounds__(256) void k_project(
     const float2* __restrict__ vol, int V, int pf, const double* __restrict__ quat,
     int R, const int* __restrict__ iCol, const int* __restrict__ iRow, int K,
     int Kp, float2* __restrict__ out, float* __restrict__ outNorm) {
-    const int r = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.x;   // rotations on x, the grid dimension with the largest limit
+    const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= Kp) return;
     const size_t o = (size_t)r * Kp + k;
     if (r >= R || k >= K) {
@@ -189,7 +189,7 @@ int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
         c->rotLoPrec = -1;
         c->shellMaxPOk = false;
     }
-    dim3 grid((c->Kp + 255) / 256, Rpad);
+    dim3 grid(Rpad, (c->Kp + 255) / 256);
     hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c->stream,
                        c->vol[cls].as<const float2>(), c->vdim[cls], c->pf, dQuat, R,
                        c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->Kp,
@@ -206,8 +206,8 @@ int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
 __global__ void k_translate(const double* __restrict__ trans, int T, int Tpad,
                             const int* __restrict__ iCol, const int* __restrict__ iRow,
                             int K, int Kp, int N, float2* __restrict__ S) {
-    const int t = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x;   // translations on x, the grid dimension with the largest limit
+    const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= Kp || t >= Tpad) return;
     float2 v = make_float2(0.f, 0.f);
     if (t < T && k < K) {
@@ -220,7 +220,7 @@ __global__ void k_translate(const double* __restrict__ trans, int T, int Tpad,
 
 int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, const int* iCol,
                          const int* iRow, int K, int Kp, int N, float2* S) {
-    dim3 grid((Kp + 255) / 256, Tpad);
+    dim3 grid(Tpad, (Kp + 255) / 256);
     hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, st, trans, T, Tpad, iCol, iRow, K, Kp, N,
                        S);
     THX_HIP(hipGetLastError());
@@ -228,7 +228,7 @@ int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, c
 }
 
 int launch_translate_table(thx_ctx* c) {
-    dim3 grid((c->Kp + 255) / 256, c->TpadS);
+    dim3 grid(c->TpadS, (c->Kp + 255) / 256);
     hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, c->stream, c->trans.as<const double>(),
                        c->T, c->TpadS, c->iCol.as<const int>(), c->iRow.as<const int>(), c->K,
                        c->Kp, c->idim, c->S.as<float2>());
@@ -2432,8 +2432,8 @@ __global__ __launch_bounds__(256) void k_synth_signal(
     const int* __restrict__ iCol, const int* __restrict__ iRow, int K, int N,
     float voltage, float Cs, float ampC, float pixelSize, float2* __restrict__ dat,
     float* __restrict__ ctfOut, double* __restrict__ energy) {
-    const int p = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.x;   // particles on x, the grid dimension with the largest limit
+    const int k = blockIdx.y * blockDim.x + threadIdx.x;
     double e = 0.0;
     if (k < K) {
         const double* q = params + 9 * (size_t)p;
@@ -2464,12 +2464,12 @@ __global__ __launch_bounds__(256) void k_synth_signal(
         __syncthreads();
     }
     // one partial per block, summed in order on the host (deterministic)
-    if (threadIdx.x == 0) energy[(size_t)p * gridDim.x + blockIdx.x] = red[0];
+    if (threadIdx.x == 0) energy[(size_t)p * gridDim.y + blockIdx.y] = red[0];
 }
 
 int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams, double* dEnergy,
                         double voltage, double Cs, double ampContrast, double pixelSize) {
-    dim3 grid((c->K + 255) / 256, P);
+    dim3 grid(P, (c->K + 255) / 256);
     hipLaunchKernelGGL(k_synth_signal, grid, dim3(256), 0, c->stream, c->vol[cls].as<const float2>(),
                        c->vdim[cls], c->pf, dParams, c->iCol.as<const int>(),
                        c->iRow.as<const int>(), c->K, c->idim, (float)voltage, (float)Cs,

@@ -25,8 +25,8 @@ __global__ void k_perturb_rot(double* __restrict__ quat, const int* __restrict__
                               const int* __restrict__ cnt, const double* __restrict__ mean,
                               const double* __restrict__ kk, double kmax, double pf, int P,
                               uint64_t seed, uint32_t phase, const double* __restrict__ dGiven) {
-    const int p = blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.x;   // particles on x, the grid dimension with the largest limit
+    const int j = blockIdx.y * blockDim.x + threadIdx.x;
     if (p >= P || j >= cnt[p]) return;
     const size_t i = (size_t)off[p] + j;
     double d[4];
@@ -57,8 +57,8 @@ __global__ void k_perturb_rot(double* __restrict__ quat, const int* __restrict__
 __global__ void k_perturb_trans(double* __restrict__ trans, const int* __restrict__ off,
                                 const int* __restrict__ cnt, const double* __restrict__ s01rho,
                                 double pf, int P, uint64_t seed, uint32_t phase) {
-    const int p = blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.x;   // particles on x, the grid dimension with the largest limit
+    const int j = blockIdx.y * blockDim.x + threadIdx.x;
     if (p >= P || j >= cnt[p]) return;
     const size_t i = (size_t)off[p] + j;
     double g[4];
@@ -92,8 +92,8 @@ __global__ void k_sample_mixture(const double* __restrict__ mix, int C, int P, i
                                  double* __restrict__ quat, double* __restrict__ qdens,
                                  double* __restrict__ trans) {
     // mix per particle: C x (m0 m1 m2 m3 k pi) then (mu_x mu_y sigma)
-    const int p = blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.x;   // particles on x, the grid dimension with the largest limit
+    const int j = blockIdx.y * blockDim.x + threadIdx.x;
     if (p >= P) return;
     const double* mp = mix + (size_t)p * (6 * C + 3);
     if (j < nRot) {
@@ -487,7 +487,7 @@ int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double
 int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
                    const double* mean, const double* kk, double kmax, double pf, int P,
                    uint64_t seed, uint32_t phase, const double* dGiven) {
-    dim3 grid((maxCnt + 255) / 256, P);
+    dim3 grid(P, (maxCnt + 255) / 256);
     hipLaunchKernelGGL(k_perturb_rot, grid, dim3(256), 0, st, quat, off, cnt, mean, kk, kmax, pf,
                        P, seed, phase, dGiven);
     THX_HIP(hipGetLastError());
@@ -496,7 +496,7 @@ int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt,
 
 int pf_perturb_trans(hipStream_t st, double* trans, const int* off, const int* cnt, int maxCnt,
                      const double* s01rho, double pf, int P, uint64_t seed, uint32_t phase) {
-    dim3 grid((maxCnt + 255) / 256, P);
+    dim3 grid(P, (maxCnt + 255) / 256);
     hipLaunchKernelGGL(k_perturb_trans, grid, dim3(256), 0, st, trans, off, cnt, s01rho, pf, P,
                        seed, phase);
     THX_HIP(hipGetLastError());
@@ -507,7 +507,7 @@ int pf_sample_mixture(hipStream_t st, const double* mix, int C, int P, int nRot,
                       uint64_t seed, uint32_t phase, double* quat, double* qdens,
                       double* trans) {
     const int n = nRot > nTr ? nRot : nTr;
-    dim3 grid((n + 255) / 256, P);
+    dim3 grid(P, (n + 255) / 256);
     hipLaunchKernelGGL(k_sample_mixture, grid, dim3(256), 0, st, mix, C, P, nRot, nTr, seed, phase,
                        quat, qdens, trans);
     THX_HIP(hipGetLastError());

@@ -123,6 +123,22 @@ def c32(a, dtype=np.float32):
     return np.ascontiguousarray(a, dtype=dtype)
 
 
+def particle_arrays(P, K, ctf, sig, dat=None):
+    """The per-particle arrays of set_particles / set_particle_images as
+    contiguous float32, checked against (P, K) (dat: (P, K) complex as
+    (P, K, 2) floats).  The library copies P * K values from each host
+    pointer, so a shorter array would be read past its end: raise instead."""
+    ctf, sig = c32(ctf), c32(sig)
+    for name, a in (("ctf", ctf), ("sig", sig)):
+        if a.shape != (P, K):
+            raise ValueError(f"{name} has shape {a.shape}, expected {(P, K)}")
+    if dat is not None:
+        dat = c32(dat)
+        if dat.size != 2 * P * K or dat.shape[:2] != (P, K):
+            raise ValueError(f"dat has shape {dat.shape}, expected {(P, K, 2)}")
+    return ctf, sig, dat
+
+
 def device_count():
     n = C.c_int(0)
     st = load().thx_device_count(C.byref(n))
@@ -234,12 +250,14 @@ class Context:
         [P][K] as for set_particles.  Returns the normalisation, FFT and
         packing times (ms)."""
         img = c32(images)
+        if img.ndim != 3 or img.shape[1] != img.shape[2]:
+            raise ValueError(f"images have shape {img.shape}, expected (P, N, N)")
         P, N = img.shape[0], img.shape[1]
-        assert img.shape == (P, N, N), img.shape
+        ctf, sig, _ = particle_arrays(P, self.K, ctf, sig)
         ms = np.zeros(3, np.float32)
         check(load().thx_ctx_set_particle_images(self._h, ptr(img), P, N, float(bg_radius),
-                                                  int(bool(soft_mask)), ptr(c32(ctf)),
-                                                  ptr(c32(sig)), ptr(ms)),
+                                                  int(bool(soft_mask)), ptr(ctf),
+                                                  ptr(sig), ptr(ms)),
               "set_particle_images")
         self.P = P
         return ms
@@ -262,8 +280,8 @@ class Context:
         return out
 
     def set_particles(self, dat, ctf, sig):
-        dat, ctf, sig = c32(dat), c32(ctf), c32(sig)
-        P = ctf.shape[0]
+        P = np.shape(ctf)[0]
+        ctf, sig, dat = particle_arrays(P, self.K, ctf, sig, dat)
         check(load().thx_ctx_set_particles(self._h, ptr(dat), ptr(ctf), ptr(sig), P),
               "set_particles")
         self.P = P
