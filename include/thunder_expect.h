@@ -274,6 +274,23 @@ int thx_pf_resample(thx_ctx* ctx, const double* w, const double* u,
                     const int* nIn, const int* nOut, int P, uint64_t seed,
                     const double* u0, int* idx, double* wNew, double* ess);
 
+/* No counterpart in the reference: thx_pf_resample for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * Every array is a device pointer on the context's device: w, u [sum nIn],
+ * nIn, nOut [P], u0 [P] or NULL, idx, wNew [sum nOut], ess [P] or NULL, with
+ * the same meaning and results as thx_pf_resample.  Calls no allocation
+ * routine: the per-particle offsets go into a static device array of
+ * THX_NOALLOC_MAX_P particles (2 MiB, held by each GPU it runs on until the
+ * process ends); larger P is THX_ERR_RANGE.  Counts are read on the device:
+ * a particle whose nIn or nOut lies outside [1, 8192] gets ess = -1 and no
+ * outputs (its slots are counted as min(count, 8192)).  Enqueued on the
+ * context stream; returns once queued (thx_ctx_sync waits).  Calls that
+ * overlap on one GPU share that array and race. */
+#define THX_NOALLOC_MAX_P 131072
+int thx_pf_resample_noalloc(thx_ctx* ctx, const double* w, const double* u,
+                            const int* nIn, const int* nOut, int P, uint64_t seed,
+                            const double* u0, int* idx, double* wNew, double* ess);
+
 /* Weighted mean rotation per particle: principal eigenvector of the scatter
  * sum w q q^T (the ACG mean inferACG estimates, src/Geometry/
  * DirectionalStat.cpp:87-244; sign chosen with w >= 0) and the eigenvalue

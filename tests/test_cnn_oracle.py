@@ -7,10 +7,15 @@ import pytest
 
 import cnn_oracle as co
 
-torch = pytest.importorskip("torch")
-F = torch.nn.functional
-
 C = 3
+
+
+def _torch():
+    """torch is imported inside the tests that use it, never at collection:
+    its wheel bundles its own HIP runtime under the same soname as the ROCm
+    install's, and a GPU test session that collected this module would
+    otherwise bind the expectation library to that runtime."""
+    return pytest.importorskip("torch")
 
 
 def _rand_problem(B=3, seed=0):
@@ -29,6 +34,8 @@ def _rand_problem(B=3, seed=0):
 
 
 def _torch_loss(p, x, q, t):
+    torch = _torch()
+    F = torch.nn.functional
     W = [torch.tensor(a, requires_grad=True) for a in co.unpack(p, C)]
     h = torch.tensor(x)
     for l, (ci, hi, c_o, ho, ks, st, pd) in enumerate(co.CONV):

@@ -15,7 +15,7 @@ LIB = os.path.join(PKG, "libthunder_expect.so")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_thx.so")
 
 HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_filter.hip",
-               "thx_projectee.hip", "thx_images.hip", "thx_api.hip"]
+               "thx_projectee.hip", "thx_images.hip", "thx_noalloc.hip", "thx_api.hip"]
 # hipFFT: the two plain 3-D transforms of the projectee preparation and the
 # batched 2-D transform of the particle images
 HIP_LIBS = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]

@@ -144,7 +144,10 @@ __global__ __launch_bounds__(256) void k_resample(
     const int p = blockIdx.x;
     if (p >= P) return;
     const int n = nIn[p], m = nOut[p];
-    if (n <= 0 || m <= 0) return;
+    if (n <= 0 || m <= 0 || n > kResampleMax) {   // no samples, or more than the LDS holds
+        if (threadIdx.x == 0 && essOut) essOut[p] = -1.0;
+        return;
+    }
     const double* wp = wShared ? w : w + offIn[p];   // wShared: one prior for every particle
     const double* up = u + offIn[p];
     __shared__ double cdf[kResampleMax];

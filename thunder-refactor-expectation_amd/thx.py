@@ -52,6 +52,7 @@ _SIGS = {
     "thx_ctx_get_local": [_p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
     "thx_pf_resample": [_p, _p, _p, _p, _p, _i, _u64, _p, _p, _p, _p],
+    "thx_pf_resample_noalloc": [_p, _p, _p, _p, _p, _i, _u64, _p, _p, _p, _p],
     "thx_pf_rot_mean": [_p, _p, _p, _p, _i, _p, _p],
     "thx_pf_perturb": [_p, _p, _p, _p, _p, _i, _p, _p, _p, _d, _u64, _i, _p],
     "thx_proposal_sample": [_p, _p, _i, _i, _i, _i, _u64, _p, _p, _p],
