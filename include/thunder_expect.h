@@ -351,6 +351,20 @@ int thx_pf_from_global_classes(thx_ctx* ctx, int nCls, int mLR, int mLT, double 
                                double minStdT, double pfS, const double* transS,
                                uint64_t seed, int* clsOut);
 
+/* As thx_pf_from_global / thx_pf_from_global_classes with the random draws
+ * given instead of taken from a seed (checking, and callers that keep their
+ * own random streams): u0R[P] in [0, 1/mLR) and u0T[P] in [0, 1/mLT) are the
+ * systematic-resampling offsets of each particle's rotations and shifts
+ * (gsl_ran_flat in Particle::resample, src/Particle.cpp:1324-1366); with
+ * nCls > 1, u0C[P] in [0, 1/nCls) is the class resampling offset and
+ * pickC[P] in [0, 1) picks the draw floor(pickC nCls) (rand(cls),
+ * src/Particle.cpp:2080-2089).  nCls == 1 uses class 0 (u0C, pickC may be
+ * NULL).  clsOut[P] (optional with nCls == 1): the classes. */
+int thx_pf_from_global_given(thx_ctx* ctx, int nCls, int mLR, int mLT, double minStdR,
+                             double minStdT, double pfS, const double* transS,
+                             const double* u0C, const double* pickC, const double* u0R,
+                             const double* u0T, int* clsOut);
+
 /* Local search phases until every particle converges or maxPhase (the
  * expectation loop, src/Optimiser.cpp:1475-1660, with updateSP :1139-1437):
  * per phase, perturbation (pfL at phase 0, else pfS) with balanced weights,
@@ -381,6 +395,19 @@ int thx_pf_rot_stats(thx_ctx* ctx, const double* quat, int n, int P, double* A,
  * (src/Particle.cpp:1083-1097, 2313-2344). */
 int thx_pf_trans_stats(thx_ctx* ctx, const double* trans, int n, int P,
                        double* m, double* s01, double* wBal);
+/* The convergence test of the local search (src/Optimiser.cpp:1480-1612,
+ * OPTIMISER_COMPRESS_CRITERIA) on scripted statistics: after phase
+ * phase0 + i (phase0 = 1 with afterGlobal, else 0) particle p has
+ * concentrations k[(i P + p) * 3 + a] and shift deviations
+ * s01[(i P + p) * 2 + a]; nPhase[p] receives the phase that ended its search
+ * (-1: none within nPh phases). */
+int thx_pf_converge(thx_ctx* ctx, const double* k, const double* s01, int nPh, int P,
+                    int afterGlobal, int* nPhase);
+/* The 4x4 helpers of the inferACG stages on n matrices A[n*16] (row-major):
+ * inv[n*16] and det[n] (inverse zero for a singular matrix), top[n*4] the
+ * unit eigenvector of the largest eigenvalue of a symmetric A, its
+ * largest-magnitude entry positive (src/Geometry/DirectionalStat.cpp:87-244). */
+int thx_pf_mat4(thx_ctx* ctx, const double* A, int n, double* inv, double* det, double* top);
 /* setPeakFactor (setFactor != 0) + keepHalfHeightPeak of P sets of n values
  * (in place), factor[P] in/out, top[P] = first maximum (src/Particle.cpp:
  * 988-1000, 1864-1954). */

@@ -46,6 +46,9 @@ def load():
         "thxo_balance_rot": ([_p, _i, _d, _i, _p], _i),
         "thxo_trans_stats": ([_p, _i, _p, _p, _p], None),
         "thxo_keep_peak": ([_p, _i, _d, _i, _p], _d),
+        "thxo_post_scan": ([_i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _d, _d, _d, _d,
+                            _d, _d, _d, _d, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p], _i),
+        "thxo_converge": ([_p, _p, _p, _i, _i, _i], _i),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -265,6 +268,46 @@ def keep_peak(u, factor=1e-3, set_factor=False):
     f = load().thxo_keep_peak(_ptr(u), u.size, C.c_double(factor), int(set_factor),
                               C.byref(top))
     return u, f, top.value
+
+
+def post_scan(wC, base, wR, wT, pR, pT, grid_q, grid_t, mLR, mLT, u0C, pickC, u0R, u0T,
+              min_std_r, min_std_t, pfS, tol=1e-3, max_iter=1000):
+    """Post-scan update of one particle, src/Optimiser.cpp:926-1084 (random draws
+    given).  wC, base [nCls]; wR [nCls][R]; wT [nCls][T].  Returns a dict with
+    cls, idxR, q, wR, idxT, t, wT, k, s, peakR."""
+    wC, base = _c(np.atleast_1d(wC), np.float64), _c(np.atleast_1d(base), np.float64)
+    nCls = wC.size
+    wR, wT = _c(wR, np.float64).reshape(nCls, -1), _c(wT, np.float64).reshape(nCls, -1)
+    R, T = wR.shape[1], wT.shape[1]
+    pR, pT = _c(pR, np.float64), _c(pT, np.float64)
+    gq, gt = _c(grid_q, np.float64).reshape(R, 4), _c(grid_t, np.float64).reshape(T, 2)
+    cls = C.c_int(0)
+    idxR, idxT = np.zeros(mLR, np.int32), np.zeros(mLT, np.int32)
+    q, wRn = np.zeros((mLR, 4)), np.zeros(mLR)
+    t, wTn = np.zeros((mLT, 2)), np.zeros(mLT)
+    k, sd, pk = np.zeros(3), np.zeros(2), C.c_double(0)
+    st = load().thxo_post_scan(nCls, _ptr(wC), _ptr(base), _ptr(wR), _ptr(wT), _ptr(pR),
+                               _ptr(pT), _ptr(gq), _ptr(gt), R, T, mLR, mLT, u0C, pickC, u0R,
+                               u0T, min_std_r, min_std_t, pfS, tol, max_iter, C.byref(cls),
+                               _ptr(idxR), _ptr(q), _ptr(wRn), _ptr(idxT), _ptr(t), _ptr(wTn),
+                               _ptr(k), _ptr(sd), C.byref(pk))
+    assert st == 0, st
+    return dict(cls=cls.value, idxR=idxR, q=q, wR=wRn, idxT=idxT, t=t, wT=wTn, k=k, s=sd,
+                peakR=pk.value)
+
+
+def converge(k, s, sD=None, phase0=1, min_phase=10):
+    """Phase at which the local search of one particle stops (-1: not within
+    the sequence), src/Optimiser.cpp:1480-1612.  k [n][3]; s [n][2] (s0, s1)
+    or [n][3] (s0, s1, rho); sD [n] (defocus deviation; constant by default)."""
+    k = _c(k, np.float64).reshape(-1, 3)
+    n = k.shape[0]
+    s = np.asarray(s, np.float64).reshape(n, -1)
+    if s.shape[1] == 2:
+        s = np.concatenate([s, np.zeros((n, 1))], axis=1)
+    s = _c(s, np.float64)
+    sD = _c(np.ones(n) if sD is None else sD, np.float64)
+    return load().thxo_converge(_ptr(k), _ptr(s), _ptr(sD), n, phase0, min_phase)
 
 
 def projectee(vol_ft, N, pf):

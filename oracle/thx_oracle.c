@@ -14,6 +14,7 @@
  */
 #include "thx_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -802,4 +803,140 @@ double thxo_keep_peak(double* u, int n, double factor, int setFactor, int* top)
     for (int i = 0; i < n; i++) u[i] = u[i] < hh ? 0.0 : u[i] - hh;
     if (top) *top = best;
     return factor;
+}
+
+/*
+ * a9: the post-scan update of one particle after the global scan of nCls
+ * classes (src/Optimiser.cpp:926-1084; 3-D, OPTIMISER_PEAK_FACTOR_C and _R on,
+ * _T off, PARTICLE_PEAK_FACTOR_C on, OPTIMISER_SCAN_SET_MIN_STD_WITH_PERTURB
+ * on, PARTICLE_RHO off, include/Config.h:81-89, 214-224).  The random draws of
+ * the reference (gsl_ran_flat for each systematic offset, gsl_rng_uniform_int
+ * for rand(cls)) are given: u0C in [0, 1/nCls), pickC in [0, 1), u0R in
+ * [0, 1/mLR), u0T in [0, 1/mLT); the shuffle before each resample is not
+ * applied.
+ *
+ *   class: u_k = wC_k exp(base_k - max base) (setUC of the marginals on one
+ *     baseline, :928-929); setPeakFactor(PAR_C) = PEAK_FACTOR_C = 1 - 1e-2
+ *     (src/Particle.cpp:1879-1886, include/Particle.h:47) and
+ *     keepHalfHeightPeak(PAR_C) (:1935-1944); resample(nCls, PAR_C) from the
+ *     uniform class prior (:1280-1322: cdf of w u normalised by its last entry,
+ *     c_j = first class with cdf >= u0C + j / nCls); rand(cls) = the draw
+ *     floor(pickC nCls) (:2080-2089);
+ *   rotations: setUR = wR of that class (:959-960), setPeakFactor(PAR_R)
+ *     (order(nR / 8), clamped to [1e-3, 0.5]) + keepHalfHeightPeak(PAR_R)
+ *     (:967-970), resample(mLR, PAR_R) with the grid prior pR (:1001;
+ *     src/Particle.cpp:1324-1366), calVari(PAR_R) (:1004);
+ *   translations: setUT = wT of that class (:961-962), no peak keeping,
+ *     resample(mLT, PAR_T) with the grid prior pT (:1002), calVari(PAR_T)
+ *     (:1005, n - 1 deviations, rho 0);
+ *   clamps (:1031-1084): k_a = max(k_a, (minStdR / pfS)^2),
+ *     s_a = max(s_a, minStdT / pfS), pfS = perturbFactorSGlobal.
+ *
+ * Inputs per class k: wC[k], base[k], wR[k R + r], wT[k T + t]; grid pR[R],
+ * pT[T], gridQ[R][4], gridT[T][2].  Outputs: *cls, q[mLR][4] and wRn[mLR]
+ * (resampled rotations and their 1/u weights, normalised), t[mLT][2], wTn[mLT],
+ * k[3], s[2], *peakR (the rotation peak factor), idxR[mLR] / idxT[mLT] (grid
+ * indices).  Returns 0, or the negative status of calVari.
+ */
+int thxo_post_scan(int nCls, const double* wC, const double* base, const double* wR,
+                   const double* wT, const double* pR, const double* pT, const double* gridQ,
+                   const double* gridT, int R, int T, int mLR, int mLT, double u0C,
+                   double pickC, double u0R, double u0T, double minStdR, double minStdT,
+                   double pfS, double tol, int maxIter, int* cls, int* idxR, double* q,
+                   double* wRn, int* idxT, double* t, double* wTn, double* k, double* s,
+                   double* peakR)
+{
+    /* class */
+    int c = 0;
+    if (nCls > 1) {
+        double bmax = base[0], umax = 0, sum = 0;
+        for (int i = 1; i < nCls; i++) if (base[i] > bmax) bmax = base[i];
+        double* u = (double*)malloc(sizeof(double) * nCls);
+        double* cdf = (double*)malloc(sizeof(double) * nCls);
+        for (int i = 0; i < nCls; i++) {
+            u[i] = wC[i] * exp(base[i] - bmax);
+            if (u[i] > umax) umax = u[i];
+        }
+        const double hh = umax * (1 - 1e-2);
+        for (int i = 0; i < nCls; i++) {
+            u[i] = u[i] < hh ? 0.0 : u[i] - hh;
+            sum += u[i];
+        }
+        /* w = uniform prior (.) u, normalised (all-zero u, which the peak
+         * keeping cannot leave unless every wC is 0: the uniform prior) */
+        double acc = 0;
+        for (int i = 0; i < nCls; i++) {
+            acc += sum > 0 ? u[i] / sum : 1.0 / nCls;
+            cdf[i] = acc;
+        }
+        for (int i = 0; i < nCls; i++) cdf[i] /= cdf[nCls - 1];
+        int j = (int)floor(pickC * nCls);
+        if (j > nCls - 1) j = nCls - 1;
+        const double uj = u0C + j * 1.0 / nCls;
+        int i = 0;
+        while (uj > cdf[i] && i < nCls - 1) i++;
+        c = i;
+        free(u);
+        free(cdf);
+    }
+    *cls = c;
+    /* rotations */
+    double* u = (double*)malloc(sizeof(double) * (R > T ? R : T));
+    memcpy(u, wR + (size_t)c * R, sizeof(double) * R);
+    *peakR = thxo_keep_peak(u, R, 0.0, 1, NULL);
+    thxo_resample(pR, u, R, mLR, u0R, idxR, wRn);
+    for (int j = 0; j < mLR; j++) memcpy(q + 4 * j, gridQ + 4 * idxR[j], sizeof(double) * 4);
+    /* translations */
+    memcpy(u, wT + (size_t)c * T, sizeof(double) * T);
+    thxo_resample(pT, u, T, mLT, u0T, idxT, wTn);
+    for (int j = 0; j < mLT; j++) memcpy(t + 2 * j, gridT + 2 * idxT[j], sizeof(double) * 2);
+    free(u);
+    /* calVari and the scanning minima */
+    double mean[4], m[2];
+    int st = thxo_cal_vari_rot(q, mLR, tol, maxIter, mean, k);
+    if (st < 0) return st;
+    thxo_trans_stats(t, mLT, m, s, NULL);
+    const double kMin = (minStdR / pfS) * (minStdR / pfS), sMin = minStdT / pfS;
+    for (int a = 0; a < 3; a++) k[a] = k[a] > kMin ? k[a] : kMin;
+    for (int a = 0; a < 2; a++) s[a] = s[a] > sMin ? s[a] : sMin;
+    return 0;
+}
+
+/*
+ * a13: the convergence test of one particle's local search
+ * (src/Optimiser.cpp:1480-1612 with OPTIMISER_COMPRESS_CRITERIA,
+ * include/Config.h:222; N_PHASE_WITH_NO_VARI_DECREASE = 1,
+ * PARTICLE_FILTER_DECREASE_FACTOR = 0.95, include/Optimiser.h:56-63).
+ * Phase i of the sequence is phase phase0 + i (phase0 = 1 after a global scan,
+ * else 0); from phase minPhase on (10 after a global scan, else 3) the
+ * particle's variances after that phase's update, variR = (k1 k2 k3)^(1/6)
+ * (Particle::variR, src/Particle.cpp:609-625), variT = sqrt of the product of
+ * the eigenvalues of [[s0^2, rho], [rho, s1^2]] (variT, :627-638) and
+ * variD = sD (variD, :640-643), are compared with the smallest seen so far
+ * (all DBL_MAX at first): a phase with none below 0.95 times its best ends the
+ * search; the bests are then lowered.  k[3 i + a], s[3 i + a] (s0, s1, rho),
+ * sD[i] for i < n.  Returns the phase that ended the search, or -1 when it
+ * runs past the sequence.
+ */
+int thxo_converge(const double* k, const double* s, const double* sD, int n, int phase0,
+                  int minPhase)
+{
+    double bR = DBL_MAX, bT = DBL_MAX, bD = DBL_MAX;
+    int nNo = 0;
+    for (int i = 0; i < n; i++) {
+        const int phase = phase0 + i;
+        if (phase < minPhase) continue;
+        const double vR = pow(k[3 * i] * k[3 * i + 1] * k[3 * i + 2], 1.0 / 6);
+        const double s0 = s[3 * i], s1 = s[3 * i + 1], rho = s[3 * i + 2];
+        /* eigenvalues of the symmetric 2x2: their product is the determinant */
+        const double vT = sqrt(s0 * s0 * s1 * s1 - rho * rho);
+        const double vD = sD[i];
+        if (vR < bR * 0.95 || vT < bT * 0.95 || vD < bD * 0.95) nNo = 0;
+        else nNo += 1;
+        if (vR < bR) bR = vR;
+        if (vT < bT) bT = vT;
+        if (vD < bD) bD = vD;
+        if (nNo == 1) return phase;
+    }
+    return -1;
 }

@@ -103,6 +103,20 @@ int thxo_balance_rot(const double* q, int n, double tol, int maxIter, double* w)
 void thxo_trans_stats(const double* t, int n, double* m, double* s, double* w);
 double thxo_keep_peak(double* u, int n, double factor, int setFactor, int* top);
 
+/* a9: post-scan update of one particle (class, rotations, shifts, calVari,
+ * scanning minima) with the random draws given. */
+int thxo_post_scan(int nCls, const double* wC, const double* base, const double* wR,
+                   const double* wT, const double* pR, const double* pT, const double* gridQ,
+                   const double* gridT, int R, int T, int mLR, int mLT, double u0C,
+                   double pickC, double u0R, double u0T, double minStdR, double minStdT,
+                   double pfS, double tol, int maxIter, int* cls, int* idxR, double* q,
+                   double* wRn, int* idxT, double* t, double* wTn, double* k, double* s,
+                   double* peakR);
+
+/* a13: phase at which a particle's local search stops (-1: not within n). */
+int thxo_converge(const double* k, const double* s, const double* sD, int n, int phase0,
+                  int minPhase);
+
 #ifdef __cplusplus
 }
 #endif

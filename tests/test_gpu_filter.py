@@ -154,3 +154,137 @@ def test_class_draw_and_per_class_states(gpu_ctx):
     gpu_ctx.pf_from_global_classes(2, *args, seed=3)
     ran = gpu_ctx.pf_refine(cls, thx.BF16, after_global=True, max_phase=12, seed=4)
     assert ran >= 1
+
+
+def _two_class_context(ctx, per=24, snr=None):
+    """Two independently seeded box-64 maps, `per` particles made from each
+    (at the workload's SNR unless given), the 1,000-rotation grid, and an FP32
+    scan of both classes."""
+    w = wl.WORKLOADS["box64"]
+    sd = wl.seeds("box64")
+    iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
+    ctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    parts = []
+    for c in range(2):
+        ctx.synth_volume(c, w["box"], w["pf"], w["n_blob"], w["blob_sigma"], w["blob_radius"],
+                         sd["volume"] + 17 * c)
+        ctx.synth_particles(c, per, w["max_shift"], w["df_min"], w["df_max"],
+                            w["snr"] if snr is None else snr, sd["particles"] + c,
+                            **wl.CTF_DEFAULTS)
+        parts.append(ctx.get_particles())
+    ctx.set_particles(*[np.concatenate([x[i] for x in parts]) for i in range(3)])
+    quat, trans = wl.grid("box64", 1000)
+    ctx.set_grid(quat, trans)
+    for c in range(2):
+        ctx.score_global(c, thx.FP32)
+    return quat, trans
+
+
+@pytest.mark.parametrize("n_cls", [1, 2])
+def test_post_scan_matches_oracle(gpu_ctx, n_cls):
+    """The whole post-scan update (src/Optimiser.cpp:926-1084: class draw,
+    rotation peak factor and half-height peak, resampling of 300 rotations and
+    25 shifts from the grid, calVari, scanning minima) against the oracle's
+    restatement with the same random draws: drawn classes and resampled grid
+    samples exactly, weights to 1e-12, peak factors to 1e-14, shift deviations
+    to 1e-12 and concentrations to 1e-8 (FP64 inferACG, summation order
+    differs) wherever the drawn rotations make calVari well posed."""
+    print(f"\n[post-scan] box64 two classes, {n_cls} used", flush=True)
+    # at SNR 0.003 the marginals are broad enough that most particles draw at
+    # least four independent rotations
+    quat, trans = _two_class_context(gpu_ctx, snr=0.003)
+    P, R, T = gpu_ctx.P, len(quat), len(trans)
+    # one class: its own marginals exactly; two: both on one baseline (the
+    # class draw's u_k; each class's rows then carry a common factor, which the
+    # peak keeping and resampling do not see beyond rounding)
+    wC, wR, wT, base = gpu_ctx.get_global(n_cls)
+    rng = np.random.default_rng(40 + n_cls)
+    mLR, mLT = 300, 25
+    u0C, pickC = rng.uniform(0, 1, P) / n_cls, rng.uniform(0, 1, P)
+    u0R, u0T = rng.uniform(0, 1, P) / mLR, rng.uniform(0, 1, P) / mLT
+    min_r, min_t, pfS = 0.1, 0.3, 0.5
+    cls = gpu_ctx.pf_from_global_given(n_cls, mLR, mLT, min_r, min_t, pfS, 1.0, u0R, u0T,
+                                       u0C if n_cls > 1 else None,
+                                       pickC if n_cls > 1 else None)
+    st = gpu_ctx.pf_get_state()
+    pR, pT = np.full(R, 1.0 / R), np.full(T, 1.0 / T)
+    kmin, compared = (min_r / pfS) ** 2, 0
+    for p in range(P):
+        ref = oracle.post_scan(wC[p, :n_cls], np.zeros(n_cls), wR[p, :n_cls], wT[p, :n_cls],
+                               pR, pT, quat, trans, mLR, mLT, u0C[p], pickC[p], u0R[p], u0T[p],
+                               min_r, min_t, pfS)
+        assert cls[p] == ref["cls"], p
+        np.testing.assert_array_equal(st["quat"][p], ref["q"])
+        np.testing.assert_allclose(st["wR"][p], ref["wR"], rtol=1e-12)
+        np.testing.assert_array_equal(st["trans"][p], ref["t"])
+        np.testing.assert_allclose(st["wT"][p], ref["wT"], rtol=1e-12)
+        assert st["peakR"][p] == pytest.approx(ref["peakR"], rel=1e-14)
+        np.testing.assert_allclose(st["s01"][p], ref["s"], rtol=1e-12)
+        # calVari's inferACG is defined only when the drawn rotations span the
+        # four dimensions of their scatter matrix; with fewer than four
+        # independent samples (a sharp peak leaves 1-3 grid rotations) its
+        # fixed point inverts a singular matrix on both sides and only the
+        # scanning minimum is certain
+        sv = np.linalg.svd(ref["q"], compute_uv=False)
+        if sv[-1] > 1e-6 * sv[0]:
+            np.testing.assert_allclose(st["k"][p], ref["k"], rtol=1e-8)
+            compared += 1
+        else:
+            assert np.all(st["k"][p] >= kmin) and np.all(ref["k"] >= kmin)
+    print(f"concentrations compared for {compared} of {P} particles (the others drew fewer "
+          f"than four independent rotations)")
+    assert compared >= P // 4
+    if n_cls > 1:
+        print(f"classes drawn: {np.bincount(cls, minlength=2)}")
+        assert len(set(cls.tolist())) == 2
+
+
+def test_converge_matches_oracle(gpu_ctx):
+    """k_converge over scripted per-phase concentrations and deviations against
+    the oracle's convergence rule: the same stopping phase for every particle,
+    after a global scan (checks from phase 10) and in a local search (from 3)."""
+    rng = np.random.default_rng(77)
+    P, nPh = 256, 40
+    # variances that shrink at a random rate, then level off at a random phase,
+    # with noise: every particle stops somewhere, many at distinct phases
+    rate = rng.uniform(0.85, 1.0, size=(P, 1))
+    stop = rng.integers(5, 35, size=(P, 1))
+    ph = np.arange(1, nPh + 1)[None, :]
+    lev = rate ** np.minimum(ph, stop) * rng.uniform(0.97, 1.03, size=(P, nPh))
+    k = np.stack([lev ** 2 * rng.uniform(0.5, 2, size=(P, 1)) for _ in range(3)], axis=2)
+    s01 = np.stack([np.sqrt(lev) * rng.uniform(0.5, 2, size=(P, 1)) for _ in range(2)], axis=2)
+    k, s01 = k.transpose(1, 0, 2), s01.transpose(1, 0, 2)      # [nPh][P][.]
+    for after_global, phase0, min_phase in ((True, 1, 10), (False, 0, 3)):
+        got = gpu_ctx.pf_converge(k, s01, after_global)
+        ref = np.array([oracle.converge(k[:, p], s01[:, p], phase0=phase0, min_phase=min_phase)
+                        for p in range(P)])
+        print(f"after_global={after_global}: stopping phases {np.bincount(ref + 1)[:15]}...")
+        np.testing.assert_array_equal(got, ref)
+        assert len(set(ref.tolist())) >= 10
+
+
+def test_mat4_helpers_against_lapack(gpu_ctx):
+    """The 4x4 inverse / determinant and the top eigenvector used by the
+    inferACG stages, against numpy's LAPACK routines (independent of the
+    oracle's own 4x4 code): general and ill-conditioned symmetric positive
+    definite matrices, and a singular one."""
+    rng = np.random.default_rng(8)
+    n = 200
+    A = rng.normal(size=(n, 4, 4))
+    spd = []
+    for i in range(n):
+        Q, _ = np.linalg.qr(rng.normal(size=(4, 4)))
+        ev = np.sort(10.0 ** rng.uniform(-4, 0, 4))[::-1]
+        ev[0] = 1.0 + rng.uniform(0.05, 1.0)          # a separated top eigenvalue
+        spd.append(Q @ np.diag(ev) @ Q.T)
+    M = np.concatenate([A, np.array(spd), np.ones((1, 4, 4))])
+    inv, det, top = gpu_ctx.pf_mat4(M)
+    for i in range(len(M) - 1):
+        ref = np.linalg.inv(M[i])
+        np.testing.assert_allclose(inv[i], ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+        assert det[i] == pytest.approx(np.linalg.det(M[i]), rel=1e-9)
+    assert det[-1] == 0.0 and np.all(inv[-1] == 0.0)
+    for i in range(n, 2 * n):
+        w, V = np.linalg.eigh(M[i])
+        v = V[:, -1] * np.sign(V[np.argmax(np.abs(V[:, -1])), -1])
+        np.testing.assert_allclose(top[i], v, atol=1e-10)

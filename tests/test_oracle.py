@@ -766,3 +766,111 @@ def test_masked_images_explicit_loops():
                         for y in range(N) for x in range(N)) for k in range(len(iCol))])
     out = oracle.particle_images(img[None], r, iCol, iRow, masked=True)[0]
     np.testing.assert_allclose(out, ref, atol=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# a9: the post-scan composite, hand-worked
+# ---------------------------------------------------------------------------
+
+def _post_scan_case():
+    rng = np.random.default_rng(12)
+    gq = rng.normal(size=(8, 4))
+    gq /= np.linalg.norm(gq, axis=1, keepdims=True)
+    gt = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 2.0], [-1.0, -1.0], [3.0, 1.0]])
+    # class 1 carries the marginals worked below; classes 0 and 2 other rows
+    wR = np.array([[1, 1, 1, 1, 1, 1, 9, 1], [6, 3, 10, 2, 9, 7, 0, 8], [9, 0, 0, 0, 0, 0, 0, 1.0]])
+    wT = np.array([[0, 0, 0, 5, 0], [1, 4, 2, 0, 3], [0, 5, 0, 0, 0.0]])
+    wC = np.array([2.7, 1.0, 0.5])
+    base = np.array([-10.0, -9.0, -12.0])
+    return gq, gt, wR, wT, wC, base
+
+
+def test_post_scan_hand_worked():
+    """Classes: u = wC exp(base - max base) = (2.7/e, 1, 0.5/e^3); half-height
+    peak at 0.99 leaves (0.00327, 0.01, 0), cdf (0.2465, 1, 1); the draws
+    u0C + j/3 = 0.05, 0.383, 0.717 give classes (0, 1, 1), so pick 0.1 -> class 0
+    and pick 0.5 -> class 1.  Rotations of class 1: (6 3 10 2 9 7 0 8), peak
+    factor clamp(9/10) = 0.5, kept (1 0 5 0 4 2 0 3), cdf / 15 = (1 1 6 6 10 12
+    12 15) / 15; u0R = 0.06 over 6 draws -> indices (0 2 2 4 5 7), weights
+    1/u normalised = (60 12 12 15 30 20) / 149.  Shifts of class 1 (1 4 2 0 3),
+    no peak keeping, cdf (.1 .5 .7 .7 1); u0T = 0.2/3 over 3 draws -> (0 1 4),
+    weights (12 3 4) / 19; samples (0,0), (1,0), (3,1): deviations sqrt(7/3) and
+    sqrt(1/3), the second raised to minStdT / pfS = 0.6."""
+    gq, gt, wR, wT, wC, base = _post_scan_case()
+    pR, pT = np.full(8, 1 / 8), np.full(5, 1 / 5)
+    kw = dict(min_std_r=0.0, min_std_t=0.9, pfS=1.5)
+    a = oracle.post_scan(wC, base, wR, wT, pR, pT, gq, gt, 6, 3, 0.05, 0.1, 0.06, 0.2 / 3, **kw)
+    assert a["cls"] == 0
+    b = oracle.post_scan(wC, base, wR, wT, pR, pT, gq, gt, 6, 3, 0.05, 0.5, 0.06, 0.2 / 3, **kw)
+    assert b["cls"] == 1
+    np.testing.assert_array_equal(b["idxR"], [0, 2, 2, 4, 5, 7])
+    np.testing.assert_allclose(b["wR"], np.array([60, 12, 12, 15, 30, 20]) / 149, rtol=1e-14)
+    np.testing.assert_array_equal(b["q"], gq[[0, 2, 2, 4, 5, 7]])
+    assert b["peakR"] == 0.5
+    np.testing.assert_array_equal(b["idxT"], [0, 1, 4])
+    np.testing.assert_allclose(b["wT"], np.array([12, 3, 4]) / 19, rtol=1e-14)
+    np.testing.assert_array_equal(b["t"], gt[[0, 1, 4]])
+    np.testing.assert_allclose(b["s"], [np.sqrt(7 / 3), 0.6], rtol=1e-14)
+    # rotation concentrations: calVari of the drawn samples (pinned on its
+    # own), each raised to (minStdR / pfS)^2; a bound between the smallest and
+    # largest k clamps exactly the ones below it
+    _, k_raw = oracle.cal_vari_rot(gq[[0, 2, 2, 4, 5, 7]], max_iter=1000)
+    kmin = 0.5 * (k_raw.min() + k_raw.max())
+    c = oracle.post_scan(wC, base, wR, wT, pR, pT, gq, gt, 6, 3, 0.05, 0.5, 0.06, 0.2 / 3,
+                         min_std_r=1.5 * np.sqrt(kmin), min_std_t=0.9, pfS=1.5)
+    np.testing.assert_allclose(c["k"], np.maximum(k_raw, kmin), rtol=1e-12)
+    assert np.any(np.isclose(c["k"], kmin, rtol=1e-12)) and np.any(c["k"] > kmin)
+
+
+def test_post_scan_single_class_ignores_class_draws():
+    gq, gt, wR, wT, wC, base = _post_scan_case()
+    pR, pT = np.full(8, 1 / 8), np.full(5, 1 / 5)
+    one = oracle.post_scan(wC[1:2], base[1:2], wR[1:2], wT[1:2], pR, pT, gq, gt, 6, 3, 0.3, 0.9,
+                           0.06, 0.2 / 3, 0.0, 0.9, 1.5)
+    assert one["cls"] == 0
+    np.testing.assert_array_equal(one["idxR"], [0, 2, 2, 4, 5, 7])
+    np.testing.assert_array_equal(one["idxT"], [0, 1, 4])
+
+
+# ---------------------------------------------------------------------------
+# a13: the convergence rule, hand-worked
+# ---------------------------------------------------------------------------
+
+def _seq(vR, vT, n):
+    """k with (k1 k2 k3)^(1/6) = vR (all equal: k = vR^2), s0 = s1 = sqrt(vT)."""
+    vR = np.broadcast_to(np.asarray(vR, float), (n,))
+    vT = np.broadcast_to(np.asarray(vT, float), (n,))
+    return np.repeat((vR ** 2)[:, None], 3, 1), np.repeat(np.sqrt(vT)[:, None], 2, 1)
+
+
+def test_converge_hand_worked():
+    # phases 1..30 after a global scan; checks start at phase 10 (index 9)
+    n = 30
+    # constant: the first check passes (bests at DBL_MAX), the next fails -> 11
+    assert oracle.converge(*_seq(0.1, 0.5, n)) == 11
+    # vR falls by 10 % per phase up to phase 15, then stays: stops at 16
+    vR = 0.9 ** np.minimum(np.arange(1, n + 1), 15)
+    assert oracle.converge(*_seq(vR, 0.5, n)) == 16
+    # a 4 % fall per phase is not below 0.95 of the best: stops at 11
+    assert oracle.converge(*_seq(0.96 ** np.arange(1, n + 1), 0.5, n)) == 11
+    # vR constant, vT falling 10 % per phase up to phase 20: stops at 21
+    vT = 0.9 ** np.minimum(np.arange(1, n + 1), 20)
+    assert oracle.converge(*_seq(0.1, vT, n)) == 21
+    # local search: phases 0.., checks from phase 3: stops at 4
+    assert oracle.converge(*_seq(0.1, 0.5, n), phase0=0, min_phase=3) == 4
+    # comparison with the best so far, not the previous phase: phase 11 rises
+    # in vR but vT falls (continue), phase 12 has vR 0.97 of the best (1.0)
+    # though 0.65 of the previous: stops at 12
+    vR = np.r_[np.ones(9), 1.0, 1.5, 0.97, np.ones(n - 12)]
+    vT = np.r_[np.ones(10), 0.5, 0.5, np.full(n - 12, 0.5)]
+    assert oracle.converge(*_seq(vR, vT, n)) == 12
+    # rho enters variT: s0 = s1 = 1 with rho 0 then 0.6 (variT 1 -> 0.8)
+    k = np.full((n, 3), 0.01)
+    s = np.zeros((n, 3))
+    s[:, :2] = 1.0
+    s[10:, 2] = 0.6
+    assert oracle.converge(k, s) == 12
+    # the defocus deviation counts too
+    assert oracle.converge(*_seq(0.1, 0.5, n), sD=0.9 ** np.arange(n)) == -1
+    # never past the sequence: -1
+    assert oracle.converge(*_seq(0.9 ** np.arange(1, 13), 0.5, 12)) == -1

@@ -27,37 +27,40 @@
 
 namespace thx {
 
-// 4x4 inverse and determinant, Gauss-Jordan with partial pivoting (FP64).
+// 4x4 inverse and determinant by the adjugate, from the twelve 2x2 minors of
+// the upper and lower row pairs (Laplace expansion along those pairs; FP64).
+// Returns false when the determinant is zero.
 __device__ bool inv4(const double* A, double* inv, double* det) {
-    double m[4][8];
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 8; j++) m[i][j] = j < 4 ? A[4 * i + j] : (j - 4 == i ? 1.0 : 0.0);
-    double d = 1.0;
-    for (int c = 0; c < 4; c++) {
-        int piv = c;
-        for (int r = c + 1; r < 4; r++)
-            if (fabs(m[r][c]) > fabs(m[piv][c])) piv = r;
-        if (m[piv][c] == 0.0) return false;
-        if (piv != c) {
-            for (int j = 0; j < 8; j++) {
-                const double t = m[piv][j];
-                m[piv][j] = m[c][j];
-                m[c][j] = t;
-            }
-            d = -d;
-        }
-        d *= m[c][c];
-        const double pv = m[c][c];
-        for (int j = 0; j < 8; j++) m[c][j] /= pv;
-        for (int r = 0; r < 4; r++)
-            if (r != c) {
-                const double f = m[r][c];
-                for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
-            }
-    }
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) inv[4 * i + j] = m[i][j + 4];
+    const double a00 = A[0], a01 = A[1], a02 = A[2], a03 = A[3];
+    const double a10 = A[4], a11 = A[5], a12 = A[6], a13 = A[7];
+    const double a20 = A[8], a21 = A[9], a22 = A[10], a23 = A[11];
+    const double a30 = A[12], a31 = A[13], a32 = A[14], a33 = A[15];
+    const double s0 = a00 * a11 - a10 * a01, s1 = a00 * a12 - a10 * a02;
+    const double s2 = a00 * a13 - a10 * a03, s3 = a01 * a12 - a11 * a02;
+    const double s4 = a01 * a13 - a11 * a03, s5 = a02 * a13 - a12 * a03;
+    const double c5 = a22 * a33 - a32 * a23, c4 = a21 * a33 - a31 * a23;
+    const double c3 = a21 * a32 - a31 * a22, c2 = a20 * a33 - a30 * a23;
+    const double c1 = a20 * a32 - a30 * a22, c0 = a20 * a31 - a30 * a21;
+    const double d = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
     if (det) *det = d;
+    if (d == 0.0) return false;
+    const double r = 1.0 / d;
+    inv[0] = (a11 * c5 - a12 * c4 + a13 * c3) * r;
+    inv[1] = (-a01 * c5 + a02 * c4 - a03 * c3) * r;
+    inv[2] = (a31 * s5 - a32 * s4 + a33 * s3) * r;
+    inv[3] = (-a21 * s5 + a22 * s4 - a23 * s3) * r;
+    inv[4] = (-a10 * c5 + a12 * c2 - a13 * c1) * r;
+    inv[5] = (a00 * c5 - a02 * c2 + a03 * c1) * r;
+    inv[6] = (-a30 * s5 + a32 * s2 - a33 * s1) * r;
+    inv[7] = (a20 * s5 - a22 * s2 + a23 * s1) * r;
+    inv[8] = (a10 * c4 - a11 * c2 + a13 * c0) * r;
+    inv[9] = (-a00 * c4 + a01 * c2 - a03 * c0) * r;
+    inv[10] = (a30 * s4 - a31 * s2 + a33 * s0) * r;
+    inv[11] = (-a20 * s4 + a21 * s2 - a23 * s0) * r;
+    inv[12] = (-a10 * c3 + a11 * c1 - a12 * c0) * r;
+    inv[13] = (a00 * c3 - a01 * c1 + a02 * c0) * r;
+    inv[14] = (-a30 * s3 + a31 * s1 - a32 * s0) * r;
+    inv[15] = (a20 * s3 - a21 * s1 + a22 * s0) * r;
     return true;
 }
 
@@ -470,6 +473,24 @@ __global__ void k_min_std(double* __restrict__ kk, double* __restrict__ s01, int
     for (int a = 0; a < 2; a++) s01[3 * p + a] = fmax(sMin, s01[3 * p + a]);
 }
 
+// The 4x4 helpers of the inferACG stages on n given matrices (checking):
+// inverse and determinant (inv4), top eigenvector of the symmetric part
+// (sym4_top).  One thread per matrix.
+__global__ void k_mat4_check(const double* __restrict__ A, int n, double* __restrict__ inv,
+                             double* __restrict__ det, double* __restrict__ top) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a[16], b[16], d = 0.0;
+    for (int e = 0; e < 16; e++) a[e] = A[16 * (size_t)i + e];
+    if (!inv4(a, b, &d))
+        for (int e = 0; e < 16; e++) b[e] = 0.0;
+    for (int e = 0; e < 16; e++) inv[16 * (size_t)i + e] = b[e];
+    det[i] = d;
+    double v[4];
+    sym4_top(a, v);
+    for (int e = 0; e < 4; e++) top[4 * (size_t)i + e] = v[e];
+}
+
 constexpr double kAcgTol = 1e-3;     // inferACG's convergence bound
 constexpr int kAcgMaxIter = 1000;
 constexpr double kTransQ = 0.05;     // TRANS_Q, include/Optimiser.h:67
@@ -583,9 +604,12 @@ static int check_scan(thx_ctx* c, int k) {
 
 // Particle states from the rotation / translation marginals of class cls0
 // (pcls == null) or of each particle's own class pcls[p] (host array).
+// u0R / u0T (host, P each, or null: drawn from the seed) are the systematic
+// resampling offsets of the rotations and translations.
 static int from_global_impl(thx_ctx* c, int cls0, const int* pcls, int nCls, int mLR, int mLT,
                             double minStdR, double minStdT, double pfS, const double* transS,
-                            uint64_t seed) {
+                            uint64_t seed, const double* u0R = nullptr,
+                            const double* u0T = nullptr) {
     THX_CHECK(mLR >= 2 && mLR <= kResampleMaxN && mLT >= 2 && mLT <= 128 &&
                   c->R <= kResampleMaxN && c->T <= kResampleMaxN,
               THX_ERR_RANGE, "pf_from_global: sizes out of range");
@@ -598,7 +622,7 @@ static int from_global_impl(thx_ctx* c, int cls0, const int* pcls, int nCls, int
     THX_TRY(pf_alloc(c, P, mLR, mLT));
     THX_HIP(hipMemcpy(c->pfTransS.p, transS, sizeof(double) * P, hipMemcpyHostToDevice));
     // marginals of the grid: copies (the peak keeping works in place)
-    DevBuf uR, uT, offR, offT, nRg, nTg;
+    DevBuf uR, uT, offR, offT, nRg, nTg, dU0R, dU0T;
     std::vector<int> oR(P), oT(P), cR(P, R), cT(P, T);
     for (int p = 0; p < P; p++) {
         oR[p] = p * R;
@@ -606,12 +630,13 @@ static int from_global_impl(thx_ctx* c, int cls0, const int* pcls, int nCls, int
     }
     int s = THX_OK;
     auto cleanup = [&]() {
-        DevBuf* b[] = {&uR, &uT, &offR, &offT, &nRg, &nTg};
+        DevBuf* b[] = {&uR, &uT, &offR, &offT, &nRg, &nTg, &dU0R, &dU0T};
         for (DevBuf* x : b) x->release();
     };
     if ((s = uR.ensure(sizeof(double) * (size_t)P * R)) || (s = uT.ensure(sizeof(double) * (size_t)P * T)) ||
         (s = offR.ensure(sizeof(int) * P)) || (s = offT.ensure(sizeof(int) * P)) ||
-        (s = nRg.ensure(sizeof(int) * P)) || (s = nTg.ensure(sizeof(int) * P))) {
+        (s = nRg.ensure(sizeof(int) * P)) || (s = nTg.ensure(sizeof(int) * P)) ||
+        (u0R && (s = upload(dU0R, u0R, P, st))) || (u0T && (s = upload(dU0T, u0T, P, st)))) {
         cleanup();
         return s;
     }
@@ -664,16 +689,18 @@ static int from_global_impl(thx_ctx* c, int cls0, const int* pcls, int nCls, int
     THX_HIP(hipGetLastError());
     // resample mLR rotations / mLT shifts from the grid (prior = the grid's)
     s = pf_resample(st, c->pR.as<double>(), uR.as<double>(), offR.as<int>(), nRg.as<int>(),
-                    c->pfOffR.as<int>(), c->pfNR.as<int>(), P, seed, 0u, nullptr,
-                    c->pfIdx.as<int>(), c->pfWNew.as<double>(), nullptr, 1);
+                    c->pfOffR.as<int>(), c->pfNR.as<int>(), P, seed, 0u,
+                    u0R ? dU0R.as<const double>() : nullptr, c->pfIdx.as<int>(),
+                    c->pfWNew.as<double>(), nullptr, 1);
     if (s) { cleanup(); return s; }
     hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLR)), dim3(256), 0, st,
                        c->quat.as<const double>(), R, 1, c->pfIdx.as<const int>(),
                        c->pfWNew.as<const double>(), mLR, 4, nullptr, P, c->pfQ.as<double>(),
                        c->pfWR.as<double>(), c->pfWRf.as<float>());
     s = pf_resample(st, c->pT.as<double>(), uT.as<double>(), offT.as<int>(), nTg.as<int>(),
-                    c->pfOffT.as<int>(), c->pfNT.as<int>(), P, seed, 1u, nullptr,
-                    c->pfIdx.as<int>(), c->pfWNew.as<double>(), nullptr, 1);
+                    c->pfOffT.as<int>(), c->pfNT.as<int>(), P, seed, 1u,
+                    u0T ? dU0T.as<const double>() : nullptr, c->pfIdx.as<int>(),
+                    c->pfWNew.as<double>(), nullptr, 1);
     if (s) { cleanup(); return s; }
     hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLT)), dim3(256), 0, st,
                        c->trans.as<const double>(), T, 1, c->pfIdx.as<const int>(),
@@ -703,17 +730,19 @@ int thx_pf_from_global(thx_ctx* c, int cls, int mLR, int mLT, double minStdR, do
     return from_global_impl(c, cls, nullptr, 0, mLR, mLT, minStdR, minStdT, pfS, transS, seed);
 }
 
-int thx_pf_from_global_classes(thx_ctx* c, int nCls, int mLR, int mLT, double minStdR,
-                               double minStdT, double pfS, const double* transS, uint64_t seed,
-                               int* clsOut) {
-    clear_error();
-    THX_CHECK(c && transS && clsOut && pfS > 0 && nCls >= 1, THX_ERR_ARG,
-              "pf_from_global_classes: bad argument");
-    for (int k = 0; k < nCls; k++) THX_TRY(check_scan(c, k));
+// Class of every particle from the class scans 0..nCls-1 (src/Optimiser.cpp:
+// 926-966): the class marginals on a common baseline, u_k = wC_k exp(base_k -
+// max base), keep their half-height peak with the fixed factor 1 - 1e-2
+// (setPeakFactor / keepHalfHeightPeak(PAR_C), src/Particle.cpp:1879-1886,
+// 1935-1944; include/Particle.h:47), are resampled to nCls systematic draws
+// u0 + j / nCls over the cumulative weights normalised by their last entry,
+// each taking the first class whose cdf reaches it (resample(k, PAR_C),
+// src/Particle.cpp:1280-1322), and the draw floor(pick nCls) is the class
+// (rand(cls), :2080-2089).  u0[p] in [0, 1/nCls), pick[p] in [0, 1).
+static int draw_classes(thx_ctx* c, int nCls, const double* u0, const double* pick, int* clsOut) {
     THX_HIP(hipSetDevice(c->device));
     THX_HIP(hipStreamSynchronize(c->stream));
     const int P = c->P;
-    // class marginals on a common baseline: u_k = wC_k exp(base_k - max base)
     std::vector<double> wC((size_t)nCls * P), base((size_t)nCls * P);
     for (int k = 0; k < nCls; k++) {
         THX_HIP(hipMemcpy(wC.data() + (size_t)k * P, c->res[k].wC.p, sizeof(double) * P,
@@ -721,8 +750,7 @@ int thx_pf_from_global_classes(thx_ctx* c, int nCls, int mLR, int mLT, double mi
         THX_HIP(hipMemcpy(base.data() + (size_t)k * P, c->res[k].base.p, sizeof(double) * P,
                           hipMemcpyDeviceToHost));
     }
-    std::vector<double> u(nCls), cum(nCls);
-    std::vector<int> pick(nCls);
+    std::vector<double> u(nCls), cdf(nCls);
     for (int p = 0; p < P; p++) {
         double bmax = -INFINITY;
         for (int k = 0; k < nCls; k++) bmax = std::max(bmax, base[(size_t)k * P + p]);
@@ -731,34 +759,65 @@ int thx_pf_from_global_classes(thx_ctx* c, int nCls, int mLR, int mLT, double mi
             u[k] = wC[(size_t)k * P + p] * std::exp(base[(size_t)k * P + p] - bmax);
             umax = std::max(umax, u[k]);
         }
-        // keepHalfHeightPeak(PAR_C) with the fixed factor PEAK_FACTOR_C = 1 - 1e-2
-        // (src/Particle.cpp:1879-1886, 1935-1944; include/Particle.h:47)
         const double hh = umax * (1.0 - 1e-2);
         double sum = 0.0;
         for (int k = 0; k < nCls; k++) {
             u[k] = u[k] < hh ? 0.0 : u[k] - hh;
             sum += u[k];
         }
-        // resample(k, PAR_C) from uniform class weights: systematic draws
-        // u0 + j / k over the cumulative weights (src/Particle.cpp:1275-1462),
-        // then rand(cls): one of the k draws, uniformly (:2080-2089)
-        uint32_t w[4];
-        Philox::draw(seed, (uint32_t)p, 0, kStreamClass, w);
-        const double u0 = Philox::u01(w[0], w[1]) / nCls, u1 = Philox::u01(w[2], w[3]);
+        // uniform prior times u, normalised (all-zero u: the uniform prior)
         double acc = 0.0;
         for (int k = 0; k < nCls; k++) {
             acc += sum > 0.0 ? u[k] / sum : 1.0 / nCls;
-            cum[k] = acc;
+            cdf[k] = acc;
         }
+        const double last = cdf[nCls - 1];
+        for (int k = 0; k < nCls; k++) cdf[k] /= last;
+        const int j = std::min(nCls - 1, (int)std::floor(pick[p] * nCls));
+        const double uj = u0[p] + (double)j / nCls;
         int k = 0;
-        for (int j = 0; j < nCls; j++) {
-            const double t = u0 + (double)j / nCls;
-            while (k < nCls - 1 && cum[k] <= t) k++;
-            pick[j] = k;
-        }
-        clsOut[p] = pick[std::min(nCls - 1, (int)(u1 * nCls))];
+        while (k < nCls - 1 && uj > cdf[k]) k++;
+        clsOut[p] = k;
     }
+    return THX_OK;
+}
+
+int thx_pf_from_global_classes(thx_ctx* c, int nCls, int mLR, int mLT, double minStdR,
+                               double minStdT, double pfS, const double* transS, uint64_t seed,
+                               int* clsOut) {
+    clear_error();
+    THX_CHECK(c && transS && clsOut && pfS > 0 && nCls >= 1, THX_ERR_ARG,
+              "pf_from_global_classes: bad argument");
+    for (int k = 0; k < nCls; k++) THX_TRY(check_scan(c, k));
+    const int P = c->P;
+    std::vector<double> u0(P), pick(P);
+    for (int p = 0; p < P; p++) {
+        uint32_t w[4];
+        Philox::draw(seed, (uint32_t)p, 0, kStreamClass, w);
+        u0[p] = Philox::u01(w[0], w[1]) / nCls;
+        pick[p] = Philox::u01(w[2], w[3]);
+    }
+    THX_TRY(draw_classes(c, nCls, u0.data(), pick.data(), clsOut));
     return from_global_impl(c, -1, clsOut, nCls, mLR, mLT, minStdR, minStdT, pfS, transS, seed);
+}
+
+int thx_pf_from_global_given(thx_ctx* c, int nCls, int mLR, int mLT, double minStdR,
+                             double minStdT, double pfS, const double* transS,
+                             const double* u0C, const double* pickC, const double* u0R,
+                             const double* u0T, int* clsOut) {
+    clear_error();
+    THX_CHECK(c && transS && u0R && u0T && pfS > 0 && nCls >= 1 &&
+                  (nCls == 1 || (u0C && pickC && clsOut)),
+              THX_ERR_ARG, "pf_from_global_given: bad argument");
+    for (int k = 0; k < nCls; k++) THX_TRY(check_scan(c, k));
+    if (nCls == 1) {
+        if (clsOut) std::fill(clsOut, clsOut + c->P, 0);
+        return from_global_impl(c, 0, nullptr, 0, mLR, mLT, minStdR, minStdT, pfS, transS, 0,
+                                u0R, u0T);
+    }
+    THX_TRY(draw_classes(c, nCls, u0C, pickC, clsOut));
+    return from_global_impl(c, -1, clsOut, nCls, mLR, mLT, minStdR, minStdT, pfS, transS, 0,
+                            u0R, u0T);
 }
 
 // Each active particle's n rotation samples and their weights, reordered in
@@ -1102,6 +1161,68 @@ int thx_pf_trans_stats(thx_ctx* c, const double* trans, int n, int P, double* m,
     if (m) THX_HIP(hipMemcpy(m, dm.p, sizeof(double) * 2 * P, hipMemcpyDeviceToHost));
     if (wBal) THX_HIP(hipMemcpy(wBal, dw.p, sizeof(double) * N, hipMemcpyDeviceToHost));
     DevBuf* all[] = {&dt, &dm, &ds, &dw};
+    for (DevBuf* b : all) b->release();
+    return THX_OK;
+}
+
+// Convergence test over scripted per-phase statistics (checking): phase i
+// (phase0 + i, phase0 = 1 after a global scan, else 0) gives every particle
+// k[i][P][3] and s01[i][P][2]; k_converge runs once per phase.
+int thx_pf_converge(thx_ctx* c, const double* k, const double* s01, int nPh, int P,
+                    int afterGlobal, int* nPhase) {
+    clear_error();
+    THX_CHECK(c && k && s01 && nPhase && nPh >= 1 && P >= 1, THX_ERR_ARG,
+              "pf_converge: bad argument");
+    THX_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const int phase0 = afterGlobal ? 1 : 0, minPhase = afterGlobal ? 10 : 3;
+    std::vector<double> s3(3 * (size_t)P * nPh, 0.0), best(2 * (size_t)P, 1.7976931348623157e308);
+    for (size_t i = 0; i < (size_t)P * nPh; i++) {
+        s3[3 * i] = s01[2 * i];
+        s3[3 * i + 1] = s01[2 * i + 1];
+    }
+    const std::vector<int> ones(P, 1), none(P, -1);
+    DevBuf dk, ds, da, db, dn;
+    int s = THX_OK;
+    if ((s = upload(dk, k, 3 * (size_t)P * nPh, st)) || (s = upload(ds, s3.data(), s3.size(), st)) ||
+        (s = upload(da, ones.data(), P, st)) || (s = upload(db, best.data(), best.size(), st)) ||
+        (s = upload(dn, none.data(), P, st)))
+        return s;
+    for (int i = 0; i < nPh; i++)
+        hipLaunchKernelGGL(k_converge, dim3(nb(P)), dim3(256), 0, st,
+                           dk.as<const double>() + 3 * (size_t)P * i,
+                           ds.as<const double>() + 3 * (size_t)P * i, da.as<int>(),
+                           db.as<double>(), dn.as<int>(), P, phase0 + i, minPhase);
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipStreamSynchronize(st));
+    THX_HIP(hipMemcpy(nPhase, dn.p, sizeof(int) * P, hipMemcpyDeviceToHost));
+    DevBuf* all[] = {&dk, &ds, &da, &db, &dn};
+    for (DevBuf* b : all) b->release();
+    return THX_OK;
+}
+
+// The 4x4 helpers of the inferACG stages on n given matrices A[n][16]
+// (checking): inv[n][16] and det[n] (zero inverse for a singular matrix),
+// top[n][4] the unit eigenvector of the largest eigenvalue of a symmetric A
+// with its largest-magnitude entry positive.
+int thx_pf_mat4(thx_ctx* c, const double* A, int n, double* inv, double* det, double* top) {
+    clear_error();
+    THX_CHECK(c && A && inv && det && top && n >= 1, THX_ERR_ARG, "pf_mat4: bad argument");
+    THX_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    DevBuf dA, di, dd, dt;
+    int s = THX_OK;
+    if ((s = upload(dA, A, 16 * (size_t)n, st)) || (s = di.ensure(sizeof(double) * 16 * n)) ||
+        (s = dd.ensure(sizeof(double) * n)) || (s = dt.ensure(sizeof(double) * 4 * n)))
+        return s;
+    hipLaunchKernelGGL(k_mat4_check, dim3(nb(n)), dim3(256), 0, st, dA.as<const double>(), n,
+                       di.as<double>(), dd.as<double>(), dt.as<double>());
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipStreamSynchronize(st));
+    THX_HIP(hipMemcpy(inv, di.p, sizeof(double) * 16 * n, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(det, dd.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    THX_HIP(hipMemcpy(top, dt.p, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+    DevBuf* all[] = {&dA, &di, &dd, &dt};
     for (DevBuf* b : all) b->release();
     return THX_OK;
 }

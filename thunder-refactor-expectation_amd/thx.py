@@ -74,6 +74,9 @@ _SIGS = {
     "thx_pf_set_state": [_p, _i, _i, _p, _p, _p, _p, _p, _p],
     "thx_pf_from_global": [_p, _i, _i, _i, _d, _d, _d, _p, _u64],
     "thx_pf_from_global_classes": [_p, _i, _i, _i, _d, _d, _d, _p, _u64, _p],
+    "thx_pf_from_global_given": [_p, _i, _i, _i, _d, _d, _d, _p, _p, _p, _p, _p, _p],
+    "thx_pf_converge": [_p, _p, _p, _i, _i, _i, _p],
+    "thx_pf_mat4": [_p, _p, _i, _p, _p, _p],
     "thx_pf_refine": [_p, _p, _i, _i, _d, _d, _i, _u64, _p],
     "thx_pf_get_state": [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "thx_pf_rot_stats": [_p, _p, _i, _i, _p, _p, _p, _p, _p],
@@ -395,6 +398,40 @@ class Context:
                                                 pf_s, ptr(transS), seed, ptr(out)),
               "pf_from_global_classes")
         return out
+
+    def pf_from_global_given(self, n_cls, mLR, mLT, min_std_r, min_std_t, pf_s, transS, u0R,
+                             u0T, u0C=None, pickC=None):
+        """pf_from_global(_classes) with the random draws given per particle
+        (u0R, u0T resampling offsets; u0C, pickC the class draw); returns the
+        classes."""
+        self.mLR, self.mLT = mLR, mLT
+        P = self.P
+        transS = np.broadcast_to(np.asarray(transS, np.float64), (P,)).copy()
+        arr = lambda a: None if a is None else c32(np.broadcast_to(a, (P,)), np.float64)
+        out = np.zeros(P, np.int32)
+        check(load().thx_pf_from_global_given(self._h, n_cls, mLR, mLT, min_std_r, min_std_t,
+                                              pf_s, ptr(transS), ptr(arr(u0C)), ptr(arr(pickC)),
+                                              ptr(arr(u0R)), ptr(arr(u0T)), ptr(out)),
+              "pf_from_global_given")
+        return out
+
+    def pf_converge(self, k, s01, after_global=True):
+        """Phase at which each particle's search stops under the scripted
+        per-phase statistics k [nPh][P][3], s01 [nPh][P][2] (-1: none)."""
+        k, s01 = c32(k, np.float64), c32(s01, np.float64)
+        nPh, P = k.shape[0], k.shape[1]
+        out = np.zeros(P, np.int32)
+        check(load().thx_pf_converge(self._h, ptr(k), ptr(s01), nPh, P, int(after_global),
+                                     ptr(out)), "pf_converge")
+        return out
+
+    def pf_mat4(self, A):
+        """(inverse, determinant, top eigenvector) of each 4x4 of A [n][4][4]."""
+        A = c32(A, np.float64).reshape(-1, 16)
+        n = A.shape[0]
+        inv, det, top = np.zeros((n, 4, 4)), np.zeros(n), np.zeros((n, 4))
+        check(load().thx_pf_mat4(self._h, ptr(A), n, ptr(inv), ptr(det), ptr(top)), "pf_mat4")
+        return inv, det, top
 
     def pf_refine(self, cls=None, precision=FP32, after_global=False, pf_l=2.0, pf_s=0.5,
                   max_phase=100, seed=1):
