@@ -64,7 +64,7 @@ int thx_device_count(int* n);
 /* 1. Reference-shaped entry points (host buffers, synchronous, device 0).   */
 /* ------------------------------------------------------------------------ */
 
-/* Replaces ExpectRotran (gpu/interface/Interface.h:213-222, host side
+/* Replaces ExpectRotran (gpu/interface/Interface.h:199-208, host side
  * cuthunder::expectRotran gpu/src/cuthunder.cu:1410-1596).
  * traP[nT*npxl] complex out: exp(-2 pi i (iCol tx + iRow ty) / idim);
  * trans[nT*2] in; rot[nR*4] quaternions (w,x,y,z) in;
@@ -73,7 +73,7 @@ int thx_ExpectRotran(float* traP, const double* trans, const double* rot,
                      double* rotMat, const int* iCol, const int* iRow,
                      int nR, int nT, int idim, int npxl);
 
-/* Replaces ExpectProject (gpu/interface/Interface.h:224-233, host side
+/* Replaces ExpectProject (gpu/interface/Interface.h:210-219, host side
  * gpu/src/cuthunder.cu:1604-1818).  volume: (vdim/2+1)*vdim*vdim complex,
  * x fastest; rotP[nR*npxl] complex out; rotMat[nR*9] column-major;
  * interp must be 1 (LINEAR_INTERP, the only mode the 3-D search uses,
@@ -82,7 +82,7 @@ int thx_ExpectProject(const float* volume, float* rotP, const double* rotMat,
                       const int* iCol, const int* iRow, int nR, int pf,
                       int interp, int vdim, int npxl);
 
-/* Replaces ExpectGlobal3D (gpu/interface/Interface.h:235-252, host side
+/* Replaces ExpectGlobal3D (gpu/interface/Interface.h:221-237, host side
  * gpu/src/cuthunder.cu:1826-2379).  Scores every image against every
  * (rotation, translation) of class kIdx and updates the marginals:
  *   wC[img*nK + kIdx]          = sum_{r,t} exp(dvp - baseL) pR[r] pT[t]
@@ -137,6 +137,16 @@ int thx_ctx_set_projectee(thx_ctx* ctx, int cls, const float* volFT, int N, int 
                           float* msOut);
 int thx_ctx_get_volume(thx_ctx* ctx, int cls, float* vol);
 
+/* Projection of nR rotations from class cls's resident volume at nPxl given
+ * pixels: out[r*nPxl + i] (complex) = the volume's trilinear value at
+ * M_r (iCol[i] pf, iRow[i] pf, 0), conjugated mirror for negative x, as
+ * Projector::project(Complex*, const dmat33&, iCol, iRow, nPxl, nThread)
+ * (src/Projector.cpp:356-374) computes it; rotMat[nR*9] column-major (the
+ * storage of an Eigen dmat33 and of kernel_getRotMat).  Synchronous, host
+ * buffers. */
+int thx_ctx_project(thx_ctx* ctx, int cls, int pf, const double* rotMat, int nR,
+                    const int* iCol, const int* iRow, int nPxl, float* out);
+
 /* Particle images to the context's particle data on the GPU: images [P][N][N]
  * real, row-major with the particle centre at pixel (N/2, N/2) as stored in
  * an MRC stack.  Each image's background (centred i^2 + j^2 > bgRadius^2)
@@ -179,7 +189,7 @@ int thx_ctx_init_sigma(thx_ctx* ctx, int nShell, int apply, double* sigOut, doub
 
 /* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)
  * packs them (src/Optimiser.cpp:7994-8121).  Replaces ExpectLocalIn +
- * ExpectLocalP (gpu/interface/Interface.h:30-62) for all particles at once. */
+ * ExpectLocalP (gpu/interface/Interface.h:31-38, 50-62) for all particles at once. */
 int thx_ctx_set_particles(thx_ctx* ctx, const float* datP, const float* ctfP,
                           const float* sigRcpP, int P);
 int thx_ctx_get_particles(thx_ctx* ctx, float* datP, float* ctfP,
@@ -218,7 +228,7 @@ int thx_ctx_score_one(thx_ctx* ctx, int cls, int p, float* out);
 /* One local-search phase for every particle of the context, batched and
  * ragged (Optimiser::updateSP, src/Optimiser.cpp:1139-1437; on the GPU path
  * ExpectLocalRTD + ExpectLocalPreI3D + ExpectLocalM, gpu/interface/
- * Interface.h:73-147, which handle one particle per call).  Particle p scores
+ * Interface.h:80-87, 107-123, 125-139, which handle one particle per call).  Particle p scores
  * its own nR[p] rotations (quaternions, concatenated over particles) and nT[p]
  * <= 128 translations (pixels, concatenated) with prior weights pR, pT (same
  * concatenation), against the volume of class cls[p] (cls may be NULL: class
@@ -226,7 +236,9 @@ int thx_ctx_score_one(thx_ctx* ctx, int cls, int p, float* out);
  * precision: THX_FP32, THX_BF16 or THX_FP16 (per-shell operand scales).
  * Marginals as in the global scan:
  *   wC[p] = sum w pR pT, wR[r] = sum_t w pT, wT[t] = sum_r w pR,
- *   w = exp(dvp - baseL[p]), baseL[p] = max dvp. */
+ *   w = exp(dvp - baseL[p]), baseL[p] = max dvp.
+ * nR[p] = 0 skips particle p (nT[p] may then be 0): wC[p] = 0,
+ * baseL[p] = -inf, no best pose (-1); at least one particle must be scored. */
 int thx_ctx_score_local(thx_ctx* ctx, const int* cls, const double* quat,
                         const int* nR, const double* trans, const int* nT,
                         const double* pR, const double* pT, int precision);

@@ -76,3 +76,17 @@ def test_particle_arrays_must_match_the_pixel_set():
         thx.particle_arrays(P, K, ok, np.ones(P * K))
     with pytest.raises(ValueError, match="dat"):
         thx.particle_arrays(P, K, ok, ok, np.zeros((P, K - 1, 2)))
+
+
+def test_cpp_adapter_is_in_the_library():
+    """The C++ adapter of include/thunder_expect.hpp is compiled into the
+    library: every class it declares has its members exported."""
+    out = subprocess.run(["nm", "-DC", "--defined-only", thx.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    for sym in ["thx::Device::Device(int)", "thx::Device::setPixels(", "thx::Device::setParticles(",
+                "thx::Projector::setProjectee(", "thx::Projector::setProjectee3D(",
+                "thx::Projector::project(", "thx::Expectation::setGlobalGrid(",
+                "thx::Expectation::expectation(", "thx::ManagedCalPoint::Init(",
+                "thx::ExpectLocalRTD(", "thx::ExpectLocalPreI3D(", "thx::ExpectLocalM("]:
+        assert sym in out, sym
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "adapter_check"))

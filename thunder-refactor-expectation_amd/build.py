@@ -16,6 +16,10 @@ ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_thx.so")
 
 HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_filter.hip",
                "thx_projectee.hip", "thx_images.hip", "thx_noalloc.hip", "thx_api.hip"]
+# host C++ built into the same library: the adapter that keeps the reference's
+# Projector / Optimiser::expectation / ExpectLocal* call surfaces
+CXX_SOURCES = ["thx_adapter.cpp"]
+ADAPTER_CHECK = os.path.join(ROOT, "tests", "cpp", "adapter_check")
 # hipFFT: the two plain 3-D transforms of the projectee preparation and the
 # batched 2-D transform of the particle images
 HIP_LIBS = ["-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
@@ -36,11 +40,29 @@ def _run(cmd):
 def build_library():
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     srcs = [os.path.join(PKG, "csrc", s) for s in HIP_SOURCES]
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(PKG, "csrc")]
+    objs = []
+    for s in CXX_SOURCES:
+        obj = os.path.join(PKG, "csrc", os.path.splitext(s)[0] + ".o")
+        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-c"] + inc +
+             ["-o", obj, os.path.join(PKG, "csrc", s)])
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"),
-                                  "-I", os.path.join(PKG, "csrc"), "-o", tmp] + srcs + HIP_LIBS)
+    # "-x none": the objects after the HIP sources are linked, not compiled as HIP
+    _run([hipcc] + HIPCC_FLAGS + inc + ["-o", tmp] + srcs + ["-x", "none"] + objs + HIP_LIBS)
     os.replace(tmp, LIB)
+    build_adapter_check()
     return LIB
+
+
+def build_adapter_check():
+    """The C++ driver of the adapter test (links the library)."""
+    tmp = ADAPTER_CHECK + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), "-o", tmp,
+          os.path.join(ROOT, "tests", "cpp", "adapter_check.cpp"), "-L", PKG, "-lthunder_expect",
+          "-Wl,-rpath,$ORIGIN/../../thunder-refactor-expectation_amd"])
+    os.replace(tmp, ADAPTER_CHECK)
+    return ADAPTER_CHECK
 
 
 def build_oracle():

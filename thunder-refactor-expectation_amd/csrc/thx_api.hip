@@ -201,10 +201,14 @@ uint32_t quat_morton(const double* q) {
     return m;
 }
 
-std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P) {
+std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P,
+                            const int* cnt) {
     std::vector<std::pair<uint64_t, int>> key(P);
     for (int p = 0; p < P; p++)
-        key[p] = {((uint64_t)(cls ? cls[p] : 0) << 32) | quat_morton(quat + 4 * (size_t)off[p]), p};
+        key[p] = {cnt && cnt[p] == 0   // no rotations: last, and its offset is not read
+                      ? ~0ull
+                      : ((uint64_t)(cls ? cls[p] : 0) << 32) | quat_morton(quat + 4 * (size_t)off[p]),
+                  p};
     std::stable_sort(key.begin(), key.end(),
                      [](const std::pair<uint64_t, int>& x, const std::pair<uint64_t, int>& y) {
                          return x.first < y.first;
@@ -679,11 +683,14 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     std::vector<int2> blocks;
     size_t Rtot = 0, Ttot = 0;
     int maxT = 0;
+    int nScored = 0;
     for (int p = 0; p < P; p++) {
-        THX_CHECK(nR[p] >= 1 && nR[p] <= 64 * kLocalRotBlock, THX_ERR_RANGE,
-                  "score_local: rotations per particle must be in [1, 8192]");
-        THX_CHECK(nT[p] >= 1 && nT[p] <= 128, THX_ERR_RANGE,
-                  "score_local: translations per particle must be in [1, 128]");
+        // nR[p] == 0 skips particle p (wC 0, baseL -inf, no best pose)
+        THX_CHECK(nR[p] >= 0 && nR[p] <= 64 * kLocalRotBlock, THX_ERR_RANGE,
+                  "score_local: rotations per particle must be in [0, 8192]");
+        THX_CHECK((nR[p] == 0 && nT[p] >= 0 && nT[p] <= 128) || (nT[p] >= 1 && nT[p] <= 128),
+                  THX_ERR_RANGE, "score_local: translations per particle must be in [1, 128]");
+        nScored += nR[p] > 0;
         if (cls) THX_CHECK(cls[p] >= 0 && cls[p] < nCls, THX_ERR_ARG, "score_local: bad class");
         rOff[p] = (int)Rtot;
         tOff[p] = (int)Ttot;
@@ -691,8 +698,9 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
         Ttot += nT[p];
         maxT = std::max(maxT, nT[p]);
     }
+    THX_CHECK(nScored > 0, THX_ERR_ARG, "score_local: no particle has rotations to score");
     // work items particle by particle in pose order (class, then first rotation)
-    for (int p : pose_order(quat, rOff.data(), cls, P)) {
+    for (int p : pose_order(quat, rOff.data(), cls, P, nR)) {
         bStart[p] = (int)blocks.size();
         for (int r0 = 0; r0 < nR[p]; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
     }
@@ -1579,6 +1587,43 @@ int thx_ExpectProject(const float* volume, float* rotP, const double* rotMat, co
         }
     } while (0);
     dV.release(); dCol.release(); dRow.release(); dM.release(); dOut.release();
+    return s;
+}
+
+int thx_ctx_project(thx_ctx* c, int cls, int pf, const double* rotMat, int nR, const int* iCol,
+                    const int* iRow, int nPxl, float* out) {
+    clear_error();
+    THX_CHECK(c && rotMat && iCol && iRow && out && nR > 0 && nPxl > 0 && pf > 0, THX_ERR_ARG,
+              "ctx_project: bad argument");
+    THX_CHECK(cls >= 0 && cls < (int)c->vol.size() && c->vol[cls].p, THX_ERR_STATE,
+              "ctx_project: no volume for this class");
+    const int vdim = c->vdim[cls];
+    double maxR = 0;
+    for (int k = 0; k < nPxl; k++) {
+        THX_CHECK(iCol[k] >= 0, THX_ERR_ARG, "ctx_project: iCol must be >= 0");
+        maxR = std::max(maxR, std::hypot((double)iCol[k], (double)iRow[k]));
+    }
+    THX_CHECK(maxR * pf + 2 <= vdim / 2, THX_ERR_RANGE,
+              "ctx_project: pixel radius too close to Nyquist of the padded volume");
+    THX_TRY(set_device(c));
+    hipStream_t st = c->stream;
+    DevBuf dCol, dRow, dM, dOut;
+    int s = THX_OK;
+    if ((s = upload(dCol, iCol, nPxl, st)) || (s = upload(dRow, iRow, nPxl, st)) ||
+        (s = upload(dM, rotMat, 9 * (size_t)nR, st)) ||
+        (s = dOut.ensure(sizeof(float2) * (size_t)nR * nPxl)))
+        return s;
+    hipLaunchKernelGGL(k_project_mat, dim3(nR, (nPxl + 255) / 256), dim3(256), 0, st,
+                       c->vol[cls].as<const float2>(), vdim, pf, dM.as<const double>(), nR,
+                       dCol.as<const int>(), dRow.as<const int>(), nPxl, dOut.as<float2>());
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(out, dOut.p, sizeof(float2) * (size_t)nR * nPxl, hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+        set_error("ctx_project: kernel or download failed");
+        s = THX_ERR_HIP;
+    }
+    DevBuf* all[] = {&dCol, &dRow, &dM, &dOut};
+    for (DevBuf* b : all) b->release();
     return s;
 }
 

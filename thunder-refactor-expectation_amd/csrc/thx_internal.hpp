@@ -245,7 +245,8 @@ constexpr int kLocalQueue = 2;
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
                        int mode = kLocalDirect);
 // particle order for the local scorer's work list (pose locality)
-std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P);
+std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P,
+                            const int* cnt = nullptr);
 uint32_t quat_morton(const double* q);
 // proposal network on its own stream after `ready`, `done` recorded at the end
 int cnn_propose_async(thx_cnn* n, thx_ctx* c, double* mixDev, hipEvent_t ready,

@@ -35,6 +35,7 @@ _SIGS = {
     "thx_ctx_set_pixels": [_p, _p, _p, _p, _i, _i, _i],
     "thx_ctx_set_volume": [_p, _i, _p, _i],
     "thx_ctx_get_volume": [_p, _i, _p],
+    "thx_ctx_project": [_p, _i, _i, _p, _i, _p, _p, _i, _p],
     "thx_ctx_set_projectee": [_p, _i, _p, _i, _i, _p],
     "thx_ctx_set_particle_images": [_p, _p, _i, _i, C.c_float, _i, _p, _p, _p],
     "thx_ctx_init_sigma": [_p, _i, _i, _p, _p, _p],
@@ -277,6 +278,16 @@ class Context:
         check(load().thx_ctx_init_sigma(self._h, int(n_shell), int(bool(apply)), ptr(sig),
                                          ptr(rcp), ptr(ms)), "init_sigma")
         return sig, rcp, float(ms[0])
+
+    def project(self, cls, rot_mat, iCol, iRow, pf=2):
+        """Slices of the resident class volume for column-major 3x3 matrices
+        rot_mat [nR][9] at the pixels (iCol, iRow): [nR][nPxl][2] float."""
+        m = c32(rot_mat, np.float64).reshape(-1, 9)
+        iCol, iRow = c32(iCol, np.int32), c32(iRow, np.int32)
+        out = np.zeros((m.shape[0], iCol.size, 2), np.float32)
+        check(load().thx_ctx_project(self._h, cls, pf, ptr(m), m.shape[0], ptr(iCol), ptr(iRow),
+                                     iCol.size, ptr(out)), "project")
+        return out
 
     def get_volume(self, cls, vdim):
         out = np.zeros((vdim, vdim, vdim // 2 + 1, 2), np.float32)
