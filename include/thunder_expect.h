@@ -153,7 +153,8 @@ int thx_ctx_project(thx_ctx* ctx, int cls, int pf, const double* rotMat, int nR,
  * mean is subtracted and the image divided by the background's sample
  * standard deviation, the image is moved to the centre-at-origin layout, FFT'd
  * (unnormalised, forward) and its pixel set gathered image-major; ctfP and
- * sigRcpP [P*K] are taken as in thx_ctx_set_particles.  Replaces the image
+ * sigRcpP [P*K] are taken as in thx_ctx_set_particles (either may be NULL:
+ * thx_ctx_set_ctf and thx_ctx_init_sigma then fill them).  Replaces the image
  * path of Optimiser::initImg (substractBgImg and fwImg,
  * src/Optimiser.cpp:4917-4951, 5003-5013; bgMeanStddev,
  * src/Image/ImageFunctions.cpp:607-620) and the datP packing of
@@ -187,11 +188,34 @@ int thx_ctx_set_particle_images(thx_ctx* ctx, const float* images, int P, int N,
 int thx_ctx_init_sigma(thx_ctx* ctx, int nShell, int apply, double* sigOut, double* sigRcpOut,
                        float* msOut);
 
+/* Optimiser::statImg's noise deviation and normaliseImg's scale for the last
+ * thx_ctx_set_particle_images call (src/Optimiser.cpp:4796-4895, 4987-5001):
+ * stdN = the mean over its P images of bgStddev(0, image, bgRadius), GSL's
+ * standard deviation about 0 with n - 1, of the background-subtracted
+ * images; scale = 1 / stdN rounded to float.  The packed data and the initial
+ * sigma carry that scale, as normaliseImg applies it to every image before
+ * fwImg.  Either output may be NULL. */
+int thx_ctx_get_image_stats(thx_ctx* ctx, double* stdN, float* scale);
+
 /* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)
  * packs them (src/Optimiser.cpp:7994-8121).  Replaces ExpectLocalIn +
- * ExpectLocalP (gpu/interface/Interface.h:31-38, 50-62) for all particles at once. */
+ * ExpectLocalP (gpu/interface/Interface.h:31-38, 50-62) for all particles at once.
+ * ctfP and sigRcpP may be NULL: then thx_ctx_set_ctf and thx_ctx_init_sigma
+ * (apply) fill them on the device; scoring before they are set is
+ * THX_ERR_STATE. */
 int thx_ctx_set_particles(thx_ctx* ctx, const float* datP, const float* ctfP,
                           const float* sigRcpP, int P);
+
+/* The particles' CTF at the pixel set, evaluated on the device from their CTF
+ * attributes: ctfAttr[P*7] = (voltage V, defocusU A, defocusV A, defocusTheta
+ * rad, Cs A, amplitudeContrast, phaseShift rad) per particle, the field order
+ * of CTFAttr (include/Database.h:108-124), so a std::vector<CTFAttr> passes
+ * as is; pixelSize in A.  ctf = -sqrt(1 - A^2) sin ki + A cos ki with
+ * ki = pi lambda defocus(angle) u^2 + pi/2 Cs lambda^3 u^4 - phaseShift
+ * (CTF(RFLOAT*, ...), src/CTF.cpp:109-145), as allocPreCal packs ctfP with
+ * OPTIMISER_CTF_ON_THE_FLY (src/Optimiser.cpp:8033-8057) after initCTF
+ * (:5027-5068).  P must be the context's particle count. */
+int thx_ctx_set_ctf(thx_ctx* ctx, const float* ctfAttr, int P, float pixelSize);
 int thx_ctx_get_particles(thx_ctx* ctx, float* datP, float* ctfP,
                           float* sigRcpP);
 

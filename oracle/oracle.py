@@ -372,23 +372,48 @@ def particle_images(images, bg_radius, iCol, iRow, masked=False):
     return image_spectra(images, bg_radius, masked)[:, rows, iCol]
 
 
-def image_spectra(images, bg_radius, masked=False):
-    """The stored half spectra [P][N][N/2 + 1] of the normalised images
-    (the steps of particle_images before the gather): row r is frequency
-    r (r < N/2) or r - N, column c frequency c."""
+def subtract_bg(images, bg_radius):
+    """Optimiser::substractBgImg (src/Optimiser.cpp:4917-4951) in FP64: each
+    image minus its background mean over its background sample standard
+    deviation (bgMeanStddev, src/Image/ImageFunctions.cpp:607-620), the
+    background being the centred pixels with i^2 + j^2 > r^2.  Returns the
+    images and the background mask."""
     images = np.asarray(images, np.float64)
     P, N, _ = images.shape
     h = N // 2
     j, i = np.meshgrid(np.arange(N) - h, np.arange(N) - h, indexing="ij")
     bg = (i * i + j * j) > bg_radius ** 2
+    out = np.empty_like(images)
+    for p in range(P):
+        v = images[p][bg]
+        out[p] = (images[p] - v.mean()) / v.std(ddof=1)
+    return out, bg
+
+
+def stat_img(images, bg):
+    """Optimiser::statImg's noise deviation (src/Optimiser.cpp:4796-4895):
+    the mean over the images of bgStddev(0, image, r)
+    (src/Image/ImageFunctions.cpp:585-596: GSL's standard deviation of the
+    background pixels about the given mean 0, n - 1 in the denominator)."""
+    n = int(bg.sum())
+    return float(np.mean([np.sqrt(np.sum(x[bg] ** 2) / (n - 1)) for x in images]))
+
+
+def image_spectra(images, bg_radius, masked=False):
+    """The stored half spectra [P][N][N/2 + 1] of the normalised images
+    (the steps of particle_images before the gather): substractBgImg,
+    statImg, maskImg (masked), normaliseImg (every image times 1 / stdN,
+    src/Optimiser.cpp:4987-5001), the centre move and fwImg.  Row r is
+    frequency r (r < N/2) or r - N, column c frequency c."""
+    y, bg = subtract_bg(images, bg_radius)
+    P, N, _ = y.shape
+    h = N // 2
+    scale = 1.0 / stat_img(y, bg)
     mask = soft_mask(N, bg_radius) if masked else 1.0
     out = np.empty((P, N, h + 1), np.complex128)
     for p in range(P):
-        x = images[p]
-        v = x[bg]
-        y = (x - v.mean()) / v.std(ddof=1) * mask
-        y = np.roll(y, (-h, -h), axis=(0, 1))   # stored[r][c] = file[(r + h) % N][(c + h) % N]
-        out[p] = np.fft.rfft2(y)
+        z = np.roll(y[p] * mask * scale, (-h, -h), axis=(0, 1))   # stored[r][c] = file[(r + h) % N][(c + h) % N]
+        out[p] = np.fft.rfft2(z)
     return out
 
 

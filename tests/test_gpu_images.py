@@ -198,3 +198,71 @@ def test_more_images_than_a_y_grid(ictx):
     assert err <= 1e-5
     np.testing.assert_array_equal(sig, np.broadcast_to(rcp[np.asarray(iSig)].astype(np.float32),
                                                        (P, K)))
+
+
+@pytest.mark.parametrize("name", ["box64", "box128", "box160", "box192", "box256"])
+def test_ctf_from_attributes_matches_oracle(ictx, name):
+    """The particles' CTF evaluated on the device from their CTF attributes
+    (thx_ctx_set_ctf: CTF(RFLOAT*), src/CTF.cpp:109-145, per particle as
+    allocPreCal packs ctfP on the fly) at every workload's pixel set, against
+    the oracle's FP32 restatement: per-particle voltage, defocus, astigmatism
+    angle, Cs, amplitude contrast and phase shift.  Both sides round the
+    phase ki to float (up to ~2,000 rad at box 256), so the bar is
+    4 ulp of the largest |ki| plus 2e-5."""
+    w = wl.WORKLOADS[name]
+    N, P = w["box"], 12
+    print(f"\n[ctf] {name}: N={N} cutoff={w['cutoff']} P={P}", flush=True)
+    iCol, iRow, iSig = wl.pixel_set(N, w["cutoff"])
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    K = len(iCol)
+    rng = np.random.default_rng(N)
+    attr = np.stack([rng.uniform(200e3, 300e3, P), rng.uniform(5e3, 40e3, P),
+                     rng.uniform(5e3, 40e3, P), rng.uniform(0, np.pi, P),
+                     rng.uniform(2.0e7, 2.7e7, P), rng.uniform(0.05, 0.15, P),
+                     np.where(np.arange(P) % 3 == 0, rng.uniform(0, 1.5, P), 0.0)], 1)
+    ictx.set_particles(np.zeros((P, K, 2), np.float32), None, np.ones((P, K), np.float32))
+    check_score = name == "box64"
+    if check_score:
+        # no CTF yet: scoring is refused, not run on undefined values
+        ictx.set_volume(0, np.zeros((2 * N, 2 * N, N + 1, 2), np.float32), 2 * N)
+        ictx.set_grid(wl.super_fibonacci(8), [[0.0, 0.0]])
+        with pytest.raises(thx.THXError, match="no CTF"):
+            ictx.score_global(0, thx.FP32)
+    px = 1.5
+    ictx.set_ctf(attr.astype(np.float32), px)
+    got = ictx.get_particles()[1]
+    worst = 0.0
+    for p in range(P):
+        a = attr[p].astype(np.float32)
+        ref = oracle.ctf(iCol, iRow, N, px, a[0], a[1], a[2], a[3], a[4], a[5], a[6])
+        lam = 12.2643247 / np.sqrt(a[0] * (1 + a[0] * 0.978466e-6))
+        u2 = (np.asarray(iCol) ** 2 + np.asarray(iRow) ** 2) / (px * N) ** 2
+        kmax = np.max(np.pi * lam * max(a[1], a[2]) * u2 + np.pi / 2 * a[4] * lam ** 3 * u2 ** 2)
+        tol = 2e-5 + 4 * kmax * 2.0 ** -23
+        err = np.abs(got[p] - ref).max()
+        worst = max(worst, err / tol)
+        assert err <= tol, (p, err, tol)
+    print(f"worst error / bar {worst:.2f}")
+    if check_score:
+        ictx.score_global(0, thx.FP32)   # now set: scoring runs
+
+
+@pytest.mark.parametrize("N,rbg", [(64, 30.0), (128, 60.0), (66, 31.0)])
+def test_image_stats_match_oracle(ictx, N, rbg):
+    """statImg's mean background deviation of the background-subtracted
+    images and normaliseImg's scale (src/Optimiser.cpp:4796-4895,
+    4987-5001) against the oracle: stdN to 1e-6 (the GPU sums the squares of
+    the FP32 images it writes, the oracle works in FP64), scale = 1 / stdN
+    rounded to float, and the packed data carry it (the image comparisons
+    above include it)."""
+    P = 40
+    iCol, iRow, iSig = wl.pixel_set(N, 24)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    imgs = _images(P, N, 5 + N)
+    ictx.set_particle_images(imgs, rbg, None, None)
+    stdN, scale = ictx.image_stats()
+    y, bg = oracle.subtract_bg(imgs, rbg)
+    ref = oracle.stat_img(y, bg)
+    print(f"\n[stats] N={N}: stdN {stdN:.9f} (oracle {ref:.9f}), scale {scale:.9f}")
+    assert abs(stdN - ref) <= 1e-6
+    assert scale == np.float32(1.0 / stdN)

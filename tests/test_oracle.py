@@ -874,3 +874,31 @@ def test_converge_hand_worked():
     assert oracle.converge(*_seq(0.1, 0.5, n), sD=0.9 ** np.arange(n)) == -1
     # never past the sequence: -1
     assert oracle.converge(*_seq(0.9 ** np.arange(1, 13), 0.5, 12)) == -1
+
+
+def test_stat_img_hand_worked():
+    """statImg's bgStddev(0, image, r): the background pixels' root mean
+    square about 0 with n - 1 (GSL's sd_m), averaged over images.  N = 4,
+    r = 1.2: the background is the pixels with centred i^2 + j^2 > 1.44, i.e.
+    all but the five of the centre cross (0,0), (+-1,0), (0,+-1): 11 pixels."""
+    N = 4
+    h = N // 2
+    j, i = np.meshgrid(np.arange(N) - h, np.arange(N) - h, indexing="ij")
+    bg = (i * i + j * j) > 1.2 ** 2
+    assert bg.sum() == 11
+    a = np.zeros((N, N))
+    a[bg] = 2.0                      # every background pixel 2: rms = 2 sqrt(11 / 10)
+    b = np.zeros((N, N))
+    b[bg] = np.r_[np.ones(10), 0.0]  # ten ones and a zero: sqrt(10 / 10) = 1
+    got = oracle.stat_img(np.stack([a, b]), bg)
+    assert got == pytest.approx(0.5 * (2 * np.sqrt(11 / 10) + 1.0), rel=1e-15)
+
+
+def test_stat_img_after_background_subtraction_is_one():
+    """After substractBgImg with the same region, every image's background
+    has mean 0 and sample deviation 1, so statImg's deviation about 0 is 1
+    and normaliseImg's scale is 1 (up to rounding)."""
+    rng = np.random.default_rng(4)
+    imgs = rng.normal(rng.uniform(-3, 3), rng.uniform(0.2, 4), size=(6, 32, 32))
+    y, bg = oracle.subtract_bg(imgs, 12.0)
+    assert oracle.stat_img(y, bg) == pytest.approx(1.0, abs=1e-12)

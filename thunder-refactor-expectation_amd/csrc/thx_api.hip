@@ -288,6 +288,7 @@ int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used) {
 // Particle input arrays for P particles (and the derived operands' rows).
 int alloc_particles(thx_ctx* c, int P) {
     c->partFromImages = false;   // set again by the images path once it has packed them
+    c->ctfReady = c->sigReady = false;
     c->P = P;
     c->Ppad = round_up(P, 128);
     THX_TRY(c->dat.ensure(sizeof(float2) * (size_t)P * c->K));
@@ -467,15 +468,39 @@ int thx_ctx_get_volume(thx_ctx* c, int cls, float* vol) {
 int thx_ctx_set_particles(thx_ctx* c, const float* datP, const float* ctfP,
                           const float* sigRcpP, int P) {
     clear_error();
-    THX_CHECK(c && datP && ctfP && sigRcpP && P > 0, THX_ERR_ARG, "set_particles: bad argument");
+    THX_CHECK(c && datP && P > 0, THX_ERR_ARG, "set_particles: bad argument");
     THX_CHECK(c->K > 0, THX_ERR_STATE, "set_particles: pixel set not defined");
     THX_TRY(set_device(c));
     THX_TRY(alloc_particles(c, P));
     const size_t n = (size_t)P * c->K;
     THX_HIP(hipMemcpy(c->dat.p, datP, sizeof(float2) * n, hipMemcpyHostToDevice));
-    THX_HIP(hipMemcpy(c->ctf.p, ctfP, sizeof(float) * n, hipMemcpyHostToDevice));
-    THX_HIP(hipMemcpy(c->sig.p, sigRcpP, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (ctfP) THX_HIP(hipMemcpy(c->ctf.p, ctfP, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (sigRcpP) THX_HIP(hipMemcpy(c->sig.p, sigRcpP, sizeof(float) * n, hipMemcpyHostToDevice));
+    c->ctfReady = ctfP != nullptr;
+    c->sigReady = sigRcpP != nullptr;
     return THX_OK;
+}
+
+int thx_ctx_set_ctf(thx_ctx* c, const float* ctfAttr, int P, float pixelSize) {
+    clear_error();
+    THX_CHECK(c && ctfAttr && P > 0 && pixelSize > 0.f, THX_ERR_ARG, "set_ctf: bad argument");
+    THX_CHECK(c->K > 0 && c->P > 0, THX_ERR_STATE, "set_ctf: pixels and particles must be set");
+    THX_CHECK(P == c->P, THX_ERR_ARG, "set_ctf: P differs from the context's particle count");
+    for (int p = 0; p < P; p++)
+        THX_CHECK(ctfAttr[7 * (size_t)p] > 0.f && ctfAttr[7 * (size_t)p + 5] >= 0.f &&
+                      ctfAttr[7 * (size_t)p + 5] <= 1.f,
+                  THX_ERR_ARG, "set_ctf: voltage must be > 0 and amplitude contrast in [0, 1]");
+    THX_TRY(set_device(c));
+    DevBuf dA;
+    THX_TRY(upload(dA, ctfAttr, 7 * (size_t)P, c->stream));
+    int s = launch_ctf_attr(c, dA.as<const float>(), P, pixelSize);
+    if (s == THX_OK && hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("set_ctf: kernel failed");
+        s = THX_ERR_HIP;
+    }
+    dA.release();
+    if (s == THX_OK) c->ctfReady = true;
+    return s;
 }
 
 int thx_ctx_get_particles(thx_ctx* c, float* datP, float* ctfP, float* sigRcpP) {
@@ -1438,6 +1463,7 @@ int thx_ctx_synth_particles(thx_ctx* c, int cls, int P, double maxShift, double 
     const double compSigma = N * std::sqrt(sn2 / 2);
     THX_TRY(launch_synth_noise(c, P, compSigma, seed ^ 0x6e6f697365ull));
     THX_HIP(hipStreamSynchronize(c->stream));
+    c->ctfReady = c->sigReady = true;
     return THX_OK;
 }
 
@@ -1650,6 +1676,7 @@ int thx_ExpectGlobal3D(const float* rotP, const float* traP, const float* datP, 
     THX_HIP(hipMemcpy(c->dat.p, datP, sizeof(float2) * n, hipMemcpyHostToDevice));
     THX_HIP(hipMemcpy(c->ctf.p, ctfP, sizeof(float) * n, hipMemcpyHostToDevice));
     THX_HIP(hipMemcpy(c->sig.p, sigRcpP, sizeof(float) * n, hipMemcpyHostToDevice));
+    c->ctfReady = c->sigReady = true;
     THX_TRY(set_grid_shape(c, nR, nT));
     THX_TRY(upload_priors(c, pR, pT));
     // projections and phases into the padded layouts (zero padding)

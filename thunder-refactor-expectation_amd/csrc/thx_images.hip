@@ -18,9 +18,11 @@
 //      dat[p][k] = FT_p[iFTHalf(iCol[k], iRow[k])] (Optimiser::allocPreCal,
 //      src/Optimiser.cpp:8005-8016; Image::iFTHalf, include/Image/Image.h:426-432).
 //
-// normaliseImg (src/Optimiser.cpp:4987-5001) then scales every image by the
-// inverse of the mean background standard deviation, which step 1 has made
-// exactly 1 when statImg measures the same region.  On request step 1 also
+// statImg (src/Optimiser.cpp:4796-4895) then measures the mean over the images
+// of their background standard deviation about 0 and normaliseImg
+// (:4987-5001) scales every image by its inverse: step 1 sums each written
+// image's background squares, k_image_scale forms the scale, and the packing
+// applies it to the transformed data (the transform is linear).  On request step 1 also
 // writes the soft-masked image (maskImg with zeroMask, src/Optimiser.cpp:
 // 4953-4985, and reMaskImg, :6053-6105: the image times the raised-cosine
 // mask, zero beyond it), the one the expectation packs (allocPreCal(mask =
@@ -62,7 +64,8 @@ __global__ void k_mask_table(int N, float r, float* __restrict__ table) {
 __global__ __launch_bounds__(256) void k_img_normalise(const float* __restrict__ src, int N,
                                                        float r2, float* __restrict__ dst,
                                                        const float* __restrict__ mask,
-                                                       float* __restrict__ dstMasked) {
+                                                       float* __restrict__ dstMasked,
+                                                       double2* __restrict__ bgStat) {
     const float* s = src + (size_t)blockIdx.x * N * N;
     float* d = dst + (size_t)blockIdx.x * N * N;
     const int h = N / 2;
@@ -107,13 +110,25 @@ __global__ __launch_bounds__(256) void k_img_normalise(const float* __restrict__
     const double sd = sqrt(rs[0] / (n - 1));
     const float m = (float)mean, inv = (float)(1.0 / sd);
     float* dm = dstMasked ? dstMasked + (size_t)blockIdx.x * N * N : nullptr;
+    double ss2 = 0.0;
     for (int e = threadIdx.x; e < N * N; e += blockDim.x) {
         const int x = e % N, y = e / N;
         const size_t o = (size_t)((y + h) % N) * N + (x + h) % N;
         const float v = (s[e] - m) * inv;
         d[o] = v;
         if (dm) dm[o] = v * mask[e];
+        if ((float)((x - h) * (x - h) + (y - h) * (y - h)) > r2) ss2 += (double)v * v;
     }
+    __syncthreads();
+    rs[threadIdx.x] = ss2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) rs[threadIdx.x] += rs[threadIdx.x + st];
+        __syncthreads();
+    }
+    // statImg's bgStddev(0, image, r) of the written image: its background
+    // sum of squares and count
+    if (threadIdx.x == 0) bgStat[blockIdx.x] = make_double2(rs[0], (double)n);
 }
 
 // Block-wide FP64 sum over NT threads (NT / 64 waves): wave shuffles, then
@@ -143,7 +158,8 @@ template <int V4, int NT>
 __global__ __launch_bounds__(NT) void k_img_normalise_reg(const float* __restrict__ src, int N,
                                                            float r2, float* __restrict__ dst,
                                                            const float* __restrict__ mask,
-                                                           float* __restrict__ dstMasked) {
+                                                           float* __restrict__ dstMasked,
+                                                           double2* __restrict__ bgStat) {
     const float4* s = reinterpret_cast<const float4*>(src + (size_t)blockIdx.x * N * N);
     float* d = dst + (size_t)blockIdx.x * N * N;
     const int h = N / 2, n4 = N * N / 4;
@@ -191,6 +207,7 @@ __global__ __launch_bounds__(NT) void k_img_normalise_reg(const float* __restric
     }
     const double sd = sqrt(block_sum<NT>(ss, red) / (n - 1.0));
     const float m = (float)mean, inv = (float)(1.0 / sd);
+    double ss2 = 0.0;   // background sum of squares of the written image
 #pragma unroll
     for (int v = 0; v < V4; v++) {
         const int e4 = threadIdx.x + NT * v;
@@ -199,6 +216,11 @@ __global__ __launch_bounds__(NT) void k_img_normalise_reg(const float* __restric
             const int yo = y + h < N ? y + h : y + h - N, xo = x0 + h < N ? x0 + h : x0 + h - N;
             const float4 o = make_float4((x[v].x - m) * inv, (x[v].y - m) * inv,
                                          (x[v].z - m) * inv, (x[v].w - m) * inv);
+            const float os[4] = {o.x, o.y, o.z, o.w};
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                if ((bgBits[(4 * v + c) >> 6] >> ((4 * v + c) & 63)) & 1ull)
+                    ss2 += (double)os[c] * os[c];
             float* row = d + (size_t)yo * N;
             float* rowM = dstMasked ? dstMasked + (size_t)blockIdx.x * N * N + (size_t)yo * N
                                     : nullptr;
@@ -220,18 +242,43 @@ __global__ __launch_bounds__(NT) void k_img_normalise_reg(const float* __restric
             }
         }
     }
+    const double t = block_sum<NT>(ss2, red);
+    if (threadIdx.x == 0) bgStat[blockIdx.x] = make_double2(t, n);
+}
+
+// Optimiser::statImg's noise deviation and normaliseImg's scale
+// (src/Optimiser.cpp:4796-4895, 4987-5001) over the P images of the call:
+// stdN = mean over images of bgStddev(0, image, r) = sqrt(sum / (n - 1)) of
+// the background-subtracted image (GSL's sd about a given mean), and
+// scale = 1 / stdN rounded to float as the reference's RFLOAT.  One block.
+__global__ __launch_bounds__(256) void k_image_scale(const double2* __restrict__ bgStat, int P,
+                                                     double* __restrict__ stdN,
+                                                     float* __restrict__ scale) {
+    __shared__ double red[4];
+    double a = 0.0;
+    for (int p = threadIdx.x; p < P; p += 256) a += sqrt(bgStat[p].x / (bgStat[p].y - 1.0));
+    const double tot = block_sum<256>(a, red);
+    if (threadIdx.x == 0) {
+        const double sN = tot / P;
+        stdN[0] = sN;
+        scale[0] = (float)(1.0 / sN);
+    }
 }
 
 // dat[p][k] = FT_p[row(k)][iCol[k]], row = iRow wrapped to [0, N), FT_p the
 // (N/2 + 1)-column half spectrum of image p.
 // Grid (P, ceil(K / 256)): particles on x, the dimension with the largest limit.
+// The normaliseImg scale (FFT(s y) = s FFT(y)) is applied here.
 __global__ void k_img_pack(const float2* __restrict__ ft, int N, const int* __restrict__ iCol,
-                           const int* __restrict__ iRow, int K, float2* __restrict__ dat) {
+                           const int* __restrict__ iRow, int K, const float* __restrict__ scale,
+                           float2* __restrict__ dat) {
     const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= K) return;
     const int p = blockIdx.x;
     const int j = iRow[k], row = j >= 0 ? j : j + N;
-    dat[(size_t)p * K + k] = ft[((size_t)p * N + row) * (N / 2 + 1) + iCol[k]];
+    const float2 f = ft[((size_t)p * N + row) * (N / 2 + 1) + iCol[k]];
+    const float sc = scale[0];
+    dat[(size_t)p * K + k] = make_float2(f.x * sc, f.y * sc);
 }
 
 // Initial noise spectrum (Optimiser::initSigma, src/Optimiser.cpp:5111-5209)
@@ -286,6 +333,7 @@ __global__ __launch_bounds__(256) void k_sigma_fold(double* __restrict__ part, i
 // For u < N/2 - 1 every pixel of the ring lies inside powerSpectrum's radius,
 // so both averages run over the same pixels.
 __global__ __launch_bounds__(256) void k_sigma_shells(const double* __restrict__ part, int N, int P,
+                                                      const float* __restrict__ scale,
                                                       double* __restrict__ out, int nShell) {
     const int u = blockIdx.x, W = N / 2 + 1, nPix = N * W;
     __shared__ double red[4];
@@ -304,7 +352,9 @@ __global__ __launch_bounds__(256) void k_sigma_shells(const double* __restrict__
     if (threadIdx.x == 0) {
         const double avgPs = sumP / ((double)P * n);
         const double ring = sumA / (double)P / n;
-        const double s2 = 0.5 * (avgPs - ring * ring);
+        // the spectra were taken before normaliseImg's scale: sigma^2 scales by its square
+        const double sc = scale[0];
+        const double s2 = 0.5 * (avgPs - ring * ring) * sc * sc;
         out[u] = s2;
         out[nShell + u] = -0.5 / s2;
     }
@@ -349,8 +399,7 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
                                 int softMask, const float* ctfP, const float* sigRcpP,
                                 float* msOut) {
     clear_error();
-    THX_CHECK(c && images && ctfP && sigRcpP && P > 0, THX_ERR_ARG,
-              "set_particle_images: bad argument");
+    THX_CHECK(c && images && P > 0, THX_ERR_ARG, "set_particle_images: bad argument");
     THX_CHECK(c->K > 0, THX_ERR_STATE, "set_particle_images: pixel set not defined");
     THX_CHECK(N >= 4 && N % 2 == 0 && N <= 4096, THX_ERR_ARG,
               "set_particle_images: N must be even, 4 .. 4096");
@@ -400,8 +449,9 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     THX_HIP(hipMemcpyAsync(c->imgIn.p, images, sizeof(float) * P * nImg, hipMemcpyHostToDevice,
                            st));
     const size_t nk = (size_t)P * c->K;
-    THX_HIP(hipMemcpyAsync(c->ctf.p, ctfP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
-    THX_HIP(hipMemcpyAsync(c->sig.p, sigRcpP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
+    if (ctfP) THX_HIP(hipMemcpyAsync(c->ctf.p, ctfP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
+    if (sigRcpP)
+        THX_HIP(hipMemcpyAsync(c->sig.p, sigRcpP, sizeof(float) * nk, hipMemcpyHostToDevice, st));
     THX_HIP(hipEventRecord(c->ev[0], st));
     // images of up to 16 K (N <= 128), 32 K (N <= 180) or, with 512-thread
     // blocks, 64 K pixels (N <= 256) are held in registers and read once;
@@ -415,21 +465,29 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
         THX_HIP(hipGetLastError());
     }
     const float* mask = c->imgMask.as<const float>();
+    THX_TRY(c->imgBgStat.ensure(sizeof(double2) * P));
+    THX_TRY(c->imgScale.ensure(16));   // double stdN, then float scale at byte 8
     if (N % 4 == 0 && N * N <= 1024 * 16)
         hipLaunchKernelGGL((k_img_normalise_reg<16, 256>), dim3(P), dim3(256), 0, st,
                            c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
-                           masked);
+                           masked, c->imgBgStat.as<double2>());
     else if (N % 4 == 0 && N * N <= 1024 * 32)
         hipLaunchKernelGGL((k_img_normalise_reg<32, 256>), dim3(P), dim3(256), 0, st,
                            c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
-                           masked);
+                           masked, c->imgBgStat.as<double2>());
     else if (N % 4 == 0 && N * N <= 2048 * 32)
         hipLaunchKernelGGL((k_img_normalise_reg<32, 512>), dim3(P), dim3(512), 0, st,
                            c->imgIn.as<const float>(), N, r2, c->imgReal.as<float>(), mask,
-                           masked);
+                           masked, c->imgBgStat.as<double2>());
     else
         hipLaunchKernelGGL(k_img_normalise, dim3(P), dim3(256), 0, st, c->imgIn.as<const float>(),
-                           N, r2, c->imgReal.as<float>(), mask, masked);
+                           N, r2, c->imgReal.as<float>(), mask, masked,
+                           c->imgBgStat.as<double2>());
+    THX_HIP(hipGetLastError());
+    double* dStdN = c->imgScale.as<double>();
+    float* dScale = reinterpret_cast<float*>(c->imgScale.as<char>() + 8);
+    hipLaunchKernelGGL(k_image_scale, dim3(1), dim3(256), 0, st, c->imgBgStat.as<const double2>(),
+                       P, dStdN, dScale);
     THX_HIP(hipGetLastError());
     THX_HIP(hipEventRecord(c->ev[1], st));
     THX_TRY(fft_status(hipfftExecR2C(plan, c->imgReal.as<hipfftReal>(), c->imgFT.as<hipfftComplex>()),
@@ -438,12 +496,14 @@ int thx_ctx_set_particle_images(thx_ctx* c, const float* images, int P, int N, f
     hipLaunchKernelGGL(k_img_pack, dim3(P, (c->K + 255) / 256), dim3(256), 0, st,
                        c->imgFT.as<const float2>() + (softMask ? P * nPix : 0), N,
                        c->iCol.as<const int>(),
-                       c->iRow.as<const int>(), c->K, c->dat.as<float2>());
+                       c->iRow.as<const int>(), c->K, dScale, c->dat.as<float2>());
     THX_HIP(hipGetLastError());
     THX_HIP(hipEventRecord(c->ev[3], st));
     THX_HIP(hipStreamSynchronize(st));
     c->imgReady = true;
     c->partFromImages = true;
+    c->ctfReady = ctfP != nullptr;
+    c->sigReady = sigRcpP != nullptr;
     c->nev = 0;   // the events now time this call
     c->evNames.clear();
     if (msOut) {
@@ -482,9 +542,11 @@ int thx_ctx_init_sigma(thx_ctx* c, int nShell, int apply, double* sigOut, double
     hipLaunchKernelGGL(k_sigma_fold, dim3(nb), dim3(256), 0, st, c->sigPart.as<double>(), G, nPix);
     THX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sigma_shells, dim3(nShell), dim3(256), 0, st, c->sigPart.as<const double>(),
-                       N, P, c->sigShell.as<double>(), nShell);
+                       N, P, reinterpret_cast<const float*>(c->imgScale.as<const char>() + 8),
+                       c->sigShell.as<double>(), nShell);
     THX_HIP(hipGetLastError());
     if (apply) {
+        c->sigReady = true;
         hipLaunchKernelGGL(k_sigma_apply, dim3(P, (c->K + 255) / 256), dim3(256), 0, st,
                            c->sigShell.as<const double>() + nShell, c->iSig.as<const int>(), c->K,
                            c->sig.as<float>());
@@ -501,5 +563,18 @@ int thx_ctx_init_sigma(thx_ctx* c, int nShell, int apply, double* sigOut, double
     c->nev = 0;
     c->evNames.clear();
     if (msOut) THX_HIP(hipEventElapsedTime(msOut, c->ev[0], c->ev[1]));
+    return THX_OK;
+}
+
+int thx_ctx_get_image_stats(thx_ctx* c, double* stdN, float* scale) {
+    clear_error();
+    THX_CHECK(c, THX_ERR_ARG, "get_image_stats: null context");
+    THX_CHECK(c->imgReady, THX_ERR_STATE,
+              "get_image_stats: no particle images (call thx_ctx_set_particle_images first)");
+    THX_HIP(hipSetDevice(c->device));
+    THX_HIP(hipStreamSynchronize(c->stream));
+    if (stdN) THX_HIP(hipMemcpy(stdN, c->imgScale.p, sizeof(double), hipMemcpyDeviceToHost));
+    if (scale)
+        THX_HIP(hipMemcpy(scale, c->imgScale.as<char>() + 8, sizeof(float), hipMemcpyDeviceToHost));
     return THX_OK;
 }

@@ -144,8 +144,10 @@ struct thx_ctx {
     void* imgPlan = nullptr;
     int imgN = 0, imgP = 0, imgB = 0;   // imgB transforms: P, or 2 P with the soft mask
     thx::DevBuf imgIn, imgReal, imgFT, imgMask;
+    thx::DevBuf imgBgStat, imgScale;  // per image background sum of squares and count; stdN, scale
     bool imgReady = false;            // imgFT holds the spectra of the last images
     bool partFromImages = false;      // the particle arrays were packed from those images
+    bool ctfReady = false, sigReady = false;   // the particles' ctf / sigRcp arrays are set
     thx::DevBuf sigPart, sigShell;    // initial sigma: per-pixel partial sums, shell results
     int pixMaxCol = 0, pixMaxAbsRow = 0;   // extent of the pixel set
     // particles (image-major, as uploaded) and derived operands
@@ -236,6 +238,7 @@ int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams,
                         double* dEnergy, double voltage, double Cs,
                         double ampContrast, double pixelSize);
 int launch_synth_noise(thx_ctx* c, int P, double noiseSigma, uint64_t seed);
+int launch_ctf_attr(thx_ctx* c, const float* dAttr, int P, float pixelSize);
 int launch_score_one(thx_ctx* c, int cls, int p, float* dOut);
 constexpr int kLocalRotBlock = 128;   // rotations per local-search block
 // local scorer launch shapes: one block per work item, or a persistent grid

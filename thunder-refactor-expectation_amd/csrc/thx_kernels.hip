@@ -137,10 +137,14 @@ int launch_vol_to_bf16(hipStream_t st, const float2* vol, size_t n, uint32_t* lo
     return THX_OK;
 }
 
-// CTF(RFLOAT* dst, ...) closed form, src/CTF.cpp:109-145, in float.
+// CTF(RFLOAT* dst, ...) closed form, src/CTF.cpp:109-145, in float (RFLOAT):
+// u = |(i, j)| / (N pixelSize), angle = atan2(j, i) - theta, defocus =
+// -(dU + dV + (dU - dV) cos 2 angle) / 2, ki = K1 defocus u^2 + K2 u^4 - phase
+// shift, ctf = -sqrt(1 - A^2) sin ki + A cos ki, K1 = pi lambda,
+// K2 = pi / 2 Cs lambda^3.
 __device__ inline float ctf_value(int ic, int ir, float pixelSize, float lambda,
                                   float dU, float dV, float theta, float Cs,
-                                  float ampC, int N) {
+                                  float ampC, int N, float phaseShift = 0.f) {
     const float w1 = sqrtf(1 - ampC * ampC), w2 = ampC;
     const float K1 = (float)M_PI * lambda;
     const float K2 = (float)(M_PI / 2 * (double)Cs * ((double)lambda * lambda * lambda));
@@ -148,8 +152,36 @@ __device__ inline float ctf_value(int ic, int ir, float pixelSize, float lambda,
     const float angle = (float)(atan2((double)ir, (double)ic) - theta);
     const float defocus = -(dU + dV + (dU - dV) * cosf(2 * angle)) / 2;
     const float u2 = u * u;
-    const float ki = K1 * defocus * u2 + K2 * (u2 * u2);
+    const float ki = K1 * defocus * u2 + K2 * (u2 * u2) - phaseShift;
     return -w1 * sinf(ki) + w2 * cosf(ki);
+}
+
+// ctf[p][k] from particle p's CTF attributes (CTFAttr, include/Database.h:
+// 108-124: voltage, defocusU, defocusV, defocusTheta, Cs, amplitudeContrast,
+// phaseShift), as allocPreCal fills ctfP with OPTIMISER_CTF_ON_THE_FLY
+// (src/Optimiser.cpp:8033-8057, include/Config.h:145).  The electron
+// wavelength 12.2643247 / sqrt(V (1 + 0.978466e-6 V)) Angstrom is formed in
+// double and rounded to float, as the reference's RFLOAT lambda.
+// Grid (P, ceil(K / 256)), particles on x.
+__global__ __launch_bounds__(256) void k_ctf_attr(const float* __restrict__ attr, int K, int N,
+                                                  float pixelSize, const int* __restrict__ iCol,
+                                                  const int* __restrict__ iRow,
+                                                  float* __restrict__ ctf) {
+    const int p = blockIdx.x, k = blockIdx.y * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const float* a = attr + 7 * (size_t)p;
+    const float V = a[0];
+    const float lambda = (float)(12.2643247 / sqrt((double)V * (1 + (double)V * 0.978466e-6)));
+    ctf[(size_t)p * K + k] =
+        ctf_value(iCol[k], iRow[k], pixelSize, lambda, a[1], a[2], a[3], a[4], a[5], N, a[6]);
+}
+
+int launch_ctf_attr(thx_ctx* c, const float* dAttr, int P, float pixelSize) {
+    hipLaunchKernelGGL(k_ctf_attr, dim3(P, (c->K + 255) / 256), dim3(256), 0, c->stream, dAttr,
+                       c->K, c->idim, pixelSize, c->iCol.as<const int>(), c->iRow.as<const int>(),
+                       c->ctf.as<float>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -273,6 +305,10 @@ __global__ __launch_bounds__(256) void k_prep(
 }
 
 int launch_prep_particles(thx_ctx* c, int precision) {
+    THX_CHECK(c->ctfReady, THX_ERR_STATE,
+              "particles have no CTF (set it with the particle arrays or thx_ctx_set_ctf)");
+    THX_CHECK(c->sigReady, THX_ERR_STATE,
+              "particles have no sigRcp (set it with the particle arrays or thx_ctx_init_sigma)");
     const float* scale = (precision == THX_FP16 || precision == THX_FP8)
                              ? c->shellScale.as<const float>() : nullptr;
     hipLaunchKernelGGL(k_prep, dim3(c->Ppad), dim3(256), 0, c->stream,

@@ -39,7 +39,9 @@ _SIGS = {
     "thx_ctx_set_projectee": [_p, _i, _p, _i, _i, _p],
     "thx_ctx_set_particle_images": [_p, _p, _i, _i, C.c_float, _i, _p, _p, _p],
     "thx_ctx_init_sigma": [_p, _i, _i, _p, _p, _p],
+    "thx_ctx_get_image_stats": [_p, _p, _p],
     "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
+    "thx_ctx_set_ctf": [_p, _p, _i, C.c_float],
     "thx_ctx_get_particles": [_p, _p, _p, _p],
     "thx_ctx_set_grid": [_p, _p, _i, _p, _i, _p, _p],
     "thx_ctx_score_global": [_p, _i, _i],
@@ -133,9 +135,10 @@ def particle_arrays(P, K, ctf, sig, dat=None):
     contiguous float32, checked against (P, K) (dat: (P, K) complex as
     (P, K, 2) floats).  The library copies P * K values from each host
     pointer, so a shorter array would be read past its end: raise instead."""
-    ctf, sig = c32(ctf), c32(sig)
+    ctf = None if ctf is None else c32(ctf)
+    sig = None if sig is None else c32(sig)
     for name, a in (("ctf", ctf), ("sig", sig)):
-        if a.shape != (P, K):
+        if a is not None and a.shape != (P, K):
             raise ValueError(f"{name} has shape {a.shape}, expected {(P, K)}")
     if dat is not None:
         dat = c32(dat)
@@ -267,6 +270,14 @@ class Context:
         self.P = P
         return ms
 
+    def image_stats(self):
+        """(stdN, scale) of the last set_particle_images call: statImg's mean
+        background deviation and normaliseImg's scale 1 / stdN."""
+        sN = np.zeros(1, np.float64)
+        sc = np.zeros(1, np.float32)
+        check(load().thx_ctx_get_image_stats(self._h, ptr(sN), ptr(sc)), "image_stats")
+        return float(sN[0]), float(sc[0])
+
     def init_sigma(self, n_shell, apply=False):
         """Initial sigma^2 and sigRcp = -0.5 / sigma^2 per shell (0 .. n_shell - 1)
         from the last set_particle_images call (Optimiser::initSigma); with
@@ -294,8 +305,16 @@ class Context:
         check(load().thx_ctx_get_volume(self._h, cls, ptr(out)), "get_volume")
         return out
 
+    def set_ctf(self, attr, pixel_size):
+        """The particles' CTF from their attributes attr [P][7] = (voltage,
+        defocusU, defocusV, defocusTheta, Cs, amplitudeContrast, phaseShift),
+        evaluated on the device (CTF(RFLOAT*), src/CTF.cpp:109-145)."""
+        attr = c32(attr).reshape(-1, 7)
+        check(load().thx_ctx_set_ctf(self._h, ptr(attr), attr.shape[0], float(pixel_size)),
+              "set_ctf")
+
     def set_particles(self, dat, ctf, sig):
-        P = np.shape(ctf)[0]
+        P = np.shape(dat)[0]
         ctf, sig, dat = particle_arrays(P, self.K, ctf, sig, dat)
         check(load().thx_ctx_set_particles(self._h, ptr(dat), ptr(ctf), ptr(sig), P),
               "set_particles")
