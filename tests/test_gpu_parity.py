@@ -254,6 +254,42 @@ def test_box128_against_oracle(gpu_ctx):
             assert rel.max() <= r_max and gap.max() <= g_max and agree >= a_min
 
 
+def test_box192_against_oracle(gpu_ctx):
+    """Box 192 (the learned-proposal workload's size), cutoff 75 (K = 8,725),
+    its 5 x 5 shift grid: the FP32 and BF16 whole-grid scans that the learned
+    step falls back to and is judged by, against the FP64 oracle.  FP32 within
+    1e-4 relative of the best log-likelihood with the identical top pose; BF16
+    at its bar (best-score error, chosen pose's FP64 gap)."""
+    print("\n[setup] box192 P=48 R=64 T=25", flush=True)
+    info = wl.setup_context(gpu_ctx, "box192", n_particles=48, n_rot=64)
+    w = wl.WORKLOADS["box192"]
+    Vb = w["box"] * w["pf"]
+    vol = gpu_ctx.get_volume(0, Vb)
+    dat, ctf, sig = gpu_ctx.get_particles()
+    R, T = len(info["quat"]), len(info["trans"])
+    assert len(info["iCol"]) == 8725 and T == 25
+    dvp = oracle.global_scan(vol, Vb, w["pf"], info["iCol"], info["iRow"], w["box"],
+                             info["quat"], info["trans"], dat, ctf, sig, np.full(R, 1.0 / R),
+                             np.full(T, 1.0 / T), f64=True, want_dvp=True)[4]
+    del vol
+    flat = dvp.reshape(len(dvp), -1)
+    best = flat.max(1)
+    for prec in ("fp32", "bf16"):
+        gpu_ctx.score_global(0, thx.PRECISIONS[prec])
+        bR, bT, bS = gpu_ctx.get_best(0)
+        chosen = flat[np.arange(len(flat)), bR * T + bT]
+        rel = np.abs(bS - best) / np.abs(best)
+        gap = (best - chosen) / np.abs(best)
+        agree = np.mean(flat.argmax(1) == bR * T + bT)
+        print(f"{prec}: max rel best err {rel.max():.2e}, max rel FP64 gap {gap.max():.2e}, "
+              f"agreement {agree:.3f}")
+        if prec == "fp32":
+            assert rel.max() <= 1e-4 and agree == 1.0
+        else:
+            r_max, a_min, g_max = LOW_BARS[prec]
+            assert rel.max() <= r_max and gap.max() <= g_max and agree >= a_min
+
+
 def test_synth_particles_deterministic(gpu_ctx):
     """The same seed gives bit-identical particles (images, CTFs, noise
     levels): the signal energy that sets the noise level is summed in a fixed
