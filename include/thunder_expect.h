@@ -541,6 +541,58 @@ int thx_ctx_synth_particles(thx_ctx* ctx, int cls, int P, double maxShift,
                             double snr, uint64_t seed, double* trueQuat,
                             double* trueShift, double* ctfParams);
 
+/* ------------------------------------------------------------------------ */
+/* On-disk formats (host only; no device work except thx_ctx_load_thu).      */
+/* ------------------------------------------------------------------------ */
+
+/* One particle of a .thu file: the 27 whitespace-separated fields of a line
+ * (include/Database.h:12-88, read by Database, src/Database.cpp:100-300),
+ * in file order. */
+typedef struct thx_thu_record {
+    float ctf[7];          /* voltage, defocusU, defocusV, defocusTheta, Cs,
+                              amplitudeContrast, phaseShift (CTFAttr order)   */
+    char particle[256];    /* "index@stack.mrcs" (1-based) or a file name      */
+    char micrograph[256];
+    double coord[2];       /* coordinate x, y on the micrograph                */
+    int group, cls;        /* group id, class id                              */
+    double quat[4];        /* rotation (w, x, y, z)                           */
+    double k[3];           /* rotation concentrations k1, k2, k3              */
+    double trans[2];       /* translation (pixels)                            */
+    double stdTrans[2];    /* translation deviations s0, s1                   */
+    double df, stdDf;      /* defocus factor and its deviation                */
+    double score;
+} thx_thu_record;
+
+/* Reads the particles of a .thu file: *n receives their number; the first
+ * min(n, cap) go to rec (cap 0, rec NULL: count only).  A line with fewer
+ * than 27 fields is an error naming the line. */
+int thx_thu_read(const char* path, thx_thu_record* rec, int cap, int* n);
+/* Writes n particles (append != 0: at the end of the file) with the field
+ * order and %18.9lf format of Optimiser::saveDatabase
+ * (src/Optimiser.cpp:8167-8268). */
+int thx_thu_write(const char* path, const thx_thu_record* rec, int n, int append);
+
+/* MRC files (ImageFile, src/Image/ImageFile.cpp:174-300): the dimensions and
+ * mode (0 int8, 1 int16, 2 float32; others THX_ERR_RANGE) of the header. */
+int thx_mrc_info(const char* path, int* nx, int* ny, int* nz, int* mode);
+/* Slices first .. first + count - 1 as float, nx fastest, after the 1024-byte
+ * header and nsymbt bytes of extended header: out[count * ny * nx]. */
+int thx_mrc_read(const char* path, int first, int count, float* out);
+/* An nx x ny x nz float32 (mode 2) file with the header fillMRCHeader writes
+ * (mapc/mapr/maps 1 2 3, 90 degree cell, "MAP "), the cell in Angstrom from
+ * pixelSize, the density range and mean, and the little-endian stamp. */
+int thx_mrc_write(const char* path, const float* data, int nx, int ny, int nz, float pixelSize);
+
+/* Particles first .. first + count - 1 of a .thu file into the context: each
+ * particle's image read from its stack under parPrefix ("index@stack" with a
+ * 1-based index, as Optimiser::initImg reads them, src/Optimiser.cpp:
+ * 4620-4660), then thx_ctx_set_particle_images (bgRadius, softMask; msOut as
+ * there) and thx_ctx_set_ctf with the particles' CTF attributes and
+ * pixelSize.  The pixel set must be set; sigRcp is left to
+ * thx_ctx_init_sigma(apply). */
+int thx_ctx_load_thu(thx_ctx* ctx, const char* thuPath, const char* parPrefix, int first,
+                     int count, float pixelSize, float bgRadius, int softMask, float* msOut);
+
 #ifdef __cplusplus
 }
 #endif

@@ -266,3 +266,42 @@ def test_image_stats_match_oracle(ictx, N, rbg):
     print(f"\n[stats] N={N}: stdN {stdN:.9f} (oracle {ref:.9f}), scale {scale:.9f}")
     assert abs(stdN - ref) <= 1e-6
     assert scale == np.float32(1.0 / stdN)
+
+
+def test_load_thu_equals_direct_upload(ictx, tmp_path):
+    """Particles read from a .thu file and its MRC stacks (thx_ctx_load_thu:
+    "index@stack" paths under a prefix, as Optimiser::initImg reads them,
+    CTF from the file's attributes) equal, bit for bit, the same images and
+    attributes given to set_particle_images + set_ctf; a sub-range of the file
+    likewise."""
+    import formats
+    N, P, rbg, px = 64, 24, 30.0, 1.5
+    iCol, iRow, iSig = wl.pixel_set(N, 24)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    imgs = _images(P, N, 123)
+    rng = np.random.default_rng(8)
+    attr = np.stack([np.full(P, 300e3), rng.uniform(1e4, 3e4, P), rng.uniform(1e4, 3e4, P),
+                     rng.uniform(0, 3, P), np.full(P, 2.7e7), np.full(P, 0.07),
+                     np.zeros(P)], 1)
+    attr = np.array([[float(f"{x:18.6f}") for x in a] for a in attr], np.float32)
+    d = tmp_path / "particles"
+    d.mkdir()
+    thx.mrc_write(str(d / "a.mrcs"), imgs[:10], px)
+    thx.mrc_write(str(d / "b.mrcs"), imgs[10:], px)
+    thu = str(tmp_path / "meta.thu")
+    with open(thu, "w") as fh:
+        for p in range(P):
+            name = f"{p + 1}@a.mrcs" if p < 10 else f"{p - 9}@b.mrcs"
+            fh.write(formats.thu_line_input(attr[p], name, "mic.mrc", (0.0, 0.0), 1))
+    for first, count in ((0, P), (5, 12)):
+        print(f"\n[thu] particles {first}..{first + count - 1}", flush=True)
+        ictx.load_thu(thu, str(d) + "/", first, count, px, rbg, soft_mask=True)
+        dat1, ctf1, _ = ictx.get_particles()
+        sl = slice(first, first + count)
+        ictx.set_particle_images(imgs[sl], rbg, None, None, soft_mask=True)
+        ictx.set_ctf(attr[sl], px)
+        dat2, ctf2, _ = ictx.get_particles()
+        np.testing.assert_array_equal(dat1, dat2)
+        np.testing.assert_array_equal(ctf1, ctf2)
+    with pytest.raises(thx.THXError, match="has 24 particles"):
+        ictx.load_thu(thu, str(d) + "/", 20, 10, px, rbg)

@@ -17,8 +17,9 @@ ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_thx.so")
 HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_filter.hip",
                "thx_projectee.hip", "thx_images.hip", "thx_noalloc.hip", "thx_api.hip"]
 # host C++ built into the same library: the adapter that keeps the reference's
-# Projector / Optimiser::expectation / ExpectLocal* call surfaces
-CXX_SOURCES = ["thx_adapter.cpp"]
+# Projector / Optimiser::expectation / ExpectLocal* call surfaces, and the
+# .thu / MRC file formats
+CXX_SOURCES = ["thx_adapter.cpp", "thx_io.cpp"]
 ADAPTER_CHECK = os.path.join(ROOT, "tests", "cpp", "adapter_check")
 # hipFFT: the two plain 3-D transforms of the projectee preparation and the
 # batched 2-D transform of the particle images

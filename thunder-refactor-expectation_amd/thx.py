@@ -89,7 +89,63 @@ _SIGS = {
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
                                 _u64, _p, _p, _p],
 }
+_SIGS.update({
+    "thx_thu_read": [C.c_char_p, _p, _i, _p],
+    "thx_thu_write": [C.c_char_p, _p, _i, _i],
+    "thx_mrc_info": [C.c_char_p, _p, _p, _p, _p],
+    "thx_mrc_read": [C.c_char_p, _i, _i, _p],
+    "thx_mrc_write": [C.c_char_p, _p, _i, _i, _i, C.c_float],
+    "thx_ctx_load_thu": [_p, C.c_char_p, C.c_char_p, _i, _i, C.c_float, C.c_float, _i, _p],
+})
 _RESTYPES = {"thx_last_error": C.c_char_p}
+
+
+class ThuRecord(C.Structure):
+    """One .thu line (thx_thu_record in thunder_expect.h)."""
+    _fields_ = [("ctf", C.c_float * 7), ("particle", C.c_char * 256),
+                ("micrograph", C.c_char * 256), ("coord", C.c_double * 2), ("group", C.c_int),
+                ("cls", C.c_int), ("quat", C.c_double * 4), ("k", C.c_double * 3),
+                ("trans", C.c_double * 2), ("stdTrans", C.c_double * 2), ("df", C.c_double),
+                ("stdDf", C.c_double), ("score", C.c_double)]
+
+
+def thu_read(path):
+    """The particles of a .thu file as a list of ThuRecord."""
+    n = C.c_int(0)
+    check(load().thx_thu_read(path.encode(), None, 0, C.byref(n)), "thu_read")
+    arr = (ThuRecord * n.value)()
+    check(load().thx_thu_read(path.encode(), C.cast(arr, C.c_void_p), n.value, C.byref(n)),
+          "thu_read")
+    return list(arr)
+
+
+def thu_write(path, records, append=False):
+    arr = (ThuRecord * len(records))(*records)
+    check(load().thx_thu_write(path.encode(), C.cast(arr, C.c_void_p), len(records),
+                               int(bool(append))), "thu_write")
+
+
+def mrc_info(path):
+    v = [C.c_int(0) for _ in range(4)]
+    check(load().thx_mrc_info(path.encode(), *[C.byref(x) for x in v]), "mrc_info")
+    return tuple(x.value for x in v)   # nx, ny, nz, mode
+
+
+def mrc_read(path, first=0, count=None):
+    nx, ny, nz, _ = mrc_info(path)
+    count = nz - first if count is None else count
+    out = np.zeros((count, ny, nx), np.float32)
+    check(load().thx_mrc_read(path.encode(), first, count, ptr(out)), "mrc_read")
+    return out
+
+
+def mrc_write(path, data, pixel_size=1.0):
+    data = c32(data)
+    if data.ndim == 2:
+        data = data[None]
+    nz, ny, nx = data.shape
+    check(load().thx_mrc_write(path.encode(), ptr(data), nx, ny, nz, float(pixel_size)),
+          "mrc_write")
 
 
 class THXError(RuntimeError):
@@ -304,6 +360,17 @@ class Context:
         out = np.zeros((vdim, vdim, vdim // 2 + 1, 2), np.float32)
         check(load().thx_ctx_get_volume(self._h, cls, ptr(out)), "get_volume")
         return out
+
+    def load_thu(self, thu_path, par_prefix, first, count, pixel_size, bg_radius,
+                 soft_mask=False):
+        """Particles first .. first + count - 1 of a .thu file (images from
+        their MRC stacks under par_prefix, CTF from their attributes)."""
+        ms = np.zeros(3, np.float32)
+        check(load().thx_ctx_load_thu(self._h, thu_path.encode(), par_prefix.encode(), first,
+                                      count, float(pixel_size), float(bg_radius),
+                                      int(bool(soft_mask)), ptr(ms)), "load_thu")
+        self.P = count
+        return ms
 
     def set_ctf(self, attr, pixel_size):
         """The particles' CTF from their attributes attr [P][7] = (voltage,
