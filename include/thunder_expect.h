@@ -197,6 +197,30 @@ int thx_ctx_init_sigma(thx_ctx* ctx, int nShell, int apply, double* sigOut, doub
  * fwImg.  Either output may be NULL. */
 int thx_ctx_get_image_stats(thx_ctx* ctx, double* stdN, float* scale);
 
+/* The noise spectrum refreshed from each particle's best pose after an
+ * expectation (Optimiser::allReduceSigma, src/Optimiser.cpp:6355-6666, 3-D,
+ * best pose only, all shells to maxR, masked and unmasked residuals mixed,
+ * recentred images).  Particle p's pose: class cls[p] (NULL: 0), quaternion
+ * quat[p*4], translation trans[p*2] (pixels) and recentring offset
+ * offset[p*2] (NULL: 0); group[p] in 0 .. nGroup-1 (NULL: one group).  The
+ * model CTF . shift . slice is formed on the half-spectrum pixels with
+ * |k| < r (the projector's maxRadius, <= N/2 - 1) from the class volumes and
+ * the particles' CTF attributes (thx_ctx_set_ctf), and compared with the
+ * masked and unmasked spectra of the last thx_ctx_set_particle_images call
+ * (with their normaliseImg scale).  Per group and shell u < nShell
+ * (<= N/2 - 1): sigM, sigN = mean over the group's particles of half the
+ * mean residual power of the shell, masked and unmasked; svd = mean of
+ * sqrt(model power / masked data power); sigma^2 = ratio sigM + (1 - ratio)
+ * alpha sigN with ratio = min(1, svd) and alpha = sqrt(pi) maskRadius /
+ * (N pixelSize) (A); sigRcp = -0.5 / sigma^2.  sigOut, sigRcpOut
+ * [nGroup*nShell] (optional); apply != 0 writes sigRcp(group, iSig[k]) into
+ * every particle's sigRcpP as allocPreCal does.  msOut (optional, 1 float):
+ * kernel time. */
+int thx_ctx_refresh_sigma(thx_ctx* ctx, const int* cls, const double* quat, const double* trans,
+                          const double* offset, const int* group, int nGroup, int r, int nShell,
+                          float maskRadius, float pixelSize, int apply, double* sigOut,
+                          double* sigRcpOut, float* msOut);
+
 /* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)
  * packs them (src/Optimiser.cpp:7994-8121).  Replaces ExpectLocalIn +
  * ExpectLocalP (gpu/interface/Interface.h:31-38, 50-62) for all particles at once.

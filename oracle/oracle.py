@@ -441,3 +441,74 @@ def init_sigma(images, bg_radius, n_shell):
         ps_avg = (avg.real[m] + avg.imag[m]).mean() ** 2
         sig[u] = 0.5 * (avg_ps - ps_avg)
     return sig, -0.5 / sig
+
+
+def spectrum_pixels(N):
+    """The stored half-spectrum pixels in the loop order of
+    IMAGE_FOR_EACH_PIXEL_FT (include/Image/Image.h:50-52): column i = 0 .. N/2,
+    signed row j = -N/2 .. N/2 - 1.  Returns (i, j, row index into the stored
+    spectrum)."""
+    j, i = np.meshgrid(np.arange(-N // 2, N // 2), np.arange(N // 2 + 1), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    return i, j, np.where(j >= 0, j, j + N)
+
+
+def refresh_sigma(vol, V, pf, specM, specN, ctf_attr, pixel_size, cls, quat, trans, offset,
+                  group, n_group, r, n_shell, mask_radius):
+    """Optimiser::allReduceSigma(mask, group) (src/Optimiser.cpp:6355-6666) in
+    FP64, 3-D, OPTIMISER_SIGMA_RANK1ST and OPTIMISER_SIGMA_WHOLE_FREQUENCY on,
+    OPTIMISER_SIGMA_MASK off, OPTIMISER_RECENTRE_IMAGE_EACH_ITERATION on
+    (include/Config.h:165-198).  Per particle l with best pose (cls, quat,
+    trans): the model M = CTF(l) . exp(-2 pi i (k . t) / N) . P_cls(R k) on the
+    pixels |k| < r (Projector::project(Image&, rot, t) with maxRadius r,
+    src/Projector.cpp:276-296, 456-464; translate(..., r, ...),
+    src/Image/ImageFunctions.cpp:158-174; CTF(Image&, ..., r),
+    src/CTF.cpp:66-105), zero elsewhere, and M' the same with t - offset.
+    With specM / specN the masked / unmasked stored half spectra [P][N][N/2+1]
+    and powerSpectrum(u) the mean of |.|^2 over the pixels of shell
+    rint(|k|) = u < n_shell (src/Functions/Spectrum.cpp:161-190):
+      sigM_g(u) = mean over the group's particles of ps(specM - M)(u) / 2,
+      sigN_g(u) = ... ps(specN - M')(u) / 2,
+      svd_g(u)  = mean of sqrt(ps(M)(u) / ps(specM)(u)),
+      sig = ratio sigM + (1 - ratio) alpha sigN, ratio = min(1, svd),
+      alpha = sqrt(pi (maskRadius / (N pixelSize))^2), sigRcp = -0.5 / sig.
+    Returns (sig, sigRcp) [n_group][n_shell]."""
+    specM, specN = np.asarray(specM), np.asarray(specN)
+    P, N = specM.shape[0], specM.shape[1]
+    i, j, row = spectrum_pixels(N)
+    u = np.rint(np.hypot(i, j)).astype(np.int64)
+    keep = u < n_shell
+    i, j, row, u = i[keep], j[keep], row[keep], u[keep]
+    cnt = np.bincount(u, minlength=n_shell).astype(np.float64)
+    inr = i * i + j * j < r * r
+    sM = np.zeros((n_group, n_shell))
+    sN = np.zeros((n_group, n_shell))
+    sv = np.zeros((n_group, n_shell))
+    ng = np.zeros(n_group)
+    vols = vol if isinstance(vol, (list, tuple)) else [vol]
+    for l in range(P):
+        a = np.asarray(ctf_attr[l], np.float32)
+        c = ctf(i, j, N, pixel_size, a[0], a[1], a[2], a[3], a[4], a[5], a[6]).astype(np.float64)
+        Pk = np.zeros(i.size, np.complex128)
+        pin = project(vols[int(cls[l])], V, rotmat(quat[l]), i[inr], j[inr], pf, f64=True)
+        Pk[inr] = pin[..., 0] + 1j * pin[..., 1]
+        t = np.asarray(trans[l], np.float64)
+        t2 = t - (np.asarray(offset[l], np.float64) if offset is not None else 0.0)
+        M = np.where(inr, c * np.exp(-2j * np.pi * (i * t[0] + j * t[1]) / N) * Pk, 0)
+        Mn = np.where(inr, c * np.exp(-2j * np.pi * (i * t2[0] + j * t2[1]) / N) * Pk, 0)
+        xm = specM[l][row, i]
+        xn = specN[l][row, i]
+        ps = lambda z: np.bincount(u, weights=np.abs(z) ** 2, minlength=n_shell) / cnt
+        g = int(group[l]) if group is not None else 0
+        sM[g] += ps(xm - M) / 2
+        sN[g] += ps(xn - Mn) / 2
+        sv[g] += np.sqrt(ps(M) / ps(xm))
+        ng[g] += 1
+    sM /= ng[:, None]
+    sN /= ng[:, None]
+    sv /= ng[:, None]
+    alpha = np.sqrt(np.pi * (mask_radius / (N * pixel_size)) ** 2)
+    ratio = np.minimum(1.0, sv)
+    sig = ratio * sM + (1 - ratio) * alpha * sN
+    with np.errstate(divide="ignore"):
+        return sig, -0.5 / sig

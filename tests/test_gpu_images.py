@@ -305,3 +305,49 @@ def test_load_thu_equals_direct_upload(ictx, tmp_path):
         np.testing.assert_array_equal(ctf1, ctf2)
     with pytest.raises(thx.THXError, match="has 24 particles"):
         ictx.load_thu(thu, str(d) + "/", 20, 10, px, rbg)
+
+
+def test_refresh_sigma_matches_oracle(ictx):
+    """The sigma refresh from best poses (Optimiser::allReduceSigma) against
+    the FP64 oracle: two classes, two groups, masked and unmasked spectra of
+    real-space images, per-particle CTF attributes, poses, shifts and
+    recentring offsets.  The GPU forms the model in FP32 (trilinear slice, CTF,
+    phase) from FP32 spectra: every shell's sigma^2 within 2e-4 relative; with
+    apply, every particle's sigRcp is its group's value at its pixel's shell."""
+    N, P, rbg, px, r = 64, 40, 30.0, 1.5, 20
+    nS = N // 2 - 1
+    w = wl.WORKLOADS["box64"]
+    print(f"\n[sigma refresh] N={N} P={P} r={r} shells={nS}", flush=True)
+    iCol, iRow, iSig = wl.pixel_set(N, 24)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    vols = []
+    for c in range(2):
+        ictx.synth_volume(c, N, 2, w["n_blob"], w["blob_sigma"], w["blob_radius"], 40 + c)
+        vols.append(ictx.get_volume(c, 2 * N))
+    imgs = _images(P, N, 77)
+    ictx.set_particle_images(imgs, rbg, None, None, soft_mask=True)
+    rng = np.random.default_rng(12)
+    attr = np.stack([np.full(P, 300e3), rng.uniform(1e4, 3e4, P), rng.uniform(1e4, 3e4, P),
+                     rng.uniform(0, 3, P), np.full(P, 2.7e7), np.full(P, 0.07),
+                     np.where(np.arange(P) % 4 == 0, 0.3, 0.0)], 1).astype(np.float32)
+    ictx.set_ctf(attr, px)
+    quat = rng.normal(size=(P, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    trans = rng.uniform(-3, 3, size=(P, 2))
+    offset = rng.uniform(-1, 1, size=(P, 2))
+    cls = rng.integers(0, 2, P).astype(np.int32)
+    grp = (np.arange(P) % 2).astype(np.int32)
+    mask_r = 40.0
+    sig, rcp, ms = ictx.refresh_sigma(quat, trans, r, nS, mask_r, px, cls=cls, offset=offset,
+                                      group=grp, n_group=2, apply=True)
+    specM = oracle.image_spectra(imgs, rbg, masked=True)
+    specN = oracle.image_spectra(imgs, rbg, masked=False)
+    ref, ref_rcp = oracle.refresh_sigma(vols, 2 * N, 2, specM, specN, attr, px, cls, quat,
+                                        trans, offset, grp, 2, r, nS, mask_r)
+    err = np.abs(sig - ref) / np.abs(ref)
+    print(f"max relative error {err.max():.2e}; kernels {ms:.3f} ms")
+    assert err.max() <= 2e-4
+    np.testing.assert_allclose(rcp, -0.5 / sig, rtol=1e-12)
+    got = ictx.get_particles()[2]
+    want = rcp[grp][:, np.asarray(iSig)].astype(np.float32)
+    np.testing.assert_array_equal(got, want)

@@ -902,3 +902,98 @@ def test_stat_img_after_background_subtraction_is_one():
     imgs = rng.normal(rng.uniform(-3, 3), rng.uniform(0.2, 4), size=(6, 32, 32))
     y, bg = oracle.subtract_bg(imgs, 12.0)
     assert oracle.stat_img(y, bg) == pytest.approx(1.0, abs=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# F4: sigma refresh from the best poses (allReduceSigma)
+# ---------------------------------------------------------------------------
+
+def _sigma_case(P=60, N=32, seed=0):
+    """A blob volume, random poses, shifts and CTF attributes, and the model
+    spectra built from the separately pinned projection, translation and CTF
+    functions (not from refresh_sigma's own formula)."""
+    rng = np.random.default_rng(seed)
+    pf, V = 2, 2 * N
+    ix, iy, iz = np.meshgrid(np.arange(V // 2 + 1), np.arange(V), np.arange(V), indexing="ij")
+    ix, iy, iz = ix.ravel(), np.where(iy.ravel() < V // 2, iy.ravel(), iy.ravel() - V), \
+        np.where(iz.ravel() < V // 2, iz.ravel(), iz.ravel() - V)
+    cen = rng.uniform(-6, 6, size=(12, 3))
+    ft = oracle.blob_ft(V, N, cen, np.ones(12), 1.5, ix, iy, iz)
+    vol = np.zeros((V, V, V // 2 + 1, 2), np.float32)
+    vol[iz % V, iy % V, ix] = ft
+    quat = rng.normal(size=(P, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    trans = rng.uniform(-2, 2, size=(P, 2))
+    attr = np.stack([np.full(P, 300e3), rng.uniform(1e4, 3e4, P), rng.uniform(1e4, 3e4, P),
+                     rng.uniform(0, 3, P), np.full(P, 2.7e7), np.full(P, 0.07), np.zeros(P)], 1)
+    return rng, vol, V, pf, quat, trans, attr
+
+
+def _model_spectra(vol, V, pf, N, quat, trans, attr, r, px):
+    i, j, row = oracle.spectrum_pixels(N)
+    inr = i * i + j * j < r * r
+    out = np.zeros((len(quat), N, N // 2 + 1), np.complex128)
+    for l in range(len(quat)):
+        a = attr[l].astype(np.float32)
+        c = oracle.ctf(i[inr], j[inr], N, px, *a)
+        s = oracle.translate(trans[l, 0], trans[l, 1], N, i[inr], j[inr])
+        p = oracle.project(vol, V, oracle.rotmat(quat[l]), i[inr], j[inr], pf, f64=True)
+        m = c * (s[:, 0] + 1j * s[:, 1]) * (p[:, 0] + 1j * p[:, 1])
+        out[l][row[inr], i[inr]] = m
+    return out
+
+
+def test_refresh_sigma_noise_free_model_gives_zero():
+    """Data equal to the model (CTF . shift . slice built from the pinned
+    pieces): every residual is zero, the model-to-data power ratio is 1, so
+    sigma is 0 in every shell inside the projection radius."""
+    N, r, px = 32, 12, 1.5
+    rng, vol, V, pf, quat, trans, attr = _sigma_case(P=8)
+    spec = _model_spectra(vol, V, pf, N, quat, trans, attr, r, px)
+    noisy = spec + rng.normal(size=spec.shape) + 1j * rng.normal(size=spec.shape)
+    sig, _ = oracle.refresh_sigma(vol, V, pf, spec, noisy, attr, px, np.zeros(8), quat, trans,
+                                  None, None, 1, r, r, 40.0)
+    assert np.abs(sig).max() <= 1e-12 * np.abs(spec).max() ** 2
+
+
+def test_refresh_sigma_without_model_is_alpha_times_half_power():
+    """A zero volume: the model is 0, the ratio 0, and sigma = alpha sigN with
+    sigN = |c|^2 / 2 for constant data c, alpha = sqrt(pi) maskRadius /
+    (N pixelSize) = sqrt(pi) 5 / 64."""
+    N, px = 32, 2.0
+    _, vol, V, pf, quat, trans, attr = _sigma_case(P=4)
+    c = 3.0 - 4.0j
+    spec = np.full((4, N, N // 2 + 1), c)
+    sig, rcp = oracle.refresh_sigma(np.zeros_like(vol), V, pf, spec, spec, attr, px, np.zeros(4),
+                                    quat, trans, None, None, 1, 10, 15, 5.0)
+    alpha = np.sqrt(np.pi) * 5.0 / (N * px)
+    np.testing.assert_allclose(sig, alpha * 25.0 / 2, rtol=1e-12)
+    np.testing.assert_allclose(rcp, -0.5 / (alpha * 12.5), rtol=1e-12)
+
+
+def test_refresh_sigma_white_noise_statistics():
+    """Model plus complex white noise of per-component deviation s: the
+    residual at the true pose is the noise, so sigM = sigN = s^2 per shell;
+    with alpha = 1 (maskRadius = N pixelSize / sqrt(pi)) sigma = s^2 whatever
+    the ratio.  Each shell within 6 standard errors, and two groups with
+    different noise levels stay apart."""
+    N, r, px, s = 32, 12, 1.5, 0.7
+    P = 60
+    rng, vol, V, pf, quat, trans, attr = _sigma_case(P=P, seed=3)
+    spec = _model_spectra(vol, V, pf, N, quat, trans, attr, r, px)
+    grp = (np.arange(P) % 2).astype(np.int32)
+    sd = np.where(grp == 0, s, 2 * s)[:, None, None]
+    data = spec + sd * (rng.normal(size=spec.shape) + 1j * rng.normal(size=spec.shape))
+    n_shell = N // 2 - 1
+    sig, _ = oracle.refresh_sigma(vol, V, pf, data, data, attr, px, np.zeros(P), quat, trans,
+                                  np.zeros((P, 2)), grp, 2, r, n_shell, N * px / np.sqrt(np.pi))
+    i, j, _ = oracle.spectrum_pixels(N)
+    u = np.rint(np.hypot(i, j)).astype(int)
+    cnt = np.bincount(u[u < n_shell], minlength=n_shell)
+    for g, sg in ((0, s), (1, 2 * s)):
+        se = sg ** 2 / np.sqrt(cnt * P / 2)
+        assert np.all(np.abs(sig[g] - sg ** 2) <= 6 * se), (g, sig[g] / sg ** 2)
+    # a pose with the wrong shift sign leaves signal in the low shells
+    bad, _ = oracle.refresh_sigma(vol, V, pf, data, data, attr, px, np.zeros(P), quat, -trans,
+                                  None, grp, 2, r, n_shell, N * px / np.sqrt(np.pi))
+    assert np.all(bad[:, 2:6] > sig[:, 2:6] * 1.5)

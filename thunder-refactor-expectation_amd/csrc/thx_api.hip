@@ -288,7 +288,7 @@ int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used) {
 // Particle input arrays for P particles (and the derived operands' rows).
 int alloc_particles(thx_ctx* c, int P) {
     c->partFromImages = false;   // set again by the images path once it has packed them
-    c->ctfReady = c->sigReady = false;
+    c->ctfReady = c->sigReady = c->ctfAttrOk = false;
     c->P = P;
     c->Ppad = round_up(P, 128);
     THX_TRY(c->dat.ensure(sizeof(float2) * (size_t)P * c->K));
@@ -491,15 +491,15 @@ int thx_ctx_set_ctf(thx_ctx* c, const float* ctfAttr, int P, float pixelSize) {
                       ctfAttr[7 * (size_t)p + 5] <= 1.f,
                   THX_ERR_ARG, "set_ctf: voltage must be > 0 and amplitude contrast in [0, 1]");
     THX_TRY(set_device(c));
-    DevBuf dA;
-    THX_TRY(upload(dA, ctfAttr, 7 * (size_t)P, c->stream));
-    int s = launch_ctf_attr(c, dA.as<const float>(), P, pixelSize);
+    // the attributes stay on the device: the sigma refresh evaluates the CTF
+    // over the whole spectrum from them
+    THX_TRY(upload(c->ctfAttr, ctfAttr, 7 * (size_t)P, c->stream));
+    int s = launch_ctf_attr(c, c->ctfAttr.as<const float>(), P, pixelSize);
     if (s == THX_OK && hipStreamSynchronize(c->stream) != hipSuccess) {
         set_error("set_ctf: kernel failed");
         s = THX_ERR_HIP;
     }
-    dA.release();
-    if (s == THX_OK) c->ctfReady = true;
+    if (s == THX_OK) c->ctfReady = c->ctfAttrOk = true;
     return s;
 }
 

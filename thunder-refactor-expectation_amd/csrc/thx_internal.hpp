@@ -148,6 +148,8 @@ struct thx_ctx {
     bool imgReady = false;            // imgFT holds the spectra of the last images
     bool partFromImages = false;      // the particle arrays were packed from those images
     bool ctfReady = false, sigReady = false;   // the particles' ctf / sigRcp arrays are set
+    thx::DevBuf ctfAttr;              // float [P][7]: CTF attributes of the last thx_ctx_set_ctf
+    bool ctfAttrOk = false;           // ctfAttr belongs to the current particles
     thx::DevBuf sigPart, sigShell;    // initial sigma: per-pixel partial sums, shell results
     int pixMaxCol = 0, pixMaxAbsRow = 0;   // extent of the pixel set
     // particles (image-major, as uploaded) and derived operands

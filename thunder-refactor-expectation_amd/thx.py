@@ -40,6 +40,8 @@ _SIGS = {
     "thx_ctx_set_particle_images": [_p, _p, _i, _i, C.c_float, _i, _p, _p, _p],
     "thx_ctx_init_sigma": [_p, _i, _i, _p, _p, _p],
     "thx_ctx_get_image_stats": [_p, _p, _p],
+    "thx_ctx_refresh_sigma": [_p, _p, _p, _p, _p, _p, _i, _i, _i, C.c_float, C.c_float, _i, _p,
+                              _p, _p],
     "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
     "thx_ctx_set_ctf": [_p, _p, _i, C.c_float],
     "thx_ctx_get_particles": [_p, _p, _p, _p],
@@ -325,6 +327,24 @@ class Context:
               "set_particle_images")
         self.P = P
         return ms
+
+    def refresh_sigma(self, quat, trans, r, n_shell, mask_radius, pixel_size, cls=None,
+                      offset=None, group=None, n_group=1, apply=False):
+        """sigma^2 and sigRcp [n_group][n_shell] refreshed from the particles'
+        best poses (Optimiser::allReduceSigma); returns (sig, sigRcp, ms)."""
+        P = self.P
+        quat = c32(quat, np.float64).reshape(P, 4)
+        trans = c32(trans, np.float64).reshape(P, 2)
+        arr = lambda a, dt, n: None if a is None else c32(a, dt).reshape(P, n) if n > 1 else c32(a, dt)
+        sig = np.zeros((n_group, n_shell))
+        rcp = np.zeros((n_group, n_shell))
+        ms = np.zeros(1, np.float32)
+        check(load().thx_ctx_refresh_sigma(self._h, ptr(arr(cls, np.int32, 1)), ptr(quat),
+                                            ptr(trans), ptr(arr(offset, np.float64, 2)),
+                                            ptr(arr(group, np.int32, 1)), n_group, r, n_shell,
+                                            float(mask_radius), float(pixel_size), int(bool(apply)),
+                                            ptr(sig), ptr(rcp), ptr(ms)), "refresh_sigma")
+        return sig, rcp, float(ms[0])
 
     def image_stats(self):
         """(stdN, scale) of the last set_particle_images call: statImg's mean
