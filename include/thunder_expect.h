@@ -566,6 +566,35 @@ int thx_ctx_synth_particles(thx_ctx* ctx, int cls, int P, double maxShift,
                             double* trueShift, double* ctfParams);
 
 /* ------------------------------------------------------------------------ */
+/* Consumer side: reconstruction insert.                                     */
+/* ------------------------------------------------------------------------ */
+
+/* The insert loop of Optimiser::reconstructRef (src/Optimiser.cpp:7000-7190)
+ * for the context's particles (from thx_ctx_set_particle_images, CTF from
+ * thx_ctx_set_ctf): particle p's m samples s = p m .. p m + m - 1 with class
+ * cls[s] (NULL: 0), quaternion quat[s*4] and shift trans[s*2] (pixels), its
+ * recentring offset offset[p*2] (NULL: 0) and weight w[p] (NULL: 1), each
+ * inserted with weight w / m (the _par[l].rand draws of mReco samples).
+ * Each sample's unmasked data at the pixel set of radius rU
+ * (allocPreCalIdx(rU, 0), allocPreCal(mask = false)), shifted by
+ * -(t - offset) and times the particle's CTF, is added into the class's F
+ * volume and CTF^2 into its T volume by trilinear weights at
+ * R (iCol pf, iRow pf, 0) (Reconstructor::insertP, src/Reconstructor.cpp:
+ * 782-860; Volume::addFT, src/Image/Volume.cpp:334-368, 555-700), and
+ * -R (t - offset, 0) is summed per class (insertDir).  Accumulators: FP64 on
+ * the (V/2+1)*V*V half grid of V = pf N per class 0 .. nCls-1, zeroed when
+ * clear != 0 (or the size changed), else added to.  msOut (optional): kernel
+ * time in ms. */
+int thx_ctx_insert(thx_ctx* ctx, const int* cls, const double* quat, const double* trans,
+                   const double* offset, int m, const double* w, int rU, int nCls, int clear,
+                   float pixelSize, float* msOut);
+/* Accumulated F [(V/2+1)*V*V complex, x fastest] and T [(V/2+1)*V*V] of class
+ * cls (FP64), the insertDir sum dir[3] and the number of samples inserted
+ * (any output may be NULL). */
+int thx_ctx_get_reconstruction(thx_ctx* ctx, int cls, double* F, double* T, double* dir,
+                               double* count);
+
+/* ------------------------------------------------------------------------ */
 /* On-disk formats (host only; no device work except thx_ctx_load_thu).      */
 /* ------------------------------------------------------------------------ */
 

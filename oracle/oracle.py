@@ -49,6 +49,7 @@ def load():
         "thxo_post_scan": ([_i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _d, _d, _d, _d,
                             _d, _d, _d, _d, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p], _i),
         "thxo_converge": ([_p, _p, _p, _i, _i, _i], _i),
+        "thxo_insert": ([_p, _p, _i, _i, _p, _p, _p, _p, _p, _i, _d], None),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -512,3 +513,55 @@ def refresh_sigma(vol, V, pf, specM, specN, ctf_attr, pixel_size, cls, quat, tra
     sig = ratio * sM + (1 - ratio) * alpha * sN
     with np.errstate(divide="ignore"):
         return sig, -0.5 / sig
+
+
+def insert(F, T, V, pf, src, ctf_, M, iCol, iRow, w):
+    """Reconstructor::insertP (src/Reconstructor.cpp:782-860), trilinear, T
+    during the insert, FP64 accumulators F [V][V][V/2+1] complex128 and T
+    [V][V][V/2+1] float64, updated in place.  src complex [K], ctf [K], M the
+    row-major rotation matrix."""
+    assert F.dtype == np.complex128 and T.dtype == np.float64
+    assert F.flags.c_contiguous and T.flags.c_contiguous
+    s = np.ascontiguousarray(np.stack([np.real(src), np.imag(src)], -1), np.float32)
+    c = _c(ctf_, np.float32)
+    iCol, iRow = _c(iCol, np.int32), _c(iRow, np.int32)
+    load().thxo_insert(_ptr(F), _ptr(T), V, pf, _ptr(s), _ptr(c),
+                       _ptr(_c(M, np.float64).reshape(9)), _ptr(iCol), _ptr(iRow), iCol.size, w)
+
+
+def reconstruct_insert(specN, ctf_attr, pixel_size, cls, quat, trans, offset, w, n_cls, V, pf,
+                       rU):
+    """The insert loop of Optimiser::reconstructRef (src/Optimiser.cpp:
+    7000-7190, 3-D, no CTF search): for each particle and each of its samples
+    (quat, trans of shape [P][m][.]), the unmasked packed data at the pixel set
+    of radius rU (allocPreCalIdx(rU, 0), allocPreCal(mask = false)) shifted by
+    -(t - offset) (translate, src/Image/ImageFunctions.cpp:233-251), its CTF
+    from the attributes, inserted with weight w (a number, or one per
+    particle) into its class's F and T.
+    Returns (F [n_cls][V][V][V/2+1] complex, T, dir [n_cls][3] = sum of
+    -R (t - offset, 0) (insertDir), counts [n_cls])."""
+    specN = np.asarray(specN)
+    P, N = specN.shape[0], specN.shape[1]
+    iCol, iRow, _ = pixel_set(N, rU)[:3]
+    iCol, iRow = np.asarray(iCol), np.asarray(iRow)
+    rows = np.where(iRow >= 0, iRow, iRow + N)
+    F = np.zeros((n_cls, V, V, V // 2 + 1), np.complex128)
+    T = np.zeros((n_cls, V, V, V // 2 + 1))
+    d = np.zeros((n_cls, 3))
+    n = np.zeros(n_cls, np.int64)
+    quat = np.asarray(quat, np.float64)
+    trans = np.asarray(trans, np.float64)
+    for l in range(P):
+        a = np.asarray(ctf_attr[l], np.float32)
+        c = ctf(iCol, iRow, N, pixel_size, *a)
+        x = specN[l][rows, iCol]
+        for m in range(quat.shape[1]):
+            k = int(cls[l][m]) if np.ndim(cls) == 2 else int(cls[l])
+            t = trans[l, m] - (np.asarray(offset[l], np.float64) if offset is not None else 0.0)
+            s = translate(-t[0], -t[1], N, iCol, iRow)
+            src = x * (s[:, 0] + 1j * s[:, 1])
+            R = rotmat(quat[l, m])
+            insert(F[k], T[k], V, pf, src, c, R, iCol, iRow, float(w[l]) if np.ndim(w) else w)
+            d[k] += -(R @ np.array([t[0], t[1], 0.0]))
+            n[k] += 1
+    return F, T, d, n

@@ -940,3 +940,48 @@ int thxo_converge(const double* k, const double* s, const double* sD, int n, int
     }
     return -1;
 }
+
+/*
+ * F2: Reconstructor::insertP(const Complex* src, const RFLOAT* ctf,
+ * const dmat33& rot, RFLOAT w, ...) (src/Reconstructor.cpp:782-860) with the
+ * trilinear kernel and T added during the insert (RECONSTRUCTOR_TRILINEAR_KERNEL,
+ * RECONSTRUCTOR_ADD_T_DURING_INSERT, include/Config.h:97-99), in FP64: for
+ * each pixel k at padded coordinates (iCol pf, iRow pf, 0) rotated by M, the
+ * value src_k ctf_k w goes to F and ctf_k^2 w to T through
+ * Volume::addFT(value, x, y, z) (src/Image/Volume.cpp:334-368): mirrored and
+ * conjugated (F only) when x < 0 (conjHalf), x0 = floor, weights
+ * w[k][j][i] of WG_TRI_INTERP_LINEAR, corners wrapped in y and z
+ * (addFTHalf, :555-700).  F[(z V + y)(V/2+1) + x] complex (re, im), T real.
+ */
+void thxo_insert(double* F, double* T, int V, int pf, const float* src, const float* ctf,
+                 const double* M, const int* iCol, const int* iRow, int K, double w)
+{
+    const int nColFT = V / 2 + 1;
+    for (int k = 0; k < K; k++) {
+        const double c0 = (double)(iCol[k] * pf), c1 = (double)(iRow[k] * pf);
+        double x = M[0] * c0 + M[1] * c1;
+        double y = M[3] * c0 + M[4] * c1;
+        double z = M[6] * c0 + M[7] * c1;
+        double re = (double)src[2 * k] * ctf[k] * w, im = (double)src[2 * k + 1] * ctf[k] * w;
+        const double t = (double)ctf[k] * ctf[k] * w;
+        if (x < 0) { x = -x; y = -y; z = -z; im = -im; }
+        const int x0 = (int)floor(x), y0 = (int)floor(y), z0 = (int)floor(z);
+        const double xd = x - x0, yd = y - y0, zd = z - z0;
+        const double wx[2] = {1 - xd, xd}, wy[2] = {1 - yd, yd}, wz[2] = {1 - zd, zd};
+        for (int a = 0; a < 2; a++) {
+            int zz = z0 + a;
+            zz = zz < 0 ? zz + V : zz;
+            for (int b = 0; b < 2; b++) {
+                int yy = y0 + b;
+                yy = yy < 0 ? yy + V : yy;
+                for (int c = 0; c < 2; c++) {
+                    const double ww = wx[c] * wy[b] * wz[a];
+                    const size_t idx = ((size_t)zz * V + yy) * nColFT + x0 + c;
+                    F[2 * idx] += re * ww;
+                    F[2 * idx + 1] += im * ww;
+                    T[idx] += t * ww;
+                }
+            }
+        }
+    }
+}

@@ -113,6 +113,10 @@ int thxo_post_scan(int nCls, const double* wC, const double* base, const double*
                    double* wRn, int* idxT, double* t, double* wTn, double* k, double* s,
                    double* peakR);
 
+/* F2: reconstruction insert of one image (trilinear, FP64 accumulators). */
+void thxo_insert(double* F, double* T, int V, int pf, const float* src, const float* ctf,
+                 const double* M, const int* iCol, const int* iRow, int K, double w);
+
 /* a13: phase at which a particle's local search stops (-1: not within n). */
 int thxo_converge(const double* k, const double* s, const double* sD, int n, int phase0,
                   int minPhase);

@@ -351,3 +351,47 @@ def test_refresh_sigma_matches_oracle(ictx):
     got = ictx.get_particles()[2]
     want = rcp[grp][:, np.asarray(iSig)].astype(np.float32)
     np.testing.assert_array_equal(got, want)
+
+
+def test_insert_matches_oracle(ictx):
+    """The reconstruction insert (Optimiser::reconstructRef's loop with
+    Reconstructor::insertP, trilinear) against the FP64 oracle: two classes,
+    two samples per particle, per-particle weights, shifts, recentring offsets
+    and CTF attributes, the unmasked spectra of real-space images.  F and T
+    within 1e-5 of their largest magnitude (the GPU forms each value in FP32
+    as the reference's RFLOAT and accumulates in FP64), the insertDir sums to
+    1e-12, and a second call without clearing doubles them."""
+    N, P, m, rbg, px, rU = 64, 12, 2, 30.0, 1.5, 20
+    V = 2 * N
+    print(f"\n[insert] N={N} P={P} samples={m} rU={rU}", flush=True)
+    iCol, iRow, iSig = wl.pixel_set(N, 24)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    imgs = _images(P, N, 41)
+    ictx.set_particle_images(imgs, rbg, None, None, soft_mask=True)
+    rng = np.random.default_rng(6)
+    attr = np.stack([np.full(P, 300e3), rng.uniform(1e4, 3e4, P), rng.uniform(1e4, 3e4, P),
+                     rng.uniform(0, 3, P), np.full(P, 2.7e7), np.full(P, 0.07),
+                     np.zeros(P)], 1).astype(np.float32)
+    ictx.set_ctf(attr, px)
+    quat = rng.normal(size=(P, m, 4))
+    quat /= np.linalg.norm(quat, axis=2, keepdims=True)
+    trans = rng.uniform(-3, 3, size=(P, m, 2))
+    offset = rng.uniform(-1, 1, size=(P, 2))
+    cls = rng.integers(0, 2, size=(P, m)).astype(np.int32)
+    w = rng.uniform(0.5, 1.0, P)
+    ms = ictx.insert(quat, trans, rU, px, cls=cls, offset=offset, w=w, n_cls=2)
+    specN = oracle.image_spectra(imgs, rbg, masked=False)
+    F0, T0, d0, n0 = oracle.reconstruct_insert(specN, attr, px, cls, quat, trans, offset, w / m,
+                                               2, V, 2, rU)
+    for k in range(2):
+        F, T, d, n = ictx.get_reconstruction(k)
+        eF = np.abs(F - F0[k]).max() / np.abs(F0[k]).max()
+        eT = np.abs(T - T0[k]).max() / np.abs(T0[k]).max()
+        print(f"class {k}: F error {eF:.2e}, T error {eT:.2e}, samples {n:.0f}; {ms:.3f} ms")
+        assert eF <= 1e-5 and eT <= 1e-5
+        np.testing.assert_allclose(d, d0[k], rtol=1e-12, atol=1e-12)
+        assert n == n0[k]
+    ictx.insert(quat, trans, rU, px, cls=cls, offset=offset, w=w, n_cls=2, clear=False)
+    F, T, d, n = ictx.get_reconstruction(1)
+    assert np.abs(F - 2 * F0[1]).max() <= 2e-5 * np.abs(F0[1]).max()
+    assert n == 2 * n0[1]

@@ -997,3 +997,64 @@ def test_refresh_sigma_white_noise_statistics():
     bad, _ = oracle.refresh_sigma(vol, V, pf, data, data, attr, px, np.zeros(P), quat, -trans,
                                   None, grp, 2, r, n_shell, N * px / np.sqrt(np.pi))
     assert np.all(bad[:, 2:6] > sig[:, 2:6] * 1.5)
+
+
+# ---------------------------------------------------------------------------
+# F2: reconstruction insert (Reconstructor::insertP, trilinear)
+# ---------------------------------------------------------------------------
+
+def test_insert_is_the_adjoint_of_projection():
+    """Inserting values x_k at the slice points of a rotation is the adjoint of
+    taking the slice (the separately pinned trilinear projection p = P V):
+    Re <F, V> = Re sum_k conj(x_k) p_k for any volume V, mirrored points
+    included; and T's entries sum to sum_k ctf_k^2 w (the trilinear weights of
+    a point sum to 1)."""
+    rng = np.random.default_rng(1)
+    N, pf = 32, 2
+    V = N * pf
+    iCol, iRow, _, _ = oracle.pixel_set(N, 14)
+    K = len(iCol)
+    vol = rng.normal(size=(V, V, V // 2 + 1, 2)).astype(np.float32)
+    volc = vol[..., 0].astype(np.float64) + 1j * vol[..., 1]
+    for trial in range(3):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        M = oracle.rotmat(q)
+        x = (rng.normal(size=K) + 1j * rng.normal(size=K)).astype(np.complex64)
+        ctf = np.ones(K, np.float32)
+        F = np.zeros((V, V, V // 2 + 1), np.complex128)
+        T = np.zeros((V, V, V // 2 + 1))
+        oracle.insert(F, T, V, pf, x, ctf, M, iCol, iRow, 1.0)
+        p = oracle.project(vol, V, M, iCol, iRow, pf, f64=True)
+        p = p[:, 0] + 1j * p[:, 1]
+        lhs = np.real(np.sum(np.conj(F) * volc))
+        rhs = np.real(np.sum(np.conj(x.astype(np.complex128)) * p))
+        assert lhs == pytest.approx(rhs, rel=1e-6)
+        c2 = rng.uniform(0.1, 1, K).astype(np.float32)
+        T2 = np.zeros_like(T)
+        oracle.insert(np.zeros_like(F), T2, V, pf, x, c2, M, iCol, iRow, 0.25)
+        assert T2.sum() == pytest.approx(0.25 * np.sum(c2.astype(np.float64) ** 2), rel=1e-12)
+
+
+def test_insert_on_grid_nodes_and_mirror():
+    """The identity rotation puts pixel (i, j) on the padded grid node
+    (2i, 2j, 0): all of x there; a half turn about z sends it to (-2i, -2j, 0),
+    stored mirrored as conj(x) at (2i, 2j, 0) -- except on the x = 0 plane,
+    where -0 is not negative and the point stays at (0, -2j, 0) as x."""
+    N, pf = 16, 2
+    V = N * pf
+    iCol, iRow = np.array([3, 0, 5]), np.array([-2, 4, 1])
+    x = np.array([1 + 2j, -3 + 0.5j, 0.25 - 1j], np.complex64)
+    turn = np.diag([-1.0, -1.0, 1.0])
+    for M in (np.eye(3), turn):
+        F = np.zeros((V, V, V // 2 + 1), np.complex128)
+        T = np.zeros((V, V, V // 2 + 1))
+        oracle.insert(F, T, V, pf, x, np.full(3, 2.0, np.float32), M, iCol, iRow, 1.0)
+        for k in range(3):
+            if M is turn and iCol[k] == 0:
+                row, want = (-2 * iRow[k]) % V, x[k]
+            else:
+                row, want = (2 * iRow[k]) % V, x[k] if M is not turn else np.conj(x[k])
+            assert F[0, row, 2 * iCol[k]] == pytest.approx(2 * want)
+            assert T[0, row, 2 * iCol[k]] == pytest.approx(4.0)
+        assert np.count_nonzero(F) == 3

@@ -149,6 +149,11 @@ struct thx_ctx {
     bool partFromImages = false;      // the particle arrays were packed from those images
     bool ctfReady = false, sigReady = false;   // the particles' ctf / sigRcp arrays are set
     thx::DevBuf ctfAttr;              // float [P][7]: CTF attributes of the last thx_ctx_set_ctf
+    // reconstruction accumulators per class: F (double2) and T (double) on the
+    // (V/2+1) V V half grid, and per class the insertDir sums and sample count
+    std::vector<thx::DevBuf> recoF, recoT;
+    std::vector<double> recoDir;
+    int recoV = 0, recoCls = 0;
     bool ctfAttrOk = false;           // ctfAttr belongs to the current particles
     thx::DevBuf sigPart, sigShell;    // initial sigma: per-pixel partial sums, shell results
     int pixMaxCol = 0, pixMaxAbsRow = 0;   // extent of the pixel set

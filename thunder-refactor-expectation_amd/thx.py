@@ -40,6 +40,8 @@ _SIGS = {
     "thx_ctx_set_particle_images": [_p, _p, _i, _i, C.c_float, _i, _p, _p, _p],
     "thx_ctx_init_sigma": [_p, _i, _i, _p, _p, _p],
     "thx_ctx_get_image_stats": [_p, _p, _p],
+    "thx_ctx_insert": [_p, _p, _p, _p, _p, _i, _p, _i, _i, _i, C.c_float, _p],
+    "thx_ctx_get_reconstruction": [_p, _i, _p, _p, _p, _p],
     "thx_ctx_refresh_sigma": [_p, _p, _p, _p, _p, _p, _i, _i, _i, C.c_float, C.c_float, _i, _p,
                               _p, _p],
     "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
@@ -345,6 +347,35 @@ class Context:
                                             float(mask_radius), float(pixel_size), int(bool(apply)),
                                             ptr(sig), ptr(rcp), ptr(ms)), "refresh_sigma")
         return sig, rcp, float(ms[0])
+
+    def insert(self, quat, trans, r_u, pixel_size, cls=None, offset=None, w=None, n_cls=1,
+               clear=True):
+        """Reconstruction insert of every particle's m samples (quat [P][m][4],
+        trans [P][m][2], cls [P][m]); returns the kernel time in ms."""
+        P = self.P
+        quat = c32(quat, np.float64).reshape(P, -1, 4)
+        m = quat.shape[1]
+        trans = c32(trans, np.float64).reshape(P, m, 2)
+        cl = None if cls is None else c32(np.broadcast_to(np.asarray(cls).reshape(P, -1), (P, m)),
+                                          np.int32)
+        off = None if offset is None else c32(offset, np.float64).reshape(P, 2)
+        ww = None if w is None else c32(w, np.float64)
+        ms = np.zeros(1, np.float32)
+        check(load().thx_ctx_insert(self._h, ptr(cl), ptr(quat), ptr(trans), ptr(off), m, ptr(ww),
+                                    int(r_u), int(n_cls), int(bool(clear)), float(pixel_size),
+                                    ptr(ms)), "insert")
+        return float(ms[0])
+
+    def get_reconstruction(self, cls=0):
+        """(F complex [V][V][V/2+1], T [V][V][V/2+1], dir[3], count)."""
+        V = self.idim * self.pf
+        F = np.zeros((V, V, V // 2 + 1), np.complex128)
+        T = np.zeros((V, V, V // 2 + 1))
+        d = np.zeros(3)
+        n = np.zeros(1)
+        check(load().thx_ctx_get_reconstruction(self._h, cls, ptr(F), ptr(T), ptr(d), ptr(n)),
+              "get_reconstruction")
+        return F, T, d, float(n[0])
 
     def image_stats(self):
         """(stdN, scale) of the last set_particle_images call: statImg's mean
