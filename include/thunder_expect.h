@@ -221,6 +221,46 @@ int thx_ctx_refresh_sigma(thx_ctx* ctx, const int* cls, const double* quat, cons
                           float maskRadius, float pixelSize, int apply, double* sigOut,
                           double* sigRcpOut, float* msOut);
 
+/* The intensity scale per group from the best poses (Optimiser::refreshScale,
+ * src/Optimiser.cpp:5715-6023, and correctScale, :5069-5109; 3-D; the
+ * unmasked images; one ratio of sums per group).  With coord != 0 particle
+ * p's model is class cls[p]'s slice at quat[p*4], shifted by trans[p*2] -
+ * offset[p*2] (offset NULL: 0); with coord == 0 (the first iteration) it is
+ * class 0's unshifted slice at quat[p*4] (NULL: identity) and only the zero
+ * frequency counts (rS = 1, rL = 0), as the reference's
+ * OPTIMISER_REFRESH_SCALE_ZERO_FREQ_NO_COORD sets it.  Over the stored pixels
+ * with rL^2 <= i^2 + j^2 < rS^2 and rL <= rint|k| < rS (rS clamped to r, the
+ * projector's radius <= N/2 - 1), with the plain complex products the
+ * reference writes (scaleDataVSPrior, :10021-10063):
+ *   sXA = sum Re(X M) c,  sAA = sum Re(M M) c^2,
+ * X the unmasked data, M the model slice, c the CTF from the attributes.
+ * scale[g] = sXA / sAA over group g's particles (group NULL: over all
+ * particles, every group the same); then, with med the median of the nGroup
+ * scales, a scale above |5 med| in magnitude becomes 5 med and one below
+ * |med / 5| becomes med / 5.  A negative mean is THX_ERR_RANGE.  apply: 0
+ * nothing; 1 (correctScale, init = false) divides every particle's spectra
+ * and packed data by scale[group[p]] and multiplies its sigRcpP by the
+ * square (sigma^2 / scale^2); 2 (init = true) multiplies every class volume
+ * by scale[group[0]].  scaleOut [nGroup] (optional); msOut: kernel time. */
+int thx_ctx_refresh_scale(thx_ctx* ctx, int coord, const int* cls, const double* quat,
+                          const double* trans, const double* offset, const int* group,
+                          int nGroup, int r, int rS, float rL, float pixelSize, int apply,
+                          double* scaleOut, float* msOut);
+
+/* Noise normalisation from the best poses (Optimiser::normCorrection,
+ * src/Optimiser.cpp:6161-6351; 3-D, no CTF search, the masked images
+ * (OPTIMISER_NORM_MASK) with the translation trans[p*2] not recentred):
+ * norm[p] = sum over the stored pixels with rL^2 <= i^2 + j^2 < rNorm^2
+ * (rNorm clamped to r) of |X_masked - c M|^2, M class cls[p]'s slice at
+ * quat[p*4] shifted by trans[p*2], c the CTF; m = the median of the norms
+ * (GSL's 0.5 quantile, rounded to float).  apply != 0 multiplies every
+ * particle's spectra (masked and unmasked) and packed data by
+ * sqrt(m / norm[p]).  normOut [P], medianOut [1] (optional); msOut: kernel
+ * time. */
+int thx_ctx_norm_correction(thx_ctx* ctx, const int* cls, const double* quat,
+                            const double* trans, int r, float rL, float rNorm, float pixelSize,
+                            int apply, double* normOut, double* medianOut, float* msOut);
+
 /* Particle arrays, image-major [P*K] as allocPreCal(pixelMajor = false)
  * packs them (src/Optimiser.cpp:7994-8121).  Replaces ExpectLocalIn +
  * ExpectLocalP (gpu/interface/Interface.h:31-38, 50-62) for all particles at once.

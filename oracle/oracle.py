@@ -565,3 +565,116 @@ def reconstruct_insert(specN, ctf_attr, pixel_size, cls, quat, trans, offset, w,
             d[k] += -(R @ np.array([t[0], t[1], 0.0]))
             n[k] += 1
     return F, T, d, n
+
+
+def gsl_median(x):
+    """median(vec, n) (src/Functions/Functions.cpp:227-233): GSL's quantile
+    0.5 of the sorted values, (1 - d) x[lhs] + d x[lhs + 1] with
+    lhs = floor((n - 1) / 2), d = (n - 1) / 2 - lhs; returned as RFLOAT
+    (float)."""
+    s = np.sort(np.asarray(x, np.float64))
+    idx = 0.5 * (s.size - 1)
+    lhs = int(np.floor(idx))
+    d = idx - lhs
+    v = s[lhs] if lhs == s.size - 1 else (1 - d) * s[lhs] + d * s[lhs + 1]
+    return float(np.float32(v))
+
+
+def _best_model(vols, V, pf, N, i, j, r, cls_l, q, t, a, pixel_size):
+    """CTF and shifted slice of one best pose on the pixels (i, j): the slice
+    P_cls(R k) on |k| < r (zero beyond), times exp(-2 pi i k.t / N) when t is
+    given (Projector::project(Image&, rot, t), src/Projector.cpp:456-464), and
+    the CTF from the attributes a (CTF(Image&, ...), src/CTF.cpp:66-105)."""
+    inr = i * i + j * j < r * r
+    Pk = np.zeros(i.size, np.complex128)
+    pin = project(vols[cls_l], V, rotmat(q), i[inr], j[inr], pf, f64=True)
+    Pk[inr] = pin[..., 0] + 1j * pin[..., 1]
+    if t is not None:
+        Pk *= np.exp(-2j * np.pi * (i * t[0] + j * t[1]) / N)
+    c = ctf(i, j, N, pixel_size, a[0], a[1], a[2], a[3], a[4], a[5], a[6]).astype(np.float64)
+    return Pk, c
+
+
+def refresh_scale(vol, V, pf, specN, ctf_attr, pixel_size, cls, quat, trans, offset, group,
+                  n_group, r, rS, rL, coord=True):
+    """Optimiser::refreshScale(coord, group) (src/Optimiser.cpp:5715-6023) in
+    FP64, 3-D, with the switches of include/Config.h: OPTIMISER_SCALE_MASK
+    off (the unmasked images), OPTIMISER_RECENTRE_IMAGE_EACH_ITERATION on
+    (translation t - offset), OPTIMISER_REFRESH_SCALE_SPECTRUM off (one ratio
+    of sums per group), OPTIMISER_REFRESH_SCALE_ZERO_FREQ_NO_COORD on (without
+    coordinates: class 0, no translation, rS = 1, rL = 0).  rS is clamped to r.
+    scaleDataVSPrior (:10021-10063) over the loop IMAGE_FOR_PIXEL_R_FT(rS + 1)
+    (every stored pixel, both halves of the i = 0 column) with
+    rL^2 <= i^2 + j^2 < rS^2 and rL <= rint|k| < rS adds, with the plain complex
+    products of the code (not X conj(M)),
+        sXA += Re(X . M) c,   sAA += Re(M . M) c^2
+    for the data X, the shifted slice M and the CTF c.  Per group
+    (group None: every particle in one sum, the same scale for all groups)
+    scale = sum sXA / sum sAA; then with med = median(scale) (float), a scale
+    above |5 med| in magnitude becomes 5 med and one below |med / 5| becomes
+    med / 5.  Returns the n_group scales."""
+    specN = np.asarray(specN)
+    P, N = specN.shape[0], specN.shape[1]
+    vols = vol if isinstance(vol, (list, tuple)) else [vol]
+    if not coord:
+        rS, rL = 1, 0.0
+    rS = min(rS, r)
+    i, j, row = spectrum_pixels(N)
+    q2 = (i * i + j * j).astype(np.float64)
+    v = np.rint(np.hypot(i, j))
+    keep = (q2 < float(rS) ** 2) & (q2 >= float(rL) ** 2) & (v < rS) & (v >= rL)
+    i, j, row = i[keep], j[keep], row[keep]
+    xa = np.zeros(n_group)
+    aa = np.zeros(n_group)
+    for l in range(P):
+        a = np.asarray(ctf_attr[l], np.float32)
+        if coord:
+            t = np.asarray(trans[l], np.float64) - (np.asarray(offset[l], np.float64)
+                                                    if offset is not None else 0.0)
+            M, c = _best_model(vols, V, pf, N, i, j, r, int(cls[l]) if cls is not None else 0,
+                               quat[l], t, a, pixel_size)
+        else:
+            M, c = _best_model(vols, V, pf, N, i, j, r, 0, quat[l], None, a, pixel_size)
+        x = specN[l][row, i]
+        g = int(group[l]) if group is not None else 0
+        xa[g] += np.sum(np.real(x * M) * c)
+        aa[g] += np.sum(np.real(M * M) * c * c)
+    if group is None:
+        scale = np.full(n_group, xa[0] / aa[0])
+    else:
+        scale = xa / aa
+    med = gsl_median(scale)
+    out = scale.copy()
+    for g in range(n_group):
+        if abs(scale[g]) > abs(med * 5):
+            out[g] = med * 5
+        elif abs(scale[g]) < abs(med / 5):
+            out[g] = med / 5
+    return out
+
+
+def norm_correction(vol, V, pf, specM, ctf_attr, pixel_size, cls, quat, trans, r, rL, rNorm):
+    """Optimiser::normCorrection (src/Optimiser.cpp:6161-6351) in FP64, 3-D,
+    no CTF search, OPTIMISER_NORM_MASK on (the masked images, translation t
+    without the offset): per particle the power of X_masked - c M over the
+    stored pixels (IMAGE_FOR_EACH_PIXEL_FT) with rL^2 <= i^2 + j^2 < rNorm^2,
+    M the slice P_cls(R k) on |k| < r shifted by t, c the CTF; m = median of
+    the powers (float); every image is then multiplied by sqrt(m / norm).
+    Returns (norm [P], m, factor [P])."""
+    specM = np.asarray(specM)
+    P, N = specM.shape[0], specM.shape[1]
+    vols = vol if isinstance(vol, (list, tuple)) else [vol]
+    rNorm = min(float(rNorm), float(r))
+    i, j, row = spectrum_pixels(N)
+    q2 = (i * i + j * j).astype(np.float64)
+    keep = (q2 >= float(rL) ** 2) & (q2 < rNorm ** 2)
+    i, j, row = i[keep], j[keep], row[keep]
+    norm = np.zeros(P)
+    for l in range(P):
+        a = np.asarray(ctf_attr[l], np.float32)
+        M, c = _best_model(vols, V, pf, N, i, j, r, int(cls[l]) if cls is not None else 0,
+                           quat[l], np.asarray(trans[l], np.float64), a, pixel_size)
+        x = specM[l][row, i]
+        norm[l] = np.sum(np.abs(x - c * M) ** 2)
+    m = gsl_median(norm)
+    return norm, m, np.sqrt(m / norm)

@@ -395,3 +395,134 @@ def test_insert_matches_oracle(ictx):
     F, T, d, n = ictx.get_reconstruction(1)
     assert np.abs(F - 2 * F0[1]).max() <= 2e-5 * np.abs(F0[1]).max()
     assert n == 2 * n0[1]
+
+
+def _model_images(ictx, N, P, seed, noise=0.3, n_cls=2, with_offset=True):
+    """Real-space images whose spectra are each particle's best-pose model
+    (CTF . shift(t - offset) . slice of its class, from the oracle's pinned
+    pieces) plus white noise, centred as a stack holds them; the classes'
+    volumes synthesised in the context.  Returns what the tests need."""
+    w = wl.WORKLOADS["box64"]
+    iCol, iRow, iSig = wl.pixel_set(N, 24)
+    ictx.set_pixels(iCol, iRow, iSig, N, 2)
+    vols = []
+    for c in range(n_cls):
+        ictx.synth_volume(c, N, 2, w["n_blob"], w["blob_sigma"], w["blob_radius"], 60 + c)
+        vols.append(ictx.get_volume(c, 2 * N))
+    rng = np.random.default_rng(seed)
+    px, r = 1.5, N // 2 - 4
+    attr = np.stack([np.full(P, 300e3), rng.uniform(1e4, 3e4, P), rng.uniform(1e4, 3e4, P),
+                     rng.uniform(0, 3, P), np.full(P, 2.7e7), np.full(P, 0.07),
+                     np.where(np.arange(P) % 5 == 0, 0.2, 0.0)], 1).astype(np.float32)
+    quat = rng.normal(size=(P, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    trans = rng.uniform(-3, 3, size=(P, 2))
+    offset = rng.uniform(-1, 1, size=(P, 2)) if with_offset else np.zeros((P, 2))
+    cls = rng.integers(0, n_cls, P).astype(np.int32)
+    i, j, row = oracle.spectrum_pixels(N)
+    inr = i * i + j * j < r * r
+    imgs = np.empty((P, N, N), np.float32)
+    for l in range(P):
+        c = oracle.ctf(i[inr], j[inr], N, px, *attr[l])
+        t = trans[l] - offset[l]
+        s = np.exp(-2j * np.pi * (i[inr] * t[0] + j[inr] * t[1]) / N)
+        p = oracle.project(vols[cls[l]], 2 * N, oracle.rotmat(quat[l]), i[inr], j[inr], 2, f64=True)
+        spec = np.zeros((N, N // 2 + 1), np.complex128)
+        spec[row[inr], i[inr]] = c * s * (p[:, 0] + 1j * p[:, 1])
+        img = np.fft.irfft2(spec, s=(N, N))
+        img = np.fft.fftshift(img)
+        imgs[l] = img + noise * img.std() * rng.normal(size=(N, N))
+    return dict(vols=vols, attr=attr, quat=quat, trans=trans, offset=offset, cls=cls, imgs=imgs,
+                px=px, r=r)
+
+
+def test_refresh_scale_matches_oracle(ictx):
+    """The intensity scale from best poses (Optimiser::refreshScale) against
+    the FP64 oracle: images made from each particle's model plus noise, two
+    classes, three groups, recentring offsets; then correctScale's two
+    applications: the images and sigRcp (data / s, sigRcp x s^2, after which
+    the scale refreshed again is 1 in every group) and the class volumes
+    (x the first particle's group scale)."""
+    N, P, rbg = 64, 45, 30.0
+    print(f"\n[refresh scale] N={N} P={P}", flush=True)
+    d = _model_images(ictx, N, P, 31)
+    ictx.set_particle_images(d["imgs"], rbg, None, None, soft_mask=True)
+    ictx.set_ctf(d["attr"], d["px"])
+    ictx.init_sigma(N // 2 - 1, apply=True)
+    grp = (np.arange(P) % 3).astype(np.int32)
+    r, rS, rL = d["r"], 20, 1.5
+    sc, ms = ictx.refresh_scale(d["quat"], d["trans"], r, rS, rL, d["px"], cls=d["cls"],
+                                offset=d["offset"], group=grp, n_group=3)
+    specN = oracle.image_spectra(d["imgs"], rbg, masked=False)
+    ref = oracle.refresh_scale(d["vols"], 2 * N, 2, specN, d["attr"], d["px"], d["cls"],
+                               d["quat"], d["trans"], d["offset"], grp, 3, r, rS, rL)
+    print(f"scales {sc}, oracle {ref}; {ms:.3f} ms")
+    np.testing.assert_allclose(sc, ref, rtol=1e-5)
+    # without groups, and without coordinates (zero frequency of class 0)
+    sc1, _ = ictx.refresh_scale(d["quat"], d["trans"], r, rS, rL, d["px"], cls=d["cls"],
+                                offset=d["offset"], n_group=3)
+    ref1 = oracle.refresh_scale(d["vols"], 2 * N, 2, specN, d["attr"], d["px"], d["cls"],
+                                d["quat"], d["trans"], d["offset"], None, 3, r, rS, rL)
+    np.testing.assert_allclose(sc1, ref1, rtol=1e-5)
+    sc0, _ = ictx.refresh_scale(d["quat"], None, r, rS, rL, d["px"], group=grp, n_group=3,
+                                coord=False)
+    ref0 = oracle.refresh_scale(d["vols"], 2 * N, 2, specN, d["attr"], d["px"], None, d["quat"],
+                                None, None, grp, 3, r, rS, rL, coord=False)
+    np.testing.assert_allclose(sc0, ref0, rtol=1e-5)
+    # correctScale(init = false)
+    dat0, _, sig0 = ictx.get_particles()
+    sc, _ = ictx.refresh_scale(d["quat"], d["trans"], r, rS, rL, d["px"], cls=d["cls"],
+                               offset=d["offset"], group=grp, n_group=3, apply=1)
+    dat1, _, sig1 = ictx.get_particles()
+    f = (1.0 / sc[grp]).astype(np.float32)[:, None, None]
+    np.testing.assert_allclose(dat1, dat0 * f, rtol=1e-6, atol=1e-6 * np.abs(dat0).max())
+    np.testing.assert_allclose(sig1, sig0 * (sc[grp] ** 2).astype(np.float32)[:, None], rtol=1e-6)
+    again, _ = ictx.refresh_scale(d["quat"], d["trans"], r, rS, rL, d["px"], cls=d["cls"],
+                                  offset=d["offset"], group=grp, n_group=3)
+    np.testing.assert_allclose(again, 1.0, rtol=1e-5)
+    # correctScale(init = true): the volumes
+    v0 = [ictx.get_volume(k, 2 * N) for k in range(2)]
+    sc, _ = ictx.refresh_scale(d["quat"], d["trans"], r, rS, rL, d["px"], cls=d["cls"],
+                               offset=d["offset"], group=grp, n_group=3, apply=2)
+    for k in range(2):
+        np.testing.assert_array_equal(ictx.get_volume(k, 2 * N),
+                                      v0[k] * np.float32(sc[grp[0]]))
+
+
+def test_norm_correction_matches_oracle(ictx):
+    """The noise normalisation from best poses (Optimiser::normCorrection)
+    against the FP64 oracle on the masked spectra: per-particle residual
+    powers within 1e-4, the median (an exact selection on the device) equal
+    to the oracle's median of the GPU's own powers, odd and even particle
+    counts; applied, the packed data are multiplied by sqrt(m / norm) and a
+    second pass sees the scaled spectra."""
+    N, rbg = 64, 30.0
+    for P in (33, 40):
+        print(f"\n[norm correction] N={N} P={P}", flush=True)
+        d = _model_images(ictx, N, P, 50 + P, noise=0.5)
+        ictx.set_particle_images(d["imgs"], rbg, None, None, soft_mask=True)
+        ictx.set_ctf(d["attr"], d["px"])
+        r, rL, rN = d["r"], 2.0, 22.5
+        # normCorrection projects at t, not t - offset: the model images carry
+        # t - offset, so pass that as the pose's translation here
+        t = d["trans"] - d["offset"]
+        norm, m, ms = ictx.norm_correction(d["quat"], t, r, rL, rN, d["px"], cls=d["cls"])
+        specM = oracle.image_spectra(d["imgs"], rbg, masked=True)
+        rn, rm, rf = oracle.norm_correction(d["vols"], 2 * N, 2, specM, d["attr"], d["px"],
+                                            d["cls"], d["quat"], t, r, rL, rN)
+        err = np.abs(norm - rn) / rn
+        print(f"norm max relative error {err.max():.2e}, median {m:.6g} vs {rm:.6g}; "
+              f"{ms:.3f} ms")
+        assert err.max() <= 1e-4
+        assert m == oracle.gsl_median(norm)
+        dat0 = ictx.get_particles()[0]
+        norm, m, _ = ictx.norm_correction(d["quat"], t, r, rL, rN, d["px"], cls=d["cls"],
+                                          apply=True)
+        f = np.sqrt(m / norm).astype(np.float32)
+        dat1 = ictx.get_particles()[0]
+        np.testing.assert_allclose(dat1, dat0 * f[:, None, None], rtol=1e-6,
+                                   atol=1e-6 * np.abs(dat0).max())
+        norm2, _, _ = ictx.norm_correction(d["quat"], t, r, rL, rN, d["px"], cls=d["cls"])
+        rn2, _, _ = oracle.norm_correction(d["vols"], 2 * N, 2, specM * f[:, None, None],
+                                           d["attr"], d["px"], d["cls"], d["quat"], t, r, rL, rN)
+        assert (np.abs(norm2 - rn2) / rn2).max() <= 1e-4

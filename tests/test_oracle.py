@@ -1058,3 +1058,104 @@ def test_insert_on_grid_nodes_and_mirror():
             assert F[0, row, 2 * iCol[k]] == pytest.approx(2 * want)
             assert T[0, row, 2 * iCol[k]] == pytest.approx(4.0)
         assert np.count_nonzero(F) == 3
+
+
+# ---------------------------------------------------------------------------
+# F4: intensity scale and noise normalisation from best poses
+# ---------------------------------------------------------------------------
+
+def test_gsl_median_hand_worked():
+    """GSL's 0.5 quantile of sorted data: the middle value for odd n, the mean
+    of the two middle values for even n, whatever the input order; rounded
+    to float."""
+    assert oracle.gsl_median([5.0, 1.0, 3.0]) == 3.0
+    assert oracle.gsl_median([4.0, 1.0, 3.0, 2.0]) == 2.5
+    assert oracle.gsl_median([7.0]) == 7.0
+    assert oracle.gsl_median([0.1, 0.2]) == float(np.float32(0.15000000000000002))
+
+
+def test_refresh_scale_zero_frequency_hand_worked():
+    """Without coordinates only the zero frequency counts, where every
+    rotation samples the volume's origin voxel: with M = 3 + 4i there,
+    data X = 1 + 2i and the CTF at u = 0 equal to the amplitude contrast A
+    (phase shift 0), the plain products give Re(X M) = 3 - 8 = -5 and
+    Re(M M) = 9 - 16 = -7, so the scale is (-5 A) / (-7 A^2) = 5 / (7 A).
+    With two groups whose data differ by the factor 2 the two scales are
+    5 / (7 A) and 10 / (7 A) (median 7.5 / (7 A), no clamp); without groups
+    both are the pooled ratio."""
+    N, A = 16, 0.5
+    V, pf = 2 * N, 2
+    vol = np.zeros((V, V, V // 2 + 1, 2), np.float32)
+    vol[0, 0, 0] = (3.0, 4.0)
+    rng = np.random.default_rng(1)
+    P = 4
+    spec = rng.normal(size=(P, N, N // 2 + 1)) * 50 + 0j
+    spec[:, 0, 0] = [1 + 2j, 2 + 4j, 1 + 2j, 2 + 4j]
+    q = rng.normal(size=(P, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    attr = np.tile([300e3, 2e4, 1.5e4, 0.3, 2.7e7, A, 0.0], (P, 1))
+    grp = np.array([0, 1, 0, 1])
+    sc = oracle.refresh_scale(vol, V, pf, spec, attr, 1.5, None, q, None, None, grp, 2, 6, 6,
+                              0.0, coord=False)
+    np.testing.assert_allclose(sc, [5 / (7 * A), 10 / (7 * A)], rtol=1e-6)
+    sc1 = oracle.refresh_scale(vol, V, pf, spec, attr, 1.5, None, q, None, None, None, 2, 6, 6,
+                               0.0, coord=False)
+    np.testing.assert_allclose(sc1, [7.5 / (7 * A)] * 2, rtol=1e-6)
+
+
+def test_refresh_scale_of_scaled_model_data():
+    """Data equal to s_g times the model (CTF . shift(t - offset) . slice, from
+    the separately pinned pieces) inside the selected pixels and large noise
+    outside them: each group's scale is s_g exactly, so the pixel selection
+    (rL <= |k|, rint|k| < rS) leaves out every other pixel; the recentring
+    offset is subtracted (data built with t - offset, scored with t and
+    offset); the median clamp moves a group 10x off the median to 5 med."""
+    N, r, px = 32, 12, 1.5
+    rS, rL = 9, 2.5
+    P = 12
+    rng, vol, V, pf, quat, trans, attr = _sigma_case(P=P, seed=5)
+    off = rng.uniform(-1, 1, size=(P, 2))
+    model = _model_spectra(vol, V, pf, N, quat, trans - off, attr, r, px)
+    i, j, row = oracle.spectrum_pixels(N)
+    q2 = i * i + j * j
+    v = np.rint(np.hypot(i, j))
+    sel = (q2 < rS * rS) & (q2 >= rL * rL) & (v < rS) & (v >= rL)
+    grp = np.arange(P) % 3
+    s = np.array([0.8, 1.3, 1.1])
+    data = 1e3 * (rng.normal(size=model.shape) + 1j * rng.normal(size=model.shape))
+    for l in range(P):
+        data[l][row[sel], i[sel]] = s[grp[l]] * model[l][row[sel], i[sel]]
+    sc = oracle.refresh_scale(vol, V, pf, data, attr, px, np.zeros(P), quat, trans, off, grp, 3,
+                              r, rS, rL)
+    np.testing.assert_allclose(sc, s, rtol=1e-6)   # FP32 shift phases in the data
+    s2 = np.array([0.8, 13.0, 1.1])   # median 1.1: 13 > 5.5 becomes 5.5
+    for l in range(P):
+        data[l][row[sel], i[sel]] = s2[grp[l]] * model[l][row[sel], i[sel]]
+    sc = oracle.refresh_scale(vol, V, pf, data, attr, px, np.zeros(P), quat, trans, off, grp, 3,
+                              r, rS, rL)
+    np.testing.assert_allclose(sc, [0.8, 5 * np.float32(1.1), 1.1], rtol=1e-6)
+
+
+def test_norm_correction_hand_worked():
+    """Masked data = CTF . shift(t) . slice plus a single residual a_l at one
+    pixel inside rL <= |k| < rNorm and large values beyond rNorm: norm_l is
+    |a_l|^2 exactly (the residual is X - c M, the pixels outside the range
+    are left out, the shift is t without any offset); for |a| = 1, 2, 3, 4
+    the median is 6.5 and the factors sqrt(6.5 / norm)."""
+    N, r, px = 32, 12, 1.5
+    P = 4
+    rng, vol, V, pf, quat, trans, attr = _sigma_case(P=P, seed=9)
+    model = _model_spectra(vol, V, pf, N, quat, trans, attr, r, px)
+    i, j, row = oracle.spectrum_pixels(N)
+    rL, rNorm = 2.0, 10.5
+    q2 = i * i + j * j
+    data = model.copy()
+    out = (q2 >= rNorm ** 2) | (q2 < rL ** 2)
+    for l in range(P):
+        data[l][row[out], i[out]] += 1e4 * (1 + 1j)
+        data[l][(N - 5) % N, 4] += (l + 1) * np.exp(1j * 0.3 * l)   # (i, j) = (4, -5)
+    norm, m, f = oracle.norm_correction(vol, V, pf, data, attr, px, np.zeros(P), quat, trans, r,
+                                        rL, rNorm)
+    np.testing.assert_allclose(norm, [1.0, 4.0, 9.0, 16.0], rtol=1e-4)
+    assert m == pytest.approx(6.5, rel=1e-4)
+    np.testing.assert_allclose(f, np.sqrt(6.5 / np.array([1.0, 4.0, 9.0, 16.0])), rtol=1e-4)

@@ -16,7 +16,7 @@ ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle_thx.so")
 
 HIP_SOURCES = ["thx_kernels.hip", "thx_pf.hip", "thx_cnn.hip", "thx_filter.hip",
                "thx_projectee.hip", "thx_images.hip", "thx_sigma.hip", "thx_insert.hip",
-               "thx_noalloc.hip",
+               "thx_scale.hip", "thx_noalloc.hip",
                "thx_api.hip"]
 # host C++ built into the same library: the adapter that keeps the reference's
 # Projector / Optimiser::expectation / ExpectLocal* call surfaces, and the

@@ -44,6 +44,10 @@ _SIGS = {
     "thx_ctx_get_reconstruction": [_p, _i, _p, _p, _p, _p],
     "thx_ctx_refresh_sigma": [_p, _p, _p, _p, _p, _p, _i, _i, _i, C.c_float, C.c_float, _i, _p,
                               _p, _p],
+    "thx_ctx_refresh_scale": [_p, _i, _p, _p, _p, _p, _p, _i, _i, _i, C.c_float, C.c_float, _i,
+                              _p, _p],
+    "thx_ctx_norm_correction": [_p, _p, _p, _p, _i, C.c_float, C.c_float, C.c_float, _i, _p, _p,
+                                _p],
     "thx_ctx_set_particles": [_p, _p, _p, _p, _i],
     "thx_ctx_set_ctf": [_p, _p, _i, C.c_float],
     "thx_ctx_get_particles": [_p, _p, _p, _p],
@@ -347,6 +351,39 @@ class Context:
                                             float(mask_radius), float(pixel_size), int(bool(apply)),
                                             ptr(sig), ptr(rcp), ptr(ms)), "refresh_sigma")
         return sig, rcp, float(ms[0])
+
+    def refresh_scale(self, quat, trans, r, rS, rL, pixel_size, cls=None, offset=None,
+                      group=None, n_group=1, coord=True, apply=0):
+        """Intensity scale per group from the best poses (Optimiser::refreshScale,
+        correctScale with apply 1 (images) or 2 (volumes)); returns (scale, ms)."""
+        P = self.P
+        q = None if quat is None else c32(quat, np.float64).reshape(P, 4)
+        t = None if trans is None else c32(trans, np.float64).reshape(P, 2)
+        o = None if offset is None else c32(offset, np.float64).reshape(P, 2)
+        cl = None if cls is None else c32(cls, np.int32)
+        g = None if group is None else c32(group, np.int32)
+        scale = np.zeros(n_group)
+        ms = np.zeros(1, np.float32)
+        check(load().thx_ctx_refresh_scale(self._h, int(bool(coord)), ptr(cl), ptr(q), ptr(t),
+                                            ptr(o), ptr(g), int(n_group), int(r), int(rS),
+                                            float(rL), float(pixel_size), int(apply), ptr(scale),
+                                            ptr(ms)), "refresh_scale")
+        return scale, float(ms[0])
+
+    def norm_correction(self, quat, trans, r, rL, r_norm, pixel_size, cls=None, apply=False):
+        """Noise normalisation from the best poses (Optimiser::normCorrection);
+        returns (norm [P], median, ms)."""
+        P = self.P
+        q = c32(quat, np.float64).reshape(P, 4)
+        t = c32(trans, np.float64).reshape(P, 2)
+        cl = None if cls is None else c32(cls, np.int32)
+        norm = np.zeros(P)
+        med = np.zeros(1)
+        ms = np.zeros(1, np.float32)
+        check(load().thx_ctx_norm_correction(self._h, ptr(cl), ptr(q), ptr(t), int(r), float(rL),
+                                              float(r_norm), float(pixel_size), int(bool(apply)),
+                                              ptr(norm), ptr(med), ptr(ms)), "norm_correction")
+        return norm, float(med[0]), float(ms[0])
 
     def insert(self, quat, trans, r_u, pixel_size, cls=None, offset=None, w=None, n_cls=1,
                clear=True):
