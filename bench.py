@@ -387,6 +387,97 @@ def run_images(name, steps):
                     "ms_per_call_with_upload only"}
 
 
+def _quat_colmajor(q):
+    """Column-major 3x3 rotation matrices of unit quaternions (w, x, y, z)."""
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)
+    return R.reshape(-1, 3, 3).transpose(0, 2, 1).reshape(-1, 9)
+
+
+def run_update(name, steps, m_reco=(1, 10, 100)):
+    """The update that follows an expectation, from each particle's best pose,
+    on the workload's particles: sigma^2 refresh (allReduceSigma), intensity
+    scale (refreshScale), noise normalisation (normCorrection) and the
+    reconstruction insert of m samples per particle (insertP) for the mReco
+    values given.  The images are each particle's model (CTF . shift .
+    slice of a blob map, the slices and the CTF made by the library) plus
+    white noise, so the scales and norms are those of real fits.  Algorithmic
+    counts: the insert adds 3 FP64 values at 8 trilinear corners per (sample,
+    pixel); the refresh kernels read each particle's spectra once (masked and
+    unmasked for sigma)."""
+    w = wl.WORKLOADS[name]
+    N, P, pf = w["box"], w["n_particles"], w["pf"]
+    px, r = 1.5, N // 2 - 1
+    rng = np.random.default_rng(23)
+    attr = np.stack([np.full(P, 300e3), rng.uniform(1e4, 3e4, P), rng.uniform(1e4, 3e4, P),
+                     rng.uniform(0, 3, P), np.full(P, 2.7e7), np.full(P, 0.07), np.zeros(P)], 1)
+    quat = rng.normal(size=(P, 4))
+    quat /= np.linalg.norm(quat, axis=1, keepdims=True)
+    trans = rng.uniform(-3, 3, (P, 2))
+    off = rng.uniform(-1, 1, (P, 2))
+    grp = (np.arange(P) % 50).astype(np.int32)
+    # the stored half-spectrum pixels inside the projector radius
+    jj, ii = np.meshgrid(np.arange(-N // 2, N // 2), np.arange(N // 2 + 1), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    inr = ii * ii + jj * jj < r * r
+    ii, jj = ii[inr].astype(np.int32), jj[inr].astype(np.int32)
+    rows = np.where(jj >= 0, jj, jj + N)
+    uctx = thx.Context(0)
+    uctx.set_pixels(ii, jj, np.zeros_like(ii), N, pf)
+    uctx.synth_volume(0, N, pf, w["n_blob"], w["blob_sigma"], w["blob_radius"], 7)
+    uctx.set_particles(np.zeros((P, ii.size, 2), np.float32), None, None)
+    uctx.set_ctf(attr, px)
+    ctf = uctx.get_particles()[1]
+    mats = _quat_colmajor(quat)
+    imgs = np.empty((P, N, N), np.float32)
+    for b in range(0, P, 1000):
+        sl = uctx.project(0, mats[b:b + 1000], ii, jj, pf)
+        for l in range(b, min(P, b + 1000)):
+            t = trans[l] - off[l]
+            spec = np.zeros((N, N // 2 + 1), np.complex128)
+            spec[rows, ii] = (ctf[l] * np.exp(-2j * np.pi * (ii * t[0] + jj * t[1]) / N)
+                              * (sl[l - b, :, 0] + 1j * sl[l - b, :, 1]))
+            img = np.fft.fftshift(np.fft.irfft2(spec, s=(N, N)))
+            imgs[l] = img + 0.5 * img.std() * rng.normal(size=(N, N))
+    iCol, iRow, iSig = wl.pixel_set(N, w["cutoff"])
+    uctx.set_pixels(iCol, iRow, iSig, N, pf)
+    uctx.set_particle_images(imgs, N / 2 - 8, None, None, soft_mask=True)
+    del imgs
+    uctx.set_ctf(attr, px)
+    uctx.init_sigma(N // 2 - 1, apply=True)
+    out = {"particles": P, "N": N, "groups": 50}
+    t = lambda f: float(np.median([f() for _ in range(max(steps, 3))]))
+    out["refresh_sigma_ms"] = t(lambda: uctx.refresh_sigma(quat, trans, r, r, N / 2 - 8, px,
+                                                           offset=off, group=grp, n_group=50)[2])
+    out["refresh_scale_ms"] = t(lambda: uctx.refresh_scale(quat, trans, r, 3 * N // 8, 1.5, px,
+                                                           offset=off, group=grp,
+                                                           n_group=50)[1])
+    out["norm_correction_ms"] = t(lambda: uctx.norm_correction(quat, trans - off, r, 0.0,
+                                                               3 * N / 8, px)[2])
+    rU = N // 2 - 4
+    Kr = int(sum(1 for j in range(-rU - 1, rU + 1) for i in range(0, rU + 2)
+                 if not (i == 0 and j < 0) and i * i + j * j < rU * rU
+                 and round(np.hypot(i, j)) < rU))
+    ins = {}
+    for m in m_reco:
+        q = quat[:, None, :] + 0.01 * rng.normal(size=(P, m, 4))
+        q /= np.linalg.norm(q, axis=2, keepdims=True)
+        tr = trans[:, None, :] + rng.uniform(-0.5, 0.5, (P, m, 2))
+        ms = float(np.median([uctx.insert(q, tr, rU, px, offset=off, w=np.ones(P))
+                              for _ in range(1 if m >= 100 else max(steps, 3))]))
+        items = float(P) * m * Kr
+        ins[f"m{m}"] = {"ms": ms, "samples": P * m, "sample_pixels": items,
+                        "atomics_per_s": 24 * items / (ms * 1e-3)}
+    uctx.close()
+    out["insert"] = ins
+    out["insert_rU"] = rU
+    out["note"] = ("insert: one thread per (sample, pixel), 24 FP64 device atomics each (3 values "
+                   "at 8 trilinear corners), into (V/2+1) V^2 double2 F and double T per class")
+    return out
+
+
 def run_projectee(name, steps):
     """Projectee preparation (Projector::setProjectee) of a workload-sized map:
     inverse FFT, pad + sinc^2 correction kernel, forward FFT, each timed on the
@@ -476,6 +567,10 @@ def main():
                     help="skip the full-size box-256 FP8 global scan")
     ap.add_argument("--no-box192", action="store_true",
                     help="skip the full-size box-192 learned-proposal workload")
+    ap.add_argument("--no-update", action="store_true",
+                    help="skip the update entry (sigma / scale / norm refresh and insert)")
+    ap.add_argument("--update-only", action="store_true",
+                    help="only the update entry, for profiling")
     ap.add_argument("--images-only", action="store_true",
                     help="only the producer side (particle images and initial sigma), for profiling")
     args = ap.parse_args()
@@ -486,6 +581,9 @@ def main():
     name = args.workload
     if args.images_only:
         print(json.dumps({"images": run_images(name, args.steps)}))
+        return
+    if args.update_only:
+        print(json.dumps({"update": run_update(name, args.steps)}))
         return
     w = wl.WORKLOADS[name]
     ctx = thx.Context(0)
@@ -594,6 +692,10 @@ def main():
     log("particle images (producer side)")
     out["images"] = run_images(name, args.steps)
     log(f"images: {out['images']}")
+    if not args.no_update:
+        log("update from best poses (sigma, scale, norm, insert)")
+        out["update"] = run_update(name, args.steps)
+        log(f"update: {out['update']}")
     out.update({
         "value": r["pose_evals_per_s"], "ms_per_step": r["ms_per_step"], "dtype": dtype,
         "config": {"workload": f"{name}: box {w['box']}, {P} particles x {R} rotations x "

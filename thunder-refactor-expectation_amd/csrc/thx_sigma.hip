@@ -105,32 +105,32 @@ __global__ __launch_bounds__(256) void k_sigma_refresh(
 }
 
 // One block per (shell u, group g): the group's particles in a fixed order
-// (thread t adds particles t, t + 256, ...; then a fixed tree), averaged and
-// combined into sigma^2 and sigRcp.
-__global__ __launch_bounds__(256) void k_sigma_groups(const double* __restrict__ per, int P,
-                                                      int nShell, const int* __restrict__ group,
-                                                      double alpha, double* __restrict__ sig,
+// (order[off[g] .. off[g + 1]), the particles sorted by group; thread t adds
+// every 256th, then a fixed tree), averaged and combined into sigma^2 and
+// sigRcp.
+__global__ __launch_bounds__(256) void k_sigma_groups(const double* __restrict__ per,
+                                                      int nShell, const int* __restrict__ order,
+                                                      const int* __restrict__ off, double alpha,
+                                                      double* __restrict__ sig,
                                                       double* __restrict__ rcp) {
     const int u = blockIdx.x, g = blockIdx.y;
-    __shared__ double red[4][256];
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int p = threadIdx.x; p < P; p += 256) {
-        if ((group ? group[p] : 0) != g) continue;
-        const double* o = per + ((size_t)p * nShell + u) * 3;
+    __shared__ double red[3][256];
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int e = off[g] + threadIdx.x; e < off[g + 1]; e += 256) {
+        const double* o = per + ((size_t)order[e] * nShell + u) * 3;
         s[0] += o[0];
         s[1] += o[1];
         s[2] += o[2];
-        s[3] += 1.0;
     }
-    for (int e = 0; e < 4; e++) red[e][threadIdx.x] = s[e];
+    for (int e = 0; e < 3; e++) red[e][threadIdx.x] = s[e];
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
         if ((int)threadIdx.x < st)
-            for (int e = 0; e < 4; e++) red[e][threadIdx.x] += red[e][threadIdx.x + st];
+            for (int e = 0; e < 3; e++) red[e][threadIdx.x] += red[e][threadIdx.x + st];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const double n = red[3][0];
+        const double n = (double)(off[g + 1] - off[g]);
         const double sM = red[0][0] / n, sN = red[1][0] / n, sv = red[2][0] / n;
         const double ratio = fmin(1.0, sv);
         const double v = ratio * sM + (1.0 - ratio) * alpha * sN;
@@ -182,6 +182,13 @@ int thx_ctx_refresh_sigma(thx_ctx* c, const int* cls, const double* quat, const 
     }
     for (int g = 0; g < nGroup; g++)
         THX_CHECK(gcount[g] > 0, THX_ERR_ARG, "refresh_sigma: a group has no particles");
+    // particles sorted by group (stable), offsets per group
+    std::vector<int> goff(nGroup + 1, 0), order(P);
+    for (int g = 0; g < nGroup; g++) goff[g + 1] = goff[g] + gcount[g];
+    {
+        std::vector<int> fill(goff.begin(), goff.end() - 1);
+        for (int p = 0; p < P; p++) order[fill[group ? group[p] : 0]++] = p;
+    }
     // the stored half-spectrum pixels of shells 0 .. nShell - 1, by shell
     std::vector<int4> pix;
     std::vector<int> off(nShell + 1, 0);
@@ -204,9 +211,9 @@ int thx_ctx_refresh_sigma(thx_ctx* c, const int* cls, const double* quat, const 
         std::sqrt(M_PI * std::pow((double)maskRadius / ((double)N * pixelSize), 2.0));
     THX_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    DevBuf dPix, dOff, dVols, dCls, dQ, dT, dO, dG, dPer, dSig;
+    DevBuf dPix, dOff, dVols, dCls, dQ, dT, dO, dG, dPer, dSig, dOrd, dGOff;
     auto cleanup = [&]() {
-        DevBuf* b[] = {&dPix, &dOff, &dVols, &dCls, &dQ, &dT, &dO, &dG, &dPer, &dSig};
+        DevBuf* b[] = {&dPix, &dOff, &dVols, &dCls, &dQ, &dT, &dO, &dG, &dPer, &dSig, &dOrd, &dGOff};
         for (DevBuf* x : b) x->release();
     };
     int s = THX_OK;
@@ -214,7 +221,8 @@ int thx_ctx_refresh_sigma(thx_ctx* c, const int* cls, const double* quat, const 
         (s = upload(dVols, vp.data(), vp.size(), st)) || (cls && (s = upload(dCls, cls, P, st))) ||
         (s = upload(dQ, quat, 4 * (size_t)P, st)) || (s = upload(dT, trans, 2 * (size_t)P, st)) ||
         (offset && (s = upload(dO, offset, 2 * (size_t)P, st))) ||
-        (group && (s = upload(dG, group, P, st))) ||
+        (group && (s = upload(dG, group, P, st))) || (s = upload(dOrd, order.data(), P, st)) ||
+        (s = upload(dGOff, goff.data(), goff.size(), st)) ||
         (s = dPer.ensure(sizeof(double) * 3 * (size_t)P * nShell)) ||
         (s = dSig.ensure(sizeof(double) * 2 * (size_t)nGroup * nShell))) {
         cleanup();
@@ -235,8 +243,8 @@ int thx_ctx_refresh_sigma(thx_ctx* c, const int* cls, const double* quat, const 
                        dT.as<const double>(), offset ? dO.as<const double>() : nullptr,
                        c->ctfAttr.as<const float>(), pixelSize, r * r, dPer.as<double>());
     hipLaunchKernelGGL(k_sigma_groups, dim3(nShell, nGroup), dim3(256), 0, st,
-                       dPer.as<const double>(), P, nShell, group ? dG.as<const int>() : nullptr,
-                       alpha, sig, rcp);
+                       dPer.as<const double>(), nShell, dOrd.as<const int>(),
+                       dGOff.as<const int>(), alpha, sig, rcp);
     if (apply)
         hipLaunchKernelGGL(k_sigma_apply_groups, dim3(P, (c->K + 255) / 256), dim3(256), 0, st,
                            rcp, nShell, group ? dG.as<const int>() : nullptr,
