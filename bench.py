@@ -510,25 +510,94 @@ def agreement(quat, trans, step, bR, bT, gR, gT):
                 angle_p99_deg=float(np.degrees(np.percentile(ang, 99))))
 
 
-def cpu_baseline(ctx, info, name, P_s=384, R_s=1000):
-    """Reference CPU loop (oracle restatement, OpenMP) on a bounded sample."""
+def cpu_host():
+    """(threads, CPU model, CPUs visible): the threads are the host's CPU share
+    for this GPU (OMP_NUM_THREADS where the host sets it, else every CPU this
+    process may run on)."""
+    seen = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or seen
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, model, seen
+
+
+def _cpu_scan(oracle, vol, V, pf, iCol, iRow, N, quat, trans, dat, ctf, sig, threads, reps):
+    """Median wall time of reps runs of the reference's CPU global scan (the
+    oracle's restatement with SIMD256Float's likelihood sum, OpenMP over
+    rotations, FP32)."""
+    R, T = len(quat), len(trans)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        oracle.global_scan(vol, V, pf, iCol, iRow, N, quat, trans, dat, ctf, sig,
+                           np.full(R, 1.0 / R), np.full(T, 1.0 / T), f64=False, threads=threads,
+                           simd=True)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def cpu_baseline(ctx, info, name, P_s=100, reps=3):
+    """The reference CPU expectation (oracle restatement of doGlobalSearch with
+    the ENABLE_SIMD_256 likelihood, OpenMP) on a P_s-particle subset of the
+    workload against its full pose grid, the rate extrapolated linearly in P
+    (per pose-eval); median of reps runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     w = wl.WORKLOADS[name]
     V = w["box"] * w["pf"]
     vol = ctx.get_volume(0, V)
     dat, ctf, sig = ctx.get_particles()
-    quat, trans = info["quat"][:R_s], info["trans"]
-    T = len(trans)
-    cores = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    oracle.global_scan(vol, V, w["pf"], info["iCol"], info["iRow"], w["box"], quat, trans,
-                       dat[:P_s], ctf[:P_s], sig[:P_s], np.full(R_s, 1.0 / R_s),
-                       np.full(T, 1.0 / T), f64=False, threads=cores)
-    dt = time.perf_counter() - t0
-    return dict(value=P_s * R_s * T / dt, unit="pose-evals/s", cores=cores, kind="port",
-                sample=f"{P_s} particles x {R_s} rotations x {T} translations of {name}, "
-                       f"FP32 reference loop order, {dt:.1f} s")
+    quat, trans = info["quat"], info["trans"]
+    R, T = len(quat), len(trans)
+    threads, model, seen = cpu_host()
+    dt = _cpu_scan(oracle, vol, V, w["pf"], info["iCol"], info["iRow"], w["box"], quat, trans,
+                   dat[:P_s], ctf[:P_s], sig[:P_s], threads, reps)
+    return dict(value=P_s * R * T / dt, unit="pose-evals/s", cores=threads, kind="port",
+                cpu_model=model, cpus_visible=seen,
+                sample=f"{P_s} particles x {R} rotations x {T} translations of {name} (the full "
+                       f"grid), FP32, the reference's SIMD256Float likelihood (AVX2) in its "
+                       f"doGlobalSearch loop order, OpenMP over rotations on {threads} threads; "
+                       f"median of {reps}: {dt:.2f} s")
+
+
+def cpu_vs_gpu_box64(steps, reps=3):
+    """The CPU-runnable workload in full (box 64: 1,000 particles x 1,000
+    rotations x 49 translations, cutoff 24): the reference CPU expectation
+    (as in cpu_baseline) and the GPU's plain FP32 and BF16 scans on the same
+    data."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    name = "box64"
+    w = wl.WORKLOADS[name]
+    c64 = thx.Context(0)
+    info = wl.setup_context(c64, name)
+    P, R, T, K = c64.P, c64.R, c64.T, c64.K
+    gpu = {}
+    for m in ("fp32", "bf16"):
+        r = run_mode(c64, m, steps, 1, P, R, T, K)
+        gpu[m] = {"value": r["pose_evals_per_s"], "ms_per_step": r["ms_per_step"]}
+    V = w["box"] * w["pf"]
+    vol = c64.get_volume(0, V)
+    dat, ctf, sig = c64.get_particles()
+    c64.close()
+    threads, model, seen = cpu_host()
+    dt = _cpu_scan(oracle, vol, V, w["pf"], info["iCol"], info["iRow"], w["box"], info["quat"],
+                   info["trans"], dat, ctf, sig, threads, reps)
+    cpu = P * R * T / dt
+    return {"workload": f"{name}: {P} particles x {R} rotations x {T} translations, K={K}",
+            "cpu": {"value": cpu, "seconds": dt, "cores": threads, "cpu_model": model,
+                    "cpus_visible": seen, "kind": "port (SIMD256Float restatement, FP32)",
+                    "runs": reps},
+            "gpu_fp32": gpu["fp32"], "gpu_bf16": gpu["bf16"],
+            "gpu_fp32_over_cpu": gpu["fp32"]["value"] / cpu,
+            "unit": "pose-evals/s"}
 
 
 def traffic_bytes(name, key, grid=None):
@@ -569,6 +638,8 @@ def main():
                     help="skip the full-size box-192 learned-proposal workload")
     ap.add_argument("--no-update", action="store_true",
                     help="skip the update entry (sigma / scale / norm refresh and insert)")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only the CPU baselines (box-128 subset, box 64 in full with its GPU scans)")
     ap.add_argument("--update-only", action="store_true",
                     help="only the update entry, for profiling")
     ap.add_argument("--images-only", action="store_true",
@@ -589,6 +660,12 @@ def main():
     ctx = thx.Context(0)
     log(f"setting up {name}")
     info = wl.setup_context(ctx, name, n_particles=args.particles or None)
+    if args.cpu_only:
+        res = {"cpu_baseline": cpu_baseline(ctx, info, name)}
+        ctx.close()
+        res["cpu_box64_full"] = cpu_vs_gpu_box64(args.steps)
+        print(json.dumps(res))
+        return
     P, R, T, K = ctx.P, ctx.R, ctx.T, ctx.K
     quat, trans = info["quat"], info["trans"]
     step = wl.sampling_step(quat)
@@ -708,6 +785,10 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(ctx, info, name)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+        log("box 64 in full, CPU and GPU")
+        out["cpu_box64_full"] = cpu_vs_gpu_box64(args.steps)
+        log(f"box 64: {out['cpu_box64_full']}")
     ctx.close()
     print(json.dumps(out))
 

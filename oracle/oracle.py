@@ -34,6 +34,9 @@ def load():
         "thxo_logdatavs_f64": ([_p, _p, _p, _p, _i], _d),
         "thxo_global_scan_f32": ([_p, _i, _i, _p, _p, _i, _i, _p, _i, _p, _i, _p, _p, _p, _i,
                                   _p, _p, _p, _p, _p, _p, _i], _i),
+        "thxo_global_scan_f32_simd256": ([_p, _i, _i, _p, _p, _i, _i, _p, _i, _p, _i, _p, _p,
+                                          _p, _i, _p, _p, _p, _p, _p, _p, _i], _i),
+        "thxo_logdatavs_pm_simd256": ([_p, _p, _p, _p, _i, _i, _p], _i),
         "thxo_global_scan_f64": ([_p, _i, _i, _p, _p, _i, _i, _p, _i, _p, _i, _p, _p, _p, _i,
                                   _p, _p, _p, _p, _p, _p, _p, _i], _i),
         "thxo_resample": ([_p, _p, _i, _i, _d, _p, _p], _i),
@@ -130,6 +133,20 @@ def logdatavs_pm(dat, pri, ctf_, sig):
     return res
 
 
+def logdatavs_pm_simd256(dat, pri, ctf_, sig):
+    """SIMD256Float (src/Optimiser.cpp:8957-9041): pixel-major dat [K][n][2],
+    ctf / sig [K][n], pri [K][2] -> [n]."""
+    dat, pri = _c(dat, np.float32), _c(pri, np.float32)
+    ctf_, sig = _c(ctf_, np.float32), _c(sig, np.float32)
+    K, n = ctf_.shape
+    out = np.zeros(n, np.float32)
+    st = load().thxo_logdatavs_pm_simd256(_ptr(dat), _ptr(pri), _ptr(ctf_), _ptr(sig), n, K,
+                                          _ptr(out))
+    if st != 0:
+        raise RuntimeError("SIMD256Float needs a CPU with AVX2")
+    return out
+
+
 def logdatavs_f64(dat, pri, ctf_, sig):
     dat, pri = _c(dat, np.float32), _c(pri, np.float32)
     ctf_, sig = _c(ctf_, np.float32), _c(sig, np.float32)
@@ -137,8 +154,9 @@ def logdatavs_f64(dat, pri, ctf_, sig):
 
 
 def global_scan(vol, V, pf, iCol, iRow, N, quat, trans, dat, ctf_, sig, pR, pT,
-                f64=True, threads=0, want_dvp=False):
-    """doGlobalSearch hot loop, src/Optimiser.cpp:711-915 (one class)."""
+                f64=True, threads=0, want_dvp=False, simd=False):
+    """doGlobalSearch hot loop, src/Optimiser.cpp:711-915 (one class); with
+    simd (FP32 only) the likelihood sum is SIMD256Float's (AVX2 required)."""
     vol = _c(vol, np.float32)
     iCol, iRow = _c(iCol, np.int32), _c(iRow, np.int32)
     quat = _c(quat, np.float64).reshape(-1, 4)
@@ -159,10 +177,12 @@ def global_scan(vol, V, pf, iCol, iRow, N, quat, trans, dat, ctf_, sig, pR, pT,
     wR = np.zeros((P, R), np.float32)
     wT = np.zeros((P, T), np.float32)
     base = np.zeros(P, np.float32)
-    st = load().thxo_global_scan_f32(_ptr(vol), V, pf, _ptr(iCol), _ptr(iRow), K, N, _ptr(quat),
-                                     R, _ptr(trans), T, _ptr(dat), _ptr(ctf_), _ptr(sig), P,
-                                     _ptr(pR), _ptr(pT), _ptr(wC), _ptr(wR), _ptr(wT),
-                                     _ptr(base), threads)
+    fn = load().thxo_global_scan_f32_simd256 if simd else load().thxo_global_scan_f32
+    st = fn(_ptr(vol), V, pf, _ptr(iCol), _ptr(iRow), K, N, _ptr(quat), R, _ptr(trans), T,
+            _ptr(dat), _ptr(ctf_), _ptr(sig), P, _ptr(pR), _ptr(pT), _ptr(wC), _ptr(wR), _ptr(wT),
+            _ptr(base), threads)
+    if st == -2:
+        raise RuntimeError("the SIMD256 scan needs a CPU with AVX2")
     assert st == 0
     return wC, wR, wT, base
 

@@ -15,6 +15,7 @@
 #include "thx_oracle.h"
 
 #include <float.h>
+#include <immintrin.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -205,6 +206,10 @@ void thxo_project_f64(double* dst, const float* vol, int V, const double* M,
  * SIMD256Float :8957-9041): pixel-major arrays idx = i * n + j;
  * result[j] += sigRcp * |dat - ctf * pri[i]|^2.
  */
+/* no contraction into FMA in the two forms of this sum, so that they agree bit
+ * for bit */
+#pragma GCC push_options
+#pragma GCC optimize("fp-contract=off")
 void thxo_logdatavs_pm(const float* dat, const float* pri, const float* ctf,
                        const float* sigRcp, int n, int m, float* result)
 {
@@ -217,6 +222,62 @@ void thxo_logdatavs_pm(const float* dat, const float* pri, const float* ctf,
             result[j] += t * sigRcp[idx];
         }
 }
+
+/*
+ * SIMD256Float (src/Optimiser.cpp:8957-9041), the AVX form of the same sum
+ * the reference's CPU expectation runs with ENABLE_SIMD_256: per pixel i the
+ * images j are taken 8 at a time, with the same operations in the same order
+ * per image (ctf * pri, dat - that, squares, their sum, times sigRcp, added to
+ * result[j]) and the same scalar tail, so every result[j] equals
+ * thxo_logdatavs_pm's bit for bit (neither is contracted into FMA).
+ * Returns 0, or -1 when the CPU has no AVX2 (nothing done).
+ */
+__attribute__((target("avx2"))) static void logdatavs_pm_avx256(
+    const float* dat, const float* pri, const float* ctf, const float* sigRcp, int n, int m,
+    float* result)
+{
+    for (int i = 0; i < m; i++) {
+        const __m256 pr = _mm256_set1_ps(pri[2 * i]), pim = _mm256_set1_ps(pri[2 * i + 1]);
+        int j = 0;
+        for (; j <= n - 8; j += 8) {
+            const size_t idx = (size_t)i * n + j;
+            const __m256 c = _mm256_loadu_ps(ctf + idx);
+            /* (re, im) pairs of 8 images -> 8 reals and 8 imaginaries */
+            const __m256 a = _mm256_loadu_ps(dat + 2 * idx);
+            const __m256 b = _mm256_loadu_ps(dat + 2 * idx + 8);
+            __m256 re = _mm256_shuffle_ps(a, b, _MM_SHUFFLE(2, 0, 2, 0));
+            __m256 im = _mm256_shuffle_ps(a, b, _MM_SHUFFLE(3, 1, 3, 1));
+            re = _mm256_castpd_ps(_mm256_permute4x64_pd(_mm256_castps_pd(re),
+                                                        _MM_SHUFFLE(3, 1, 2, 0)));
+            im = _mm256_castpd_ps(_mm256_permute4x64_pd(_mm256_castps_pd(im),
+                                                        _MM_SHUFFLE(3, 1, 2, 0)));
+            __m256 tr = _mm256_mul_ps(c, pr), ti = _mm256_mul_ps(c, pim);
+            tr = _mm256_sub_ps(re, tr);
+            ti = _mm256_sub_ps(im, ti);
+            tr = _mm256_mul_ps(tr, tr);
+            ti = _mm256_mul_ps(ti, ti);
+            tr = _mm256_add_ps(tr, ti);
+            tr = _mm256_mul_ps(tr, _mm256_loadu_ps(sigRcp + idx));
+            _mm256_storeu_ps(result + j, _mm256_add_ps(_mm256_loadu_ps(result + j), tr));
+        }
+        for (; j < n; j++) {
+            const size_t idx = (size_t)i * n + j;
+            float cr = ctf[idx] * pri[2 * i], ci = ctf[idx] * pri[2 * i + 1];
+            float dr = dat[2 * idx] - cr, di = dat[2 * idx + 1] - ci;
+            float t = dr * dr + di * di;
+            result[j] += t * sigRcp[idx];
+        }
+    }
+}
+
+int thxo_logdatavs_pm_simd256(const float* dat, const float* pri, const float* ctf,
+                              const float* sigRcp, int n, int m, float* result)
+{
+    if (!__builtin_cpu_supports("avx2")) return -1;
+    logdatavs_pm_avx256(dat, pri, ctf, sigRcp, n, m, result);
+    return 0;
+}
+#pragma GCC pop_options
 
 /* FP64 reference of the same sum for one image-major particle. */
 double thxo_logdatavs_f64(const float* dat, const float* pri, const float* ctf,
@@ -242,15 +303,16 @@ double thxo_logdatavs_f64(const float* dat, const float* pri, const float* ctf,
  * pixel-major order the CPU path uses (allocPreCal, :7994-8121).
  * wC[P], wR[P*R], wT[P*T], baseL[P].
  */
-int thxo_global_scan_f32(const float* vol, int V, int pf,
-                         const int* iCol, const int* iRow, int K, int N,
-                         const double* quat, int R, const double* trans, int T,
-                         const float* datP, const float* ctfP,
-                         const float* sigRcpP, int P,
-                         const double* pR, const double* pT,
-                         float* wC, float* wR, float* wT, float* baseL,
-                         int nThreads)
+static int global_scan_f32(const float* vol, int V, int pf,
+                           const int* iCol, const int* iRow, int K, int N,
+                           const double* quat, int R, const double* trans, int T,
+                           const float* datP, const float* ctfP,
+                           const float* sigRcpP, int P,
+                           const double* pR, const double* pT,
+                           float* wC, float* wR, float* wT, float* baseL,
+                           int nThreads, int simd)
 {
+    if (simd && !__builtin_cpu_supports("avx2")) return -2;
 #ifdef _OPENMP
     if (nThreads > 0) omp_set_num_threads(nThreads);
 #else
@@ -304,7 +366,10 @@ int thxo_global_scan_f32(const float* vol, int V, int pf,
                     priAllP[2 * i + 1] = ar * bi + ai * br;
                 }
                 memset(res, 0, sizeof(float) * P);
-                thxo_logdatavs_pm(datPM, priAllP, ctfPM, sigPM, P, K, res);
+                if (simd)
+                    logdatavs_pm_avx256(datPM, priAllP, ctfPM, sigPM, P, K, res);
+                else
+                    thxo_logdatavs_pm(datPM, priAllP, ctfPM, sigPM, P, K, res);
                 for (int l = 0; l < P; l++) {
 #ifdef _OPENMP
                     omp_set_lock(&mtx[l]);
@@ -337,6 +402,35 @@ int thxo_global_scan_f32(const float* vol, int V, int pf,
 #endif
     free(datPM); free(ctfPM); free(sigPM); free(traP);
     return 0;
+}
+
+int thxo_global_scan_f32(const float* vol, int V, int pf,
+                         const int* iCol, const int* iRow, int K, int N,
+                         const double* quat, int R, const double* trans, int T,
+                         const float* datP, const float* ctfP,
+                         const float* sigRcpP, int P,
+                         const double* pR, const double* pT,
+                         float* wC, float* wR, float* wT, float* baseL,
+                         int nThreads)
+{
+    return global_scan_f32(vol, V, pf, iCol, iRow, K, N, quat, R, trans, T, datP, ctfP, sigRcpP,
+                           P, pR, pT, wC, wR, wT, baseL, nThreads, 0);
+}
+
+/* The same scan with the likelihood sum of SIMD256Float (the reference's
+ * ENABLE_SIMD_256 build, src/Optimiser.cpp:811-818): -2 when the CPU has no
+ * AVX2. */
+int thxo_global_scan_f32_simd256(const float* vol, int V, int pf,
+                                 const int* iCol, const int* iRow, int K, int N,
+                                 const double* quat, int R, const double* trans, int T,
+                                 const float* datP, const float* ctfP,
+                                 const float* sigRcpP, int P,
+                                 const double* pR, const double* pT,
+                                 float* wC, float* wR, float* wT, float* baseL,
+                                 int nThreads)
+{
+    return global_scan_f32(vol, V, pf, iCol, iRow, K, N, quat, R, trans, T, datP, ctfP, sigRcpP,
+                           P, pR, pT, wC, wR, wT, baseL, nThreads, 1);
 }
 
 /*

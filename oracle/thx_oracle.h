@@ -51,6 +51,11 @@ void thxo_project_f64(double* dst, const float* vol, int V, const double* M,
 void thxo_logdatavs_pm(const float* dat, const float* pri, const float* ctf,
                        const float* sigRcp, int n, int m, float* result);
 
+/* a7 as SIMD256Float computes it (AVX, 8 images per vector): equal to
+ * thxo_logdatavs_pm bit for bit; -1 without AVX2. */
+int thxo_logdatavs_pm_simd256(const float* dat, const float* pri, const float* ctf,
+                              const float* sigRcp, int n, int m, float* result);
+
 /* a7 in FP64 for one image-major particle; pri = S_t (.) P_r already formed. */
 double thxo_logdatavs_f64(const float* dat, const float* pri, const float* ctf,
                           const float* sigRcp, int K);
@@ -64,6 +69,17 @@ int thxo_global_scan_f32(const float* vol, int V, int pf,
                          const double* pR, const double* pT,
                          float* wC, float* wR, float* wT, float* baseL,
                          int nThreads);
+
+/* a8 with the SIMD256Float likelihood sum (the reference's ENABLE_SIMD_256
+ * build); -2 without AVX2. */
+int thxo_global_scan_f32_simd256(const float* vol, int V, int pf,
+                                 const int* iCol, const int* iRow, int K, int N,
+                                 const double* quat, int R, const double* trans, int T,
+                                 const float* datP, const float* ctfP,
+                                 const float* sigRcpP, int P,
+                                 const double* pR, const double* pT,
+                                 float* wC, float* wR, float* wT, float* baseL,
+                                 int nThreads);
 
 /* a8 in FP64 with a two-pass max, scores also returned when dvp != NULL. */
 int thxo_global_scan_f64(const float* vol, int V, int pf,

@@ -262,6 +262,21 @@ def test_logdatavs_zero_at_truth_and_expansion():
         assert v == pytest.approx(res2[j], rel=1e-5)
 
 
+def test_simd256_likelihood_equals_scalar():
+    """SIMD256Float (8 images per AVX vector, deinterleaved pairs, scalar tail)
+    gives the scalar pixel-major sum bit for bit, for image counts that are and
+    are not multiples of 8 (tail only, vector only, both)."""
+    rng = np.random.default_rng(14)
+    K = 97
+    for n in (3, 8, 16, 21):
+        pri = rng.normal(size=(K, 2)).astype(np.float32)
+        ctf = rng.uniform(-1, 1, size=(K, n)).astype(np.float32)
+        sig = -rng.uniform(0.5, 2, size=(K, n)).astype(np.float32)
+        dat = rng.normal(size=(K, n, 2)).astype(np.float32)
+        np.testing.assert_array_equal(oracle.logdatavs_pm_simd256(dat, pri, ctf, sig),
+                                      oracle.logdatavs_pm(dat, pri, ctf, sig))
+
+
 @pytest.fixture(scope="module")
 def scan_case():
     V, N, pf = 32, 16, 2
@@ -328,6 +343,24 @@ def test_global_scan_f32_streaming_matches_f64(scan_case):
         np.testing.assert_allclose(w32[0], w64[0], rtol=1e-4)
         np.testing.assert_allclose(w32[1], w64[1], rtol=1e-4, atol=1e-6 * w64[1].max())
         np.testing.assert_allclose(w32[2], w64[2], rtol=1e-4, atol=1e-6 * w64[2].max())
+
+
+def test_global_scan_simd256_equals_scalar(scan_case):
+    """The scan with SIMD256Float's sum (the reference's ENABLE_SIMD_256 build)
+    equals the scalar scan bit for bit on one thread (same accumulation
+    order), on a case with 13 particles (one vector and a tail)."""
+    c = scan_case
+    rng = np.random.default_rng(2)
+    P = 13
+    dat = np.concatenate([c["dat"], rng.normal(size=(P - 5,) + c["dat"].shape[1:]) * 0.01])
+    ctf = np.concatenate([c["ctf"], rng.uniform(-1, 1, size=(P - 5, c["ctf"].shape[1]))])
+    sig = np.full((P, c["ctf"].shape[1]), -2.0)
+    args = (c["hv"], c["V"], c["pf"], c["iCol"], c["iRow"], c["N"], c["quat"], c["trans"],
+            dat, ctf, sig, c["pR"], c["pT"])
+    a = oracle.global_scan(*args, f64=False, threads=1)
+    b = oracle.global_scan(*args, f64=False, threads=1, simd=True)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
 
 
 def test_local_scan_reduces_to_global(scan_case):
