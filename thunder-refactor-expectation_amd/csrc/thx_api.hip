@@ -262,8 +262,8 @@ static int default_ctx(thx_ctx** out) {
     return THX_OK;
 }
 
-// BF16 copies of the class volumes for the 16-bit local scorers: rebuilt on
-// their next use after the volume changes.
+// Bricked BF16 copies of the class volumes for the 16-bit local scorers:
+// rebuilt on their next use after the volume changes.
 void invalidate_vol_lo(thx_ctx* c, int cls) {
     if ((int)c->volLoOk.size() <= cls) c->volLoOk.resize(cls + 1, 0);
     c->volLoOk[cls] = 0;
@@ -276,9 +276,8 @@ int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used) {
     for (int k = 0; k < nCls; k++) {
         if (!used[k] || c->volLoOk[k]) continue;
         const int V = c->vdim[k];
-        const size_t n = (size_t)(V / 2 + 1) * V * V;
-        THX_TRY(c->volLo[k].ensure(sizeof(uint32_t) * n));
-        THX_TRY(launch_vol_to_bf16(c->stream, c->vol[k].as<const float2>(), n,
+        THX_TRY(c->volLo[k].ensure(vb_bytes(V)));
+        THX_TRY(launch_vol_bricked(c->stream, c->vol[k].as<const float2>(), V,
                                    c->volLo[k].as<uint32_t>()));
         c->volLoOk[k] = 1;
     }
@@ -724,14 +723,25 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
         maxT = std::max(maxT, nT[p]);
     }
     THX_CHECK(nScored > 0, THX_ERR_ARG, "score_local: no particle has rotations to score");
-    // work items particle by particle in pose order (class, then first rotation)
-    for (int p : pose_order(quat, rOff.data(), cls, P, nR)) {
-        bStart[p] = (int)blocks.size();
-        for (int r0 = 0; r0 < nR[p]; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
+    // work items particle by particle in pose order (class, then first
+    // rotation), grouped by translation tile: particles with at most 32, 64 and
+    // 128 translations each go to their own scorer launch with that tile, so a
+    // ragged batch pays for its padding per particle, not for its largest T_p
+    std::fill(bStart.begin(), bStart.end(), 0);
+    const std::vector<int> order = pose_order(quat, rOff.data(), cls, P, nR);
+    int tileStart[kLocalTiles + 1];
+    for (int ti = 0; ti < kLocalTiles; ti++) {
+        tileStart[ti] = (int)blocks.size();
+        for (int p : order) {
+            if (nR[p] == 0 || local_tile_index(nT[p]) != ti) continue;
+            bStart[p] = (int)blocks.size();
+            for (int r0 = 0; r0 < nR[p]; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
+        }
     }
+    tileStart[kLocalTiles] = (int)blocks.size();
     bStart[P] = (int)blocks.size();
     THX_CHECK(Rtot < (1u << 31) && Ttot < (1u << 31), THX_ERR_RANGE, "score_local: batch too large");
-    const int TPAD = maxT <= 32 ? 32 : (maxT <= 64 ? 64 : 128);
+    const int TPAD = kLocalTile[local_tile_index(maxT)];
     const int nblk = (int)blocks.size();
     // each particle's rotations in pose order, so the 128 rotations of a work
     // item are a compact cluster and their gathers share cache lines; wR and
@@ -800,7 +810,7 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     THX_TRY(c->locBMax.ensure(sizeof(float) * nblk));
     THX_TRY(c->locBWC.ensure(sizeof(float) * nblk));
     THX_TRY(c->locBArg.ensure(sizeof(int) * nblk));
-    THX_TRY(c->locBWT.ensure(sizeof(float) * (size_t)nblk * TPAD));
+    THX_TRY(c->locBWT.ensure(sizeof(float) * (size_t)nblk * kLocalWtStride));
     THX_TRY(c->locWRrel.ensure(sizeof(float) * Rtot));
     THX_TRY(c->locWC.ensure(sizeof(double) * P));
     THX_TRY(c->locWR.ensure(sizeof(double) * Rtot));
@@ -839,9 +849,27 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     const int prepPrec = precision == THX_FP16 ? THX_FP16 : THX_FP32;
     if (precision == THX_FP16) THX_TRY(launch_local_shell_scale(c, used));
     THX_TRY(launch_prep_particles(c, prepPrec));
-    const int mode = choose_local_mode(c, precision, TPAD, nblk, b);
-    if (mode < 0) return THX_ERR_HIP;
-    c->locModeUsed = mode;
+    // launch shape per tile class (the tuner's choice, or the requested one)
+    int modes[kLocalTiles] = {};
+    int most = 0;
+    for (int ti = 0; ti < kLocalTiles; ti++) {
+        const int n = tileStart[ti + 1] - tileStart[ti];
+        if (n == 0) continue;
+        const LocalBatch bt = local_batch_range(b, tileStart[ti]);
+        modes[ti] = choose_local_mode(c, precision, kLocalTile[ti], n, bt);
+        if (modes[ti] < 0) return THX_ERR_HIP;
+        if (n > most) { most = n; c->locModeUsed = modes[ti]; }
+    }
+    // the scorer launches of all tile classes
+    auto score_all = [&]() -> int {
+        for (int ti = 0; ti < kLocalTiles; ti++) {
+            const int n = tileStart[ti + 1] - tileStart[ti];
+            if (n == 0) continue;
+            THX_TRY(launch_score_local(c, precision, kLocalTile[ti], n,
+                                       local_batch_range(b, tileStart[ti]), modes[ti]));
+        }
+        return THX_OK;
+    };
     if (c->locGraphOn) {
         THX_TRY(c->locCounter.ensure(8 * sizeof(int)));   // no allocation inside the capture
         // one captured graph per batch shape and buffer set: operand prep,
@@ -859,8 +887,12 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
              cls ? c->locPcls.p : nullptr, c->locBMax.p, c->locBWC.p, c->locBArg.p, c->locBWT.p,
              c->locWRrel.p, c->locWC.p, c->locWR.p, c->locWT.p, c->locBase.p, c->locBestR.p,
              c->locBestT.p, c->locCounter.p});
-        const long long shape[] = {P, nblk, TPAD, precision, mode, c->K, c->Kp, c->idim};
-        key.insert(key.end(), shape, shape + 8);
+        const long long shape[] = {P, nblk, precision, c->K, c->Kp, c->idim};
+        key.insert(key.end(), shape, shape + 6);
+        for (int ti = 0; ti < kLocalTiles; ti++) {
+            key.push_back(tileStart[ti + 1] - tileStart[ti]);
+            key.push_back(modes[ti]);
+        }
         if (!c->locGraph || key != c->locGraphKey) {
             if (c->locGraph) {
                 (void)hipGraphExecDestroy(c->locGraph);
@@ -868,9 +900,9 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
             }
             THX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             int cs = launch_prep_particles(c, prepPrec);
-            if (cs == THX_OK) cs = launch_score_local(c, precision, TPAD, nblk, b, mode);
+            if (cs == THX_OK) cs = score_all();
             if (cs == THX_OK)
-                cs = launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(),
+                cs = launch_merge_local(c, P, kLocalWtStride, b, c->locWC.as<double>(),
                                         c->locWR.as<double>(), c->locWT.as<double>(),
                                         c->locBase.as<double>(), c->locBestR.as<int>(),
                                         c->locBestT.as<int>());
@@ -895,9 +927,9 @@ int thx_ctx_score_local(thx_ctx* c, const int* cls, const double* quat, const in
     THX_TRY(record(c, "start"));
     THX_TRY(launch_prep_particles(c, prepPrec));
     THX_TRY(record(c, "prep"));
-    THX_TRY(launch_score_local(c, precision, TPAD, nblk, b, mode));
+    THX_TRY(score_all());
     THX_TRY(record(c, "score_local"));
-    THX_TRY(launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
+    THX_TRY(launch_merge_local(c, P, kLocalWtStride, b, c->locWC.as<double>(), c->locWR.as<double>(),
                                c->locWT.as<double>(), c->locBase.as<double>(),
                                c->locBestR.as<int>(), c->locBestT.as<int>()));
     THX_TRY(record(c, "merge_local"));

@@ -937,7 +937,7 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     THX_TRY(c->locBMax.ensure(sizeof(float) * (size_t)P * nbp));
     THX_TRY(c->locBWC.ensure(sizeof(float) * (size_t)P * nbp));
     THX_TRY(c->locBArg.ensure(sizeof(int) * (size_t)P * nbp));
-    THX_TRY(c->locBWT.ensure(sizeof(float) * (size_t)P * nbp * TPAD));
+    THX_TRY(c->locBWT.ensure(sizeof(float) * (size_t)P * nbp * kLocalWtStride));
     THX_TRY(c->locWRrel.ensure(sizeof(float) * (size_t)P * mLR));
     THX_TRY(c->locWC.ensure(sizeof(double) * P));
     THX_TRY(c->locWR.ensure(sizeof(double) * (size_t)P * mLR));
@@ -1028,7 +1028,7 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
         if (mode < 0) return THX_ERR_HIP;
         c->locModeUsed = mode;
         THX_TRY(launch_score_local(c, precision, TPAD, (int)blocks.size(), b, mode));
-        THX_TRY(launch_merge_local(c, P, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
+        THX_TRY(launch_merge_local(c, P, kLocalWtStride, b, c->locWC.as<double>(), c->locWR.as<double>(),
                                    c->locWT.as<double>(), c->locBase.as<double>(),
                                    c->locBestR.as<int>(), c->locBestT.as<int>()));
         // half-height peak of the rotation marginal, top samples

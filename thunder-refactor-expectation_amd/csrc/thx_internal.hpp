@@ -103,7 +103,7 @@ struct LocalBatch {
     float* bMax = nullptr;                 // [nblk]
     float* bWC = nullptr;                  // [nblk]
     int* bArg = nullptr;                   // [nblk]
-    float* bWT = nullptr;                  // [nblk][TPAD]
+    float* bWT = nullptr;                  // [nblk][kLocalWtStride] (selection scorer: [nblk][TPAD])
     float* wRrel = nullptr;                // [Rtot]
     // equal counts for every particle (rOff / nR / tOff / nT / bStart null):
     // particle p has affR rotations from p * affR, affT translations from
@@ -124,8 +124,8 @@ struct thx_ctx {
     // pixel set
     int K = 0, Kp = 0, idim = 0, pf = 2, maxShell = 0;
     thx::DevBuf iCol, iRow, iSig;
-    // volumes per class, and their BF16 copies for the 16-bit local scorers
-    // (built on first use after a volume changes)
+    // volumes per class, and their bricked BF16 copies for the 16-bit local
+    // scorers (built on first use after a volume changes)
     std::vector<thx::DevBuf> vol;
     std::vector<int> vdim;
     std::vector<thx::DevBuf> volLo;
@@ -248,6 +248,22 @@ int launch_synth_noise(thx_ctx* c, int P, double noiseSigma, uint64_t seed);
 int launch_ctf_attr(thx_ctx* c, const float* dAttr, int P, float pixelSize);
 int launch_score_one(thx_ctx* c, int cls, int p, float* dOut);
 constexpr int kLocalRotBlock = 128;   // rotations per local-search block
+// translation tiles of the local scorer (rows of a block's A operand) and the
+// stride of its per-item translation partials (the largest tile)
+constexpr int kLocalTiles = 3;
+constexpr int kLocalTile[kLocalTiles] = {32, 64, 128};
+constexpr int kLocalWtStride = 128;
+inline int local_tile_index(int T) { return T <= 32 ? 0 : (T <= 64 ? 1 : 2); }
+// the work items from `start` on (per-item outputs offset with them)
+inline LocalBatch local_batch_range(const LocalBatch& b, int start) {
+    LocalBatch r = b;
+    r.blocks = b.blocks + start;
+    r.bMax = b.bMax + start;
+    r.bWC = b.bWC + start;
+    r.bArg = b.bArg + start;
+    r.bWT = b.bWT + (size_t)start * kLocalWtStride;
+    return r;
+}
 // local scorer launch shapes: one block per work item, or a persistent grid
 // pulling items from a counter
 constexpr int kLocalDirect = 1;
@@ -266,13 +282,34 @@ int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used);
 // BF16 copies of the used class volumes (built where missing or stale)
 int ensure_vol_lo(thx_ctx* c, const std::vector<char>& used);
 void invalidate_vol_lo(thx_ctx* c, int cls);
-int launch_vol_to_bf16(hipStream_t st, const float2* vol, size_t n, uint32_t* lo);
+// The 16-bit local scorers gather from a bricked BF16 copy of each class
+// volume.  Element (x, y, z) holds the BF16 complex voxels x and x + 1 of row
+// (y, z) and, with kVbRows = 2, also of row ((y + 1) mod V, z): 8 or 16
+// bytes, so a trilinear corner set is four or two loads.  Elements are
+// grouped in bricks of kVbX x kVbY x kVbZ (one 128-byte cache line), bricks
+// x fastest, then y, then z.  The x + 1 voxel of the last column is 0.
+#ifndef THX_VB_ROWS
+#define THX_VB_ROWS 1
+#define THX_VB_X 1
+#define THX_VB_Y 4
+#define THX_VB_Z 4
+#endif
+constexpr int kVbRows = THX_VB_ROWS, kVbX = THX_VB_X, kVbY = THX_VB_Y, kVbZ = THX_VB_Z;
+constexpr int kVbPer = 16 / kVbRows;   // elements per brick
+static_assert(kVbX * kVbY * kVbZ == kVbPer, "a brick is one 128-byte line");
+__host__ __device__ inline int vb_nbx(int V) { return (V / 2 + 1 + kVbX - 1) / kVbX; }
+__host__ __device__ inline int vb_nby(int V) { return (V + kVbY - 1) / kVbY; }
+__host__ __device__ inline int vb_nbz(int V) { return (V + kVbZ - 1) / kVbZ; }
+__host__ __device__ inline size_t vb_bytes(int V) {
+    return (size_t)vb_nbx(V) * vb_nby(V) * vb_nbz(V) * 128;
+}
+int launch_vol_bricked(hipStream_t st, const float2* vol, int V, uint32_t* vb);
 // frees the projectee FFT plans and scratch
 void projectee_release(thx_ctx* c);
 void images_release(thx_ctx* c);
 int alloc_particles(thx_ctx* c, int P);
 int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b);
-int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
+int launch_merge_local(thx_ctx* c, int P, int wtStride, const LocalBatch& b, double* wC, double* wR,
                        double* wT, double* base, int* bestR, int* bestT,
                        int rb = kLocalRotBlock);
 

@@ -41,52 +41,157 @@ typedef int intx8 __attribute__((ext_vector_type(8)));
 // Geometry helpers (device)
 // ---------------------------------------------------------------------------
 
-// The same interpolation from the BF16 copy of the volume (one 4-byte word
-// per voxel: re in the low half, im in the high half); corners are widened to
-// FP32 and weighted in FP32.
-__device__ inline float2 sample_trilinear_lo(const uint32_t* __restrict__ vol, int V, float x,
-                                             float y, float z) {
-    bool conj = false;
-    if (x < 0) { x = -x; y = -y; z = -z; conj = true; }
-    const int x0 = (int)floorf(x), y0 = (int)floorf(y), z0 = (int)floorf(z);
-    const float xd = x - x0, yd = y - y0, zd = z - z0;
-    const float v0[2] = {1 - xd, xd}, v1[2] = {1 - yd, yd}, v2[2] = {1 - zd, zd};
-    const int nColFT = V / 2 + 1;
-    float re = 0.f, im = 0.f;
+// Raw buffer resource over `bytes` bytes of device memory (gfx9 dword3:
+// 32-bit data format, no swizzle).  Loads through it take 32-bit byte offsets
+// and return 0 past the end instead of faulting.
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+typedef unsigned int u32x2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4v __attribute__((ext_vector_type(4)));
+
+// NS trilinear samples of a half-complex volume (same arithmetic as
+// sample_trilinear) through a buffer resource, with every corner load of the
+// NS points issued before any is used: four loads per point, one per (y, z)
+// corner row, each reading the x0 and x0 + 1 voxels together (8 bytes from
+// the bricked BF16 copy, LO; 16 from the FP32 volume, linear layout).  Byte
+// offsets are 32-bit (volumes below 2 GiB).  Points the caller marks !ok
+// read element 0 and return 0.
+template <bool LO, int NS>
+__device__ __forceinline__ void sample_trilinear_n(__amdgpu_buffer_rsrc_t rs, int V,
+                                                   const float (&px)[NS], const float (&py)[NS],
+                                                   const float (&pz)[NS], const bool (&ok)[NS],
+                                                   float2 (&out)[NS]) {
+    const int nc = V / 2 + 1;
+    const uint32_t rowY = (uint32_t)vb_nbx(V) * 128u, rowZ = (uint32_t)vb_nby(V) * rowY;
+    constexpr int NL = LO ? 2 * (3 - kVbRows) : 4;   // loads per point
+    float fx[NS], fy[NS], fz[NS];
+    bool cj[NS];
+    uint32_t off[NS][4];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-        int zz = z0 + k;
-        zz = zz < 0 ? zz + V : zz;
+    for (int s = 0; s < NS; s++) {
+        float x = px[s], y = py[s], z = pz[s];
+        cj[s] = x < 0;
+        if (cj[s]) { x = -x; y = -y; z = -z; }
+        if (!ok[s]) { x = 0.f; y = 0.f; z = 0.f; }
+        const int x0 = (int)floorf(x), y0 = (int)floorf(y), z0 = (int)floorf(z);
+        fx[s] = x - x0; fy[s] = y - y0; fz[s] = z - z0;
+        const int ya = y0 < 0 ? y0 + V : y0, yb = y0 + 1 < 0 ? y0 + 1 + V : y0 + 1;
+        const int za = z0 < 0 ? z0 + V : z0, zb = z0 + 1 < 0 ? z0 + 1 + V : z0 + 1;
+        if constexpr (LO) {
+            // byte offset of element (x, y, z): brick and in-brick parts of each
+            // coordinate (all non-negative here: unsigned shifts and masks)
+            constexpr uint32_t EB = 8u * kVbRows;
+            const uint32_t ux = x0, uya = ya, uyb = yb, uza = za, uzb = zb;
+            const uint32_t xo = (ux / kVbX) * 128u + (ux % kVbX) * EB;
+            const uint32_t ya_o = (uya / kVbY) * rowY + (uya % kVbY) * (kVbX * EB);
+            const uint32_t za_o = (uza / kVbZ) * rowZ + (uza % kVbZ) * (kVbX * kVbY * EB);
+            const uint32_t zb_o = (uzb / kVbZ) * rowZ + (uzb % kVbZ) * (kVbX * kVbY * EB);
+            if constexpr (kVbRows == 2) {
+                // one element holds rows ya and ya + 1 (= yb, wrapped)
+                off[s][0] = za_o + ya_o + xo;
+                off[s][1] = zb_o + ya_o + xo;
+            } else {
+                const uint32_t yb_o = (uyb / kVbY) * rowY + (uyb % kVbY) * (kVbX * EB);
+                off[s][0] = za_o + ya_o + xo;
+                off[s][1] = za_o + yb_o + xo;
+                off[s][2] = zb_o + ya_o + xo;
+                off[s][3] = zb_o + yb_o + xo;
+            }
+        } else {
+            off[s][0] = (uint32_t)((za * V + ya) * nc + x0) * 8u;
+            off[s][1] = (uint32_t)((za * V + yb) * nc + x0) * 8u;
+            off[s][2] = (uint32_t)((zb * V + ya) * nc + x0) * 8u;
+            off[s][3] = (uint32_t)((zb * V + yb) * nc + x0) * 8u;
+        }
+    }
+    // corner words per point in the reference's order: z, then y, then x;
+    // each word is one BF16 complex voxel (LO) or one float (FP32, re / im)
+    uint32_t w[NS][16];
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            int yy = y0 + j;
-            yy = yy < 0 ? yy + V : yy;
-            const uint32_t* row = vol + ((size_t)zz * V + yy) * nColFT;
+    for (int s = 0; s < NS; s++) {
+        if constexpr (!LO || kVbRows == 2) {
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const uint32_t d = row[x0 + i];
-                const float w = v0[i] * v1[j] * v2[k];
-                re += w * __uint_as_float(d << 16);
-                im += w * __uint_as_float(d & 0xffff0000u);
+            for (int c = 0; c < NL; c++) {
+                const u32x4v d = __builtin_amdgcn_raw_buffer_load_b128(rs, off[s][c], 0, 0);
+                w[s][4 * c] = d.x; w[s][4 * c + 1] = d.y; w[s][4 * c + 2] = d.z; w[s][4 * c + 3] = d.w;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const u32x2v d = __builtin_amdgcn_raw_buffer_load_b64(rs, off[s][c], 0, 0);
+                w[s][2 * c] = d.x; w[s][2 * c + 1] = d.y;
             }
         }
     }
-    return conj ? make_float2(re, -im) : make_float2(re, im);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const float wz[2] = {1 - fz[s], fz[s]}, wy[2] = {1 - fy[s], fy[s]};
+        const float wx[2] = {1 - fx[s], fx[s]};
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {       // corner row: (z, y) = (c >> 1, c & 1)
+            const float w0 = wx[0] * wy[c & 1] * wz[c >> 1];
+            const float w1 = wx[1] * wy[c & 1] * wz[c >> 1];
+            if constexpr (LO) {
+                const uint32_t a = w[s][2 * c], b = w[s][2 * c + 1];
+                re += w0 * __uint_as_float(a << 16);
+                im += w0 * __uint_as_float(a & 0xffff0000u);
+                re += w1 * __uint_as_float(b << 16);
+                im += w1 * __uint_as_float(b & 0xffff0000u);
+            } else {
+                re += __uint_as_float(w[s][4 * c]) * w0;
+                im += __uint_as_float(w[s][4 * c + 1]) * w0;
+                re += __uint_as_float(w[s][4 * c + 2]) * w1;
+                im += __uint_as_float(w[s][4 * c + 3]) * w1;
+            }
+        }
+        out[s] = ok[s] ? make_float2(re, cj[s] ? -im : im) : make_float2(0.f, 0.f);
+    }
 }
 
-// lo[i] = (bf16(vol[i].x), bf16(vol[i].y)), round to nearest even
-__global__ void k_vol_to_bf16(const float2* __restrict__ vol, size_t n, uint32_t* __restrict__ lo) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float2 v = vol[i];
-    const __bf16 a = (__bf16)v.x, b = (__bf16)v.y;
-    lo[i] = (uint32_t)__builtin_bit_cast(unsigned short, a) |
-            ((uint32_t)__builtin_bit_cast(unsigned short, b) << 16);
+__device__ inline unsigned short bf16_bits(float x) {
+    return __builtin_bit_cast(unsigned short, (__bf16)x);
 }
 
-int launch_vol_to_bf16(hipStream_t st, const float2* vol, size_t n, uint32_t* lo) {
-    hipLaunchKernelGGL(k_vol_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vol, n,
-                       lo);
+// Bricked BF16 copy of a half-complex volume (layout in thx_internal.hpp):
+// one thread per element, round to nearest even; padding elements and the
+// x + 1 voxel of the last column are 0.
+__global__ void k_vol_bricked(const float2* __restrict__ vol, int V, int nbx, int nby,
+                              size_t nElem, uint32_t* __restrict__ vb) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nElem) return;
+    const int within = (int)(e % kVbPer);
+    const size_t brick = e / kVbPer;
+    const int bx = (int)(brick % nbx), by = (int)((brick / nbx) % nby);
+    const int bz = (int)(brick / ((size_t)nbx * nby));
+    const int x = bx * kVbX + within % kVbX;
+    const int y = by * kVbY + (within / kVbX) % kVbY;
+    const int z = bz * kVbZ + within / (kVbX * kVbY);
+    const int nc = V / 2 + 1;
+    uint32_t o[2 * kVbRows] = {};
+    if (x < nc && y < V && z < V) {
+#pragma unroll
+        for (int r = 0; r < kVbRows; r++) {
+            const size_t i = ((size_t)z * V + (y + r) % V) * nc + x;
+            const float2 a = vol[i];
+            o[2 * r] = (uint32_t)bf16_bits(a.x) | ((uint32_t)bf16_bits(a.y) << 16);
+            if (x + 1 < nc) {
+                const float2 b = vol[i + 1];
+                o[2 * r + 1] = (uint32_t)bf16_bits(b.x) | ((uint32_t)bf16_bits(b.y) << 16);
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < 2 * kVbRows; w++) vb[e * 2 * kVbRows + w] = o[w];
+}
+
+int launch_vol_bricked(hipStream_t st, const float2* vol, int V, uint32_t* vb) {
+    const size_t n = vb_bytes(V) / (8 * kVbRows);
+    hipLaunchKernelGGL(k_vol_bricked, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, vol, V,
+                       vb_nbx(V), vb_nby(V), n, vb);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
@@ -1530,9 +1635,9 @@ __device__ __forceinline__ void score_local_item(int item,
     constexpr int MT = TPAD / 32;                   // 32-row MFMA tiles
     __shared__ __attribute__((aligned(16))) E As[TPAD * LD];
     __shared__ __attribute__((aligned(16))) E Bs[kNBL * LD];
-    __shared__ double rotM[kNBL][6];
     __shared__ float trs[TPAD][2];
     __shared__ float Cs[kNBL];
+    __shared__ float Cpart[4][kNBL];
     __shared__ float redMax[4];
     __shared__ int redArg[4];
     __shared__ float rowPart[4][TPAD];
@@ -1543,18 +1648,25 @@ __device__ __forceinline__ void score_local_item(int item,
     const int nr = min(kNBL, Rp - r0);
     const int rbase = rOff[p] + r0, tbase = tOff[p];
     const int cl = pcls ? pcls[p] : 0;
-    const float2* vol = vols[cl];
-    const uint32_t* volLo = F32 ? nullptr : volsLo[cl];
+    // the class volume (FP32) or its BF16 copy as a buffer resource
+    const uint32_t vbytes = F32 ? (uint32_t)(V / 2 + 1) * V * V * 8u : (uint32_t)vb_bytes(V);
+    const __amdgpu_buffer_rsrc_t vrs =
+        F32 ? buf_rsrc(vols[cl], vbytes) : buf_rsrc(volsLo[cl], vbytes);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l32 = lane & 31, hl = lane >> 5;
 
-    if (tid < kNBL) {
+    // each lane owns two of the item's rotations, lane and lane + 64, with
+    // their first two matrix columns in registers (zero past nr)
+    double rm[2][6];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
         double m[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-        if (tid < nr) quat_cols01(quat + 4 * (size_t)(rbase + tid), m);
-        rotM[tid][0] = m[0][0]; rotM[tid][1] = m[0][1];
-        rotM[tid][2] = m[1][0]; rotM[tid][3] = m[1][1];
-        rotM[tid][4] = m[2][0]; rotM[tid][5] = m[2][1];
+        if (lane + 64 * h < nr) quat_cols01(quat + 4 * (size_t)(rbase + lane + 64 * h), m);
+        rm[h][0] = m[0][0]; rm[h][1] = m[0][1];
+        rm[h][2] = m[1][0]; rm[h][3] = m[1][1];
+        rm[h][4] = m[2][0]; rm[h][5] = m[2][1];
     }
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
     if (tid < TPAD) {
         float tx = 0.f, ty = 0.f;
         if (tid < Tp) {
@@ -1569,9 +1681,7 @@ __device__ __forceinline__ void score_local_item(int item,
     floatx16 acc[MT];
 #pragma unroll
     for (int a = 0; a < MT; a++) acc[a] = floatx16{0};
-    float cacc[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) cacc[q] = 0.f;
+    float cacc[2] = {0.f, 0.f};   // norm-term sums of the lane's two rotations
     const float2* yrow = Y + (size_t)p * Kp;
     const float* wrow = Wt + (size_t)p * Kp;
     const int grp = tid >> 5;            // 8 groups of 32 lanes
@@ -1597,33 +1707,62 @@ __device__ __forceinline__ void score_local_item(int item,
                 }
             }
         }
-        // ---- B: slices of the block's rotations at pixel k ----
+        // ---- B: slices of the item's rotations at the chunk's pixels ----
+        // wave w fills pixels 8w .. 8w + 7 of the chunk for all 128 rotations:
+        // the 64 lanes of one load gather the same pixel of 64 neighbouring
+        // rotations (the item's rotations are a compact cluster in pose
+        // order), so at the low and middle radii they share cache lines
         {
-            const float wk = wrow[k];
-            const bool kok = k < K;
-            // FP16: the slice is divided by its shell's scale (Y carries it)
-            float inv = 1.f;
-            if constexpr (PREC == THX_FP16) inv = 1.f / shellScale[iSig[k]];
-            const double c0 = (double)(ic * pf), c1 = (double)(ir * pf);
+            const int kb = k0 + 8 * wv;
+            // NS samples per step with all their corner loads in flight
+#ifndef THX_GATHER_NS
+#define THX_GATHER_NS 4
+#endif
+            constexpr int NS = F32 ? 2 : THX_GATHER_NS;
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int rr = grp + 8 * q;
-                float2 v = make_float2(0.f, 0.f);
-                if (kok && rr < nr) {
-                    const double* m = rotM[rr];
-                    const float x = (float)(m[0] * c0 + m[1] * c1);
-                    const float yy = (float)(m[2] * c0 + m[3] * c1);
-                    const float z = (float)(m[4] * c0 + m[5] * c1);
-                    if constexpr (F32) v = sample_trilinear(vol, V, x, yy, z);
-                    else v = sample_trilinear_lo(volLo, V, x, yy, z);
+            for (int h = 0; h < 2; h++) {
+                const int rr = lane + 64 * h;
+                const bool rok = rr < nr;
+                float2 v8[8];
+#pragma unroll
+                for (int j0 = 0; j0 < 8; j0 += NS) {
+                    float px[NS], py[NS], pz[NS];
+                    bool ok[NS];
+                    float2 v[NS];
+#pragma unroll
+                    for (int s = 0; s < NS; s++) {
+                        const int kk = kb + j0 + s;
+                        const double c0 = (double)(iCol[kk] * pf), c1 = (double)(iRow[kk] * pf);
+                        ok[s] = rok && kk < K;
+                        px[s] = (float)(rm[h][0] * c0 + rm[h][1] * c1);
+                        py[s] = (float)(rm[h][2] * c0 + rm[h][3] * c1);
+                        pz[s] = (float)(rm[h][4] * c0 + rm[h][5] * c1);
+                    }
+                    sample_trilinear_n<!F32, NS>(vrs, V, px, py, pz, ok, v);
+#pragma unroll
+                    for (int s = 0; s < NS; s++) v8[j0 + s] = v[s];
                 }
-                cacc[q] += wk * (v.x * v.x + v.y * v.y);
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    cacc[h] += wrow[kb + j] * (v8[j].x * v8[j].x + v8[j].y * v8[j].y);
                 if constexpr (F32) {
-                    Bs[rr * LD + 2 * l32] = v.x;
-                    Bs[rr * LD + 2 * l32 + 1] = v.y;
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        Bs[rr * LD + 2 * (8 * wv + j)] = v8[j].x;
+                        Bs[rr * LD + 2 * (8 * wv + j) + 1] = v8[j].y;
+                    }
                 } else {
-                    *reinterpret_cast<uint32_t*>(Bs + rr * LD + 2 * l32) =
-                        pack16<PREC>(v.x * inv, v.y * inv);
+                    // FP16: the slice is divided by its shell's scale (Y carries it)
+                    uint32_t pk[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        float inv = 1.f;
+                        if constexpr (PREC == THX_FP16) inv = 1.f / shellScale[iSig[kb + j]];
+                        pk[j] = pack16<PREC>(v8[j].x * inv, v8[j].y * inv);
+                    }
+                    uint4* dst = reinterpret_cast<uint4*>(Bs + rr * LD + 16 * wv);
+                    dst[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+                    dst[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
                 }
             }
         }
@@ -1660,20 +1799,11 @@ __device__ __forceinline__ void score_local_item(int item,
         __syncthreads();
     }
 
-    // ---- norm term per rotation: sum of cacc over the 32 pixel lanes ----
-#pragma unroll
-    for (int q = 0; q < 16; q++) cacc[q] += __shfl_xor(cacc[q], 16);
-#pragma unroll
-    for (int s = 8; s >= 1; s >>= 1) {
-        const bool up = (l32 & s) != 0;
-#pragma unroll
-        for (int j = 0; j < s; j++) {
-            const float send = up ? cacc[j] : cacc[j + s];
-            const float keep = up ? cacc[j + s] : cacc[j];
-            cacc[j] = keep + __shfl_xor(send, s);
-        }
-    }
-    if (l32 < 16) Cs[grp + 8 * (l32 & 15)] = cacc[0];
+    // ---- norm term per rotation: the four waves' pixel sums ----
+    Cpart[wave][lane] = cacc[0];
+    Cpart[wave][lane + 64] = cacc[1];
+    __syncthreads();
+    if (tid < kNBL) Cs[tid] = (Cpart[0][tid] + Cpart[1][tid]) + (Cpart[2][tid] + Cpart[3][tid]);
     __syncthreads();
 
     // ---- dvp, block maximum ----
@@ -1755,7 +1885,7 @@ __device__ __forceinline__ void score_local_item(int item,
     __shared__ float rowW[TPAD];
     if (tid < TPAD) {
         const float rsum = rowPart[0][tid] + rowPart[1][tid] + rowPart[2][tid] + rowPart[3][tid];
-        bWT[(size_t)item * TPAD + tid] = rsum;
+        bWT[(size_t)item * kLocalWtStride + tid] = rsum;
         rowW[tid] = tid < Tp ? rsum * pTf[tbase + tid] : 0.f;
     }
     __syncthreads();
@@ -1815,7 +1945,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local_q(
         __syncthreads();   // the previous item's LDS traffic is over
         if (threadIdx.x == 0) item = atomicAdd(&counter[x], 1);
         __syncthreads();
-        const int j = item;
+        const int j = __builtin_amdgcn_readfirstlane(item);   // block-uniform: scalar registers
         if (j >= n) {      // this range is drained: the next XCD's
             if (++h == 8) break;
             x = (x + 1) % 8;
@@ -1895,6 +2025,12 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
               THX_ERR_ARG, "local scorer supports FP32, BF16 and FP16");
     THX_CHECK(precision == THX_FP32 || b.volsLo, THX_ERR_STATE,
               "local scorer: the 16-bit forms need the BF16 volume copies");
+    {
+        // the gathers address the volume with 32-bit byte offsets
+        const long long Vd = (long long)c->idim * c->pf;
+        THX_CHECK((Vd / 2 + 1) * Vd * Vd * 8 < (1ll << 31) && vb_bytes((int)Vd) < (1ull << 31),
+                  THX_ERR_RANGE, "local scorer: padded volume (or its bricked copy) above 2 GiB");
+    }
     int grid = 0;
     if (mode == kLocalQueue) {
         THX_TRY(c->locCounter.ensure(8 * sizeof(int)));
@@ -1922,7 +2058,7 @@ __global__ __launch_bounds__(256) void k_merge_local(
     const float* __restrict__ bMax, const float* __restrict__ bWC, const int* __restrict__ bArg,
     const float* __restrict__ bWT, const float* __restrict__ wRrel, const int* __restrict__ bStart,
     const int* __restrict__ rOff, const int* __restrict__ nR, const int* __restrict__ tOff,
-    const int* __restrict__ nT, int affR, int affT, int affB, int TPAD, int rb,
+    const int* __restrict__ nT, int affR, int affT, int affB, int wtStride, int rb,
     double* __restrict__ wC, double* __restrict__ wR, double* __restrict__ wT,
     double* __restrict__ base, int* __restrict__ bestR, int* __restrict__ bestT) {
     const int p = blockIdx.x;
@@ -1972,7 +2108,7 @@ __global__ __launch_bounds__(256) void k_merge_local(
         wR[ro + r] = f[r / rb] * (double)wRrel[ro + r];
     for (int t = threadIdx.x; t < Tp; t += blockDim.x) {
         double s = 0.0;
-        for (int i = 0; i < nb; i++) s += f[i] * bWT[(size_t)(b0 + i) * TPAD + t];
+        for (int i = 0; i < nb; i++) s += f[i] * bWT[(size_t)(b0 + i) * wtStride + t];
         wT[to + t] = s;
     }
 }
@@ -2042,11 +2178,11 @@ int launch_gather_particles(thx_ctx* c, const int* idx, int n, float2* dat2, flo
     return THX_OK;
 }
 
-int launch_merge_local(thx_ctx* c, int P, int TPAD, const LocalBatch& b, double* wC, double* wR,
-                       double* wT, double* base, int* bestR, int* bestT, int rb) {
+int launch_merge_local(thx_ctx* c, int P, int wtStride, const LocalBatch& b, double* wC,
+                       double* wR, double* wT, double* base, int* bestR, int* bestT, int rb) {
     hipLaunchKernelGGL(k_merge_local, dim3(P), dim3(256), 0, c->stream, b.bMax, b.bWC, b.bArg,
                        b.bWT, b.wRrel, b.bStart, b.rOff, b.nR, b.tOff, b.nT, b.affR, b.affT,
-                       b.affB, TPAD, rb, wC, wR, wT, base, bestR, bestT);
+                       b.affB, wtStride, rb, wC, wR, wT, base, bestR, bestT);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
