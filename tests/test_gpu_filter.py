@@ -288,3 +288,38 @@ def test_mat4_helpers_against_lapack(gpu_ctx):
         w, V = np.linalg.eigh(M[i])
         v = V[:, -1] * np.sign(V[np.argmax(np.abs(V[:, -1])), -1])
         np.testing.assert_allclose(top[i], v, atol=1e-10)
+
+
+def test_refine_phase_graph_replays_equal_eager(gpu_ctx):
+    """The local search's phases as one captured graph per sample-buffer
+    assignment, replayed for every phase (the work list, perturbation scale
+    and random-stream phases come from device memory): the final states equal
+    the eager run's bit for bit, for both launch shapes of the scorer, and on
+    a second search that reuses the cached graphs after eager work on the
+    same stream."""
+    print("\n[setup] box64 P=64 R=1000 T=49 refine: graph vs eager", flush=True)
+    wl.setup_context(gpu_ctx, "box64", n_particles=64, n_rot=1000)
+    gpu_ctx.score_global(0, thx.FP32)
+    ms = wl.WORKLOADS["box64"]["n_rot"]
+    args = (0, 300, 25, ms ** (-1.0 / 3), 0.3, 0.5, 1.0)
+    keys = ("quat", "trans", "wR", "wT", "k", "s01", "topR", "topT", "peakR", "nPhase")
+
+    def run(mode, graph):
+        gpu_ctx.set_local_options(mode, graph)
+        gpu_ctx.pf_from_global(*args, seed=3)
+        ran = gpu_ctx.pf_refine(None, thx.BF16, after_global=True, max_phase=14, seed=4)
+        return ran, gpu_ctx.pf_get_state()
+
+    try:
+        for mode in (1, 2):
+            ran0, ref = run(mode, False)
+            assert ran0 >= 10
+            for rep in range(2):
+                ran, st = run(mode, True)
+                assert ran == ran0
+                for k in keys:
+                    np.testing.assert_array_equal(st[k], ref[k], err_msg=f"{k} mode {mode} rep {rep}")
+                gpu_ctx.score_global(0, thx.FP32)   # eager work between the replays
+                gpu_ctx.sync()
+    finally:
+        gpu_ctx.set_local_options(0, False)

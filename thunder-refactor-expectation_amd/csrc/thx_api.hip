@@ -348,6 +348,9 @@ int thx_ctx_destroy(thx_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->locGraph) (void)hipGraphExecDestroy(c->locGraph);
+    for (auto& g : c->pfGraphs) (void)hipGraphExecDestroy(g.second);
+    c->pfGraphs.clear();
+    c->pfCountsH.release();
     if (c->lpReady) (void)hipEventDestroy(c->lpReady);
     if (c->lpDone) (void)hipEventDestroy(c->lpDone);
     if (c->lpRead) (void)hipEventDestroy(c->lpRead);
@@ -373,6 +376,7 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->pfWNew, &c->pfK, &c->pfS, &c->pfMean, &c->pfTopR, &c->pfTopT,
                        &c->pfPeak, &c->pfTransS, &c->pfBest, &c->pfActive, &c->pfNPhase,
                        &c->pfOffR, &c->pfOffT, &c->pfNR, &c->pfNT, &c->pfNRa, &c->pfNTa,
+                       &c->pfOrder, &c->pfCounts,
                        &c->locCounter};
     for (DevBuf* b : lbufs) b->release();
     for (auto& v : c->vol) v.release();

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <initializer_list>
 #include <vector>
 
 #include "thx_internal.hpp"
@@ -285,12 +286,14 @@ __global__ __launch_bounds__(256) void k_rot_stats(
 // balance weights 1 / N2(t - m; s0, s1) normalised (balanceWeight(PAR_T),
 // :2313-2344).  s01 (in: the perturbation's deviations; out: the new ones).
 __global__ void k_trans_stats(double* __restrict__ trans, int n, const int* __restrict__ active,
-                              int P, int perturb, double* __restrict__ s01, double pf,
+                              int P, int perturb, double* __restrict__ s01, PhaseScale pfs,
                               const double* __restrict__ transS, double transQ, uint64_t seed,
-                              uint32_t phase, double* __restrict__ mOut, double* __restrict__ wBal,
+                              PhaseArg pha, double* __restrict__ mOut, double* __restrict__ wBal,
                               float* __restrict__ wBalF) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P || (active && !active[p])) return;
+    const double pf = pfs.get();
+    const uint32_t phase = pha.get();
     double* t = trans + (size_t)p * n * 2;
     if (perturb) {
         const double s0 = s01[3 * p], s1 = s01[3 * p + 1];
@@ -443,9 +446,10 @@ __global__ void k_keep_rows(const double* __restrict__ src, double* __restrict__
 // (N_PHASE_WITH_NO_VARI_DECREASE = 1, include/Optimiser.h:56-63).
 __global__ void k_converge(const double* __restrict__ kk, const double* __restrict__ s01,
                            int* __restrict__ active, double* __restrict__ best,
-                           int* __restrict__ nPhase, int P, int phase, int minPhase) {
+                           int* __restrict__ nPhase, int P, PhaseArg pha, int minPhase) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P || !active[p]) return;
+    const int phase = (int)pha.get();
     if (phase < minPhase) return;
     const double vR = pow(kk[3 * p] * kk[3 * p + 1] * kk[3 * p + 2], 1.0 / 6);
     const double vT = s01[3 * p] * s01[3 * p + 1];
@@ -711,8 +715,9 @@ static int from_global_impl(thx_ctx* c, int cls0, const int* pcls, int nCls, int
                        nullptr, 6, kAcgTol, kAcgMaxIter, nullptr, c->pfMean.as<double>(),
                        c->pfK.as<double>(), nullptr, nullptr, nullptr);
     hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, c->pfTr.as<double>(), mLT,
-                       nullptr, P, 0, c->pfS.as<double>(), 0.0, c->pfTransS.as<const double>(),
-                       kTransQ, seed, 0u, nullptr, nullptr, nullptr);
+                       nullptr, P, 0, c->pfS.as<double>(), scale_host(0.0),
+                       c->pfTransS.as<const double>(), kTransQ, seed, phase_host(0u), nullptr,
+                       nullptr, nullptr);
     const double kMin = (minStdR / pfS) * (minStdR / pfS), sMin = minStdT / pfS;
     hipLaunchKernelGGL(k_min_std, dim3(nb(P)), dim3(256), 0, st, c->pfK.as<double>(),
                        c->pfS.as<double>(), P, kMin, sMin);
@@ -894,6 +899,238 @@ __global__ __launch_bounds__(256) void k_sort_samples(double* __restrict__ q,
     }
 }
 
+// Work list of a local-search phase, built on the device from the active
+// flags (one block): the active particles in `order` (pose order), nbp work
+// items of up to 128 rotations each, every particle's first item (bStart),
+// its sample counts nRa = mLR / nTa = mLT (0 once converged), counts[1] =
+// items, counts[2] = active particles.
+__global__ __launch_bounds__(1024) void k_phase_items(const int* __restrict__ order,
+                                                      const int* __restrict__ active, int P,
+                                                      int mLR, int mLT, int nbp,
+                                                      int2* __restrict__ blocks,
+                                                      int* __restrict__ bStart,
+                                                      int* __restrict__ nRa,
+                                                      int* __restrict__ nTa,
+                                                      int* __restrict__ counts) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x, per = (P + 1023) / 1024;
+    const int a = min(P, t * per), e = min(P, a + per);
+    int n = 0;
+    for (int i = a; i < e; i++) n += active[order[i]] != 0;
+    part[t] = n;
+    __syncthreads();
+    for (int s = 1; s < 1024; s <<= 1) {   // inclusive scan of the segment counts
+        const int v = t >= s ? part[t - s] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int rank = part[t] - n;   // active particles in the segments before this one
+    for (int i = a; i < e; i++) {
+        const int p = order[i];
+        const bool on = active[p] != 0;
+        nRa[p] = on ? mLR : 0;
+        nTa[p] = on ? mLT : 0;
+        bStart[p] = rank * nbp;
+        if (on) {
+            for (int j = 0; j < nbp; j++)
+                blocks[(size_t)rank * nbp + j] = make_int2(p, j * kLocalRotBlock);
+            rank++;
+        }
+    }
+    if (t == 1023) {
+        bStart[P] = part[1023] * nbp;
+        counts[1] = part[1023] * nbp;
+        counts[2] = part[1023];
+    }
+}
+
+// End of a phase (one block): counts[3] = particles still active, and the
+// phase counter the next phase's kernels read advances.
+__global__ __launch_bounds__(1024) void k_phase_end(const int* __restrict__ active, int P,
+                                                    int* __restrict__ counts) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    int n = 0;
+    for (int i = t; i < P; i += 1024) n += active[i] != 0;
+    part[t] = n;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) part[t] += part[t + s];
+        __syncthreads();
+    }
+    if (t == 0) {
+        counts[3] = part[0];
+        counts[0] += 1;
+    }
+}
+
+__global__ void k_set_phase(int* __restrict__ counts, int phase) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counts[0] = phase;
+}
+
+namespace {
+// The fixed inputs of one local search and the sample buffers of a phase
+// (the current sets q / tr and the resampled ones q2 / tr2, which swap roles
+// every phase).
+struct RefineSetup {
+    int P, mLR, mLT, nbp, TPAD, precision, mode, minPhase;
+    uint64_t seed;
+    double pfL, pfS;
+    const int* pcls;
+};
+struct PhaseBufs {
+    double *q, *q2, *tr, *tr2;
+};
+}  // namespace
+
+// One local-search phase over every particle (call stack of updateSP,
+// src/Optimiser.cpp:1139-1437, and the convergence test of :1475-1660),
+// enqueued on the context stream with no host data: the work list comes from
+// the active flags, the perturbation scale and random-stream phases from the
+// device phase counter, so the same launches serve every phase (eagerly or
+// as a replayed graph).
+static int enqueue_phase(thx_ctx* c, const RefineSetup& r, const PhaseBufs& pb) {
+    hipStream_t st = c->stream;
+    const int P = r.P, mLR = r.mLR, mLT = r.mLT;
+    int* dPh = c->pfCounts.as<int>();
+    const int* dAct = c->pfActive.as<const int>();
+    hipLaunchKernelGGL(k_phase_items, dim3(1), dim3(1024), 0, st, c->pfOrder.as<const int>(), dAct,
+                       P, mLR, mLT, r.nbp, c->locBlocks.as<int2>(), c->locBStart.as<int>(),
+                       c->pfNRa.as<int>(), c->pfNTa.as<int>(), dPh);
+    // rotations: mean of the current samples, perturbation, balance + calVari
+    hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, pb.q, mLR, dAct, 2, kAcgTol,
+                       kAcgMaxIter, nullptr, c->pfMean.as<double>(), nullptr, nullptr, nullptr,
+                       nullptr);
+    THX_TRY(pf_perturb_rot(st, pb.q, c->pfOffR.as<int>(), c->pfNRa.as<int>(), mLR,
+                           c->pfMean.as<double>(), c->pfK.as<double>(), 1.0,
+                           scale_dev(dPh, r.pfL, r.pfS), P, r.seed, phase_dev(dPh, 2, 0)));
+    hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, pb.q, mLR, dAct, 1 | 4 | 8, kAcgTol,
+                       kAcgMaxIter, nullptr, nullptr, c->pfK.as<double>(), c->pfWR.as<double>(),
+                       c->pfWRf.as<float>(), nullptr);
+    hipLaunchKernelGGL(k_sort_samples, dim3(P), dim3(256), 0, st, pb.q, c->pfWR.as<double>(),
+                       c->pfWRf.as<float>(), mLR, dAct);
+    // shifts: perturbation, re-centring, balance + calVari
+    hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, pb.tr, mLT, dAct, P, 1,
+                       c->pfS.as<double>(), scale_dev(dPh, r.pfL, r.pfS),
+                       c->pfTransS.as<const double>(), kTransQ, r.seed, phase_dev(dPh, 2, 1),
+                       nullptr, c->pfWT.as<double>(), c->pfWTf.as<float>());
+    THX_HIP(hipGetLastError());
+    // scoring of every active particle's samples (projection fused)
+    LocalBatch b;
+    b.vols = c->locVols.as<const float2* const>();
+    b.volsLo = r.precision != THX_FP32 ? c->locVolsLo.as<const uint32_t* const>() : nullptr;
+    b.pcls = r.pcls;
+    b.quat = pb.q;
+    b.pRf = c->pfWRf.as<const float>();
+    b.rOff = c->pfOffR.as<const int>();
+    b.nR = c->pfNRa.as<const int>();
+    b.trans = pb.tr;
+    b.pTf = c->pfWTf.as<const float>();
+    b.tOff = c->pfOffT.as<const int>();
+    b.nT = c->pfNTa.as<const int>();
+    b.blocks = c->locBlocks.as<const int2>();
+    b.bStart = c->locBStart.as<const int>();
+    b.bMax = c->locBMax.as<float>();
+    b.bWC = c->locBWC.as<float>();
+    b.bArg = c->locBArg.as<int>();
+    b.bWT = c->locBWT.as<float>();
+    b.wRrel = c->locWRrel.as<float>();
+    THX_TRY(launch_score_local(c, r.precision, r.TPAD, P * r.nbp, b, r.mode, dPh + 1));
+    THX_TRY(launch_merge_local(c, P, kLocalWtStride, b, c->locWC.as<double>(),
+                               c->locWR.as<double>(), c->locWT.as<double>(),
+                               c->locBase.as<double>(), c->locBestR.as<int>(),
+                               c->locBestT.as<int>()));
+    // half-height peak of the rotation marginal, top samples
+    hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWR.as<double>(), mLR, dAct,
+                       c->pfPeak.as<double>(), 0, 1, (const double*)pb.q, 4,
+                       c->pfTopR.as<double>(), nullptr);
+    hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWT.as<double>(), mLT, dAct,
+                       c->pfPeak.as<double>(), 0, 0, (const double*)pb.tr, 2,
+                       c->pfTopT.as<double>(), nullptr);
+    // resampling of both sets into q2 / tr2
+    THX_TRY(pf_resample(st, c->pfWR.as<double>(), c->locWR.as<double>(), c->pfOffR.as<int>(),
+                        c->pfNRa.as<int>(), c->pfOffR.as<int>(), c->pfNRa.as<int>(), P, r.seed,
+                        phase_dev(dPh, 2, 1000), c->pfIdx.as<int>(), c->pfWNew.as<double>()));
+    hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLR)), dim3(256), 0, st,
+                       (const double*)pb.q, mLR, 0, c->pfIdx.as<const int>(),
+                       c->pfWNew.as<const double>(), mLR, 4, dAct, P, pb.q2, c->pfWR.as<double>(),
+                       c->pfWRf.as<float>());
+    THX_TRY(pf_resample(st, c->pfWT.as<double>(), c->locWT.as<double>(), c->pfOffT.as<int>(),
+                        c->pfNTa.as<int>(), c->pfOffT.as<int>(), c->pfNTa.as<int>(), P, r.seed,
+                        phase_dev(dPh, 2, 1001), c->pfIdx.as<int>(), c->pfWNew.as<double>()));
+    hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLT)), dim3(256), 0, st,
+                       (const double*)pb.tr, mLT, 0, c->pfIdx.as<const int>(),
+                       c->pfWNew.as<const double>(), mLT, 2, dAct, P, pb.tr2, c->pfWT.as<double>(),
+                       c->pfWTf.as<float>());
+    // converged particles were gathered neither way: their rows are copied
+    hipLaunchKernelGGL(k_keep_rows, dim3(nb((size_t)P * mLR * 4)), dim3(256), 0, st,
+                       (const double*)pb.q, pb.q2, mLR * 4, dAct, P);
+    hipLaunchKernelGGL(k_keep_rows, dim3(nb((size_t)P * mLT * 2)), dim3(256), 0, st,
+                       (const double*)pb.tr, pb.tr2, mLT * 2, dAct, P);
+    hipLaunchKernelGGL(k_converge, dim3(nb(P)), dim3(256), 0, st, c->pfK.as<const double>(),
+                       c->pfS.as<const double>(), c->pfActive.as<int>(), c->pfBest.as<double>(),
+                       c->pfNPhase.as<int>(), P, phase_dev(dPh, 1, 0), r.minPhase);
+    hipLaunchKernelGGL(k_phase_end, dim3(1), dim3(1024), 0, st, dAct, P, dPh);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// The phase graph for this setup and buffer assignment: reused while every
+// pointer and scalar it holds is unchanged, else captured (at most four are
+// kept: the two buffer assignments of up to two setups).
+static int phase_graph(thx_ctx* c, const RefineSetup& r, const PhaseBufs& pb,
+                       hipGraphExec_t* out) {
+    std::vector<long long> key;
+    auto add = [&](const void* x) { key.push_back((long long)(uintptr_t)x); };
+    for (const void* x : std::initializer_list<const void*>{pb.q, pb.q2, pb.tr,
+                          pb.tr2, c->pfWR.p, c->pfWT.p, c->pfWRf.p, c->pfWTf.p,
+                          c->pfIdx.p, c->pfWNew.p, c->pfK.p, c->pfS.p, c->pfMean.p, c->pfTopR.p,
+                          c->pfTopT.p, c->pfPeak.p, c->pfTransS.p, c->pfBest.p, c->pfActive.p,
+                          c->pfNPhase.p, c->pfOffR.p, c->pfOffT.p, c->pfNRa.p, c->pfNTa.p,
+                          c->pfOrder.p, c->pfCounts.p, c->locBlocks.p, c->locBStart.p,
+                          c->locBMax.p, c->locBWC.p, c->locBArg.p, c->locBWT.p, c->locWRrel.p,
+                          c->locWC.p, c->locWR.p, c->locWT.p, c->locBase.p, c->locBestR.p,
+                          c->locBestT.p, c->locVols.p, c->locVolsLo.p, r.pcls,
+                          c->Y.p, c->Wt.p, c->Aterm.p, c->iCol.p, c->iRow.p, c->iSig.p,
+                          c->shellScale.p, c->locCounter.p})
+        add(x);
+    double d[2] = {r.pfL, r.pfS};
+    long long dd[2];
+    std::memcpy(dd, d, sizeof dd);
+    for (long long v : std::initializer_list<long long>{(long long)r.P, (long long)r.mLR, (long long)r.mLT, (long long)r.TPAD,
+                        (long long)r.precision, (long long)r.mode, (long long)r.minPhase,
+                        (long long)r.seed, dd[0], dd[1], (long long)c->K, (long long)c->Kp,
+                        (long long)c->idim, (long long)c->pf})
+        key.push_back(v);
+    for (auto& g : c->pfGraphs)
+        if (g.first == key) {
+            *out = g.second;
+            return THX_OK;
+        }
+    hipStream_t st = c->stream;
+    THX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int cs = enqueue_phase(c, r, pb);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &g);
+    if (cs != THX_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return cs;
+    }
+    THX_CHECK(ee == hipSuccess && g, THX_ERR_HIP, "pf_refine: graph capture failed");
+    hipGraphExec_t ge = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    THX_CHECK(ie == hipSuccess, THX_ERR_HIP, "pf_refine: graph instantiation failed");
+    if (c->pfGraphs.size() >= 4) {
+        (void)hipGraphExecDestroy(c->pfGraphs.front().second);
+        c->pfGraphs.erase(c->pfGraphs.begin());
+    }
+    c->pfGraphs.push_back({key, ge});
+    *out = ge;
+    return THX_OK;
+}
+
 int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, double pfL,
                   double pfS, int maxPhase, uint64_t seed, int* phasesRun) {
     clear_error();
@@ -932,12 +1169,26 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
         THX_TRY(upload(c->locVolsLo, vl.data(), vl.size(), st));
     }
     if (cls) THX_TRY(upload(c->locPcls, cls, P, st));
-    const int TPAD = mLT <= 32 ? 32 : (mLT <= 64 ? 64 : 128);
-    const int nbp = (mLR + kLocalRotBlock - 1) / kLocalRotBlock;
-    THX_TRY(c->locBMax.ensure(sizeof(float) * (size_t)P * nbp));
-    THX_TRY(c->locBWC.ensure(sizeof(float) * (size_t)P * nbp));
-    THX_TRY(c->locBArg.ensure(sizeof(int) * (size_t)P * nbp));
-    THX_TRY(c->locBWT.ensure(sizeof(float) * (size_t)P * nbp * kLocalWtStride));
+    RefineSetup r;
+    r.P = P;
+    r.mLR = mLR;
+    r.mLT = mLT;
+    r.TPAD = kLocalTile[local_tile_index(mLT)];
+    r.nbp = (mLR + kLocalRotBlock - 1) / kLocalRotBlock;
+    r.precision = precision;
+    r.minPhase = afterGlobal ? 10 : 3;   // MIN_N_PHASE_PER_ITER_{GLOBAL,LOCAL}
+    r.seed = seed;
+    r.pfL = pfL;
+    r.pfS = pfS;
+    r.pcls = cls ? c->locPcls.as<const int>() : nullptr;
+    const size_t nItemMax = (size_t)P * r.nbp;
+    THX_CHECK(nItemMax < (1u << 31), THX_ERR_RANGE, "pf_refine: too many work items");
+    THX_TRY(c->locBMax.ensure(sizeof(float) * nItemMax));
+    THX_TRY(c->locBWC.ensure(sizeof(float) * nItemMax));
+    THX_TRY(c->locBArg.ensure(sizeof(int) * nItemMax));
+    THX_TRY(c->locBWT.ensure(sizeof(float) * nItemMax * kLocalWtStride));
+    THX_TRY(c->locBlocks.ensure(sizeof(int2) * nItemMax));
+    THX_TRY(c->locBStart.ensure(sizeof(int) * (P + 1)));
     THX_TRY(c->locWRrel.ensure(sizeof(float) * (size_t)P * mLR));
     THX_TRY(c->locWC.ensure(sizeof(double) * P));
     THX_TRY(c->locWR.ensure(sizeof(double) * (size_t)P * mLR));
@@ -947,15 +1198,14 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     THX_TRY(c->locBestT.ensure(sizeof(int) * P));
     THX_TRY(c->pfNRa.ensure(sizeof(int) * P));
     THX_TRY(c->pfNTa.ensure(sizeof(int) * P));
+    THX_TRY(c->pfCounts.ensure(4 * sizeof(int)));
+    THX_TRY(c->pfCountsH.ensure(4 * sizeof(int)));
+    THX_TRY(c->locCounter.ensure(8 * sizeof(int)));   // no allocation inside a capture
     c->locValid = false;   // the loc* buffers now hold this search's phases
     // FP16: per-shell operand scales (particles and volumes stay fixed over the phases)
     if (precision == THX_FP16) THX_TRY(launch_local_shell_scale(c, used));
     THX_TRY(launch_prep_particles(c, precision == THX_FP16 ? THX_FP16 : THX_FP32));
     const int start = afterGlobal ? 1 : 0;
-    const int minPhase = afterGlobal ? 10 : 3;   // MIN_N_PHASE_PER_ITER_{GLOBAL,LOCAL}
-    std::vector<int> act(P), nRa(P), nTa(P), bStart(P + 1);
-    std::vector<int2> blocks;
-    THX_HIP(hipMemcpy(act.data(), c->pfActive.p, sizeof(int) * P, hipMemcpyDeviceToHost));
     // work-list order by pose (the particles move little over the phases):
     // the first sample of each particle
     std::vector<double> q0(4 * (size_t)P);
@@ -964,51 +1214,24 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
                         4 * sizeof(double), P, hipMemcpyDeviceToHost));
     for (int p = 0; p < P; p++) off0[p] = p;
     const std::vector<int> order = pose_order(q0.data(), off0.data(), cls, P);
+    THX_TRY(upload(c->pfOrder, order.data(), P, st));
+    // the first phase's work list, for the launch-shape tuner
+    int* cH = c->pfCountsH.as<int>();
+    int* dPh = c->pfCounts.as<int>();
+    hipLaunchKernelGGL(k_set_phase, dim3(1), dim3(64), 0, st, dPh, start);
+    hipLaunchKernelGGL(k_phase_items, dim3(1), dim3(1024), 0, st, c->pfOrder.as<const int>(),
+                       c->pfActive.as<const int>(), P, mLR, mLT, r.nbp, c->locBlocks.as<int2>(),
+                       c->locBStart.as<int>(), c->pfNRa.as<int>(), c->pfNTa.as<int>(), dPh);
+    THX_HIP(hipGetLastError());
+    THX_HIP(hipMemcpyAsync(cH, dPh, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    THX_HIP(hipStreamSynchronize(st));
     int phase = start, ran = 0;
-    for (; phase < maxPhase; phase++) {
-        int nAct = 0;
-        blocks.clear();
-        for (int p : order) {
-            bStart[p] = (int)blocks.size();
-            nRa[p] = act[p] ? mLR : 0;
-            nTa[p] = act[p] ? mLT : 0;
-            if (act[p]) {
-                nAct++;
-                for (int r0 = 0; r0 < mLR; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
-            }
-        }
-        bStart[P] = (int)blocks.size();
-        if (nAct == 0) break;
-        ran++;
-        const double pf = phase == 0 ? pfL : pfS;
-        const int* dAct = c->pfActive.as<const int>();
-        // rotations: mean of the current samples, perturbation, balance + calVari
-        hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
-                           dAct, 2, kAcgTol, kAcgMaxIter, nullptr, c->pfMean.as<double>(),
-                           nullptr, nullptr, nullptr, nullptr);
-        THX_HIP(hipMemcpyAsync(c->pfNRa.p, nRa.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
-        THX_HIP(hipMemcpyAsync(c->pfNTa.p, nTa.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
-        THX_TRY(pf_perturb_rot(st, c->pfQ.as<double>(), c->pfOffR.as<int>(), c->pfNRa.as<int>(),
-                               mLR, c->pfMean.as<double>(), c->pfK.as<double>(), 1.0, pf, P, seed,
-                               (uint32_t)(2 * phase), nullptr));
-        hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
-                           dAct, 1 | 4 | 8, kAcgTol, kAcgMaxIter, nullptr, nullptr,
-                           c->pfK.as<double>(), c->pfWR.as<double>(), c->pfWRf.as<float>(), nullptr);
-        hipLaunchKernelGGL(k_sort_samples, dim3(P), dim3(256), 0, st, c->pfQ.as<double>(),
-                           c->pfWR.as<double>(), c->pfWRf.as<float>(), mLR, dAct);
-        // shifts: perturbation, re-centring, balance + calVari
-        hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, c->pfTr.as<double>(), mLT,
-                           dAct, P, 1, c->pfS.as<double>(), pf, c->pfTransS.as<const double>(),
-                           kTransQ, seed, (uint32_t)(2 * phase + 1), nullptr,
-                           c->pfWT.as<double>(), c->pfWTf.as<float>());
-        THX_HIP(hipGetLastError());
-        // scoring of every active particle's samples (projection fused)
-        THX_TRY(upload(c->locBlocks, blocks.data(), blocks.size(), st));
-        THX_TRY(upload(c->locBStart, bStart.data(), P + 1, st));
+    if (cH[2] > 0 && phase < maxPhase) {
+        // the first phase's current samples, scored by both launch shapes
         LocalBatch b;
         b.vols = c->locVols.as<const float2* const>();
         b.volsLo = precision != THX_FP32 ? c->locVolsLo.as<const uint32_t* const>() : nullptr;
-        b.pcls = cls ? c->locPcls.as<const int>() : nullptr;
+        b.pcls = r.pcls;
         b.quat = c->pfQ.as<const double>();
         b.pRf = c->pfWRf.as<const float>();
         b.rOff = c->pfOffR.as<const int>();
@@ -1024,51 +1247,35 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
         b.bArg = c->locBArg.as<int>();
         b.bWT = c->locBWT.as<float>();
         b.wRrel = c->locWRrel.as<float>();
-        const int mode = choose_local_mode(c, precision, TPAD, (int)blocks.size(), b);
-        if (mode < 0) return THX_ERR_HIP;
-        c->locModeUsed = mode;
-        THX_TRY(launch_score_local(c, precision, TPAD, (int)blocks.size(), b, mode));
-        THX_TRY(launch_merge_local(c, P, kLocalWtStride, b, c->locWC.as<double>(), c->locWR.as<double>(),
-                                   c->locWT.as<double>(), c->locBase.as<double>(),
-                                   c->locBestR.as<int>(), c->locBestT.as<int>()));
-        // half-height peak of the rotation marginal, top samples
-        hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWR.as<double>(), mLR,
-                           dAct, c->pfPeak.as<double>(), 0, 1, c->pfQ.as<const double>(), 4,
-                           c->pfTopR.as<double>(), nullptr);
-        hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWT.as<double>(), mLT,
-                           dAct, c->pfPeak.as<double>(), 0, 0, c->pfTr.as<const double>(), 2,
-                           c->pfTopT.as<double>(), nullptr);
-        // resampling of both sets
-        THX_TRY(pf_resample(st, c->pfWR.as<double>(), c->locWR.as<double>(), c->pfOffR.as<int>(),
-                            c->pfNRa.as<int>(), c->pfOffR.as<int>(), c->pfNRa.as<int>(), P, seed,
-                            (uint32_t)(1000 + 2 * phase), nullptr, c->pfIdx.as<int>(),
-                            c->pfWNew.as<double>(), nullptr));
-        hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLR)), dim3(256), 0, st,
-                           c->pfQ.as<const double>(), mLR, 0, c->pfIdx.as<const int>(),
-                           c->pfWNew.as<const double>(), mLR, 4, dAct, P, c->pfQ2.as<double>(),
-                           c->pfWR.as<double>(), c->pfWRf.as<float>());
-        THX_TRY(pf_resample(st, c->pfWT.as<double>(), c->locWT.as<double>(), c->pfOffT.as<int>(),
-                            c->pfNTa.as<int>(), c->pfOffT.as<int>(), c->pfNTa.as<int>(), P, seed,
-                            (uint32_t)(1001 + 2 * phase), nullptr, c->pfIdx.as<int>(),
-                            c->pfWNew.as<double>(), nullptr));
-        hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLT)), dim3(256), 0, st,
-                           c->pfTr.as<const double>(), mLT, 0, c->pfIdx.as<const int>(),
-                           c->pfWNew.as<const double>(), mLT, 2, dAct, P, c->pfTr2.as<double>(),
-                           c->pfWT.as<double>(), c->pfWTf.as<float>());
-        // the gathered sets become current (converged particles were copied
-        // neither way: keep their rows by copying them back first)
-        hipLaunchKernelGGL(k_keep_rows, dim3(nb((size_t)P * mLR * 4)), dim3(256), 0, st,
-                           c->pfQ.as<const double>(), c->pfQ2.as<double>(), mLR * 4, dAct, P);
-        hipLaunchKernelGGL(k_keep_rows, dim3(nb((size_t)P * mLT * 2)), dim3(256), 0, st,
-                           c->pfTr.as<const double>(), c->pfTr2.as<double>(), mLT * 2, dAct, P);
+        r.mode = choose_local_mode(c, precision, r.TPAD, cH[1], b);
+        if (r.mode < 0) return THX_ERR_HIP;
+        c->locModeUsed = r.mode;
+    }
+    // the phases: the same launches every time, replayed as a graph when
+    // graphs are on; the sample buffers swap roles every phase
+    for (; phase < maxPhase && cH[2] > 0; phase++) {
+        const bool odd = ((phase - start) & 1) != 0;
+        PhaseBufs pb;
+        pb.q = odd ? c->pfQ2.as<double>() : c->pfQ.as<double>();
+        pb.q2 = odd ? c->pfQ.as<double>() : c->pfQ2.as<double>();
+        pb.tr = odd ? c->pfTr2.as<double>() : c->pfTr.as<double>();
+        pb.tr2 = odd ? c->pfTr.as<double>() : c->pfTr2.as<double>();
+        if (c->locGraphOn) {
+            hipGraphExec_t g = nullptr;
+            THX_TRY(phase_graph(c, r, pb, &g));
+            THX_HIP(hipGraphLaunch(g, st));
+        } else {
+            THX_TRY(enqueue_phase(c, r, pb));
+        }
+        THX_HIP(hipMemcpyAsync(cH, dPh, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+        THX_HIP(hipStreamSynchronize(st));
+        ran++;
+        cH[2] = cH[3];   // the next phase's active particles
+    }
+    // the current sample sets are in pfQ / pfTr again
+    if (ran & 1) {
         std::swap(c->pfQ, c->pfQ2);
         std::swap(c->pfTr, c->pfTr2);
-        hipLaunchKernelGGL(k_converge, dim3(nb(P)), dim3(256), 0, st, c->pfK.as<const double>(),
-                           c->pfS.as<const double>(), c->pfActive.as<int>(), c->pfBest.as<double>(),
-                           c->pfNPhase.as<int>(), P, phase, minPhase);
-        THX_HIP(hipGetLastError());
-        THX_HIP(hipMemcpyAsync(act.data(), c->pfActive.p, sizeof(int) * P, hipMemcpyDeviceToHost, st));
-        THX_HIP(hipStreamSynchronize(st));
     }
     if (phasesRun) *phasesRun = ran;
     c->pfLastPhase = phase;
@@ -1147,8 +1354,8 @@ int thx_pf_trans_stats(thx_ctx* c, const double* trans, int n, int P, double* m,
         (s = ds.ensure(sizeof(double) * 3 * P)) || (s = dw.ensure(sizeof(double) * N)))
         return s;
     hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, dt.as<double>(), n, nullptr,
-                       P, 0, ds.as<double>(), 0.0, nullptr, kTransQ, 0ull, 0u, dm.as<double>(),
-                       dw.as<double>(), nullptr);
+                       P, 0, ds.as<double>(), scale_host(0.0), nullptr, kTransQ, 0ull,
+                       phase_host(0u), dm.as<double>(), dw.as<double>(), nullptr);
     THX_HIP(hipGetLastError());
     THX_HIP(hipStreamSynchronize(st));
     std::vector<double> s3(3 * (size_t)P);
@@ -1192,7 +1399,8 @@ int thx_pf_converge(thx_ctx* c, const double* k, const double* s01, int nPh, int
         hipLaunchKernelGGL(k_converge, dim3(nb(P)), dim3(256), 0, st,
                            dk.as<const double>() + 3 * (size_t)P * i,
                            ds.as<const double>() + 3 * (size_t)P * i, da.as<int>(),
-                           db.as<double>(), dn.as<int>(), P, phase0 + i, minPhase);
+                           db.as<double>(), dn.as<int>(), P, phase_host((uint32_t)(phase0 + i)),
+                           minPhase);
     THX_HIP(hipGetLastError());
     THX_HIP(hipStreamSynchronize(st));
     THX_HIP(hipMemcpy(nPhase, dn.p, sizeof(int) * P, hipMemcpyDeviceToHost));

@@ -217,6 +217,13 @@ struct thx_ctx {
     thx::DevBuf pfQ, pfQ2, pfTr, pfTr2, pfWR, pfWT, pfWRf, pfWTf, pfIdx, pfWNew, pfK, pfS,
         pfMean, pfTopR, pfTopT, pfPeak, pfTransS, pfBest, pfActive, pfNPhase, pfOffR, pfOffT,
         pfNR, pfNT, pfNRa, pfNTa;
+    // device work list of the phases: particle order, phase counter and counts
+    // ([0] phase, [1] work items, [2] active at the start, [3] active at the
+    // end), their pinned read-back, and the captured phase graphs (one per
+    // assignment of the sample buffers, which swap every phase)
+    thx::DevBuf pfOrder, pfCounts;
+    thx::HostBuf pfCountsH;
+    std::vector<std::pair<std::vector<long long>, hipGraphExec_t>> pfGraphs;
     // timing
     hipEvent_t ev[16] = {};
     int nev = 0;
@@ -268,8 +275,9 @@ inline LocalBatch local_batch_range(const LocalBatch& b, int start) {
 // pulling items from a counter
 constexpr int kLocalDirect = 1;
 constexpr int kLocalQueue = 2;
+// nItemsDev: the item count in device memory (nblk then bounds it)
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
-                       int mode = kLocalDirect);
+                       int mode = kLocalDirect, const int* nItemsDev = nullptr);
 // particle order for the local scorer's work list (pose locality)
 std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P,
                             const int* cnt = nullptr);
@@ -313,10 +321,37 @@ int launch_merge_local(thx_ctx* c, int P, int wtStride, const LocalBatch& b, dou
                        double* wT, double* base, int* bestR, int* bestT,
                        int rb = kLocalRotBlock);
 
+// Per-phase scalars of the local search that a replayed graph reads from
+// device memory.  With `dev` set, PhaseArg is mul * (*dev) + add and
+// PhaseScale is `first` at phase 0 and `rest` after (the phase counter the
+// phase's last kernel advances); otherwise both are their `host` value.
+struct PhaseArg {
+    const int* dev = nullptr;
+    int mul = 0, add = 0;
+    uint32_t host = 0;
+    __device__ uint32_t get() const { return dev ? (uint32_t)(mul * *dev + add) : host; }
+};
+struct PhaseScale {
+    const int* dev = nullptr;
+    double first = 0.0, rest = 0.0, host = 0.0;
+    __device__ double get() const { return dev ? (*dev == 0 ? first : rest) : host; }
+};
+inline PhaseArg phase_host(uint32_t v) { PhaseArg a; a.host = v; return a; }
+inline PhaseArg phase_dev(const int* d, int mul, int add) {
+    PhaseArg a; a.dev = d; a.mul = mul; a.add = add; return a;
+}
+inline PhaseScale scale_host(double v) { PhaseScale a; a.host = v; return a; }
+inline PhaseScale scale_dev(const int* d, double first, double rest) {
+    PhaseScale a; a.dev = d; a.first = first; a.rest = rest; return a;
+}
+
 // Particle-filter launchers (thx_pf.hip).
 int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
                    const double* mean, const double* kk, double kmax, double pf, int P,
                    uint64_t seed, uint32_t phase, const double* dGiven);
+int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
+                   const double* mean, const double* kk, double kmax, PhaseScale pf, int P,
+                   uint64_t seed, PhaseArg phase);
 int pf_perturb_trans(hipStream_t st, double* trans, const int* off, const int* cnt, int maxCnt,
                      const double* s01rho, double pf, int P, uint64_t seed, uint32_t phase);
 int pf_sample_mixture(hipStream_t st, const double* mix, int C, int P, int nRot, int nTr,
@@ -326,6 +361,9 @@ int pf_resample(hipStream_t st, const double* w, const double* u, const int* off
                 const int* nIn, const int* offOut, const int* nOut, int P, uint64_t seed,
                 uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess,
                 int wShared = 0);
+int pf_resample(hipStream_t st, const double* w, const double* u, const int* offIn,
+                const int* nIn, const int* offOut, const int* nOut, int P, uint64_t seed,
+                PhaseArg phase, int* idx, double* wNew);
 int pf_rot_mean(hipStream_t st, const double* quat, const double* w, const int* off,
                 const int* cnt, int P, double* mean, double* kk);
 constexpr int kResampleMaxN = 8192;

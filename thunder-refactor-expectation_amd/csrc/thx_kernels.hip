@@ -1912,10 +1912,14 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
     const double* __restrict__ trans, const float* __restrict__ pTf,
     const int* __restrict__ tOff, const int* __restrict__ nT, const int2* __restrict__ blocks,
     float* __restrict__ bMax, float* __restrict__ bWC, int* __restrict__ bArg,
-    float* __restrict__ bWT, float* __restrict__ wRrel) {
+    float* __restrict__ bWT, float* __restrict__ wRrel, const int* __restrict__ nDev) {
+    // nDev: the item count in device memory (a list built on the device; the
+    // grid is its upper bound), else one block per item
+    const int n = nDev ? *nDev : (int)gridDim.x;
+    if ((int)blockIdx.x >= n) return;
     // the list is in pose order: each XCD walks a contiguous range of it, so
     // concurrent blocks of one XCD gather from overlapping volume regions
-    score_local_item<PREC, TPAD>(xcd_logical(blockIdx.x, gridDim.x), vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+    score_local_item<PREC, TPAD>(xcd_logical(blockIdx.x, n), vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
 }
 
 // Persistent launch: a grid sized to the resident capacity pulls work items
@@ -1935,9 +1939,10 @@ __global__ __launch_bounds__(256, 2) void k_score_local_q(
     const double* __restrict__ trans, const float* __restrict__ pTf,
     const int* __restrict__ tOff, const int* __restrict__ nT, const int2* __restrict__ blocks,
     float* __restrict__ bMax, float* __restrict__ bWC, int* __restrict__ bArg,
-    float* __restrict__ bWT, float* __restrict__ wRrel, int nItems,
-    int* __restrict__ counter) {
+    float* __restrict__ bWT, float* __restrict__ wRrel, int nItemsArg,
+    int* __restrict__ counter, const int* __restrict__ nDev) {
     __shared__ int item;
+    const int nItems = nDev ? *nDev : nItemsArg;   // nDev: count in device memory
     const int q = nItems / 8, rmd = nItems % 8;
     int h = 0, x = blockIdx.x % 8;
     for (;;) {
@@ -1971,7 +1976,7 @@ static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int 
                               const double* quat, const float* pRf, const int* rOff,
                               const int* nR, const double* trans, const float* pTf,
                               const int* tOff, const int* nT, const int2* blocks, float* bMax,
-                              float* bWC, int* bArg, float* bWT, float* wRrel) {
+                              float* bWC, int* bArg, float* bWT, float* wRrel, const int* nDev) {
 #define THX_LOCAL_ARGS vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, \
                        quat, pRf, rOff, \
                        nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel
@@ -1979,18 +1984,18 @@ static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int 
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, counter);
         const int g = std::min(grid, nblk);
         if (TPAD == 32)
-            hipLaunchKernelGGL((k_score_local_q<PREC, 32>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter);
+            hipLaunchKernelGGL((k_score_local_q<PREC, 32>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
         else if (TPAD == 64)
-            hipLaunchKernelGGL((k_score_local_q<PREC, 64>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter);
+            hipLaunchKernelGGL((k_score_local_q<PREC, 64>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
         else
-            hipLaunchKernelGGL((k_score_local_q<PREC, 128>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter);
+            hipLaunchKernelGGL((k_score_local_q<PREC, 128>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
     } else {
         if (TPAD == 32)
-            hipLaunchKernelGGL((k_score_local<PREC, 32>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+            hipLaunchKernelGGL((k_score_local<PREC, 32>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
         else if (TPAD == 64)
-            hipLaunchKernelGGL((k_score_local<PREC, 64>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+            hipLaunchKernelGGL((k_score_local<PREC, 64>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
         else
-            hipLaunchKernelGGL((k_score_local<PREC, 128>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS);
+            hipLaunchKernelGGL((k_score_local<PREC, 128>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
     }
 #undef THX_LOCAL_ARGS
 }
@@ -2020,7 +2025,7 @@ static int local_queue_grid(int precision, int TPAD) {
 }
 
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
-                       int mode) {
+                       int mode, const int* nItemsDev) {
     THX_CHECK(precision == THX_FP32 || precision == THX_BF16 || precision == THX_FP16,
               THX_ERR_ARG, "local scorer supports FP32, BF16 and FP16");
     THX_CHECK(precision == THX_FP32 || b.volsLo, THX_ERR_STATE,
@@ -2044,7 +2049,8 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
                           c->Kp, c->idim,                                                      \
                           c->Y.as<const float2>(), c->Wt.as<const float>(),                    \
                           c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR, b.trans,   \
-                          b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel)
+                          b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel, \
+                          nItemsDev)
     if (precision == THX_FP32) THX_LOCAL_CALL(THX_FP32);
     else if (precision == THX_BF16) THX_LOCAL_CALL(THX_BF16);
     else THX_LOCAL_CALL(THX_FP16);

@@ -23,11 +23,13 @@ namespace thx {
 // ---------------------------------------------------------------------------
 __global__ void k_perturb_rot(double* __restrict__ quat, const int* __restrict__ off,
                               const int* __restrict__ cnt, const double* __restrict__ mean,
-                              const double* __restrict__ kk, double kmax, double pf, int P,
-                              uint64_t seed, uint32_t phase, const double* __restrict__ dGiven) {
+                              const double* __restrict__ kk, double kmax, PhaseScale pfs, int P,
+                              uint64_t seed, PhaseArg pha, const double* __restrict__ dGiven) {
     const int p = blockIdx.x;   // particles on x, the grid dimension with the largest limit
     const int j = blockIdx.y * blockDim.x + threadIdx.x;
     if (p >= P || j >= cnt[p]) return;
+    const double pf = pfs.get();
+    const uint32_t phase = pha.get();
     const size_t i = (size_t)off[p] + j;
     double d[4];
     if (dGiven) {
@@ -139,10 +141,11 @@ constexpr int kResampleMax = 8192;
 __global__ __launch_bounds__(256) void k_resample(
     const double* __restrict__ w, const double* __restrict__ u, const int* __restrict__ offIn,
     const int* __restrict__ nIn, const int* __restrict__ offOut, const int* __restrict__ nOut,
-    int P, uint64_t seed, uint32_t phase, const double* __restrict__ u0Given,
+    int P, uint64_t seed, PhaseArg pha, const double* __restrict__ u0Given,
     int* __restrict__ idx, double* __restrict__ wNew, double* __restrict__ essOut, int wShared) {
     const int p = blockIdx.x;
     if (p >= P) return;
+    const uint32_t phase = pha.get();
     const int n = nIn[p], m = nOut[p];
     if (n <= 0 || m <= 0 || n > kResampleMax) {   // no samples, or more than the LDS holds
         if (threadIdx.x == 0 && essOut) essOut[p] = -1.0;
@@ -491,8 +494,20 @@ int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt,
                    const double* mean, const double* kk, double kmax, double pf, int P,
                    uint64_t seed, uint32_t phase, const double* dGiven) {
     dim3 grid(P, (maxCnt + 255) / 256);
+    hipLaunchKernelGGL(k_perturb_rot, grid, dim3(256), 0, st, quat, off, cnt, mean, kk, kmax,
+                       scale_host(pf), P, seed, phase_host(phase), dGiven);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+// the same with the perturbation scale and the random-stream phase read from
+// device memory by the kernel (a graph replayed over the local-search phases)
+int pf_perturb_rot(hipStream_t st, double* quat, const int* off, const int* cnt, int maxCnt,
+                   const double* mean, const double* kk, double kmax, PhaseScale pf, int P,
+                   uint64_t seed, PhaseArg phase) {
+    dim3 grid(P, (maxCnt + 255) / 256);
     hipLaunchKernelGGL(k_perturb_rot, grid, dim3(256), 0, st, quat, off, cnt, mean, kk, kmax, pf,
-                       P, seed, phase, dGiven);
+                       P, seed, phase, (const double*)nullptr);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
@@ -522,7 +537,16 @@ int pf_resample(hipStream_t st, const double* w, const double* u, const int* off
                 uint32_t phase, const double* u0Given, int* idx, double* wNew, double* ess,
                 int wShared) {
     hipLaunchKernelGGL(k_resample, dim3(P), dim3(256), 0, st, w, u, offIn, nIn, offOut, nOut, P,
-                       seed, phase, u0Given, idx, wNew, ess, wShared);
+                       seed, phase_host(phase), u0Given, idx, wNew, ess, wShared);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+int pf_resample(hipStream_t st, const double* w, const double* u, const int* offIn,
+                const int* nIn, const int* offOut, const int* nOut, int P, uint64_t seed,
+                PhaseArg phase, int* idx, double* wNew) {
+    hipLaunchKernelGGL(k_resample, dim3(P), dim3(256), 0, st, w, u, offIn, nIn, offOut, nOut, P,
+                       seed, phase, (const double*)nullptr, idx, wNew, (double*)nullptr, 0);
     THX_HIP(hipGetLastError());
     return THX_OK;
 }
