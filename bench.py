@@ -122,6 +122,100 @@ def run_learned(ctx, net, cal, steps, warmup, P, R, T):
                 kernels_ms=kern, fallback=float(np.mean(nfs)))
 
 
+def run_learned_finetune(name, net, cal, step, sigma_c=0.5, sigma_a=0.1,
+                         budget_m=(0.5, 1, 2, 3, 4, 6, 8), lr=2e-4, chunk=8192, steps=2):
+    """What the learned step costs when the map changes between iterations.
+    The next map is the reference map with every blob centre moved by
+    N(0, sigma_c^2) voxels per axis and every amplitude scaled by
+    1 + N(0, sigma_a^2) (the kind of change one refinement iteration makes);
+    the particles are made from it.  The network trained on the previous map
+    is fine-tuned from its weights on particles of the new map in increments
+    (budget_m, millions of samples) until the learned step's assignments agree
+    with the plain FP32 scan on >= 99% of the particles, with the calibrated
+    selection and fallback rule unchanged.  per_iteration_ms = the learned
+    step plus the fine-tuning it needed, the whole cost per refinement
+    iteration."""
+    w = wl.WORKLOADS[name]
+    sd = wl.seeds(name)
+    iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
+    ectx = thx.Context(0)
+    tctx = thx.Context(0)
+    for c in (ectx, tctx):
+        c.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    cen = ectx.synth_volume(0, w["box"], w["pf"], w["n_blob"], w["blob_sigma"],
+                            w["blob_radius"], sd["volume"])
+    rng = np.random.default_rng(4242)
+    cenB = cen + rng.normal(0.0, sigma_c, cen.shape)
+    ampB = 1.0 + rng.normal(0.0, sigma_a, len(cen))
+    for c in (ectx, tctx):
+        c.synth_volume_blobs(0, w["box"], w["pf"], cenB, w["blob_sigma"], ampB)
+    P = w["n_particles"]
+    ectx.synth_particles(0, P, w["max_shift"], w["df_min"], w["df_max"], w["snr"],
+                         sd["particles"] + 31, **wl.CTF_DEFAULTS)
+    quat, trans = wl.grid(name)
+    ectx.set_grid(quat, trans)
+    ectx.score_global(0, thx.FP32)
+    gR, gT, _ = ectx.get_best(0)
+    args = (net, 0, cal["n_rot"], cal["n_trans"], 0.0, cal["spread_max"], cal["tail_max"],
+            cal["score_drop"], thx.PRECISIONS[cal["fallback"]])
+
+    def evaluate():
+        ectx.expect_learned(*args)
+        ectx.sync()
+        t0 = time.perf_counter()
+        nfs = [ectx.expect_learned(*args) for _ in range(steps)]
+        ectx.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        bR, bT = ectx.get_learned()[:2]
+        return {"agreement_with_plain_fp32": agreement(quat, trans, step, bR, bT, gR, gT),
+                "fallback_particles": float(np.mean(nfs)), "ms_per_step": ms}
+
+    before = evaluate()
+    log(f"finetune: previous map's network on the new map: {before}")
+    curve, trained, t_train = [], 0, 0.0
+    if before["agreement_with_plain_fp32"]["within_one_step"] >= 0.99:
+        budget_m = ()   # holds without fine-tuning
+    for m in budget_m:
+        n_new = int(m * 2 ** 20) - trained
+        t0 = time.perf_counter()
+        for ch in range(max(1, n_new // chunk)):
+            q, t, _ = tctx.synth_particles(0, chunk, w["max_shift"], w["df_min"], w["df_max"],
+                                           float(rng.uniform(0.01, 0.1)), 90000 + trained // chunk + ch,
+                                           **wl.CTF_DEFAULTS)
+            net.train_ctx(tctx, q, t, 512, 1, lr)
+        tctx.sync()
+        t_train += time.perf_counter() - t0
+        trained += max(1, n_new // chunk) * chunk
+        e = evaluate()
+        e.update(samples=trained, training_seconds=t_train)
+        curve.append(e)
+        log(f"finetune: {e}")
+        if e["agreement_with_plain_fp32"]["within_one_step"] >= 0.99:
+            break
+    ectx.close()
+    tctx.close()
+    last = curve[-1] if curve else dict(before, samples=0, training_seconds=0.0)
+    return {"map_change": f"blob centres + N(0, {sigma_c}^2) vox per axis (blob sigma "
+                          f"{w['blob_sigma']} vox), amplitudes x (1 + N(0, {sigma_a}^2)); "
+                          f"{P} particles of the new map",
+            "previous_network_on_new_map": before,
+            "fine_tune": {"lr": lr, "batch": 512, "curve": curve},
+            "fine_tune_samples": last["samples"], "fine_tune_seconds": last["training_seconds"],
+            "agreement_with_plain_fp32": last["agreement_with_plain_fp32"],
+            "per_iteration_ms": last["ms_per_step"] + 1e3 * last["training_seconds"]}
+
+
+def run_learned_map_changes(name, net, cal, step, changes=((0.5, 0.1), (1.0, 0.2), (2.0, 0.3))):
+    """run_learned_finetune for map changes of growing size, each from the
+    weights the previous one left (as successive refinement iterations would
+    fine-tune them)."""
+    out = []
+    for sc, sa in changes:
+        out.append(run_learned_finetune(name, net, cal, step, sigma_c=sc, sigma_a=sa,
+                                        budget_m=(0.125, 0.25, 0.5, 1, 2, 4)))
+    return out
+
+
 def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
     """Particle states from the last global scan, then `phases` local-search
     phases (perturb, balance, BF16 scoring with fused projection, peak,
@@ -643,6 +737,8 @@ def main():
                     help="skip the full-size box-256 FP8 global scan")
     ap.add_argument("--no-box192", action="store_true",
                     help="skip the full-size box-192 learned-proposal workload")
+    ap.add_argument("--no-finetune", action="store_true",
+                    help="skip the changed-map fine-tuning measurement of the learned step")
     ap.add_argument("--no-update", action="store_true",
                     help="skip the update entry (sigma / scale / norm refresh and insert)")
     ap.add_argument("--cpu-only", action="store_true",
@@ -754,6 +850,16 @@ def main():
                           "fallback_particles": r["fallback"], "training": train,
                           "calibration": cal, "kernels_ms": r["kernels_ms"],
                           "sampling_step_deg": float(np.degrees(step))}
+        if not args.no_finetune:
+            log("learned step on a changed map (fine-tuning)")
+            fts = run_learned_map_changes(name, net, cal, step)
+            for ft in fts:
+                ft["per_iteration_speedup_vs_plain_fp32"] = f["ms_per_step"] / ft["per_iteration_ms"]
+                log(f"{ft['map_change']}: fine-tuning {ft['fine_tune_samples']} samples, "
+                    f"{ft['fine_tune_seconds']:.1f} s, {ft['per_iteration_ms']:.0f} ms per iteration")
+            out["learned"]["changed_maps"] = fts
+            # the first change is the size one refinement iteration makes
+            out["learned"]["per_iteration_ms"] = fts[0]["per_iteration_ms"]
         net.close()
     log("local search")
     out["local_search"] = run_local_search(ctx, info, name, args.local_phases)

@@ -591,6 +591,12 @@ int thx_ctx_get_learned(thx_ctx* ctx, int* bestR, int* bestT, double* score,
 int thx_ctx_synth_volume(thx_ctx* ctx, int cls, int idim, int pf, int nBlob,
                          double sigma, double rBall, uint64_t seed,
                          double* centresOut);
+/* The same from given blob centres centres[nBlob*3] (voxels from the box
+ * centre) and amplitudes amps[nBlob] (NULL: all 1), e.g. a map perturbed as
+ * one refinement iteration would change it. */
+int thx_ctx_synth_volume_blobs(thx_ctx* ctx, int cls, int idim, int pf, int nBlob,
+                               double sigma, const double* centres,
+                               const double* amps);
 
 /* Particle images X = CTF (.) S_t (.) P_r + complex white noise at the
  * given real-space SNR, at the context's pixel set, from class cls.

@@ -81,6 +81,30 @@ def test_synth_volume_matches_closed_form(gpu_ctx, small):
     assert np.abs(got - ref).max() <= 2e-5 * scale
 
 
+def test_synth_volume_blobs_matches_closed_form():
+    """Blob map from given centres and amplitudes (the changed map of the
+    fine-tuning measurement) against the oracle's closed-form transform, and
+    the seeded map equals the same centres given explicitly, bit for bit."""
+    w = wl.WORKLOADS["box64"]
+    rng = np.random.default_rng(11)
+    cen = rng.uniform(-20, 20, size=(30, 3))
+    amp = rng.uniform(0.5, 1.5, 30)
+    with thx.Context(0) as c:
+        iCol, iRow, iSig = wl.pixel_set(BOX, w["cutoff"])
+        c.set_pixels(iCol, iRow, iSig, BOX, PF)
+        c.synth_volume_blobs(0, BOX, PF, cen, w["blob_sigma"], amp)
+        vol = c.get_volume(0, V)
+        seeded = c.synth_volume(1, BOX, PF, w["n_blob"], w["blob_sigma"], w["blob_radius"], 77)
+        c.synth_volume_blobs(2, BOX, PF, seeded, w["blob_sigma"])
+        np.testing.assert_array_equal(c.get_volume(1, V), c.get_volume(2, V))
+    ix = rng.integers(0, V // 2 + 1, 300)
+    iy = rng.integers(-V // 2, V // 2, 300)
+    iz = rng.integers(-V // 2, V // 2, 300)
+    ref = oracle.blob_ft(V, BOX, cen, amp, w["blob_sigma"], ix, iy, iz)
+    got = vol[iz % V, iy % V, ix]
+    assert np.abs(got - ref).max() <= 2e-5 * np.abs(ref).max()
+
+
 def test_synth_ctf_matches_oracle(gpu_ctx, small):
     cp = small["ctf_params"]
     c = wl.CTF_DEFAULTS

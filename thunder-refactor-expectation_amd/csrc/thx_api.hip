@@ -1420,6 +1420,18 @@ int thx_ctx_synth_volume(thx_ctx* c, int cls, int idim, int pf, int nBlob, doubl
         for (int a = 0; a < 3; a++) cen[3 * b + a] = g[a] / nrm * rad;
     }
     if (centresOut) std::memcpy(centresOut, cen.data(), sizeof(double) * cen.size());
+    return thx_ctx_synth_volume_blobs(c, cls, idim, pf, nBlob, sigma, cen.data(), nullptr);
+}
+
+int thx_ctx_synth_volume_blobs(thx_ctx* c, int cls, int idim, int pf, int nBlob, double sigma,
+                               const double* centres, const double* amps) {
+    clear_error();
+    THX_CHECK(c && cls >= 0 && idim > 0 && idim % 2 == 0 && pf > 0 && nBlob > 0 && sigma > 0 &&
+                  centres,
+              THX_ERR_ARG, "synth_volume: bad argument");
+    THX_TRY(set_device(c));
+    std::vector<double> cen(centres, centres + 3 * (size_t)nBlob);
+    if (amps) cen.insert(cen.end(), amps, amps + nBlob);
     const int V = idim * pf;
     if ((int)c->vol.size() <= cls) {
         c->vol.resize(cls + 1);
@@ -1431,7 +1443,8 @@ int thx_ctx_synth_volume(thx_ctx* c, int cls, int idim, int pf, int nBlob, doubl
     DevBuf dc;
     THX_TRY(dc.ensure(sizeof(double) * cen.size()));
     THX_HIP(hipMemcpy(dc.p, cen.data(), sizeof(double) * cen.size(), hipMemcpyHostToDevice));
-    int s = launch_synth_volume(c, cls, idim, pf, dc.as<const double>(), nBlob, sigma);
+    int s = launch_synth_volume(c, cls, idim, pf, dc.as<const double>(), nBlob, sigma,
+                                amps ? dc.as<const double>() + 3 * (size_t)nBlob : nullptr);
     hipError_t e = hipStreamSynchronize(c->stream);
     dc.release();
     if (s != THX_OK) return s;

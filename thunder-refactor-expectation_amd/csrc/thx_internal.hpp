@@ -247,7 +247,8 @@ int launch_score(thx_ctx* c, int precision);
 int score_tile_cols(int precision);
 int launch_merge(thx_ctx* c, int cls);
 int launch_synth_volume(thx_ctx* c, int cls, int idim, int pf,
-                        const double* dCentres, int nBlob, double sigma);
+                        const double* dCentres, int nBlob, double sigma,
+                        const double* dAmps = nullptr);
 int launch_synth_signal(thx_ctx* c, int cls, int P, const double* dParams,
                         double* dEnergy, double voltage, double Cs,
                         double ampContrast, double pixelSize);

@@ -2491,12 +2491,13 @@ int launch_score_one(thx_ctx* c, int cls, int p, float* dOut) {
 // Analytic FT of the blob sum on the padded half-complex grid:
 // F(m) = (2 pi)^{3/2} s^3 exp(-2 pi^2 s^2 |m|^2 / V^2) sum_b exp(-2 pi i m.c_b / V).
 __global__ __launch_bounds__(256) void k_synth_volume(
-    const double* __restrict__ centres, int nBlob, double sigma, int V,
-    float2* __restrict__ vol) {
+    const double* __restrict__ centres, const double* __restrict__ amps, int nBlob, double sigma,
+    int V, float2* __restrict__ vol) {
     const int nColFT = V / 2 + 1;
     const size_t n = (size_t)nColFT * V * V;
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ double cs[3 * 256];
+    __shared__ double am[256];   // blob amplitudes (1 without amps)
     double re = 0.0, im = 0.0;
     int mx = 0, my = 0, mz = 0;
     if (idx < n) {
@@ -2510,14 +2511,16 @@ __global__ __launch_bounds__(256) void k_synth_volume(
         __syncthreads();
         for (int i = threadIdx.x; i < 3 * 256; i += blockDim.x)
             cs[i] = (b0 * 3 + i < 3 * nBlob) ? centres[b0 * 3 + i] : 0.0;
+        for (int i = threadIdx.x; i < 256; i += blockDim.x)
+            am[i] = (b0 + i < nBlob) ? (amps ? amps[b0 + i] : 1.0) : 0.0;
         __syncthreads();
         const int nb = min(256, nBlob - b0);
         for (int b = 0; b < nb; b++) {
             const double ph = -2.0 * (mx * cs[3 * b] + my * cs[3 * b + 1] + mz * cs[3 * b + 2]) / V;
             double sn, c;
             sincospi(ph, &sn, &c);
-            re += c;
-            im += sn;
+            re += am[b] * c;
+            im += am[b] * sn;
         }
     }
     if (idx < n) {
@@ -2529,11 +2532,11 @@ __global__ __launch_bounds__(256) void k_synth_volume(
 }
 
 int launch_synth_volume(thx_ctx* c, int cls, int idim, int pf, const double* dCentres,
-                        int nBlob, double sigma) {
+                        int nBlob, double sigma, const double* dAmps) {
     const int V = idim * pf;
     const size_t n = (size_t)(V / 2 + 1) * V * V;
     hipLaunchKernelGGL(k_synth_volume, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       dCentres, nBlob, sigma, V, c->vol[cls].as<float2>());
+                       dCentres, dAmps, nBlob, sigma, V, c->vol[cls].as<float2>());
     THX_HIP(hipGetLastError());
     return THX_OK;
 }

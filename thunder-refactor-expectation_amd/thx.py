@@ -94,6 +94,7 @@ _SIGS = {
     "thx_pf_trans_stats": [_p, _p, _i, _i, _p, _p, _p],
     "thx_pf_keep_peak": [_p, _p, _i, _i, _p, _i, _p],
     "thx_ctx_synth_volume": [_p, _i, _i, _i, _i, _d, _d, _u64, _p],
+    "thx_ctx_synth_volume_blobs": [_p, _i, _i, _i, _i, _d, _p, _p],
     "thx_ctx_synth_particles": [_p, _i, _i, _d, _d, _d, _d, _d, _d, _d, _d,
                                 _u64, _p, _p, _p],
 }
@@ -744,6 +745,13 @@ class Context:
         check(load().thx_ctx_synth_volume(self._h, cls, idim, pf, nBlob, sigma, rBall, seed,
                                           ptr(cen)), "synth_volume")
         return cen
+
+    def synth_volume_blobs(self, cls, idim, pf, centres, sigma, amps=None):
+        centres = c32(centres, np.float64).reshape(-1, 3)
+        a = None if amps is None else c32(amps, np.float64)
+        assert a is None or a.size == len(centres)
+        check(load().thx_ctx_synth_volume_blobs(self._h, cls, idim, pf, len(centres), sigma,
+                                                ptr(centres), ptr(a)), "synth_volume_blobs")
 
     def synth_particles(self, cls, P, maxShift, dfMin, dfMax, snr, seed, voltage=300e3,
                         Cs=2.7e7, ampContrast=0.07, pixelSize=1.5):
