@@ -122,6 +122,66 @@ def run_learned(ctx, net, cal, steps, warmup, P, R, T):
                 kernels_ms=kern, fallback=float(np.mean(nfs)))
 
 
+def run_learned_seeded(ctx, info, name, net, step, gR, phases=3, mLR=300, mLT=25):
+    """The proposal seeding the particle filter (thx_ctx_expect_seeded): per
+    particle mLR rotations and mLT shifts drawn from the network's mixture,
+    scored once, states from them, then `phases` local-search phases, with no
+    fallback and with the ESS < 20 fallback to the whole-grid scan; beside it
+    the reference's flow on the same particles: whole-grid FP32 scan, states
+    from it, the same phases.  Final poses (each particle's top sample) are
+    compared with the true poses and with the plain FP32 scan's grid pose."""
+    w = wl.WORKLOADS[name]
+    quat = info["quat"]
+    common = dict(min_std_r=w["n_rot"] ** (-1.0 / 3), min_std_t=0.3, pf_l=2.0, pf_s=0.5,
+                  transS=1.0, max_phase=1 + phases)
+
+    def summary(st, dt, ran, extra):
+        err = wl.geodesic_angle(st["topR"], info["true_quat"])
+        grid = wl.geodesic_angle(st["topR"], quat[gR]) <= step * (1 + 1e-6)
+        d = {"ms": 1e3 * dt, "phases": ran,
+             "median_error_deg_vs_truth": float(np.degrees(np.median(err))),
+             "p90_error_deg_vs_truth": float(np.degrees(np.quantile(err, 0.9))),
+             "within_one_grid_step_of_plain_fp32": float(np.mean(grid))}
+        d.update(extra)
+        return d
+
+    ctx.set_local_options(0, True)
+
+    def reference():
+        ctx.score_global(0, thx.FP32)
+        ctx.pf_from_global(0, mLR, mLT, common["min_std_r"], common["min_std_t"], common["pf_s"],
+                           1.0, seed=3)
+        return ctx.pf_refine(None, thx.BF16, after_global=True, pf_l=2.0, pf_s=0.5,
+                             max_phase=common["max_phase"], seed=4)
+
+    out = {}
+    reference()   # untimed: tuner and phase graphs
+    ctx.sync()
+    t0 = time.perf_counter()
+    ran = reference()
+    ctx.sync()
+    out["global_scan_then_local"] = summary(ctx.pf_get_state(), time.perf_counter() - t0, ran,
+                                            {"form": "whole-grid FP32 scan, states from it, "
+                                                     "BF16 local phases"})
+    for tag, ess in (("seeded_no_fallback", 0.0), ("seeded_ess20", 20.0)):
+        ctx.expect_seeded(net, 0, mLR, mLT, thx.BF16, ess_min=ess, fallback=thx.BF16, seed=3,
+                          **common)
+        ctx.sync()
+        t0 = time.perf_counter()
+        nf, ran = ctx.expect_seeded(net, 0, mLR, mLT, thx.BF16, ess_min=ess, fallback=thx.BF16,
+                                    seed=3, **common)
+        ctx.sync()
+        out[tag] = summary(ctx.pf_get_state(), time.perf_counter() - t0, ran,
+                           {"fallback_particles": nf,
+                            "form": f"{mLR} rotations x {mLT} shifts drawn from the proposal, "
+                                    f"scored (BF16), states, BF16 local phases"
+                                    + (f"; ESS < {ess:g}: whole-grid BF16 scan" if ess else "")})
+    ctx.set_local_options(0, False)
+    out["speedup_vs_global_scan_then_local"] = (out["global_scan_then_local"]["ms"]
+                                                / out["seeded_no_fallback"]["ms"])
+    return out
+
+
 def run_learned_finetune(name, net, cal, step, sigma_c=0.5, sigma_a=0.1,
                          budget_m=(0.5, 1, 2, 3, 4, 6, 8), lr=2e-4, chunk=8192, steps=2):
     """What the learned step costs when the map changes between iterations.
@@ -850,6 +910,10 @@ def main():
                           "fallback_particles": r["fallback"], "training": train,
                           "calibration": cal, "kernels_ms": r["kernels_ms"],
                           "sampling_step_deg": float(np.degrees(step))}
+        log("proposal-seeded particle filter")
+        out["learned"]["seeded_particle_filter"] = run_learned_seeded(ctx, info, name, net, step, gR)
+        log(f"seeded: {out['learned']['seeded_particle_filter']}")
+        run_mode(ctx, args.mixed, 1, 0, P, R, T, K)   # the mixed scan's marginals again
         if not args.no_finetune:
             log("learned step on a changed map (fine-tuning)")
             fts = run_learned_map_changes(name, net, cal, step)

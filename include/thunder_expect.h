@@ -573,6 +573,28 @@ int thx_ctx_expect_learned(thx_ctx* ctx, thx_cnn* net, int cls, int nSelR,
                            int nSelT, double essMin, double spreadMax,
                            double tailMax, double scoreDrop,
                            int fallbackPrecision, int* nFallback);
+/* Local-search expectation seeded by the learned proposal, in place of
+ * Particle::reset / load (src/Particle.cpp:87-169, 401-554): per particle,
+ * mLR rotations and mLT translations are drawn from the network's mixture
+ * (as thx_proposal_sample), their ACG concentrations, shift deviations and
+ * balance weights taken (calVari, balanceWeight), and every particle's
+ * samples scored once by the local scorer (precision; projection fused); the
+ * peak factor, half-height peak, resampling by the scores, calVari and the
+ * scanning minima (minStdR, minStdT, pfS) then give the particle-filter
+ * state.  Where the ESS of a particle's rotation weights, (sum w)^2 /
+ * sum w^2, is below essMin (<= 0: never), the uniform global search is the
+ * fallback: the whole grid of the context scanned at fallbackPrecision and
+ * the state from that scan (as thx_pf_from_global).  Then the local phases
+ * of thx_pf_refine for every particle (after-global phase rules, pfL, pfS,
+ * maxPhase, seed + 1); transS[P] is the shift search scale per particle.
+ * *nFallback and *phasesRun (optional) receive the particles that fell back
+ * and the phases run; the state is read with thx_pf_get_state.  No
+ * counterpart in the reference (it has no learned proposal). */
+int thx_ctx_expect_seeded(thx_ctx* ctx, thx_cnn* net, int cls, int mLR, int mLT,
+                          int precision, double essMin, int fallbackPrecision,
+                          double minStdR, double minStdT, double pfL, double pfS,
+                          const double* transS, int maxPhase, uint64_t seed,
+                          int* nFallback, int* phasesRun);
 /* Per particle: best grid rotation / translation indices, best
  * log-likelihood, ESS, spread, tail depth and whether it fell back (any
  * output may be NULL). */

@@ -277,3 +277,67 @@ def test_whole_selection_shapes(gpu_ctx, R, side):
     assert agree >= 0.85
     assert rel_gap.max() <= 2e-3
     assert rel_score.max() <= 2e-3
+
+
+def test_seeded_forced_fallback_equals_global_path(gpu_ctx, setup):
+    """Proposal-seeded local search with the ESS threshold above any ESS:
+    every particle takes the uniform global search, so the result must equal,
+    bit for bit, the reference's own path on the same particles: whole-grid
+    scan, particle states from it (thx_pf_from_global with the fallback's
+    seed), local phases."""
+    info, dvp, net = setup
+    args = dict(min_std_r=192 ** (-1.0 / 3), min_std_t=0.3, pf_l=2.0, pf_s=0.5, transS=1.0,
+                max_phase=12)
+    keys = ("quat", "trans", "wR", "wT", "k", "s01", "topR", "topT", "peakR", "nPhase")
+    nf, ph = gpu_ctx.expect_seeded(net, 0, 100, 25, thx.BF16, ess_min=1e9, fallback=thx.BF16,
+                                   seed=5, **args)
+    assert nf == P and ph >= 10
+    st = gpu_ctx.pf_get_state()
+    gpu_ctx.score_global(0, thx.BF16)
+    gpu_ctx.pf_from_global(0, 100, 25, args["min_std_r"], args["min_std_t"], args["pf_s"], 1.0,
+                           seed=5 ^ 0x5eed)
+    ph2 = gpu_ctx.pf_refine(None, thx.BF16, after_global=True, pf_l=2.0, pf_s=0.5, max_phase=12,
+                            seed=6)
+    ref = gpu_ctx.pf_get_state()
+    assert ph2 == ph
+    for k in keys:
+        np.testing.assert_array_equal(st[k], ref[k], err_msg=k)
+
+
+def test_seeded_local_search_refines():
+    """Proposal-seeded local search without fallback, against the reference's
+    flow on the same particles (whole-grid FP32 scan, states from it, local
+    phases): with a trained network the seeded search converges and its final
+    poses are as close to the truth, far below the grid's spacing, without the
+    whole-grid scan."""
+    import learned
+    print("\n[setup] box64 P=128 R=1000 T=49 seeded local search", flush=True)
+    w = wl.WORKLOADS["box64"]
+    with thx.Context(0) as ctx, thx.Context(0) as tctx:
+        info = wl.setup_context(ctx, "box64", n_particles=128, n_rot=1000)
+        step = wl.sampling_step(info["quat"])
+        tctx.set_pixels(info["iCol"], info["iRow"], info["iSig"], w["box"], w["pf"])
+        tctx.synth_volume(0, w["box"], w["pf"], w["n_blob"], w["blob_sigma"], w["blob_radius"],
+                          wl.seeds("box64")["volume"])
+        net = thx.Proposal(components=8, seed=11)
+        learned.train_proposal(net, tctx, "box64", n_samples=60 * 8192)
+        args = dict(min_std_r=1000 ** (-1.0 / 3), min_std_t=0.3, pf_l=2.0, pf_s=0.5,
+                    transS=1.0, max_phase=30)
+        nf, ph = ctx.expect_seeded(net, 0, 300, 25, thx.BF16, ess_min=0.0, seed=21, **args)
+        st = ctx.pf_get_state()
+        net.close()
+        ctx.score_global(0, thx.FP32)
+        ctx.pf_from_global(0, 300, 25, args["min_std_r"], args["min_std_t"], args["pf_s"], 1.0,
+                           seed=21)
+        ctx.pf_refine(None, thx.BF16, after_global=True, pf_l=2.0, pf_s=0.5, max_phase=30,
+                      seed=22)
+        ref = ctx.pf_get_state()
+    err = wl.geodesic_angle(st["topR"], info["true_quat"])
+    err_ref = wl.geodesic_angle(ref["topR"], info["true_quat"])
+    print(f"seeded: {ph} phases, converged {np.mean(st['nPhase'] >= 0):.2f}, median error "
+          f"{np.degrees(np.median(err)):.2f} deg; global scan + local: "
+          f"{np.degrees(np.median(err_ref)):.2f} deg (grid step {np.degrees(step):.1f})")
+    assert nf == 0
+    assert np.mean(st["nPhase"] >= 0) >= 0.9
+    assert np.median(err) < 0.25 * step
+    assert np.median(err) <= 1.5 * np.median(err_ref) + np.radians(0.2)

@@ -1033,8 +1033,10 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
 // Whole-grid scan (run_scan) of the particles idx: their rows are gathered,
 // and the context's particle inputs and this class's result slot are
 // swapped for the gathered rows and lpRes for the duration of the scan.
-static int scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int>& idx,
-                       const int* pinnedIdx = nullptr) {
+}  // extern "C"
+
+int thx::scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int>& idx,
+                     const int* pinnedIdx) {
     const int n = (int)idx.size();
     const size_t nk = (size_t)n * c->K;
     if (pinnedIdx) {   // a pinned copy of idx: the upload does not block the host
@@ -1066,6 +1068,8 @@ static int scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int
     c->Ppad = Ppad0;
     return s;
 }
+
+extern "C" {
 
 int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSelT, double essMin,
                            double spreadMax, double tailMax, double scoreDrop,

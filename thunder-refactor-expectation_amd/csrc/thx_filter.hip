@@ -1282,6 +1282,291 @@ int thx_pf_refine(thx_ctx* c, const int* cls, int precision, int afterGlobal, do
     return THX_OK;
 }
 
+// Effective sample size of each particle's row of n weights, (sum w)^2 /
+// sum w^2 (the commented-out Particle::neff, src/Particle.cpp:1647-1651, of
+// the normalised weights).  One block per particle.
+__global__ __launch_bounds__(256) void k_row_ess(const double* __restrict__ w, int n, int P,
+                                                 double* __restrict__ ess) {
+    const int p = blockIdx.x;
+    if (p >= P) return;
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double v = w[(size_t)p * n + i];
+        a += v;
+        b += v * v;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+            s2[threadIdx.x] += s2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ess[p] = s2[0] > 0 ? s1[0] * s1[0] / s2[0] : 0.0;
+}
+
+}  // extern "C"
+
+// dst row idx[i] = src row i (rows of rowLen elements), i < n
+template <class T>
+__global__ void k_scatter_rows(const T* __restrict__ src, int rowLen, const int* __restrict__ idx,
+                               int n, T* __restrict__ dst) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * rowLen) return;
+    const size_t i = e / rowLen, j = e % rowLen;
+    dst[(size_t)idx[i] * rowLen + j] = src[e];
+}
+
+template <class T>
+static int scatter_rows(hipStream_t st, const DevBuf& src, int rowLen, const int* idx, int n,
+                        DevBuf& dst) {
+    hipLaunchKernelGGL(k_scatter_rows<T>, dim3(nb((size_t)n * rowLen)), dim3(256), 0, st,
+                       src.as<const T>(), rowLen, idx, n, dst.as<T>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
+extern "C" {
+
+// Particle-filter states of the particles idx from their whole-grid scan of
+// class cls (the marginals scan_subset left in lpRes): the post-scan update
+// of thx_pf_from_global computed for those rows alone, then scattered into
+// the context's full state arrays.
+static int states_from_subset_scan(thx_ctx* c, int cls, const std::vector<int>& idx, int mLR,
+                                   int mLT, double minStdR, double minStdT, double pfS,
+                                   const std::vector<double>& transSAll, uint64_t seed) {
+    const int n = (int)idx.size(), P0 = c->P, Ppad0 = c->Ppad;
+    hipStream_t st = c->stream;
+    DevBuf dIdx;
+    THX_TRY(upload(dIdx, idx.data(), n, st));
+    std::vector<double> tS(n);
+    for (int i = 0; i < n; i++) tS[i] = transSAll[idx[i]];
+    // the full state arrays step aside; from_global_impl fills fresh ones for
+    // the n rows (scratch buffers are shared)
+    DevBuf* state[] = {&c->pfQ, &c->pfTr, &c->pfWR, &c->pfWRf, &c->pfWT, &c->pfWTf, &c->pfK,
+                       &c->pfS, &c->pfPeak, &c->pfMean, &c->pfTransS};
+    constexpr int NS = sizeof(state) / sizeof(state[0]);
+    DevBuf saved[NS];
+    for (int i = 0; i < NS; i++) std::swap(saved[i], *state[i]);
+    std::swap(c->res[cls], c->lpRes);
+    c->P = n;
+    c->Ppad = round_up(n, 128);
+    int s = from_global_impl(c, cls, nullptr, 1, mLR, mLT, minStdR, minStdT, pfS, tS.data(), seed);
+    c->P = P0;
+    c->Ppad = Ppad0;
+    std::swap(c->res[cls], c->lpRes);
+    if (s == THX_OK) {
+        const int* di = dIdx.as<const int>();
+        const int lens[NS] = {4 * mLR, 2 * mLT, mLR, mLR, mLT, mLT, 3, 3, 1, 4, 1};
+        for (int i = 0; i < NS && s == THX_OK; i++) {
+            if (state[i] == &c->pfWRf || state[i] == &c->pfWTf)
+                s = scatter_rows<float>(st, *state[i], lens[i], di, n, saved[i]);
+            else
+                s = scatter_rows<double>(st, *state[i], lens[i], di, n, saved[i]);
+        }
+        if (s == THX_OK && hipStreamSynchronize(st) != hipSuccess) {
+            set_error("seeded: scattering the fallback states failed");
+            s = THX_ERR_HIP;
+        }
+    }
+    for (int i = 0; i < NS; i++) {
+        state[i]->release();
+        std::swap(saved[i], *state[i]);
+    }
+    dIdx.release();
+    c->pfP = P0;
+    c->pfR = mLR;
+    c->pfT = mLT;
+    return s;
+}
+
+int thx_ctx_expect_seeded(thx_ctx* c, thx_cnn* net, int cls, int mLR, int mLT, int precision,
+                          double essMin, int fallbackPrecision, double minStdR, double minStdT,
+                          double pfL, double pfS, const double* transS, int maxPhase,
+                          uint64_t seed, int* nFallback, int* phasesRun) {
+    clear_error();
+    THX_CHECK(c && net && transS, THX_ERR_ARG, "expect_seeded: null argument");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16 || precision == THX_FP16,
+              THX_ERR_ARG, "expect_seeded: precision must be FP32, BF16 or FP16");
+    THX_CHECK(mLR >= 2 && mLR <= kResampleMaxN && mLT >= 2 && mLT <= 128, THX_ERR_RANGE,
+              "expect_seeded: sample counts out of range");
+    THX_CHECK(c->K > 0 && c->P > 0 && cls >= 0 && cls < (int)c->vol.size() &&
+                  c->vdim[cls] == c->idim * c->pf,
+              THX_ERR_STATE, "expect_seeded: pixels, particles and the class volume needed");
+    THX_CHECK(c->ctfReady && c->sigReady, THX_ERR_STATE,
+              "expect_seeded: particle CTF and noise spectrum must be set");
+    int C = 0;
+    THX_TRY(thx_cnn_components(net, &C));
+    THX_HIP(hipSetDevice(c->device));
+    const int P = c->P;
+    hipStream_t st = c->stream;
+    std::vector<double> tS(transS, transS + P);
+    // 1. the proposal's mixture for every particle, then mLR rotations and mLT
+    //    shifts drawn from it: the particle filter's samples (Particle::reset
+    //    / load draw them from ACG distributions, src/Particle.cpp:87-169,
+    //    401-554)
+    THX_TRY(pf_alloc(c, P, mLR, mLT));
+    THX_HIP(hipMemcpy(c->pfTransS.p, tS.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    THX_TRY(c->lpMix.ensure(sizeof(double) * (size_t)P * (6 * C + 3)));
+    if (!c->lpReady) THX_HIP(hipEventCreateWithFlags(&c->lpReady, hipEventDisableTiming));
+    if (!c->lpDone) THX_HIP(hipEventCreateWithFlags(&c->lpDone, hipEventDisableTiming));
+    THX_HIP(hipEventRecord(c->lpReady, st));
+    THX_TRY(cnn_propose_async(net, c, c->lpMix.as<double>(), c->lpReady, c->lpDone));
+    THX_HIP(hipStreamWaitEvent(st, c->lpDone, 0));
+    THX_TRY(pf_sample_mixture(st, c->lpMix.as<const double>(), C, P, mLR, mLT, seed, 0u,
+                              c->pfQ.as<double>(), c->pfWNew.as<double>(), c->pfTr.as<double>()));
+    // 2. the samples' statistics: ACG mean and concentrations, balance weights
+    //    (the sample density's inverse), shift means and deviations
+    hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
+                       nullptr, 1 | 2 | 4 | 8, kAcgTol, kAcgMaxIter, nullptr, c->pfMean.as<double>(),
+                       c->pfK.as<double>(), c->pfWR.as<double>(), c->pfWRf.as<float>(), nullptr);
+    hipLaunchKernelGGL(k_sort_samples, dim3(P), dim3(256), 0, st, c->pfQ.as<double>(),
+                       c->pfWR.as<double>(), c->pfWRf.as<float>(), mLR, (const int*)nullptr);
+    hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, c->pfTr.as<double>(), mLT,
+                       nullptr, P, 0, c->pfS.as<double>(), scale_host(0.0),
+                       c->pfTransS.as<const double>(), kTransQ, seed, phase_host(0u), nullptr,
+                       c->pfWT.as<double>(), c->pfWTf.as<float>());
+    THX_HIP(hipGetLastError());
+    // 3. one scoring pass of the seeded samples (the local scorer, every
+    //    particle in one launch), the peak factor and top samples, the ESS of
+    //    each particle's rotation weights
+    std::vector<char> used(c->vol.size(), 0);
+    used[cls] = 1;
+    std::vector<const float2*> vp(c->vol.size());
+    for (size_t k = 0; k < c->vol.size(); k++) vp[k] = c->vol[k].as<const float2>();
+    THX_TRY(upload(c->locVols, vp.data(), vp.size(), st));
+    if (precision != THX_FP32) {
+        THX_TRY(ensure_vol_lo(c, used));
+        std::vector<const uint32_t*> vl(c->vol.size(), nullptr);
+        vl[cls] = c->volLo[cls].as<const uint32_t>();
+        THX_TRY(upload(c->locVolsLo, vl.data(), vl.size(), st));
+    }
+    std::vector<int> clsv(P, cls);
+    THX_TRY(upload(c->locPcls, clsv.data(), P, st));
+    if (precision == THX_FP16) THX_TRY(launch_local_shell_scale(c, used));
+    THX_TRY(launch_prep_particles(c, precision == THX_FP16 ? THX_FP16 : THX_FP32));
+    const int nbp = (mLR + kLocalRotBlock - 1) / kLocalRotBlock;
+    const size_t nItem = (size_t)P * nbp;
+    THX_TRY(c->locBMax.ensure(sizeof(float) * nItem));
+    THX_TRY(c->locBWC.ensure(sizeof(float) * nItem));
+    THX_TRY(c->locBArg.ensure(sizeof(int) * nItem));
+    THX_TRY(c->locBWT.ensure(sizeof(float) * nItem * kLocalWtStride));
+    THX_TRY(c->locWRrel.ensure(sizeof(float) * (size_t)P * mLR));
+    THX_TRY(c->locWC.ensure(sizeof(double) * P));
+    THX_TRY(c->locWR.ensure(sizeof(double) * (size_t)P * mLR));
+    THX_TRY(c->locWT.ensure(sizeof(double) * (size_t)P * mLT));
+    THX_TRY(c->locBase.ensure(sizeof(double) * P));
+    THX_TRY(c->locBestR.ensure(sizeof(int) * P));
+    THX_TRY(c->locBestT.ensure(sizeof(int) * P));
+    std::vector<int2> blocks;
+    std::vector<int> bStart(P + 1);
+    for (int p = 0; p < P; p++) {
+        bStart[p] = (int)blocks.size();
+        for (int r0 = 0; r0 < mLR; r0 += kLocalRotBlock) blocks.push_back(make_int2(p, r0));
+    }
+    bStart[P] = (int)blocks.size();
+    THX_TRY(upload(c->locBlocks, blocks.data(), blocks.size(), st));
+    THX_TRY(upload(c->locBStart, bStart.data(), P + 1, st));
+    LocalBatch b;
+    b.vols = c->locVols.as<const float2* const>();
+    b.volsLo = precision != THX_FP32 ? c->locVolsLo.as<const uint32_t* const>() : nullptr;
+    b.pcls = c->locPcls.as<const int>();
+    b.quat = c->pfQ.as<const double>();
+    b.pRf = c->pfWRf.as<const float>();
+    b.rOff = c->pfOffR.as<const int>();
+    b.nR = c->pfNR.as<const int>();
+    b.trans = c->pfTr.as<const double>();
+    b.pTf = c->pfWTf.as<const float>();
+    b.tOff = c->pfOffT.as<const int>();
+    b.nT = c->pfNT.as<const int>();
+    b.blocks = c->locBlocks.as<const int2>();
+    b.bStart = c->locBStart.as<const int>();
+    b.bMax = c->locBMax.as<float>();
+    b.bWC = c->locBWC.as<float>();
+    b.bArg = c->locBArg.as<int>();
+    b.bWT = c->locBWT.as<float>();
+    b.wRrel = c->locWRrel.as<float>();
+    const int TPAD = kLocalTile[local_tile_index(mLT)];
+    const int mode = choose_local_mode(c, precision, TPAD, (int)blocks.size(), b);
+    if (mode < 0) return THX_ERR_HIP;
+    THX_TRY(launch_score_local(c, precision, TPAD, (int)blocks.size(), b, mode));
+    THX_TRY(launch_merge_local(c, P, kLocalWtStride, b, c->locWC.as<double>(),
+                               c->locWR.as<double>(), c->locWT.as<double>(),
+                               c->locBase.as<double>(), c->locBestR.as<int>(),
+                               c->locBestT.as<int>()));
+    DevBuf dEss;
+    THX_TRY(dEss.ensure(sizeof(double) * P));
+    hipLaunchKernelGGL(k_row_ess, dim3(P), dim3(256), 0, st, c->locWR.as<const double>(), mLR, P,
+                       dEss.as<double>());
+    // setPeakFactor + keepHalfHeightPeak of the scored rotations, tops
+    hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWR.as<double>(), mLR,
+                       nullptr, c->pfPeak.as<double>(), 1, 1, c->pfQ.as<const double>(), 4,
+                       c->pfTopR.as<double>(), nullptr);
+    hipLaunchKernelGGL(k_keep_peak, dim3(P), dim3(256), 0, st, c->locWT.as<double>(), mLT,
+                       nullptr, c->pfPeak.as<double>(), 0, 0, c->pfTr.as<const double>(), 2,
+                       c->pfTopT.as<double>(), nullptr);
+    // 4. resampling of both sets by their scores, calVari, scanning minima:
+    //    the particle-filter state the local phases start from
+    THX_TRY(pf_resample(st, c->pfWR.as<double>(), c->locWR.as<double>(), c->pfOffR.as<int>(),
+                        c->pfNR.as<int>(), c->pfOffR.as<int>(), c->pfNR.as<int>(), P, seed,
+                        phase_host(1u), c->pfIdx.as<int>(), c->pfWNew.as<double>()));
+    hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLR)), dim3(256), 0, st,
+                       c->pfQ.as<const double>(), mLR, 0, c->pfIdx.as<const int>(),
+                       c->pfWNew.as<const double>(), mLR, 4, nullptr, P, c->pfQ2.as<double>(),
+                       c->pfWR.as<double>(), c->pfWRf.as<float>());
+    THX_TRY(pf_resample(st, c->pfWT.as<double>(), c->locWT.as<double>(), c->pfOffT.as<int>(),
+                        c->pfNT.as<int>(), c->pfOffT.as<int>(), c->pfNT.as<int>(), P, seed,
+                        phase_host(2u), c->pfIdx.as<int>(), c->pfWNew.as<double>()));
+    hipLaunchKernelGGL(k_gather_samples, dim3(nb((size_t)P * mLT)), dim3(256), 0, st,
+                       c->pfTr.as<const double>(), mLT, 0, c->pfIdx.as<const int>(),
+                       c->pfWNew.as<const double>(), mLT, 2, nullptr, P, c->pfTr2.as<double>(),
+                       c->pfWT.as<double>(), c->pfWTf.as<float>());
+    std::swap(c->pfQ, c->pfQ2);
+    std::swap(c->pfTr, c->pfTr2);
+    hipLaunchKernelGGL(k_rot_stats, dim3(P), dim3(256), 0, st, c->pfQ.as<const double>(), mLR,
+                       nullptr, 6, kAcgTol, kAcgMaxIter, nullptr, c->pfMean.as<double>(),
+                       c->pfK.as<double>(), nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL(k_trans_stats, dim3(nb(P)), dim3(256), 0, st, c->pfTr.as<double>(), mLT,
+                       nullptr, P, 0, c->pfS.as<double>(), scale_host(0.0),
+                       c->pfTransS.as<const double>(), kTransQ, seed, phase_host(0u), nullptr,
+                       nullptr, nullptr);
+    const double kMin = (minStdR / pfS) * (minStdR / pfS), sMin = minStdT / pfS;
+    hipLaunchKernelGGL(k_min_std, dim3(nb(P)), dim3(256), 0, st, c->pfK.as<double>(),
+                       c->pfS.as<double>(), P, kMin, sMin);
+    THX_HIP(hipGetLastError());
+    std::vector<double> ess(P);
+    THX_HIP(hipMemcpyAsync(ess.data(), dEss.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
+    THX_HIP(hipStreamSynchronize(st));
+    dEss.release();
+    // 5. the uniform global search where the weights collapsed: whole-grid
+    //    scan of those particles and their states from it
+    std::vector<int> fell;
+    if (essMin > 0)
+        for (int p = 0; p < P; p++)
+            if (!(ess[p] >= essMin)) fell.push_back(p);
+    if (!fell.empty()) {
+        THX_CHECK(c->R > 0 && c->T > 0 && c->rotP.p && c->normP.p &&
+                      c->rotP.bytes >= sizeof(float2) * (size_t)c->Rpad * c->Kp &&
+                      c->normP.bytes >= sizeof(float) * (size_t)c->Rpad * c->Kp,
+                  THX_ERR_STATE, "expect_seeded: the fallback needs the context's grid");
+        // the grid's slices of this class (the scan reads the projections)
+        THX_TRY(launch_project(c, cls, c->quat.as<const double>(), c->R, c->Rpad,
+                               c->rotP.as<float2>(), c->normP.as<float>()));
+        THX_TRY(scan_subset(c, cls, fallbackPrecision, fell));
+        THX_TRY(states_from_subset_scan(c, cls, fell, mLR, mLT, minStdR, minStdT, pfS, tS,
+                                        seed ^ 0x5eedull));
+    }
+    if (nFallback) *nFallback = (int)fell.size();
+    c->pfValid = true;
+    // 6. the local phases for every particle
+    return thx_pf_refine(c, cls ? clsv.data() : nullptr, precision, 1, pfL, pfS, maxPhase,
+                         seed + 1, phasesRun);
+}
+
 int thx_pf_get_state(thx_ctx* c, double* quat, double* trans, double* wR, double* wT, double* k,
                      double* s01, double* topR, double* topT, double* peakR, int* nPhase) {
     clear_error();

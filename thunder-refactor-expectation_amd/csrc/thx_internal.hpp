@@ -313,6 +313,10 @@ __host__ __device__ inline size_t vb_bytes(int V) {
     return (size_t)vb_nbx(V) * vb_nby(V) * vb_nbz(V) * 128;
 }
 int launch_vol_bricked(hipStream_t st, const float2* vol, int V, uint32_t* vb);
+// Whole-grid scan (class cls, precision) of the particles idx: their marginals
+// land in ctx->lpRes, rows in the order of idx (pinnedIdx: a pinned copy)
+int scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int>& idx,
+                const int* pinnedIdx = nullptr);
 // frees the projectee FFT plans and scratch
 void projectee_release(thx_ctx* c);
 void images_release(thx_ctx* c);

@@ -81,6 +81,8 @@ _SIGS = {
     "thx_cnn_components": [_p, _p],
     "thx_cnn_set_precision": [_p, C.c_int],
     "thx_ctx_expect_learned": [_p, _p, _i, _i, _i, _d, _d, _d, _d, _i, _p],
+    "thx_ctx_expect_seeded": [_p, _p, _i, _i, _i, _i, _d, _i, _d, _d, _d, _d, _p, _i, _u64, _p,
+                              _p],
     "thx_ctx_get_learned": [_p, _p, _p, _p, _p, _p, _p, _p],
     "thx_pf_set_state": [_p, _i, _i, _p, _p, _p, _p, _p, _p],
     "thx_pf_from_global": [_p, _i, _i, _i, _d, _d, _d, _p, _u64],
@@ -674,6 +676,20 @@ class Context:
                                             spread_max, tail_max, score_drop,
                                             fallback_precision, C.byref(nf)), "expect_learned")
         return nf.value
+
+    def expect_seeded(self, net, cls=0, mLR=256, mLT=25, precision=BF16, ess_min=20.0,
+                      fallback=BF16, min_std_r=0.0, min_std_t=0.0, pf_l=2.0, pf_s=0.5,
+                      transS=1.0, max_phase=100, seed=1):
+        """Local search seeded by the proposal (see thx_ctx_expect_seeded);
+        returns (particles that fell back, phases run)."""
+        self.mLR, self.mLT = mLR, mLT
+        tS = np.broadcast_to(np.asarray(transS, np.float64), (self.P,)).copy()
+        nf, ph = C.c_int(0), C.c_int(0)
+        check(load().thx_ctx_expect_seeded(self._h, net._h, cls, mLR, mLT, precision, ess_min,
+                                           fallback, min_std_r, min_std_t, pf_l, pf_s, ptr(tS),
+                                           max_phase, seed, C.byref(nf), C.byref(ph)),
+              "expect_seeded")
+        return nf.value, ph.value
 
     def get_learned(self):
         """(bestR, bestT, score, ess, spread, tail, fell_back) per particle."""
