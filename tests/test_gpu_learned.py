@@ -286,6 +286,8 @@ def test_seeded_forced_fallback_equals_global_path(gpu_ctx, setup):
     scan, particle states from it (thx_pf_from_global with the fallback's
     seed), local phases."""
     info, dvp, net = setup
+    # the module's particles and grid again (other tests here replace them)
+    wl.setup_context(gpu_ctx, "box64", n_particles=P, n_rot=R)
     args = dict(min_std_r=192 ** (-1.0 / 3), min_std_t=0.3, pf_l=2.0, pf_s=0.5, transS=1.0,
                 max_phase=12)
     keys = ("quat", "trans", "wR", "wT", "k", "s01", "topR", "topT", "peakR", "nPhase")
