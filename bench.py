@@ -304,7 +304,7 @@ def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
     # rotations about each particle's true pose, 4 degrees spread, 25 shifts):
     # 2 work items per particle, a launch size no other local launch here has,
     # so the profile's FETCH / WRITE figure for that grid is this launch's
-    mode = ctx.local_mode()
+    mode, ns = ctx.local_shape()
     rng = np.random.default_rng(256)
     nRr = 256
     qr = np.repeat(info["true_quat"], nRr, axis=0) + rng.normal(0, 0.035, size=(P * nRr, 4))
@@ -335,7 +335,8 @@ def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
             "pose_evals_per_s": poses * ran / dt,
             "trilinear_samples_per_s": samples * ran / dt,
             "median_error_deg_vs_truth": float(np.degrees(np.median(err))),
-            "launch": ["", "direct", "persistent work queue"][max(0, mode)],
+            "launch": ["", "direct", "persistent work queue"][max(0, mode)]
+                      + f", {ns} samples in flight per gather step",
             "roofline_batch": f"{P} particles x {nRr} rotations (4 deg about the true pose) "
                               f"x {mLT} shifts, BF16, direct launch",
             "roofline": roof,
@@ -373,10 +374,10 @@ def run_local_box160(steps=1):
     pR, pT = np.repeat(1.0 / nR, nR), np.repeat(1.0 / nT, nT)
     cls = (np.arange(P) % 3).astype(np.int32)
     args = (quat, nR, trans, nT, pR, pT, cls, thx.FP16)
-    # the direct launch (what the tuner picks at this size), without the
-    # tuner's extra timing runs; the call's launches (operand prep, one scorer
-    # per translation tile, merge) captured in a graph by the first call and
-    # replayed by the timed ones
+    # the direct launch (what the tuner picks at this size; forced, to spare
+    # the persistent form's timing runs), the gather depth tuned by the first
+    # call; the call's launches (operand prep, one scorer per translation tile,
+    # merge) captured in a graph by the first call and replayed by the timed ones
     lctx.set_local_options(1, True)
     lctx.score_local(*args)
     lctx.sync()
@@ -386,7 +387,7 @@ def run_local_box160(steps=1):
     lctx.sync()
     dt = (time.perf_counter() - t0) / steps
     kern = lctx.last_timing()
-    mode = lctx.local_mode()
+    mode, ns = lctx.local_shape()
     K = len(iCol)
     lctx.close()
     samples = float(nR.sum()) * K
@@ -396,7 +397,8 @@ def run_local_box160(steps=1):
             "ms_per_call_with_upload": 1e3 * dt, "kernels_ms": kern,
             "pose_evals_per_s": float(np.sum(nR.astype(np.int64) * nT)) / t_k,
             "trilinear_samples_per_s": samples / t_k,
-            "launch": ["", "direct", "persistent work queue"][max(0, mode)] + ", graph replay",
+            "launch": ["", "direct", "persistent work queue"][max(0, mode)]
+                      + f", {ns} samples in flight per gather step, graph replay",
             "precision": "fp16"}
 
 

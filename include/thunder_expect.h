@@ -338,14 +338,20 @@ int thx_ctx_get_local(thx_ctx* ctx, double* wC, double* wR, double* wT,
                       double* baseL, int* bestR, int* bestT);
 
 /* Launch shape of the local scorer (no reference counterpart): mode 0 =
- * autotuned (both shapes are timed on the first batch of each size class and
- * the faster is kept), 1 = one block per work item (particle x 128
- * rotations), 2 = a persistent grid sized to the resident capacity pulling
- * work items from a counter.  graph != 0 captures the local phase (operand
- * prep, scorer, merge) in a hipGraph, replayed while the batch shape and the
- * context's buffers stay the same. */
+ * autotuned (the candidate shapes are timed on the first batch of each size
+ * class and the fastest is kept), 1 = one block per work item (particle x
+ * 128 rotations), 2 = a persistent grid sized to the resident capacity
+ * pulling work items from a counter.  graph != 0 captures the local call
+ * (operand prep, scorers, merge) and the particle filter's phases in
+ * hipGraphs, replayed while the batch shape and the context's buffers stay
+ * the same. */
 int thx_ctx_set_local_options(thx_ctx* ctx, int mode, int graph);
-/* Launch shape used by the last local phase (1 or 2). */
+/* Gather depth of the 16-bit local scorers (no reference counterpart): the
+ * trilinear samples per step whose corner loads are all in flight, 4 or 8;
+ * 0 (the default) lets the tuner time both. */
+int thx_ctx_set_local_gather(thx_ctx* ctx, int ns);
+/* Launch shape used by the last local call: the form (1 or 2) plus 16 x the
+ * gather depth. */
 int thx_ctx_local_mode(thx_ctx* ctx, int* mode);
 
 /* Waits for all work queued on the context stream. */

@@ -215,6 +215,20 @@ def test_local_launch_shapes_and_graph_replays(gpu_ctx, local_case):
         np.testing.assert_array_equal(a, b)
     for a, b in zip(outs[1], outs[0]):
         np.testing.assert_array_equal(a, b)
+    # the 16-bit forms at both gather depths and launch forms: bit-identical
+    for prec in (thx.BF16, thx.FP16):
+        res = {}
+        for ns in (4, 8):
+            for mode in (1, 2):
+                gpu_ctx.set_local_gather(ns)
+                gpu_ctx.set_local_options(mode, False)
+                gpu_ctx.score_local(*batches[0], precision=prec)
+                res[(ns, mode)] = gpu_ctx.get_local()
+                assert gpu_ctx.local_shape() == (mode, ns)
+        for k in res:
+            for a, b in zip(res[k], res[(4, 1)]):
+                np.testing.assert_array_equal(a, b, err_msg=str(k))
+    gpu_ctx.set_local_gather(0)
     for prec in (thx.BF16, thx.FP16):
         eager = []
         gpu_ctx.set_local_options(0, False)

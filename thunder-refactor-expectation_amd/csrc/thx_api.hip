@@ -218,38 +218,54 @@ std::vector<int> pose_order(const double* quat, const int* off, const int* cls, 
     return order;
 }
 
-// Launch shape of the local scorer for this batch: the requested one, or the
-// faster of the two timed once per (precision, padding, item-count bucket).
+// Launch shape of the local scorer for this batch: the launch form (direct or
+// persistent) and, in the 16-bit forms, the gather depth (4 or 8 samples in
+// flight); each not requested is tuned: the first batch of each (precision,
+// translation tile, power-of-two bucket of the item count) runs every
+// candidate shape once, timed with events, and the fastest is kept.
 int choose_local_mode(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b) {
-    if (c->locModeReq == kLocalDirect || c->locModeReq == kLocalQueue) return c->locModeReq;
+    std::vector<int> modes, nss;
+    if (c->locModeReq == kLocalDirect || c->locModeReq == kLocalQueue) modes = {c->locModeReq};
+    else modes = {kLocalDirect, kLocalQueue};
+    if (precision == THX_FP32) nss = {4};
+    else if (c->locNsReq == 4 || c->locNsReq == 8) nss = {c->locNsReq};
+    else nss = {4, 8};
+    std::vector<int> cand;
+    for (int ns : nss)
+        for (int m : modes) cand.push_back(local_shape(m, ns));
+    if (cand.size() == 1) return cand[0];
     int lg = 0;
     while ((1 << lg) < nblk) lg++;
-    const long long key = (long long)precision * 1000000 + TPAD * 1000 + lg;
+    const long long key = (((long long)precision * 1000 + TPAD) * 100 + lg) * 1000 +
+                          c->locModeReq * 100 + c->locNsReq;
     for (auto& kv : c->locTuned)
         if (kv.first == key) return kv.second;
-    hipEvent_t e[3] = {};
-    for (int i = 0; i < 3; i++) THX_HIP(hipEventCreate(&e[i]));
+    const int n = (int)cand.size();
+    std::vector<hipEvent_t> e(n + 1, nullptr);
     int s = THX_OK;
-    float tD = 0.f, tQ = 0.f;
-    if ((s = (hipEventRecord(e[0], c->stream) == hipSuccess ? THX_OK : THX_ERR_HIP)) == THX_OK &&
-        (s = launch_score_local(c, precision, TPAD, nblk, b, kLocalDirect)) == THX_OK &&
-        (s = (hipEventRecord(e[1], c->stream) == hipSuccess ? THX_OK : THX_ERR_HIP)) == THX_OK &&
-        (s = launch_score_local(c, precision, TPAD, nblk, b, kLocalQueue)) == THX_OK &&
-        (s = (hipEventRecord(e[2], c->stream) == hipSuccess ? THX_OK : THX_ERR_HIP)) == THX_OK &&
-        hipEventSynchronize(e[2]) == hipSuccess) {
-        (void)hipEventElapsedTime(&tD, e[0], e[1]);
-        (void)hipEventElapsedTime(&tQ, e[1], e[2]);
-    } else if (s == THX_OK) {
-        s = THX_ERR_HIP;
+    for (int i = 0; i <= n && s == THX_OK; i++)
+        if (hipEventCreate(&e[i]) != hipSuccess) s = THX_ERR_HIP;
+    if (s == THX_OK && hipEventRecord(e[0], c->stream) != hipSuccess) s = THX_ERR_HIP;
+    for (int i = 0; i < n && s == THX_OK; i++) {
+        s = launch_score_local(c, precision, TPAD, nblk, b, cand[i]);
+        if (s == THX_OK && hipEventRecord(e[i + 1], c->stream) != hipSuccess) s = THX_ERR_HIP;
     }
-    for (int i = 0; i < 3; i++) (void)hipEventDestroy(e[i]);
+    if (s == THX_OK && hipEventSynchronize(e[n]) != hipSuccess) s = THX_ERR_HIP;
+    int best = cand[0];
+    float tBest = 0.f;
+    for (int i = 0; i < n && s == THX_OK; i++) {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, e[i], e[i + 1]);
+        if (i == 0 || t < tBest) { tBest = t; best = cand[i]; }
+    }
+    for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
     if (s != THX_OK) {
         set_error("local scorer autotuning failed");
         return -1;
     }
-    const int mode = tQ < tD ? kLocalQueue : kLocalDirect;
-    c->locTuned.push_back({key, mode});
-    return mode;
+    c->locTuned.push_back({key, best});
+    return best;
 }
 
 static std::mutex g_default_mu;
@@ -946,6 +962,14 @@ int thx_ctx_set_local_options(thx_ctx* c, int mode, int graph) {
     THX_CHECK(c && mode >= 0 && mode <= 2, THX_ERR_ARG, "set_local_options: bad argument");
     c->locModeReq = mode;
     c->locGraphOn = graph != 0;
+    return THX_OK;
+}
+
+int thx_ctx_set_local_gather(thx_ctx* c, int ns) {
+    clear_error();
+    THX_CHECK(c && (ns == 0 || ns == 4 || ns == 8), THX_ERR_ARG,
+              "set_local_gather: samples in flight must be 0 (tuned), 4 or 8");
+    c->locNsReq = ns;
     return THX_OK;
 }
 

@@ -195,7 +195,8 @@ struct thx_ctx {
     // launch shape of the local scorer: requested (0 = autotuned), the tuned
     // choice per (precision, translation padding, work-item count bucket),
     // the last one used, and the captured graph of a local phase
-    int locModeReq = 0, locModeUsed = 0;
+    int locModeReq = 0, locModeUsed = 0;   // launch form requested / last used
+    int locNsReq = 0;                       // gather depth requested (0: tuned)
     bool locGraphOn = false;
     std::vector<std::pair<long long, int>> locTuned;
     thx::DevBuf locCounter;
@@ -276,9 +277,15 @@ inline LocalBatch local_batch_range(const LocalBatch& b, int start) {
 // pulling items from a counter
 constexpr int kLocalDirect = 1;
 constexpr int kLocalQueue = 2;
+// A launch shape of the local scorer: the launch form (direct / persistent)
+// and the gather depth ns, the trilinear samples per step whose corner loads
+// are all in flight together (4 or 8 in the 16-bit forms; 0 means 4).
+inline int local_shape(int mode, int ns) { return mode | (ns << 4); }
+inline int local_shape_mode(int shape) { return shape & 15; }
+inline int local_shape_ns(int shape) { return (shape >> 4) == 8 ? 8 : 4; }
 // nItemsDev: the item count in device memory (nblk then bounds it)
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
-                       int mode = kLocalDirect, const int* nItemsDev = nullptr);
+                       int shape = kLocalDirect, const int* nItemsDev = nullptr);
 // particle order for the local scorer's work list (pose locality)
 std::vector<int> pose_order(const double* quat, const int* off, const int* cls, int P,
                             const int* cnt = nullptr);

@@ -1616,7 +1616,7 @@ int launch_merge(thx_ctx* c, int cls) {
 // ---------------------------------------------------------------------------
 constexpr int kNBL = 128;
 
-template <int PREC, int TPAD>
+template <int PREC, int TPAD, int NSG>
 __device__ __forceinline__ void score_local_item(int item, 
     const float2* const* __restrict__ vols, const uint32_t* const* __restrict__ volsLo,
     const int* __restrict__ pcls, int V, int pf,
@@ -1714,11 +1714,9 @@ __device__ __forceinline__ void score_local_item(int item,
         // order), so at the low and middle radii they share cache lines
         {
             const int kb = k0 + 8 * wv;
-            // NS samples per step with all their corner loads in flight
-#ifndef THX_GATHER_NS
-#define THX_GATHER_NS 4
-#endif
-            constexpr int NS = F32 ? 2 : THX_GATHER_NS;
+            // NS samples per step with all their corner loads in flight (NSG for
+            // the 16-bit forms, a launch-shape choice of the tuner)
+            constexpr int NS = F32 ? 2 : NSG;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int rr = lane + 64 * h;
@@ -1900,7 +1898,7 @@ __device__ __forceinline__ void score_local_item(int item,
 
 
 // Direct launch: one block per work item (particle, 128 rotations).
-template <int PREC, int TPAD>
+template <int PREC, int TPAD, int NSG>
 __global__ __launch_bounds__(256, 2) void k_score_local(
     const float2* const* __restrict__ vols, const uint32_t* const* __restrict__ volsLo,
     const int* __restrict__ pcls, int V, int pf,
@@ -1919,7 +1917,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
     if ((int)blockIdx.x >= n) return;
     // the list is in pose order: each XCD walks a contiguous range of it, so
     // concurrent blocks of one XCD gather from overlapping volume regions
-    score_local_item<PREC, TPAD>(xcd_logical(blockIdx.x, n), vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+    score_local_item<PREC, TPAD, NSG>(xcd_logical(blockIdx.x, n), vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
 }
 
 // Persistent launch: a grid sized to the resident capacity pulls work items
@@ -1927,7 +1925,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local(
 // order) is cut into 8 contiguous ranges, one per XCD (workgroups are dealt
 // to the XCDs round-robin, so block b starts on range b % 8); a block that
 // drains its range moves on to the next ones.
-template <int PREC, int TPAD>
+template <int PREC, int TPAD, int NSG>
 __global__ __launch_bounds__(256, 2) void k_score_local_q(
     const float2* const* __restrict__ vols, const uint32_t* const* __restrict__ volsLo,
     const int* __restrict__ pcls, int V, int pf,
@@ -1957,7 +1955,7 @@ __global__ __launch_bounds__(256, 2) void k_score_local_q(
             continue;
         }
         const int it = lo + j;
-        score_local_item<PREC, TPAD>(it, vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
+        score_local_item<PREC, TPAD, NSG>(it, vols, volsLo, pcls, V, pf, iCol, iRow, iSig, shellScale, K, Kp, N, Y, Wt, Aterm, quat, pRf, rOff, nR, trans, pTf, tOff, nT, blocks, bMax, bWC, bArg, bWT, wRrel);
     }
 }
 
@@ -1966,7 +1964,7 @@ __global__ void k_zero_int(int* __restrict__ counter) {
     if (threadIdx.x < 8 && blockIdx.x == 0) counter[threadIdx.x] = 0;
 }
 
-template <int PREC>
+template <int PREC, int NSG>
 static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int grid, int* counter,
                               const float2* const* vols, const uint32_t* const* volsLo,
                               const int* pcls, int V, int pf,
@@ -1984,48 +1982,50 @@ static void launch_local_prec(hipStream_t st, int TPAD, int nblk, int mode, int 
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, counter);
         const int g = std::min(grid, nblk);
         if (TPAD == 32)
-            hipLaunchKernelGGL((k_score_local_q<PREC, 32>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
+            hipLaunchKernelGGL((k_score_local_q<PREC, 32, NSG>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
         else if (TPAD == 64)
-            hipLaunchKernelGGL((k_score_local_q<PREC, 64>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
+            hipLaunchKernelGGL((k_score_local_q<PREC, 64, NSG>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
         else
-            hipLaunchKernelGGL((k_score_local_q<PREC, 128>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
+            hipLaunchKernelGGL((k_score_local_q<PREC, 128, NSG>), dim3(g), dim3(256), 0, st, THX_LOCAL_ARGS, nblk, counter, nDev);
     } else {
         if (TPAD == 32)
-            hipLaunchKernelGGL((k_score_local<PREC, 32>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
+            hipLaunchKernelGGL((k_score_local<PREC, 32, NSG>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
         else if (TPAD == 64)
-            hipLaunchKernelGGL((k_score_local<PREC, 64>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
+            hipLaunchKernelGGL((k_score_local<PREC, 64, NSG>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
         else
-            hipLaunchKernelGGL((k_score_local<PREC, 128>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
+            hipLaunchKernelGGL((k_score_local<PREC, 128, NSG>), dim3(nblk), dim3(256), 0, st, THX_LOCAL_ARGS, nDev);
     }
 #undef THX_LOCAL_ARGS
 }
 
 // resident work-item capacity of the persistent launch: blocks per CU x CUs
-static int local_queue_grid(int precision, int TPAD) {
+template <int PREC, int NSG>
+static const void* local_queue_fn(int TPAD) {
+    return TPAD == 32 ? (const void*)k_score_local_q<PREC, 32, NSG>
+           : TPAD == 64 ? (const void*)k_score_local_q<PREC, 64, NSG>
+                        : (const void*)k_score_local_q<PREC, 128, NSG>;
+}
+
+static int local_queue_grid(int precision, int TPAD, int ns) {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 1024;
-    const void* fn = nullptr;
-    if (precision == THX_FP32)
-        fn = TPAD == 32 ? (const void*)k_score_local_q<THX_FP32, 32>
-             : TPAD == 64 ? (const void*)k_score_local_q<THX_FP32, 64>
-                          : (const void*)k_score_local_q<THX_FP32, 128>;
-    else if (precision == THX_BF16)
-        fn = TPAD == 32 ? (const void*)k_score_local_q<THX_BF16, 32>
-             : TPAD == 64 ? (const void*)k_score_local_q<THX_BF16, 64>
-                          : (const void*)k_score_local_q<THX_BF16, 128>;
-    else
-        fn = TPAD == 32 ? (const void*)k_score_local_q<THX_FP16, 32>
-             : TPAD == 64 ? (const void*)k_score_local_q<THX_FP16, 64>
-                          : (const void*)k_score_local_q<THX_FP16, 128>;
+    const void* fn = precision == THX_FP32 ? local_queue_fn<THX_FP32, 4>(TPAD)
+                     : precision == THX_BF16 ? (ns == 8 ? local_queue_fn<THX_BF16, 8>(TPAD)
+                                                        : local_queue_fn<THX_BF16, 4>(TPAD))
+                                             : (ns == 8 ? local_queue_fn<THX_FP16, 8>(TPAD)
+                                                        : local_queue_fn<THX_FP16, 4>(TPAD));
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0) != hipSuccess || per < 1)
         per = 2;
     return cus * per;
 }
 
 int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const LocalBatch& b,
-                       int mode, const int* nItemsDev) {
+                       int shape, const int* nItemsDev) {
+    const int mode = local_shape_mode(shape), ns = local_shape_ns(shape);
+    THX_CHECK(mode == kLocalDirect || mode == kLocalQueue, THX_ERR_ARG,
+              "local scorer: unknown launch shape");
     THX_CHECK(precision == THX_FP32 || precision == THX_BF16 || precision == THX_FP16,
               THX_ERR_ARG, "local scorer supports FP32, BF16 and FP16");
     THX_CHECK(precision == THX_FP32 || b.volsLo, THX_ERR_STATE,
@@ -2039,10 +2039,10 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
     int grid = 0;
     if (mode == kLocalQueue) {
         THX_TRY(c->locCounter.ensure(8 * sizeof(int)));
-        grid = local_queue_grid(precision, TPAD);
+        grid = local_queue_grid(precision, TPAD, ns);
     }
-#define THX_LOCAL_CALL(PR)                                                                      \
-    launch_local_prec<PR>(c->stream, TPAD, nblk, mode, grid, c->locCounter.as<int>(), b.vols,  \
+#define THX_LOCAL_CALL(PR, NG)                                                                  \
+    launch_local_prec<PR, NG>(c->stream, TPAD, nblk, mode, grid, c->locCounter.as<int>(), b.vols,  \
                           b.volsLo, b.pcls, c->idim * c->pf, c->pf, c->iCol.as<const int>(),             \
                           c->iRow.as<const int>(), c->iSig.as<const int>(),                    \
                           PR == THX_FP16 ? c->shellScale.as<const float>() : nullptr, c->K,    \
@@ -2051,9 +2051,11 @@ int launch_score_local(thx_ctx* c, int precision, int TPAD, int nblk, const Loca
                           c->Aterm.as<const double>(), b.quat, b.pRf, b.rOff, b.nR, b.trans,   \
                           b.pTf, b.tOff, b.nT, b.blocks, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel, \
                           nItemsDev)
-    if (precision == THX_FP32) THX_LOCAL_CALL(THX_FP32);
-    else if (precision == THX_BF16) THX_LOCAL_CALL(THX_BF16);
-    else THX_LOCAL_CALL(THX_FP16);
+    if (precision == THX_FP32) THX_LOCAL_CALL(THX_FP32, 4);
+    else if (precision == THX_BF16 && ns == 8) THX_LOCAL_CALL(THX_BF16, 8);
+    else if (precision == THX_BF16) THX_LOCAL_CALL(THX_BF16, 4);
+    else if (ns == 8) THX_LOCAL_CALL(THX_FP16, 8);
+    else THX_LOCAL_CALL(THX_FP16, 4);
 #undef THX_LOCAL_CALL
     THX_HIP(hipGetLastError());
     return THX_OK;

@@ -59,6 +59,7 @@ _SIGS = {
     "thx_ctx_sync": [_p],
     "thx_ctx_set_local_options": [_p, _i, _i],
     "thx_ctx_local_mode": [_p, _p],
+    "thx_ctx_set_local_gather": [_p, _i],
     "thx_ctx_score_local": [_p, _p, _p, _p, _p, _p, _p, _p, _i],
     "thx_ctx_get_local": [_p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
@@ -555,9 +556,19 @@ class Context:
         check(load().thx_ctx_set_local_options(self._h, mode, int(graph)), "set_local_options")
 
     def local_mode(self):
+        """Launch form of the last local call: 1 direct, 2 persistent."""
+        return self.local_shape()[0]
+
+    def local_shape(self):
+        """(launch form, gather depth) of the last local call."""
         m = C.c_int(0)
         check(load().thx_ctx_local_mode(self._h, C.byref(m)), "local_mode")
-        return m.value
+        return m.value & 15, (m.value >> 4) or 4
+
+    def set_local_gather(self, ns=0):
+        """Samples in flight per gather step of the 16-bit local scorers (4, 8;
+        0 = tuned)."""
+        check(load().thx_ctx_set_local_gather(self._h, ns), "set_local_gather")
 
     # ---- particle-filter local search ----
     def pf_set_state(self, quat, trans, k, s01, peakR, transS):
