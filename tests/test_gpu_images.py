@@ -50,7 +50,7 @@ def _images(P, N, seed):
 
 @pytest.mark.parametrize("masked", [False, True])
 @pytest.mark.parametrize("N,cutoff,rbg", [(64, 24, 30.0), (128, 50, 60.0), (160, 64, 76.0),
-                                          (64, 24, 40.0), (66, 24, 31.0), (192, 70, 90.0),
+                                          (64, 24, 40.0), (66, 24, 31.0), (192, 75, 90.0),
                                           (256, 100, 120.0), (100, 40, 48.0),
                                           (180, 70, 85.0)])
 def test_particle_images_match_oracle(ictx, N, cutoff, rbg, masked):
