@@ -1231,14 +1231,16 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     } while (0)
     THX_GLOAD16(0);
     THX_STAGE16(0);
-    if (kChunk < Kp) THX_GLOAD16(kChunk);
+    THX_GLOAD16(kChunk < Kp ? kChunk : 0);
     __syncthreads();
     int buf = 0;
     for (int k0 = 0; k0 < Kp; k0 += kChunk) {
-        if (k0 + kChunk < Kp) {
-            THX_STAGE16(buf ^ 1);
-            if (k0 + 2 * kChunk < Kp) THX_GLOAD16(k0 + 2 * kChunk);
-        }
+        // the next chunk's staging and the loads of the one after, both
+        // unconditional (past the end they restage / reload the last chunk into
+        // the stage no MFMA reads): conditional loads make the compiler merge
+        // old and new register values with copies on every iteration
+        THX_STAGE16(buf ^ 1);
+        THX_GLOAD16(k0 + 2 * kChunk < Kp ? k0 + 2 * kChunk : Kp - kChunk);
         // ---- MFMA on the current buffer ----
         if constexpr (F8) {
             // one 32x32x64 e4m3 step per tile: lane half hl takes bytes 32 hl .. 32 hl + 31
@@ -2246,20 +2248,25 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     constexpr int NJ = TPAD / (NTH / 16);
     const int l8 = tid & 7, rq = tid >> 3;
     const int l16 = tid & 15, tq = tid >> 4;
-    const uint32_t* brow[4];
+    // per-lane 32-bit byte offsets from uniform bases (the tables stay below
+    // 4 GiB): each load is a scalar base plus a vector offset, with no 64-bit
+    // address arithmetic per load
+    const char* bBase = reinterpret_cast<const char*>(rotLo);
+    const char* sBase = reinterpret_cast<const char*>(S);
+    const char* yBase = reinterpret_cast<const char*>(Y + (size_t)pd * Kp);
+    const char* wBase = reinterpret_cast<const char*>(Wt + (size_t)pd * Kp);
+    uint32_t boff[4], soff[NJ];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int row = rrow[rq + (NTH / 8) * q];
-        brow[q] = rotLo + (size_t)(row >= 0 ? row : 0) * Kp + 4 * l8;
+        boff[q] = ((uint32_t)(row >= 0 ? row : 0) * (uint32_t)Kp + 4u * l8) * 4u;
     }
-    const float2* srow[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int t = trow[tq + (NTH / 16) * j];
-        srow[j] = S + (size_t)(t >= 0 ? t : 0) * Kp + 2 * l16;
+        soff[j] = ((uint32_t)(t >= 0 ? t : 0) * (uint32_t)Kp + 2u * l16) * 8u;
     }
-    const float2* yrow = Y + (size_t)pd * Kp;
-    const float* wrow = Wt + (size_t)pd * Kp;
+    const uint32_t yoff = 16u * l16, woff = 16u * l8;
 
     floatx16 acc[MT];
 #pragma unroll
@@ -2269,12 +2276,13 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     float4 sv[NJ];
     float4 yv, wv;
     auto gload = [&](int k0) {
+        const uint32_t kb = (uint32_t)k0 * 4u, ks = (uint32_t)k0 * 8u;   // chunk byte offsets
 #pragma unroll
-        for (int q = 0; q < 4; q++) bv[q] = *reinterpret_cast<const u32x4*>(brow[q] + k0);
+        for (int q = 0; q < 4; q++) bv[q] = *reinterpret_cast<const u32x4*>(bBase + (boff[q] + kb));
 #pragma unroll
-        for (int j = 0; j < NJ; j++) sv[j] = *reinterpret_cast<const float4*>(srow[j] + k0);
-        yv = *reinterpret_cast<const float4*>(yrow + k0 + 2 * l16);
-        wv = *reinterpret_cast<const float4*>(wrow + k0 + 4 * l8);
+        for (int j = 0; j < NJ; j++) sv[j] = *reinterpret_cast<const float4*>(sBase + (soff[j] + ks));
+        yv = *reinterpret_cast<const float4*>(yBase + (yoff + ks));
+        wv = *reinterpret_cast<const float4*>(wBase + (woff + kb));
     };
     gload(0);
     for (int k0 = 0; k0 < Kp; k0 += kChunk) {
@@ -2305,7 +2313,10 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
             *reinterpret_cast<uint2*>(As + (tq + (NTH / 16) * j) * LD + 4 * l16) = z;
         }
         __syncthreads();
-        if (k0 + kChunk < Kp) gload(k0 + kChunk);
+        // the next chunk's loads, unconditional (the last iteration reloads its
+        // own chunk, unused): a conditional load would make the compiler merge
+        // old and new register values with copies on every iteration
+        gload(k0 + kChunk < Kp ? k0 + kChunk : k0);
         const int nrow = 32 * wave + l32;
 #pragma unroll
         for (int kk = 0; kk < 64; kk += 16) {
@@ -2428,6 +2439,9 @@ int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, co
                      c->Wt.as<const float>(), c->Aterm.as<const double>(), ridx, tidx,     \
                      nSelR, nSelT, pRu, pTu, pmap, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel
     THX_CHECK(RB == 128 || RB == 256, THX_ERR_ARG, "score_sel: rotation block must be 128 or 256");
+    // the loads address the slice and phase tables with 32-bit byte offsets
+    THX_CHECK(c->rotLo.bytes <= (1ull << 32) && c->S.bytes <= (1ull << 32), THX_ERR_RANGE,
+              "score_sel: slice or phase table above 4 GiB");
     if (RB == 128) {
         if (TPAD == 32)
             hipLaunchKernelGGL((k_score_sel<32, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
