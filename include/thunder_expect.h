@@ -579,6 +579,12 @@ int thx_ctx_expect_learned(thx_ctx* ctx, thx_cnn* net, int cls, int nSelR,
                            int nSelT, double essMin, double spreadMax,
                            double tailMax, double scoreDrop,
                            int fallbackPrecision, int* nFallback);
+/* Block order of the learned expectation's selected-pose scorer (no
+ * reference counterpart): on = 1 (the default) sorts the particles by a
+ * Morton code of their proposal's dominant mode and deals them so that each
+ * XCD scores a contiguous run of that order (overlapping selections share its
+ * L2); 0 takes them in index order.  The results are the same either way. */
+int thx_ctx_set_select_order(thx_ctx* ctx, int on);
 /* Local-search expectation seeded by the learned proposal, in place of
  * Particle::reset / load (src/Particle.cpp:87-169, 401-554): per particle,
  * mLR rotations and mLT translations are drawn from the network's mixture

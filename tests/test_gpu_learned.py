@@ -20,6 +20,9 @@
   is held to the whole-selection bars against the FP64 oracle.
 * BF16 proposal passes: the mixture from BF16 operands stays within stated
   bounds of the same network's FP32-operand mixture.
+* Particle order of the selected-pose scorer (blocks dealt in proposal-mode
+  order, or in index order): every result equal bit for bit, for one, two
+  and three rotation blocks per particle.
 * What training buys: at a fixed small selection the trained network's
   assignments agree with the plain FP32 scan on >= 80% of particles and at
   least 0.3 more often than an untrained network's.
@@ -212,6 +215,26 @@ def test_selection_and_statistics(gpu_ctx, setup):
         d2 = np.sum((info["trans"] - mix[p, 6 * C:6 * C + 2]) ** 2, 1)
         assert np.sum(d2 < d2[bT[p]] - 1e-9) < 9, p
     assert np.all((ess >= 1.0 - 1e-9) & (ess <= nsel + 1e-9))
+
+
+def test_select_order_does_not_change_results():
+    print("\n[setup] box64 P=60 R=600 T=49 select order", flush=True)
+    ctx = thx.Context(0)
+    wl.setup_context(ctx, "box64", n_particles=60, n_rot=600)
+    net = thx.Proposal(components=4, seed=3)
+    try:
+        for nr, nt in ((600, 49), (300, 25), (160, 9), (100, 25)):
+            out = {}
+            for on in (True, False):
+                ctx.set_select_order(on)
+                nf = ctx.expect_learned(net, 0, nr, nt, 0.0, 0.0, 0.0, 0.0, thx.BF16)
+                out[on] = (nf,) + tuple(ctx.get_learned())
+            for a, b in zip(out[True], out[False]):
+                assert np.array_equal(a, b), (nr, nt)
+    finally:
+        ctx.set_select_order(True)
+        net.close()
+        ctx.close()
 
 
 def test_trained_network_gains(gpu_ctx):

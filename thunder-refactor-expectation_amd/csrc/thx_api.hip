@@ -385,7 +385,8 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->locBWT, &c->locWRrel, &c->locWC, &c->locWR, &c->locWT, &c->locBase,
                        &c->locBestR, &c->locBestT, &c->img, &c->lpMix, &c->lpRidx,
                        &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpSpread, &c->lpTail,
-                       &c->lpPmap, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpQf, &c->lpRes.wC,
+                       &c->lpPmap, &c->lpOrder, &c->lpOrdKeyA, &c->lpOrdKeyB, &c->lpOrdVal,
+                       &c->lpOrdTmp, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpQf, &c->lpRes.wC,
                        &c->lpRes.wR, &c->lpRes.wT, &c->lpRes.base, &c->lpRes.bestR,
                        &c->lpRes.bestT, &c->lpRes.bestScore, &c->pfQ, &c->pfQ2, &c->pfTr,
                        &c->pfTr2, &c->pfWR, &c->pfWT, &c->pfWRf, &c->pfWTf, &c->pfIdx,
@@ -973,6 +974,13 @@ int thx_ctx_set_local_gather(thx_ctx* c, int ns) {
     return THX_OK;
 }
 
+int thx_ctx_set_select_order(thx_ctx* c, int on) {
+    clear_error();
+    THX_CHECK(c && (on == 0 || on == 1), THX_ERR_ARG, "set_select_order: on must be 0 or 1");
+    c->selOrder = on != 0;
+    return THX_OK;
+}
+
 int thx_ctx_local_mode(thx_ctx* c, int* mode) {
     clear_error();
     THX_CHECK(c && mode, THX_ERR_ARG, "local_mode: bad argument");
@@ -1014,9 +1022,10 @@ int thx_ctx_get_local(thx_ctx* c, double* wC, double* wR, double* wT, double* ba
 
 // Scores nP particles (rows pmap or 0..nP-1) against per-particle selections
 // (ridx / tidx, null = whole grid) with the gather scorer and merges in FP64
-// into the loc* result buffers.
+// into the loc* result buffers.  `order` (nP particle indices, or null) is the
+// order in which the blocks take the particles; the results do not depend on it.
 static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, int nSelR,
-                          int nSelT, const int* pmap) {
+                          int nSelT, const int* pmap, const int* order) {
     const int TPAD = nSelT <= 32 ? 32 : (nSelT <= 64 ? 64 : 128);
     // 256 rotations per block share one staged Z tile when there are enough
     const int RB = nSelR > kLocalRotBlock ? 2 * kLocalRotBlock : kLocalRotBlock;
@@ -1047,7 +1056,7 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
     b.bWT = c->locBWT.as<float>();
     b.wRrel = c->locWRrel.as<float>();
     THX_TRY(launch_score_sel(c, TPAD, RB, (int)nblk, ridx, tidx, nSelR, nSelT, 1.f / c->R,
-                             1.f / c->T, pmap, b));
+                             1.f / c->T, pmap, order, b));
     THX_TRY(launch_merge_local(c, nP, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
                                c->locWT.as<double>(), c->locBase.as<double>(),
                                c->locBestR.as<int>(), c->locBestT.as<int>(), RB));
@@ -1143,8 +1152,25 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
                            c->trans.as<double>(), c->T, nSelR, nSelT, c->lpRidx.as<int>(),
                            c->lpRlogq.as<float>(), c->lpTidx.as<int>(), c->lpSpread.as<float>(),
                            c->lpQf.as<float4>()));
+    const int* order = nullptr;
+    if (c->selOrder) {
+        // particles in proposal-mode order: blocks resident together share rows
+        size_t tmpBytes = 0;
+        THX_TRY(pf_select_order(st, nullptr, C, P, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                &tmpBytes));
+        THX_TRY(c->lpOrder.ensure(sizeof(int) * P));
+        THX_TRY(c->lpOrdKeyA.ensure(sizeof(uint32_t) * P));
+        THX_TRY(c->lpOrdKeyB.ensure(sizeof(uint32_t) * P));
+        THX_TRY(c->lpOrdVal.ensure(sizeof(int) * P));
+        THX_TRY(c->lpOrdTmp.ensure(tmpBytes));
+        THX_TRY(pf_select_order(st, c->lpMix.as<double>(), C, P, c->lpOrdKeyA.as<uint32_t>(),
+                                c->lpOrdKeyB.as<uint32_t>(), c->lpOrdVal.as<int>(),
+                                c->lpOrder.as<int>(), c->lpOrdTmp.p, &tmpBytes));
+        order = c->lpOrder.as<int>();
+    }
     THX_TRY(record(c, "select"));
-    THX_TRY(score_selected(c, P, c->lpRidx.as<int>(), c->lpTidx.as<int>(), nSelR, nSelT, nullptr));
+    THX_TRY(score_selected(c, P, c->lpRidx.as<int>(), c->lpTidx.as<int>(), nSelR, nSelT, nullptr,
+                           order));
     THX_TRY(c->lpESS.ensure(sizeof(double) * P));
     THX_TRY(c->lpTail.ensure(sizeof(float) * P));
     THX_TRY(launch_sel_stats(st, c->locWR.as<double>(), c->lpRlogq.as<float>(),

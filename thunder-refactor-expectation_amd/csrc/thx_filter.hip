@@ -833,18 +833,6 @@ int thx_pf_from_global_given(thx_ctx* c, int nCls, int mLR, int mLT, double minS
 // sets larger than kSortMaxN are left as they are.
 constexpr int kSortMaxN = 1024;
 
-__device__ inline uint32_t quat_morton_dev(const double* q) {
-    const double s = q[0] < 0 ? -1.0 : 1.0;
-    uint32_t m = 0, c[3];
-    for (int a = 0; a < 3; a++) {
-        const double v = fmin(1.0, fmax(-1.0, s * q[a + 1]));
-        c[a] = (uint32_t)fmin(1023.0, floor((v + 1.0) * 512.0));
-    }
-    for (int b = 9; b >= 0; b--)
-        for (int a = 0; a < 3; a++) m = (m << 1) | ((c[a] >> b) & 1u);
-    return m;
-}
-
 __global__ __launch_bounds__(256) void k_sort_samples(double* __restrict__ q,
                                                       double* __restrict__ w,
                                                       float* __restrict__ wf, int n,

@@ -206,6 +206,9 @@ struct thx_ctx {
     thx::DevBuf img;
     // grid-restricted learned proposal: mixture, selections, results
     thx::DevBuf lpMix, lpRidx, lpRlogq, lpTidx, lpESS, lpSpread, lpTail, lpPmap;
+    // selected-pose scorer's particle order (Morton keys, sort scratch)
+    thx::DevBuf lpOrder, lpOrdKeyA, lpOrdKeyB, lpOrdVal, lpOrdTmp;
+    bool selOrder = true;
     thx::DevBuf lpDat, lpCtf, lpSig;      // gathered rows of fallen-back particles
     thx::ClassResult lpRes;               // their whole-grid scan
     std::vector<int> lpBestR, lpBestT, lpFell;
@@ -320,6 +323,20 @@ __host__ __device__ inline size_t vb_bytes(int V) {
     return (size_t)vb_nbx(V) * vb_nby(V) * vb_nbz(V) * 128;
 }
 int launch_vol_bricked(hipStream_t st, const float2* vol, int V, uint32_t* vb);
+// 30-bit Morton code of (x, y, z) of a unit quaternion taken with w >= 0
+// (q = -q), 10 bits per axis over [-1, 1]: nearby rotations get nearby codes
+// except across the w = 0 seam
+__device__ inline uint32_t quat_morton_dev(const double* q) {
+    const double s = q[0] < 0 ? -1.0 : 1.0;
+    uint32_t m = 0, c[3];
+    for (int a = 0; a < 3; a++) {
+        const double v = fmin(1.0, fmax(-1.0, s * q[a + 1]));
+        c[a] = (uint32_t)fmin(1023.0, floor((v + 1.0) * 512.0));
+    }
+    for (int b = 9; b >= 0; b--)
+        for (int a = 0; a < 3; a++) m = (m << 1) | ((c[a] >> b) & 1u);
+    return m;
+}
 // Whole-grid scan (class cls, precision) of the particles idx: their marginals
 // land in ctx->lpRes, rows in the order of idx (pinnedIdx: a pinned copy)
 int scan_subset(thx_ctx* c, int cls, int precision, const std::vector<int>& idx,
@@ -383,9 +400,14 @@ int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double
                    const double* gridT, int T, int nSelR, int nSelT, int* ridx, float* rlogq,
                    int* tidx, float* spread,
                    float4* gridQf);
+// Order of the particles for the selected-pose scorer: by a Morton code of
+// the dominant component's mean of each particle's proposal.  tmp == nullptr
+// returns the scratch size in *tmpBytes; keyA/keyB/val hold P entries.
+int pf_select_order(hipStream_t st, const double* mix, int C, int P, uint32_t* keyA,
+                    uint32_t* keyB, int* val, int* order, void* tmp, size_t* tmpBytes);
 int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, const int* tidx,
                      int nSelR, int nSelT, float pRu, float pTu, const int* pmap,
-                     const LocalBatch& b);
+                     const int* order, const LocalBatch& b);
 int launch_sel_stats(hipStream_t st, const double* wR, const float* rlogq, const int* ridx,
                      const int* tidx, int* bestR, int* bestT, int nSelR, int nSelT, int P,
                      double* ess, float* tail);

@@ -2202,8 +2202,13 @@ int launch_merge_local(thx_ctx* c, int P, int wtStride, const LocalBatch& b, dou
 // its nSelR selected grid rotations (rows of the shared slice table rotLo,
 // BF16) against its nSelT selected grid translations (rows of the phase
 // table S); the norm term is accumulated in FP32 from the 16-bit slice values.  A
-// block owns one particle x up to 128 of its selected rotations x all its
+// block owns one particle x up to RB of its selected rotations x all its
 // selected translations; epilogue and merge are those of the local scorer.
+// With an order table, hardware block b takes the particle order[j / nbp]
+// with j = xcd_logical(b): each XCD walks a contiguous run of the ordered
+// particles, so the blocks resident on one XCD at a time score particles whose
+// selected slice rows overlap and share its L2.  Outputs are indexed by the
+// particle's own block (p nbp + rotation block) either way.
 // ---------------------------------------------------------------------------
 template <int TPAD, int RB>
 __global__ __launch_bounds__(2 * RB) void k_score_sel(
@@ -2211,8 +2216,9 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     const float2* __restrict__ S, int Kp, const float2* __restrict__ Y,
     const float* __restrict__ Wt, const double* __restrict__ Aterm, const int* __restrict__ ridx,
     const int* __restrict__ tidx, int nSelR, int nSelT, float pRu, float pTu,
-    const int* __restrict__ pmap, float* __restrict__ bMax, float* __restrict__ bWC,
-    int* __restrict__ bArg, float* __restrict__ bWT, float* __restrict__ wRrel) {
+    const int* __restrict__ pmap, const int* __restrict__ order, float* __restrict__ bMax,
+    float* __restrict__ bWC, int* __restrict__ bArg, float* __restrict__ bWT,
+    float* __restrict__ wRrel) {
     constexpr int LD = 72;
     constexpr int MT = TPAD / 32;
     constexpr int NTH = 2 * RB, NW = NTH / 64;   // 128 rotations: 4 waves; 256: 8
@@ -2227,7 +2233,12 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     __shared__ float rowW[TPAD];
 
     const int nbp = (nSelR + RB - 1) / RB;
-    const int p = blockIdx.x / nbp, r0 = (blockIdx.x % nbp) * RB;
+    int bl = blockIdx.x;   // the particle's own block index
+    if (order) {
+        const int j = xcd_logical(blockIdx.x, gridDim.x);
+        bl = order[j / nbp] * nbp + j % nbp;
+    }
+    const int p = bl / nbp, r0 = (bl % nbp) * RB;
     const int pd = pmap ? pmap[p] : p;   // particle row of Y / Wt / Aterm
     const int nr = min(RB, nSelR - r0);
     const int Tp = nSelT;
@@ -2317,6 +2328,11 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
         // own chunk, unused): a conditional load would make the compiler merge
         // old and new register values with copies on every iteration
         gload(k0 + kChunk < Kp ? k0 + kChunk : k0);
+        // keep the loads ahead of the MFMA steps: left free, the scheduler
+        // interleaves the norm term's dot products with the MFMAs and sinks
+        // the loads (which overwrite those registers) to the end of the
+        // iteration, where nothing hides their latency
+        __builtin_amdgcn_sched_barrier(0);
         const int nrow = 32 * wave + l32;
 #pragma unroll
         for (int kk = 0; kk < 64; kk += 16) {
@@ -2418,26 +2434,27 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     if (tid < TPAD) {
         float rsum = 0.f;
         for (int w = 0; w < NW; w++) rsum += rowPart[w][tid];
-        bWT[(size_t)blockIdx.x * TPAD + tid] = rsum;
+        bWT[(size_t)bl * TPAD + tid] = rsum;
         rowW[tid] = tid < Tp ? rsum * pTu : 0.f;
     }
     __syncthreads();
     if (tid == 0) {
         float wc = 0.f;
         for (int t = 0; t < TPAD; t++) wc += rowW[t];
-        bMax[blockIdx.x] = M;
-        bWC[blockIdx.x] = wc;
-        bArg[blockIdx.x] = arg < 0 ? -1 : ((r0 + arg / TPAD) * kNBL + arg % TPAD);
+        bMax[bl] = M;
+        bWC[bl] = wc;
+        bArg[bl] = arg < 0 ? -1 : ((r0 + arg / TPAD) * kNBL + arg % TPAD);
     }
 }
 
 int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, const int* tidx,
                      int nSelR, int nSelT, float pRu, float pTu, const int* pmap,
-                     const LocalBatch& b) {
+                     const int* order, const LocalBatch& b) {
 #define THX_SEL_ARGS c->rotLo.as<const uint32_t>(), c->normP.as<const float>(),            \
                      c->S.as<const float2>(), c->Kp, c->Y.as<const float2>(),              \
                      c->Wt.as<const float>(), c->Aterm.as<const double>(), ridx, tidx,     \
-                     nSelR, nSelT, pRu, pTu, pmap, b.bMax, b.bWC, b.bArg, b.bWT, b.wRrel
+                     nSelR, nSelT, pRu, pTu, pmap, order, b.bMax, b.bWC, b.bArg, b.bWT,   \
+                     b.wRrel
     THX_CHECK(RB == 128 || RB == 256, THX_ERR_ARG, "score_sel: rotation block must be 128 or 256");
     // the loads address the slice and phase tables with 32-bit byte offsets
     THX_CHECK(c->rotLo.bytes <= (1ull << 32) && c->S.bytes <= (1ull << 32), THX_ERR_RANGE,

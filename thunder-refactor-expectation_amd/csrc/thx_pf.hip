@@ -12,6 +12,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include <hipcub/device/device_radix_sort.hpp>
+
 #include "thx_internal.hpp"
 
 namespace thx {
@@ -469,6 +471,42 @@ __global__ __launch_bounds__(256) void k_select_grid(
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) base += red[w];
         __syncthreads();
     }
+}
+
+// Sort key of each particle for the selected-pose scorer: the Morton code of
+// the mean of its proposal's dominant component, the one with the highest
+// density at its mean (pi_c k_c^{-3/2}: the selection's component density
+// pi k^{-3/2} / (2 pi^2) / ((1 - 1/k) (m.q)^2 + 1/k)^2 at m.q = +-1).
+// Particles with nearby keys select overlapping grid rows.
+__global__ void k_sel_keys(const double* __restrict__ mix, int C, int P,
+                           uint32_t* __restrict__ key, int* __restrict__ val) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const double* mp = mix + (size_t)p * (6 * C + 3);
+    int best = 0;
+    double bv = -1.0;
+    for (int c = 0; c < C; c++) {
+        const double k = fmax(mp[6 * c + 4], 1e-300);
+        const double v = mp[6 * c + 5] / (k * sqrt(k));
+        if (v > bv) { bv = v; best = c; }
+    }
+    key[p] = quat_morton_dev(mp + 6 * best);
+    val[p] = p;
+}
+
+int pf_select_order(hipStream_t st, const double* mix, int C, int P, uint32_t* keyA,
+                    uint32_t* keyB, int* val, int* order, void* tmp, size_t* tmpBytes) {
+    if (!tmp) {
+        THX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, *tmpBytes, keyA, keyB, val, order, P,
+                                                   0, 30, st));
+        return THX_OK;
+    }
+    hipLaunchKernelGGL(k_sel_keys, dim3((P + 255) / 256), dim3(256), 0, st, mix, C, P, keyA, val);
+    THX_HIP(hipGetLastError());
+    // stable LSD radix sort over the 30 key bits: equal keys keep particle order
+    THX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, *tmpBytes, keyA, keyB, val, order, P, 0, 30,
+                                               st));
+    return THX_OK;
 }
 
 int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double* gridQ, int R,

@@ -60,6 +60,7 @@ _SIGS = {
     "thx_ctx_set_local_options": [_p, _i, _i],
     "thx_ctx_local_mode": [_p, _p],
     "thx_ctx_set_local_gather": [_p, _i],
+    "thx_ctx_set_select_order": [_p, _i],
     "thx_ctx_score_local": [_p, _p, _p, _p, _p, _p, _p, _p, _i],
     "thx_ctx_get_local": [_p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
@@ -564,6 +565,11 @@ class Context:
         m = C.c_int(0)
         check(load().thx_ctx_local_mode(self._h, C.byref(m)), "local_mode")
         return m.value & 15, (m.value >> 4) or 4
+
+    def set_select_order(self, on=True):
+        """Particle order of the learned expectation's selected-pose scorer
+        (proposal-mode order when on; index order when off)."""
+        check(load().thx_ctx_set_select_order(self._h, int(bool(on))), "set_select_order")
 
     def set_local_gather(self, ns=0):
         """Samples in flight per gather step of the 16-bit local scorers (4, 8;
