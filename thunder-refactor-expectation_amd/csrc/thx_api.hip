@@ -386,7 +386,8 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->locBestR, &c->locBestT, &c->img, &c->lpMix, &c->lpRidx,
                        &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpSpread, &c->lpTail,
                        &c->lpPmap, &c->lpOrder, &c->lpOrdKeyA, &c->lpOrdKeyB, &c->lpOrdVal,
-                       &c->lpOrdTmp, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpQf, &c->lpRes.wC,
+                       &c->lpOrdTmp, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpQf, &c->S16,
+                       &c->lpRes.wC,
                        &c->lpRes.wR, &c->lpRes.wT, &c->lpRes.base, &c->lpRes.bestR,
                        &c->lpRes.bestT, &c->lpRes.bestScore, &c->pfQ, &c->pfQ2, &c->pfTr,
                        &c->pfTr2, &c->pfWR, &c->pfWT, &c->pfWRf, &c->pfWTf, &c->pfIdx,
@@ -1055,6 +1056,8 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
     b.bArg = c->locBArg.as<int>();
     b.bWT = c->locBWT.as<float>();
     b.wRrel = c->locWRrel.as<float>();
+    // the phase table as the scorer reads it (FP16), from the current S
+    THX_TRY(launch_phase_half(c));
     THX_TRY(launch_score_sel(c, TPAD, RB, (int)nblk, ridx, tidx, nSelR, nSelT, 1.f / c->R,
                              1.f / c->T, pmap, order, b));
     THX_TRY(launch_merge_local(c, nP, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),

@@ -132,6 +132,7 @@ struct thx_ctx {
     std::vector<char> volLoOk;
     thx::DevBuf locVolsLo;
     thx::DevBuf lpQf;        // grid quaternions as float4 for the selection
+    thx::DevBuf S16;         // FP16 copy of the phase table S (selected-pose scorer)
     thx::HostBuf lpStage;    // pinned: per-particle results read back in one batch
     // ordering events between the context stream and the proposal network's
     hipEvent_t lpReady = nullptr, lpDone = nullptr, lpRead = nullptr;
@@ -240,6 +241,8 @@ namespace thx {
 int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
                    float2* out, float* outNorm);
 int launch_translate_table(thx_ctx* c);
+// S16 <- S in FP16 (TpadS x Kp), on the context stream
+int launch_phase_half(thx_ctx* c);
 int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, const int* iCol,
                          const int* iRow, int K, int Kp, int N, float2* S);
 int launch_prep_particles(thx_ctx* c, int precision);

@@ -299,6 +299,26 @@ int launch_translate_raw(hipStream_t st, const double* trans, int T, int Tpad, c
     return THX_OK;
 }
 
+// FP16 copy of the phase table (re, im packed in 32 bits) for the
+// selected-pose scorer.
+__global__ void k_phase_half(const float2* __restrict__ S, size_t n, uint32_t* __restrict__ S16) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 v = S[i];
+    const unsigned short lo = __builtin_bit_cast(unsigned short, (_Float16)v.x);
+    const unsigned short hi = __builtin_bit_cast(unsigned short, (_Float16)v.y);
+    S16[i] = (uint32_t)lo | ((uint32_t)hi << 16);
+}
+
+int launch_phase_half(thx_ctx* c) {
+    const size_t n = (size_t)c->TpadS * c->Kp;
+    THX_TRY(c->S16.ensure(sizeof(uint32_t) * n));
+    hipLaunchKernelGGL(k_phase_half, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       c->S.as<const float2>(), n, c->S16.as<uint32_t>());
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
 int launch_translate_table(thx_ctx* c) {
     dim3 grid(c->TpadS, (c->Kp + 255) / 256);
     hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, c->stream, c->trans.as<const double>(),
@@ -2201,7 +2221,9 @@ int launch_merge_local(thx_ctx* c, int P, int wtStride, const LocalBatch& b, dou
 // Selected-pose scorer of the grid-restricted proposal.  Particle p scores
 // its nSelR selected grid rotations (rows of the shared slice table rotLo,
 // BF16) against its nSelT selected grid translations (rows of the phase
-// table S); the norm term is accumulated in FP32 from the 16-bit slice values.  A
+// table, read from its FP16 copy S16: a unit phase in FP16 is within 2^-12
+// per component, well inside the BF16 rounding of Z = Y conj(S) that
+// follows); the norm term is accumulated in FP32 from the 16-bit slice values.  A
 // block owns one particle x up to RB of its selected rotations x all its
 // selected translations; epilogue and merge are those of the local scorer.
 // With an order table, hardware block b takes the particle order[j / nbp]
@@ -2213,7 +2235,7 @@ int launch_merge_local(thx_ctx* c, int P, int wtStride, const LocalBatch& b, dou
 template <int TPAD, int RB>
 __global__ __launch_bounds__(2 * RB) void k_score_sel(
     const uint32_t* __restrict__ rotLo, const float* __restrict__ normP,
-    const float2* __restrict__ S, int Kp, const float2* __restrict__ Y,
+    const uint32_t* __restrict__ S16, int Kp, const float2* __restrict__ Y,
     const float* __restrict__ Wt, const double* __restrict__ Aterm, const int* __restrict__ ridx,
     const int* __restrict__ tidx, int nSelR, int nSelT, float pRu, float pTu,
     const int* __restrict__ pmap, const int* __restrict__ order, float* __restrict__ bMax,
@@ -2249,21 +2271,22 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     if (tid < TPAD) trow[tid] = tid < Tp ? (tidx ? tidx[(size_t)p * nSelT + tid] : tid) : -1;
     __syncthreads();
 
-    // Loads are 16 bytes per lane: slice rows as 4 packed pixels (lanes
-    // l8 = tid & 7 of eight consecutive lanes cover a row's 32-pixel chunk,
-    // rows rq + NTH/8 q), phase rows as 2 complex pixels (l16 = tid & 15,
-    // rows tq + NTH/16 j).  The norm term is taken from the 16-bit slice values, so
-    // no |P|^2 table is read.  Rows past the selection read row 0 of their
-    // table: their columns (rotations) and rows (translations) of the tile are
-    // masked in the epilogue, so the loads need no predicate.
-    constexpr int NJ = TPAD / (NTH / 16);
+    // Loads are 16 bytes per lane, 4 pixels: lanes l8 = tid & 7 of eight
+    // consecutive lanes cover a row's 32-pixel chunk, slice rows rq + NTH/8 q
+    // and phase rows rq + NTH/8 j (the particle's Y and weights at the same
+    // pixels).  The norm term is taken from the 16-bit slice values, so no
+    // |P|^2 table is read.  Rows past the selection read row 0 of their table:
+    // their columns (rotations) and rows (translations) of the tile are masked
+    // in the epilogue, so the loads need no predicate; phase rows past TPAD
+    // (TPAD < NTH/8) are loaded but not staged.
+    constexpr int SR = NTH / 8;                  // phase rows per pass
+    constexpr int NJ = (TPAD + SR - 1) / SR;
     const int l8 = tid & 7, rq = tid >> 3;
-    const int l16 = tid & 15, tq = tid >> 4;
     // per-lane 32-bit byte offsets from uniform bases (the tables stay below
     // 4 GiB): each load is a scalar base plus a vector offset, with no 64-bit
     // address arithmetic per load
     const char* bBase = reinterpret_cast<const char*>(rotLo);
-    const char* sBase = reinterpret_cast<const char*>(S);
+    const char* sBase = reinterpret_cast<const char*>(S16);
     const char* yBase = reinterpret_cast<const char*>(Y + (size_t)pd * Kp);
     const char* wBase = reinterpret_cast<const char*>(Wt + (size_t)pd * Kp);
     uint32_t boff[4], soff[NJ];
@@ -2274,25 +2297,27 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
     }
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        const int t = trow[tq + (NTH / 16) * j];
-        soff[j] = ((uint32_t)(t >= 0 ? t : 0) * (uint32_t)Kp + 2u * l16) * 8u;
+        const int tr = rq + SR * j;
+        const int t = tr < TPAD ? trow[tr] : -1;
+        soff[j] = ((uint32_t)(t >= 0 ? t : 0) * (uint32_t)Kp + 4u * l8) * 4u;
     }
-    const uint32_t yoff = 16u * l16, woff = 16u * l8;
+    const uint32_t yoff = 32u * l8, woff = 16u * l8;
 
     floatx16 acc[MT];
 #pragma unroll
     for (int a = 0; a < MT; a++) acc[a] = floatx16{0};
     float cpart[4] = {0.f, 0.f, 0.f, 0.f};
     u32x4 bv[4];
-    float4 sv[NJ];
-    float4 yv, wv;
+    u32x4 sv[NJ];
+    float4 yv0, yv1, wv;
     auto gload = [&](int k0) {
         const uint32_t kb = (uint32_t)k0 * 4u, ks = (uint32_t)k0 * 8u;   // chunk byte offsets
 #pragma unroll
         for (int q = 0; q < 4; q++) bv[q] = *reinterpret_cast<const u32x4*>(bBase + (boff[q] + kb));
 #pragma unroll
-        for (int j = 0; j < NJ; j++) sv[j] = *reinterpret_cast<const float4*>(sBase + (soff[j] + ks));
-        yv = *reinterpret_cast<const float4*>(yBase + (yoff + ks));
+        for (int j = 0; j < NJ; j++) sv[j] = *reinterpret_cast<const u32x4*>(sBase + (soff[j] + kb));
+        yv0 = *reinterpret_cast<const float4*>(yBase + (yoff + ks));
+        yv1 = *reinterpret_cast<const float4*>(yBase + (yoff + ks + 16u));
         wv = *reinterpret_cast<const float4*>(wBase + (woff + kb));
     };
     gload(0);
@@ -2315,13 +2340,20 @@ __global__ __launch_bounds__(2 * RB) void k_score_sel(
             *reinterpret_cast<u32x4*>(Bs + (rq + (NTH / 8) * q) * LD + 8 * l8) = bv[q];
         }
         // ---- A: Z = Y conj(S_t) ----
+        const float yr[4] = {yv0.x, yv0.z, yv1.x, yv1.z}, yi[4] = {yv0.y, yv0.w, yv1.y, yv1.w};
 #pragma unroll
         for (int j = 0; j < NJ; j++) {
-            const float4 sj = sv[j];
-            uint2 z;
-            z.x = pack16<THX_BF16>(yv.x * sj.x + yv.y * sj.y, yv.y * sj.x - yv.x * sj.y);
-            z.y = pack16<THX_BF16>(yv.z * sj.z + yv.w * sj.w, yv.w * sj.z - yv.z * sj.w);
-            *reinterpret_cast<uint2*>(As + (tq + (NTH / 16) * j) * LD + 4 * l16) = z;
+            const int tr = rq + SR * j;
+            u32x4 z;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const uint32_t u = sv[j][e];
+                const float sr = (float)__builtin_bit_cast(_Float16, (unsigned short)(u & 0xffffu));
+                const float si = (float)__builtin_bit_cast(_Float16, (unsigned short)(u >> 16));
+                z[e] = pack16<THX_BF16>(yr[e] * sr + yi[e] * si, yi[e] * sr - yr[e] * si);
+            }
+            if (SR * NJ == TPAD || tr < TPAD)
+                *reinterpret_cast<u32x4*>(As + tr * LD + 8 * l8) = z;
         }
         __syncthreads();
         // the next chunk's loads, unconditional (the last iteration reloads its
@@ -2451,14 +2483,16 @@ int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, co
                      int nSelR, int nSelT, float pRu, float pTu, const int* pmap,
                      const int* order, const LocalBatch& b) {
 #define THX_SEL_ARGS c->rotLo.as<const uint32_t>(), c->normP.as<const float>(),            \
-                     c->S.as<const float2>(), c->Kp, c->Y.as<const float2>(),              \
+                     c->S16.as<const uint32_t>(), c->Kp, c->Y.as<const float2>(),          \
                      c->Wt.as<const float>(), c->Aterm.as<const double>(), ridx, tidx,     \
                      nSelR, nSelT, pRu, pTu, pmap, order, b.bMax, b.bWC, b.bArg, b.bWT,   \
                      b.wRrel
     THX_CHECK(RB == 128 || RB == 256, THX_ERR_ARG, "score_sel: rotation block must be 128 or 256");
     // the loads address the slice and phase tables with 32-bit byte offsets
-    THX_CHECK(c->rotLo.bytes <= (1ull << 32) && c->S.bytes <= (1ull << 32), THX_ERR_RANGE,
+    THX_CHECK(c->rotLo.bytes <= (1ull << 32) && c->S16.bytes <= (1ull << 32), THX_ERR_RANGE,
               "score_sel: slice or phase table above 4 GiB");
+    THX_CHECK(c->S16.bytes >= sizeof(uint32_t) * (size_t)c->TpadS * c->Kp, THX_ERR_STATE,
+              "score_sel: FP16 phase table not built");
     if (RB == 128) {
         if (TPAD == 32)
             hipLaunchKernelGGL((k_score_sel<32, 128>), dim3(nblk), dim3(256), 0, c->stream, THX_SEL_ARGS);
