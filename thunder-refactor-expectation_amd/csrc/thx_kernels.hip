@@ -1189,32 +1189,40 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     //      step; rows m = tid / 16 + 32 v (v = 0..3).  Row m is particle
     //      m / TC, translation m % TC: a thread needs NT = TC / 32 phase rows
     //      and 4 / NT particle rows, one float4 each.
+    //      With TC = 32 (8 particles x 32 translations) a thread instead owns
+    //      particles pi, pi + 4 and translations ti + 8 u (u = 0..3), with
+    //      ti = rbase % 8 and pi = rbase / 8: 2 particle rows and 4 phase
+    //      rows (6 loads, not 9), rows (pi + 4 j) TC + ti + 8 u.
     //   B: lane part8 = tid % 8 owns 4 complex pixels of rows tid / 8 + 64 v
     //      (FP8: lane part4 = tid % 4 owns 8 pixels of rows tid / 4 + 128 v).
-    constexpr int NT = TC / 32;
-    constexpr int NY = NV / NT;
+    constexpr bool T32 = TC == 32;
+    constexpr int NT = T32 ? 4 : TC / 32;
+    constexpr int NY = T32 ? 2 : NV / NT;
     const int part = tid & 15, rbase = tid >> 4;      // rbase 0..31
     const int part8 = tid & 7, rb8 = tid >> 3;        // rb8 0..63
-    const float4* yp = reinterpret_cast<const float4*>(Y + (size_t)(p0 + rbase / TC) * Kp) + part;
-    const float4* sp = reinterpret_cast<const float4*>(S + (size_t)(t0 + rbase % TC) * Kp) + part;
+    const int ti = rbase & 7, pi = rbase >> 3;        // TC = 32 roles
+    const float4* yp = reinterpret_cast<const float4*>(
+                           Y + (size_t)(p0 + (T32 ? pi : rbase / TC)) * Kp) + part;
+    const float4* sp = reinterpret_cast<const float4*>(
+                           S + (size_t)(t0 + (T32 ? ti : rbase % TC)) * Kp) + part;
     const int part4 = tid & 3, rb4 = tid >> 2;        // rb4 0..127 (FP8)
     const u32x4* bp = F8 ? reinterpret_cast<const u32x4*>(
                                reinterpret_cast<const unsigned char*>(Bop) + (size_t)(r0 + rb4) * 2 * Kp) + part4
                          : reinterpret_cast<const u32x4*>(Bop + (size_t)(r0 + rb8) * Kp) + part8;
     const size_t rowF4 = (size_t)Kp / 2;
     const size_t rowU4 = F8 ? (size_t)Kp / 8 : (size_t)Kp / 4;   // B row length in 16-byte units
-    // row m = rbase + 32 v: particle (rbase + 32 v) / TC, translation (rbase + 32 v) % TC
-    //   TC = 32:  particle v,        translation rbase           (NY = 4, NT = 1)
-    //   TC = 64:  particle v / 2,    translation rbase + 32 (v%2) (NY = 2, NT = 2)
-    //   TC = 128: particle 0,        translation rbase + 32 v     (NY = 1, NT = 4)
+    // staged row of step v (v = 0 .. 7): Y row v / NT, phase row v % NT
+    //   TC = 32:  particle pi + 4 (v/4), translation ti + 8 (v%4)  (NY = 2, NT = 4)
+    //   TC = 64:  particle v / 2,    translation rbase + 32 (v%2) (NY = 4, NT = 2)
+    //   TC = 128: particle v / 4 (= 0 .. 1), translation rbase + 32 (v%4) (NY = 2, NT = 4)
 
     // (plain macro, not a lambda: a lambda capturing the staging arrays by
     // reference keeps them in scratch memory instead of registers)
 #define THX_GLOAD16(K0)                                                          \
     do {                                                                         \
         const int f4_ = (K0) / 2, u4_ = (K0) / 4;                                \
-        _Pragma("unroll") for (int q = 0; q < NY; q++) ry[q] = yp[(size_t)q * rowF4 + f4_]; \
-        _Pragma("unroll") for (int u = 0; u < NT; u++) rs[u] = sp[(size_t)32 * u * rowF4 + f4_]; \
+        _Pragma("unroll") for (int q = 0; q < NY; q++) ry[q] = yp[(size_t)(T32 ? 4 : 1) * q * rowF4 + f4_]; \
+        _Pragma("unroll") for (int u = 0; u < NT; u++) rs[u] = sp[(size_t)(T32 ? 8 : 32) * u * rowF4 + f4_]; \
         if constexpr (F8) {                                                      \
             _Pragma("unroll") for (int v = 0; v < 2; v++) rbv[v] = bp[(size_t)128 * v * rowU4 + (K0) / 8]; \
         } else {                                                                 \
@@ -1228,17 +1236,19 @@ __global__ __launch_bounds__(512, 1) void k_score16(
     do {                                                                                       \
         _Pragma("unroll") for (int v = 0; v < NV; v++) {                                       \
             const float4 y = ry[v / NT], sv = rs[v % NT];                                      \
+            /* row of (Y row v / NT, phase row v % NT) */                                      \
+            const int arow_ = T32 ? (pi + 4 * (v / NT)) * TC + ti + 8 * (v % NT) : rbase + 32 * v; \
             const float z0r = y.x * sv.x + y.y * sv.y, z0i = y.y * sv.x - y.x * sv.y;         \
             const float z1r = y.z * sv.z + y.w * sv.w, z1i = y.w * sv.z - y.z * sv.w;         \
             if constexpr (F8) {                                                                \
                 int z8 = __builtin_amdgcn_cvt_pk_fp8_f32(z0r, z0i, 0, false);                  \
                 z8 = __builtin_amdgcn_cvt_pk_fp8_f32(z1r, z1i, z8, true);                      \
-                *reinterpret_cast<int*>(AsB[BUF] + (rbase + 32 * v) * LDB + 4 * part) = z8;   \
+                *reinterpret_cast<int*>(AsB[BUF] + arow_ * LDB + 4 * part) = z8;              \
             } else {                                                                           \
                 uint2 z;                                                                       \
                 z.x = pack16<PREC>(z0r, z0i);                                                  \
                 z.y = pack16<PREC>(z1r, z1i);                                                  \
-                *reinterpret_cast<uint2*>(AsB[BUF] + (rbase + 32 * v) * LDB + 8 * part) = z;  \
+                *reinterpret_cast<uint2*>(AsB[BUF] + arow_ * LDB + 8 * part) = z;             \
             }                                                                                  \
         }                                                                                      \
         if constexpr (F8) {                                                                    \
