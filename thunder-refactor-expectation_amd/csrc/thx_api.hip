@@ -417,7 +417,7 @@ int thx_ctx_set_pixels(thx_ctx* c, const int* iCol, const int* iRow, const int* 
     THX_CHECK(c && iCol && iRow, THX_ERR_ARG, "set_pixels: null argument");
     THX_CHECK(K > 0 && idim > 0 && pf > 0, THX_ERR_ARG, "set_pixels: bad size");
     THX_TRY(set_device(c));
-    const int Kp = round_up(K, kChunk);
+    const int Kp = round_up(K, kKAlign);
     std::vector<int> col(Kp, 0), row(Kp, 0), sg(Kp, 0);
     int maxShell = 0;
     double maxR = 0;
@@ -1773,7 +1773,7 @@ int thx_ExpectGlobal3D(const float* rotP, const float* traP, const float* datP, 
     THX_TRY(set_device(c));
     // pixel set of npxl entries (shell indices are not needed at FP32)
     c->K = npxl;
-    c->Kp = round_up(npxl, kChunk);
+    c->Kp = round_up(npxl, kKAlign);
     c->maxShell = 0;
     THX_TRY(c->iSig.ensure(sizeof(int) * c->Kp));
     THX_HIP(hipMemset(c->iSig.p, 0, sizeof(int) * c->Kp));

@@ -76,6 +76,9 @@ int upload(DevBuf& d, const T* h, size_t n, hipStream_t st) {
 constexpr int kTileM = 128;
 constexpr int kTileN = 128;
 constexpr int kChunk = 32;     // complex pixels per K step
+// padded pixel count Kp: a multiple of two K steps (zero pixels past K), so
+// the whole-grid scorer's loop, unrolled by two, has no odd tail
+constexpr int kKAlign = 2 * kChunk;
 constexpr int kMaxTiles = 8192; // nTc * nRb per particle handled by the merge
 
 struct ClassResult {
