@@ -315,8 +315,12 @@ def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
                     precision=thx.BF16)
     ctx.sync()
     t_sc = ctx.last_timing()["score_local"] * 1e-3
+    _, ns_r = ctx.local_shape()   # gather depth of this launch (tuned for its size)
     ctx.set_local_options(0, False)
-    key = "k_score_local"
+    # the launch's own kernel (BF16 = 1, 32-row translation tile, its gather
+    # depth): the tuner runs both depths on this size once, so the profile
+    # holds a figure for each at this grid
+    key = f"k_score_local<1, 32, {ns_r}>"
     traffic = traffic_bytes(name, key, grid=P * (nRr // 128) * 256)
     samples_r = float(P) * nRr * K
     # logical gather bytes: 8 BF16 complex corners (4 bytes each) per trilinear sample
@@ -338,7 +342,7 @@ def run_local_search(ctx, info, name, phases, mLR=300, mLT=25):
             "launch": ["", "direct", "persistent work queue"][max(0, mode)]
                       + f", {ns} samples in flight per gather step",
             "roofline_batch": f"{P} particles x {nRr} rotations (4 deg about the true pose) "
-                              f"x {mLT} shifts, BF16, direct launch",
+                              f"x {mLT} shifts, BF16, direct launch, {ns_r} samples in flight",
             "roofline": roof,
             "note": "raw pose-evals (each particle's own samples); projection fused into the "
                     "scorer, one trilinear slice per (particle, rotation)"}

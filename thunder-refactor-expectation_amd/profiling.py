@@ -47,7 +47,7 @@ PREC = {"0": "fp32", "1": "bf16", "2": "fp16", "3": "fp8"}
 
 
 def simple_key(s):
-    m = re.match(r"k_score16<(\d), \d+>", s)
+    m = re.match(r"k_score16<(\d), \d+(?:, \d+)?>", s)
     if m:
         return f"k_score16_{PREC[m.group(1)]}"
     m = re.match(r"k_score<(\d), \d+>", s)
@@ -173,14 +173,23 @@ def summarize(a):
           "|---|---|---|---|---|---|"]
     # every grid size of every kernel, for lookups of one specific launch
     by_grid = {}
+    ambiguous = set()
     for (k, g), vs in fc.items():
         if (k, g) not in wc:
             continue
         e = {"read_GB": sum(vs) / len(vs) * 1024 * fs / 1e9,
              "write_GB": sum(wc[(k, g)]) / len(wc[(k, g)]) * 1024 * ws / 1e9}
-        for name in (k, simple_key(k)):
-            if name:
-                by_grid.setdefault(name, {})[str(g)] = e
+        by_grid.setdefault(k, {})[str(g)] = e
+        sk = simple_key(k)
+        if sk:
+            # under the short name a grid two variants share is ambiguous: dropped
+            # (the full name still has it)
+            seen = by_grid.setdefault(sk, {})
+            if str(g) in seen or (sk, str(g)) in ambiguous:
+                seen.pop(str(g), None)
+                ambiguous.add((sk, str(g)))
+            else:
+                seen[str(g)] = e
     for k, (g, f, w) in sorted(best.items(), key=lambda x: -(x[1][1] * fs + x[1][2] * ws)):
         e = {"grid": g, "read_GB": f * 1024 * fs / 1e9, "write_GB": w * 1024 * ws / 1e9}
         kern[k] = dict(e, by_grid=by_grid.get(k, {}))
