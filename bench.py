@@ -94,9 +94,26 @@ def prepare_learned(name, step, train_samples, cal_particles, rule=""):
                          sd["particles"] + 7919, **wl.CTF_DEFAULTS)
     quat, trans = wl.grid(name)
     cctx.set_grid(quat, trans)
+    # a second held-out set from the map as one refinement iteration might
+    # change it (blob centres + N(0, 0.5^2) voxels, amplitudes x (1 + N(0,
+    # 0.1^2)); its own seeds): the rule must hold there too, so that a small
+    # map change needs no new rule
     t0 = time.perf_counter()
-    cal = learned.calibrate(cctx, net, step, log=log)
+    rctx = thx.Context(0)
+    rctx.set_pixels(iCol, iRow, iSig, w["box"], w["pf"])
+    cen = rctx.synth_volume(0, w["box"], w["pf"], w["n_blob"], w["blob_sigma"],
+                            w["blob_radius"], sd["volume"])
+    rng = np.random.default_rng(1717)
+    rctx.synth_volume_blobs(0, w["box"], w["pf"], cen + rng.normal(0.0, 0.5, cen.shape),
+                            w["blob_sigma"], 1.0 + rng.normal(0.0, 0.1, len(cen)))
+    rctx.synth_particles(0, cal_particles, w["max_shift"], w["df_min"], w["df_max"], w["snr"],
+                         sd["particles"] + 7919 + 1, **wl.CTF_DEFAULTS)
+    rctx.set_grid(quat, trans)
+    cal = learned.calibrate(cctx, net, step, log=log, robust=(rctx,))
     cal["seconds"] = time.perf_counter() - t0
+    cal["robust_set"] = ("reference map with blob centres + N(0, 0.5^2) vox, amplitudes "
+                         "x (1 + N(0, 0.1^2)), seed 1717")
+    rctx.close()
     cctx.close()
     cal["calibrated"] = True
     log(f"calibration: {cal}")
@@ -105,7 +122,7 @@ def prepare_learned(name, step, train_samples, cal_particles, rule=""):
 
 def run_learned(ctx, net, cal, steps, warmup, P, R, T):
     args = (net, 0, cal["n_rot"], cal["n_trans"], 0.0, cal["spread_max"], cal["tail_max"],
-            cal["score_drop"], thx.PRECISIONS[cal["fallback"]])
+            cal["score_drop"], learned.fallback_setup(ctx, cal["fallback"]))
     for _ in range(warmup):
         ctx.expect_learned(*args)
     ctx.sync()
@@ -192,9 +209,9 @@ def run_learned_finetune(name, net, cal, step, sigma_c=0.5, sigma_a=0.1,
     is fine-tuned from its weights on particles of the new map in increments
     (budget_m, millions of samples) until the learned step's assignments agree
     with the plain FP32 scan on >= 99% of the particles, with the calibrated
-    selection and fallback rule unchanged.  per_iteration_ms = the learned
-    step plus the fine-tuning it needed, the whole cost per refinement
-    iteration."""
+    selection and fallback rule unchanged (it was calibrated to hold on a
+    perturbed map too).  per_iteration_ms = the learned step plus the
+    fine-tuning it needed, the whole cost per refinement iteration."""
     w = wl.WORKLOADS[name]
     sd = wl.seeds(name)
     iCol, iRow, iSig = wl.pixel_set(w["box"], w["cutoff"])
@@ -217,7 +234,7 @@ def run_learned_finetune(name, net, cal, step, sigma_c=0.5, sigma_a=0.1,
     ectx.score_global(0, thx.FP32)
     gR, gT, _ = ectx.get_best(0)
     args = (net, 0, cal["n_rot"], cal["n_trans"], 0.0, cal["spread_max"], cal["tail_max"],
-            cal["score_drop"], thx.PRECISIONS[cal["fallback"]])
+            cal["score_drop"], learned.fallback_setup(ectx, cal["fallback"]))
 
     def evaluate():
         ectx.expect_learned(*args)
@@ -770,6 +787,24 @@ def cpu_vs_gpu_box64(steps, reps=3):
             "unit": "pose-evals/s"}
 
 
+def sel_roofline(name, K, n, n_rot, n_trans, ms):
+    """Roofline of one selected-pose scorer launch over n particles x n_rot x
+    n_trans selected poses that took `ms` (the stage: scorer, FP64 merge and
+    statistics); traffic from the profile for that template variant and grid."""
+    fl = 4.0 * K * n * n_rot * n_trans + 2.0 * K * n * n_rot
+    tpad = 32 if n_trans <= 32 else (64 if n_trans <= 64 else 128)
+    rb = 256 if n_rot > 128 else 128
+    nbp = (n_rot + rb - 1) // rb
+    key = f"k_score_sel<{tpad}, {rb}>"
+    t = ms * 1e-3
+    return {"bound": "mfma", "achieved": fl / t / 1e12, "peak": PEAK["bf16"] / 1e12,
+            "unit": "TFLOP/s", "frac": fl / t / PEAK["bf16"],
+            "traffic": traffic_bytes(name, key, grid=n * nbp * 2 * rb), "kernel": key,
+            "kernel_ms": ms, "flops_per_launch": fl,
+            "flops_definition": "4*K per scored selected (particle, rotation, translation) "
+                                "+ 2*K per (particle, selected rotation) for the norm term"}
+
+
 def traffic_bytes(name, key, grid=None):
     """HBM bytes per launch of a kernel (key as in the committed summary, e.g.
     "k_score_sel") from the rocprofv3 FETCH_SIZE + WRITE_SIZE passes over this
@@ -899,26 +934,43 @@ def main():
         r = run_learned(ctx, net, cal, args.steps, args.warmup, P, R, T)
         bR, bT, sc, ess, spread, tail, fell = ctx.get_learned()
         log(f"learned: {r['ms_per_step']:.1f} ms/step {r['kernels_ms']}")
-        dtype = "bf16" if cal["fallback"] == "bf16" else f"bf16 ({cal['fallback']} fallback scan)"
+        fbk = cal["fallback"]
+        if fbk.startswith("sel"):
+            dtype = "bf16"
+            fb_form = (f"fallback: the proposal's {fbk[3:].replace('x', ' rotations x ')} "
+                       f"translations (calibrated rule)")
+        else:
+            dtype = "bf16" if fbk == "bf16" else f"bf16 ({fbk} fallback scan)"
+            fb_form = f"whole-grid {fbk.upper()} fallback (calibrated rule)"
         form = (f"mixed + learned proposal: {cal['n_rot']} rotations x {cal['n_trans']} "
-                f"translations per particle from the grid, BF16 cross term, whole-grid "
-                f"{cal['fallback'].upper()} fallback (calibrated rule)")
-        nsel = float(P) * cal["n_rot"] * cal["n_trans"]
-        # cross term 4K per scored pose + norm term 2K per (particle, rotation),
-        # both computed inside the kernel
-        fl = 4.0 * K * nsel + 2.0 * K * P * cal["n_rot"]
-        t_sel = r["kernels_ms"]["score_sel"] * 1e-3
-        roof = {"bound": "mfma", "achieved": fl / t_sel / 1e12,
-                "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
-                "frac": fl / t_sel / PEAK["bf16"],
-                "traffic": traffic_bytes(name, "k_score_sel"), "kernel": "k_score_sel",
-                "kernel_ms": r["kernels_ms"]["score_sel"], "flops_per_launch": fl,
-                "flops_definition": "4*K per scored selected (particle, rotation, translation) "
-                                    "+ 2*K per (particle, selected rotation) for the norm term"}
+                f"translations per particle from the grid, BF16 cross term, {fb_form}")
+        # the main selection's launch: cross term 4K per scored pose + norm
+        # term 2K per (particle, rotation), both computed inside the kernel
+        roof = sel_roofline(name, K, P, cal["n_rot"], cal["n_trans"],
+                            r["kernels_ms"]["score_sel"])
         out["learned"] = {"agreement_with_plain_fp32": agreement(quat, trans, step, bR, bT, gR, gT),
                           "fallback_particles": r["fallback"], "training": train,
                           "calibration": cal, "kernels_ms": r["kernels_ms"],
                           "sampling_step_deg": float(np.degrees(step))}
+        if "same_map_rule" in cal:
+            # the fastest rule calibrated on the reference map alone (not required
+            # to hold on a changed map): what the step costs when the map is reused
+            c0 = dict(cal["same_map_rule"])
+            r0 = run_learned(ctx, net, c0, args.steps, args.warmup, P, R, T)
+            bR0, bT0 = ctx.get_learned()[:2]
+            out["learned"]["same_map_rule"] = dict(
+                rule={k: c0[k] for k in ("n_rot", "n_trans", "spread_max", "tail_max",
+                                         "score_drop", "fallback")},
+                ms_per_step=r0["ms_per_step"], pose_evals_per_s=r0["pose_evals_per_s"],
+                fallback_particles=r0["fallback"], kernels_ms=r0["kernels_ms"],
+                agreement_with_plain_fp32=agreement(quat, trans, step, bR0, bT0, gR, gT),
+                calibrated_accuracy_on_changed_map=c0["cal_accuracy_robust"])
+            learned.fallback_setup(ctx, cal["fallback"])
+        if fbk.startswith("sel") and "fallback_score" in r["kernels_ms"]:
+            # the flagged particles' wider selection: the same kernel, its own launch
+            nr2, nt2 = (int(x) for x in fbk[3:].split("x"))
+            out["learned"]["fallback_roofline"] = sel_roofline(
+                name, K, int(round(r["fallback"])), nr2, nt2, r["kernels_ms"]["fallback_score"])
         log("proposal-seeded particle filter")
         out["learned"]["seeded_particle_filter"] = run_learned_seeded(ctx, info, name, net, step, gR)
         log(f"seeded: {out['learned']['seeded_particle_filter']}")

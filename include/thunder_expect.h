@@ -585,6 +585,14 @@ int thx_ctx_expect_learned(thx_ctx* ctx, thx_cnn* net, int cls, int nSelR,
  * XCD scores a contiguous run of that order (overlapping selections share its
  * L2); 0 takes them in index order.  The results are the same either way. */
 int thx_ctx_set_select_order(thx_ctx* ctx, int on);
+/* Fallback of the learned expectation (no reference counterpart): with
+ * nSelR > 0 a flagged particle is rescored over the nSelR (<= 16384) rotations
+ * and nSelT (<= 128) translations of the grid its proposal ranks highest, by
+ * the selected-pose scorer, instead of the whole-grid scan (fallbackPrecision
+ * is then unused); 0, 0 (the default) restores the whole-grid scan.  The ESS
+ * and tail depth that thx_ctx_get_learned returns stay those of the first
+ * selection. */
+int thx_ctx_set_learned_fallback_selection(thx_ctx* ctx, int nSelR, int nSelT);
 /* Local-search expectation seeded by the learned proposal, in place of
  * Particle::reset / load (src/Particle.cpp:87-169, 401-554): per particle,
  * mLR rotations and mLT translations are drawn from the network's mixture

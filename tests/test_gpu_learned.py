@@ -20,6 +20,9 @@
   is held to the whole-selection bars against the FP64 oracle.
 * BF16 proposal passes: the mixture from BF16 operands stays within stated
   bounds of the same network's FP32-operand mixture.
+* Fallback by a wider selection: flagged particles get exactly the pose and
+  score of the wider selection scored for every particle, the others those
+  of the first selection (bit for bit), with half and with all flagged.
 * Particle order of the selected-pose scorer (blocks dealt in proposal-mode
   order, or in index order): every result equal bit for bit, for one, two
   and three rotation blocks per particle.
@@ -215,6 +218,34 @@ def test_selection_and_statistics(gpu_ctx, setup):
         d2 = np.sum((info["trans"] - mix[p, 6 * C:6 * C + 2]) ** 2, 1)
         assert np.sum(d2 < d2[bT[p]] - 1e-9) < 9, p
     assert np.all((ess >= 1.0 - 1e-9) & (ess <= nsel + 1e-9))
+
+
+def test_wide_fallback_equals_wide_selection():
+    print("\n[setup] box64 P=60 R=600 T=49 wide fallback", flush=True)
+    ctx = thx.Context(0)
+    wl.setup_context(ctx, "box64", n_particles=60, n_rot=600)
+    net = thx.Proposal(components=4, seed=3)
+    try:
+        ctx.set_learned_fallback_selection(0, 0)
+        ctx.expect_learned(net, 0, 300, 49)
+        wide = ctx.get_learned()
+        ctx.expect_learned(net, 0, 64, 25)
+        first = ctx.get_learned()
+        spread = first[4]
+        ctx.set_learned_fallback_selection(300, 49)
+        for smax in (float(np.median(spread)), 0.5):
+            nf = ctx.expect_learned(net, 0, 64, 25, 0.0, smax, 0.0, 0.0, thx.BF16)
+            got = ctx.get_learned()
+            fell = got[6].astype(bool)
+            assert nf == int(fell.sum()) and nf > 0
+            assert np.array_equal(fell, ~(spread <= smax))
+            for i in (0, 1, 2):   # best rotation, translation, score
+                assert np.array_equal(got[i][fell], wide[i][fell]), (smax, i)
+                assert np.array_equal(got[i][~fell], first[i][~fell]), (smax, i)
+    finally:
+        ctx.set_learned_fallback_selection(0, 0)
+        net.close()
+        ctx.close()
 
 
 def test_select_order_does_not_change_results():

@@ -975,6 +975,17 @@ int thx_ctx_set_local_gather(thx_ctx* c, int ns) {
     return THX_OK;
 }
 
+int thx_ctx_set_learned_fallback_selection(thx_ctx* c, int nSelR, int nSelT) {
+    clear_error();
+    THX_CHECK(c && nSelR >= 0 && nSelT >= 0 && (nSelR == 0) == (nSelT == 0), THX_ERR_ARG,
+              "set_learned_fallback_selection: sizes must both be positive or both 0");
+    THX_CHECK(nSelR <= 64 * 2 * kLocalRotBlock && nSelT <= 128, THX_ERR_RANGE,
+              "set_learned_fallback_selection: at most 16384 rotations and 128 translations");
+    c->lpWideR = nSelR;
+    c->lpWideT = nSelT;
+    return THX_OK;
+}
+
 int thx_ctx_set_select_order(thx_ctx* c, int on) {
     clear_error();
     THX_CHECK(c && (on == 0 || on == 1), THX_ERR_ARG, "set_select_order: on must be 0 or 1");
@@ -1063,6 +1074,63 @@ static int score_selected(thx_ctx* c, int nP, const int* ridx, const int* tidx, 
     THX_TRY(launch_merge_local(c, nP, TPAD, b, c->locWC.as<double>(), c->locWR.as<double>(),
                                c->locWT.as<double>(), c->locBase.as<double>(),
                                c->locBestR.as<int>(), c->locBestT.as<int>(), RB));
+    return THX_OK;
+}
+
+// Order in which the selected-pose scorer takes the n particles whose
+// mixtures are `mix` (proposal-mode order; null when ordering is off).
+static int select_order(thx_ctx* c, const double* mix, int C, int n, const int** order) {
+    *order = nullptr;
+    if (!c->selOrder) return THX_OK;
+    // particles in proposal-mode order: blocks resident together share rows
+    size_t tmpBytes = 0;
+    THX_TRY(pf_select_order(c->stream, nullptr, C, n, nullptr, nullptr, nullptr, nullptr,
+                            nullptr, &tmpBytes));
+    THX_TRY(c->lpOrder.ensure(sizeof(int) * n));
+    THX_TRY(c->lpOrdKeyA.ensure(sizeof(uint32_t) * n));
+    THX_TRY(c->lpOrdKeyB.ensure(sizeof(uint32_t) * n));
+    THX_TRY(c->lpOrdVal.ensure(sizeof(int) * n));
+    THX_TRY(c->lpOrdTmp.ensure(tmpBytes));
+    THX_TRY(pf_select_order(c->stream, mix, C, n, c->lpOrdKeyA.as<uint32_t>(),
+                            c->lpOrdKeyB.as<uint32_t>(), c->lpOrdVal.as<int>(),
+                            c->lpOrder.as<int>(), c->lpOrdTmp.p, &tmpBytes));
+    *order = c->lpOrder.as<int>();
+    return THX_OK;
+}
+
+// Fallback by a wider selection: the n flagged particles (pinned indices
+// idx) are rescored over the lpWideR x lpWideT most probable grid poses of
+// their proposals with the selected-pose scorer; best pose (grid indices) and
+// score land in locBestR / locBestT / locBase, rows in the order of idx.
+static int rescore_wide(thx_ctx* c, int C, int n, const int* pinnedIdx) {
+    hipStream_t st = c->stream;
+    const int nR2 = std::min(c->lpWideR, c->R), nT2 = std::min(c->lpWideT, c->T);
+    const int mixLen = 6 * C + 3;
+    THX_TRY(c->lpPmap.ensure(sizeof(int) * n));
+    THX_HIP(hipMemcpyAsync(c->lpPmap.p, pinnedIdx, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    THX_TRY(c->lpMixSub.ensure(sizeof(double) * (size_t)n * mixLen));
+    THX_TRY(launch_gather_rows(st, c->lpMix.as<double>(), mixLen, c->lpPmap.as<int>(), n,
+                               c->lpMixSub.as<double>()));
+    THX_TRY(c->lpRidx.ensure(sizeof(int) * (size_t)n * nR2));
+    THX_TRY(c->lpRlogq.ensure(sizeof(float) * (size_t)n * nR2));
+    THX_TRY(c->lpTidx.ensure(sizeof(int) * (size_t)n * nT2));
+    THX_TRY(c->lpSpread.ensure(sizeof(float) * n));
+    THX_TRY(pf_select_grid(st, c->lpMixSub.as<double>(), C, n, c->quat.as<double>(), c->R,
+                           c->trans.as<double>(), c->T, nR2, nT2, c->lpRidx.as<int>(),
+                           c->lpRlogq.as<float>(), c->lpTidx.as<int>(), c->lpSpread.as<float>(),
+                           c->lpQf.as<float4>()));
+    const int* order = nullptr;
+    THX_TRY(select_order(c, c->lpMixSub.as<double>(), C, n, &order));
+    THX_TRY(record(c, "fallback_select"));
+    THX_TRY(score_selected(c, n, c->lpRidx.as<int>(), c->lpTidx.as<int>(), nR2, nT2,
+                           c->lpPmap.as<int>(), order));
+    THX_TRY(c->lpESS.ensure(sizeof(double) * n));
+    THX_TRY(c->lpTail.ensure(sizeof(float) * n));
+    THX_TRY(launch_sel_stats(st, c->locWR.as<double>(), c->lpRlogq.as<float>(),
+                             c->lpRidx.as<int>(), c->lpTidx.as<int>(), c->locBestR.as<int>(),
+                             c->locBestT.as<int>(), nR2, nT2, n, c->lpESS.as<double>(),
+                             c->lpTail.as<float>()));
+    THX_TRY(record(c, "fallback_score"));
     return THX_OK;
 }
 
@@ -1156,21 +1224,7 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
                            c->lpRlogq.as<float>(), c->lpTidx.as<int>(), c->lpSpread.as<float>(),
                            c->lpQf.as<float4>()));
     const int* order = nullptr;
-    if (c->selOrder) {
-        // particles in proposal-mode order: blocks resident together share rows
-        size_t tmpBytes = 0;
-        THX_TRY(pf_select_order(st, nullptr, C, P, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                &tmpBytes));
-        THX_TRY(c->lpOrder.ensure(sizeof(int) * P));
-        THX_TRY(c->lpOrdKeyA.ensure(sizeof(uint32_t) * P));
-        THX_TRY(c->lpOrdKeyB.ensure(sizeof(uint32_t) * P));
-        THX_TRY(c->lpOrdVal.ensure(sizeof(int) * P));
-        THX_TRY(c->lpOrdTmp.ensure(tmpBytes));
-        THX_TRY(pf_select_order(st, c->lpMix.as<double>(), C, P, c->lpOrdKeyA.as<uint32_t>(),
-                                c->lpOrdKeyB.as<uint32_t>(), c->lpOrdVal.as<int>(),
-                                c->lpOrder.as<int>(), c->lpOrdTmp.p, &tmpBytes));
-        order = c->lpOrder.as<int>();
-    }
+    THX_TRY(select_order(c, c->lpMix.as<double>(), C, P, &order));
     THX_TRY(record(c, "select"));
     THX_TRY(score_selected(c, P, c->lpRidx.as<int>(), c->lpTidx.as<int>(), nSelR, nSelT, nullptr,
                            order));
@@ -1239,17 +1293,29 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
         // fallback's results are read back into it)
         int* pinned = c->lpStage.as<int>();
         std::copy(flagged.begin(), flagged.end(), pinned);
-        THX_TRY(scan_subset(c, cls, fallbackPrecision, flagged, pinned));
-        c->nev = nev0;   // keep the stage events of this call
-        c->evNames.resize(nev0);
+        const void *srcS, *srcR, *srcT;
+        if (c->lpWideR > 0) {
+            // a wider selection of the same proposals instead of the whole grid
+            // (its two stages are timed: fallback_select, fallback_score)
+            THX_TRY(rescore_wide(c, C, nf, pinned));
+            srcS = c->locBase.p;
+            srcR = c->locBestR.p;
+            srcT = c->locBestT.p;
+        } else {
+            THX_TRY(scan_subset(c, cls, fallbackPrecision, flagged, pinned));
+            srcS = c->lpRes.bestScore.p;
+            srcR = c->lpRes.bestR.p;
+            srcT = c->lpRes.bestT.p;
+            c->nev = nev0;   // keep the stage events of this call, not the scan's
+            c->evNames.resize(nev0);
+        }
         // the staging holds P >= nf entries of each kind
         double* fS = c->lpStage.as<double>();
         int* fR = reinterpret_cast<int*>(fS + nf);
         int* fT = fR + nf;
-        THX_HIP(hipMemcpyAsync(fS, c->lpRes.bestScore.p, sizeof(double) * nf,
-                               hipMemcpyDeviceToHost, st));
-        THX_HIP(hipMemcpyAsync(fR, c->lpRes.bestR.p, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
-        THX_HIP(hipMemcpyAsync(fT, c->lpRes.bestT.p, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
+        THX_HIP(hipMemcpyAsync(fS, srcS, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+        THX_HIP(hipMemcpyAsync(fR, srcR, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
+        THX_HIP(hipMemcpyAsync(fT, srcT, sizeof(int) * nf, hipMemcpyDeviceToHost, st));
         THX_HIP(hipStreamSynchronize(st));
         for (int i = 0; i < nf; i++) {
             const int p = flagged[i];

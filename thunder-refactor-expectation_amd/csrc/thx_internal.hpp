@@ -213,6 +213,9 @@ struct thx_ctx {
     // selected-pose scorer's particle order (Morton keys, sort scratch)
     thx::DevBuf lpOrder, lpOrdKeyA, lpOrdKeyB, lpOrdVal, lpOrdTmp;
     bool selOrder = true;
+    // fallback of the learned expectation by a wider selection (0: whole grid)
+    int lpWideR = 0, lpWideT = 0;
+    thx::DevBuf lpMixSub;    // mixture rows of the flagged particles
     thx::DevBuf lpDat, lpCtf, lpSig;      // gathered rows of fallen-back particles
     thx::ClassResult lpRes;               // their whole-grid scan
     std::vector<int> lpBestR, lpBestT, lpFell;
@@ -409,6 +412,9 @@ int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double
 // Order of the particles for the selected-pose scorer: by a Morton code of
 // the dominant component's mean of each particle's proposal.  tmp == nullptr
 // returns the scratch size in *tmpBytes; keyA/keyB/val hold P entries.
+// dst row i = src row idx[i] (rows of rowLen doubles), i < n
+int launch_gather_rows(hipStream_t st, const double* src, int rowLen, const int* idx, int n,
+                       double* dst);
 int pf_select_order(hipStream_t st, const double* mix, int C, int P, uint32_t* keyA,
                     uint32_t* keyB, int* val, int* order, void* tmp, size_t* tmpBytes);
 int launch_score_sel(thx_ctx* c, int TPAD, int RB, int nblk, const int* ridx, const int* tidx,

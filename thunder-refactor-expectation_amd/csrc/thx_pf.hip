@@ -494,6 +494,24 @@ __global__ void k_sel_keys(const double* __restrict__ mix, int C, int P,
     val[p] = p;
 }
 
+__global__ void k_gather_rows(const double* __restrict__ src, int rowLen,
+                              const int* __restrict__ idx, int n, double* __restrict__ dst) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * rowLen) return;
+    const size_t i = e / rowLen, j = e % rowLen;
+    dst[e] = src[(size_t)idx[i] * rowLen + j];
+}
+
+int launch_gather_rows(hipStream_t st, const double* src, int rowLen, const int* idx, int n,
+                       double* dst) {
+    if (n <= 0) return THX_OK;
+    const size_t m = (size_t)n * rowLen;
+    hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, src,
+                       rowLen, idx, n, dst);
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
 int pf_select_order(hipStream_t st, const double* mix, int C, int P, uint32_t* keyA,
                     uint32_t* keyB, int* val, int* order, void* tmp, size_t* tmpBytes) {
     if (!tmp) {

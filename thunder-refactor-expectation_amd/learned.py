@@ -51,94 +51,170 @@ def train_proposal(net, ctx, workload, n_samples=31_457_280, chunk=8192, batch=5
 # they pay for.
 CANDIDATES = [(128, 32), (256, 32), (512, 32), (128, 64), (192, 64), (256, 64), (300, 49),
               (500, 49), (512, 64), (256, 121)]
-SPREAD = [30.0, 50.0, 80.0, 0.0]
-TAIL = [4.0, 5.0, 6.0, 0.0]
+SPREAD = [20.0, 30.0, 50.0, 80.0, 0.0]
+TAIL = [3.0, 4.0, 5.0, 6.0, 0.0]
 DROP = [100.0, 150.0, 0.0]
+# fallbacks by a wider selection of the same proposal, (rotations, translations)
+WIDE = [(1024, 64), (1024, 128), (2048, 64), (2048, 128)]   # clipped to the grid
 
 
-def calibrate(ctx, net, step, target=0.99, margin=0.003, candidates=CANDIDATES, log=None,
-              fallbacks=("bf16", "fp8"), n_measure=6):
-    """Choose selection sizes, fallback thresholds and the fallback scan's
-    precision on the held-out set in `ctx` (particles, grid and class-0
-    volume set).  A particle that falls back is counted correct when that
-    precision's whole-grid scan agrees with the plain FP32 scan on it.
-    Returns a dict with the chosen n_rot, n_trans, spread_max, tail_max,
-    score_drop, fallback and the calibration statistics."""
+def wide_name(nr, nt):
+    return f"sel{nr}x{nt}"
+
+
+def fallback_setup(ctx, name):
+    """Put the fallback `name` on `ctx` ("bf16" / "fp16" / "fp8": the whole-grid
+    scan in that precision; "sel<R>x<T>": the R x T most probable grid poses of
+    the proposal) and return the precision argument of expect_learned."""
+    if name.startswith("sel"):
+        nr, nt = (int(x) for x in name[3:].split("x"))
+        ctx.set_learned_fallback_selection(nr, nt)
+        return thx.BF16
+    ctx.set_learned_fallback_selection(0, 0)
+    return thx.PRECISIONS[name]
+
+
+def _held_out(ctx, net, step, candidates, fallbacks, wide):
+    """One held-out set's statistics: per fallback name, (ms per particle,
+    agreement with the plain FP32 scan per particle); per candidate selection
+    size, (ms per particle of the learned step without fallback, agreement,
+    spread, tail depth, best score) per particle."""
     quat = ctx.grid_quat
     P = ctx.P
     ctx.score_global(0, thx.FP32)
     gR, _, _ = ctx.get_best(0)
-    full = {}   # precision -> (ms per particle of the whole-grid scan, agreement per particle)
+
+    def agree(bR):
+        return wl.geodesic_angle(quat[bR], quat[gR]) <= step * (1 + 1e-6)
+
+    full = {}
     for name in fallbacks:
         # timed on a second call: the first may grow buffers inside its timed stages
         for _ in range(2):
             ctx.score_global(0, thx.PRECISIONS[name])
         ctx.sync()
         t = sum(ctx.last_timing().values()) / P
-        fR, _, _ = ctx.get_best(0)
-        full[name] = (t, wl.geodesic_angle(quat[fR], quat[gR]) <= step * (1 + 1e-6))
-    ok_rules = []   # every rule that meets the target, with its predicted time
-    best = None
+        full[name] = (t, agree(ctx.get_best(0)[0]))
+    ctx.set_learned_fallback_selection(0, 0)
+    for nr, nt in sorted({(min(nr, ctx.R), min(nt, ctx.T)) for nr, nt in wide}):
+        # a wider selection scored for every particle: its agreement per particle,
+        # and its selection + scoring time per particle
+        for _ in range(2):
+            ctx.expect_learned(net, 0, nr, nt)
+        tm = ctx.last_timing()
+        t = (tm.get("select", 0.0) + tm.get("score_sel", 0.0)) / P
+        full[wide_name(nr, nt)] = (t, agree(ctx.get_learned()[0]))
+    cand = {}
     for nr, nt in candidates:
         for _ in range(2):   # the second call is the timed one, as above
             ctx.expect_learned(net, 0, nr, nt)
         tm = ctx.last_timing()
-        t_sel = sum(v for k, v in tm.items() if k != "fallback") / P
+        t_sel = sum(v for k, v in tm.items() if not k.startswith("fallback")) / P
         bR, bT, sc, ess, spread, tail, fell = ctx.get_learned()
-        ok = wl.geodesic_angle(quat[bR], quat[gR]) <= step * (1 + 1e-6)
-        med = np.median(sc)
+        cand[(nr, nt)] = (t_sel, agree(bR), spread, tail, sc)
+    return full, cand
+
+
+def calibrate(ctx, net, step, target=0.99, margin=0.003, candidates=CANDIDATES, log=None,
+              fallbacks=("bf16", "fp8"), wide=WIDE, n_measure=6, robust=()):
+    """Choose selection sizes, fallback thresholds and the fallback on the
+    held-out set in `ctx` (particles, grid and class-0 volume set).  The
+    fallback is the whole-grid scan in one of `fallbacks` precisions or a
+    wider selection of the same proposal (`wide` sizes); a particle that falls
+    back is counted correct when that fallback alone agrees with the plain
+    FP32 scan on it.  A rule must reach target + margin on `ctx` and on every
+    context in `robust` (held-out sets of other maps, e.g. the reference map
+    as one refinement iteration would change it); times are those of `ctx`.
+    Returns a dict with the chosen n_rot, n_trans, spread_max, tail_max,
+    score_drop, fallback (a name for fallback_setup) and the calibration
+    statistics; the contexts' fallback is left at the whole-grid scan."""
+    P = ctx.P
+    sets = [_held_out(c, net, step, candidates, fallbacks, wide) for c in (ctx,) + tuple(robust)]
+    full0 = sets[0][0]
+    ok_rules = []   # every rule that meets the target on every set, with its predicted time
+    ok_rules0 = []  # those that meet it on ctx's set alone
+    for nr, nt in candidates:
+        t_sel = sets[0][1][(nr, nt)][0]
         for sm in SPREAD:
             for tmx in TAIL:
                 for dz in DROP:
-                    fb = np.zeros(P, bool)
-                    if sm > 0:
-                        fb |= spread > sm
-                    if tmx > 0:
-                        fb |= tail > tmx
-                    if dz > 0:
-                        fb |= sc < med - dz
-                    for name, (t_full, ok_full) in full.items():
-                        acc = float(np.mean(np.where(fb, ok_full, ok)))
-                        cost = t_sel + fb.mean() * t_full
-                        if acc >= target + margin:
-                            ok_rules.append(dict(n_rot=nr, n_trans=nt, spread_max=sm, tail_max=tmx,
-                                                 score_drop=dz, fallback=name, cal_accuracy=acc,
-                                                 cal_fallback=float(fb.mean()),
-                                                 cal_accuracy_no_fallback=float(ok.mean()),
-                                                 pred_ms_per_particle=float(cost)))
+                    accs, fbs = {}, []
+                    for full, cand in sets:
+                        _, ok, spread, tail, sc = cand[(nr, nt)]
+                        fb = np.zeros(len(ok), bool)
+                        if sm > 0:
+                            fb |= spread > sm
+                        if tmx > 0:
+                            fb |= tail > tmx
+                        if dz > 0:
+                            fb |= sc < np.median(sc) - dz
+                        fbs.append(fb)
+                        for name, (_, ok_full) in full.items():
+                            accs.setdefault(name, []).append(
+                                float(np.mean(np.where(fb, ok_full, ok))))
+                    for name, acc in accs.items():
+                        if acc[0] < target + margin:
+                            continue
+                        rule = dict(n_rot=nr, n_trans=nt, spread_max=sm, tail_max=tmx,
+                                    score_drop=dz, fallback=name, cal_accuracy=acc[0],
+                                    cal_accuracy_robust=acc[1:],
+                                    cal_fallback=float(fbs[0].mean()),
+                                    cal_accuracy_no_fallback=float(sets[0][1][(nr, nt)][1].mean()),
+                                    pred_ms_per_particle=float(
+                                        t_sel + fbs[0].mean() * full0[name][0]))
+                        ok_rules0.append(rule)
+                        if min(acc) >= target + margin:
+                            ok_rules.append(rule)
         if log:
-            log(f"calibrate {nr}x{nt}: no-fallback agreement {ok.mean():.4f}")
-    # the prediction prices a fallback particle at the whole-grid scan's
-    # per-particle time, which a small flagged subset exceeds (gathering its
-    # rows, a split norm term, fixed launches): the few cheapest predicted rules
-    # are run as they are on the held-out set and the fastest measured is kept
-    ok_rules.sort(key=lambda r: r["pred_ms_per_particle"])
-    for r in ok_rules[:n_measure]:
-        args = (net, 0, r["n_rot"], r["n_trans"], 0.0, r["spread_max"], r["tail_max"],
-                r["score_drop"], thx.PRECISIONS[r["fallback"]])
-        ctx.expect_learned(*args)   # untimed: may grow buffers
-        ctx.sync()
-        ts = []
-        for _ in range(3):   # the fastest of three calls
-            t0 = time.perf_counter()
-            ctx.expect_learned(*args)
-            ctx.sync()
-            ts.append(time.perf_counter() - t0)
-        r["measured_ms_per_particle"] = 1e3 * min(ts) / P
-        if best is None or r["measured_ms_per_particle"] < best["measured_ms_per_particle"]:
-            best = r
-    if best is not None:
-        best["rules_measured"] = [{k: r[k] for k in ("n_rot", "n_trans", "spread_max", "tail_max",
-                                                     "score_drop", "fallback",
-                                                     "pred_ms_per_particle",
-                                                     "measured_ms_per_particle")}
-                                  for r in ok_rules[:n_measure]]
+            log(f"calibrate {nr}x{nt}: no-fallback agreement "
+                f"{[float(c[(nr, nt)][1].mean()) for _, c in sets]}")
+    # the prediction prices a fallback particle at the fallback's per-particle
+    # time over a whole set, which a small flagged subset exceeds (gathering its
+    # rows, fixed launches): the few cheapest predicted rules are run as they
+    # are on the held-out set and the fastest measured is kept
+    def measure(rules):
+        rules.sort(key=lambda r: r["pred_ms_per_particle"])
+        top = None
+        for r in rules[:n_measure]:
+            if "measured_ms_per_particle" not in r:
+                args = (net, 0, r["n_rot"], r["n_trans"], 0.0, r["spread_max"], r["tail_max"],
+                        r["score_drop"], fallback_setup(ctx, r["fallback"]))
+                ctx.expect_learned(*args)   # untimed: may grow buffers
+                ctx.sync()
+                ts = []
+                for _ in range(3):   # the fastest of three calls
+                    t0 = time.perf_counter()
+                    ctx.expect_learned(*args)
+                    ctx.sync()
+                    ts.append(time.perf_counter() - t0)
+                r["measured_ms_per_particle"] = 1e3 * min(ts) / P
+            if top is None or r["measured_ms_per_particle"] < top["measured_ms_per_particle"]:
+                top = r
+        if top is not None:
+            top = dict(top)
+            top["rules_measured"] = [{k: r[k] for k in (
+                "n_rot", "n_trans", "spread_max", "tail_max", "score_drop", "fallback",
+                "pred_ms_per_particle", "measured_ms_per_particle")} for r in rules[:n_measure]]
+        return top
+
+    best = measure(ok_rules)
+    if robust:
+        # the fastest rule for ctx's map alone, for comparison
+        best0 = measure(ok_rules0)
+        if best is not None and best0 is not None:
+            best["same_map_rule"] = {k: best0[k] for k in (
+                "n_rot", "n_trans", "spread_max", "tail_max", "score_drop", "fallback",
+                "cal_accuracy", "cal_accuracy_robust", "cal_fallback",
+                "measured_ms_per_particle")}
+    ctx.set_learned_fallback_selection(0, 0)
     if best is None:   # nothing meets the target: every particle falls back
         best = dict(n_rot=candidates[0][0], n_trans=candidates[0][1], spread_max=0.5,
                     tail_max=0.0, score_drop=0.0, fallback=fallbacks[0], cal_accuracy=float("nan"),
                     cal_fallback=1.0, cal_accuracy_no_fallback=float("nan"),
-                    pred_ms_per_particle=full[fallbacks[0]][0])
-    best["whole_grid_agreement"] = {k: float(np.mean(v[1])) for k, v in full.items()}
+                    pred_ms_per_particle=full0[fallbacks[0]][0])
+    best["fallback_agreement"] = [{k: float(np.mean(v[1])) for k, v in full.items()}
+                                  for full, _ in sets]
     best["cal_particles"] = P
+    best["cal_sets"] = 1 + len(robust)
     best["target"] = target
     return best

@@ -61,6 +61,7 @@ _SIGS = {
     "thx_ctx_local_mode": [_p, _p],
     "thx_ctx_set_local_gather": [_p, _i],
     "thx_ctx_set_select_order": [_p, _i],
+    "thx_ctx_set_learned_fallback_selection": [_p, _i, _i],
     "thx_ctx_score_local": [_p, _p, _p, _p, _p, _p, _p, _p, _i],
     "thx_ctx_get_local": [_p, _p, _p, _p, _p, _p, _p],
     "thx_ctx_last_timing": [_p, _p, _i, _p, _i, _p],
@@ -565,6 +566,13 @@ class Context:
         m = C.c_int(0)
         check(load().thx_ctx_local_mode(self._h, C.byref(m)), "local_mode")
         return m.value & 15, (m.value >> 4) or 4
+
+    def set_learned_fallback_selection(self, n_rot=0, n_trans=0):
+        """Fallback of the learned expectation: rescore flagged particles over
+        their proposal's n_rot x n_trans most probable grid poses (0, 0: the
+        whole-grid scan)."""
+        check(load().thx_ctx_set_learned_fallback_selection(self._h, n_rot, n_trans),
+              "set_learned_fallback_selection")
 
     def set_select_order(self, on=True):
         """Particle order of the learned expectation's selected-pose scorer
