@@ -574,6 +574,7 @@ struct thx_cnn {
     DevBuf gact[7];
     DevBuf loss, qt, tt, mix, imgs;
     DevBuf wpart;            // split-K partial weight gradients
+    DevBuf wT;               // conv 1-3 weights as BF16 [co][tap][c] (proposal passes)
     long step = 0;
     int inferPrec = THX_BF16;   // operand precision of the proposal passes
 };
@@ -632,56 +633,189 @@ static float* G_(thx_cnn* n, int i) { return n->grads.as<float>() + n->off[i]; }
 static unsigned nblk(size_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 // ---------------------------------------------------------------------------
-// First layer of the proposal passes as a direct convolution (1 -> 16
-// channels, 5 x 5, stride 2, padding 2, bias + ReLU, FP32): one image per
-// block, staged in LDS with a zero border; thread t computes output row
-// t / 8, columns 4 (t % 8) .. + 3 for all 16 channels from the 5 x 11 input
-// patch in registers; the weights and biases are read at block-uniform
-// addresses (scalar loads).  The implicit GEMM spends this layer on
-// scattered im2col loads for a 25-deep product.
+// Proposal passes (inference, BF16 operands, FP32 accumulation): the four
+// conv layers of one image in one block, the activations kept in LDS as BF16
+// with the channel innermost ([y][x][c]).  Layer 0 (1 channel in, 5x5) is a
+// direct FP32 convolution over the image in LDS, each thread 4 positions x
+// 16 channels from a 5 x 11 patch in registers.  A 3x3 stride-2 convolution is then a sum
+// over the 9 taps of 16-channel steps: for tap (ky, kx) and channels
+// 16 j + 8 h .. + 7 a lane's B fragment (output position n, lane half h) is
+// one 16-byte LDS read at input pixel (2 oy - 1 + ky, 2 ox - 1 + kx), zero
+// outside the image; its A fragment (output channel m) is 16 bytes of the
+// weights rearranged as [co][tap][c] (k_conv_wt).  One v_mfma_f32_32x32x16_bf16
+// per (32 x 32 tile, step).  The output tiles: layer 1 32 channels x 256
+// positions (two per wave), layer 2 64 x 64 (one per wave), layer 3 128 x 16
+// (one 32-channel tile per wave, positions padded to 32).  Bias and ReLU as
+// in the FP32 layers; layer 3's output goes to global memory in the FP32
+// [c][y][x] layout the fully connected layer reads.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_conv0_direct(const float* __restrict__ in,
-                                                      const float* __restrict__ w,
-                                                      const float* __restrict__ bias,
-                                                      float* __restrict__ out) {
-    using G = Geo<0>;
-    static_assert(G::CI == 1 && G::KS == 5 && G::S == 2 && G::PD == 2 && G::HI == 64 &&
-                  G::HO == 32 && G::CO == 16, "layer shape");
-    constexpr int W = G::HI + 2 * G::PD;   // 68
-    __shared__ __attribute__((aligned(16))) float im[W * W];
-    const int b = blockIdx.x, tid = threadIdx.x;
+typedef unsigned int cnn_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short bf16_bits(float v) {
+    return __builtin_bit_cast(unsigned short, (__bf16)v);
+}
+
+// wT[(co * 9 + tap) * CI + c] = bf16(w[(co * CI + c) * 9 + tap])
+__global__ void k_conv_wt(const float* __restrict__ w, int CO, int CI,
+                          unsigned short* __restrict__ wT) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= CO * CI * 9) return;
+    const int c = i % CI, tap = (i / CI) % 9, co = i / (CI * 9);
+    wT[i] = bf16_bits(w[(co * CI + c) * 9 + tap]);
+}
+
+// one (32 x 32 output tile, 16-channel step) of a 3x3 stride-2 pad-1 layer
+// reading a [HI][HI][CI] BF16 input from LDS; pos = output position of this
+// lane's column (-1: padding column)
+template <int HI, int HO, int CI>
+__device__ inline void conv_tap_step(const unsigned short* __restrict__ x,
+                                     const unsigned short* __restrict__ wT, int co, int pos,
+                                     int tap, int j, int h, floatx16& acc) {
+    const int ky = tap / 3, kx = tap % 3;
+    const cnn_u32x4 a = *reinterpret_cast<const cnn_u32x4*>(wT + (co * 9 + tap) * CI + 16 * j + 8 * h);
+    cnn_u32x4 b = {0u, 0u, 0u, 0u};
+    if (pos >= 0) {
+        const int iy = 2 * (pos / HO) - 1 + ky, ix = 2 * (pos % HO) - 1 + kx;
+        if (iy >= 0 && iy < HI && ix >= 0 && ix < HI)
+            b = *reinterpret_cast<const cnn_u32x4*>(x + (iy * HI + ix) * CI + 16 * j + 8 * h);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                  __builtin_bit_cast(bf16x8_t, b), acc, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_conv0123_bf16(
+    const float* __restrict__ img, const float* __restrict__ w0, const float* __restrict__ b0,
+    const unsigned short* __restrict__ wT1, const float* __restrict__ b1,
+    const unsigned short* __restrict__ wT2, const float* __restrict__ b2,
+    const unsigned short* __restrict__ wT3, const float* __restrict__ b3,
+    float* __restrict__ act4) {
+    using G0 = Geo<0>;
+    static_assert(G0::CI == 1 && G0::KS == 5 && G0::S == 2 && G0::PD == 2 && G0::HI == 64 &&
+                  G0::HO == 32 && G0::CO == 16, "layer 0 shape");
+    static_assert(Geo<1>::CI == 16 && Geo<1>::HI == 32 && Geo<1>::CO == 32 && Geo<2>::CO == 64 &&
+                  Geo<3>::CO == 128 && Geo<3>::HO == 4, "layer shapes");
+    constexpr int W = G0::HI + 2 * G0::PD;   // 68: the zero-padded image
+    // the padded image (FP32) and, once layer 0 has read it, layer 1's output
+    __shared__ __attribute__((aligned(16))) float u[W * W];                    // 18.5 KB
+    __shared__ __attribute__((aligned(16))) unsigned short x1[32 * 32 * 16];   // 32 KB
+    static_assert(sizeof(u) >= 16 * 16 * 32 * sizeof(unsigned short), "x2 fits the image");
+    float* im = u;
+    unsigned short* x2 = reinterpret_cast<unsigned short*>(u);   // [16][16][32]
+    unsigned short* x3 = x1;   // [8][8][64], 8 KB, once layer 1 has read x1
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, h = lane >> 5;
+    // ---- layer 0 (FP32, direct): 1 -> 16 channels, 5x5 stride 2 ----
     for (int e = tid; e < W * W; e += 256) im[e] = 0.f;
     __syncthreads();
-    const float4* src = reinterpret_cast<const float4*>(in + (size_t)b * 4096);
+    {
+        const float4* src = reinterpret_cast<const float4*>(img + (size_t)b * 4096);
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int e = tid + 256 * q;            // float4 index: row e / 16, columns 4 (e % 16)
-        const float4 v = src[e];
-        float* d = im + (e / 16 + G::PD) * W + 4 * (e % 16) + G::PD;
-        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + 256 * q;   // float4 index: row e / 16, columns 4 (e % 16)
+            const float4 v = src[e];
+            float* d = im + (e / 16 + G0::PD) * W + 4 * (e % 16) + G0::PD;
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
     }
     __syncthreads();
-    const int oy = tid >> 3, ox0 = 4 * (tid & 7);
-    float x[5][11];
-#pragma unroll
-    for (int ky = 0; ky < 5; ky++)
-#pragma unroll
-        for (int c = 0; c < 11; c++) x[ky][c] = im[(2 * oy + ky) * W + 2 * ox0 + c];
-    float* o = out + (size_t)b * G::CO * 1024 + oy * 32 + ox0;
-    for (int co = 0; co < G::CO; co++) {
-        float a0 = bias[co], a1 = a0, a2 = a0, a3 = a0;
+    {
+        const int oy = tid >> 3, ox0 = 4 * (tid & 7);
+        float x[5][11];
 #pragma unroll
         for (int ky = 0; ky < 5; ky++)
 #pragma unroll
-            for (int kx = 0; kx < 5; kx++) {
-                const float wv = w[co * 25 + ky * 5 + kx];
-                a0 += wv * x[ky][kx];
-                a1 += wv * x[ky][kx + 2];
-                a2 += wv * x[ky][kx + 4];
-                a3 += wv * x[ky][kx + 6];
+            for (int c = 0; c < 11; c++) x[ky][c] = im[(2 * oy + ky) * W + 2 * ox0 + c];
+        // two channels at a time: 4 positions x 2 channels, stored as BF16 pairs
+        for (int co = 0; co < G0::CO; co += 2) {
+            float a[2][4];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                a[d][0] = a[d][1] = a[d][2] = a[d][3] = b0[co + d];
+#pragma unroll
+                for (int ky = 0; ky < 5; ky++)
+#pragma unroll
+                    for (int kx = 0; kx < 5; kx++) {
+                        const float wv = w0[(co + d) * 25 + ky * 5 + kx];
+                        a[d][0] += wv * x[ky][kx];
+                        a[d][1] += wv * x[ky][kx + 2];
+                        a[d][2] += wv * x[ky][kx + 4];
+                        a[d][3] += wv * x[ky][kx + 6];
+                    }
             }
-        *reinterpret_cast<float4*>(o + co * 1024) =
-            make_float4(fmaxf(a0, 0.f), fmaxf(a1, 0.f), fmaxf(a2, 0.f), fmaxf(a3, 0.f));
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                ushort2 v;
+                v.x = bf16_bits(fmaxf(a[0][q], 0.f));
+                v.y = bf16_bits(fmaxf(a[1][q], 0.f));
+                *reinterpret_cast<ushort2*>(x1 + (oy * 32 + ox0 + q) * 16 + co) = v;
+            }
+        }
+    }
+    __syncthreads();
+    // accumulator row i of a tile: channel 8 (i / 4) + 4 h + i % 4 of the tile
+    // ---- layer 1: 32 channels x 256 positions, tiles 2 wave, 2 wave + 1 ----
+    {
+        floatx16 acc[2] = {floatx16{0}, floatx16{0}};
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+                conv_tap_step<32, 16, 16>(x1, wT1, l32, 32 * (2 * wave + t) + l32, tap, 0, h, acc[t]);
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int pos = 32 * (2 * wave + t) + l32;
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+                const int co = 8 * g + 4 * h;
+                ushort4 v;
+                v.x = bf16_bits(fmaxf(acc[t][4 * g + 0] + b1[co + 0], 0.f));
+                v.y = bf16_bits(fmaxf(acc[t][4 * g + 1] + b1[co + 1], 0.f));
+                v.z = bf16_bits(fmaxf(acc[t][4 * g + 2] + b1[co + 2], 0.f));
+                v.w = bf16_bits(fmaxf(acc[t][4 * g + 3] + b1[co + 3], 0.f));
+                *reinterpret_cast<ushort4*>(x2 + pos * 32 + co) = v;
+            }
+        }
+    }
+    __syncthreads();
+    // ---- layer 2: 64 channels x 64 positions, one 32 x 32 tile per wave ----
+    {
+        const int mt = wave & 1, nt = wave >> 1;
+        floatx16 acc = {0};
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                conv_tap_step<16, 8, 32>(x2, wT2, 32 * mt + l32, 32 * nt + l32, tap, j, h, acc);
+        const int pos = 32 * nt + l32;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int co = 32 * mt + 8 * g + 4 * h;
+            ushort4 v;
+            v.x = bf16_bits(fmaxf(acc[4 * g + 0] + b2[co + 0], 0.f));
+            v.y = bf16_bits(fmaxf(acc[4 * g + 1] + b2[co + 1], 0.f));
+            v.z = bf16_bits(fmaxf(acc[4 * g + 2] + b2[co + 2], 0.f));
+            v.w = bf16_bits(fmaxf(acc[4 * g + 3] + b2[co + 3], 0.f));
+            *reinterpret_cast<ushort4*>(x3 + pos * 64 + co) = v;
+        }
+    }
+    __syncthreads();
+    // ---- layer 3: 128 channels x 16 positions (columns 16..31 padding) ----
+    {
+        floatx16 acc = {0};
+        const int pos = l32 < 16 ? l32 : -1;
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                conv_tap_step<8, 4, 64>(x3, wT3, 32 * wave + l32, pos, tap, j, h, acc);
+        if (pos >= 0) {
+            float* o = act4 + (size_t)b * 128 * 16 + pos;
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int co = 32 * wave + 8 * (i >> 2) + 4 * h + (i & 3);
+                o[co * 16] = fmaxf(acc[i] + b3[co], 0.f);
+            }
+        }
     }
 }
 
@@ -700,16 +834,30 @@ template <bool BF>
 static int cnn_forward_p(thx_cnn* n, int B, const float* in) {
     hipStream_t st = n->stream;
     const float* x = in ? in : n->act[0].as<const float>();
+    if constexpr (!BF)
+        THX_TRY((conv_fwd<0, 1, 4, BF>(st, x, P_(n, 0), P_(n, 1), n->act[1].as<float>(), B)));
     if constexpr (BF) {
-        hipLaunchKernelGGL(k_conv0_direct, dim3(B), dim3(256), 0, st, x, P_(n, 0), P_(n, 1),
-                           n->act[1].as<float>());
+        // the four conv layers fused, activations in LDS (layer 1-3 weights
+        // rearranged each pass: 95 k values)
+        const int ci[3] = {16, 32, 64}, co[3] = {32, 64, 128};
+        size_t o[4] = {0};
+        for (int l = 0; l < 3; l++) o[l + 1] = o[l] + (size_t)co[l] * ci[l] * 9;
+        THX_TRY(n->wT.ensure(sizeof(unsigned short) * o[3]));
+        unsigned short* wT = n->wT.as<unsigned short>();
+        for (int l = 0; l < 3; l++) {
+            const int cnt = co[l] * ci[l] * 9;
+            hipLaunchKernelGGL(k_conv_wt, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                               P_(n, 2 + 2 * l), co[l], ci[l], wT + o[l]);
+        }
+        hipLaunchKernelGGL(k_conv0123_bf16, dim3(B), dim3(256), 0, st, x, P_(n, 0), P_(n, 1),
+                           wT + o[0], P_(n, 3), wT + o[1], P_(n, 5), wT + o[2], P_(n, 7),
+                           n->act[4].as<float>());
         THX_HIP(hipGetLastError());
     } else {
-        THX_TRY((conv_fwd<0, 1, 4, BF>(st, x, P_(n, 0), P_(n, 1), n->act[1].as<float>(), B)));
+        THX_TRY((conv_fwd<1, 1, 4, BF>(st, n->act[1].as<float>(), P_(n, 2), P_(n, 3), n->act[2].as<float>(), B)));
+        THX_TRY((conv_fwd<2, 2, 2, BF>(st, n->act[2].as<float>(), P_(n, 4), P_(n, 5), n->act[3].as<float>(), B)));
+        THX_TRY((conv_fwd<3, 2, 2, BF>(st, n->act[3].as<float>(), P_(n, 6), P_(n, 7), n->act[4].as<float>(), B)));
     }
-    THX_TRY((conv_fwd<1, 1, 4, BF>(st, n->act[1].as<float>(), P_(n, 2), P_(n, 3), n->act[2].as<float>(), B)));
-    THX_TRY((conv_fwd<2, 2, 2, BF>(st, n->act[2].as<float>(), P_(n, 4), P_(n, 5), n->act[3].as<float>(), B)));
-    THX_TRY((conv_fwd<3, 2, 2, BF>(st, n->act[3].as<float>(), P_(n, 6), P_(n, 7), n->act[4].as<float>(), B)));
     THX_TRY((gemm<2, 2, true, RowOp, RowOp, FcStore, BF>(
         st, RowOp{n->act[4].as<float>(), kFcIn}, RowOp{P_(n, 8), kFcIn},
         FcStore{n->act[5].as<float>(), P_(n, 9), kFcHid, 1}, B, kFcHid, kFcIn)));
@@ -866,7 +1014,7 @@ int thx_cnn_destroy(thx_cnn* n) {
     (void)hipSetDevice(n->device);
     (void)hipStreamSynchronize(n->stream);
     DevBuf* b[] = {&n->params, &n->grads, &n->adam_m, &n->adam_v, &n->loss, &n->qt, &n->tt,
-                   &n->mix, &n->imgs, &n->wpart};
+                   &n->mix, &n->imgs, &n->wpart, &n->wT};
     for (DevBuf* x : b) x->release();
     for (int i = 0; i < 7; i++) {
         n->act[i].release();
