@@ -387,6 +387,7 @@ int thx_ctx_destroy(thx_ctx* c) {
                        &c->lpRlogq, &c->lpTidx, &c->lpESS, &c->lpSpread, &c->lpTail,
                        &c->lpPmap, &c->lpOrder, &c->lpOrdKeyA, &c->lpOrdKeyB, &c->lpOrdVal,
                        &c->lpOrdTmp, &c->lpDat, &c->lpCtf, &c->lpSig, &c->lpQf, &c->S16,
+                       &c->lpStageD,
                        &c->lpRes.wC,
                        &c->lpRes.wR, &c->lpRes.wT, &c->lpRes.base, &c->lpRes.bestR,
                        &c->lpRes.bestT, &c->lpRes.bestScore, &c->pfQ, &c->pfQ2, &c->pfTr,
@@ -1245,12 +1246,14 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     int* hT = hR + P;
     float* hSpread = reinterpret_cast<float*>(hT + P);
     float* hTail = hSpread + P;
-    THX_HIP(hipMemcpyAsync(hScore, c->locBase.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
-    THX_HIP(hipMemcpyAsync(hEss, c->lpESS.p, sizeof(double) * P, hipMemcpyDeviceToHost, st));
-    THX_HIP(hipMemcpyAsync(hR, c->locBestR.p, sizeof(int) * P, hipMemcpyDeviceToHost, st));
-    THX_HIP(hipMemcpyAsync(hT, c->locBestT.p, sizeof(int) * P, hipMemcpyDeviceToHost, st));
-    THX_HIP(hipMemcpyAsync(hSpread, c->lpSpread.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
-    THX_HIP(hipMemcpyAsync(hTail, c->lpTail.p, sizeof(float) * P, hipMemcpyDeviceToHost, st));
+    // packed on the device in the staging layout: one copy instead of six
+    const size_t stageBytes = (2 * sizeof(double) + 4 * sizeof(int)) * (size_t)P;
+    THX_TRY(c->lpStageD.ensure(stageBytes));
+    THX_TRY(launch_pack_learned(st, c->locBase.as<const double>(), c->lpESS.as<const double>(),
+                                c->locBestR.as<const int>(), c->locBestT.as<const int>(),
+                                c->lpSpread.as<const float>(), c->lpTail.as<const float>(), P,
+                                c->lpStageD.p));
+    THX_HIP(hipMemcpyAsync(hScore, c->lpStageD.p, stageBytes, hipMemcpyDeviceToHost, st));
     if (!c->lpRead) THX_HIP(hipEventCreateWithFlags(&c->lpRead, hipEventDisableTiming));
     THX_HIP(hipEventRecord(c->lpRead, st));
     // a 16- or 8-bit fallback scan needs the slices' per-shell maxima: queued

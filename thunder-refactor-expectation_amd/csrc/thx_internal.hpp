@@ -137,6 +137,7 @@ struct thx_ctx {
     thx::DevBuf lpQf;        // grid quaternions as float4 for the selection
     thx::DevBuf S16;         // FP16 copy of the phase table S (selected-pose scorer)
     thx::HostBuf lpStage;    // pinned: per-particle results read back in one batch
+    thx::DevBuf lpStageD;    // the same batch packed on the device
     // ordering events between the context stream and the proposal network's
     hipEvent_t lpReady = nullptr, lpDone = nullptr, lpRead = nullptr;
     // projectee preparation: FFT plans (hipfftHandle) for one (N, pf), scratch
@@ -412,6 +413,11 @@ int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double
 // Order of the particles for the selected-pose scorer: by a Morton code of
 // the dominant component's mean of each particle's proposal.  tmp == nullptr
 // returns the scratch size in *tmpBytes; keyA/keyB/val hold P entries.
+// The learned step's per-particle results packed as [score P][ess P] doubles,
+// [bestR P][bestT P] ints, [spread P][tail P] floats into dst
+int launch_pack_learned(hipStream_t st, const double* score, const double* ess, const int* bestR,
+                        const int* bestT, const float* spread, const float* tail, int P,
+                        void* dst);
 // dst row i = src row idx[i] (rows of rowLen doubles), i < n
 int launch_gather_rows(hipStream_t st, const double* src, int rowLen, const int* idx, int n,
                        double* dst);

@@ -495,7 +495,9 @@ int ensure_slice_shell_max(thx_ctx* c) {
     if (c->shellMaxPOk) return THX_OK;
     float* maxP = c->shellMax.as<float>();
     THX_HIP(hipMemsetAsync(maxP, 0, sizeof(float) * nShell, c->stream));
-    dim3 gridP((c->K + 255) / 256, 16);
+    // 128 row slices: each thread walks Rpad / 128 rows (16 slices left the
+    // walk latency-bound: 0.14 ms for 5,000 x 3,853)
+    dim3 gridP((c->K + 255) / 256, 128);
     hipLaunchKernelGGL(k_shell_max, gridP, dim3(256), 0, c->stream, c->rotP.as<const float2>(),
                        c->Rpad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxP);
     THX_HIP(hipGetLastError());
@@ -517,7 +519,7 @@ int launch_shell_scale(thx_ctx* c, int precision) {
                        (const float*)nullptr, c->Aterm.as<double>(), c->Y.as<float2>(),
                        c->Wt.as<float>());
     THX_HIP(hipGetLastError());
-    dim3 gridY((c->K + 255) / 256, 16);
+    dim3 gridY((c->K + 255) / 256, 128);   // 128 row slices, as for the slices
     hipLaunchKernelGGL(k_shell_max, gridY, dim3(256), 0, c->stream, c->Y.as<const float2>(),
                        c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxY);
     THX_HIP(hipGetLastError());
@@ -580,7 +582,7 @@ int launch_local_shell_scale(thx_ctx* c, const std::vector<char>& used) {
                        (const float*)nullptr, c->Aterm.as<double>(), c->Y.as<float2>(),
                        c->Wt.as<float>());
     THX_HIP(hipGetLastError());
-    dim3 gridY((c->K + 255) / 256, 16);
+    dim3 gridY((c->K + 255) / 256, 128);   // 128 row slices, as for the slices
     hipLaunchKernelGGL(k_shell_max, gridY, dim3(256), 0, c->stream, c->Y.as<const float2>(),
                        c->Ppad, c->Kp, c->K, c->iSig.as<const int>(), nShell, maxY);
     THX_HIP(hipGetLastError());

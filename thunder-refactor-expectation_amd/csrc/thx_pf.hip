@@ -494,6 +494,32 @@ __global__ void k_sel_keys(const double* __restrict__ mix, int C, int P,
     val[p] = p;
 }
 
+__global__ void k_pack_learned(const double* __restrict__ score, const double* __restrict__ ess,
+                               const int* __restrict__ bestR, const int* __restrict__ bestT,
+                               const float* __restrict__ spread, const float* __restrict__ tail,
+                               int P, char* __restrict__ dst) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double* d = reinterpret_cast<double*>(dst);
+    d[p] = score[p];
+    d[P + p] = ess[p];
+    int* i = reinterpret_cast<int*>(d + 2 * (size_t)P);
+    i[p] = bestR[p];
+    i[P + p] = bestT[p];
+    float* f = reinterpret_cast<float*>(i + 2 * (size_t)P);
+    f[p] = spread[p];
+    f[P + p] = tail[p];
+}
+
+int launch_pack_learned(hipStream_t st, const double* score, const double* ess, const int* bestR,
+                        const int* bestT, const float* spread, const float* tail, int P,
+                        void* dst) {
+    hipLaunchKernelGGL(k_pack_learned, dim3((P + 255) / 256), dim3(256), 0, st, score, ess, bestR,
+                       bestT, spread, tail, P, static_cast<char*>(dst));
+    THX_HIP(hipGetLastError());
+    return THX_OK;
+}
+
 __global__ void k_gather_rows(const double* __restrict__ src, int rowLen,
                               const int* __restrict__ idx, int n, double* __restrict__ dst) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
