@@ -311,14 +311,42 @@ __device__ inline float block_count_ge(const float* v, int n, float thr, int* re
     return (float)r;
 }
 
-// select the nSel largest of v[0..n) into out (indices), writes dens[j] = v
+// block_count_ge over values held in registers (thread t: v[t + blockDim j])
+template <int NPT>
+__device__ inline float block_count_ge_reg(const float (&vr)[NPT], float thr, int* red) {
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < NPT; j++) c += vr[j] >= thr;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    int r = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += red[w];
+    __syncthreads();
+    return (float)r;
+}
+
+// select the nSel largest of v[0..n) into out (indices), writes dens[j] = v.
+// NPT > 0: n <= NPT blockDim, and the bisection counts over register copies
+// of the values instead of re-reading LDS each step.
+template <int NPT>
 __device__ void block_topk(const float* v, int n, int nSel, int* out, float* outv, float lo,
                            float hi, int* red, int* cnt) {
     // bisection for thr: count(v >= thr) >= nSel with thr as large as possible
     // (the count is block-uniform, so the early exit is too)
+    float vr[NPT > 0 ? NPT : 1];
+    if constexpr (NPT > 0) {
+#pragma unroll
+        for (int j = 0; j < NPT; j++) {
+            const int i = threadIdx.x + blockDim.x * j;
+            vr[j] = i < n ? v[i] : -INFINITY;
+        }
+    }
     for (int it = 0; it < 40; it++) {
         const float mid = 0.5f * (lo + hi);
-        const float cnt0 = block_count_ge(v, n, mid, red);
+        float cnt0;
+        if constexpr (NPT > 0) cnt0 = block_count_ge_reg<NPT>(vr, mid, red);
+        else cnt0 = block_count_ge(v, n, mid, red);
         if (cnt0 >= nSel) lo = mid; else hi = mid;
         if (cnt0 == nSel) break;
     }
@@ -364,6 +392,7 @@ __global__ void k_quat_to_f4(const double* __restrict__ q, int R, float4* __rest
                             (float)q[4 * r + 3]);
 }
 
+template <int NPT>
 __global__ __launch_bounds__(256) void k_select_grid(
     const double* __restrict__ mix, int C, const float4* __restrict__ gridQ, int R,
     const double* __restrict__ gridT, int T, int nSelR, int nSelT, int* __restrict__ ridx,
@@ -436,7 +465,7 @@ __global__ __launch_bounds__(256) void k_select_grid(
         if (threadIdx.x == 0) spread[p] = rmx[0] * rmx[0] / rmn[0];
         __syncthreads();
     }
-    block_topk(v, R, nSelR, ridx + (size_t)p * nSelR, rlogq + (size_t)p * nSelR, loR, hiR, red,
+    block_topk<NPT>(v, R, nSelR, ridx + (size_t)p * nSelR, rlogq + (size_t)p * nSelR, loR, hiR, red,
                &cnt);
     // translations: Gaussian log density (the constant does not matter)
     const double* sh = mp + 6 * C;
@@ -559,14 +588,22 @@ int pf_select_grid(hipStream_t st, const double* mix, int C, int P, const double
     THX_CHECK(C >= 1 && C <= 16 && R <= kSelMaxR && T <= 512 && nSelR <= R && nSelT <= T,
               THX_ERR_RANGE, "select_grid: sizes out of range");
     hipLaunchKernelGGL(k_quat_to_f4, dim3((R + 255) / 256), dim3(256), 0, st, gridQ, R, gridQf);
-    if (sizeof(float) * R > 32768)
-        THX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_select_grid),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(sizeof(float) * R)));
-    hipLaunchKernelGGL(k_select_grid, dim3(P), dim3(256), sizeof(float) * R, st, mix, C, gridQf,
-                       R, gridT, T, nSelR, nSelT, ridx, rlogq, tidx, spread);
-    THX_HIP(hipGetLastError());
-    return THX_OK;
+    // grids up to 8,192 rotations count in registers (32 per thread at most)
+    auto launch = [&](auto kern) -> int {
+        if (sizeof(float) * R > 32768)
+            THX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(sizeof(float) * R)));
+        hipLaunchKernelGGL(kern, dim3(P), dim3(256), sizeof(float) * R, st, mix, C, gridQf, R,
+                           gridT, T, nSelR, nSelT, ridx, rlogq, tidx, spread);
+        THX_HIP(hipGetLastError());
+        return THX_OK;
+    };
+    if (R <= 256 * 8) return launch(&k_select_grid<8>);
+    if (R <= 256 * 16) return launch(&k_select_grid<16>);
+    if (R <= 256 * 24) return launch(&k_select_grid<24>);
+    if (R <= 256 * 32) return launch(&k_select_grid<32>);
+    return launch(&k_select_grid<0>);
 }
 
 // ---------------------------------------------------------------------------
