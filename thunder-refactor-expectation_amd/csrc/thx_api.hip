@@ -1205,10 +1205,10 @@ int thx_ctx_expect_learned(thx_ctx* c, thx_cnn* net, int cls, int nSelR, int nSe
     // context stream projects the grid and prepares the particle operands
     THX_HIP(hipEventRecord(c->lpReady, st));
     THX_TRY(cnn_propose_async(net, c, c->lpMix.as<double>(), c->lpReady, c->lpDone));
-    // slices of the grid, once for all particles; BF16 table + FP32 |P|^2
+    // slices of the grid, once for all particles; BF16 table (written in the
+    // same pass) + FP32 |P|^2
     THX_TRY(launch_project(c, cls, c->quat.as<const double>(), c->R, c->Rpad,
-                           c->rotP.as<float2>(), c->normP.as<float>()));
-    THX_TRY(launch_convert_lo(c, THX_BF16));
+                           c->rotP.as<float2>(), c->normP.as<float>(), c->rotLo.as<uint32_t>()));
     THX_TRY(record(c, "project"));
     THX_TRY(launch_prep_particles(c, THX_FP32));
     THX_TRY(record(c, "prep"));

@@ -245,8 +245,9 @@ struct thx_ctx {
 
 namespace thx {
 // Kernel launchers (thx_kernels.hip).  All enqueue on ctx->stream.
+// outLo (optional, with out == rotP): the BF16 table written in the same pass
 int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
-                   float2* out, float* outNorm);
+                   float2* out, float* outNorm, uint32_t* outLo = nullptr);
 int launch_translate_table(thx_ctx* c);
 // S16 <- S in FP16 (TpadS x Kp), on the context stream
 int launch_phase_half(thx_ctx* c);

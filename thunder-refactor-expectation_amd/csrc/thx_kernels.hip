@@ -234,7 +234,8 @@ int launch_ctf_attr(thx_ctx* c, const float* dAttr, int P, float pixelSize) {
 __global__ __launch_bounds__(256) void k_project(
     const float2* __restrict__ vol, int V, int pf, const double* __restrict__ quat,
     int R, const int* __restrict__ iCol, const int* __restrict__ iRow, int K,
-    int Kp, float2* __restrict__ out, float* __restrict__ outNorm) {
+    int Kp, float2* __restrict__ out, float* __restrict__ outNorm,
+    uint32_t* __restrict__ outLo) {
     const int r = blockIdx.x;   // rotations on x, the grid dimension with the largest limit
     const int k = blockIdx.y * blockDim.x + threadIdx.x;
     if (k >= Kp) return;
@@ -242,6 +243,7 @@ __global__ __launch_bounds__(256) void k_project(
     if (r >= R || k >= K) {
         out[o] = make_float2(0.f, 0.f);
         if (outNorm) outNorm[o] = 0.f;
+        if (outLo) outLo[o] = 0u;
         return;
     }
     double m[3][2];
@@ -253,20 +255,29 @@ __global__ __launch_bounds__(256) void k_project(
     const float2 v = sample_trilinear(vol, V, x, y, z);
     out[o] = v;
     if (outNorm) outNorm[o] = v.x * v.x + v.y * v.y;
+    if (outLo) {   // as k_convert_lo<BF16> would convert it
+        const uint32_t lo = __builtin_bit_cast(unsigned short, (__bf16)v.x);
+        const uint32_t hi = __builtin_bit_cast(unsigned short, (__bf16)v.y);
+        outLo[o] = lo | (hi << 16);
+    }
 }
 
 int launch_project(thx_ctx* c, int cls, const double* dQuat, int R, int Rpad,
-                   float2* out, float* outNorm) {
+                   float2* out, float* outNorm, uint32_t* outLo) {
     if (out == c->rotP.as<float2>()) {   // the 16-bit table and the slice maxima are stale
         c->rotLoPrec = -1;
         c->shellMaxPOk = false;
     }
+    THX_CHECK(!outLo || (out == c->rotP.as<float2>() && outLo == c->rotLo.as<uint32_t>() &&
+                         c->rotLo.bytes >= sizeof(uint32_t) * (size_t)Rpad * c->Kp),
+              THX_ERR_STATE, "project: the BF16 table goes with the context's slices");
     dim3 grid(Rpad, (c->Kp + 255) / 256);
     hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c->stream,
                        c->vol[cls].as<const float2>(), c->vdim[cls], c->pf, dQuat, R,
                        c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->Kp,
-                       out, outNorm);
+                       out, outNorm, outLo);
     THX_HIP(hipGetLastError());
+    if (outLo) c->rotLoPrec = THX_BF16;
     return THX_OK;
 }
 
