@@ -548,6 +548,10 @@ int thx_cnn_train_step(thx_cnn* net, const float* images, int B, const double* q
                        const double* t, float lr, double* loss);
 /* 64 x 64 images of the context's particles (device; optional host copy). */
 int thx_ctx_images(thx_ctx* ctx, float* host);
+/* The same images made as the proposal passes make them (no reference
+ * counterpart): precision THX_FP32 (as thx_ctx_images) or THX_BF16 (the
+ * transform on BF16 operands the BF16 proposal passes use). */
+int thx_ctx_images_prec(thx_ctx* ctx, int precision, float* host);
 /* Train on the context's particles with known poses q[P][4], t[P][2]. */
 int thx_cnn_train_ctx(thx_cnn* net, thx_ctx* ctx, const double* q, const double* t,
                       int B, int epochs, float lr, double* lastLoss);

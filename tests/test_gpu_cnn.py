@@ -1,6 +1,9 @@
 """Learned-proposal CNN kernels against the FP64 oracle (needs gfx950).
 
-Tolerances: images to 5e-4 of their unit scale (FP32 sums of ~500 terms); forward outputs to 1e-4
+Tolerances: images to 5e-4 of their unit scale (FP32 sums of ~500 terms), the
+BF16 transform of the proposal passes to 0.01 RMS and 0.05 max of it (three
+BF16 roundings of 2^-9 relative: coefficients, twiddles, the first pass's
+output); forward outputs to 1e-4
 relative; loss to 1e-5 relative; parameter gradients, per tensor, to 2e-3 of
 that tensor's largest gradient (FP32 accumulation over up to 2.6e5 terms).
 """
@@ -29,6 +32,25 @@ def test_images_match_numpy(gpu_ctx):
         im = 2 * np.real(np.sum(X[:, None, None] * np.exp(ph), 0))
         im = (im - im.mean()) / im.std()
         assert np.abs(img[p] - im).max() < 5e-4
+
+
+def test_images_bf16_match_numpy(gpu_ctx):
+    print("\n[setup] box64 P=6 images, BF16 transform", flush=True)
+    info = wl.setup_context(gpu_ctx, "box64", n_particles=6, n_rot=16)
+    dat, ctf, _ = gpu_ctx.get_particles()
+    img = thx.ctx_images(gpu_ctx, thx.BF16)
+    iCol, iRow = info["iCol"], info["iRow"]
+    sel = (iCol < 32) & (iRow >= -32) & (iRow < 32) & ~((iCol == 0) & (iRow == 0))
+    y, x = np.mgrid[0:64, 0:64]
+    for p in range(6):
+        X = np.where(ctf[p, sel] >= 0, 1, -1) * (dat[p, sel, 0] + 1j * dat[p, sel, 1])
+        ph = 2j * np.pi * (iCol[sel][:, None, None] * x + iRow[sel][:, None, None] * y) / 64
+        im = 2 * np.real(np.sum(X[:, None, None] * np.exp(ph), 0))
+        im = (im - im.mean()) / im.std()
+        d = img[p] - im
+        print(f"particle {p}: rms {np.sqrt(np.mean(d * d)):.2e} max {np.abs(d).max():.2e}")
+        assert np.sqrt(np.mean(d * d)) < 0.01
+        assert np.abs(d).max() < 0.05
 
 
 @pytest.fixture(scope="module")

@@ -192,6 +192,186 @@ __global__ __launch_bounds__(256) void k_images(const float2* __restrict__ dat,
 }
 
 // ---------------------------------------------------------------------------
+// The same images for the BF16 proposal passes, both passes on
+// v_mfma_f32_32x32x16_bf16 (FP32 accumulation): G, the pass-1 twiddle matrix
+// and H are rounded to BF16.  Pass 1 as above, G in BF16 rows (16-byte
+// fragment reads) against a block-shared table B1t[n][k]; pass 2 takes H from
+// pass 1's accumulators with its k order chosen to match them: step
+// (mt, q0, part) covers rows j = 32 mt + 8 q + 4 h + r (q = q0, q0 + 1,
+// r = 0..3) of Hr (part 0) or Hi (part 1), lane half h holding
+// acc[mt][part nt + ...][4 q0 .. 4 q0 + 7]; the matching A values
+// cos / -sin((j - 32) y) come from a block-shared table C2[y][k'] in that
+// order.  The image stays in registers through the normalisation.
+// ---------------------------------------------------------------------------
+typedef __bf16 img_bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int img_u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kImgLDB = 72;    // BF16 row stride of G and B1t (64 + 8)
+constexpr int kImgLDC = 136;   // BF16 row stride of C2 (128 + 8)
+
+__device__ inline unsigned short img_bf16(float v) {
+    return __builtin_bit_cast(unsigned short, (__bf16)v);
+}
+
+__global__ __launch_bounds__(256) void k_images_bf16(const float2* __restrict__ dat,
+                                                     const float* __restrict__ ctf,
+                                                     const int* __restrict__ iCol,
+                                                     const int* __restrict__ iRow, int K, int P,
+                                                     float* __restrict__ img) {
+    __shared__ __attribute__((aligned(16))) unsigned short G[4][64 * kImgLDB];
+    __shared__ __attribute__((aligned(16))) unsigned short B1t[128 * kImgLDB];
+    __shared__ __attribute__((aligned(16))) unsigned short C2[64 * kImgLDC];
+    __shared__ float2 tw[64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l32 = lane & 31, hl = lane >> 5;
+    const int p = blockIdx.x * 4 + wave;
+    if (tid < 64) {
+        float sn, cs;
+        sincospif(tid / 32.f, &sn, &cs);
+        tw[tid] = make_float2(cs, sn);
+    }
+    unsigned short* g = G[wave];
+    for (int e = lane; e < 64 * kImgLDB; e += 64) g[e] = 0;
+    __syncthreads();
+    // B1t[n][k]: n < 64: Hr at x = n (cos for k < 32, -sin for k >= 32);
+    // n >= 64: Hi at x = n - 64 (sin, cos); i = k & 31
+    for (int e = tid; e < 128 * 64; e += 256) {
+        const int n = e >> 6, k = e & 63, i = k & 31, x = n & 63;
+        const float2 w = tw[(i * x) & 63];
+        const bool re = k < 32;
+        const float v = n < 64 ? (re ? w.x : -w.y) : (re ? w.y : w.x);
+        B1t[n * kImgLDB + k] = img_bf16(v);
+    }
+    // C2[y][k']: k' = 16 step + 8 h + e, step = 4 mt + 2 (q0 / 2) + part,
+    // j = 32 mt + 8 (q0 + e / 4) + 4 h + e % 4; cos((j - 32) y) for Hr, -sin for Hi
+    for (int e2 = tid; e2 < 64 * 128; e2 += 256) {
+        const int y = e2 >> 7, kk = e2 & 127;
+        const int step = kk >> 4, h = (kk >> 3) & 1, e = kk & 7;
+        const int mt = step >> 2, q0 = 2 * ((step >> 1) & 1), part = step & 1;
+        const int j = 32 * mt + 8 * (q0 + (e >> 2)) + 4 * h + (e & 3);
+        const float2 w = tw[((j - 32) * y) & 63];
+        C2[y * kImgLDC + kk] = img_bf16(part == 0 ? w.x : -w.y);
+    }
+    if (p < P) {
+        constexpr int U = 8;
+        for (int k0 = 0; k0 < K; k0 += 64 * U) {
+            float2 x[U];
+            float cv[U];
+            int ij[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int k = k0 + 64 * u + lane;
+                ij[u] = -1;
+                if (k < K) {
+                    const int i = iCol[k], j = iRow[k];
+                    if (i < 32 && j >= -32 && j < 32 && !(i == 0 && j == 0)) {
+                        ij[u] = (j + 32) * kImgLDB + i;
+                        x[u] = dat[(size_t)p * K + k];
+                        cv[u] = ctf[(size_t)p * K + k];
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (ij[u] >= 0) {
+                    const float f = cv[u] >= 0.f ? 2.f : -2.f;
+                    g[ij[u]] = img_bf16(f * x[u].x);
+                    g[ij[u] + 32] = img_bf16(f * x[u].y);
+                }
+        }
+    }
+    __syncthreads();
+    if (p >= P) return;   // no block-wide synchronisation follows
+
+    // ---- pass 1: acc[mt][nt], rows j = 32 mt + .., columns n = 32 nt + l32
+    floatx16 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = floatx16{0};
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+        img_u32x4 fa[2], fb[4];
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+            fa[mt] = *reinterpret_cast<const img_u32x4*>(g + (32 * mt + l32) * kImgLDB + 16 * st + 8 * hl);
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+            fb[nt] = *reinterpret_cast<const img_u32x4*>(B1t + (32 * nt + l32) * kImgLDB + 16 * st + 8 * hl);
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(img_bf16x8, fa[mt]), __builtin_bit_cast(img_bf16x8, fb[nt]),
+                    acc[mt][nt], 0, 0, 0);
+    }
+
+    // ---- pass 2: out[my][nx] rows y = 32 my + .., columns x = 32 nx + l32
+    floatx16 out[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) out[a][b] = floatx16{0};
+#pragma unroll
+    for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+        for (int qp = 0; qp < 2; qp++)
+#pragma unroll
+            for (int part = 0; part < 2; part++) {
+                const int step = 4 * mt + 2 * qp + part;
+                img_u32x4 fa[2];
+#pragma unroll
+                for (int my = 0; my < 2; my++)
+                    fa[my] = *reinterpret_cast<const img_u32x4*>(C2 + (32 * my + l32) * kImgLDC + 16 * step + 8 * hl);
+#pragma unroll
+                for (int nx = 0; nx < 2; nx++) {
+                    const floatx16& hsrc = acc[mt][2 * part + nx];
+                    img_u32x4 fbv;
+#pragma unroll
+                    for (int w = 0; w < 4; w++) {
+                        const int e0 = 8 * qp + 2 * w;   // accumulator elements 4 q0 + 2 w, + 1
+                        fbv[w] = (unsigned)img_bf16(hsrc[e0]) | ((unsigned)img_bf16(hsrc[e0 + 1]) << 16);
+                    }
+#pragma unroll
+                    for (int my = 0; my < 2; my++)
+                        out[my][nx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(img_bf16x8, fa[my]), __builtin_bit_cast(img_bf16x8, fbv),
+                            out[my][nx], 0, 0, 0);
+                }
+            }
+
+    // ---- normalisation over the 4096 pixels of the wave's image
+    float s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int my = 0; my < 2; my++)
+#pragma unroll
+        for (int nx = 0; nx < 2; nx++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const float v = out[my][nx][e];
+                s1 += v;
+                s2 += v * v;
+            }
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+    }
+    const float mean = s1 / 4096.f;
+    const float var = fmaxf(s2 / 4096.f - mean * mean, 1e-20f);
+    const float inv = rsqrtf(var);
+    float* im = img + (size_t)p * 4096;
+#pragma unroll
+    for (int my = 0; my < 2; my++)
+#pragma unroll
+        for (int nx = 0; nx < 2; nx++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int y = 32 * my + 8 * (e >> 2) + 4 * hl + (e & 3);
+                im[y * 64 + 32 * nx + l32] = (out[my][nx][e] - mean) * inv;
+            }
+}
+
+// ---------------------------------------------------------------------------
 // Layers as GEMMs on FP32 MFMA.
 // C[m][n] = sum_k A(m, k) B(n, k): a block of 256 threads holds WM x WN wave
 // tiles of 32 x 32 (v_mfma_f32_32x32x2f32); k advances 16 at a time through
@@ -948,9 +1128,17 @@ int cnn_propose_async(thx_cnn* n, thx_ctx* c, double* mixDev, hipEvent_t ready,
     const int BI = std::min(kInferBatch, c->P);
     THX_TRY(ensure_infer(n, BI));
     THX_HIP(hipStreamWaitEvent(n->stream, ready, 0));
-    hipLaunchKernelGGL(k_images, dim3((c->P + 3) / 4), dim3(256), 0, n->stream, c->dat.as<const float2>(),
-                       c->ctf.as<const float>(), c->iCol.as<const int>(), c->iRow.as<const int>(),
-                       c->K, c->P, c->img.as<float>());
+    // the images in the precision of the proposal passes
+    if (n->inferPrec == THX_BF16)
+        hipLaunchKernelGGL(k_images_bf16, dim3((c->P + 3) / 4), dim3(256), 0, n->stream,
+                           c->dat.as<const float2>(), c->ctf.as<const float>(),
+                           c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->P,
+                           c->img.as<float>());
+    else
+        hipLaunchKernelGGL(k_images, dim3((c->P + 3) / 4), dim3(256), 0, n->stream,
+                           c->dat.as<const float2>(), c->ctf.as<const float>(),
+                           c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->P,
+                           c->img.as<float>());
     THX_HIP(hipGetLastError());
     for (int b0 = 0; b0 < c->P; b0 += BI) {
         const int B = std::min(BI, c->P - b0);
@@ -1132,15 +1320,26 @@ int thx_cnn_train_step(thx_cnn* n, const float* images, int B, const double* q, 
     return cnn_run(n, images, nullptr, B, q, t, loss, nullptr, lr, nullptr);
 }
 
-// Images of the context's particles (device), optional host copy [P][64][64].
-int thx_ctx_images(thx_ctx* c, float* host) {
+// Images of the context's particles (device), optional host copy [P][64][64];
+// precision THX_FP32 (training, thx_ctx_images) or THX_BF16 (the transform
+// the BF16 proposal passes use).
+int thx_ctx_images_prec(thx_ctx* c, int precision, float* host) {
     clear_error();
     THX_CHECK(c && c->P > 0 && c->K > 0, THX_ERR_STATE, "ctx_images: no particles");
+    THX_CHECK(precision == THX_FP32 || precision == THX_BF16, THX_ERR_ARG,
+              "ctx_images: precision must be THX_FP32 or THX_BF16");
     THX_HIP(hipSetDevice(c->device));
     THX_TRY(c->img.ensure(sizeof(float) * 4096 * (size_t)c->P));
-    hipLaunchKernelGGL(k_images, dim3((c->P + 3) / 4), dim3(256), 0, c->stream, c->dat.as<const float2>(),
-                       c->ctf.as<const float>(), c->iCol.as<const int>(), c->iRow.as<const int>(),
-                       c->K, c->P, c->img.as<float>());
+    if (precision == THX_BF16)
+        hipLaunchKernelGGL(k_images_bf16, dim3((c->P + 3) / 4), dim3(256), 0, c->stream,
+                           c->dat.as<const float2>(), c->ctf.as<const float>(),
+                           c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->P,
+                           c->img.as<float>());
+    else
+        hipLaunchKernelGGL(k_images, dim3((c->P + 3) / 4), dim3(256), 0, c->stream,
+                           c->dat.as<const float2>(), c->ctf.as<const float>(),
+                           c->iCol.as<const int>(), c->iRow.as<const int>(), c->K, c->P,
+                           c->img.as<float>());
     THX_HIP(hipGetLastError());
     if (host) {
         THX_HIP(hipStreamSynchronize(c->stream));
@@ -1148,6 +1347,8 @@ int thx_ctx_images(thx_ctx* c, float* host) {
     }
     return THX_OK;
 }
+
+int thx_ctx_images(thx_ctx* c, float* host) { return thx_ctx_images_prec(c, THX_FP32, host); }
 
 // Train on the context's particles (their images) with known poses: epochs
 // passes of minibatches of size B in order, Adam with learning rate lr.
@@ -1171,7 +1372,8 @@ int thx_cnn_train_ctx(thx_cnn* n, thx_ctx* c, const double* q, const double* t, 
 int thx_cnn_propose_ctx(thx_cnn* n, thx_ctx* c, double* mixHost, double* mixDev) {
     clear_error();
     THX_CHECK(n && c && (mixHost || mixDev), THX_ERR_ARG, "cnn_propose_ctx: bad argument");
-    THX_TRY(thx_ctx_images(c, nullptr));
+    // the images as the learned expectation's proposal passes make them
+    THX_TRY(thx_ctx_images_prec(c, n->inferPrec == THX_BF16 ? THX_BF16 : THX_FP32, nullptr));
     THX_HIP(hipStreamSynchronize(c->stream));
     const int BI = std::min(kInferBatch, c->P);
     THX_TRY(ensure_infer(n, BI));

@@ -79,6 +79,7 @@ _SIGS = {
     "thx_cnn_loss_grad": [_p, _p, _i, _p, _p, _p, _p],
     "thx_cnn_train_step": [_p, _p, _i, _p, _p, C.c_float, _p],
     "thx_ctx_images": [_p, _p],
+    "thx_ctx_images_prec": [_p, _i, _p],
     "thx_cnn_train_ctx": [_p, _p, _p, _p, _i, _i, C.c_float, _p],
     "thx_cnn_propose_ctx": [_p, _p, _p, _p],
     "thx_cnn_components": [_p, _p],
@@ -880,7 +881,12 @@ class Proposal:
         return mix
 
 
-def ctx_images(ctx):
+def ctx_images(ctx, precision=None):
+    """64 x 64 images of the context's particles (FP32 transform; BF16: the
+    transform the BF16 proposal passes use)."""
     out = np.zeros((ctx.P, 64, 64), np.float32)
-    check(load().thx_ctx_images(ctx._h, ptr(out)), "ctx_images")
+    if precision is None:
+        check(load().thx_ctx_images(ctx._h, ptr(out)), "ctx_images")
+    else:
+        check(load().thx_ctx_images_prec(ctx._h, precision, ptr(out)), "ctx_images")
     return out
