@@ -311,73 +311,127 @@ __device__ inline float block_count_ge(const float* v, int n, float thr, int* re
     return (float)r;
 }
 
-// block_count_ge over values held in registers (thread t: v[t + blockDim j])
+// block_count_ge over values held in registers, with one barrier: the wave
+// counts go to slot `par` of red (red[16 par + wave]), and consecutive calls
+// alternate the slot, so a slot is rewritten only after every thread has
+// passed the barrier of the call in between (and so finished reading it)
 template <int NPT>
-__device__ inline float block_count_ge_reg(const float (&vr)[NPT], float thr, int* red) {
+__device__ inline int block_count_ge_reg(const float (&vr)[NPT], float thr, int* red, int par) {
     int c = 0;
 #pragma unroll
     for (int j = 0; j < NPT; j++) c += vr[j] >= thr;
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    if ((threadIdx.x & 63) == 0) red[16 * par + (threadIdx.x >> 6)] = c;
     __syncthreads();
     int r = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += red[w];
-    __syncthreads();
-    return (float)r;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r += red[16 * par + w];
+    return r;
 }
 
-// select the nSel largest of v[0..n) into out (indices), writes dens[j] = v.
-// NPT > 0: n <= NPT blockDim, and the bisection counts over register copies
-// of the values instead of re-reading LDS each step.
+// exclusive prefix sum of x over the block in thread order, and its total
+// (red: one int per wave, not in use by any thread in flight)
+__device__ inline int block_excl_scan(int x, int* red, int& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int inc = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) red[wave] = inc;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+        const int t = red[w];
+        if (w < wave) before += t;
+        total += t;
+    }
+    return before + inc - x;
+}
+
+// select the nSel largest of v[0..n) into out (indices) and outv (values):
+// threshold thr by bisection (count(v >= thr) >= nSel, thr as large as
+// possible); the output lists the values above the bisection's upper bound
+// first, then those in [thr, upper bound] until nSel are taken, each part in
+// index order.
+// NPT > 0 (n <= NPT blockDim): thread t holds the contiguous run
+// [t m, t m + m) of v in registers (m = ceil(n / blockDim)), so thread order is
+// index order and the ordered compaction is one block-wide prefix sum.
+// NPT == 0: counts and compaction read LDS, 256 values per step.
 template <int NPT>
 __device__ void block_topk(const float* v, int n, int nSel, int* out, float* outv, float lo,
                            float hi, int* red, int* cnt) {
-    // bisection for thr: count(v >= thr) >= nSel with thr as large as possible
-    // (the count is block-uniform, so the early exit is too)
-    float vr[NPT > 0 ? NPT : 1];
     if constexpr (NPT > 0) {
+        const int m = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+        const int i0 = (int)threadIdx.x * m;
+        float vr[NPT];
+#pragma unroll
+        for (int j = 0; j < NPT; j++) vr[j] = (j < m && i0 + j < n) ? v[i0 + j] : -INFINITY;
+        // the count is block-uniform, so the early exit is too
+        for (int it = 0; it < 40; it++) {
+            const float mid = 0.5f * (lo + hi);
+            const int cnt0 = block_count_ge_reg<NPT>(vr, mid, red, it & 1);
+            if (cnt0 >= nSel) lo = mid; else hi = mid;
+            if (cnt0 == nSel) break;
+        }
+        const float thr = lo;
+        int c0 = 0, c1 = 0;
 #pragma unroll
         for (int j = 0; j < NPT; j++) {
-            const int i = threadIdx.x + blockDim.x * j;
-            vr[j] = i < n ? v[i] : -INFINITY;
+            c0 += vr[j] > hi;
+            c1 += vr[j] >= thr && vr[j] <= hi;
         }
-    }
-    for (int it = 0; it < 40; it++) {
-        const float mid = 0.5f * (lo + hi);
-        float cnt0;
-        if constexpr (NPT > 0) cnt0 = block_count_ge_reg<NPT>(vr, mid, red);
-        else cnt0 = block_count_ge(v, n, mid, red);
-        if (cnt0 >= nSel) lo = mid; else hi = mid;
-        if (cnt0 == nSel) break;
-    }
-    const float thr = lo;
-    if (threadIdx.x == 0) *cnt = 0;
-    __syncthreads();
-    // strictly above the threshold first, then ties at the threshold, in index order
-    for (int pass = 0; pass < 2; pass++) {
-        for (int base = 0; base < n; base += blockDim.x) {
-            const int i = base + threadIdx.x;
-            bool take = false;
-            if (i < n) take = pass == 0 ? v[i] > hi : (v[i] >= thr && v[i] <= hi);
-            // ordered compaction within the block
-            const unsigned long long m = __ballot(take);
-            const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-            const int before = __popcll(m & ((1ull << lane) - 1));
-            red[wave] = __popcll(m);
-            __syncthreads();
-            int off = *cnt;
-            for (int w = 0; w < wave; w++) off += red[w];
-            if (take && off + before < nSel) {
-                out[off + before] = i;
-                outv[off + before] = v[i];
+        // both counts in one scan: each stays below 2^16 (n <= 8192)
+        int total;
+        const int before = block_excl_scan(c0 | (c1 << 16), red + 32, total);
+        int o0 = before & 0xffff, o1 = (total & 0xffff) + (before >> 16);
+#pragma unroll
+        for (int j = 0; j < NPT; j++) {
+            const float x = vr[j];
+            if (x > hi) {
+                if (o0 < nSel) { out[o0] = i0 + j; outv[o0] = x; }
+                o0++;
+            } else if (x >= thr && x <= hi) {
+                if (o1 < nSel) { out[o1] = i0 + j; outv[o1] = x; }
+                o1++;
             }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                int tot = 0;
-                for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += red[w];
-                *cnt = min(nSel, *cnt + tot);
+        }
+        __syncthreads();   // red is free again for the caller
+    } else {
+        for (int it = 0; it < 40; it++) {
+            const float mid = 0.5f * (lo + hi);
+            const float cnt0 = block_count_ge(v, n, mid, red);
+            if (cnt0 >= nSel) lo = mid; else hi = mid;
+            if (cnt0 == nSel) break;
+        }
+        const float thr = lo;
+        if (threadIdx.x == 0) *cnt = 0;
+        __syncthreads();
+        for (int pass = 0; pass < 2; pass++) {
+            for (int base = 0; base < n; base += blockDim.x) {
+                const int i = base + threadIdx.x;
+                bool take = false;
+                if (i < n) take = pass == 0 ? v[i] > hi : (v[i] >= thr && v[i] <= hi);
+                // ordered compaction within the block
+                const unsigned long long msk = __ballot(take);
+                const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+                const int before = __popcll(msk & ((1ull << lane) - 1));
+                red[wave] = __popcll(msk);
+                __syncthreads();
+                int off = *cnt;
+                for (int w = 0; w < wave; w++) off += red[w];
+                if (take && off + before < nSel) {
+                    out[off + before] = i;
+                    outv[off + before] = v[i];
+                }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    int tot = 0;
+                    for (int w = 0; w < (int)(blockDim.x >> 6); w++) tot += red[w];
+                    *cnt = min(nSel, *cnt + tot);
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
 }
